@@ -1,0 +1,167 @@
+/*
+ * typecv_ba.h — C ABI of the MI355X bundle-adjustment engine behind `bundleAdjustGpu`.
+ *
+ * The engine runs the whole Levenberg–Marquardt loop of the reference operator on one gfx950 GPU:
+ * linearisation, point elimination, matrix-free Schur products, block-Jacobi PCG, back-substitution,
+ * gauge handling and the LM accept/reject rule. The host (a Node-API addon, the Python mirror in
+ * `typecv_amd/bundle.py`, or any FFI) keeps the reference's validation, packing and unpacking and
+ * hands over the two packed arrays of `packBundleScene`:
+ *
+ *   scene  f64 [poses 12C][points 3P][models 6C: fx fy cx cy k1 k2][pixels 2M]
+ *   index  u32 [obsCamera M][obsPoint M][pointStart P+1][cameraStart C+1]
+ *              [cameraObservations M][cameraActive C]                       (3M + P + 2C + 2 words)
+ *
+ * and receives the array of `packBundleResult`:
+ *
+ *   out    f64 [poses 12C][points 3P][state 26] (+ [log focal scale] on the shared-focal route)
+ *
+ * State words (reference `BUNDLE_STATE`, src/sfm/bundle/gpu/bundle-shared.ts:38-54):
+ *   0 lambda, 1 LM active, 2 baseline, 3..5 gauge origin, 6 cost, 7 squared error,
+ *   8 initial cost, 9 initial squared error, 10 accepted, 11 rejected, 12 commit, 13 gauge scale,
+ *   14 input error (1 baseline, 2 cheirality), 15 trial valid, 16 CG steps, 17 failed solves,
+ *   18 rho, 19 PCG active, 20 PCG failed, 21 |r0|^2, 22 |r|^2, 23 steps, 24 initial RMSE,
+ *   25 final RMSE (24/25 hold the RMSEs in `out`).
+ *
+ * A failed linear solve is not an error: it shows in words 11 and 17 with the accepted geometry
+ * unchanged. Input errors detected on the device (words 14) are reported in `out` as the reference
+ * reports them; the host turns them into its errors (reference `checkBundleOutput`,
+ * src/sfm/bundle/gpu/bundle-host.ts:278-283).
+ *
+ * Thread safety: one context serialises its own calls; distinct contexts may be used from
+ * distinct threads.
+ */
+#ifndef TYPECV_BA_H
+#define TYPECV_BA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TCV_BA_STATE_WORDS 26
+
+/* Arithmetic forms. */
+enum tcv_ba_precision {
+  TCV_BA_FP64 = 0,  /* plain IEEE binary64 everywhere */
+  TCV_BA_MIXED = 1  /* FP32 cross blocks and Schur products with FP64 accumulation, FP64 V^-1,
+                       preconditioner, dots and LM decisions, outer iterative refinement */
+};
+
+/* Status codes of every entry point. */
+enum tcv_ba_status {
+  TCV_BA_OK = 0,
+  TCV_BA_ERR_ARGUMENT = 1, /* sizes/options outside the reference's admitted ranges */
+  TCV_BA_ERR_INDEX = 2,    /* the index array is not a packBundleScene layout of these sizes */
+  TCV_BA_ERR_DEVICE = 3,   /* a HIP runtime call failed */
+  TCV_BA_ERR_MEMORY = 4,   /* device allocation failed (the reference's budget refusal) */
+  TCV_BA_ERR_NO_DEVICE = 5 /* no gfx950 device is visible */
+};
+
+/*
+ * The problem and options. Mirrors `BundleControl` (reference src/sfm/bundle/gpu/bundle.shade.ts:17-48
+ * and joint-focal.shade.ts:14-42) for a resident run, plus the host-side loop options of
+ * `bundleAdjustGpuFixed` (bundle.ts:89-132) and `bundleAdjustJointFocalGpu` (joint-focal.ts:78-130).
+ */
+typedef struct tcv_ba_control {
+  uint32_t cameras;            /* C >= 2 */
+  uint32_t points;             /* P: finite points, then directional points */
+  uint32_t directional_points; /* trailing points that hold unit world bearings */
+  uint32_t observations;       /* M */
+  uint32_t anchor;             /* camera index of anchorViewId */
+  uint32_t scale_view;         /* camera index of scaleViewId */
+  uint32_t normalize_scale;    /* 1 when exactly one camera is fixed */
+  uint32_t shared_focal;       /* 0: fixed intrinsics (bundle.shade.ts); 1: shared log focal */
+  uint32_t iterations;         /* LM iterations, 0..100 */
+  uint32_t max_cg;             /* PCG iterations per LM iteration, 1..256 */
+  uint32_t precision;          /* enum tcv_ba_precision */
+  uint32_t flags;              /* 0; TCV_BA_FLAG_TIMING times the Schur sweeps with HIP events */
+  double huber;                /* Huber threshold in pixels, [1e-12, 1e6] */
+  double relative_tolerance;   /* PCG relative residual tolerance, [1e-12, 1e-2] */
+  double stall_tolerance;      /* host LM stall rule; <= 1e-9 leaves only the device rule */
+} tcv_ba_control;
+
+#define TCV_BA_FLAG_TIMING 1u
+
+/* What one solve did. Times are milliseconds of host wall clock unless named otherwise. */
+typedef struct tcv_ba_report {
+  double solve_ms;          /* device-resident solve: setup + LM loop, inputs already in HBM */
+  double setup_ms;          /* index checks, tile tables, camera-major permutation */
+  double schur_ms;          /* HIP-event time of all Schur sweeps (TCV_BA_FLAG_TIMING) */
+  double schur_point_ms;    /* of which the point sweep */
+  double schur_camera_ms;   /* of which the camera sweep */
+  double refinement_steps;  /* outer refinement corrections taken (mixed form) */
+  uint64_t lm_iterations;   /* LM iterations executed (accepted + rejected) */
+  uint64_t cg_iterations;   /* PCG iterations that completed */
+  uint64_t matvecs;         /* Schur products launched */
+  uint64_t flag_reads;      /* host reads of the activity words */
+  uint64_t device_bytes;    /* device memory held by the context after the solve */
+  uint64_t point_tiles;     /* wave tiles of whole points */
+  uint64_t camera_tasks;    /* camera-list tasks */
+} tcv_ba_report;
+
+typedef struct tcv_ba_context tcv_ba_context;
+
+/* Size of the `out` array in doubles: 12C + 3P + 26 (+1 with shared focal). */
+size_t tcv_ba_output_size(const tcv_ba_control* control);
+/* Size of the scene array (12C + 3P + 6C + 2M) and of the index array (3M + P + 2C + 2). */
+size_t tcv_ba_scene_size(const tcv_ba_control* control);
+size_t tcv_ba_index_size(const tcv_ba_control* control);
+
+/* Human-readable text of a status code. */
+const char* tcv_ba_status_string(int status);
+/* The last error text of this thread (HIP error names included), or "". */
+const char* tcv_ba_last_error(void);
+
+/*
+ * Create / destroy a context on HIP device `device`. The context owns the device buffers, streams
+ * and pinned flag words and keeps them across solves (grown as needed). Replaces the buffer pool of
+ * the reference's `ShadeDevice` (src/device/gpu/device.ts:608-710).
+ */
+int tcv_ba_create(int device, tcv_ba_context** out);
+void tcv_ba_destroy(tcv_ba_context* context);
+/* Device bytes currently held by the context (the reference's `statistics.liveBytes`). */
+uint64_t tcv_ba_device_bytes(const tcv_ba_context* context);
+/* Release every device buffer of the context (the reference's `drop`). */
+int tcv_ba_release(tcv_ba_context* context);
+
+/*
+ * Solve from host arrays: upload, run the LM loop, read back `out`. The call that the Node-API
+ * addon runs as async work for `bundleAdjustGpu` (reference src/sfm/bundle/gpu/bundle.ts:803-839:
+ * everything inside `device.enqueue`, bundle.ts:178-799 and joint-focal.ts equivalents).
+ */
+int tcv_ba_solve(tcv_ba_context* context, const tcv_ba_control* control, const double* scene,
+                 const uint32_t* index, double* out, tcv_ba_report* report);
+
+/*
+ * Solve from device arrays already resident in HBM (`scene`, `index`, `out` are device pointers of
+ * the context's device). `scene` is read, never written. Work runs on `stream` (a hipStream_t, or
+ * NULL for the context's own stream) and the call returns when `out` is complete.
+ */
+int tcv_ba_solve_device(tcv_ba_context* context, const tcv_ba_control* control,
+                        const double* scene, const uint32_t* index, double* out,
+                        tcv_ba_report* report, void* stream);
+
+/*
+ * One linear system of the LM loop, for checks and benchmarks: linearise the scene at damping
+ * `lambda` (and log focal scale `log_focal` on the shared-focal route), assemble the point blocks,
+ * camera blocks and block-Jacobi preconditioner, and return
+ *   rhs[m]    the reduced right-hand side (reference `cameraRhs` + focal RHS),
+ *   sp[m]     the matrix-free Schur product S·p for the host vector p[m] (m = 6C + shared_focal),
+ *   z[m]      the preconditioned vector M^-1·p,
+ *   status[1] 0 when every point block and preconditioner block is valid, 1 otherwise.
+ * `repeat` > 1 repeats the Schur product (timing); the result is that of the last product.
+ * Host pointers. Reference kernels: linearizeBundle, bundlePointBlocks, bundleCameraBlocks,
+ * bundleSchurPoint/bundleSchurCamera (bundle.shade.ts:185-225, 251-489; joint-focal.shade.ts).
+ */
+int tcv_ba_schur_probe(tcv_ba_context* context, const tcv_ba_control* control,
+                       const double* scene, const uint32_t* index, double lambda,
+                       double log_focal, const double* p, double* rhs, double* sp, double* z,
+                       int32_t* status, uint32_t repeat, tcv_ba_report* report);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TYPECV_BA_H */

@@ -1,0 +1,107 @@
+"""GPU parity: the HIP engine against the CPU oracle on the same seeded inputs.
+
+Bars (stated here): the assembled linear system (RHS, preconditioned vector,
+matrix-free Schur product) within 1e-10 relative to the oracle's; a whole solve's final cost within
+1e-9 relative and every pose/point coordinate within 1e-7; integer results (accepted/rejected
+steps, failures, input-error words) equal.
+"""
+import numpy as np
+import pytest
+
+from typecv_amd import bundle as B
+from typecv_amd import synthetic as S
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+FIXED = lambda C: min(96, max(24, 12 * C))  # noqa: E731
+SHARED = lambda C: min(256, max(32, 12 * C + 1))  # noqa: E731
+
+
+def packed_problem(model, options):
+    shared = bool(options.get("refineSharedFocal"))
+    problem = B.bundle_problem(model, options, SHARED if shared else FIXED)
+    packed = B.pack_bundle_scene(problem["input"], problem["byId"], problem["active"])
+    control = B.make_control(
+        packed.C, packed.P, packed.M, directional=packed.P - problem["finitePoints"],
+        anchor=problem["byId"][problem["anchorId"]], scale_view=problem["byId"][problem["scaleId"]],
+        normalize_scale=problem["fixed"] == 1, shared_focal=shared,
+        iterations=problem["iterations"], max_cg=problem["maxCg"], huber=problem["huber"],
+        tolerance=problem["tolerance"], precision=options.get("precision", "fp64"))
+    return packed, control
+
+
+def cases():
+    yield "bundle_fixture", S.bundle_fixture(), {"iterations": 20}
+    yield "bundle_fixture_outlier", S.bundle_fixture(outlier=True), {"iterations": 20, "huberPx": 1}
+    yield "joint_focal", S.joint_focal_fixture()["input"], {
+        "iterations": 40, "refineSharedFocal": True, "maxCgIterations": 256,
+        "cgRelativeTolerance": 1e-10}
+    yield "ring_320", S.ring_bundle_fixture(views=64, points=960), {"iterations": 8}
+
+
+CASES = list(cases())
+
+
+def packed_cases():
+    ring, _ = S.ring_scene(seed=3, cameras=10, points=400)
+    yield "ring_scene", ring, B.make_control(ring.C, ring.P, ring.M, iterations=20, max_cg=50)
+    traj, _ = S.trajectory_scene(cameras=40, points=3000, seed=5)
+    yield "trajectory_shared", traj, B.make_control(
+        traj.C, traj.P, traj.M, iterations=15, max_cg=256, shared_focal=True, huber=1.0)
+
+
+PACKED = list(packed_cases())
+
+
+@pytest.mark.parametrize("name,model,options", CASES, ids=[c[0] for c in CASES])
+def test_schur_probe_matches_oracle(device, name, model, options):
+    print(f"\n[probe] {name}", flush=True)
+    packed, control = packed_problem(model, options)
+    m = control.cameras * 6 + control.shared_focal
+    rng = np.random.default_rng(7)
+    p = rng.normal(size=m)
+    for lam in (1e-3, 1.0):
+        rhs, sp, z, status = O.schur_probe(control, packed, lam, 0.01, p)
+        grhs, gsp, gz, gstatus, _ = B.schur_probe(device, control, packed, lam, 0.01, p)
+        assert gstatus == status
+        for a, b in ((grhs, rhs), (gsp, sp), (gz, z)):
+            assert np.max(np.abs(a - b)) <= 1e-10 * max(1.0, np.max(np.abs(b)))
+
+
+def assert_solve_parity(got, expected, base):
+    cost = expected[base + 6]
+    assert abs(got[base + 6] - cost) <= 1e-9 * max(1.0, abs(cost)), (got[base + 6], cost)
+    assert np.max(np.abs(got[:base] - expected[:base])) <= 1e-7
+    for word in (10, 11, 14, 17):
+        assert got[base + word] == expected[base + word], word
+
+
+@pytest.mark.parametrize("name,model,options", CASES, ids=[c[0] for c in CASES])
+def test_solve_matches_oracle(device, name, model, options):
+    print(f"\n[solve] {name}", flush=True)
+    packed, control = packed_problem(model, options)
+    expected, _ = O.solve(control, packed)
+    got, report = B.solve_packed(device, control, packed)
+    base = packed.C * 12 + packed.P * 3
+    assert_solve_parity(got, expected, base)
+    if control.shared_focal:
+        assert abs(got[base + 26] - expected[base + 26]) <= 1e-9
+
+
+@pytest.mark.parametrize("name,packed,control", PACKED, ids=[c[0] for c in PACKED])
+def test_packed_solve_matches_oracle(device, name, packed, control):
+    print(f"\n[solve] {name} C={packed.C} P={packed.P} M={packed.M}", flush=True)
+    expected, _ = O.solve(control, packed)
+    got, report = B.solve_packed(device, control, packed)
+    base = packed.C * 12 + packed.P * 3
+    assert expected[base + 6] < expected[base + 8]
+    assert_solve_parity(got, expected, base)
+
+
+def test_solve_is_deterministic(device):
+    print("\n[determinism]", flush=True)
+    packed, control = PACKED[1][1], PACKED[1][2]
+    a, _ = B.solve_packed(device, control, packed)
+    b, _ = B.solve_packed(device, control, packed)
+    assert np.array_equal(a, b)

@@ -1,0 +1,650 @@
+// ba_engine.hip — the host side of the MI355X bundle-adjustment engine and its C ABI
+// (include/typecv_ba.h). One context per HIP device owns the device arrays (grown, never shrunk,
+// until tcv_ba_release), a stream, and pinned words for the flags the LM loop reads.
+//
+// Host loop (reference bundleAdjustGpuFixed, src/sfm/bundle/gpu/bundle.ts:178-799, and
+// bundleAdjustJointFocalGpu, joint-focal.ts:340-900): initialise and measure the cost; per LM
+// iteration assemble, initialise PCG, run PCG in chunks with one read of the state words after each
+// chunk, build and score the candidate, accept or reject, read the state; pack the result. Every
+// decision is taken on the device from the state words; the host only reads them to stop
+// launching work.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/typecv_ba.h"
+#include "ba_kernels.hip"
+
+using namespace tcvba;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+void set_error(const std::string& s) { g_last_error = s; }
+
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+}  // namespace
+
+struct tcv_ba_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  double* h_state = nullptr;  // pinned copy of the state words
+  uint32_t* h_error = nullptr;
+  std::vector<DevBuf*> all;
+  DevBuf scene, trial, index, out, tiles, tasks, task_start, cam_point, Wp, Wc, pblk, tile_fail,
+      tile_focal, task_part, camA, camInv, camRhs, camHcf, camFocal, vec, state, cost_part, error,
+      probe;
+  std::vector<hipEvent_t> events;
+  tcv_ba_context() {
+    all = {&scene, &trial,     &index,      &out,      &tiles,  &tasks,  &task_start, &cam_point,
+           &Wp,    &Wc,        &pblk,       &tile_fail, &tile_focal, &task_part, &camA, &camInv,
+           &camRhs, &camHcf,   &camFocal,   &vec,      &state,  &cost_part, &error, &probe};
+  }
+};
+
+#define HIP_TRY(call)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess) {                                                              \
+      set_error(std::string(#call) + ": " + hipGetErrorName(e_) + " (" +                \
+                hipGetErrorString(e_) + ")");                                            \
+      return e_ == hipErrorOutOfMemory ? TCV_BA_ERR_MEMORY : TCV_BA_ERR_DEVICE;          \
+    }                                                                                    \
+  } while (0)
+
+namespace {
+
+int ensure(DevBuf& b, size_t bytes) {
+  if (bytes == 0) bytes = 8;
+  if (b.bytes >= bytes) return TCV_BA_OK;
+  if (b.ptr) HIP_TRY(hipFree(b.ptr));
+  b.ptr = nullptr;
+  b.bytes = 0;
+  HIP_TRY(hipMalloc(&b.ptr, bytes));
+  b.bytes = bytes;
+  return TCV_BA_OK;
+}
+
+#define TRY(expr)               \
+  do {                          \
+    int s_ = (expr);            \
+    if (s_ != TCV_BA_OK) return s_; \
+  } while (0)
+
+// The checks of bundleProblem (src/sfm/bundle/gpu/bundle-host.ts:43-70) that the packed arrays
+// still carry, and the u32 bounds of bundle.ts:118-127.
+int check_control(const tcv_ba_control* c) {
+  if (!c) return TCV_BA_ERR_ARGUMENT;
+  const uint64_t C = c->cameras, P = c->points, M = c->observations;
+  const bool ok = C >= 2 && P >= 1 && c->directional_points < P && M >= 1 &&
+                  c->iterations <= 100 && c->max_cg >= 1 && c->max_cg <= 256 &&
+                  c->huber >= 1e-12 && c->huber <= 1e6 && c->relative_tolerance >= 1e-12 &&
+                  c->relative_tolerance <= 1e-2 && c->anchor < C && c->scale_view < C &&
+                  c->anchor != c->scale_view && c->normalize_scale <= 1 && c->shared_focal <= 1 &&
+                  (c->precision == TCV_BA_FP64 || c->precision == TCV_BA_MIXED) &&
+                  3 * M + P + 2 * C + 2 <= 0xffffffffull && 18 * C + 3 * P + 2 * M <= 0xffffffffull &&
+                  c->stall_tolerance >= 0 && c->stall_tolerance < 1;
+  if (!ok) {
+    set_error("control outside the admitted ranges (bundleProblem)");
+    return TCV_BA_ERR_ARGUMENT;
+  }
+  return TCV_BA_OK;
+}
+
+struct Plan {
+  std::vector<uint32_t> tiles, tasks, task_start;
+};
+
+// Wave tiles of whole points (at most 64 observations and 64 points; a longer point alone) and
+// camera-list tasks of at most `task_obs` slots (each camera has at least one task).
+int build_plan(const std::vector<uint32_t>& ps, const std::vector<uint32_t>& cs, uint32_t P,
+               uint32_t C, uint32_t M, uint32_t task_obs, Plan& plan) {
+  if (ps[0] != 0 || ps[P] != M || cs[0] != 0 || cs[C] != M) {
+    set_error("index offsets do not cover the observations");
+    return TCV_BA_ERR_INDEX;
+  }
+  for (uint32_t p = 0; p < P; p++)
+    if (ps[p + 1] < ps[p]) {
+      set_error("point offsets decrease");
+      return TCV_BA_ERR_INDEX;
+    }
+  for (uint32_t c = 0; c < C; c++)
+    if (cs[c + 1] < cs[c]) {
+      set_error("camera offsets decrease");
+      return TCV_BA_ERR_INDEX;
+    }
+  plan.tiles.clear();
+  plan.tiles.push_back(0);
+  uint32_t first = 0;
+  for (uint32_t p = 1; p < P; p++) {
+    const bool too_many_obs = ps[p + 1] - ps[first] > (uint32_t)TILE;
+    const bool too_many_points = p - first >= (uint32_t)TILE;
+    if (too_many_obs || too_many_points) {
+      plan.tiles.push_back(p);
+      first = p;
+    }
+  }
+  plan.tiles.push_back(P);
+  plan.tasks.clear();
+  plan.task_start.assign(C + 1, 0);
+  for (uint32_t c = 0; c < C; c++) {
+    plan.task_start[c] = (uint32_t)(plan.tasks.size() / 3);
+    const uint32_t s = cs[c], e = cs[c + 1];
+    uint32_t at = s;
+    do {
+      const uint32_t end = e - at > task_obs ? at + task_obs : e;
+      plan.tasks.push_back(c);
+      plan.tasks.push_back(at);
+      plan.tasks.push_back(end);
+      at = end;
+    } while (at < e);
+  }
+  plan.task_start[C] = (uint32_t)(plan.tasks.size() / 3);
+  return TCV_BA_OK;
+}
+
+uint32_t blocks_for(size_t n, uint32_t threads) { return (uint32_t)((n + threads - 1) / threads); }
+
+// The launches of one form (W storage type and route).
+template <class WT, bool SHARED>
+struct Launch {
+  static void point_pass(const Problem& q, hipStream_t s) {
+    hipLaunchKernelGGL((k_point_pass<WT, SHARED>), dim3(q.n_tiles), dim3(TILE), 0, s, q);
+  }
+  static void camera_pass(const Problem& q, hipStream_t s) {
+    hipLaunchKernelGGL((k_camera_pass<WT, SHARED>), dim3(q.n_tasks), dim3(TASK_THREADS), 0, s, q);
+    hipLaunchKernelGGL((k_camera_finish<SHARED>), dim3(blocks_for(q.C, 64)), dim3(64), 0, s, q);
+  }
+  static void init_cg(const Problem& q, hipStream_t s) {
+    hipLaunchKernelGGL((k_init_cg<SHARED>), dim3(1), dim3(CG_THREADS), 0, s, q);
+  }
+  static void point_sweep(const Problem& q, hipStream_t s) {
+    hipLaunchKernelGGL((k_point_sweep<WT, SHARED, SWEEP_SCHUR>), dim3(q.n_tiles), dim3(TILE), 0, s,
+                       q);
+  }
+  static void camera_sweep(const Problem& q, hipStream_t s) {
+    hipLaunchKernelGGL((k_camera_sweep<WT, SHARED>), dim3(q.n_tasks), dim3(TASK_THREADS), 0, s, q);
+  }
+  static void advance(const Problem& q, hipStream_t s) {
+    hipLaunchKernelGGL((k_cg_advance<SHARED>), dim3(1), dim3(CG_THREADS), 0, s, q);
+  }
+  static void update_points(const Problem& q, hipStream_t s) {
+    hipLaunchKernelGGL((k_point_sweep<WT, SHARED, SWEEP_UPDATE>), dim3(q.n_tiles), dim3(TILE), 0,
+                       s, q);
+  }
+  static void probe(const Problem& q, const double* v, double* sv, double* zv, double* rhs,
+                    hipStream_t s) {
+    hipLaunchKernelGGL((k_probe<SHARED>), dim3(1), dim3(CG_THREADS), 0, s, q, v, sv, zv, rhs);
+  }
+};
+
+struct Ops {
+  void (*point_pass)(const Problem&, hipStream_t);
+  void (*camera_pass)(const Problem&, hipStream_t);
+  void (*init_cg)(const Problem&, hipStream_t);
+  void (*point_sweep)(const Problem&, hipStream_t);
+  void (*camera_sweep)(const Problem&, hipStream_t);
+  void (*advance)(const Problem&, hipStream_t);
+  void (*update_points)(const Problem&, hipStream_t);
+  void (*probe)(const Problem&, const double*, double*, double*, double*, hipStream_t);
+};
+
+template <class WT, bool SHARED>
+Ops ops_for() {
+  using L = Launch<WT, SHARED>;
+  return Ops{L::point_pass,   L::camera_pass, L::init_cg,       L::point_sweep,
+             L::camera_sweep, L::advance,     L::update_points, L::probe};
+}
+
+Ops select_ops(const tcv_ba_control* c) {
+  const bool shared = c->shared_focal != 0;
+  if (c->precision == TCV_BA_MIXED) return shared ? ops_for<float, true>() : ops_for<float, false>();
+  return shared ? ops_for<double, true>() : ops_for<double, false>();
+}
+
+constexpr uint32_t kTaskObs = 1024;
+
+// Allocate, validate the index and build the plan. `scene`/`index` are device pointers.
+int setup(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene, const uint32_t* index,
+          hipStream_t st, Problem& q, tcv_ba_report* rep) {
+  const double t0 = now_ms();
+  TRY(check_control(c));
+  const uint32_t C = c->cameras, P = c->points, M = c->observations;
+  const uint32_t shared = c->shared_focal ? 1u : 0u;
+  const size_t scene_size = (size_t)C * 18 + (size_t)P * 3 + (size_t)M * 2;
+  const size_t trial_size = (size_t)C * 12 + (size_t)P * 3;
+  const size_t ws = c->precision == TCV_BA_MIXED ? 4 : 8;
+  const uint32_t K = shared ? 21 : 18;
+  const uint32_t m = C * 6 + shared;
+  // offsets of both sections, read back for the plan
+  std::vector<uint32_t> ps(P + 1), cs(C + 1);
+  HIP_TRY(hipMemcpyAsync(ps.data(), index + 2 * (size_t)M, (P + 1) * 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(cs.data(), index + 2 * (size_t)M + P + 1, (C + 1) * 4,
+                         hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  Plan plan;
+  TRY(build_plan(ps, cs, P, C, M, kTaskObs, plan));
+  const uint32_t n_tiles = (uint32_t)plan.tiles.size() - 1;
+  const uint32_t n_tasks = (uint32_t)plan.tasks.size() / 3;
+  const uint32_t n_cost = blocks_for(M, 256);
+  TRY(ensure(ctx->scene, scene_size * 8));
+  TRY(ensure(ctx->trial, trial_size * 8));
+  TRY(ensure(ctx->tiles, plan.tiles.size() * 4));
+  TRY(ensure(ctx->tasks, plan.tasks.size() * 4));
+  TRY(ensure(ctx->task_start, plan.task_start.size() * 4));
+  TRY(ensure(ctx->cam_point, (size_t)M * 4));
+  TRY(ensure(ctx->Wp, (size_t)K * M * ws));
+  TRY(ensure(ctx->Wc, (size_t)K * M * ws));
+  TRY(ensure(ctx->pblk, (size_t)P * 12 * 8));
+  TRY(ensure(ctx->tile_fail, (size_t)n_tiles * 4));
+  TRY(ensure(ctx->tile_focal, (size_t)n_tiles * 16));
+  TRY(ensure(ctx->task_part, (size_t)n_tasks * ASM_WORDS * 8));
+  TRY(ensure(ctx->camA, (size_t)C * 36 * 8));
+  TRY(ensure(ctx->camInv, (size_t)C * 36 * 8));
+  TRY(ensure(ctx->camRhs, (size_t)C * 6 * 8));
+  TRY(ensure(ctx->camHcf, (size_t)C * 6 * 8));
+  TRY(ensure(ctx->camFocal, (size_t)C * 2 * 8));
+  TRY(ensure(ctx->vec, ((size_t)m * 5 + (size_t)P * 3 + C) * 8));
+  TRY(ensure(ctx->state, S_WORDS * 8));
+  TRY(ensure(ctx->cost_part, (size_t)n_cost * 3 * 8));
+  TRY(ensure(ctx->error, 8));
+  HIP_TRY(hipMemcpyAsync(ctx->tiles.ptr, plan.tiles.data(), plan.tiles.size() * 4,
+                         hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(ctx->tasks.ptr, plan.tasks.data(), plan.tasks.size() * 4,
+                         hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(ctx->task_start.ptr, plan.task_start.data(), plan.task_start.size() * 4,
+                         hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(ctx->scene.ptr, scene, scene_size * 8, hipMemcpyDeviceToDevice, st));
+  HIP_TRY(hipMemsetAsync(ctx->trial.ptr, 0, trial_size * 8, st));
+  HIP_TRY(hipMemsetAsync(ctx->vec.ptr, 0, ((size_t)m * 5 + (size_t)P * 3 + C) * 8, st));
+  HIP_TRY(hipMemsetAsync(ctx->error.ptr, 0, 8, st));
+  q = Problem{};
+  q.C = C;
+  q.P = P;
+  q.M = M;
+  q.m = m;
+  q.K = K;
+  q.shared = shared;
+  q.first_directional = P - c->directional_points;
+  q.anchor = c->anchor;
+  q.scale_view = c->scale_view;
+  q.normalize_scale = c->normalize_scale;
+  q.n_tiles = n_tiles;
+  q.n_tasks = n_tasks;
+  q.huber = c->huber;
+  q.tol = c->relative_tolerance;
+  q.scene = (double*)ctx->scene.ptr;
+  q.trial = (double*)ctx->trial.ptr;
+  q.index = index;
+  q.tiles = (const uint32_t*)ctx->tiles.ptr;
+  q.tasks = (const uint32_t*)ctx->tasks.ptr;
+  q.task_start = (const uint32_t*)ctx->task_start.ptr;
+  q.cam_point = (uint32_t*)ctx->cam_point.ptr;
+  q.Wp = ctx->Wp.ptr;
+  q.Wc = ctx->Wc.ptr;
+  q.pblk = (double*)ctx->pblk.ptr;
+  q.tile_fail = (uint32_t*)ctx->tile_fail.ptr;
+  q.tile_focal = (double*)ctx->tile_focal.ptr;
+  q.task_part = (double*)ctx->task_part.ptr;
+  q.camA = (double*)ctx->camA.ptr;
+  q.camInv = (double*)ctx->camInv.ptr;
+  q.camRhs = (double*)ctx->camRhs.ptr;
+  q.camHcf = (double*)ctx->camHcf.ptr;
+  q.camFocal = (double*)ctx->camFocal.ptr;
+  q.vec = (double*)ctx->vec.ptr;
+  q.state = (double*)ctx->state.ptr;
+  q.cost_part = (double*)ctx->cost_part.ptr;
+  q.error = (uint32_t*)ctx->error.ptr;
+  q.n_cost_blocks = n_cost;
+  hipLaunchKernelGGL(k_check_points, dim3(blocks_for(P, 256)), dim3(256), 0, st, q);
+  hipLaunchKernelGGL(k_check_tasks, dim3(n_tasks), dim3(256), 0, st, q);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(ctx->h_error, ctx->error.ptr, 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if (*ctx->h_error != 0) {
+    set_error(*ctx->h_error & 1 ? "observation point/camera ids do not match the point offsets"
+                                : "camera lists do not match the observation cameras");
+    return TCV_BA_ERR_INDEX;
+  }
+  if (rep) {
+    rep->setup_ms = now_ms() - t0;
+    rep->point_tiles = n_tiles;
+    rep->camera_tasks = n_tasks;
+  }
+  return TCV_BA_OK;
+}
+
+int read_state(tcv_ba_context* ctx, const Problem& q, hipStream_t st, tcv_ba_report* rep) {
+  HIP_TRY(hipMemcpyAsync(ctx->h_state, q.state, S_WORDS * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if (rep) rep->flag_reads++;
+  return TCV_BA_OK;
+}
+
+int ensure_events(tcv_ba_context* ctx, size_t n) {
+  while (ctx->events.size() < n) {
+    hipEvent_t e;
+    HIP_TRY(hipEventCreate(&e));
+    ctx->events.push_back(e);
+  }
+  return TCV_BA_OK;
+}
+
+// The LM loop on a prepared problem.
+int run_lm(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, hipStream_t st,
+           tcv_ba_report* rep) {
+  const Ops ops = select_ops(c);
+  const bool timing = (c->flags & TCV_BA_FLAG_TIMING) != 0 && rep;
+  hipLaunchKernelGGL(k_init_bundle, dim3(1), dim3(64), 0, st, q);
+  hipLaunchKernelGGL(k_cost<false>, dim3(q.n_cost_blocks), dim3(256), 0, st, q);
+  hipLaunchKernelGGL(k_finish_cost<false>, dim3(1), dim3(CG_THREADS), 0, st, q);
+  HIP_TRY(hipGetLastError());
+  TRY(read_state(ctx, q, st, rep));
+  bool lm_active = ctx->h_state[S_ACTIVE] != 0;
+  double monitor_cost = ctx->h_state[S_COST];
+  const bool monitor = c->stall_tolerance > 1e-9;
+  uint32_t hint = 8;
+  const uint32_t update_blocks = blocks_for(q.C > q.P ? q.C : q.P, 256);
+  const uint32_t trial_blocks = blocks_for((size_t)q.C * 12 + (size_t)q.P * 3, 256);
+  if (timing) TRY(ensure_events(ctx, 3 * 256 + 3));
+  for (uint32_t iteration = 0; iteration < c->iterations && lm_active; iteration++) {
+    if (rep) rep->lm_iterations++;
+    ops.point_pass(q, st);
+    ops.camera_pass(q, st);
+    ops.init_cg(q, st);
+    HIP_TRY(hipGetLastError());
+    uint32_t done = 0, chunk = hint;
+    while (done < c->max_cg) {
+      const uint32_t n = chunk < c->max_cg - done ? chunk : c->max_cg - done;
+      for (uint32_t k = 0; k < n; k++) {
+        if (timing) HIP_TRY(hipEventRecord(ctx->events[3 * k], st));
+        ops.point_sweep(q, st);
+        if (timing) HIP_TRY(hipEventRecord(ctx->events[3 * k + 1], st));
+        ops.camera_sweep(q, st);
+        if (timing) HIP_TRY(hipEventRecord(ctx->events[3 * k + 2], st));
+        ops.advance(q, st);
+      }
+      HIP_TRY(hipGetLastError());
+      if (rep) rep->matvecs += n;
+      done += n;
+      TRY(read_state(ctx, q, st, rep));
+      if (timing) {
+        for (uint32_t k = 0; k < n; k++) {
+          float a = 0, b = 0;
+          HIP_TRY(hipEventElapsedTime(&a, ctx->events[3 * k], ctx->events[3 * k + 1]));
+          HIP_TRY(hipEventElapsedTime(&b, ctx->events[3 * k + 1], ctx->events[3 * k + 2]));
+          rep->schur_point_ms += a;
+          rep->schur_camera_ms += b;
+          rep->schur_ms += a + b;
+        }
+      }
+      if (ctx->h_state[S_PCG_ACTIVE] == 0) break;
+      chunk = chunk * 2 > 64 ? 64 : chunk * 2;
+    }
+    // The next LM iteration starts with a chunk as long as this one's PCG run.
+    const uint32_t steps = (uint32_t)ctx->h_state[S_STEPS];
+    hint = steps < 1 ? 1 : (steps > 64 ? 64 : steps);
+    hipLaunchKernelGGL(k_update_cameras, dim3(blocks_for(q.C, 64)), dim3(64), 0, st, q);
+    ops.update_points(q, st);
+    hipLaunchKernelGGL(k_gauge, dim3(1), dim3(256), 0, st, q);
+    hipLaunchKernelGGL(k_normalize, dim3(update_blocks), dim3(256), 0, st, q);
+    hipLaunchKernelGGL(k_cost<true>, dim3(q.n_cost_blocks), dim3(256), 0, st, q);
+    hipLaunchKernelGGL(k_finish_cost<true>, dim3(1), dim3(CG_THREADS), 0, st, q);
+    hipLaunchKernelGGL(k_commit, dim3(trial_blocks), dim3(256), 0, st, q);
+    HIP_TRY(hipGetLastError());
+    TRY(read_state(ctx, q, st, rep));
+    lm_active = ctx->h_state[S_ACTIVE] != 0;
+    if (monitor && lm_active) {
+      const double next = ctx->h_state[S_COST], change = monitor_cost - next;
+      const bool accepted = ctx->h_state[S_COMMIT] == 1;
+      monitor_cost = next;
+      if (accepted && change < c->stall_tolerance * (next > 1 ? next : 1)) lm_active = false;
+    }
+  }
+  if (rep) {
+    rep->cg_iterations = (uint64_t)ctx->h_state[S_CG_STEPS];
+    rep->lm_iterations = (uint64_t)(ctx->h_state[S_ACCEPTED] + ctx->h_state[S_REJECTED]);
+  }
+  return TCV_BA_OK;
+}
+
+uint64_t held_bytes(const tcv_ba_context* ctx) {
+  uint64_t total = 0;
+  for (const DevBuf* b : ctx->all) total += b->bytes;
+  return total;
+}
+
+int solve_device(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene,
+                 const uint32_t* index, double* out, tcv_ba_report* rep, hipStream_t st) {
+  const double t0 = now_ms();
+  HIP_TRY(hipSetDevice(ctx->device));
+  Problem q;
+  TRY(setup(ctx, c, scene, index, st, q, rep));
+  TRY(run_lm(ctx, c, q, st, rep));
+  const size_t total = (size_t)q.C * 12 + (size_t)q.P * 3 + 27;
+  hipLaunchKernelGGL(k_pack, dim3(blocks_for(total, 256)), dim3(256), 0, st, q, out);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(st));
+  if (rep) {
+    rep->solve_ms = now_ms() - t0;
+    rep->device_bytes = held_bytes(ctx);
+  }
+  return TCV_BA_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C ABI
+
+extern "C" {
+
+size_t tcv_ba_output_size(const tcv_ba_control* c) {
+  return (size_t)c->cameras * 12 + (size_t)c->points * 3 + TCV_BA_STATE_WORDS +
+         (c->shared_focal ? 1 : 0);
+}
+size_t tcv_ba_scene_size(const tcv_ba_control* c) {
+  return (size_t)c->cameras * 18 + (size_t)c->points * 3 + (size_t)c->observations * 2;
+}
+size_t tcv_ba_index_size(const tcv_ba_control* c) {
+  return 3 * (size_t)c->observations + c->points + 2 * (size_t)c->cameras + 2;
+}
+
+const char* tcv_ba_status_string(int status) {
+  switch (status) {
+    case TCV_BA_OK: return "ok";
+    case TCV_BA_ERR_ARGUMENT: return "Invalid GPU BA scene/options";
+    case TCV_BA_ERR_INDEX: return "GPU BA index arrays are not a packed scene of these sizes";
+    case TCV_BA_ERR_DEVICE: return "GPU BA device call failed";
+    case TCV_BA_ERR_MEMORY: return "GPU BA needs more device memory than is free";
+    case TCV_BA_ERR_NO_DEVICE: return "no gfx950 HIP device is visible";
+    default: return "unknown status";
+  }
+}
+
+const char* tcv_ba_last_error(void) { return g_last_error.c_str(); }
+
+int tcv_ba_create(int device, tcv_ba_context** out) {
+  if (!out) return TCV_BA_ERR_ARGUMENT;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) {
+    set_error("no HIP device " + std::to_string(device));
+    return TCV_BA_ERR_NO_DEVICE;
+  }
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, device));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    set_error(std::string("device is ") + prop.gcnArchName + ", the engine is built for gfx950");
+    return TCV_BA_ERR_NO_DEVICE;
+  }
+  HIP_TRY(hipSetDevice(device));
+  tcv_ba_context* ctx = new tcv_ba_context();
+  ctx->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&ctx->h_state, S_WORDS * 8, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&ctx->h_error, 8, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    set_error(std::string("context set-up: ") + hipGetErrorName(e));
+    tcv_ba_destroy(ctx);
+    return TCV_BA_ERR_DEVICE;
+  }
+  *out = ctx;
+  return TCV_BA_OK;
+}
+
+int tcv_ba_release(tcv_ba_context* ctx) {
+  if (!ctx) return TCV_BA_ERR_ARGUMENT;
+  HIP_TRY(hipSetDevice(ctx->device));
+  if (ctx->stream) HIP_TRY(hipStreamSynchronize(ctx->stream));
+  for (DevBuf* b : ctx->all) {
+    if (b->ptr) HIP_TRY(hipFree(b->ptr));
+    b->ptr = nullptr;
+    b->bytes = 0;
+  }
+  return TCV_BA_OK;
+}
+
+void tcv_ba_destroy(tcv_ba_context* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  tcv_ba_release(ctx);
+  for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
+  if (ctx->h_state) (void)hipHostFree(ctx->h_state);
+  if (ctx->h_error) (void)hipHostFree(ctx->h_error);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+uint64_t tcv_ba_device_bytes(const tcv_ba_context* ctx) { return ctx ? held_bytes(ctx) : 0; }
+
+int tcv_ba_solve_device(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene,
+                        const uint32_t* index, double* out, tcv_ba_report* rep, void* stream) {
+  if (!ctx || !c || !scene || !index || !out) return TCV_BA_ERR_ARGUMENT;
+  if (rep) std::memset(rep, 0, sizeof *rep);
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  return solve_device(ctx, c, scene, index, out, rep, st);
+}
+
+int tcv_ba_solve(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene,
+                 const uint32_t* index, double* out, tcv_ba_report* rep) {
+  if (!ctx || !c || !scene || !index || !out) return TCV_BA_ERR_ARGUMENT;
+  if (rep) std::memset(rep, 0, sizeof *rep);
+  TRY(check_control(c));
+  const double t0 = now_ms();
+  HIP_TRY(hipSetDevice(ctx->device));
+  const size_t ss = tcv_ba_scene_size(c), is = tcv_ba_index_size(c), os = tcv_ba_output_size(c);
+  // The inputs live in the probe/out/index buffers of the context: scene staging shares `probe`.
+  TRY(ensure(ctx->probe, ss * 8));
+  TRY(ensure(ctx->index, is * 4));
+  TRY(ensure(ctx->out, os * 8));
+  hipStream_t st = ctx->stream;
+  HIP_TRY(hipMemcpyAsync(ctx->probe.ptr, scene, ss * 8, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(ctx->index.ptr, index, is * 4, hipMemcpyHostToDevice, st));
+  TRY(solve_device(ctx, c, (const double*)ctx->probe.ptr, (const uint32_t*)ctx->index.ptr,
+                   (double*)ctx->out.ptr, rep, st));
+  HIP_TRY(hipMemcpyAsync(out, ctx->out.ptr, os * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if (rep) rep->device_bytes = held_bytes(ctx);
+  (void)t0;
+  return TCV_BA_OK;
+}
+
+int tcv_ba_schur_probe(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene,
+                       const uint32_t* index, double lambda, double log_focal, const double* p,
+                       double* rhs, double* sp, double* z, int32_t* status, uint32_t repeat,
+                       tcv_ba_report* rep) {
+  if (!ctx || !c || !scene || !index || !p || !rhs || !sp || !z || !status)
+    return TCV_BA_ERR_ARGUMENT;
+  if (rep) std::memset(rep, 0, sizeof *rep);
+  TRY(check_control(c));
+  HIP_TRY(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const size_t ss = tcv_ba_scene_size(c), is = tcv_ba_index_size(c);
+  const uint32_t m = c->cameras * 6 + (c->shared_focal ? 1 : 0);
+  TRY(ensure(ctx->probe, ss * 8));
+  TRY(ensure(ctx->index, is * 4));
+  TRY(ensure(ctx->out, (size_t)m * 4 * 8));
+  HIP_TRY(hipMemcpyAsync(ctx->probe.ptr, scene, ss * 8, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(ctx->index.ptr, index, is * 4, hipMemcpyHostToDevice, st));
+  Problem q;
+  TRY(setup(ctx, c, (const double*)ctx->probe.ptr, (const uint32_t*)ctx->index.ptr, st, q, rep));
+  const Ops ops = select_ops(c);
+  // state: lambda, LM active, the log focal scale, PCG active (so that the sweeps run)
+  std::vector<double> init(S_WORDS, 0.0);
+  init[S_LAMBDA] = lambda;
+  init[S_ACTIVE] = 1;
+  init[S_LOG_FOCAL] = log_focal;
+  init[S_PCG_ACTIVE] = 1;
+  HIP_TRY(hipMemcpyAsync(q.state, init.data(), S_WORDS * 8, hipMemcpyHostToDevice, st));
+  double* dv = (double*)ctx->out.ptr;
+  double* dsp = dv + m;
+  double* dz = dv + 2 * (size_t)m;
+  double* drhs = dv + 3 * (size_t)m;
+  HIP_TRY(hipMemcpyAsync(dv, p, (size_t)m * 8, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(q.vec + 2 * (size_t)m, p, (size_t)m * 8, hipMemcpyHostToDevice, st));
+  ops.point_pass(q, st);
+  ops.camera_pass(q, st);
+  // the focal scalars of initializeBundleCg without touching the PCG words
+  ops.init_cg(q, st);
+  HIP_TRY(hipMemcpyAsync(q.vec + 2 * (size_t)m, p, (size_t)m * 8, hipMemcpyHostToDevice, st));
+  const bool timing = (c->flags & TCV_BA_FLAG_TIMING) != 0 && rep;
+  if (repeat < 1) repeat = 1;
+  TRY(ensure_events(ctx, 3));
+  // keep the sweeps running: PCG active
+  const double one = 1;
+  HIP_TRY(hipMemcpyAsync(q.state + S_PCG_ACTIVE, &one, 8, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  for (uint32_t r = 0; r < repeat; r++) {
+    if (timing) HIP_TRY(hipEventRecord(ctx->events[0], st));
+    ops.point_sweep(q, st);
+    if (timing) HIP_TRY(hipEventRecord(ctx->events[1], st));
+    ops.camera_sweep(q, st);
+    if (timing) HIP_TRY(hipEventRecord(ctx->events[2], st));
+    if (timing) {
+      HIP_TRY(hipEventSynchronize(ctx->events[2]));
+      float a = 0, b = 0;
+      HIP_TRY(hipEventElapsedTime(&a, ctx->events[0], ctx->events[1]));
+      HIP_TRY(hipEventElapsedTime(&b, ctx->events[1], ctx->events[2]));
+      rep->schur_point_ms += a;
+      rep->schur_camera_ms += b;
+      rep->schur_ms += a + b;
+    }
+  }
+  if (rep) rep->matvecs = repeat;
+  ops.probe(q, dv, dsp, dz, drhs, st);
+  HIP_TRY(hipGetLastError());
+  // validity: every point tile and camera block
+  std::vector<uint32_t> fail(q.n_tiles);
+  std::vector<double> inv((size_t)q.C * 36);
+  HIP_TRY(hipMemcpyAsync(fail.data(), q.tile_fail, (size_t)q.n_tiles * 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(inv.data(), q.camInv, (size_t)q.C * 36 * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(sp, dsp, (size_t)m * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(z, dz, (size_t)m * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(rhs, drhs, (size_t)m * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(ctx->h_state, q.state, S_WORDS * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  int32_t bad = 0;
+  for (uint32_t t = 0; t < q.n_tiles; t++) bad |= fail[t] != 0;
+  for (uint32_t k = 0; k < q.C; k++) bad |= !(inv[(size_t)k * 36] > 0);
+  if (c->shared_focal && !(ctx->h_state[S_FOCAL_INV] > 0)) bad = 1;
+  *status = bad;
+  if (rep) rep->device_bytes = held_bytes(ctx);
+  return TCV_BA_OK;
+}
+
+}  // extern "C"

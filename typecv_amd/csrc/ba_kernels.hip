@@ -1,0 +1,1047 @@
+// ba_kernels.hip — gfx950 kernels of the bundle-adjustment LM loop (included by ba_engine.hip).
+//
+// Kernel map (reference entry points in src/sfm/bundle/gpu/bundle.shade.ts "b:" and
+// joint-focal.shade.ts "j:"):
+//   k_check_points / k_check_tasks   index validation, camera-major point ids (host packing checks)
+//   k_init_bundle                    initializeBundle b:235-250 / j:269-286
+//   k_cost / k_finish_cost           bundleMetricPartials + finishBundleMetrics b:319-377,
+//                                    bundleCandidatePartials + acceptBundleCandidate b:1040-1096
+//   k_point_pass                     linearizeBundle + bundlePointBlocks b:251-297, 378-422 and the
+//                                    focal point terms of bundleFocalPoints j:1181-1204, fused:
+//                                    one wave per tile of whole points, one lane per observation
+//   k_camera_pass / k_camera_finish  bundleCameraBlocks / bundleCameraRows b:423-555 with
+//                                    invertBundleCameraColumns b:655-694 and the focal camera terms
+//                                    bundleFocalCameraRows/Observations j:1138-1171: one workgroup
+//                                    per camera-list task, fixed-order task sums, Cholesky inverse
+//   k_init_cg                        initializeBundleCg b:695-741 / j:646-700 with dampBundleFocal
+//   k_point_sweep<SCHUR>             bundleSchurPoint b:742-745 (body b:185-203, j:198-218)
+//   k_camera_sweep                   bundleSchurCamera* b:746-882 / j:705-870 (camera partials)
+//   k_cg_advance                     finishes S·p (A p, focal row) then advanceBundleCg b:883-929
+//   k_update_cameras / k_point_sweep<UPDATE>   updateBundleCandidate b:930-961 / j:925-960
+//   k_gauge                          bundleCandidateGauge b:962-993 / j:961-1000
+//   k_normalize                      normalizeBundleCandidate b:994-1023
+//   k_commit / k_pack                commitBundleCandidate b:1109-1114, packBundleResult b:1115-1125
+#include "ba_kernels.h"
+
+namespace tcvba {
+
+// ------------------------------------------------------------------ small helpers
+
+__device__ __forceinline__ bool variable_camera(const Problem& q, uint32_t c) {
+  return camera_active(q)[c] != 0;
+}
+__device__ __forceinline__ bool directional_point(const Problem& q, uint32_t p) {
+  return p >= q.first_directional;
+}
+
+// Model of camera c at log focal scale `fscale` (= focalScale(log), 1 on the fixed route).
+__device__ __forceinline__ Model camera_model(const Problem& q, uint32_t c, double fscale) {
+  const double* s = q.scene + (size_t)q.C * 12 + (size_t)q.P * 3 + (size_t)c * 6;
+  Model k{s[0], s[1], s[2], s[3], s[4], s[5]};
+  if (q.shared) {
+    k.fx = s[0] * fscale;
+    k.fy = s[1] * fscale;
+  }
+  return k;
+}
+
+// The sum of v over the workgroup, identical in every thread (butterfly within each wave, then
+// the wave sums in wave order). `lds` holds one double per wave. Call from uniform control flow.
+__device__ __forceinline__ double block_sum(double v, double* lds) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, waves = blockDim.x >> 6;
+  __syncthreads();
+  if (lane == 0) lds[wave] = v;
+  __syncthreads();
+  double s = 0;
+  for (int w = 0; w < waves; w++) s += lds[w];
+  return s;
+}
+
+// Linearisation of one observation (linearizeBundle b:251-297 / j:287-336).
+struct Lin {
+  double Jc[12];  // d(u,v)/d(omega, v): row u then row v (zero for a fixed camera)
+  double Jp[6];   // d(u,v)/dX: row u then row v
+  double r[2];    // projected - observed pixel
+  double w;       // Huber weight
+  double Jf[2];   // d(u,v)/d(log f) = pixel - principal point
+};
+
+__device__ __forceinline__ bool linearize(const Problem& q, uint32_t c, uint32_t p, uint32_t i,
+                                          double fscale, bool var, Lin& L) {
+  const double* pose = q.scene + (size_t)c * 12;
+  const double* X = q.scene + (size_t)q.C * 12 + (size_t)p * 3;
+  const bool dir = directional_point(q, p);
+  double R[12];
+#pragma unroll
+  for (int j = 0; j < 12; j++) R[j] = pose[j];
+  const double x0 = X[0], x1 = X[1], x2 = X[2];
+  double qc[3];
+#pragma unroll
+  for (int r = 0; r < 3; r++) {
+    qc[r] = R[r * 3] * x0 + R[r * 3 + 1] * x1 + R[r * 3 + 2] * x2;
+    if (!dir) qc[r] += R[9 + r];
+  }
+  const Model k = camera_model(q, c, fscale);
+  double px, py, du[3], dv[3];
+  if (!project(k, qc, px, py, du, dv)) return false;
+  const double* obs = q.scene + (size_t)q.C * 18 + (size_t)q.P * 3 + (size_t)i * 2;
+  const double ru = px - obs[0], rv = py - obs[1];
+  const double squared = ru * ru + rv * rv, error = sqrt(squared);
+  L.w = error > q.huber ? q.huber / error : 1.0;
+  L.r[0] = ru;
+  L.r[1] = rv;
+  if (var) {
+    // poseRow, numerics.shade.ts:181-183; directional points have no translation columns
+    L.Jc[0] = du[2] * qc[1] - du[1] * qc[2];
+    L.Jc[1] = du[0] * qc[2] - du[2] * qc[0];
+    L.Jc[2] = du[1] * qc[0] - du[0] * qc[1];
+    L.Jc[6] = dv[2] * qc[1] - dv[1] * qc[2];
+    L.Jc[7] = dv[0] * qc[2] - dv[2] * qc[0];
+    L.Jc[8] = dv[1] * qc[0] - dv[0] * qc[1];
+    L.Jc[3] = dir ? 0.0 : du[0];
+    L.Jc[4] = dir ? 0.0 : du[1];
+    L.Jc[5] = dir ? 0.0 : du[2];
+    L.Jc[9] = dir ? 0.0 : dv[0];
+    L.Jc[10] = dir ? 0.0 : dv[1];
+    L.Jc[11] = dir ? 0.0 : dv[2];
+  } else {
+#pragma unroll
+    for (int j = 0; j < 12; j++) L.Jc[j] = 0;
+  }
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    L.Jp[j] = du[0] * R[j] + du[1] * R[3 + j] + du[2] * R[6 + j];
+    L.Jp[3 + j] = dv[0] * R[j] + dv[1] * R[3 + j] + dv[2] * R[6 + j];
+  }
+  L.Jf[0] = px - k.cx;
+  L.Jf[1] = py - k.cy;
+  return true;
+}
+
+// Cross block W = w Jc^T Jp (18 words, [k*3+j]) and the focal row w Jf^T Jp (3 words).
+__device__ __forceinline__ void cross_block(const Lin& L, bool shared, double W[21]) {
+#pragma unroll
+  for (int k = 0; k < 6; k++)
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+      W[k * 3 + j] = L.w * L.Jc[k] * L.Jp[j] + L.w * L.Jc[6 + k] * L.Jp[3 + j];
+  if (shared) {
+#pragma unroll
+    for (int j = 0; j < 3; j++) W[18 + j] = L.w * L.Jf[0] * L.Jp[j] + L.w * L.Jf[1] * L.Jp[3 + j];
+  }
+}
+
+template <class WT>
+__device__ __forceinline__ WT* wrow(void* base, uint32_t k, uint32_t M) {
+  return static_cast<WT*>(base) + (size_t)k * M;
+}
+
+// ------------------------------------------------------------------ index checks
+
+// Every observation of point p names p, a camera below C and lies in the point's range.
+__global__ void k_check_points(Problem q) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= q.P) return;
+  const uint32_t* ps = point_start(q);
+  for (uint32_t i = ps[p]; i < ps[p + 1]; i++)
+    if (obs_point(q)[i] != p || obs_camera(q)[i] >= q.C) atomicOr(q.error, 1u);
+}
+
+// Every slot of a camera task names an observation of that camera, ascending within the camera's
+// list; writes the camera-major point ids.
+__global__ void k_check_tasks(Problem q) {
+  const uint32_t t = blockIdx.x;
+  const uint32_t c = q.tasks[3 * t], s = q.tasks[3 * t + 1], e = q.tasks[3 * t + 2];
+  const uint32_t list = camera_start(q)[c];
+  for (uint32_t n = s + threadIdx.x; n < e; n += blockDim.x) {
+    const uint32_t i = camera_obs(q)[n];
+    bool bad = i >= q.M;
+    if (!bad) bad = obs_camera(q)[i] != c;
+    if (!bad && n > list) bad = camera_obs(q)[n - 1] >= i;
+    if (bad) {
+      atomicOr(q.error, 2u);
+      q.cam_point[n] = 0;
+    } else {
+      q.cam_point[n] = obs_point(q)[i];
+    }
+  }
+}
+
+// ------------------------------------------------------------------ LM set-up and cost
+
+__device__ __forceinline__ void center_of(const double* p, double o[3]) {
+  o[0] = -p[0] * p[9] - p[3] * p[10] - p[6] * p[11];
+  o[1] = -p[1] * p[9] - p[4] * p[10] - p[7] * p[11];
+  o[2] = -p[2] * p[9] - p[5] * p[10] - p[8] * p[11];
+}
+
+// initializeBundle: lambda 1e-3, gauge origin at the anchor centre, baseline to the scale view.
+__global__ void k_init_bundle(Problem q) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double* s = q.state;
+  for (int j = 0; j < S_WORDS; j++) s[j] = 0;
+  s[S_LAMBDA] = 1e-3;
+  s[S_ACTIVE] = 1;
+  double o[3], v[3];
+  center_of(q.scene + (size_t)q.anchor * 12, o);
+  center_of(q.scene + (size_t)q.scale_view * 12, v);
+  const double d0 = o[0] - v[0], d1 = o[1] - v[1], d2 = o[2] - v[2];
+  const double baseline = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+  s[S_BASELINE] = baseline;
+  s[S_ORIGIN] = o[0];
+  s[S_ORIGIN + 1] = o[1];
+  s[S_ORIGIN + 2] = o[2];
+  if (!(baseline > 1e-10)) {
+    s[S_INPUT_ERROR] = 1;
+    s[S_ACTIVE] = 0;
+  }
+}
+
+// Robust cost, squared error and invalid count of 256 observations per workgroup: of the scene
+// (metricLane b:301-314) or of the trial (candidateLane b:1024-1037, 1e30 for an invalid lane).
+template <bool CANDIDATE>
+__global__ __launch_bounds__(256) void k_cost(Problem q) {
+  __shared__ double lds[3][4];
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  double cost = 0, sq = 0, bad = 0;
+  if (i < q.M) {
+    const uint32_t c = obs_camera(q)[i], p = obs_point(q)[i];
+    const double* base = CANDIDATE ? q.trial : q.scene;
+    const double* pose = base + (size_t)c * 12;
+    const double* X = base + (size_t)q.C * 12 + (size_t)p * 3;
+    const double fscale = q.shared ? focal_scale(q.state[CANDIDATE ? S_TRIAL_FOCAL : S_LOG_FOCAL]) : 1.0;
+    bool ok = true;
+    if (CANDIDATE) ok = fmax(fabs(X[0]), fmax(fabs(X[1]), fabs(X[2]))) <= 1e9;
+    double px = 0, py = 0, du[3], dv[3];
+    if (ok) {
+      const bool dir = directional_point(q, p);
+      double qc[3];
+      for (int r = 0; r < 3; r++) {
+        qc[r] = pose[r * 3] * X[0] + pose[r * 3 + 1] * X[1] + pose[r * 3 + 2] * X[2];
+        if (!dir) qc[r] += pose[9 + r];
+      }
+      ok = project(camera_model(q, c, fscale), qc, px, py, du, dv);
+    }
+    if (ok) {
+      const double* obs = q.scene + (size_t)q.C * 18 + (size_t)q.P * 3 + (size_t)i * 2;
+      const double x = px - obs[0], y = py - obs[1];
+      sq = x * x + y * y;
+      cost = robust_cost(sqrt(sq), q.huber);
+    } else if (CANDIDATE) {
+      cost = sq = 1e30;
+    } else {
+      bad = 1;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    cost += __shfl_xor(cost, o, 64);
+    sq += __shfl_xor(sq, o, 64);
+    bad += __shfl_xor(bad, o, 64);
+  }
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    lds[0][wave] = cost;
+    lds[1][wave] = sq;
+    lds[2][wave] = bad;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const double* v = lds[threadIdx.x];
+    q.cost_part[(size_t)blockIdx.x * 3 + threadIdx.x] = (v[0] + v[1]) + (v[2] + v[3]);
+  }
+}
+
+// finishBundleMetrics (CANDIDATE = false) or acceptBundleCandidate (true).
+template <bool CANDIDATE>
+__global__ __launch_bounds__(CG_THREADS) void k_finish_cost(Problem q) {
+  __shared__ double lds[CG_THREADS / 64];
+  double* s = q.state;
+  double cost = 0, sq = 0, bad = 0;
+  for (uint32_t b = threadIdx.x; b < q.n_cost_blocks; b += CG_THREADS) {
+    cost += q.cost_part[(size_t)b * 3];
+    sq += q.cost_part[(size_t)b * 3 + 1];
+    bad += q.cost_part[(size_t)b * 3 + 2];
+  }
+  const double active = s[S_ACTIVE], valid = s[S_TRIAL_VALID], current = s[S_COST];
+  const double input_error = s[S_INPUT_ERROR];
+  cost = block_sum(cost, lds);
+  sq = block_sum(sq, lds);
+  bad = block_sum(bad, lds);
+  if (threadIdx.x != 0) return;
+  if (!CANDIDATE) {
+    if (input_error != 0) return;
+    if (bad > 0) {
+      s[S_INPUT_ERROR] = 2;
+      s[S_ACTIVE] = 0;
+      return;
+    }
+    s[S_COST] = cost;
+    s[S_SSE] = sq;
+    s[S_COST0] = cost;
+    s[S_SSE0] = sq;
+    if (cost < 1e-20) s[S_ACTIVE] = 0;
+    return;
+  }
+  s[S_COMMIT] = 0;
+  if (active == 0) return;
+  if (valid != 0 && cost < current) {
+    const double change = current - cost;
+    s[S_COST] = cost;
+    s[S_SSE] = sq;
+    s[S_ACCEPTED] += 1;
+    s[S_COMMIT] = 1;
+    s[S_LAMBDA] = fmax(1e-10, s[S_LAMBDA] / 3);
+    if (change < 1e-9 * fmax(1.0, cost)) s[S_ACTIVE] = 0;
+  } else {
+    s[S_REJECTED] += 1;
+    s[S_LAMBDA] *= 10;
+    if (s[S_PCG_FAILED] != 0) s[S_FAILED] += 1;
+    if (s[S_LAMBDA] > 1e12) s[S_ACTIVE] = 0;
+  }
+}
+
+// ------------------------------------------------------------------ assembly
+
+// One wave per tile of whole points, one lane per observation: linearise, write the cross blocks
+// (point-major), stage the point terms in LDS; then one lane per point sums its observations in
+// order, damps V, inverts it and writes the point block. The shared route adds the point's share of
+// the reduced focal diagonal and RHS (bundleFocalPoints), summed per tile in point order.
+template <class WT, bool SHARED>
+__global__ __launch_bounds__(TILE) void k_point_pass(Problem q) {
+  constexpr int NS = SHARED ? 13 : 10;  // staged words per observation: V 6, b 3, [fc 3], bad
+  __shared__ double st[TILE][NS];
+  __shared__ double fsum[TILE][2];
+  if (q.state[S_ACTIVE] == 0) return;
+  const uint32_t t = blockIdx.x, lane = threadIdx.x;
+  const uint32_t fp = q.tiles[t], ep = q.tiles[t + 1];
+  const uint32_t* ps = point_start(q);
+  const uint32_t s = ps[fp], e = ps[ep];
+  const uint32_t my = fp + lane;
+  const bool has = my < ep;
+  const uint32_t mys = has ? ps[my] : 0, mye = has ? ps[my + 1] : 0;
+  const double lambda = q.state[S_LAMBDA];
+  const double fscale = SHARED ? focal_scale(q.state[S_LOG_FOCAL]) : 1.0;
+  double V[6] = {0, 0, 0, 0, 0, 0}, b[3] = {0, 0, 0}, fc[3] = {0, 0, 0};
+  bool bad = false;
+  const uint32_t K = SHARED ? 21 : 18;
+  for (uint32_t base = s; base < e; base += TILE) {
+    const uint32_t i = base + lane;
+    if (i < e) {
+      const uint32_t c = obs_camera(q)[i], p = obs_point(q)[i];
+      Lin L;
+      const bool ok = linearize(q, c, p, i, fscale, variable_camera(q, c), L);
+      double W[21];
+      if (ok) {
+        cross_block(L, SHARED, W);
+      } else {
+        for (int k = 0; k < 21; k++) W[k] = 0;
+      }
+      for (uint32_t k = 0; k < K; k++) wrow<WT>(q.Wp, k, q.M)[i] = (WT)W[k];
+      double* o = st[lane];
+      if (ok) {
+        const double w = L.w;
+        const double a0 = w * L.Jp[0], a1 = w * L.Jp[1], a2 = w * L.Jp[2];
+        const double c0 = w * L.Jp[3], c1 = w * L.Jp[4], c2 = w * L.Jp[5];
+        o[0] = a0 * L.Jp[0] + c0 * L.Jp[3];  // V00
+        o[1] = a1 * L.Jp[0] + c1 * L.Jp[3];  // V10
+        o[2] = a2 * L.Jp[0] + c2 * L.Jp[3];  // V20
+        o[3] = a1 * L.Jp[1] + c1 * L.Jp[4];  // V11
+        o[4] = a2 * L.Jp[1] + c2 * L.Jp[4];  // V21
+        o[5] = a2 * L.Jp[2] + c2 * L.Jp[5];  // V22
+        o[6] = -(a0 * L.r[0]) - c0 * L.r[1];
+        o[7] = -(a1 * L.r[0]) - c1 * L.r[1];
+        o[8] = -(a2 * L.r[0]) - c2 * L.r[1];
+        if (SHARED) {
+          o[9] = W[18];
+          o[10] = W[19];
+          o[11] = W[20];
+        }
+        o[NS - 1] = 0;
+      } else {
+        for (int k = 0; k < NS - 1; k++) o[k] = 0;
+        o[NS - 1] = 1;
+      }
+    }
+    __syncthreads();
+    if (has) {
+      const uint32_t lo = mys > base ? mys : base, hi = mye < base + TILE ? mye : base + TILE;
+      for (uint32_t j = lo; j < hi; j++) {
+        const double* o = st[j - base];
+        for (int k = 0; k < 6; k++) V[k] += o[k];
+        for (int k = 0; k < 3; k++) b[k] += o[6 + k];
+        if (SHARED)
+          for (int k = 0; k < 3; k++) fc[k] += o[9 + k];
+        bad = bad || o[NS - 1] != 0;
+      }
+    }
+    __syncthreads();
+  }
+  bool failed = false;
+  double dF = 0, dR = 0;
+  if (has) {
+    double inv[9];
+    failed = bad;
+    if (!failed) {
+      double A[9] = {V[0], V[1], V[2], V[1], V[3], V[4], V[2], V[4], V[5]};
+      A[0] += lambda * fmax(A[0], 1e-10);
+      A[4] += lambda * fmax(A[4], 1e-10);
+      A[8] += lambda * fmax(A[8], 1e-10);
+      failed = !inverse3(A, inv);
+    }
+    double* out = q.pblk + (size_t)my * 12;
+    if (failed) {
+      for (int k = 0; k < 12; k++) out[k] = 0;
+    } else {
+      for (int k = 0; k < 9; k++) out[k] = inv[k];
+      for (int k = 0; k < 3; k++) out[9 + k] = b[k];
+      if (SHARED) {
+        for (int j = 0; j < 3; j++) {
+          double tj = 0;
+          for (int k = 0; k < 3; k++) tj += fc[k] * inv[k * 3 + j];
+          dF += tj * fc[j];
+          dR += tj * b[j];
+        }
+      }
+    }
+  }
+  const unsigned long long any = __ballot(failed);
+  if (SHARED) {
+    fsum[lane][0] = dF;
+    fsum[lane][1] = dR;
+    __syncthreads();
+  }
+  if (lane == 0) {
+    q.tile_fail[t] = any != 0ull;
+    if (SHARED) {
+      double f0 = 0, f1 = 0;
+      for (uint32_t j = 0; j < ep - fp && j < (uint32_t)TILE; j++) {
+        f0 += fsum[j][0];
+        f1 += fsum[j][1];
+      }
+      q.tile_focal[2 * (size_t)t] = f0;
+      q.tile_focal[2 * (size_t)t + 1] = f1;
+    }
+  }
+}
+
+// Packed upper triangle of a symmetric 6x6 block.
+__host__ __device__ constexpr int up6(int j, int l) { return j * 6 - j * (j - 1) / 2 + (l - j); }
+
+// One workgroup per camera-list task: relinearise each observation of the task in camera-major
+// order, write its cross block to the camera-major copy, and sum
+//   U = sum w Jc^T Jc (21), Q = sum W V^-1 W^T (21), rhs = -sum w Jc^T r - W V^-1 b_p (6)
+//   [shared] H_cf = sum w Jc^T Jf (6), H_ff = sum w Jf^T Jf, b_f = -sum w Jf^T r
+// into ASM_WORDS per task, in a fixed order.
+template <class WT, bool SHARED>
+__global__ __launch_bounds__(TASK_THREADS) void k_camera_pass(Problem q) {
+  constexpr int NV = SHARED ? 56 : 48;
+  __shared__ double red[TASK_THREADS / 64][NV];
+  if (q.state[S_ACTIVE] == 0) return;
+  const uint32_t t = blockIdx.x;
+  const uint32_t c = q.tasks[3 * t], s = q.tasks[3 * t + 1], e = q.tasks[3 * t + 2];
+  const bool var = variable_camera(q, c);
+  const double fscale = SHARED ? focal_scale(q.state[S_LOG_FOCAL]) : 1.0;
+  const uint32_t K = SHARED ? 21 : 18;
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) acc[k] = 0;
+  for (uint32_t n = s + threadIdx.x; n < e; n += TASK_THREADS) {
+    const uint32_t i = camera_obs(q)[n], p = q.cam_point[n];
+    Lin L;
+    const bool ok = linearize(q, c, p, i, fscale, var, L);
+    double W[21];
+    if (ok) {
+      cross_block(L, SHARED, W);
+    } else {
+      for (int k = 0; k < 21; k++) W[k] = 0;
+    }
+    for (uint32_t k = 0; k < K; k++) wrow<WT>(q.Wc, k, q.M)[n] = (WT)W[k];
+    if (!ok) continue;
+    const double w = L.w;
+    if (var) {
+      const double* pb = q.pblk + (size_t)p * 12;
+      double Vi[9], bp[3];
+      for (int k = 0; k < 9; k++) Vi[k] = pb[k];
+      for (int k = 0; k < 3; k++) bp[k] = pb[9 + k];
+      double T[18];
+#pragma unroll
+      for (int j = 0; j < 6; j++)
+#pragma unroll
+        for (int l = 0; l < 3; l++)
+          T[j * 3 + l] = W[j * 3] * Vi[l] + W[j * 3 + 1] * Vi[3 + l] + W[j * 3 + 2] * Vi[6 + l];
+#pragma unroll
+      for (int j = 0; j < 6; j++) {
+        const double wj0 = w * L.Jc[j], wj1 = w * L.Jc[6 + j];
+#pragma unroll
+        for (int l = j; l < 6; l++) {
+          acc[up6(j, l)] += wj0 * L.Jc[l] + wj1 * L.Jc[6 + l];
+          acc[21 + up6(j, l)] +=
+              T[j * 3] * W[l * 3] + T[j * 3 + 1] * W[l * 3 + 1] + T[j * 3 + 2] * W[l * 3 + 2];
+        }
+        acc[42 + j] += -(wj0 * L.r[0]) - wj1 * L.r[1] -
+                       (T[j * 3] * bp[0] + T[j * 3 + 1] * bp[1] + T[j * 3 + 2] * bp[2]);
+        if (SHARED) acc[48 + j] += wj0 * L.Jf[0] + wj1 * L.Jf[1];
+      }
+    }
+    if (SHARED) {
+      acc[54] += w * (L.Jf[0] * L.Jf[0] + L.Jf[1] * L.Jf[1]);
+      acc[55] += -(w * (L.Jf[0] * L.r[0] + L.Jf[1] * L.r[1]));
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NV; k++)
+    for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_xor(acc[k], o, 64);
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+    for (int k = 0; k < NV; k++) red[wave][k] = acc[k];
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double v = 0;
+    for (int w = 0; w < TASK_THREADS / 64; w++) v += red[w][threadIdx.x];
+    q.task_part[(size_t)t * ASM_WORDS + threadIdx.x] = v;
+  }
+}
+
+// One thread per camera: sum the camera's tasks in order, damp, form the Schur diagonal block
+// S_cc = A - Q and its inverse (block-Jacobi preconditioner, failure marker -1), as
+// bundleCameraBlocks b:423-489 does for the whole camera.
+template <bool SHARED>
+__global__ void k_camera_finish(Problem q) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= q.C || q.state[S_ACTIVE] == 0) return;
+  constexpr int NV = SHARED ? 56 : 48;
+  double acc[NV];
+  for (int k = 0; k < NV; k++) acc[k] = 0;
+  for (uint32_t t = q.task_start[c]; t < q.task_start[c + 1]; t++)
+    for (int k = 0; k < NV; k++) acc[k] += q.task_part[(size_t)t * ASM_WORDS + k];
+  double* A = q.camA + (size_t)c * 36;
+  double* inv = q.camInv + (size_t)c * 36;
+  double* rhs = q.camRhs + (size_t)c * 6;
+  if (SHARED) {
+    q.camFocal[2 * (size_t)c] = acc[54];
+    q.camFocal[2 * (size_t)c + 1] = acc[55];
+  }
+  if (!variable_camera(q, c)) {
+    for (int j = 0; j < 36; j++) A[j] = inv[j] = (j % 7 == 0) ? 1.0 : 0.0;
+    for (int j = 0; j < 6; j++) {
+      rhs[j] = 0;
+      if (SHARED) q.camHcf[(size_t)c * 6 + j] = 0;
+    }
+    return;
+  }
+  const double lambda = q.state[S_LAMBDA];
+  double a[36], S[36];
+  for (int j = 0; j < 6; j++)
+    for (int l = j; l < 6; l++) {
+      a[j * 6 + l] = a[l * 6 + j] = acc[up6(j, l)];
+      S[j * 6 + l] = S[l * 6 + j] = acc[21 + up6(j, l)];
+    }
+  for (int j = 0; j < 6; j++) a[j * 6 + j] += lambda * fmax(a[j * 6 + j], 1e-10);
+  for (int j = 0; j < 36; j++) {
+    A[j] = a[j];
+    S[j] = a[j] - S[j];
+  }
+  for (int j = 0; j < 6; j++) {
+    rhs[j] = acc[42 + j];
+    if (SHARED) q.camHcf[(size_t)c * 6 + j] = acc[48 + j];
+  }
+  Chol6 f;
+  if (!cholesky6(S, f)) {
+    for (int j = 0; j < 36; j++) inv[j] = 0;
+    inv[0] = -1;
+    return;
+  }
+  double M[36];
+  for (int column = 0; column < 6; column++) {
+    double e6[6] = {0, 0, 0, 0, 0, 0}, x[6];
+    e6[column] = 1;
+    chol6_solve(f, e6, x);
+    for (int j = 0; j < 6; j++) M[j * 6 + column] = x[j];
+  }
+  for (int j = 0; j < 6; j++)
+    for (int k = j + 1; k < 6; k++) {
+      const double v = (M[j * 6 + k] + M[k * 6 + j]) * 0.5;
+      M[j * 6 + k] = v;
+      M[k * 6 + j] = v;
+    }
+  for (int j = 0; j < 36; j++) inv[j] = M[j];
+}
+
+// ------------------------------------------------------------------ PCG
+
+__device__ __forceinline__ double* vx(const Problem& q) { return q.vec; }
+__device__ __forceinline__ double* vr(const Problem& q) { return q.vec + q.m; }
+__device__ __forceinline__ double* vp(const Problem& q) { return q.vec + 2 * (size_t)q.m; }
+__device__ __forceinline__ double* vs(const Problem& q) { return q.vec + 3 * (size_t)q.m; }
+__device__ __forceinline__ double* vz(const Problem& q) { return q.vec + 4 * (size_t)q.m; }
+__device__ __forceinline__ double* vy(const Problem& q) { return q.vec + 5 * (size_t)q.m; }
+__device__ __forceinline__ double* vfpart(const Problem& q) {
+  return q.vec + 5 * (size_t)q.m + 3 * (size_t)q.P;
+}
+
+// z_c = M_c^-1 r_c for one camera.
+__device__ __forceinline__ void precondition(const Problem& q, uint32_t c, const double* r,
+                                             double* z) {
+  const double* inv = q.camInv + (size_t)c * 36;
+  double rc[6];
+  for (int k = 0; k < 6; k++) rc[k] = r[c * 6 + k];
+  for (int j = 0; j < 6; j++) {
+    double v = 0;
+    for (int k = 0; k < 6; k++) v += inv[j * 6 + k] * rc[k];
+    z[c * 6 + j] = v;
+  }
+}
+
+// initializeBundleCg (with dampBundleFocal and the rest of bundleFocalPoints on the shared route):
+// point and camera block checks, x = 0, r = b, z = M^-1 r, p = z, rho, |r0|^2.
+template <bool SHARED>
+__global__ __launch_bounds__(CG_THREADS) void k_init_cg(Problem q) {
+  __shared__ double lds[CG_THREADS / 64];
+  double* s = q.state;
+  const uint32_t tid = threadIdx.x;
+  const double active = s[S_ACTIVE], lambda = s[S_LAMBDA];
+  __syncthreads();
+  if (tid == 0)
+    for (int j = S_RHO; j < 26; j++) s[j] = 0;
+  if (active == 0) return;
+  double fail = 0;
+  for (uint32_t t = tid; t < q.n_tiles; t += CG_THREADS) fail += q.tile_fail[t] ? 1.0 : 0.0;
+  for (uint32_t c = tid; c < q.C; c += CG_THREADS)
+    if (!(q.camInv[(size_t)c * 36] > 0)) fail += 1;
+  double hff = 0, bf = 0, tf = 0, tr = 0;
+  if (SHARED) {
+    for (uint32_t c = tid; c < q.C; c += CG_THREADS) {
+      hff += q.camFocal[2 * (size_t)c];
+      bf += q.camFocal[2 * (size_t)c + 1];
+    }
+    for (uint32_t t = tid; t < q.n_tiles; t += CG_THREADS) {
+      tf += q.tile_focal[2 * (size_t)t];
+      tr += q.tile_focal[2 * (size_t)t + 1];
+    }
+    hff = block_sum(hff, lds);
+    bf = block_sum(bf, lds);
+    tf = block_sum(tf, lds);
+    tr = block_sum(tr, lds);
+  }
+  double finv = 0, frhs = 0;
+  if (SHARED) {
+    const double damped = hff + lambda * fmax(hff, 1e-10);
+    const double diag = damped - tf;
+    frhs = bf - tr;
+    finv = diag > 0 ? 1 / diag : -1.0;
+    if (tid == 0) {
+      s[S_FOCAL_DIAG] = damped;
+      s[S_FOCAL_INV] = finv;
+      s[S_FOCAL_RHS] = frhs;
+    }
+    if (!(finv > 0)) fail += 1;
+  }
+  fail = block_sum(fail, lds);
+  if (fail > 0) {
+    if (tid == 0) s[S_PCG_FAILED] = 1;
+    return;
+  }
+  double rho = 0, res = 0;
+  double* x = vx(q);
+  double* r = vr(q);
+  double* p = vp(q);
+  double* z = vz(q);
+  for (uint32_t c = tid; c < q.C; c += CG_THREADS) {
+    for (int j = 0; j < 6; j++) {
+      x[c * 6 + j] = 0;
+      r[c * 6 + j] = q.camRhs[(size_t)c * 6 + j];
+    }
+    precondition(q, c, r, z);
+    for (int j = 0; j < 6; j++) {
+      p[c * 6 + j] = z[c * 6 + j];
+      rho += r[c * 6 + j] * z[c * 6 + j];
+      res += r[c * 6 + j] * r[c * 6 + j];
+    }
+  }
+  if (SHARED && tid == 0) {
+    const uint32_t f = q.m - 1;
+    const double zf = finv * frhs;
+    x[f] = 0;
+    r[f] = frhs;
+    z[f] = zf;
+    p[f] = zf;
+    rho += frhs * zf;
+    res += frhs * frhs;
+  }
+  rho = block_sum(rho, lds);
+  res = block_sum(res, lds);
+  if (tid == 0) {
+    s[S_RHO] = rho;
+    s[S_R0] = res;
+    s[S_R] = res;
+    if (res > 0 && rho > 0)
+      s[S_PCG_ACTIVE] = 1;
+    else if (res > 1e-24)
+      s[S_PCG_FAILED] = 1;
+  }
+}
+
+// The point sweep over tiles of whole points, one lane per observation.
+//   SCHUR:  y_p = V_p^-1 sum_i (W_i^T v_c(i) + w_fi v_f)                (bundleSchurPointAt)
+//   UPDATE: X_p' = X_p + V_p^-1 (b_p - sum_i (W_i^T x_c(i) + w_fi x_f))  (updateBundleCandidate)
+enum SweepMode { SWEEP_SCHUR = 0, SWEEP_UPDATE = 1 };
+
+template <class WT, bool SHARED, int MODE>
+__global__ __launch_bounds__(TILE) void k_point_sweep(Problem q) {
+  __shared__ double st[TILE][3];
+  if (MODE == SWEEP_SCHUR && q.state[S_PCG_ACTIVE] == 0) return;
+  if (MODE == SWEEP_UPDATE && (q.state[S_ACTIVE] == 0 || q.state[S_PCG_FAILED] != 0)) return;
+  const double* v = MODE == SWEEP_SCHUR ? vp(q) : vx(q);
+  const uint32_t t = blockIdx.x, lane = threadIdx.x;
+  const uint32_t fp = q.tiles[t], ep = q.tiles[t + 1];
+  const uint32_t* ps = point_start(q);
+  const uint32_t s = ps[fp], e = ps[ep];
+  const uint32_t my = fp + lane;
+  const bool has = my < ep;
+  const uint32_t mys = has ? ps[my] : 0, mye = has ? ps[my + 1] : 0;
+  const double vf = SHARED ? v[q.m - 1] : 0.0;
+  double acc[3] = {0, 0, 0};
+  for (uint32_t base = s; base < e; base += TILE) {
+    const uint32_t i = base + lane;
+    if (i < e) {
+      const uint32_t c = obs_camera(q)[i];
+      double vc[6];
+      for (int k = 0; k < 6; k++) vc[k] = v[c * 6 + k];
+      double tj[3];
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        double a = 0;
+#pragma unroll
+        for (int k = 0; k < 6; k++) a += (double)wrow<WT>(q.Wp, k * 3 + j, q.M)[i] * vc[k];
+        if (SHARED) a += (double)wrow<WT>(q.Wp, 18 + j, q.M)[i] * vf;
+        tj[j] = a;
+      }
+      st[lane][0] = tj[0];
+      st[lane][1] = tj[1];
+      st[lane][2] = tj[2];
+    }
+    __syncthreads();
+    if (has) {
+      const uint32_t lo = mys > base ? mys : base, hi = mye < base + TILE ? mye : base + TILE;
+      for (uint32_t j = lo; j < hi; j++) {
+        acc[0] += st[j - base][0];
+        acc[1] += st[j - base][1];
+        acc[2] += st[j - base][2];
+      }
+    }
+    __syncthreads();
+  }
+  if (!has) return;
+  const double* pb = q.pblk + (size_t)my * 12;
+  if (MODE == SWEEP_SCHUR) {
+    double* y = vy(q) + (size_t)my * 3;
+    for (int j = 0; j < 3; j++) y[j] = pb[j * 3] * acc[0] + pb[j * 3 + 1] * acc[1] + pb[j * 3 + 2] * acc[2];
+  } else {
+    const double r0 = pb[9] - acc[0], r1 = pb[10] - acc[1], r2 = pb[11] - acc[2];
+    const size_t at = (size_t)q.C * 12 + (size_t)my * 3;
+    for (int j = 0; j < 3; j++)
+      q.trial[at + j] = q.scene[at + j] + (pb[j * 3] * r0 + pb[j * 3 + 1] * r1 + pb[j * 3 + 2] * r2);
+  }
+}
+
+// The camera sweep: one workgroup per camera-list task streams the camera-major cross blocks and
+// gathers y of each slot's point: part[t] = sum_n W_n y_p(n) (6) and sum_n w_fn . y_p(n) (focal).
+template <class WT, bool SHARED>
+__global__ __launch_bounds__(TASK_THREADS) void k_camera_sweep(Problem q) {
+  constexpr int NV = SHARED ? 7 : 6;
+  __shared__ double red[TASK_THREADS / 64][NV];
+  if (q.state[S_PCG_ACTIVE] == 0) return;
+  const uint32_t t = blockIdx.x;
+  const uint32_t s = q.tasks[3 * t + 1], e = q.tasks[3 * t + 2];
+  const double* y = vy(q);
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) acc[k] = 0;
+  for (uint32_t n = s + threadIdx.x; n < e; n += TASK_THREADS) {
+    const uint32_t p = q.cam_point[n];
+    const double y0 = y[(size_t)p * 3], y1 = y[(size_t)p * 3 + 1], y2 = y[(size_t)p * 3 + 2];
+#pragma unroll
+    for (int j = 0; j < NV; j++)
+      acc[j] += (double)wrow<WT>(q.Wc, j * 3, q.M)[n] * y0 +
+                (double)wrow<WT>(q.Wc, j * 3 + 1, q.M)[n] * y1 +
+                (double)wrow<WT>(q.Wc, j * 3 + 2, q.M)[n] * y2;
+  }
+#pragma unroll
+  for (int k = 0; k < NV; k++)
+    for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_xor(acc[k], o, 64);
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+    for (int k = 0; k < NV; k++) red[wave][k] = acc[k];
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double v = 0;
+    for (int w = 0; w < TASK_THREADS / 64; w++) v += red[w][threadIdx.x];
+    q.task_part[(size_t)t * 8 + threadIdx.x] = v;
+  }
+}
+
+// Finish S·v from the camera partials: (S v)_c = A_c v_c [+ H_cf v_f] - sum_tasks part_c, and the
+// focal row H_ff v_f + sum_c (H_cf . v_c - part_c[6]). Returns this thread's share of v . S v.
+// Uniform control flow (uses block_sum).
+template <bool SHARED>
+__device__ double schur_finish(const Problem& q, const double* v, double* sv, double* lds) {
+  const uint32_t tid = threadIdx.x;
+  const double vf = SHARED ? v[q.m - 1] : 0.0;
+  double dot = 0, fsum = 0;
+  for (uint32_t c = tid; c < q.C; c += CG_THREADS) {
+    double sums[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t t = q.task_start[c]; t < q.task_start[c + 1]; t++)
+      for (int j = 0; j < (SHARED ? 7 : 6); j++) sums[j] += q.task_part[(size_t)t * 8 + j];
+    const double* A = q.camA + (size_t)c * 36;
+    double vc[6];
+    for (int k = 0; k < 6; k++) vc[k] = v[c * 6 + k];
+    for (int j = 0; j < 6; j++) {
+      double a = SHARED ? q.camHcf[(size_t)c * 6 + j] * vf : 0.0;
+      for (int k = 0; k < 6; k++) a += A[j * 6 + k] * vc[k];
+      a -= sums[j];
+      sv[c * 6 + j] = a;
+      dot += vc[j] * a;
+    }
+    if (SHARED) {
+      double f = 0;
+      for (int k = 0; k < 6; k++) f += q.camHcf[(size_t)c * 6 + k] * vc[k];
+      fsum += f - sums[6];
+    }
+  }
+  if (SHARED) {
+    fsum = block_sum(fsum, lds);
+    const double sf = q.state[S_FOCAL_DIAG] * vf + fsum;
+    if (tid == 0) {
+      sv[q.m - 1] = sf;
+      dot += vf * sf;
+    }
+  }
+  return dot;
+}
+
+// advanceBundleCg b:883-929 / advancePcg(false) j:874-924, after finishing S p.
+template <bool SHARED>
+__global__ __launch_bounds__(CG_THREADS) void k_cg_advance(Problem q) {
+  __shared__ double lds[CG_THREADS / 64];
+  double* s = q.state;
+  if (s[S_PCG_ACTIVE] == 0) return;
+  const uint32_t tid = threadIdx.x;
+  const double rho0 = s[S_RHO], r0 = s[S_R0];
+  double* x = vx(q);
+  double* r = vr(q);
+  double* p = vp(q);
+  double* S = vs(q);
+  double* z = vz(q);
+  double den = block_sum(schur_finish<SHARED>(q, p, S, lds), lds);
+  if (!(den > 0)) {
+    if (tid == 0) {
+      s[S_PCG_ACTIVE] = 0;
+      s[S_PCG_FAILED] = 1;
+    }
+    return;
+  }
+  const double alpha = rho0 / den;
+  if (!(fabs(alpha) < 1e20)) {
+    if (tid == 0) {
+      s[S_PCG_ACTIVE] = 0;
+      s[S_PCG_FAILED] = 1;
+    }
+    return;
+  }
+  double res = 0, rho = 0;
+  for (uint32_t c = tid; c < q.C; c += CG_THREADS) {
+    for (int j = 0; j < 6; j++) {
+      const uint32_t i = c * 6 + j;
+      x[i] += alpha * p[i];
+      r[i] -= alpha * S[i];
+      res += r[i] * r[i];
+    }
+    precondition(q, c, r, z);
+    for (int j = 0; j < 6; j++) rho += z[c * 6 + j] * r[c * 6 + j];
+  }
+  if (SHARED && tid == 0) {
+    const uint32_t f = q.m - 1;
+    x[f] += alpha * p[f];
+    r[f] -= alpha * S[f];
+    res += r[f] * r[f];
+    z[f] = s[S_FOCAL_INV] * r[f];
+    rho += z[f] * r[f];
+  }
+  res = block_sum(res, lds);
+  rho = block_sum(rho, lds);
+  if (tid == 0) {
+    s[S_CG_STEPS] += 1;
+    s[S_STEPS] += 1;
+    s[S_R] = res;
+  }
+  if (res <= q.tol * q.tol * r0) {
+    if (tid == 0) {
+      s[S_PCG_ACTIVE] = 0;
+      s[S_CONVERGED] = 1;
+    }
+    return;
+  }
+  if (!(rho > 0)) {
+    if (tid == 0) {
+      s[S_PCG_ACTIVE] = 0;
+      s[S_PCG_FAILED] = 1;
+    }
+    return;
+  }
+  const double beta = rho / rho0;
+  for (uint32_t c = tid; c < q.C; c += CG_THREADS)
+    for (int j = 0; j < 6; j++) p[c * 6 + j] = z[c * 6 + j] + beta * p[c * 6 + j];
+  if (SHARED && tid == 0) p[q.m - 1] = z[q.m - 1] + beta * p[q.m - 1];
+  if (tid == 0) s[S_RHO] = rho;
+}
+
+// The linear-system probe: S v, M^-1 v and the RHS of the assembled system.
+template <bool SHARED>
+__global__ __launch_bounds__(CG_THREADS) void k_probe(Problem q, const double* v, double* sv,
+                                                      double* zv, double* rhs) {
+  __shared__ double lds[CG_THREADS / 64];
+  (void)schur_finish<SHARED>(q, v, sv, lds);
+  for (uint32_t c = threadIdx.x; c < q.C; c += CG_THREADS) {
+    precondition(q, c, v, zv);
+    for (int j = 0; j < 6; j++) rhs[c * 6 + j] = q.camRhs[(size_t)c * 6 + j];
+  }
+  if (SHARED && threadIdx.x == 0) {
+    zv[q.m - 1] = q.state[S_FOCAL_INV] * v[q.m - 1];
+    rhs[q.m - 1] = q.state[S_FOCAL_RHS];
+  }
+}
+
+// ------------------------------------------------------------------ candidate
+
+// Cameras of the candidate: movePose for variable cameras, the trial log focal on the shared route.
+__global__ void k_update_cameras(Problem q) {
+  const double* s = q.state;
+  if (s[S_ACTIVE] == 0 || s[S_PCG_FAILED] != 0) return;
+  const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+  const double* x = vx(q);
+  if (h == 0 && q.shared) q.state[S_TRIAL_FOCAL] = s[S_LOG_FOCAL] + x[q.m - 1];
+  if (h >= q.C) return;
+  double pose[12], moved[12];
+  for (int j = 0; j < 12; j++) pose[j] = q.scene[(size_t)h * 12 + j];
+  if (variable_camera(q, h)) {
+    double d[6];
+    for (int j = 0; j < 6; j++) d[j] = x[h * 6 + j];
+    move_pose(pose, d, moved);
+  } else {
+    for (int j = 0; j < 12; j++) moved[j] = pose[j];
+  }
+  for (int j = 0; j < 12; j++) q.trial[(size_t)h * 12 + j] = moved[j];
+}
+
+// bundleCandidateGauge: reject an unconverged solve, check the focal proposal, the baseline, the
+// scale and every trial pose; on success set the gauge scale and the trial-valid word.
+__global__ __launch_bounds__(256) void k_gauge(Problem q) {
+  __shared__ double lds[4];
+  double* s = q.state;
+  const uint32_t tid = threadIdx.x;
+  const double active = s[S_ACTIVE], failed = s[S_PCG_FAILED], r0 = s[S_R0], r = s[S_R];
+  const double trial_focal = s[S_TRIAL_FOCAL];
+  __syncthreads();
+  if (tid == 0) s[S_TRIAL_VALID] = 0;
+  if (active == 0 || failed != 0) return;
+  if (r0 > 0 && r > q.tol * q.tol * r0) {
+    if (tid == 0) s[S_PCG_FAILED] = 1;
+    return;
+  }
+  double bad = 0;
+  if (q.shared) {
+    if (!(fabs(trial_focal) < 700)) return;
+    const double fs = focal_scale(trial_focal);
+    for (uint32_t c = tid; c < q.C; c += 256) {
+      const double* m = q.scene + (size_t)q.C * 12 + (size_t)q.P * 3 + (size_t)c * 6;
+      if (!(m[0] * fs >= 1e-6 && m[1] * fs >= 1e-6 && m[0] * fs <= 1e6 && m[1] * fs <= 1e6)) bad += 1;
+    }
+  }
+  for (uint32_t c = tid; c < q.C; c += 256) {
+    const double* p = q.trial + (size_t)c * 12;
+    for (int j = 0; j < 12; j++)
+      if (!(fabs(p[j]) <= 1e9)) bad += 1;
+    for (int j = 0; j < 3; j++)
+      for (int k = 0; k < 3; k++) {
+        double d = 0;
+        for (int l = 0; l < 3; l++) d += p[j * 3 + l] * p[k * 3 + l];
+        if (j == k) d -= 1;
+        if (!(fabs(d) < 1e-6)) bad += 1;
+      }
+  }
+  bad = block_sum(bad, lds);
+  if (tid != 0) return;
+  double c[3];
+  center_of(q.trial + (size_t)q.scale_view * 12, c);
+  const double d0 = s[S_ORIGIN] - c[0], d1 = s[S_ORIGIN + 1] - c[1], d2 = s[S_ORIGIN + 2] - c[2];
+  const double baseline = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+  if (!(baseline > 1e-12)) return;
+  double scale = 1;
+  if (q.normalize_scale) scale = s[S_BASELINE] / baseline;
+  if (!(scale > 1e-9 && scale < 1e9)) return;
+  if (bad > 0) return;
+  s[S_GAUGE] = scale;
+  s[S_TRIAL_VALID] = 1;
+}
+
+// normalizeBundleCandidate: unit directional points; the similarity X' = o + s (X - o),
+// t' = s t + (s - 1) R o about the gauge origin when one camera is fixed.
+__global__ void k_normalize(Problem q) {
+  const double* s = q.state;
+  if (s[S_ACTIVE] == 0 || s[S_TRIAL_VALID] == 0) return;
+  const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+  const double g = s[S_GAUGE];
+  const double o[3] = {s[S_ORIGIN], s[S_ORIGIN + 1], s[S_ORIGIN + 2]};
+  if (h < q.P && directional_point(q, h)) {
+    double* d = q.trial + (size_t)q.C * 12 + (size_t)h * 3;
+    const double norm = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    if (norm > 1e-12) {
+      d[0] = d[0] / norm;
+      d[1] = d[1] / norm;
+      d[2] = d[2] / norm;
+    }
+  }
+  if (!q.normalize_scale) return;
+  if (h < q.P && !directional_point(q, h)) {
+    double* X = q.trial + (size_t)q.C * 12 + (size_t)h * 3;
+    for (int j = 0; j < 3; j++) X[j] = o[j] + g * (X[j] - o[j]);
+  }
+  if (h < q.C && h != q.anchor) {
+    double* p = q.trial + (size_t)h * 12;
+    double rot[3];
+    for (int r = 0; r < 3; r++)
+      rot[r] = (p[r * 3] * o[0] + p[r * 3 + 1] * o[1] + p[r * 3 + 2] * o[2] + p[9 + r]) - p[9 + r];
+    const double t0 = g * p[9] + (g - 1) * rot[0];
+    const double t1 = g * p[10] + (g - 1) * rot[1];
+    const double t2 = g * p[11] + (g - 1) * rot[2];
+    p[9] = t0;
+    p[10] = t1;
+    p[11] = t2;
+  }
+}
+
+// commitBundleCandidate: the trial becomes the scene when accepted.
+__global__ void k_commit(Problem q) {
+  if (q.state[S_COMMIT] == 0) return;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t total = (size_t)q.C * 12 + (size_t)q.P * 3;
+  if (i < total) q.scene[i] = q.trial[i];
+  if (i == 0 && q.shared) q.state[S_LOG_FOCAL] = q.state[S_TRIAL_FOCAL];
+}
+
+// packBundleResult: coordinates, state words, RMSEs (and the log focal scale).
+__global__ void k_pack(Problem q, double* out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t total = (size_t)q.C * 12 + (size_t)q.P * 3;
+  if (i < total) out[i] = q.scene[i];
+  if (i < 26) {
+    double v = q.state[i];
+    if (i == 24) v = sqrt(q.state[S_SSE0] / (double)(float)q.M);
+    if (i == 25) v = sqrt(q.state[S_SSE] / (double)(float)q.M);
+    out[total + i] = v;
+  }
+  if (i == 0 && q.shared) out[total + 26] = q.state[S_LOG_FOCAL];
+}
+
+}  // namespace tcvba
