@@ -1,0 +1,172 @@
+"""Benchmark of the MI355X bundle-adjustment engine on the shared-focal trajectory scene.
+
+Workload (one GPU): 500 cameras on a smooth closed trajectory, 200,000 points each seen by 5-20
+cameras (~1.9M observations), N(0, 0.7 px) noise, 2 % uniform outliers, common focal error
+U(0.95, 1.05); joint pose/point/shared-focal LM with Huber delta = 1 px, 30 LM iterations, PCG up
+to 256 iterations (the route's default), FP64. A step is one whole solve from the device-resident
+packed scene (tcv_ba_solve_device); the scene is generated and uploaded before timing.
+
+Prints one JSON line: LM iterations/s over the timed steps (`value`), observations/s, the Schur
+matvec roofline (algorithmic bytes of the two sweeps / their HIP-event time), and the CPU oracle
+timed on a bounded sample of the same solve on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = ("LM iters/sec + obs/sec, mixed vs FP64 at ≤1e-6 rel cost; Schur matvec HBM GB/s")
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def matvec_bytes(C: int, P: int, M: int, shared: bool, s_w: int = 8, s_v: int = 8) -> int:
+    """Algorithmic HBM bytes of one Schur product (both sweeps): every cross block read once per
+    sweep (w words), the camera id (point sweep) and point id (camera sweep) of each observation,
+    V^-1 and the point offsets, y written and read back, the camera blocks, offsets and vectors."""
+    w = 21 if shared else 18
+    total = M * (2 * w * s_w + 8) + P * (9 * s_v + 4 + 24 + 24) + C * (36 * s_w + 4 + 3 * 6 * 8)
+    if shared:
+        total += C * 6 * 8 * 2 + 24
+    return total
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cameras", type=int, default=500)
+    ap.add_argument("--points", type=int, default=200_000)
+    ap.add_argument("--iterations", type=int, default=30)
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
+    ap.add_argument("--cpu-lm", type=int, default=2, help="LM iterations of the CPU sample (0: skip)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(the matvec would shard by points with one all-reduce of 6C+1 doubles per PCG "
+              "iteration); run it with --gpus 1", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    from typecv_amd import _native
+    from typecv_amd import bundle as B
+    from typecv_amd import synthetic as S
+
+    t0 = time.time()
+    packed, info = S.trajectory_scene(cameras=args.cameras, points=args.points, seed=2)
+    C, P, M = packed.C, packed.P, packed.M
+    log(f"scene C={C} P={P} M={M} generated in {time.time() - t0:.1f}s")
+    control = B.make_control(C, P, M, shared_focal=True, huber=1.0, iterations=args.iterations,
+                             max_cg=min(256, max(32, 12 * C + 1)), precision=args.precision,
+                             timing=True)
+    lib = _native.library()
+    device = B.HipDevice(0)
+    scene = torch.from_numpy(packed.scene).cuda()
+    index = torch.from_numpy(packed.index.view(np.int32)).cuda()
+    out = torch.empty(lib.tcv_ba_output_size(ctypes.byref(control)), dtype=torch.float64,
+                      device="cuda")
+    torch.cuda.synchronize()
+
+    def solve() -> dict:
+        report = _native.Report()
+        status = lib.tcv_ba_solve_device(device.handle, ctypes.byref(control), scene.data_ptr(),
+                                         index.data_ptr(), out.data_ptr(), ctypes.byref(report),
+                                         None)
+        _native.check(status, "tcv_ba_solve_device")
+        return report.as_dict()
+
+    for k in range(args.warmup):
+        r = solve()
+        log(f"warmup {k}: {r['solve_ms']:.1f} ms, {r['lm_iterations']} LM, {r['cg_iterations']} CG")
+    torch.cuda.synchronize()
+    reports = []
+    start = time.perf_counter()
+    for k in range(args.steps):
+        reports.append(solve())
+        log(f"step {k}: {reports[-1]['solve_ms']:.1f} ms")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - start
+    values = out.cpu().numpy()
+    base = C * 12 + P * 3
+    lm = sum(r["lm_iterations"] for r in reports)
+    matvecs = sum(r["matvecs"] for r in reports)
+    schur_ms = sum(r["schur_ms"] for r in reports)
+    point_ms = sum(r["schur_point_ms"] for r in reports)
+    camera_ms = sum(r["schur_camera_ms"] for r in reports)
+    s_w = 4 if args.precision == "mixed" else 8
+    bytes_per = matvec_bytes(C, P, M, True, s_w=s_w)
+    mv_ms = schur_ms / max(1, matvecs)
+    achieved = bytes_per / (mv_ms * 1e-3) / 1e9 if mv_ms > 0 else 0.0
+    result = {
+        "metric": METRIC,
+        "value": lm / elapsed,
+        "unit": "LM iterations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32-products/f64-accumulation",
+        "data": "synthetic (seeded trajectory scene, generated on the host, resident in HBM)",
+        "config": {
+            "workload": "shared-focal LM bundle adjustment, trajectory scene",
+            "cameras": C, "points": P, "observations": M, "lm_iterations_max": args.iterations,
+            "max_cg": control.max_cg, "huber_px": 1.0, "precision": args.precision,
+            "outliers": info["outliers"], "focal_factor": info["focal_factor"],
+        },
+        "observations_per_s": M * lm / elapsed,
+        "lm_iterations_per_step": lm / args.steps,
+        "cg_iterations_per_step": sum(r["cg_iterations"] for r in reports) / args.steps,
+        "matvecs_per_step": matvecs / args.steps,
+        "final_cost": float(values[base + 6]),
+        "initial_cost": float(values[base + 8]),
+        "final_rmse_px": float(values[base + 25]),
+        "schur_matvec": {
+            "ms": mv_ms, "point_sweep_ms": point_ms / max(1, matvecs),
+            "camera_sweep_ms": camera_ms / max(1, matvecs), "bytes": bytes_per,
+            "share_of_solve": schur_ms / max(1e-9, sum(r["solve_ms"] for r in reports)),
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "Schur matvec: k_point_sweep + k_camera_sweep",
+        },
+    }
+    device.dispose()
+    if args.cpu_lm > 0:
+        from oracle import oracle as O
+
+        threads = O.set_threads(0)
+        cpu_control = B.make_control(C, P, M, shared_focal=True, huber=1.0,
+                                     iterations=args.iterations, max_cg=control.max_cg)
+        log(f"CPU oracle sample: {args.cpu_lm} LM iterations on {threads} threads")
+        _, st = O.solve(cpu_control, packed, max_lm=args.cpu_lm)
+        result["cpu_baseline"] = {
+            "value": st["lm_iterations"] / (st["lm_ms"] * 1e-3),
+            "unit": "LM iterations/s", "cores": threads, "kind": "port",
+            "sample": f"first {int(st['lm_iterations'])} LM iterations of the same solve "
+                      f"({int(st['cg_iterations'])} PCG iterations, {st['lm_ms'] / 1e3:.1f} s), "
+                      "C oracle -O3 OpenMP",
+        }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

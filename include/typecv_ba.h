@@ -88,13 +88,14 @@ typedef struct tcv_ba_control {
 typedef struct tcv_ba_report {
   double solve_ms;          /* device-resident solve: setup + LM loop, inputs already in HBM */
   double setup_ms;          /* index checks, tile tables, camera-major permutation */
-  double schur_ms;          /* HIP-event time of all Schur sweeps (TCV_BA_FLAG_TIMING) */
+  double schur_ms;          /* HIP-event time of the active Schur sweeps (TCV_BA_FLAG_TIMING) */
   double schur_point_ms;    /* of which the point sweep */
   double schur_camera_ms;   /* of which the camera sweep */
   double refinement_steps;  /* outer refinement corrections taken (mixed form) */
   uint64_t lm_iterations;   /* LM iterations executed (accepted + rejected) */
   uint64_t cg_iterations;   /* PCG iterations that completed */
-  uint64_t matvecs;         /* Schur products launched */
+  uint64_t matvecs;         /* Schur products computed while PCG was active */
+  uint64_t matvec_launches; /* Schur products launched (chunks end past convergence) */
   uint64_t flag_reads;      /* host reads of the activity words */
   uint64_t device_bytes;    /* device memory held by the context after the solve */
   uint64_t point_tiles;     /* wave tiles of whole points */

@@ -1125,6 +1125,35 @@ int tcv_oracle_schur_probe(const tcv_ba_control* ctl, const double* scene, const
   return TCV_BA_OK;
 }
 
+/*
+ * One LM iteration's linear solve: assemble at `lambda` / `log_focal`, initializeBundleCg, then
+ * PCG (advanceBundleCg) for at most control->max_cg iterations. Returns x (m) and
+ * stats[0..3] = {PCG steps, converged, failed, final |r|^2 / |r0|^2}.
+ */
+int tcv_oracle_pcg(const tcv_ba_control* ctl, const double* scene, const uint32_t* index,
+                   double lambda, double log_focal, double* x, double* stats) {
+  Oracle o;
+  if (!oracle_init(&o, ctl, scene, index)) return TCV_BA_ERR_MEMORY;
+  for (int j = 0; j < 26; j++) o.state[j] = 0;
+  o.state[0] = lambda;
+  o.state[1] = 1;
+  if (o.shared) o.focal[o.C * 6 + 3] = o.focal[o.C * 6 + 4] = log_focal;
+  assemble(&o);
+  initialize_cg(&o);
+  for (uint32_t k = 0; k < ctl->max_cg && o.state[19] != 0; k++) {
+    schur_points(&o, VP(&o), VY(&o));
+    schur_cameras(&o, VP(&o), VY(&o), VS(&o));
+    advance_cg(&o);
+  }
+  memcpy(x, VX(&o), (size_t)o.m * sizeof(double));
+  stats[0] = o.state[23];
+  stats[1] = o.state[24];
+  stats[2] = o.state[20];
+  stats[3] = o.state[21] > 0 ? o.state[22] / o.state[21] : 0;
+  oracle_free(&o);
+  return TCV_BA_OK;
+}
+
 /* Scalar helpers, exported for the pins. */
 int tcv_oracle_project(const double* model, const double* q, double* out8) {
   Model k = {model[0], model[1], model[2], model[3], model[4], model[5]};

@@ -35,6 +35,9 @@ def library() -> ctypes.CDLL:
                                            ctypes.c_double, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                            vp, vp]
     lib.tcv_oracle_schur_probe.restype = ctypes.c_int
+    lib.tcv_oracle_pcg.argtypes = [P(_native.Control), vp, vp, ctypes.c_double, ctypes.c_double,
+                                   vp, vp]
+    lib.tcv_oracle_pcg.restype = ctypes.c_int
     lib.tcv_oracle_project.argtypes = [vp, vp, vp]
     lib.tcv_oracle_project.restype = ctypes.c_int
     lib.tcv_oracle_radial_limit.argtypes = [vp]
@@ -106,6 +109,19 @@ def schur_probe(control: _native.Control, packed: PackedScene, lam: float, log_f
     if blocks:
         return rhs, sp, z, int(status[0]), out
     return rhs, sp, z, int(status[0])
+
+
+def pcg(control: _native.Control, packed: PackedScene, lam: float, log_focal: float = 0.0):
+    """PCG solution x of one assembled system and {steps, converged, failed, ratio}."""
+    m = control.cameras * 6 + (1 if control.shared_focal else 0)
+    x, stats = np.empty(m), np.zeros(4)
+    scene = np.ascontiguousarray(packed.scene, dtype=np.float64)
+    index = np.ascontiguousarray(packed.index, dtype=np.uint32)
+    r = library().tcv_oracle_pcg(ctypes.byref(control), _ptr(scene), _ptr(index), float(lam),
+                                 float(log_focal), _ptr(x), _ptr(stats))
+    if r != 0:
+        raise RuntimeError(f"oracle pcg failed ({r})")
+    return x, {"steps": stats[0], "converged": stats[1], "failed": stats[2], "ratio": stats[3]}
 
 
 def project(model, q):

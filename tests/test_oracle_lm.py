@@ -1,0 +1,114 @@
+"""The whole LM operator on the CPU oracle against the invariants the reference's tests assert
+(tests/bundle-adjustment.test.ts:31-161, tests/typeshade/cpu.test.mjs:533-607,
+tests/typeshade/focal.test.mjs:13-94). The same invariants run on the GPU engine in
+tests/test_gpu_operator.py."""
+import math
+
+import numpy as np
+
+from oracle import oracle as O
+from typecv_amd import bundle as B
+from typecv_amd import synthetic as S
+
+FIXED = lambda C: min(96, max(24, 12 * C))  # noqa: E731
+SHARED = lambda C: min(256, max(32, 12 * C + 1))  # noqa: E731
+
+
+def run(model, options):
+    shared = bool(options.get("refineSharedFocal"))
+    prob = B.bundle_problem(model, options, SHARED if shared else FIXED)
+    packed = B.pack_bundle_scene(prob["input"], prob["byId"], prob["active"])
+    control = B.make_control(
+        packed.C, packed.P, packed.M, directional=packed.P - prob["finitePoints"],
+        anchor=prob["byId"][prob["anchorId"]], scale_view=prob["byId"][prob["scaleId"]],
+        normalize_scale=prob["fixed"] == 1, shared_focal=shared, iterations=prob["iterations"],
+        max_cg=prob["maxCg"], huber=prob["huber"], tolerance=prob["tolerance"],
+        stall_tolerance=B.bundle_stall_tolerance(options))
+    out, stats = O.solve(control, packed)
+    base = packed.C * 12 + packed.P * 3
+    return packed, out, base
+
+
+def center(pose12):
+    R = pose12[:9].reshape(3, 3)
+    return -R.T @ pose12[9:]
+
+
+def test_fixture_converges_and_keeps_gauge():
+    model = S.bundle_fixture()
+    packed, out, base = run(model, {})
+    assert out[base + 6] < 0.05 * out[base + 8]
+    assert out[base + 25] < 0.3
+    assert np.array_equal(out[:12], packed.scene[:12])  # anchor pose bit for bit
+    baseline0 = np.linalg.norm(center(packed.scene[:12]) - center(packed.scene[12:24]))
+    baseline1 = np.linalg.norm(center(out[:12]) - center(out[12:24]))
+    assert abs(baseline1 - baseline0) < 1e-10
+    assert out[base + 2] == baseline0
+
+
+def test_zero_iterations_is_identity():
+    packed, out, base = run(S.bundle_fixture(), {"iterations": 0})
+    assert np.array_equal(out[:base], packed.scene[:base])
+    assert out[base + 6] == out[base + 8] and out[base + 10] == 0
+
+
+def test_huber_keeps_the_outlier_and_converges():
+    packed, out, base = run(S.bundle_fixture(outlier=True), {"huberPx": 1})
+    assert out[base + 6] < out[base + 8]
+    assert packed.M == 160  # the observation stays in the problem
+
+
+def test_shifted_anchor_gauge():
+    packed, out, base = run(S.bundle_fixture(shift=(0.3, -0.2, 0.5)), {})
+    assert out[base + 6] < 0.05 * out[base + 8]
+    assert np.array_equal(out[:12], packed.scene[:12])
+
+
+def test_forced_pcg_failure_rolls_back_exactly():
+    packed, out, base = run(S.bundle_fixture(), {"iterations": 1, "maxCgIterations": 1,
+                                                 "cgRelativeTolerance": 1e-12})
+    assert out[base + 10] == 0 and out[base + 11] == 1 and out[base + 17] == 1
+    assert np.array_equal(out[:base], packed.scene[:base])
+
+
+def test_zero_baseline_and_cheirality_are_input_errors():
+    model = S.bundle_fixture()
+    model["views"][1]["pose"] = {k: np.array(v) for k, v in model["views"][0]["pose"].items()}
+    _, out, base = run(model, {})
+    assert out[base + 14] == 1
+    model = S.bundle_fixture()
+    model["landmarks"][3]["xyz"] = np.array([0.0, 0.0, -50.0])
+    _, out, base = run(model, {})
+    assert out[base + 14] == 2
+
+
+def test_local_window_keeps_fixed_poses():
+    packed, out, base = run(S.bundle_fixture(views=5), {"variableViewIds": [30, 40]})
+    for c in (0, 1, 4):
+        assert np.array_equal(out[c * 12: c * 12 + 12], packed.scene[c * 12: c * 12 + 12])
+    assert out[base + 6] < out[base + 8]
+
+
+def test_joint_focal_recovers_ground_truth():
+    fixture = S.joint_focal_fixture()
+    packed, out, base = run(fixture["input"], {"iterations": 40, "refineSharedFocal": True,
+                                                "maxCgIterations": 256,
+                                                "cgRelativeTolerance": 1e-10})
+    assert out[base + 6] < 1e-7 and out[base + 25] < 1e-4 and out[base + 8] > 50000
+    fx = fixture["input"]["views"][0]["camera"]["fx"] * math.exp(out[base + 26])
+    assert abs(fx - 800) < 0.001
+
+
+def test_joint_focal_with_every_pose_fixed_still_moves_the_focal():
+    fixture = S.joint_focal_fixture(points=20, views=4)
+    packed, out, base = run(fixture["input"], {"iterations": 15, "refineSharedFocal": True,
+                                                "variableViewIds": []})
+    assert out[base + 6] < out[base + 8]
+    assert np.array_equal(out[:packed.C * 12], packed.scene[:packed.C * 12])
+    assert out[base + 26] > 0  # the focal grows towards the truth (input scaled by 0.75)
+
+
+def test_stall_tolerance_ends_the_loop_early():
+    _, a, base = run(S.bundle_fixture(), {})
+    _, b, _ = run(S.bundle_fixture(), {"stallTolerance": 0.5})
+    assert b[base + 10] < a[base + 10]

@@ -74,6 +74,7 @@ class Report(ctypes.Structure):
         ("lm_iterations", ctypes.c_uint64),
         ("cg_iterations", ctypes.c_uint64),
         ("matvecs", ctypes.c_uint64),
+        ("matvec_launches", ctypes.c_uint64),
         ("flag_reads", ctypes.c_uint64),
         ("device_bytes", ctypes.c_uint64),
         ("point_tiles", ctypes.c_uint64),

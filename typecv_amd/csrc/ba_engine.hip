@@ -218,6 +218,7 @@ Ops select_ops(const tcv_ba_control* c) {
 }
 
 constexpr uint32_t kTaskObs = 1024;
+constexpr uint32_t kChunk = 8;  // PCG iterations between reads of the state words
 
 // Allocate, validate the index and build the plan. `scene`/`index` are device pointers.
 int setup(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene, const uint32_t* index,
@@ -358,7 +359,7 @@ int run_lm(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, hipSt
   bool lm_active = ctx->h_state[S_ACTIVE] != 0;
   double monitor_cost = ctx->h_state[S_COST];
   const bool monitor = c->stall_tolerance > 1e-9;
-  uint32_t hint = 8;
+  uint32_t hint = 16;
   const uint32_t update_blocks = blocks_for(q.C > q.P ? q.C : q.P, 256);
   const uint32_t trial_blocks = blocks_for((size_t)q.C * 12 + (size_t)q.P * 3, 256);
   if (timing) TRY(ensure_events(ctx, 3 * 256 + 3));
@@ -368,7 +369,11 @@ int run_lm(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, hipSt
     ops.camera_pass(q, st);
     ops.init_cg(q, st);
     HIP_TRY(hipGetLastError());
+    // PCG in chunks with one read of the state words after each. The first chunk is 3/4 of the
+    // previous LM iteration's PCG run, later ones kChunk iterations; a chunk that outlives
+    // convergence costs only guarded no-op launches (state word 19), as in the reference.
     uint32_t done = 0, chunk = hint;
+    double matvecs_before = ctx->h_state[S_MATVECS];
     while (done < c->max_cg) {
       const uint32_t n = chunk < c->max_cg - done ? chunk : c->max_cg - done;
       for (uint32_t k = 0; k < n; k++) {
@@ -380,11 +385,14 @@ int run_lm(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, hipSt
         ops.advance(q, st);
       }
       HIP_TRY(hipGetLastError());
-      if (rep) rep->matvecs += n;
+      if (rep) rep->matvec_launches += n;
       done += n;
       TRY(read_state(ctx, q, st, rep));
+      const uint32_t active = (uint32_t)(ctx->h_state[S_MATVECS] - matvecs_before);
+      matvecs_before = ctx->h_state[S_MATVECS];
+      if (rep) rep->matvecs += active;
       if (timing) {
-        for (uint32_t k = 0; k < n; k++) {
+        for (uint32_t k = 0; k < active && k < n; k++) {
           float a = 0, b = 0;
           HIP_TRY(hipEventElapsedTime(&a, ctx->events[3 * k], ctx->events[3 * k + 1]));
           HIP_TRY(hipEventElapsedTime(&b, ctx->events[3 * k + 1], ctx->events[3 * k + 2]));
@@ -394,11 +402,10 @@ int run_lm(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, hipSt
         }
       }
       if (ctx->h_state[S_PCG_ACTIVE] == 0) break;
-      chunk = chunk * 2 > 64 ? 64 : chunk * 2;
+      chunk = kChunk;
     }
-    // The next LM iteration starts with a chunk as long as this one's PCG run.
     const uint32_t steps = (uint32_t)ctx->h_state[S_STEPS];
-    hint = steps < 1 ? 1 : (steps > 64 ? 64 : steps);
+    hint = steps * 3 / 4 < kChunk ? kChunk : steps * 3 / 4;
     hipLaunchKernelGGL(k_update_cameras, dim3(blocks_for(q.C, 64)), dim3(64), 0, st, q);
     ops.update_points(q, st);
     hipLaunchKernelGGL(k_gauge, dim3(1), dim3(256), 0, st, q);

@@ -34,6 +34,7 @@ enum : int {
   S_FOCAL_DIAG = 28,     // damped focal diagonal H_ff (focal[6C])
   S_FOCAL_INV = 29,      // focal preconditioner 1 / reduced diagonal, or -1 (focal[6C+1])
   S_FOCAL_RHS = 30,      // reduced focal RHS (focal[6C+2])
+  S_MATVECS = 31,        // Schur products that ran while PCG was active (whole solve)
   S_WORDS = 32
 };
 

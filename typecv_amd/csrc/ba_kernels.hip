@@ -834,6 +834,7 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_advance(Problem q) {
   double* S = vs(q);
   double* z = vz(q);
   double den = block_sum(schur_finish<SHARED>(q, p, S, lds), lds);
+  if (tid == 0) s[S_MATVECS] += 1;
   if (!(den > 0)) {
     if (tid == 0) {
       s[S_PCG_ACTIVE] = 0;
