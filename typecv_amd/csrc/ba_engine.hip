@@ -159,12 +159,14 @@ int build_plan(const std::vector<uint32_t>& ps, const std::vector<uint32_t>& cs,
 }
 
 uint32_t blocks_for(size_t n, uint32_t threads) { return (uint32_t)((n + threads - 1) / threads); }
+uint32_t tile_blocks(const Problem& q) { return blocks_for(q.n_tiles, TILE_WAVES); }
 
 // The launches of one form (W storage type and route).
 template <class WT, bool SHARED>
 struct Launch {
   static void point_pass(const Problem& q, hipStream_t s) {
-    hipLaunchKernelGGL((k_point_pass<WT, SHARED>), dim3(q.n_tiles), dim3(TILE), 0, s, q);
+    hipLaunchKernelGGL((k_point_pass<WT, SHARED>), dim3(tile_blocks(q)), dim3(TILE * TILE_WAVES),
+                       0, s, q);
   }
   static void camera_pass(const Problem& q, hipStream_t s) {
     hipLaunchKernelGGL((k_camera_pass<WT, SHARED>), dim3(q.n_tasks), dim3(TASK_THREADS), 0, s, q);
@@ -174,8 +176,8 @@ struct Launch {
     hipLaunchKernelGGL((k_init_cg<SHARED>), dim3(1), dim3(CG_THREADS), 0, s, q);
   }
   static void point_sweep(const Problem& q, hipStream_t s) {
-    hipLaunchKernelGGL((k_point_sweep<WT, SHARED, SWEEP_SCHUR>), dim3(q.n_tiles), dim3(TILE), 0, s,
-                       q);
+    hipLaunchKernelGGL((k_point_sweep<WT, SHARED, SWEEP_SCHUR>), dim3(tile_blocks(q)),
+                       dim3(TILE * TILE_WAVES), 0, s, q);
   }
   static void camera_sweep(const Problem& q, hipStream_t s) {
     hipLaunchKernelGGL((k_camera_sweep<WT, SHARED>), dim3(q.n_tasks), dim3(TASK_THREADS), 0, s, q);
@@ -184,8 +186,8 @@ struct Launch {
     hipLaunchKernelGGL((k_cg_advance<SHARED>), dim3(1), dim3(CG_THREADS), 0, s, q);
   }
   static void update_points(const Problem& q, hipStream_t s) {
-    hipLaunchKernelGGL((k_point_sweep<WT, SHARED, SWEEP_UPDATE>), dim3(q.n_tiles), dim3(TILE), 0,
-                       s, q);
+    hipLaunchKernelGGL((k_point_sweep<WT, SHARED, SWEEP_UPDATE>), dim3(tile_blocks(q)),
+                       dim3(TILE * TILE_WAVES), 0, s, q);
   }
   static void probe(const Problem& q, const double* v, double* sv, double* zv, double* rhs,
                     hipStream_t s) {

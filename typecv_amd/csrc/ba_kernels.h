@@ -39,6 +39,13 @@ enum : int {
 };
 
 constexpr int TILE = 64;           // observations per wave tile
+#ifndef TCV_TILE_WAVES
+#define TCV_TILE_WAVES 4
+#endif
+#ifndef TCV_W_NONTEMPORAL
+#define TCV_W_NONTEMPORAL 1
+#endif
+constexpr int TILE_WAVES = TCV_TILE_WAVES;  // wave tiles per workgroup of the point-tile kernels
 constexpr int TASK_THREADS = 256;  // threads of a camera-task workgroup
 constexpr int CG_THREADS = 1024;   // threads of the single-workgroup PCG kernels
 constexpr int ASM_WORDS = 56;      // per-task camera assembly sums (see camera_pass)

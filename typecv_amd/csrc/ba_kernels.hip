@@ -58,6 +58,26 @@ __device__ __forceinline__ double block_sum(double v, double* lds) {
   return s;
 }
 
+// Orders one wave's LDS writes before its later LDS reads. The point-tile kernels give every wave
+// its own LDS slice and their loops have per-wave trip counts, so they synchronise per wave, never
+// per workgroup.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// A cross-block word: streamed once per sweep, so it is loaded non-temporally to leave the caches
+// to the gathered vectors.
+template <class WT>
+__device__ __forceinline__ double wload(const WT* p) {
+#if TCV_W_NONTEMPORAL
+  return (double)__builtin_nontemporal_load(p);
+#else
+  return (double)*p;
+#endif
+}
+
 // Linearisation of one observation (linearizeBundle b:251-297 / j:287-336).
 struct Lin {
   double Jc[12];  // d(u,v)/d(omega, v): row u then row v (zero for a fixed camera)
@@ -308,12 +328,16 @@ __global__ __launch_bounds__(CG_THREADS) void k_finish_cost(Problem q) {
 // order, damps V, inverts it and writes the point block. The shared route adds the point's share of
 // the reduced focal diagonal and RHS (bundleFocalPoints), summed per tile in point order.
 template <class WT, bool SHARED>
-__global__ __launch_bounds__(TILE) void k_point_pass(Problem q) {
+__global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_pass(Problem q) {
   constexpr int NS = SHARED ? 13 : 10;  // staged words per observation: V 6, b 3, [fc 3], bad
-  __shared__ double st[TILE][NS];
-  __shared__ double fsum[TILE][2];
+  __shared__ double st_all[TILE_WAVES][TILE][NS];
+  __shared__ double fsum_all[TILE_WAVES][TILE][2];
   if (q.state[S_ACTIVE] == 0) return;
-  const uint32_t t = blockIdx.x, lane = threadIdx.x;
+  const uint32_t wave = threadIdx.x / TILE, lane = threadIdx.x % TILE;
+  const uint32_t t = blockIdx.x * TILE_WAVES + wave;
+  if (t >= q.n_tiles) return;
+  auto& st = st_all[wave];
+  auto& fsum = fsum_all[wave];
   const uint32_t fp = q.tiles[t], ep = q.tiles[t + 1];
   const uint32_t* ps = point_start(q);
   const uint32_t s = ps[fp], e = ps[ep];
@@ -363,7 +387,7 @@ __global__ __launch_bounds__(TILE) void k_point_pass(Problem q) {
         o[NS - 1] = 1;
       }
     }
-    __syncthreads();
+    wave_sync();
     if (has) {
       const uint32_t lo = mys > base ? mys : base, hi = mye < base + TILE ? mye : base + TILE;
       for (uint32_t j = lo; j < hi; j++) {
@@ -375,7 +399,7 @@ __global__ __launch_bounds__(TILE) void k_point_pass(Problem q) {
         bad = bad || o[NS - 1] != 0;
       }
     }
-    __syncthreads();
+    wave_sync();
   }
   bool failed = false;
   double dF = 0, dR = 0;
@@ -409,7 +433,7 @@ __global__ __launch_bounds__(TILE) void k_point_pass(Problem q) {
   if (SHARED) {
     fsum[lane][0] = dF;
     fsum[lane][1] = dR;
-    __syncthreads();
+    wave_sync();
   }
   if (lane == 0) {
     q.tile_fail[t] = any != 0ull;
@@ -580,17 +604,16 @@ __device__ __forceinline__ double* vfpart(const Problem& q) {
   return q.vec + 5 * (size_t)q.m + 3 * (size_t)q.P;
 }
 
-// z_c = M_c^-1 r_c for one camera.
-__device__ __forceinline__ void precondition(const Problem& q, uint32_t c, const double* r,
-                                             double* z) {
-  const double* inv = q.camInv + (size_t)c * 36;
-  double rc[6];
-  for (int k = 0; k < 6; k++) rc[k] = r[c * 6 + k];
-  for (int j = 0; j < 6; j++) {
-    double v = 0;
-    for (int k = 0; k < 6; k++) v += inv[j * 6 + k] * rc[k];
-    z[c * 6 + j] = v;
-  }
+// Row `row` = 6c + j of a 6x6-block-diagonal product: sum_k B_c[j][k] v[6c + k]. Consecutive
+// rows read consecutive 48-byte block rows, so a wave's loads are coalesced.
+__device__ __forceinline__ double block_row(const double* blocks, uint32_t row, const double* v) {
+  const uint32_t c = row / 6;
+  const double* b = blocks + (size_t)c * 36 + (size_t)(row - c * 6) * 6;
+  const double* vc = v + (size_t)c * 6;
+  double a = 0;
+#pragma unroll
+  for (int k = 0; k < 6; k++) a += b[k] * vc[k];
+  return a;
 }
 
 // initializeBundleCg (with dampBundleFocal and the rest of bundleFocalPoints on the shared route):
@@ -647,17 +670,16 @@ __global__ __launch_bounds__(CG_THREADS) void k_init_cg(Problem q) {
   double* r = vr(q);
   double* p = vp(q);
   double* z = vz(q);
-  for (uint32_t c = tid; c < q.C; c += CG_THREADS) {
-    for (int j = 0; j < 6; j++) {
-      x[c * 6 + j] = 0;
-      r[c * 6 + j] = q.camRhs[(size_t)c * 6 + j];
-    }
-    precondition(q, c, r, z);
-    for (int j = 0; j < 6; j++) {
-      p[c * 6 + j] = z[c * 6 + j];
-      rho += r[c * 6 + j] * z[c * 6 + j];
-      res += r[c * 6 + j] * r[c * 6 + j];
-    }
+  const uint32_t n = q.C * 6;
+  for (uint32_t row = tid; row < n; row += CG_THREADS) {
+    const double rv = q.camRhs[row];
+    const double zv = block_row(q.camInv, row, q.camRhs);
+    x[row] = 0;
+    r[row] = rv;
+    z[row] = zv;
+    p[row] = zv;
+    rho += rv * zv;
+    res += rv * rv;
   }
   if (SHARED && tid == 0) {
     const uint32_t f = q.m - 1;
@@ -688,12 +710,15 @@ __global__ __launch_bounds__(CG_THREADS) void k_init_cg(Problem q) {
 enum SweepMode { SWEEP_SCHUR = 0, SWEEP_UPDATE = 1 };
 
 template <class WT, bool SHARED, int MODE>
-__global__ __launch_bounds__(TILE) void k_point_sweep(Problem q) {
-  __shared__ double st[TILE][3];
+__global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_sweep(Problem q) {
+  __shared__ double st_all[TILE_WAVES][TILE][3];
   if (MODE == SWEEP_SCHUR && q.state[S_PCG_ACTIVE] == 0) return;
   if (MODE == SWEEP_UPDATE && (q.state[S_ACTIVE] == 0 || q.state[S_PCG_FAILED] != 0)) return;
   const double* v = MODE == SWEEP_SCHUR ? vp(q) : vx(q);
-  const uint32_t t = blockIdx.x, lane = threadIdx.x;
+  const uint32_t wave = threadIdx.x / TILE, lane = threadIdx.x % TILE;
+  const uint32_t t = blockIdx.x * TILE_WAVES + wave;
+  if (t >= q.n_tiles) return;
+  auto& st = st_all[wave];
   const uint32_t fp = q.tiles[t], ep = q.tiles[t + 1];
   const uint32_t* ps = point_start(q);
   const uint32_t s = ps[fp], e = ps[ep];
@@ -713,15 +738,15 @@ __global__ __launch_bounds__(TILE) void k_point_sweep(Problem q) {
       for (int j = 0; j < 3; j++) {
         double a = 0;
 #pragma unroll
-        for (int k = 0; k < 6; k++) a += (double)wrow<WT>(q.Wp, k * 3 + j, q.M)[i] * vc[k];
-        if (SHARED) a += (double)wrow<WT>(q.Wp, 18 + j, q.M)[i] * vf;
+        for (int k = 0; k < 6; k++) a += wload(wrow<WT>(q.Wp, k * 3 + j, q.M) + i) * vc[k];
+        if (SHARED) a += wload(wrow<WT>(q.Wp, 18 + j, q.M) + i) * vf;
         tj[j] = a;
       }
       st[lane][0] = tj[0];
       st[lane][1] = tj[1];
       st[lane][2] = tj[2];
     }
-    __syncthreads();
+    wave_sync();
     if (has) {
       const uint32_t lo = mys > base ? mys : base, hi = mye < base + TILE ? mye : base + TILE;
       for (uint32_t j = lo; j < hi; j++) {
@@ -730,7 +755,7 @@ __global__ __launch_bounds__(TILE) void k_point_sweep(Problem q) {
         acc[2] += st[j - base][2];
       }
     }
-    __syncthreads();
+    wave_sync();
   }
   if (!has) return;
   const double* pb = q.pblk + (size_t)my * 12;
@@ -763,9 +788,9 @@ __global__ __launch_bounds__(TASK_THREADS) void k_camera_sweep(Problem q) {
     const double y0 = y[(size_t)p * 3], y1 = y[(size_t)p * 3 + 1], y2 = y[(size_t)p * 3 + 2];
 #pragma unroll
     for (int j = 0; j < NV; j++)
-      acc[j] += (double)wrow<WT>(q.Wc, j * 3, q.M)[n] * y0 +
-                (double)wrow<WT>(q.Wc, j * 3 + 1, q.M)[n] * y1 +
-                (double)wrow<WT>(q.Wc, j * 3 + 2, q.M)[n] * y2;
+      acc[j] += wload(wrow<WT>(q.Wc, j * 3, q.M) + n) * y0 +
+                wload(wrow<WT>(q.Wc, j * 3 + 1, q.M) + n) * y1 +
+                wload(wrow<WT>(q.Wc, j * 3 + 2, q.M) + n) * y2;
   }
 #pragma unroll
   for (int k = 0; k < NV; k++)
@@ -786,30 +811,23 @@ __global__ __launch_bounds__(TASK_THREADS) void k_camera_sweep(Problem q) {
 // Uniform control flow (uses block_sum).
 template <bool SHARED>
 __device__ double schur_finish(const Problem& q, const double* v, double* sv, double* lds) {
-  const uint32_t tid = threadIdx.x;
+  const uint32_t tid = threadIdx.x, n = q.C * 6;
   const double vf = SHARED ? v[q.m - 1] : 0.0;
   double dot = 0, fsum = 0;
-  for (uint32_t c = tid; c < q.C; c += CG_THREADS) {
-    double sums[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (uint32_t row = tid; row < n; row += CG_THREADS) {
+    const uint32_t c = row / 6, j = row - c * 6;
+    double sum = 0;
     for (uint32_t t = q.task_start[c]; t < q.task_start[c + 1]; t++)
-      for (int j = 0; j < (SHARED ? 7 : 6); j++) sums[j] += q.task_part[(size_t)t * 8 + j];
-    const double* A = q.camA + (size_t)c * 36;
-    double vc[6];
-    for (int k = 0; k < 6; k++) vc[k] = v[c * 6 + k];
-    for (int j = 0; j < 6; j++) {
-      double a = SHARED ? q.camHcf[(size_t)c * 6 + j] * vf : 0.0;
-      for (int k = 0; k < 6; k++) a += A[j * 6 + k] * vc[k];
-      a -= sums[j];
-      sv[c * 6 + j] = a;
-      dot += vc[j] * a;
-    }
-    if (SHARED) {
-      double f = 0;
-      for (int k = 0; k < 6; k++) f += q.camHcf[(size_t)c * 6 + k] * vc[k];
-      fsum += f - sums[6];
-    }
+      sum += q.task_part[(size_t)t * 8 + j];
+    double a = SHARED ? q.camHcf[row] * vf : 0.0;
+    a += block_row(q.camA, row, v);
+    a -= sum;
+    sv[row] = a;
+    dot += v[row] * a;
+    if (SHARED) fsum += q.camHcf[row] * v[row];
   }
   if (SHARED) {
+    for (uint32_t t = tid; t < q.n_tasks; t += CG_THREADS) fsum -= q.task_part[(size_t)t * 8 + 6];
     fsum = block_sum(fsum, lds);
     const double sf = q.state[S_FOCAL_DIAG] * vf + fsum;
     if (tid == 0) {
@@ -851,15 +869,12 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_advance(Problem q) {
     return;
   }
   double res = 0, rho = 0;
-  for (uint32_t c = tid; c < q.C; c += CG_THREADS) {
-    for (int j = 0; j < 6; j++) {
-      const uint32_t i = c * 6 + j;
-      x[i] += alpha * p[i];
-      r[i] -= alpha * S[i];
-      res += r[i] * r[i];
-    }
-    precondition(q, c, r, z);
-    for (int j = 0; j < 6; j++) rho += z[c * 6 + j] * r[c * 6 + j];
+  const uint32_t n = q.C * 6;
+  for (uint32_t i = tid; i < n; i += CG_THREADS) {
+    x[i] += alpha * p[i];
+    const double ri = r[i] - alpha * S[i];
+    r[i] = ri;
+    res += ri * ri;
   }
   if (SHARED && tid == 0) {
     const uint32_t f = q.m - 1;
@@ -868,6 +883,12 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_advance(Problem q) {
     res += r[f] * r[f];
     z[f] = s[S_FOCAL_INV] * r[f];
     rho += z[f] * r[f];
+  }
+  __syncthreads();  // every row of r before the block products
+  for (uint32_t i = tid; i < n; i += CG_THREADS) {
+    const double zi = block_row(q.camInv, i, r);
+    z[i] = zi;
+    rho += zi * r[i];
   }
   res = block_sum(res, lds);
   rho = block_sum(rho, lds);
@@ -891,8 +912,7 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_advance(Problem q) {
     return;
   }
   const double beta = rho / rho0;
-  for (uint32_t c = tid; c < q.C; c += CG_THREADS)
-    for (int j = 0; j < 6; j++) p[c * 6 + j] = z[c * 6 + j] + beta * p[c * 6 + j];
+  for (uint32_t i = tid; i < n; i += CG_THREADS) p[i] = z[i] + beta * p[i];
   if (SHARED && tid == 0) p[q.m - 1] = z[q.m - 1] + beta * p[q.m - 1];
   if (tid == 0) s[S_RHO] = rho;
 }
@@ -903,9 +923,9 @@ __global__ __launch_bounds__(CG_THREADS) void k_probe(Problem q, const double* v
                                                       double* zv, double* rhs) {
   __shared__ double lds[CG_THREADS / 64];
   (void)schur_finish<SHARED>(q, v, sv, lds);
-  for (uint32_t c = threadIdx.x; c < q.C; c += CG_THREADS) {
-    precondition(q, c, v, zv);
-    for (int j = 0; j < 6; j++) rhs[c * 6 + j] = q.camRhs[(size_t)c * 6 + j];
+  for (uint32_t row = threadIdx.x; row < q.C * 6; row += CG_THREADS) {
+    zv[row] = block_row(q.camInv, row, v);
+    rhs[row] = q.camRhs[row];
   }
   if (SHARED && threadIdx.x == 0) {
     zv[q.m - 1] = q.state[S_FOCAL_INV] * v[q.m - 1];
