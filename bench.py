@@ -51,6 +51,11 @@ def main() -> int:
     ap.add_argument("--points", type=int, default=200_000)
     ap.add_argument("--iterations", type=int, default=30)
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
+    ap.add_argument("--implicit", action="store_true",
+                    help="relinearise cross blocks in every Schur sweep instead of streaming them")
+    ap.add_argument("--refine", type=float, default=0.0,
+                    help="mixed form: FP64 residual target of the refinement (0: the PCG tolerance)")
+    ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-lm", type=int, default=2, help="LM iterations of the CPU sample (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -67,12 +72,13 @@ def main() -> int:
     from typecv_amd import synthetic as S
 
     t0 = time.time()
-    packed, info = S.trajectory_scene(cameras=args.cameras, points=args.points, seed=2)
+    packed, info = S.trajectory_scene(cameras=args.cameras, points=args.points, seed=args.seed)
     C, P, M = packed.C, packed.P, packed.M
     log(f"scene C={C} P={P} M={M} generated in {time.time() - t0:.1f}s")
     control = B.make_control(C, P, M, shared_focal=True, huber=1.0, iterations=args.iterations,
                              max_cg=min(256, max(32, 12 * C + 1)), precision=args.precision,
-                             timing=True)
+                             refinement_tolerance=args.refine, timing=True,
+                             implicit=args.implicit)
     lib = _native.library()
     device = B.HipDevice(0)
     scene = torch.from_numpy(packed.scene).cuda()
@@ -129,7 +135,9 @@ def main() -> int:
             "cameras": C, "points": P, "observations": M, "lm_iterations_max": args.iterations,
             "max_cg": control.max_cg, "huber_px": 1.0, "precision": args.precision,
             "outliers": info["outliers"], "focal_factor": info["focal_factor"],
+            "operator": "implicit" if args.implicit else "stored",
         },
+        "refinements_per_step": sum(r["refinement_steps"] for r in reports) / args.steps,
         "observations_per_s": M * lm / elapsed,
         "lm_iterations_per_step": lm / args.steps,
         "cg_iterations_per_step": sum(r["cg_iterations"] for r in reports) / args.steps,

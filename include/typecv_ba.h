@@ -80,9 +80,14 @@ typedef struct tcv_ba_control {
   double huber;                /* Huber threshold in pixels, [1e-12, 1e6] */
   double relative_tolerance;   /* PCG relative residual tolerance, [1e-12, 1e-2] */
   double stall_tolerance;      /* host LM stall rule; <= 1e-9 leaves only the device rule */
+  double refinement_tolerance; /* mixed form: relative FP64 normal-equation residual that the
+                                  outer refinement reaches (0: relative_tolerance) */
 } tcv_ba_control;
 
 #define TCV_BA_FLAG_TIMING 1u
+/* PCG (and the probe) use the implicit FP64 Schur operator: every cross block is relinearised from
+ * the scene in each sweep instead of streamed from storage. Same results to rounding. */
+#define TCV_BA_FLAG_IMPLICIT 2u
 
 /* What one solve did. Times are milliseconds of host wall clock unless named otherwise. */
 typedef struct tcv_ba_report {

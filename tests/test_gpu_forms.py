@@ -1,0 +1,88 @@
+"""The engine's other arithmetic forms against the FP64 oracle.
+
+* Implicit FP64 operator (TCV_BA_FLAG_IMPLICIT: cross blocks relinearised in every sweep instead of
+  streamed): the assembled system within 1e-10 relative and whole solves to the FP64 bars of
+  tests/test_gpu_parity.py (cost 1e-9 relative, coordinates 1e-7·max(1, |x|)).
+* Mixed precision (FP32 cross blocks and Schur sweeps with FP64 accumulation; FP64 V^-1,
+  preconditioner, PCG dots and LM decisions; outer refinement of the FP64 normal-equation
+  residual): the final cost within 1e-6 relative of the FP64 oracle, with the
+  pose/point error against the oracle printed and bounded by 1e-5·max(1, |x|).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from typecv_amd import bundle as B
+from typecv_amd import synthetic as S
+
+from test_gpu_parity import CASES, PACKED, assert_solve_parity, packed_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def with_flags(control, **kw):
+    c = B.make_control(control.cameras, control.points, control.observations,
+                       directional=control.directional_points, anchor=control.anchor,
+                       scale_view=control.scale_view, normalize_scale=bool(control.normalize_scale),
+                       shared_focal=bool(control.shared_focal), iterations=control.iterations,
+                       max_cg=control.max_cg, huber=control.huber,
+                       tolerance=control.relative_tolerance, **kw)
+    return c
+
+
+@pytest.mark.parametrize("name,model,options", CASES, ids=[c[0] for c in CASES])
+def test_implicit_operator_matches_oracle(device, name, model, options):
+    print(f"\n[implicit probe] {name}", flush=True)
+    packed, control = packed_problem(model, options)
+    implicit = with_flags(control, implicit=True)
+    m = control.cameras * 6 + control.shared_focal
+    p = np.random.default_rng(11).normal(size=m)
+    rhs, sp, z, status = O.schur_probe(control, packed, 0.05, 0.01, p)
+    grhs, gsp, gz, gstatus, _ = B.schur_probe(device, implicit, packed, 0.05, 0.01, p)
+    assert gstatus == status
+    for a, b in ((grhs, rhs), (gsp, sp), (gz, z)):
+        assert np.max(np.abs(a - b)) <= 1e-10 * max(1.0, np.max(np.abs(b)))
+
+
+@pytest.mark.parametrize("name,packed,control", PACKED, ids=[c[0] for c in PACKED])
+def test_implicit_solve_matches_oracle(device, name, packed, control):
+    print(f"\n[implicit solve] {name}", flush=True)
+    expected, _ = O.solve(control, packed)
+    got, _ = B.solve_packed(device, with_flags(control, implicit=True), packed)
+    assert_solve_parity(got, expected, packed.C * 12 + packed.P * 3)
+
+
+MIXED_CASES = [(n, m, o) for n, m, o in CASES if n != "ring_320"]
+
+
+@pytest.mark.parametrize("name,model,options", MIXED_CASES, ids=[c[0] for c in MIXED_CASES])
+def test_mixed_solve_against_fp64_oracle(device, name, model, options):
+    print(f"\n[mixed] {name}", flush=True)
+    packed, control = packed_problem(model, options)
+    expected, _ = O.solve(control, packed)
+    got, report = B.solve_packed(device, with_flags(control, precision="mixed"), packed)
+    base = packed.C * 12 + packed.P * 3
+    cost = expected[base + 6]
+    rel = abs(got[base + 6] - cost) / max(1.0, abs(cost))
+    coord = np.max(np.abs(got[:base] - expected[:base]) / np.maximum(1.0, np.abs(expected[:base])))
+    print(f"  final cost {got[base + 6]:.12g} vs {cost:.12g} (rel {rel:.2e}); "
+          f"max coordinate error {coord:.2e}; refinements {report['refinement_steps']:.0f}")
+    assert rel <= 1e-6
+    assert coord <= 1e-5
+
+
+@pytest.mark.parametrize("name,packed,control", PACKED, ids=[c[0] for c in PACKED])
+def test_mixed_packed_against_fp64_oracle(device, name, packed, control):
+    print(f"\n[mixed] {name} M={packed.M}", flush=True)
+    expected, _ = O.solve(control, packed)
+    for refine in (0.0, 1e-10):
+        got, report = B.solve_packed(
+            device, with_flags(control, precision="mixed", refinement_tolerance=refine), packed)
+        base = packed.C * 12 + packed.P * 3
+        cost = expected[base + 6]
+        rel = abs(got[base + 6] - cost) / max(1.0, abs(cost))
+        coord = np.max(np.abs(got[:base] - expected[:base]) / np.maximum(1.0, np.abs(expected[:base])))
+        print(f"  refine {refine:g}: cost rel {rel:.2e}, coordinate {coord:.2e}, "
+              f"refinements {report['refinement_steps']:.0f}, CG {report['cg_iterations']}")
+        assert rel <= 1e-6
+        assert coord <= 1e-5

@@ -12,6 +12,7 @@ LIBRARY = Path(__file__).resolve().parent / "libtypecv_ba.so"
 FP64 = 0
 MIXED = 1
 FLAG_TIMING = 1
+FLAG_IMPLICIT = 2
 
 STATUS = {
     0: "ok",
@@ -58,6 +59,7 @@ class Control(ctypes.Structure):
         ("huber", ctypes.c_double),
         ("relative_tolerance", ctypes.c_double),
         ("stall_tolerance", ctypes.c_double),
+        ("refinement_tolerance", ctypes.c_double),
     ]
 
 

@@ -315,15 +315,17 @@ def make_control(C: int, P: int, M: int, *, directional: int = 0, anchor: int = 
                  scale_view: int = 1, normalize_scale: bool = True, shared_focal: bool = False,
                  iterations: int = 20, max_cg: int = 24, precision: str = "fp64",
                  huber: float = 2.0, tolerance: float = 1e-6, stall_tolerance: float = 1e-9,
-                 timing: bool = False) -> _native.Control:
+                 refinement_tolerance: float = 0.0, timing: bool = False,
+                 implicit: bool = False) -> _native.Control:
     """The `tcv_ba_control` of a problem."""
+    flags = (_native.FLAG_TIMING if timing else 0) | (_native.FLAG_IMPLICIT if implicit else 0)
     return _native.Control(
         cameras=C, points=P, directional_points=directional, observations=M, anchor=anchor,
         scale_view=scale_view, normalize_scale=int(normalize_scale),
         shared_focal=int(shared_focal), iterations=iterations, max_cg=max_cg,
         precision=_native.MIXED if precision == "mixed" else _native.FP64,
-        flags=_native.FLAG_TIMING if timing else 0, huber=huber, relative_tolerance=tolerance,
-        stall_tolerance=stall_tolerance)
+        flags=flags, huber=huber, relative_tolerance=tolerance,
+        stall_tolerance=stall_tolerance, refinement_tolerance=refinement_tolerance)
 
 
 def solve_packed(device: HipDevice, control: _native.Control, packed: PackedScene):

@@ -98,7 +98,10 @@ int check_control(const tcv_ba_control* c) {
                   c->anchor != c->scale_view && c->normalize_scale <= 1 && c->shared_focal <= 1 &&
                   (c->precision == TCV_BA_FP64 || c->precision == TCV_BA_MIXED) &&
                   3 * M + P + 2 * C + 2 <= 0xffffffffull && 18 * C + 3 * P + 2 * M <= 0xffffffffull &&
-                  c->stall_tolerance >= 0 && c->stall_tolerance < 1;
+                  c->stall_tolerance >= 0 && c->stall_tolerance < 1 &&
+                  (c->refinement_tolerance == 0 ||
+                   (c->refinement_tolerance >= 1e-14 && c->refinement_tolerance <= 1e-2)) &&
+                  (c->flags & ~(TCV_BA_FLAG_TIMING | TCV_BA_FLAG_IMPLICIT)) == 0;
   if (!ok) {
     set_error("control outside the admitted ranges (bundleProblem)");
     return TCV_BA_ERR_ARGUMENT;
@@ -193,6 +196,23 @@ struct Launch {
                     hipStream_t s) {
     hipLaunchKernelGGL((k_probe<SHARED>), dim3(1), dim3(CG_THREADS), 0, s, q, v, sv, zv, rhs);
   }
+  // The implicit FP64 operator: sweeps of the vector at q.vec + offset (x: 0, p: 2m).
+  static void implicit_point_sweep(const Problem& q, size_t offset, hipStream_t s) {
+    hipLaunchKernelGGL((k_point_sweep_implicit<SHARED>), dim3(tile_blocks(q)),
+                       dim3(TILE * TILE_WAVES), 0, s, q, (const double*)(q.vec + offset));
+  }
+  static void implicit_p_sweep(const Problem& q, hipStream_t s) {
+    implicit_point_sweep(q, 2 * (size_t)q.m, s);
+  }
+  static void implicit_camera_sweep(const Problem& q, hipStream_t s) {
+    hipLaunchKernelGGL((k_camera_sweep_implicit<SHARED>), dim3(q.n_tasks), dim3(TASK_THREADS), 0,
+                       s, q);
+  }
+  static void refine(const Problem& q, hipStream_t s) {
+    implicit_point_sweep(q, 0, s);
+    implicit_camera_sweep(q, s);
+    hipLaunchKernelGGL((k_refine<SHARED>), dim3(1), dim3(CG_THREADS), 0, s, q);
+  }
 };
 
 struct Ops {
@@ -204,23 +224,32 @@ struct Ops {
   void (*advance)(const Problem&, hipStream_t);
   void (*update_points)(const Problem&, hipStream_t);
   void (*probe)(const Problem&, const double*, double*, double*, double*, hipStream_t);
+  void (*refine)(const Problem&, hipStream_t);
 };
 
 template <class WT, bool SHARED>
-Ops ops_for() {
+Ops ops_for(bool implicit) {
   using L = Launch<WT, SHARED>;
-  return Ops{L::point_pass,   L::camera_pass, L::init_cg,       L::point_sweep,
-             L::camera_sweep, L::advance,     L::update_points, L::probe};
+  Ops o{L::point_pass,   L::camera_pass, L::init_cg,       L::point_sweep, L::camera_sweep,
+        L::advance,      L::update_points, L::probe,       L::refine};
+  if (implicit) {
+    o.point_sweep = L::implicit_p_sweep;
+    o.camera_sweep = L::implicit_camera_sweep;
+  }
+  return o;
 }
 
 Ops select_ops(const tcv_ba_control* c) {
   const bool shared = c->shared_focal != 0;
-  if (c->precision == TCV_BA_MIXED) return shared ? ops_for<float, true>() : ops_for<float, false>();
-  return shared ? ops_for<double, true>() : ops_for<double, false>();
+  const bool implicit = (c->flags & TCV_BA_FLAG_IMPLICIT) != 0;
+  if (c->precision == TCV_BA_MIXED)
+    return shared ? ops_for<float, true>(implicit) : ops_for<float, false>(implicit);
+  return shared ? ops_for<double, true>(implicit) : ops_for<double, false>(implicit);
 }
 
 constexpr uint32_t kTaskObs = 1024;
 constexpr uint32_t kChunk = 8;  // PCG iterations between reads of the state words
+constexpr uint32_t kMaxRefinements = 4;  // mixed form: refinement restarts per LM iteration
 
 // Allocate, validate the index and build the plan. `scene`/`index` are device pointers.
 int setup(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene, const uint32_t* index,
@@ -251,8 +280,9 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene, con
   TRY(ensure(ctx->tasks, plan.tasks.size() * 4));
   TRY(ensure(ctx->task_start, plan.task_start.size() * 4));
   TRY(ensure(ctx->cam_point, (size_t)M * 4));
-  TRY(ensure(ctx->Wp, (size_t)K * M * ws));
-  TRY(ensure(ctx->Wc, (size_t)K * M * ws));
+  const size_t w_words = (size_t)K * (((size_t)M + 63) / 64) * 64;
+  TRY(ensure(ctx->Wp, w_words * ws));
+  TRY(ensure(ctx->Wc, w_words * ws));
   TRY(ensure(ctx->pblk, (size_t)P * 12 * 8));
   TRY(ensure(ctx->tile_fail, (size_t)n_tiles * 4));
   TRY(ensure(ctx->tile_focal, (size_t)n_tiles * 16));
@@ -291,6 +321,7 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene, con
   q.n_tasks = n_tasks;
   q.huber = c->huber;
   q.tol = c->relative_tolerance;
+  q.refine_tol = c->refinement_tolerance > 0 ? c->refinement_tolerance : c->relative_tolerance;
   q.scene = (double*)ctx->scene.ptr;
   q.trial = (double*)ctx->trial.ptr;
   q.index = index;
@@ -348,6 +379,48 @@ int ensure_events(tcv_ba_context* ctx, size_t n) {
   return TCV_BA_OK;
 }
 
+// PCG in chunks with one read of the state words after each: the first chunk `first` iterations,
+// later ones kChunk, at most `budget` in all. A chunk that outlives convergence costs only guarded
+// no-op launches (state word 19), as the reference's chunks do. With TCV_BA_FLAG_TIMING the two
+// sweeps of every product that ran while PCG was active are timed with HIP events.
+int run_pcg(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, const Ops& ops,
+            hipStream_t st, tcv_ba_report* rep, uint32_t first, uint32_t budget) {
+  const bool timing = (c->flags & TCV_BA_FLAG_TIMING) != 0 && rep;
+  uint32_t done = 0, chunk = first;
+  double matvecs_before = ctx->h_state[S_MATVECS];
+  while (done < budget) {
+    const uint32_t n = chunk < budget - done ? chunk : budget - done;
+    for (uint32_t k = 0; k < n; k++) {
+      if (timing) HIP_TRY(hipEventRecord(ctx->events[3 * k], st));
+      ops.point_sweep(q, st);
+      if (timing) HIP_TRY(hipEventRecord(ctx->events[3 * k + 1], st));
+      ops.camera_sweep(q, st);
+      if (timing) HIP_TRY(hipEventRecord(ctx->events[3 * k + 2], st));
+      ops.advance(q, st);
+    }
+    HIP_TRY(hipGetLastError());
+    if (rep) rep->matvec_launches += n;
+    done += n;
+    TRY(read_state(ctx, q, st, rep));
+    const uint32_t active = (uint32_t)(ctx->h_state[S_MATVECS] - matvecs_before);
+    matvecs_before = ctx->h_state[S_MATVECS];
+    if (rep) rep->matvecs += active;
+    if (timing) {
+      for (uint32_t k = 0; k < active && k < n; k++) {
+        float a = 0, b = 0;
+        HIP_TRY(hipEventElapsedTime(&a, ctx->events[3 * k], ctx->events[3 * k + 1]));
+        HIP_TRY(hipEventElapsedTime(&b, ctx->events[3 * k + 1], ctx->events[3 * k + 2]));
+        rep->schur_point_ms += a;
+        rep->schur_camera_ms += b;
+        rep->schur_ms += a + b;
+      }
+    }
+    if (ctx->h_state[S_PCG_ACTIVE] == 0) break;
+    chunk = kChunk;
+  }
+  return TCV_BA_OK;
+}
+
 // The LM loop on a prepared problem.
 int run_lm(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, hipStream_t st,
            tcv_ba_report* rep) {
@@ -371,40 +444,19 @@ int run_lm(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, hipSt
     ops.camera_pass(q, st);
     ops.init_cg(q, st);
     HIP_TRY(hipGetLastError());
-    // PCG in chunks with one read of the state words after each. The first chunk is 3/4 of the
-    // previous LM iteration's PCG run, later ones kChunk iterations; a chunk that outlives
-    // convergence costs only guarded no-op launches (state word 19), as in the reference.
-    uint32_t done = 0, chunk = hint;
-    double matvecs_before = ctx->h_state[S_MATVECS];
-    while (done < c->max_cg) {
-      const uint32_t n = chunk < c->max_cg - done ? chunk : c->max_cg - done;
-      for (uint32_t k = 0; k < n; k++) {
-        if (timing) HIP_TRY(hipEventRecord(ctx->events[3 * k], st));
-        ops.point_sweep(q, st);
-        if (timing) HIP_TRY(hipEventRecord(ctx->events[3 * k + 1], st));
-        ops.camera_sweep(q, st);
-        if (timing) HIP_TRY(hipEventRecord(ctx->events[3 * k + 2], st));
-        ops.advance(q, st);
+    TRY(run_pcg(ctx, c, q, ops, st, rep, hint, c->max_cg));
+    // The mixed form: replace the FP32-operator residual by the FP64 one and restart PCG from x
+    // until the FP64 residual meets the refinement tolerance (each restart has its own budget).
+    if (c->precision == TCV_BA_MIXED) {
+      for (uint32_t k = 0; k < kMaxRefinements; k++) {
+        if (!(ctx->h_state[S_CONVERGED] != 0 && ctx->h_state[S_PCG_FAILED] == 0)) break;
+        ops.refine(q, st);
+        HIP_TRY(hipGetLastError());
+        TRY(read_state(ctx, q, st, rep));
+        if (rep) rep->refinement_steps = ctx->h_state[S_REFINEMENTS];
+        if (ctx->h_state[S_PCG_ACTIVE] == 0) break;
+        TRY(run_pcg(ctx, c, q, ops, st, rep, kChunk, c->max_cg));
       }
-      HIP_TRY(hipGetLastError());
-      if (rep) rep->matvec_launches += n;
-      done += n;
-      TRY(read_state(ctx, q, st, rep));
-      const uint32_t active = (uint32_t)(ctx->h_state[S_MATVECS] - matvecs_before);
-      matvecs_before = ctx->h_state[S_MATVECS];
-      if (rep) rep->matvecs += active;
-      if (timing) {
-        for (uint32_t k = 0; k < active && k < n; k++) {
-          float a = 0, b = 0;
-          HIP_TRY(hipEventElapsedTime(&a, ctx->events[3 * k], ctx->events[3 * k + 1]));
-          HIP_TRY(hipEventElapsedTime(&b, ctx->events[3 * k + 1], ctx->events[3 * k + 2]));
-          rep->schur_point_ms += a;
-          rep->schur_camera_ms += b;
-          rep->schur_ms += a + b;
-        }
-      }
-      if (ctx->h_state[S_PCG_ACTIVE] == 0) break;
-      chunk = kChunk;
     }
     const uint32_t steps = (uint32_t)ctx->h_state[S_STEPS];
     hint = steps * 3 / 4 < kChunk ? kChunk : steps * 3 / 4;

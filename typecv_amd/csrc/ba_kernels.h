@@ -35,12 +35,16 @@ enum : int {
   S_FOCAL_INV = 29,      // focal preconditioner 1 / reduced diagonal, or -1 (focal[6C+1])
   S_FOCAL_RHS = 30,      // reduced focal RHS (focal[6C+2])
   S_MATVECS = 31,        // Schur products that ran while PCG was active (whole solve)
-  S_WORDS = 32
+  S_REFINEMENTS = 32,    // mixed form: refinement restarts taken (whole solve)
+  S_WORDS = 40
 };
 
 constexpr int TILE = 64;           // observations per wave tile
 #ifndef TCV_TILE_WAVES
 #define TCV_TILE_WAVES 4
+#endif
+#ifndef TCV_W_BLOCKED
+#define TCV_W_BLOCKED 1
 #endif
 #ifndef TCV_W_NONTEMPORAL
 #define TCV_W_NONTEMPORAL 1
@@ -54,6 +58,7 @@ struct Problem {
   uint32_t C, P, M, m, K, shared, first_directional, anchor, scale_view, normalize_scale;
   uint32_t n_tiles, n_tasks;
   double huber, tol;
+  double refine_tol;  // mixed form: relative FP64 residual of the outer refinement
   double* scene;
   double* trial;
   const uint32_t* index;

@@ -152,9 +152,18 @@ __device__ __forceinline__ void cross_block(const Lin& L, bool shared, double W[
   }
 }
 
-template <class WT>
-__device__ __forceinline__ WT* wrow(void* base, uint32_t k, uint32_t M) {
-  return static_cast<WT*>(base) + (size_t)k * M;
+// Word k of the cross block of observation (or camera slot) i. Blocked layout [M/64][K][64]: the 64
+// observations of a wave read K runs of 64 consecutive words that lie next to each other, one
+// contiguous stretch of HBM per wave instead of K distant streams.
+template <class WT, bool SHARED>
+__device__ __forceinline__ WT* wptr(void* base, uint32_t k, uint32_t i, uint32_t M) {
+  constexpr uint32_t K = SHARED ? 21 : 18;
+#if TCV_W_BLOCKED
+  (void)M;
+  return static_cast<WT*>(base) + ((size_t)(i >> 6) * K + k) * 64 + (i & 63);
+#else
+  return static_cast<WT*>(base) + (size_t)k * M + i;
+#endif
 }
 
 // ------------------------------------------------------------------ index checks
@@ -361,7 +370,7 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_pass(Problem q) {
       } else {
         for (int k = 0; k < 21; k++) W[k] = 0;
       }
-      for (uint32_t k = 0; k < K; k++) wrow<WT>(q.Wp, k, q.M)[i] = (WT)W[k];
+      for (uint32_t k = 0; k < K; k++) *wptr<WT, SHARED>(q.Wp, k, i, q.M) = (WT)W[k];
       double* o = st[lane];
       if (ok) {
         const double w = L.w;
@@ -480,7 +489,7 @@ __global__ __launch_bounds__(TASK_THREADS) void k_camera_pass(Problem q) {
     } else {
       for (int k = 0; k < 21; k++) W[k] = 0;
     }
-    for (uint32_t k = 0; k < K; k++) wrow<WT>(q.Wc, k, q.M)[n] = (WT)W[k];
+    for (uint32_t k = 0; k < K; k++) *wptr<WT, SHARED>(q.Wc, k, n, q.M) = (WT)W[k];
     if (!ok) continue;
     const double w = L.w;
     if (var) {
@@ -738,8 +747,8 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_sweep(Problem q) {
       for (int j = 0; j < 3; j++) {
         double a = 0;
 #pragma unroll
-        for (int k = 0; k < 6; k++) a += wload(wrow<WT>(q.Wp, k * 3 + j, q.M) + i) * vc[k];
-        if (SHARED) a += wload(wrow<WT>(q.Wp, 18 + j, q.M) + i) * vf;
+        for (int k = 0; k < 6; k++) a += wload(wptr<WT, SHARED>(q.Wp, k * 3 + j, i, q.M)) * vc[k];
+        if (SHARED) a += wload(wptr<WT, SHARED>(q.Wp, 18 + j, i, q.M)) * vf;
         tj[j] = a;
       }
       st[lane][0] = tj[0];
@@ -788,9 +797,9 @@ __global__ __launch_bounds__(TASK_THREADS) void k_camera_sweep(Problem q) {
     const double y0 = y[(size_t)p * 3], y1 = y[(size_t)p * 3 + 1], y2 = y[(size_t)p * 3 + 2];
 #pragma unroll
     for (int j = 0; j < NV; j++)
-      acc[j] += wload(wrow<WT>(q.Wc, j * 3, q.M) + n) * y0 +
-                wload(wrow<WT>(q.Wc, j * 3 + 1, q.M) + n) * y1 +
-                wload(wrow<WT>(q.Wc, j * 3 + 2, q.M) + n) * y2;
+      acc[j] += wload(wptr<WT, SHARED>(q.Wc, j * 3, n, q.M)) * y0 +
+                wload(wptr<WT, SHARED>(q.Wc, j * 3 + 1, n, q.M)) * y1 +
+                wload(wptr<WT, SHARED>(q.Wc, j * 3 + 2, n, q.M)) * y2;
   }
 #pragma unroll
   for (int k = 0; k < NV; k++)
@@ -930,6 +939,163 @@ __global__ __launch_bounds__(CG_THREADS) void k_probe(Problem q, const double* v
   if (SHARED && threadIdx.x == 0) {
     zv[q.m - 1] = q.state[S_FOCAL_INV] * v[q.m - 1];
     rhs[q.m - 1] = q.state[S_FOCAL_RHS];
+  }
+}
+
+// ------------------------------------------------------------------ implicit FP64 operator
+
+// The two Schur sweeps with every cross block relinearised from the scene in FP64 instead of read
+// from storage: (W^T v)_j = w (Jp_u[j] (Jc_u . v_c + Jf_u v_f) + Jp_v[j] (Jc_v . v_c + Jf_v v_f))
+// and (W y)_k = w (Jc_u[k] (Jp_u . y) + Jc_v[k] (Jp_v . y)). The mixed form computes its
+// refinement residual b - S x with this operator, so FP64 cross blocks never need storing.
+template <bool SHARED>
+__global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_sweep_implicit(Problem q,
+                                                                          const double* v) {
+  __shared__ double st_all[TILE_WAVES][TILE][3];
+  const uint32_t wave = threadIdx.x / TILE, lane = threadIdx.x % TILE;
+  const uint32_t t = blockIdx.x * TILE_WAVES + wave;
+  if (t >= q.n_tiles) return;
+  auto& st = st_all[wave];
+  const uint32_t fp = q.tiles[t], ep = q.tiles[t + 1];
+  const uint32_t* ps = point_start(q);
+  const uint32_t s = ps[fp], e = ps[ep];
+  const uint32_t my = fp + lane;
+  const bool has = my < ep;
+  const uint32_t mys = has ? ps[my] : 0, mye = has ? ps[my + 1] : 0;
+  const double vf = SHARED ? v[q.m - 1] : 0.0;
+  const double fscale = SHARED ? focal_scale(q.state[S_LOG_FOCAL]) : 1.0;
+  double acc[3] = {0, 0, 0};
+  for (uint32_t base = s; base < e; base += TILE) {
+    const uint32_t i = base + lane;
+    if (i < e) {
+      const uint32_t c = obs_camera(q)[i], p = obs_point(q)[i];
+      Lin L;
+      double tj[3] = {0, 0, 0};
+      if (linearize(q, c, p, i, fscale, variable_camera(q, c), L)) {
+        double a0 = 0, a1 = 0;
+        for (int k = 0; k < 6; k++) {
+          a0 += L.Jc[k] * v[c * 6 + k];
+          a1 += L.Jc[6 + k] * v[c * 6 + k];
+        }
+        if (SHARED) {
+          a0 += L.Jf[0] * vf;
+          a1 += L.Jf[1] * vf;
+        }
+        for (int j = 0; j < 3; j++) tj[j] = L.w * (L.Jp[j] * a0 + L.Jp[3 + j] * a1);
+      }
+      st[lane][0] = tj[0];
+      st[lane][1] = tj[1];
+      st[lane][2] = tj[2];
+    }
+    wave_sync();
+    if (has) {
+      const uint32_t lo = mys > base ? mys : base, hi = mye < base + TILE ? mye : base + TILE;
+      for (uint32_t j = lo; j < hi; j++) {
+        acc[0] += st[j - base][0];
+        acc[1] += st[j - base][1];
+        acc[2] += st[j - base][2];
+      }
+    }
+    wave_sync();
+  }
+  if (!has) return;
+  const double* pb = q.pblk + (size_t)my * 12;
+  double* y = vy(q) + (size_t)my * 3;
+  for (int j = 0; j < 3; j++) y[j] = pb[j * 3] * acc[0] + pb[j * 3 + 1] * acc[1] + pb[j * 3 + 2] * acc[2];
+}
+
+template <bool SHARED>
+__global__ __launch_bounds__(TASK_THREADS) void k_camera_sweep_implicit(Problem q) {
+  constexpr int NV = SHARED ? 7 : 6;
+  __shared__ double red[TASK_THREADS / 64][NV];
+  const uint32_t t = blockIdx.x;
+  const uint32_t c = q.tasks[3 * t], s = q.tasks[3 * t + 1], e = q.tasks[3 * t + 2];
+  const bool var = variable_camera(q, c);
+  const double fscale = SHARED ? focal_scale(q.state[S_LOG_FOCAL]) : 1.0;
+  const double* y = vy(q);
+  double acc[NV];
+  for (int k = 0; k < NV; k++) acc[k] = 0;
+  for (uint32_t n = s + threadIdx.x; n < e; n += TASK_THREADS) {
+    const uint32_t i = camera_obs(q)[n], p = q.cam_point[n];
+    Lin L;
+    if (!linearize(q, c, p, i, fscale, var, L)) continue;
+    const double y0 = y[(size_t)p * 3], y1 = y[(size_t)p * 3 + 1], y2 = y[(size_t)p * 3 + 2];
+    const double u0 = L.w * (L.Jp[0] * y0 + L.Jp[1] * y1 + L.Jp[2] * y2);
+    const double u1 = L.w * (L.Jp[3] * y0 + L.Jp[4] * y1 + L.Jp[5] * y2);
+    for (int k = 0; k < 6; k++) acc[k] += L.Jc[k] * u0 + L.Jc[6 + k] * u1;
+    if (SHARED) acc[6] += L.Jf[0] * u0 + L.Jf[1] * u1;
+  }
+  for (int k = 0; k < NV; k++)
+    for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_xor(acc[k], o, 64);
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+    for (int k = 0; k < NV; k++) red[wave][k] = acc[k];
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double v = 0;
+    for (int w = 0; w < TASK_THREADS / 64; w++) v += red[w][threadIdx.x];
+    q.task_part[(size_t)t * 8 + threadIdx.x] = v;
+  }
+}
+
+// The refinement step of the mixed form, after the implicit sweeps of x: the FP64 residual
+// r = b - S x replaces the recurrence residual. When |r|^2 <= refine_tol^2 |r0|^2 the solve is
+// done (words 22, 24); otherwise PCG restarts from x with z = M^-1 r, p = z (words 18, 19).
+// Runs only after a converged PCG pass (word 24 set, word 20 clear).
+template <bool SHARED>
+__global__ __launch_bounds__(CG_THREADS) void k_refine(Problem q) {
+  __shared__ double lds[CG_THREADS / 64];
+  double* s = q.state;
+  const double converged = s[S_CONVERGED], failed = s[S_PCG_FAILED], r0 = s[S_R0];
+  const double active = s[S_ACTIVE];
+  if (active == 0 || converged == 0 || failed != 0) return;
+  const uint32_t tid = threadIdx.x, n = q.C * 6;
+  double* x = vx(q);
+  double* r = vr(q);
+  double* p = vp(q);
+  double* S = vs(q);
+  double* z = vz(q);
+  (void)schur_finish<SHARED>(q, x, S, lds);
+  __syncthreads();
+  double res = 0;
+  for (uint32_t i = tid; i < n; i += CG_THREADS) {
+    const double ri = q.camRhs[i] - S[i];
+    r[i] = ri;
+    res += ri * ri;
+  }
+  if (SHARED && tid == 0) {
+    const uint32_t f = q.m - 1;
+    r[f] = s[S_FOCAL_RHS] - S[f];
+    res += r[f] * r[f];
+  }
+  res = block_sum(res, lds);
+  if (res <= q.refine_tol * q.refine_tol * r0) {
+    if (tid == 0) s[S_R] = res;
+    return;
+  }
+  double rho = 0;
+  for (uint32_t i = tid; i < n; i += CG_THREADS) {
+    const double zi = block_row(q.camInv, i, r);
+    z[i] = zi;
+    p[i] = zi;
+    rho += zi * r[i];
+  }
+  if (SHARED && tid == 0) {
+    const uint32_t f = q.m - 1;
+    z[f] = s[S_FOCAL_INV] * r[f];
+    p[f] = z[f];
+    rho += z[f] * r[f];
+  }
+  rho = block_sum(rho, lds);
+  if (tid == 0) {
+    s[S_R] = res;
+    s[S_RHO] = rho;
+    s[S_CONVERGED] = 0;
+    s[S_REFINEMENTS] += 1;
+    if (rho > 0)
+      s[S_PCG_ACTIVE] = 1;
+    else
+      s[S_PCG_FAILED] = 1;
   }
 }
 
