@@ -150,6 +150,22 @@ int tcv_ba_solve_device(tcv_ba_context* context, const tcv_ba_control* control,
                         tcv_ba_report* report, void* stream);
 
 /*
+ * Solve `count` independent problems together (host arrays, one set per problem). The problems
+ * become one block-diagonal problem whose every kernel launch covers all of them, while each keeps
+ * its own LM and PCG state (damping, accept/reject, convergence): the same results as `count`
+ * separate tcv_ba_solve calls, to rounding. All controls must share every option except the
+ * sizes, gauge (anchor, scale_view, normalize_scale) and directional_points. `outs[b]` receives
+ * problem b's packBundleResult array. `finish_ms` (optional, `count` entries) receives each
+ * problem's time from the start of the LM loop until its loop ended. The reference solves its
+ * local/windowed bundles one operator call at a time (src/sfm/mapper/mapper.ts:873-908 via
+ * extractLocalBundle, src/sfm/bundle/local-bundle.ts:130-190); this entry has no single
+ * counterpart there and replaces a loop of `bundleAdjustGpu` calls.
+ */
+int tcv_ba_solve_batch(tcv_ba_context* context, uint32_t count, const tcv_ba_control* controls,
+                       const double* const* scenes, const uint32_t* const* indices,
+                       double* const* outs, tcv_ba_report* report, double* finish_ms);
+
+/*
  * One linear system of the LM loop, for checks and benchmarks: linearise the scene at damping
  * `lambda` (and log focal scale `log_focal` on the shared-focal route), assemble the point blocks,
  * camera blocks and block-Jacobi preconditioner, and return

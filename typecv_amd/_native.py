@@ -36,6 +36,7 @@ EXPORTS = (
     "tcv_ba_release",
     "tcv_ba_solve",
     "tcv_ba_solve_device",
+    "tcv_ba_solve_batch",
     "tcv_ba_schur_probe",
 )
 
@@ -130,6 +131,10 @@ def library() -> ctypes.CDLL:
                                         ctypes.c_void_p, ctypes.c_void_p, P(Report),
                                         ctypes.c_void_p]
     lib.tcv_ba_solve_device.restype = ctypes.c_int
+    lib.tcv_ba_solve_batch.argtypes = [ctypes.c_void_p, ctypes.c_uint32, c_control,
+                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                       P(Report), ctypes.c_void_p]
+    lib.tcv_ba_solve_batch.restype = ctypes.c_int
     lib.tcv_ba_schur_probe.argtypes = [ctypes.c_void_p, c_control, ctypes.c_void_p,
                                        ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,

@@ -352,6 +352,33 @@ def solve_packed(device: HipDevice, control: _native.Control, packed: PackedScen
     return out, report.as_dict()
 
 
+def solve_batch(device: HipDevice, controls, packeds):
+    """`tcv_ba_solve_batch`: many independent problems in one batched solve.
+    Returns (list of out arrays, report dict, per-problem LM-loop time in ms)."""
+    lib = _native.library()
+    n = len(controls)
+    if n != len(packeds) or n < 1:
+        raise ValueError("one control per packed problem")
+    arr = (_native.Control * n)(*controls)
+    scenes = [np.ascontiguousarray(p.scene, dtype=np.float64) for p in packeds]
+    indices = [np.ascontiguousarray(p.index, dtype=np.uint32) for p in packeds]
+    outs = [np.empty(lib.tcv_ba_output_size(ctypes.byref(c)), dtype=np.float64) for c in controls]
+    ptr = ctypes.c_void_p
+    scene_ptrs = (ptr * n)(*[s.ctypes.data for s in scenes])
+    index_ptrs = (ptr * n)(*[x.ctypes.data for x in indices])
+    out_ptrs = (ptr * n)(*[o.ctypes.data for o in outs])
+    finish = np.zeros(n)
+    report = _native.Report()
+
+    def job():
+        _native.check(lib.tcv_ba_solve_batch(device.handle, n, arr, scene_ptrs, index_ptrs,
+                                             out_ptrs, ctypes.byref(report), finish.ctypes.data),
+                      "tcv_ba_solve_batch")
+
+    device.enqueue(job)
+    return outs, report.as_dict(), finish
+
+
 def schur_probe(device: HipDevice, control: _native.Control, packed: PackedScene, lam: float,
                 log_focal: float, p: np.ndarray, repeat: int = 1):
     """`tcv_ba_schur_probe`: (rhs, S p, M^-1 p, status, report)."""

@@ -1,15 +1,17 @@
 // ba_engine.hip — the host side of the MI355X bundle-adjustment engine and its C ABI
 // (include/typecv_ba.h). One context per HIP device owns the device arrays (grown, never shrunk,
-// until tcv_ba_release), a stream, and pinned words for the flags the LM loop reads.
+// until tcv_ba_release), a stream, and pinned words for the state the LM loop reads.
 //
 // Host loop (reference bundleAdjustGpuFixed, src/sfm/bundle/gpu/bundle.ts:178-799, and
 // bundleAdjustJointFocalGpu, joint-focal.ts:340-900): initialise and measure the cost; per LM
 // iteration assemble, initialise PCG, run PCG in chunks with one read of the state words after each
 // chunk, build and score the candidate, accept or reject, read the state; pack the result. Every
-// decision is taken on the device from the state words; the host only reads them to stop
-// launching work.
+// decision is taken on the device from each bundle's state words; the host only reads them to
+// stop launching work. A batch of bundles runs in lockstep: every launch covers all bundles and
+// the kernels skip the bundles whose LM or PCG has ended.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -43,17 +45,20 @@ double now_ms() {
 struct tcv_ba_context {
   int device = 0;
   hipStream_t stream = nullptr;
-  double* h_state = nullptr;  // pinned copy of the state words
+  double* h_state = nullptr;  // pinned copy of the state words of every bundle
+  size_t h_state_words = 0;
   uint32_t* h_error = nullptr;
   std::vector<DevBuf*> all;
-  DevBuf scene, trial, index, out, tiles, tasks, task_start, cam_point, Wp, Wc, pblk, tile_fail,
-      tile_focal, task_part, camA, camInv, camRhs, camHcf, camFocal, vec, state, cost_part, error,
-      probe;
+  DevBuf scene, trial, index, out, bundles, tiles, tile_bundle, tasks, task_start, cost_blocks,
+      camera_bundle, point_bundle, cam_point, Wp, Wc, pblk, tile_fail, tile_focal, task_part, camA,
+      camInv, camRhs, camHcf, camFocal, vec, state, cost_part, error, probe;
   std::vector<hipEvent_t> events;
   tcv_ba_context() {
-    all = {&scene, &trial,     &index,      &out,      &tiles,  &tasks,  &task_start, &cam_point,
-           &Wp,    &Wc,        &pblk,       &tile_fail, &tile_focal, &task_part, &camA, &camInv,
-           &camRhs, &camHcf,   &camFocal,   &vec,      &state,  &cost_part, &error, &probe};
+    all = {&scene,      &trial,        &index,     &out,       &bundles,    &tiles,
+           &tile_bundle, &tasks,       &task_start, &cost_blocks, &camera_bundle,
+           &point_bundle, &cam_point,  &Wp,        &Wc,        &pblk,       &tile_fail,
+           &tile_focal, &task_part,    &camA,      &camInv,    &camRhs,     &camHcf,
+           &camFocal,   &vec,          &state,     &cost_part, &error,      &probe};
   }
 };
 
@@ -65,6 +70,12 @@ struct tcv_ba_context {
                 hipGetErrorString(e_) + ")");                                            \
       return e_ == hipErrorOutOfMemory ? TCV_BA_ERR_MEMORY : TCV_BA_ERR_DEVICE;          \
     }                                                                                    \
+  } while (0)
+
+#define TRY(expr)                   \
+  do {                              \
+    int s_ = (expr);                \
+    if (s_ != TCV_BA_OK) return s_; \
   } while (0)
 
 namespace {
@@ -80,11 +91,15 @@ int ensure(DevBuf& b, size_t bytes) {
   return TCV_BA_OK;
 }
 
-#define TRY(expr)               \
-  do {                          \
-    int s_ = (expr);            \
-    if (s_ != TCV_BA_OK) return s_; \
-  } while (0)
+int ensure_host_state(tcv_ba_context* ctx, size_t words) {
+  if (ctx->h_state_words >= words) return TCV_BA_OK;
+  if (ctx->h_state) HIP_TRY(hipHostFree(ctx->h_state));
+  ctx->h_state = nullptr;
+  ctx->h_state_words = 0;
+  HIP_TRY(hipHostMalloc((void**)&ctx->h_state, words * 8, hipHostMallocDefault));
+  ctx->h_state_words = words;
+  return TCV_BA_OK;
+}
 
 // The checks of bundleProblem (src/sfm/bundle/gpu/bundle-host.ts:43-70) that the packed arrays
 // still carry, and the u32 bounds of bundle.ts:118-127.
@@ -109,14 +124,31 @@ int check_control(const tcv_ba_control* c) {
   return TCV_BA_OK;
 }
 
+// The options every bundle of a batch shares.
+bool same_options(const tcv_ba_control& a, const tcv_ba_control& b) {
+  return a.shared_focal == b.shared_focal && a.iterations == b.iterations &&
+         a.max_cg == b.max_cg && a.precision == b.precision && a.flags == b.flags &&
+         a.huber == b.huber && a.relative_tolerance == b.relative_tolerance &&
+         a.stall_tolerance == b.stall_tolerance &&
+         a.refinement_tolerance == b.refinement_tolerance;
+}
+
 struct Plan {
-  std::vector<uint32_t> tiles, tasks, task_start;
+  std::vector<Bundle> bundles;
+  std::vector<uint32_t> tiles, tile_bundle, tasks, task_start, cost_blocks, camera_bundle,
+      point_bundle;
 };
 
-// Wave tiles of whole points (at most 64 observations and 64 points; a longer point alone) and
-// camera-list tasks of at most `task_obs` slots (each camera has at least one task).
-int build_plan(const std::vector<uint32_t>& ps, const std::vector<uint32_t>& cs, uint32_t P,
-               uint32_t C, uint32_t M, uint32_t task_obs, Plan& plan) {
+constexpr uint32_t kTaskObs = 1024;
+constexpr uint32_t kChunk = 8;  // PCG iterations between reads of the state words
+constexpr uint32_t kMaxRefinements = 4;  // mixed form: refinement restarts per LM iteration
+
+// Bundle ranges, wave tiles of whole points of one bundle (at most 64 observations and 64 points;
+// a longer point alone), camera-list tasks of at most kTaskObs slots (each camera has at least
+// one), cost blocks of at most COST_OBS observations of one bundle.
+int build_plan(const std::vector<uint32_t>& ps, const std::vector<uint32_t>& cs,
+               const tcv_ba_control* controls, uint32_t B, uint32_t C, uint32_t P, uint32_t M,
+               Plan& plan) {
   if (ps[0] != 0 || ps[P] != M || cs[0] != 0 || cs[C] != M) {
     set_error("index offsets do not cover the observations");
     return TCV_BA_ERR_INDEX;
@@ -131,33 +163,80 @@ int build_plan(const std::vector<uint32_t>& ps, const std::vector<uint32_t>& cs,
       set_error("camera offsets decrease");
       return TCV_BA_ERR_INDEX;
     }
-  plan.tiles.clear();
+  plan = Plan{};
   plan.tiles.push_back(0);
-  uint32_t first = 0;
-  for (uint32_t p = 1; p < P; p++) {
-    const bool too_many_obs = ps[p + 1] - ps[first] > (uint32_t)TILE;
-    const bool too_many_points = p - first >= (uint32_t)TILE;
-    if (too_many_obs || too_many_points) {
-      plan.tiles.push_back(p);
-      first = p;
-    }
-  }
-  plan.tiles.push_back(P);
-  plan.tasks.clear();
   plan.task_start.assign(C + 1, 0);
-  for (uint32_t c = 0; c < C; c++) {
-    plan.task_start[c] = (uint32_t)(plan.tasks.size() / 3);
-    const uint32_t s = cs[c], e = cs[c + 1];
-    uint32_t at = s;
-    do {
-      const uint32_t end = e - at > task_obs ? at + task_obs : e;
-      plan.tasks.push_back(c);
-      plan.tasks.push_back(at);
-      plan.tasks.push_back(end);
-      at = end;
-    } while (at < e);
+  plan.camera_bundle.resize(C);
+  plan.point_bundle.resize(P);
+  uint32_t cam = 0, pt = 0, obs = 0;
+  for (uint32_t b = 0; b < B; b++) {
+    const tcv_ba_control& k = controls[b];
+    Bundle g{};
+    g.cam0 = cam;
+    g.cam1 = cam + k.cameras;
+    g.pt0 = pt;
+    g.pt1 = pt + k.points;
+    g.obs0 = obs;
+    g.obs1 = obs + k.observations;
+    if (ps[g.pt0] != g.obs0 || ps[g.pt1] != g.obs1 || cs[g.cam0] != g.obs0 || cs[g.cam1] != g.obs1) {
+      set_error("bundle offsets do not match the bundle sizes");
+      return TCV_BA_ERR_INDEX;
+    }
+    g.first_directional = g.pt1 - k.directional_points;
+    g.anchor = cam + k.anchor;
+    g.scale_view = cam + k.scale_view;
+    g.normalize_scale = k.normalize_scale;
+    g.focal_row = 6 * C + b;
+    // tiles of whole points of this bundle
+    g.tile0 = (uint32_t)plan.tiles.size() - 1;
+    uint32_t first = g.pt0;
+    for (uint32_t p = g.pt0 + 1; p < g.pt1; p++) {
+      const bool too_many_obs = ps[p + 1] - ps[first] > (uint32_t)TILE;
+      const bool too_many_points = p - first >= (uint32_t)TILE;
+      if (too_many_obs || too_many_points) {
+        plan.tiles.push_back(p);
+        first = p;
+      }
+    }
+    plan.tiles.push_back(g.pt1);
+    g.tile1 = (uint32_t)plan.tiles.size() - 1;
+    while (plan.tile_bundle.size() < g.tile1) plan.tile_bundle.push_back(b);
+    // camera tasks
+    g.task0 = (uint32_t)(plan.tasks.size() / 4);
+    for (uint32_t c = g.cam0; c < g.cam1; c++) {
+      plan.task_start[c] = (uint32_t)(plan.tasks.size() / 4);
+      plan.camera_bundle[c] = b;
+      uint32_t at = cs[c];
+      const uint32_t e = cs[c + 1];
+      do {
+        const uint32_t end = e - at > kTaskObs ? at + kTaskObs : e;
+        plan.tasks.push_back(c);
+        plan.tasks.push_back(at);
+        plan.tasks.push_back(end);
+        plan.tasks.push_back(b);
+        at = end;
+      } while (at < e);
+    }
+    g.task1 = (uint32_t)(plan.tasks.size() / 4);
+    for (uint32_t p = g.pt0; p < g.pt1; p++) plan.point_bundle[p] = b;
+    // cost blocks
+    g.cost0 = (uint32_t)(plan.cost_blocks.size() / 3);
+    for (uint32_t at = g.obs0; at < g.obs1; at += COST_OBS) {
+      plan.cost_blocks.push_back(b);
+      plan.cost_blocks.push_back(at);
+      plan.cost_blocks.push_back(std::min<uint32_t>(at + COST_OBS, g.obs1));
+    }
+    g.cost1 = (uint32_t)(plan.cost_blocks.size() / 3);
+    plan.bundles.push_back(g);
+    cam = g.cam1;
+    pt = g.pt1;
+    obs = g.obs1;
   }
-  plan.task_start[C] = (uint32_t)(plan.tasks.size() / 3);
+  plan.task_start[C] = (uint32_t)(plan.tasks.size() / 4);
+  if (cam != C || pt != P || obs != M) {
+    set_error("bundle sizes do not add up to the batch");
+    return TCV_BA_ERR_INDEX;
+  }
   return TCV_BA_OK;
 }
 
@@ -176,7 +255,7 @@ struct Launch {
     hipLaunchKernelGGL((k_camera_finish<SHARED>), dim3(blocks_for(q.C, 64)), dim3(64), 0, s, q);
   }
   static void init_cg(const Problem& q, hipStream_t s) {
-    hipLaunchKernelGGL((k_init_cg<SHARED>), dim3(1), dim3(CG_THREADS), 0, s, q);
+    hipLaunchKernelGGL((k_init_cg<SHARED>), dim3(q.B), dim3(CG_THREADS), 0, s, q);
   }
   static void point_sweep(const Problem& q, hipStream_t s) {
     hipLaunchKernelGGL((k_point_sweep<WT, SHARED, SWEEP_SCHUR>), dim3(tile_blocks(q)),
@@ -186,7 +265,7 @@ struct Launch {
     hipLaunchKernelGGL((k_camera_sweep<WT, SHARED>), dim3(q.n_tasks), dim3(TASK_THREADS), 0, s, q);
   }
   static void advance(const Problem& q, hipStream_t s) {
-    hipLaunchKernelGGL((k_cg_advance<SHARED>), dim3(1), dim3(CG_THREADS), 0, s, q);
+    hipLaunchKernelGGL((k_cg_advance<SHARED>), dim3(q.B), dim3(CG_THREADS), 0, s, q);
   }
   static void update_points(const Problem& q, hipStream_t s) {
     hipLaunchKernelGGL((k_point_sweep<WT, SHARED, SWEEP_UPDATE>), dim3(tile_blocks(q)),
@@ -196,22 +275,22 @@ struct Launch {
                     hipStream_t s) {
     hipLaunchKernelGGL((k_probe<SHARED>), dim3(1), dim3(CG_THREADS), 0, s, q, v, sv, zv, rhs);
   }
-  // The implicit FP64 operator: sweeps of the vector at q.vec + offset (x: 0, p: 2m).
-  static void implicit_point_sweep(const Problem& q, size_t offset, hipStream_t s) {
-    hipLaunchKernelGGL((k_point_sweep_implicit<SHARED>), dim3(tile_blocks(q)),
-                       dim3(TILE * TILE_WAVES), 0, s, q, (const double*)(q.vec + offset));
-  }
+  // The implicit FP64 operator on p (PCG, guarded per bundle) and the refinement step on x.
   static void implicit_p_sweep(const Problem& q, hipStream_t s) {
-    implicit_point_sweep(q, 2 * (size_t)q.m, s);
+    hipLaunchKernelGGL((k_point_sweep_implicit<SHARED>), dim3(tile_blocks(q)),
+                       dim3(TILE * TILE_WAVES), 0, s, q, (const double*)(q.vec + 2 * (size_t)q.m),
+                       1);
   }
   static void implicit_camera_sweep(const Problem& q, hipStream_t s) {
     hipLaunchKernelGGL((k_camera_sweep_implicit<SHARED>), dim3(q.n_tasks), dim3(TASK_THREADS), 0,
-                       s, q);
+                       s, q, 1);
   }
   static void refine(const Problem& q, hipStream_t s) {
-    implicit_point_sweep(q, 0, s);
-    implicit_camera_sweep(q, s);
-    hipLaunchKernelGGL((k_refine<SHARED>), dim3(1), dim3(CG_THREADS), 0, s, q);
+    hipLaunchKernelGGL((k_point_sweep_implicit<SHARED>), dim3(tile_blocks(q)),
+                       dim3(TILE * TILE_WAVES), 0, s, q, (const double*)q.vec, 0);
+    hipLaunchKernelGGL((k_camera_sweep_implicit<SHARED>), dim3(q.n_tasks), dim3(TASK_THREADS), 0,
+                       s, q, 0);
+    hipLaunchKernelGGL((k_refine<SHARED>), dim3(q.B), dim3(CG_THREADS), 0, s, q);
   }
 };
 
@@ -230,8 +309,8 @@ struct Ops {
 template <class WT, bool SHARED>
 Ops ops_for(bool implicit) {
   using L = Launch<WT, SHARED>;
-  Ops o{L::point_pass,   L::camera_pass, L::init_cg,       L::point_sweep, L::camera_sweep,
-        L::advance,      L::update_points, L::probe,       L::refine};
+  Ops o{L::point_pass, L::camera_pass,   L::init_cg, L::point_sweep, L::camera_sweep,
+        L::advance,    L::update_points, L::probe,   L::refine};
   if (implicit) {
     o.point_sweep = L::implicit_p_sweep;
     o.camera_sweep = L::implicit_camera_sweep;
@@ -247,40 +326,58 @@ Ops select_ops(const tcv_ba_control* c) {
   return shared ? ops_for<double, true>(implicit) : ops_for<double, false>(implicit);
 }
 
-constexpr uint32_t kTaskObs = 1024;
-constexpr uint32_t kChunk = 8;  // PCG iterations between reads of the state words
-constexpr uint32_t kMaxRefinements = 4;  // mixed form: refinement restarts per LM iteration
+template <class T>
+int upload(DevBuf& buf, const std::vector<T>& v, hipStream_t st) {
+  TRY(ensure(buf, v.size() * sizeof(T)));
+  if (!v.empty())
+    HIP_TRY(hipMemcpyAsync(buf.ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+  return TCV_BA_OK;
+}
 
-// Allocate, validate the index and build the plan. `scene`/`index` are device pointers.
-int setup(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene, const uint32_t* index,
-          hipStream_t st, Problem& q, tcv_ba_report* rep) {
+// Allocate, validate the index and build the plan of a batch of B bundles whose packed arrays
+// `scene` / `index` (device pointers) hold the whole batch.
+int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const double* scene,
+          const uint32_t* index, hipStream_t st, Problem& q, tcv_ba_report* rep) {
   const double t0 = now_ms();
-  TRY(check_control(c));
-  const uint32_t C = c->cameras, P = c->points, M = c->observations;
+  if (B < 1) return TCV_BA_ERR_ARGUMENT;
+  uint64_t C64 = 0, P64 = 0, M64 = 0;
+  for (uint32_t b = 0; b < B; b++) {
+    TRY(check_control(&controls[b]));
+    if (!same_options(controls[0], controls[b])) {
+      set_error("the bundles of a batch must share every option but their sizes and gauge");
+      return TCV_BA_ERR_ARGUMENT;
+    }
+    C64 += controls[b].cameras;
+    P64 += controls[b].points;
+    M64 += controls[b].observations;
+  }
+  if (3 * M64 + P64 + 2 * C64 + 2 > 0xffffffffull || 18 * C64 + 3 * P64 + 2 * M64 > 0xffffffffull) {
+    set_error("batch exceeds the u32 index bounds");
+    return TCV_BA_ERR_ARGUMENT;
+  }
+  const tcv_ba_control* c = &controls[0];
+  const uint32_t C = (uint32_t)C64, P = (uint32_t)P64, M = (uint32_t)M64;
   const uint32_t shared = c->shared_focal ? 1u : 0u;
   const size_t scene_size = (size_t)C * 18 + (size_t)P * 3 + (size_t)M * 2;
   const size_t trial_size = (size_t)C * 12 + (size_t)P * 3;
   const size_t ws = c->precision == TCV_BA_MIXED ? 4 : 8;
   const uint32_t K = shared ? 21 : 18;
-  const uint32_t m = C * 6 + shared;
-  // offsets of both sections, read back for the plan
+  const uint32_t m = C * 6 + shared * B;
   std::vector<uint32_t> ps(P + 1), cs(C + 1);
   HIP_TRY(hipMemcpyAsync(ps.data(), index + 2 * (size_t)M, (P + 1) * 4, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipMemcpyAsync(cs.data(), index + 2 * (size_t)M + P + 1, (C + 1) * 4,
                          hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
   Plan plan;
-  TRY(build_plan(ps, cs, P, C, M, kTaskObs, plan));
+  TRY(build_plan(ps, cs, controls, B, C, P, M, plan));
   const uint32_t n_tiles = (uint32_t)plan.tiles.size() - 1;
-  const uint32_t n_tasks = (uint32_t)plan.tasks.size() / 3;
-  const uint32_t n_cost = blocks_for(M, 256);
+  const uint32_t n_tasks = (uint32_t)plan.tasks.size() / 4;
+  const uint32_t n_cost = (uint32_t)plan.cost_blocks.size() / 3;
+  const size_t w_words = (size_t)K * (((size_t)M + 63) / 64) * 64;
+  const size_t vec_words = (size_t)m * 5 + (size_t)P * 3;
   TRY(ensure(ctx->scene, scene_size * 8));
   TRY(ensure(ctx->trial, trial_size * 8));
-  TRY(ensure(ctx->tiles, plan.tiles.size() * 4));
-  TRY(ensure(ctx->tasks, plan.tasks.size() * 4));
-  TRY(ensure(ctx->task_start, plan.task_start.size() * 4));
   TRY(ensure(ctx->cam_point, (size_t)M * 4));
-  const size_t w_words = (size_t)K * (((size_t)M + 63) / 64) * 64;
   TRY(ensure(ctx->Wp, w_words * ws));
   TRY(ensure(ctx->Wc, w_words * ws));
   TRY(ensure(ctx->pblk, (size_t)P * 12 * 8));
@@ -292,42 +389,49 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene, con
   TRY(ensure(ctx->camRhs, (size_t)C * 6 * 8));
   TRY(ensure(ctx->camHcf, (size_t)C * 6 * 8));
   TRY(ensure(ctx->camFocal, (size_t)C * 2 * 8));
-  TRY(ensure(ctx->vec, ((size_t)m * 5 + (size_t)P * 3 + C) * 8));
-  TRY(ensure(ctx->state, S_WORDS * 8));
+  TRY(ensure(ctx->vec, vec_words * 8));
+  TRY(ensure(ctx->state, (size_t)B * S_WORDS * 8));
   TRY(ensure(ctx->cost_part, (size_t)n_cost * 3 * 8));
   TRY(ensure(ctx->error, 8));
-  HIP_TRY(hipMemcpyAsync(ctx->tiles.ptr, plan.tiles.data(), plan.tiles.size() * 4,
-                         hipMemcpyHostToDevice, st));
-  HIP_TRY(hipMemcpyAsync(ctx->tasks.ptr, plan.tasks.data(), plan.tasks.size() * 4,
-                         hipMemcpyHostToDevice, st));
-  HIP_TRY(hipMemcpyAsync(ctx->task_start.ptr, plan.task_start.data(), plan.task_start.size() * 4,
-                         hipMemcpyHostToDevice, st));
+  TRY(ensure_host_state(ctx, (size_t)B * S_WORDS));
+  TRY(upload(ctx->bundles, plan.bundles, st));
+  TRY(upload(ctx->tiles, plan.tiles, st));
+  TRY(upload(ctx->tile_bundle, plan.tile_bundle, st));
+  TRY(upload(ctx->tasks, plan.tasks, st));
+  TRY(upload(ctx->task_start, plan.task_start, st));
+  TRY(upload(ctx->cost_blocks, plan.cost_blocks, st));
+  TRY(upload(ctx->camera_bundle, plan.camera_bundle, st));
+  TRY(upload(ctx->point_bundle, plan.point_bundle, st));
   HIP_TRY(hipMemcpyAsync(ctx->scene.ptr, scene, scene_size * 8, hipMemcpyDeviceToDevice, st));
   HIP_TRY(hipMemsetAsync(ctx->trial.ptr, 0, trial_size * 8, st));
-  HIP_TRY(hipMemsetAsync(ctx->vec.ptr, 0, ((size_t)m * 5 + (size_t)P * 3 + C) * 8, st));
+  HIP_TRY(hipMemsetAsync(ctx->vec.ptr, 0, vec_words * 8, st));
+  HIP_TRY(hipMemsetAsync(ctx->state.ptr, 0, (size_t)B * S_WORDS * 8, st));
   HIP_TRY(hipMemsetAsync(ctx->error.ptr, 0, 8, st));
   q = Problem{};
   q.C = C;
   q.P = P;
   q.M = M;
+  q.B = B;
   q.m = m;
   q.K = K;
   q.shared = shared;
-  q.first_directional = P - c->directional_points;
-  q.anchor = c->anchor;
-  q.scale_view = c->scale_view;
-  q.normalize_scale = c->normalize_scale;
   q.n_tiles = n_tiles;
   q.n_tasks = n_tasks;
+  q.n_cost_blocks = n_cost;
   q.huber = c->huber;
   q.tol = c->relative_tolerance;
   q.refine_tol = c->refinement_tolerance > 0 ? c->refinement_tolerance : c->relative_tolerance;
   q.scene = (double*)ctx->scene.ptr;
   q.trial = (double*)ctx->trial.ptr;
   q.index = index;
+  q.bundles = (const Bundle*)ctx->bundles.ptr;
   q.tiles = (const uint32_t*)ctx->tiles.ptr;
+  q.tile_bundle = (const uint32_t*)ctx->tile_bundle.ptr;
   q.tasks = (const uint32_t*)ctx->tasks.ptr;
   q.task_start = (const uint32_t*)ctx->task_start.ptr;
+  q.cost_blocks = (const uint32_t*)ctx->cost_blocks.ptr;
+  q.camera_bundle = (const uint32_t*)ctx->camera_bundle.ptr;
+  q.point_bundle = (const uint32_t*)ctx->point_bundle.ptr;
   q.cam_point = (uint32_t*)ctx->cam_point.ptr;
   q.Wp = ctx->Wp.ptr;
   q.Wc = ctx->Wc.ptr;
@@ -344,7 +448,6 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene, con
   q.state = (double*)ctx->state.ptr;
   q.cost_part = (double*)ctx->cost_part.ptr;
   q.error = (uint32_t*)ctx->error.ptr;
-  q.n_cost_blocks = n_cost;
   hipLaunchKernelGGL(k_check_points, dim3(blocks_for(P, 256)), dim3(256), 0, st, q);
   hipLaunchKernelGGL(k_check_tasks, dim3(n_tasks), dim3(256), 0, st, q);
   HIP_TRY(hipGetLastError());
@@ -364,10 +467,27 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene, con
 }
 
 int read_state(tcv_ba_context* ctx, const Problem& q, hipStream_t st, tcv_ba_report* rep) {
-  HIP_TRY(hipMemcpyAsync(ctx->h_state, q.state, S_WORDS * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(ctx->h_state, q.state, (size_t)q.B * S_WORDS * 8, hipMemcpyDeviceToHost,
+                         st));
   HIP_TRY(hipStreamSynchronize(st));
   if (rep) rep->flag_reads++;
   return TCV_BA_OK;
+}
+
+double word(const tcv_ba_context* ctx, uint32_t b, int w) {
+  return ctx->h_state[(size_t)b * S_WORDS + w];
+}
+
+bool any_word(const tcv_ba_context* ctx, const Problem& q, int w) {
+  for (uint32_t b = 0; b < q.B; b++)
+    if (word(ctx, b, w) != 0) return true;
+  return false;
+}
+
+double max_word(const tcv_ba_context* ctx, const Problem& q, int w) {
+  double v = 0;
+  for (uint32_t b = 0; b < q.B; b++) v = std::max(v, word(ctx, b, w));
+  return v;
 }
 
 int ensure_events(tcv_ba_context* ctx, size_t n) {
@@ -382,12 +502,12 @@ int ensure_events(tcv_ba_context* ctx, size_t n) {
 // PCG in chunks with one read of the state words after each: the first chunk `first` iterations,
 // later ones kChunk, at most `budget` in all. A chunk that outlives convergence costs only guarded
 // no-op launches (state word 19), as the reference's chunks do. With TCV_BA_FLAG_TIMING the two
-// sweeps of every product that ran while PCG was active are timed with HIP events.
+// sweeps of every launch in which some bundle's PCG was active are timed with HIP events.
 int run_pcg(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, const Ops& ops,
             hipStream_t st, tcv_ba_report* rep, uint32_t first, uint32_t budget) {
   const bool timing = (c->flags & TCV_BA_FLAG_TIMING) != 0 && rep;
   uint32_t done = 0, chunk = first;
-  double matvecs_before = ctx->h_state[S_MATVECS];
+  double matvecs_before = max_word(ctx, q, S_MATVECS);
   while (done < budget) {
     const uint32_t n = chunk < budget - done ? chunk : budget - done;
     for (uint32_t k = 0; k < n; k++) {
@@ -402,8 +522,9 @@ int run_pcg(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, cons
     if (rep) rep->matvec_launches += n;
     done += n;
     TRY(read_state(ctx, q, st, rep));
-    const uint32_t active = (uint32_t)(ctx->h_state[S_MATVECS] - matvecs_before);
-    matvecs_before = ctx->h_state[S_MATVECS];
+    const double now = max_word(ctx, q, S_MATVECS);
+    const uint32_t active = (uint32_t)(now - matvecs_before);
+    matvecs_before = now;
     if (rep) rep->matvecs += active;
     if (timing) {
       for (uint32_t k = 0; k < active && k < n; k++) {
@@ -415,31 +536,49 @@ int run_pcg(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, cons
         rep->schur_ms += a + b;
       }
     }
-    if (ctx->h_state[S_PCG_ACTIVE] == 0) break;
+    if (!any_word(ctx, q, S_PCG_ACTIVE)) break;
     chunk = kChunk;
   }
   return TCV_BA_OK;
 }
 
-// The LM loop on a prepared problem.
+// Ends the PCG of every bundle whose budget ran out (the reference stops at maxCgIterations);
+// its candidate is then rejected by the gauge's convergence check, as in the reference.
+__global__ void k_stop_pcg(Problem q) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < q.B) q.state[(size_t)b * S_WORDS + S_PCG_ACTIVE] = 0;
+}
+
+// The LM loop on a prepared batch. `finish_ms` (optional, B entries) receives each bundle's time
+// from the start of the loop until its LM loop ended.
 int run_lm(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, hipStream_t st,
-           tcv_ba_report* rep) {
+           tcv_ba_report* rep, double* finish_ms) {
   const Ops ops = select_ops(c);
   const bool timing = (c->flags & TCV_BA_FLAG_TIMING) != 0 && rep;
-  hipLaunchKernelGGL(k_init_bundle, dim3(1), dim3(64), 0, st, q);
-  hipLaunchKernelGGL(k_cost<false>, dim3(q.n_cost_blocks), dim3(256), 0, st, q);
-  hipLaunchKernelGGL(k_finish_cost<false>, dim3(1), dim3(CG_THREADS), 0, st, q);
+  const double t0 = now_ms();
+  hipLaunchKernelGGL(k_init_bundle, dim3(q.B), dim3(64), 0, st, q);
+  hipLaunchKernelGGL(k_cost<false>, dim3(q.n_cost_blocks), dim3(COST_OBS), 0, st, q);
+  hipLaunchKernelGGL(k_finish_cost<false>, dim3(q.B), dim3(COST_OBS), 0, st, q);
   HIP_TRY(hipGetLastError());
   TRY(read_state(ctx, q, st, rep));
-  bool lm_active = ctx->h_state[S_ACTIVE] != 0;
-  double monitor_cost = ctx->h_state[S_COST];
   const bool monitor = c->stall_tolerance > 1e-9;
+  std::vector<double> monitor_cost(q.B), finish(q.B, -1.0);
+  std::vector<char> active(q.B);
+  for (uint32_t b = 0; b < q.B; b++) {
+    monitor_cost[b] = word(ctx, b, S_COST);
+    active[b] = word(ctx, b, S_ACTIVE) != 0;
+    if (!active[b]) finish[b] = now_ms() - t0;
+  }
   uint32_t hint = 16;
   const uint32_t update_blocks = blocks_for(q.C > q.P ? q.C : q.P, 256);
   const uint32_t trial_blocks = blocks_for((size_t)q.C * 12 + (size_t)q.P * 3, 256);
   if (timing) TRY(ensure_events(ctx, 3 * 256 + 3));
-  for (uint32_t iteration = 0; iteration < c->iterations && lm_active; iteration++) {
-    if (rep) rep->lm_iterations++;
+  auto any_active = [&]() {
+    for (uint32_t b = 0; b < q.B; b++)
+      if (active[b]) return true;
+    return false;
+  };
+  for (uint32_t iteration = 0; iteration < c->iterations && any_active(); iteration++) {
     ops.point_pass(q, st);
     ops.camera_pass(q, st);
     ops.init_cg(q, st);
@@ -448,38 +587,72 @@ int run_lm(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, hipSt
     // The mixed form: replace the FP32-operator residual by the FP64 one and restart PCG from x
     // until the FP64 residual meets the refinement tolerance (each restart has its own budget).
     if (c->precision == TCV_BA_MIXED) {
+      if (any_word(ctx, q, S_PCG_ACTIVE))
+        hipLaunchKernelGGL(k_stop_pcg, dim3(blocks_for(q.B, 64)), dim3(64), 0, st, q);
       for (uint32_t k = 0; k < kMaxRefinements; k++) {
-        if (!(ctx->h_state[S_CONVERGED] != 0 && ctx->h_state[S_PCG_FAILED] == 0)) break;
+        bool pending = false;
+        for (uint32_t b = 0; b < q.B; b++)
+          pending |= word(ctx, b, S_CONVERGED) != 0 && word(ctx, b, S_PCG_FAILED) == 0 &&
+                     word(ctx, b, S_ACTIVE) != 0;
+        if (!pending) break;
         ops.refine(q, st);
         HIP_TRY(hipGetLastError());
         TRY(read_state(ctx, q, st, rep));
-        if (rep) rep->refinement_steps = ctx->h_state[S_REFINEMENTS];
-        if (ctx->h_state[S_PCG_ACTIVE] == 0) break;
+        if (!any_word(ctx, q, S_PCG_ACTIVE)) break;
         TRY(run_pcg(ctx, c, q, ops, st, rep, kChunk, c->max_cg));
+        if (any_word(ctx, q, S_PCG_ACTIVE))
+          hipLaunchKernelGGL(k_stop_pcg, dim3(blocks_for(q.B, 64)), dim3(64), 0, st, q);
+      }
+      if (rep) {
+        double total = 0;
+        for (uint32_t b = 0; b < q.B; b++) total += word(ctx, b, S_REFINEMENTS);
+        rep->refinement_steps = total;
       }
     }
-    const uint32_t steps = (uint32_t)ctx->h_state[S_STEPS];
+    const uint32_t steps = (uint32_t)max_word(ctx, q, S_STEPS);
     hint = steps * 3 / 4 < kChunk ? kChunk : steps * 3 / 4;
     hipLaunchKernelGGL(k_update_cameras, dim3(blocks_for(q.C, 64)), dim3(64), 0, st, q);
     ops.update_points(q, st);
-    hipLaunchKernelGGL(k_gauge, dim3(1), dim3(256), 0, st, q);
+    hipLaunchKernelGGL(k_gauge, dim3(q.B), dim3(256), 0, st, q);
     hipLaunchKernelGGL(k_normalize, dim3(update_blocks), dim3(256), 0, st, q);
-    hipLaunchKernelGGL(k_cost<true>, dim3(q.n_cost_blocks), dim3(256), 0, st, q);
-    hipLaunchKernelGGL(k_finish_cost<true>, dim3(1), dim3(CG_THREADS), 0, st, q);
+    hipLaunchKernelGGL(k_cost<true>, dim3(q.n_cost_blocks), dim3(COST_OBS), 0, st, q);
+    hipLaunchKernelGGL(k_finish_cost<true>, dim3(q.B), dim3(COST_OBS), 0, st, q);
     hipLaunchKernelGGL(k_commit, dim3(trial_blocks), dim3(256), 0, st, q);
     HIP_TRY(hipGetLastError());
     TRY(read_state(ctx, q, st, rep));
-    lm_active = ctx->h_state[S_ACTIVE] != 0;
-    if (monitor && lm_active) {
-      const double next = ctx->h_state[S_COST], change = monitor_cost - next;
-      const bool accepted = ctx->h_state[S_COMMIT] == 1;
-      monitor_cost = next;
-      if (accepted && change < c->stall_tolerance * (next > 1 ? next : 1)) lm_active = false;
+    for (uint32_t b = 0; b < q.B; b++) {
+      if (!active[b]) continue;
+      bool on = word(ctx, b, S_ACTIVE) != 0;
+      if (monitor && on) {
+        // the host stall rule of bundleStallMonitor (bundle-shared.ts:83-123)
+        const double next = word(ctx, b, S_COST), change = monitor_cost[b] - next;
+        const bool accepted = word(ctx, b, S_COMMIT) == 1;
+        monitor_cost[b] = next;
+        if (accepted && change < c->stall_tolerance * (next > 1 ? next : 1)) {
+          const double zero = 0;
+          HIP_TRY(hipMemcpyAsync(q.state + (size_t)b * S_WORDS + S_ACTIVE, &zero, 8,
+                                 hipMemcpyHostToDevice, st));
+          on = false;
+        }
+      }
+      if (!on) {
+        active[b] = 0;
+        finish[b] = now_ms() - t0;
+      }
     }
   }
+  HIP_TRY(hipStreamSynchronize(st));
+  const double end = now_ms() - t0;
+  if (finish_ms)
+    for (uint32_t b = 0; b < q.B; b++) finish_ms[b] = finish[b] < 0 ? end : finish[b];
   if (rep) {
-    rep->cg_iterations = (uint64_t)ctx->h_state[S_CG_STEPS];
-    rep->lm_iterations = (uint64_t)(ctx->h_state[S_ACCEPTED] + ctx->h_state[S_REJECTED]);
+    double cg = 0, lm = 0;
+    for (uint32_t b = 0; b < q.B; b++) {
+      cg += word(ctx, b, S_CG_STEPS);
+      lm += word(ctx, b, S_ACCEPTED) + word(ctx, b, S_REJECTED);
+    }
+    rep->cg_iterations = (uint64_t)cg;
+    rep->lm_iterations = (uint64_t)lm;
   }
   return TCV_BA_OK;
 }
@@ -490,14 +663,17 @@ uint64_t held_bytes(const tcv_ba_context* ctx) {
   return total;
 }
 
-int solve_device(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene,
-                 const uint32_t* index, double* out, tcv_ba_report* rep, hipStream_t st) {
+// Solve a batch whose packed arrays are on the device; `out` receives the batch's coordinates
+// then (26 + shared) state words per bundle.
+int solve_device(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B,
+                 const double* scene, const uint32_t* index, double* out, tcv_ba_report* rep,
+                 double* finish_ms, hipStream_t st) {
   const double t0 = now_ms();
   HIP_TRY(hipSetDevice(ctx->device));
   Problem q;
-  TRY(setup(ctx, c, scene, index, st, q, rep));
-  TRY(run_lm(ctx, c, q, st, rep));
-  const size_t total = (size_t)q.C * 12 + (size_t)q.P * 3 + 27;
+  TRY(setup(ctx, controls, B, scene, index, st, q, rep));
+  TRY(run_lm(ctx, &controls[0], q, st, rep, finish_ms));
+  const size_t total = (size_t)q.C * 12 + (size_t)q.P * 3 + (size_t)B * (26 + q.shared);
   hipLaunchKernelGGL(k_pack, dim3(blocks_for(total, 256)), dim3(256), 0, st, q, out);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(st));
@@ -557,10 +733,13 @@ int tcv_ba_create(int device, tcv_ba_context** out) {
   tcv_ba_context* ctx = new tcv_ba_context();
   ctx->device = device;
   hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipHostMalloc((void**)&ctx->h_state, S_WORDS * 8, hipHostMallocDefault);
   if (e == hipSuccess) e = hipHostMalloc((void**)&ctx->h_error, 8, hipHostMallocDefault);
   if (e != hipSuccess) {
     set_error(std::string("context set-up: ") + hipGetErrorName(e));
+    tcv_ba_destroy(ctx);
+    return TCV_BA_ERR_DEVICE;
+  }
+  if (ensure_host_state(ctx, S_WORDS) != TCV_BA_OK) {
     tcv_ba_destroy(ctx);
     return TCV_BA_ERR_DEVICE;
   }
@@ -598,7 +777,7 @@ int tcv_ba_solve_device(tcv_ba_context* ctx, const tcv_ba_control* c, const doub
   if (!ctx || !c || !scene || !index || !out) return TCV_BA_ERR_ARGUMENT;
   if (rep) std::memset(rep, 0, sizeof *rep);
   hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
-  return solve_device(ctx, c, scene, index, out, rep, st);
+  return solve_device(ctx, c, 1, scene, index, out, rep, nullptr, st);
 }
 
 int tcv_ba_solve(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene,
@@ -606,22 +785,103 @@ int tcv_ba_solve(tcv_ba_context* ctx, const tcv_ba_control* c, const double* sce
   if (!ctx || !c || !scene || !index || !out) return TCV_BA_ERR_ARGUMENT;
   if (rep) std::memset(rep, 0, sizeof *rep);
   TRY(check_control(c));
-  const double t0 = now_ms();
   HIP_TRY(hipSetDevice(ctx->device));
   const size_t ss = tcv_ba_scene_size(c), is = tcv_ba_index_size(c), os = tcv_ba_output_size(c);
-  // The inputs live in the probe/out/index buffers of the context: scene staging shares `probe`.
-  TRY(ensure(ctx->probe, ss * 8));
+  TRY(ensure(ctx->probe, ss * 8));  // staging of the host scene
   TRY(ensure(ctx->index, is * 4));
   TRY(ensure(ctx->out, os * 8));
   hipStream_t st = ctx->stream;
   HIP_TRY(hipMemcpyAsync(ctx->probe.ptr, scene, ss * 8, hipMemcpyHostToDevice, st));
   HIP_TRY(hipMemcpyAsync(ctx->index.ptr, index, is * 4, hipMemcpyHostToDevice, st));
-  TRY(solve_device(ctx, c, (const double*)ctx->probe.ptr, (const uint32_t*)ctx->index.ptr,
-                   (double*)ctx->out.ptr, rep, st));
+  TRY(solve_device(ctx, c, 1, (const double*)ctx->probe.ptr, (const uint32_t*)ctx->index.ptr,
+                   (double*)ctx->out.ptr, rep, nullptr, st));
   HIP_TRY(hipMemcpyAsync(out, ctx->out.ptr, os * 8, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
   if (rep) rep->device_bytes = held_bytes(ctx);
-  (void)t0;
+  return TCV_BA_OK;
+}
+
+int tcv_ba_solve_batch(tcv_ba_context* ctx, uint32_t count, const tcv_ba_control* controls,
+                       const double* const* scenes, const uint32_t* const* indices,
+                       double* const* outs, tcv_ba_report* rep, double* finish_ms) {
+  if (!ctx || count < 1 || !controls || !scenes || !indices || !outs) return TCV_BA_ERR_ARGUMENT;
+  if (rep) std::memset(rep, 0, sizeof *rep);
+  for (uint32_t b = 0; b < count; b++) {
+    TRY(check_control(&controls[b]));
+    if (!scenes[b] || !indices[b] || !outs[b]) return TCV_BA_ERR_ARGUMENT;
+  }
+  HIP_TRY(hipSetDevice(ctx->device));
+  const double t0 = now_ms();
+  // The batch as one block-diagonal packed problem: every section concatenated in bundle order,
+  // ids shifted by the bundle's first camera, point and observation.
+  uint64_t C = 0, P = 0, M = 0;
+  for (uint32_t b = 0; b < count; b++) {
+    C += controls[b].cameras;
+    P += controls[b].points;
+    M += controls[b].observations;
+  }
+  if (3 * M + P + 2 * C + 2 > 0xffffffffull || 18 * C + 3 * P + 2 * M > 0xffffffffull) {
+    set_error("batch exceeds the u32 index bounds");
+    return TCV_BA_ERR_ARGUMENT;
+  }
+  std::vector<double> scene(18 * C + 3 * P + 2 * M);
+  std::vector<uint32_t> index(3 * M + P + 2 * C + 2);
+  const size_t o_pt = 12 * C, o_model = 12 * C + 3 * P, o_pix = 18 * C + 3 * P;
+  const size_t i_op = M, i_ps = 2 * M, i_cs = 2 * M + P + 1, i_co = 2 * M + P + C + 2,
+               i_ca = 3 * M + P + C + 2;
+  uint64_t c0 = 0, p0 = 0, m0 = 0;
+  for (uint32_t b = 0; b < count; b++) {
+    const tcv_ba_control& k = controls[b];
+    const double* s = scenes[b];
+    const uint32_t* x = indices[b];
+    const size_t Cb = k.cameras, Pb = k.points, Mb = k.observations;
+    std::memcpy(&scene[12 * c0], s, 12 * Cb * 8);
+    std::memcpy(&scene[o_pt + 3 * p0], s + 12 * Cb, 3 * Pb * 8);
+    std::memcpy(&scene[o_model + 6 * c0], s + 12 * Cb + 3 * Pb, 6 * Cb * 8);
+    std::memcpy(&scene[o_pix + 2 * m0], s + 18 * Cb + 3 * Pb, 2 * Mb * 8);
+    for (size_t i = 0; i < Mb; i++) {
+      if (x[i] >= Cb || x[Mb + i] >= Pb || x[2 * Mb + Pb + Cb + 2 + i] >= Mb) {
+        set_error("bundle " + std::to_string(b) + ": index entry out of range");
+        return TCV_BA_ERR_INDEX;
+      }
+      index[m0 + i] = (uint32_t)(x[i] + c0);
+      index[i_op + m0 + i] = (uint32_t)(x[Mb + i] + p0);
+      index[i_co + m0 + i] = (uint32_t)(x[2 * Mb + Pb + Cb + 2 + i] + m0);
+    }
+    for (size_t p = 0; p <= Pb; p++) index[i_ps + p0 + p] = (uint32_t)(x[2 * Mb + p] + m0);
+    for (size_t c = 0; c <= Cb; c++) index[i_cs + c0 + c] = (uint32_t)(x[2 * Mb + Pb + 1 + c] + m0);
+    for (size_t c = 0; c < Cb; c++) index[i_ca + c0 + c] = x[3 * Mb + Pb + Cb + 2 + c];
+    c0 += Cb;
+    p0 += Pb;
+    m0 += Mb;
+  }
+  const size_t words = 26 + (controls[0].shared_focal ? 1 : 0);
+  const size_t out_size = 12 * C + 3 * P + count * words;
+  TRY(ensure(ctx->probe, scene.size() * 8));
+  TRY(ensure(ctx->index, index.size() * 4));
+  TRY(ensure(ctx->out, out_size * 8));
+  hipStream_t st = ctx->stream;
+  HIP_TRY(hipMemcpyAsync(ctx->probe.ptr, scene.data(), scene.size() * 8, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(ctx->index.ptr, index.data(), index.size() * 4, hipMemcpyHostToDevice, st));
+  TRY(solve_device(ctx, controls, count, (const double*)ctx->probe.ptr,
+                   (const uint32_t*)ctx->index.ptr, (double*)ctx->out.ptr, rep, finish_ms, st));
+  std::vector<double> all(out_size);
+  HIP_TRY(hipMemcpyAsync(all.data(), ctx->out.ptr, out_size * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  c0 = p0 = 0;
+  for (uint32_t b = 0; b < count; b++) {
+    const size_t Cb = controls[b].cameras, Pb = controls[b].points;
+    double* o = outs[b];
+    std::memcpy(o, &all[12 * c0], 12 * Cb * 8);
+    std::memcpy(o + 12 * Cb, &all[12 * C + 3 * p0], 3 * Pb * 8);
+    std::memcpy(o + 12 * Cb + 3 * Pb, &all[12 * C + 3 * P + b * words], words * 8);
+    c0 += Cb;
+    p0 += Pb;
+  }
+  if (rep) {
+    rep->solve_ms = now_ms() - t0;
+    rep->device_bytes = held_bytes(ctx);
+  }
   return TCV_BA_OK;
 }
 
@@ -643,7 +903,7 @@ int tcv_ba_schur_probe(tcv_ba_context* ctx, const tcv_ba_control* c, const doubl
   HIP_TRY(hipMemcpyAsync(ctx->probe.ptr, scene, ss * 8, hipMemcpyHostToDevice, st));
   HIP_TRY(hipMemcpyAsync(ctx->index.ptr, index, is * 4, hipMemcpyHostToDevice, st));
   Problem q;
-  TRY(setup(ctx, c, (const double*)ctx->probe.ptr, (const uint32_t*)ctx->index.ptr, st, q, rep));
+  TRY(setup(ctx, c, 1, (const double*)ctx->probe.ptr, (const uint32_t*)ctx->index.ptr, st, q, rep));
   const Ops ops = select_ops(c);
   // state: lambda, LM active, the log focal scale, PCG active (so that the sweeps run)
   std::vector<double> init(S_WORDS, 0.0);
@@ -657,19 +917,17 @@ int tcv_ba_schur_probe(tcv_ba_context* ctx, const tcv_ba_control* c, const doubl
   double* dz = dv + 2 * (size_t)m;
   double* drhs = dv + 3 * (size_t)m;
   HIP_TRY(hipMemcpyAsync(dv, p, (size_t)m * 8, hipMemcpyHostToDevice, st));
-  HIP_TRY(hipMemcpyAsync(q.vec + 2 * (size_t)m, p, (size_t)m * 8, hipMemcpyHostToDevice, st));
   ops.point_pass(q, st);
   ops.camera_pass(q, st);
-  // the focal scalars of initializeBundleCg without touching the PCG words
-  ops.init_cg(q, st);
+  ops.init_cg(q, st);  // the focal scalars of the assembled system
+  // the probe vector is the PCG direction p; PCG active so that the sweeps run
   HIP_TRY(hipMemcpyAsync(q.vec + 2 * (size_t)m, p, (size_t)m * 8, hipMemcpyHostToDevice, st));
-  const bool timing = (c->flags & TCV_BA_FLAG_TIMING) != 0 && rep;
-  if (repeat < 1) repeat = 1;
-  TRY(ensure_events(ctx, 3));
-  // keep the sweeps running: PCG active
   const double one = 1;
   HIP_TRY(hipMemcpyAsync(q.state + S_PCG_ACTIVE, &one, 8, hipMemcpyHostToDevice, st));
   HIP_TRY(hipStreamSynchronize(st));
+  const bool timing = (c->flags & TCV_BA_FLAG_TIMING) != 0 && rep;
+  if (repeat < 1) repeat = 1;
+  TRY(ensure_events(ctx, 3));
   for (uint32_t r = 0; r < repeat; r++) {
     if (timing) HIP_TRY(hipEventRecord(ctx->events[0], st));
     ops.point_sweep(q, st);

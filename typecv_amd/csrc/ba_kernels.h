@@ -1,17 +1,24 @@
 // ba_kernels.h — the gfx950 kernels of one LM iteration of the bundle adjustment.
 //
-// Data layout in HBM (C cameras, P points, M observations, K = 18 or 21 cross words):
-//   scene, trial, index    the reference's packed arrays (include/typecv_ba.h)
-//   Wp[K][M]               cross blocks W_i = w Jc^T Jp (+ the focal row w Jf^T Jp), one component
-//                          per row, observations in point-major (input) order: the point sweep
-//   Wc[K][M]               the same blocks in camera-major order (the order of the reference's
-//                          camera lists): the camera sweep streams them
-//   camPoint[M]            the point of each camera-major slot
+// A launch covers a batch of B independent bundles concatenated into one block-diagonal problem:
+// global cameras, points and observations are the bundles' own in order, so every per-observation,
+// per-point, per-camera, per-tile and per-task kernel runs over all bundles at once, and each
+// bundle keeps its own LM and PCG state words (its own damping, accept/reject and convergence). A
+// single solve is a batch of one.
+//
+// Data layout in HBM (C cameras, P points, M observations in all, K = 18 or 21 cross words):
+//   scene, trial, index    the reference's packed arrays (include/typecv_ba.h) of the whole batch
+//   Wp                     cross blocks W_i = w Jc^T Jp (+ the focal row w Jf^T Jp), observations in
+//                          point-major (input) order: the point sweep streams it
+//   Wc                     the same blocks in camera-major order (each camera's list): the camera
+//                          sweep streams it. Both blocked [M/64][K][64] (see wptr)
+//   cam_point[M]           the point of each camera-major slot
 //   pblk[P][12]            V^-1 (9, symmetrised) and the point RHS b_p (3)
-//   tiles[T+1]             first point of each wave tile: whole points, at most 64 observations
-//                          (a point with more observations is a tile of its own)
-//   tasks[3·N], taskStart  camera-list tasks (camera, first slot, end slot), at most `task_obs`
-//                          slots each, and the first task of each camera
+//   tiles[T+1]             first point of each wave tile: whole points of one bundle, at most 64
+//                          observations (a point with more observations is a tile of its own)
+//   tasks[4·N], task_start camera-list tasks (camera, first slot, end slot, bundle), at most 1024
+//                          slots, and the first task of each camera
+//   rows                   camera c owns unknowns 6c..6c+5; bundle b's focal unknown is 6C + b
 // Every reduction has a fixed order, so a run is bitwise reproducible; no float atomics.
 #pragma once
 
@@ -22,7 +29,7 @@
 
 namespace tcvba {
 
-// State words: the reference's 26 (BUNDLE_STATE) and the device's own focal words.
+// State words of each bundle: the reference's 26 (BUNDLE_STATE) and the device's own.
 enum : int {
   S_LAMBDA = 0, S_ACTIVE = 1, S_BASELINE = 2, S_ORIGIN = 3, S_COST = 6, S_SSE = 7,
   S_COST0 = 8, S_SSE0 = 9, S_ACCEPTED = 10, S_REJECTED = 11, S_COMMIT = 12, S_GAUGE = 13,
@@ -39,7 +46,7 @@ enum : int {
   S_WORDS = 40
 };
 
-constexpr int TILE = 64;           // observations per wave tile
+constexpr int TILE = 64;  // observations per wave tile
 #ifndef TCV_TILE_WAVES
 #define TCV_TILE_WAVES 4
 #endif
@@ -51,23 +58,40 @@ constexpr int TILE = 64;           // observations per wave tile
 #endif
 constexpr int TILE_WAVES = TCV_TILE_WAVES;  // wave tiles per workgroup of the point-tile kernels
 constexpr int TASK_THREADS = 256;  // threads of a camera-task workgroup
-constexpr int CG_THREADS = 1024;   // threads of the single-workgroup PCG kernels
-constexpr int ASM_WORDS = 56;      // per-task camera assembly sums (see camera_pass)
+constexpr int CG_THREADS = 1024;   // threads of the per-bundle PCG kernels
+constexpr int COST_OBS = 256;      // observations per cost block
+constexpr int ASM_WORDS = 56;      // per-task camera assembly sums (see k_camera_pass)
+
+// One bundle of the batch: its global ranges and gauge.
+struct Bundle {
+  uint32_t cam0, cam1, pt0, pt1, obs0, obs1;
+  uint32_t tile0, tile1, task0, task1, cost0, cost1;
+  uint32_t first_directional;  // global index of the bundle's first directional point
+  uint32_t anchor, scale_view;  // global camera indices
+  uint32_t normalize_scale;
+  uint32_t focal_row;  // 6C + b
+  uint32_t pad;
+};
 
 struct Problem {
-  uint32_t C, P, M, m, K, shared, first_directional, anchor, scale_view, normalize_scale;
-  uint32_t n_tiles, n_tasks;
+  uint32_t C, P, M, B, m, K, shared;
+  uint32_t n_tiles, n_tasks, n_cost_blocks;
   double huber, tol;
   double refine_tol;  // mixed form: relative FP64 residual of the outer refinement
   double* scene;
   double* trial;
   const uint32_t* index;
-  const uint32_t* tiles;
-  const uint32_t* tasks;
-  const uint32_t* task_start;
+  const Bundle* bundles;
+  const uint32_t* tiles;          // n_tiles + 1 first points
+  const uint32_t* tile_bundle;    // n_tiles
+  const uint32_t* tasks;          // 4 per task: camera, first slot, end slot, bundle
+  const uint32_t* task_start;     // C + 1
+  const uint32_t* cost_blocks;    // 3 per block: bundle, first and end observation
+  const uint32_t* camera_bundle;  // C
+  const uint32_t* point_bundle;   // P
   uint32_t* cam_point;
-  void* Wp;  // WT[K][M]
-  void* Wc;  // WT[K][M]
+  void* Wp;
+  void* Wc;
   double* pblk;
   uint32_t* tile_fail;
   double* tile_focal;  // 2 per tile
@@ -77,11 +101,10 @@ struct Problem {
   double* camRhs;      // 6 per camera
   double* camHcf;      // 6 per camera
   double* camFocal;    // 2 per camera: H_ff and b_f shares
-  double* vec;         // x r p s z (m each), y (3P), fpart (C)
-  double* state;
+  double* vec;         // x r p s z (m each), y (3P)
+  double* state;       // S_WORDS per bundle
   double* cost_part;   // 3 per cost block
   uint32_t* error;     // index validation flag
-  uint32_t n_cost_blocks;
 };
 
 __host__ __device__ inline const uint32_t* obs_camera(const Problem& q) { return q.index; }

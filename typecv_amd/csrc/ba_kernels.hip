@@ -1,9 +1,9 @@
 // ba_kernels.hip — gfx950 kernels of the bundle-adjustment LM loop (included by ba_engine.hip).
 //
 // Kernel map (reference entry points in src/sfm/bundle/gpu/bundle.shade.ts "b:" and
-// joint-focal.shade.ts "j:"):
+// joint-focal.shade.ts "j:"). "Per bundle" kernels run one workgroup per bundle of the batch.
 //   k_check_points / k_check_tasks   index validation, camera-major point ids (host packing checks)
-//   k_init_bundle                    initializeBundle b:235-250 / j:269-286
+//   k_init_bundle (per bundle)       initializeBundle b:235-250 / j:269-286
 //   k_cost / k_finish_cost           bundleMetricPartials + finishBundleMetrics b:319-377,
 //                                    bundleCandidatePartials + acceptBundleCandidate b:1040-1096
 //   k_point_pass                     linearizeBundle + bundlePointBlocks b:251-297, 378-422 and the
@@ -13,12 +13,12 @@
 //                                    invertBundleCameraColumns b:655-694 and the focal camera terms
 //                                    bundleFocalCameraRows/Observations j:1138-1171: one workgroup
 //                                    per camera-list task, fixed-order task sums, Cholesky inverse
-//   k_init_cg                        initializeBundleCg b:695-741 / j:646-700 with dampBundleFocal
+//   k_init_cg (per bundle)           initializeBundleCg b:695-741 / j:646-700 with dampBundleFocal
 //   k_point_sweep<SCHUR>             bundleSchurPoint b:742-745 (body b:185-203, j:198-218)
 //   k_camera_sweep                   bundleSchurCamera* b:746-882 / j:705-870 (camera partials)
-//   k_cg_advance                     finishes S·p (A p, focal row) then advanceBundleCg b:883-929
+//   k_cg_advance (per bundle)        finishes S·p (A p, focal row) then advanceBundleCg b:883-929
 //   k_update_cameras / k_point_sweep<UPDATE>   updateBundleCandidate b:930-961 / j:925-960
-//   k_gauge                          bundleCandidateGauge b:962-993 / j:961-1000
+//   k_gauge (per bundle)             bundleCandidateGauge b:962-993 / j:961-1000
 //   k_normalize                      normalizeBundleCandidate b:994-1023
 //   k_commit / k_pack                commitBundleCandidate b:1109-1114, packBundleResult b:1115-1125
 #include "ba_kernels.h"
@@ -27,14 +27,17 @@ namespace tcvba {
 
 // ------------------------------------------------------------------ small helpers
 
+__device__ __forceinline__ double* bstate(const Problem& q, uint32_t b) {
+  return q.state + (size_t)b * S_WORDS;
+}
 __device__ __forceinline__ bool variable_camera(const Problem& q, uint32_t c) {
   return camera_active(q)[c] != 0;
 }
-__device__ __forceinline__ bool directional_point(const Problem& q, uint32_t p) {
-  return p >= q.first_directional;
+__device__ __forceinline__ bool directional_point(const Bundle& g, uint32_t p) {
+  return p >= g.first_directional;
 }
 
-// Model of camera c at log focal scale `fscale` (= focalScale(log), 1 on the fixed route).
+// Model of camera c at focal scale `fscale` (= focalScale(log), 1 on the fixed route).
 __device__ __forceinline__ Model camera_model(const Problem& q, uint32_t c, double fscale) {
   const double* s = q.scene + (size_t)q.C * 12 + (size_t)q.P * 3 + (size_t)c * 6;
   Model k{s[0], s[1], s[2], s[3], s[4], s[5]};
@@ -88,10 +91,9 @@ struct Lin {
 };
 
 __device__ __forceinline__ bool linearize(const Problem& q, uint32_t c, uint32_t p, uint32_t i,
-                                          double fscale, bool var, Lin& L) {
+                                          double fscale, bool var, bool dir, Lin& L) {
   const double* pose = q.scene + (size_t)c * 12;
   const double* X = q.scene + (size_t)q.C * 12 + (size_t)p * 3;
-  const bool dir = directional_point(q, p);
   double R[12];
 #pragma unroll
   for (int j = 0; j < 12; j++) R[j] = pose[j];
@@ -166,22 +168,32 @@ __device__ __forceinline__ WT* wptr(void* base, uint32_t k, uint32_t i, uint32_t
 #endif
 }
 
+__device__ __forceinline__ double* vx(const Problem& q) { return q.vec; }
+__device__ __forceinline__ double* vr(const Problem& q) { return q.vec + q.m; }
+__device__ __forceinline__ double* vp(const Problem& q) { return q.vec + 2 * (size_t)q.m; }
+__device__ __forceinline__ double* vs(const Problem& q) { return q.vec + 3 * (size_t)q.m; }
+__device__ __forceinline__ double* vz(const Problem& q) { return q.vec + 4 * (size_t)q.m; }
+__device__ __forceinline__ double* vy(const Problem& q) { return q.vec + 5 * (size_t)q.m; }
+
 // ------------------------------------------------------------------ index checks
 
-// Every observation of point p names p, a camera below C and lies in the point's range.
+// Every observation of point p names p and a camera of p's bundle, and lies in p's range.
 __global__ void k_check_points(Problem q) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= q.P) return;
+  const Bundle& g = q.bundles[q.point_bundle[p]];
   const uint32_t* ps = point_start(q);
-  for (uint32_t i = ps[p]; i < ps[p + 1]; i++)
-    if (obs_point(q)[i] != p || obs_camera(q)[i] >= q.C) atomicOr(q.error, 1u);
+  for (uint32_t i = ps[p]; i < ps[p + 1]; i++) {
+    const uint32_t c = obs_camera(q)[i];
+    if (obs_point(q)[i] != p || c < g.cam0 || c >= g.cam1) atomicOr(q.error, 1u);
+  }
 }
 
 // Every slot of a camera task names an observation of that camera, ascending within the camera's
 // list; writes the camera-major point ids.
 __global__ void k_check_tasks(Problem q) {
   const uint32_t t = blockIdx.x;
-  const uint32_t c = q.tasks[3 * t], s = q.tasks[3 * t + 1], e = q.tasks[3 * t + 2];
+  const uint32_t c = q.tasks[4 * t], s = q.tasks[4 * t + 1], e = q.tasks[4 * t + 2];
   const uint32_t list = camera_start(q)[c];
   for (uint32_t n = s + threadIdx.x; n < e; n += blockDim.x) {
     const uint32_t i = camera_obs(q)[n];
@@ -205,16 +217,19 @@ __device__ __forceinline__ void center_of(const double* p, double o[3]) {
   o[2] = -p[2] * p[9] - p[5] * p[10] - p[8] * p[11];
 }
 
-// initializeBundle: lambda 1e-3, gauge origin at the anchor centre, baseline to the scale view.
+// initializeBundle, per bundle: lambda 1e-3, gauge origin at the anchor centre, baseline to the
+// scale view.
 __global__ void k_init_bundle(Problem q) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double* s = q.state;
+  if (threadIdx.x != 0) return;
+  const uint32_t b = blockIdx.x;
+  const Bundle& g = q.bundles[b];
+  double* s = bstate(q, b);
   for (int j = 0; j < S_WORDS; j++) s[j] = 0;
   s[S_LAMBDA] = 1e-3;
   s[S_ACTIVE] = 1;
   double o[3], v[3];
-  center_of(q.scene + (size_t)q.anchor * 12, o);
-  center_of(q.scene + (size_t)q.scale_view * 12, v);
+  center_of(q.scene + (size_t)g.anchor * 12, o);
+  center_of(q.scene + (size_t)g.scale_view * 12, v);
   const double d0 = o[0] - v[0], d1 = o[1] - v[1], d2 = o[2] - v[2];
   const double baseline = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
   s[S_BASELINE] = baseline;
@@ -227,24 +242,29 @@ __global__ void k_init_bundle(Problem q) {
   }
 }
 
-// Robust cost, squared error and invalid count of 256 observations per workgroup: of the scene
-// (metricLane b:301-314) or of the trial (candidateLane b:1024-1037, 1e30 for an invalid lane).
+// Robust cost, squared error and invalid count of one cost block (at most 256 observations of
+// one bundle): of the scene (metricLane b:301-314) or of the trial (candidateLane b:1024-1037,
+// 1e30 for an invalid lane).
 template <bool CANDIDATE>
-__global__ __launch_bounds__(256) void k_cost(Problem q) {
-  __shared__ double lds[3][4];
-  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+__global__ __launch_bounds__(COST_OBS) void k_cost(Problem q) {
+  __shared__ double lds[3][COST_OBS / 64];
+  const uint32_t blk = blockIdx.x;
+  const uint32_t b = q.cost_blocks[3 * blk], e = q.cost_blocks[3 * blk + 2];
+  const uint32_t i = q.cost_blocks[3 * blk + 1] + threadIdx.x;
+  const Bundle& g = q.bundles[b];
   double cost = 0, sq = 0, bad = 0;
-  if (i < q.M) {
+  if (i < e) {
     const uint32_t c = obs_camera(q)[i], p = obs_point(q)[i];
     const double* base = CANDIDATE ? q.trial : q.scene;
     const double* pose = base + (size_t)c * 12;
     const double* X = base + (size_t)q.C * 12 + (size_t)p * 3;
-    const double fscale = q.shared ? focal_scale(q.state[CANDIDATE ? S_TRIAL_FOCAL : S_LOG_FOCAL]) : 1.0;
+    const double* s = bstate(q, b);
+    const double fscale = q.shared ? focal_scale(s[CANDIDATE ? S_TRIAL_FOCAL : S_LOG_FOCAL]) : 1.0;
     bool ok = true;
     if (CANDIDATE) ok = fmax(fabs(X[0]), fmax(fabs(X[1]), fabs(X[2]))) <= 1e9;
     double px = 0, py = 0, du[3], dv[3];
     if (ok) {
-      const bool dir = directional_point(q, p);
+      const bool dir = directional_point(g, p);
       double qc[3];
       for (int r = 0; r < 3; r++) {
         qc[r] = pose[r * 3] * X[0] + pose[r * 3 + 1] * X[1] + pose[r * 3 + 2] * X[2];
@@ -277,20 +297,22 @@ __global__ __launch_bounds__(256) void k_cost(Problem q) {
   __syncthreads();
   if (threadIdx.x < 3) {
     const double* v = lds[threadIdx.x];
-    q.cost_part[(size_t)blockIdx.x * 3 + threadIdx.x] = (v[0] + v[1]) + (v[2] + v[3]);
+    q.cost_part[(size_t)blk * 3 + threadIdx.x] = (v[0] + v[1]) + (v[2] + v[3]);
   }
 }
 
-// finishBundleMetrics (CANDIDATE = false) or acceptBundleCandidate (true).
+// Per bundle: finishBundleMetrics (CANDIDATE = false) or acceptBundleCandidate (true).
 template <bool CANDIDATE>
-__global__ __launch_bounds__(CG_THREADS) void k_finish_cost(Problem q) {
-  __shared__ double lds[CG_THREADS / 64];
-  double* s = q.state;
+__global__ __launch_bounds__(COST_OBS) void k_finish_cost(Problem q) {
+  __shared__ double lds[COST_OBS / 64];
+  const uint32_t b = blockIdx.x;
+  const Bundle& g = q.bundles[b];
+  double* s = bstate(q, b);
   double cost = 0, sq = 0, bad = 0;
-  for (uint32_t b = threadIdx.x; b < q.n_cost_blocks; b += CG_THREADS) {
-    cost += q.cost_part[(size_t)b * 3];
-    sq += q.cost_part[(size_t)b * 3 + 1];
-    bad += q.cost_part[(size_t)b * 3 + 2];
+  for (uint32_t k = g.cost0 + threadIdx.x; k < g.cost1; k += COST_OBS) {
+    cost += q.cost_part[(size_t)k * 3];
+    sq += q.cost_part[(size_t)k * 3 + 1];
+    bad += q.cost_part[(size_t)k * 3 + 2];
   }
   const double active = s[S_ACTIVE], valid = s[S_TRIAL_VALID], current = s[S_COST];
   const double input_error = s[S_INPUT_ERROR];
@@ -341,10 +363,14 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_pass(Problem q) {
   constexpr int NS = SHARED ? 13 : 10;  // staged words per observation: V 6, b 3, [fc 3], bad
   __shared__ double st_all[TILE_WAVES][TILE][NS];
   __shared__ double fsum_all[TILE_WAVES][TILE][2];
-  if (q.state[S_ACTIVE] == 0) return;
-  const uint32_t wave = threadIdx.x / TILE, lane = threadIdx.x % TILE;
+  // the wave's index is uniform across its lanes: keep the tile's loads and branches scalar
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / TILE), lane = threadIdx.x % TILE;
   const uint32_t t = blockIdx.x * TILE_WAVES + wave;
   if (t >= q.n_tiles) return;
+  const uint32_t b = q.tile_bundle[t];
+  const double* sb = bstate(q, b);
+  if (sb[S_ACTIVE] == 0) return;
+  const Bundle& g = q.bundles[b];
   auto& st = st_all[wave];
   auto& fsum = fsum_all[wave];
   const uint32_t fp = q.tiles[t], ep = q.tiles[t + 1];
@@ -353,9 +379,9 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_pass(Problem q) {
   const uint32_t my = fp + lane;
   const bool has = my < ep;
   const uint32_t mys = has ? ps[my] : 0, mye = has ? ps[my + 1] : 0;
-  const double lambda = q.state[S_LAMBDA];
-  const double fscale = SHARED ? focal_scale(q.state[S_LOG_FOCAL]) : 1.0;
-  double V[6] = {0, 0, 0, 0, 0, 0}, b[3] = {0, 0, 0}, fc[3] = {0, 0, 0};
+  const double lambda = sb[S_LAMBDA];
+  const double fscale = SHARED ? focal_scale(sb[S_LOG_FOCAL]) : 1.0;
+  double V[6] = {0, 0, 0, 0, 0, 0}, bp[3] = {0, 0, 0}, fc[3] = {0, 0, 0};
   bool bad = false;
   const uint32_t K = SHARED ? 21 : 18;
   for (uint32_t base = s; base < e; base += TILE) {
@@ -363,7 +389,8 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_pass(Problem q) {
     if (i < e) {
       const uint32_t c = obs_camera(q)[i], p = obs_point(q)[i];
       Lin L;
-      const bool ok = linearize(q, c, p, i, fscale, variable_camera(q, c), L);
+      const bool ok =
+          linearize(q, c, p, i, fscale, variable_camera(q, c), directional_point(g, p), L);
       double W[21];
       if (ok) {
         cross_block(L, SHARED, W);
@@ -402,7 +429,7 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_pass(Problem q) {
       for (uint32_t j = lo; j < hi; j++) {
         const double* o = st[j - base];
         for (int k = 0; k < 6; k++) V[k] += o[k];
-        for (int k = 0; k < 3; k++) b[k] += o[6 + k];
+        for (int k = 0; k < 3; k++) bp[k] += o[6 + k];
         if (SHARED)
           for (int k = 0; k < 3; k++) fc[k] += o[9 + k];
         bad = bad || o[NS - 1] != 0;
@@ -427,13 +454,13 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_pass(Problem q) {
       for (int k = 0; k < 12; k++) out[k] = 0;
     } else {
       for (int k = 0; k < 9; k++) out[k] = inv[k];
-      for (int k = 0; k < 3; k++) out[9 + k] = b[k];
+      for (int k = 0; k < 3; k++) out[9 + k] = bp[k];
       if (SHARED) {
         for (int j = 0; j < 3; j++) {
           double tj = 0;
           for (int k = 0; k < 3; k++) tj += fc[k] * inv[k * 3 + j];
           dF += tj * fc[j];
-          dR += tj * b[j];
+          dR += tj * bp[j];
         }
       }
     }
@@ -470,11 +497,14 @@ template <class WT, bool SHARED>
 __global__ __launch_bounds__(TASK_THREADS) void k_camera_pass(Problem q) {
   constexpr int NV = SHARED ? 56 : 48;
   __shared__ double red[TASK_THREADS / 64][NV];
-  if (q.state[S_ACTIVE] == 0) return;
   const uint32_t t = blockIdx.x;
-  const uint32_t c = q.tasks[3 * t], s = q.tasks[3 * t + 1], e = q.tasks[3 * t + 2];
+  const uint32_t c = q.tasks[4 * t], s = q.tasks[4 * t + 1], e = q.tasks[4 * t + 2];
+  const uint32_t b = q.camera_bundle[c];
+  const double* sb = bstate(q, b);
+  if (sb[S_ACTIVE] == 0) return;
+  const Bundle& g = q.bundles[b];
   const bool var = variable_camera(q, c);
-  const double fscale = SHARED ? focal_scale(q.state[S_LOG_FOCAL]) : 1.0;
+  const double fscale = SHARED ? focal_scale(sb[S_LOG_FOCAL]) : 1.0;
   const uint32_t K = SHARED ? 21 : 18;
   double acc[NV];
 #pragma unroll
@@ -482,7 +512,7 @@ __global__ __launch_bounds__(TASK_THREADS) void k_camera_pass(Problem q) {
   for (uint32_t n = s + threadIdx.x; n < e; n += TASK_THREADS) {
     const uint32_t i = camera_obs(q)[n], p = q.cam_point[n];
     Lin L;
-    const bool ok = linearize(q, c, p, i, fscale, var, L);
+    const bool ok = linearize(q, c, p, i, fscale, var, directional_point(g, p), L);
     double W[21];
     if (ok) {
       cross_block(L, SHARED, W);
@@ -494,9 +524,9 @@ __global__ __launch_bounds__(TASK_THREADS) void k_camera_pass(Problem q) {
     const double w = L.w;
     if (var) {
       const double* pb = q.pblk + (size_t)p * 12;
-      double Vi[9], bp[3];
+      double Vi[9], bpv[3];
       for (int k = 0; k < 9; k++) Vi[k] = pb[k];
-      for (int k = 0; k < 3; k++) bp[k] = pb[9 + k];
+      for (int k = 0; k < 3; k++) bpv[k] = pb[9 + k];
       double T[18];
 #pragma unroll
       for (int j = 0; j < 6; j++)
@@ -513,7 +543,7 @@ __global__ __launch_bounds__(TASK_THREADS) void k_camera_pass(Problem q) {
               T[j * 3] * W[l * 3] + T[j * 3 + 1] * W[l * 3 + 1] + T[j * 3 + 2] * W[l * 3 + 2];
         }
         acc[42 + j] += -(wj0 * L.r[0]) - wj1 * L.r[1] -
-                       (T[j * 3] * bp[0] + T[j * 3 + 1] * bp[1] + T[j * 3 + 2] * bp[2]);
+                       (T[j * 3] * bpv[0] + T[j * 3 + 1] * bpv[1] + T[j * 3 + 2] * bpv[2]);
         if (SHARED) acc[48 + j] += wj0 * L.Jf[0] + wj1 * L.Jf[1];
       }
     }
@@ -542,7 +572,9 @@ __global__ __launch_bounds__(TASK_THREADS) void k_camera_pass(Problem q) {
 template <bool SHARED>
 __global__ void k_camera_finish(Problem q) {
   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= q.C || q.state[S_ACTIVE] == 0) return;
+  if (c >= q.C) return;
+  const double* sb = bstate(q, q.camera_bundle[c]);
+  if (sb[S_ACTIVE] == 0) return;
   constexpr int NV = SHARED ? 56 : 48;
   double acc[NV];
   for (int k = 0; k < NV; k++) acc[k] = 0;
@@ -563,7 +595,7 @@ __global__ void k_camera_finish(Problem q) {
     }
     return;
   }
-  const double lambda = q.state[S_LAMBDA];
+  const double lambda = sb[S_LAMBDA];
   double a[36], S[36];
   for (int j = 0; j < 6; j++)
     for (int l = j; l < 6; l++) {
@@ -585,52 +617,45 @@ __global__ void k_camera_finish(Problem q) {
     inv[0] = -1;
     return;
   }
-  double M[36];
+  double Mi[36];
   for (int column = 0; column < 6; column++) {
     double e6[6] = {0, 0, 0, 0, 0, 0}, x[6];
     e6[column] = 1;
     chol6_solve(f, e6, x);
-    for (int j = 0; j < 6; j++) M[j * 6 + column] = x[j];
+    for (int j = 0; j < 6; j++) Mi[j * 6 + column] = x[j];
   }
   for (int j = 0; j < 6; j++)
     for (int k = j + 1; k < 6; k++) {
-      const double v = (M[j * 6 + k] + M[k * 6 + j]) * 0.5;
-      M[j * 6 + k] = v;
-      M[k * 6 + j] = v;
+      const double v = (Mi[j * 6 + k] + Mi[k * 6 + j]) * 0.5;
+      Mi[j * 6 + k] = v;
+      Mi[k * 6 + j] = v;
     }
-  for (int j = 0; j < 36; j++) inv[j] = M[j];
+  for (int j = 0; j < 36; j++) inv[j] = Mi[j];
 }
 
 // ------------------------------------------------------------------ PCG
-
-__device__ __forceinline__ double* vx(const Problem& q) { return q.vec; }
-__device__ __forceinline__ double* vr(const Problem& q) { return q.vec + q.m; }
-__device__ __forceinline__ double* vp(const Problem& q) { return q.vec + 2 * (size_t)q.m; }
-__device__ __forceinline__ double* vs(const Problem& q) { return q.vec + 3 * (size_t)q.m; }
-__device__ __forceinline__ double* vz(const Problem& q) { return q.vec + 4 * (size_t)q.m; }
-__device__ __forceinline__ double* vy(const Problem& q) { return q.vec + 5 * (size_t)q.m; }
-__device__ __forceinline__ double* vfpart(const Problem& q) {
-  return q.vec + 5 * (size_t)q.m + 3 * (size_t)q.P;
-}
 
 // Row `row` = 6c + j of a 6x6-block-diagonal product: sum_k B_c[j][k] v[6c + k]. Consecutive
 // rows read consecutive 48-byte block rows, so a wave's loads are coalesced.
 __device__ __forceinline__ double block_row(const double* blocks, uint32_t row, const double* v) {
   const uint32_t c = row / 6;
-  const double* b = blocks + (size_t)c * 36 + (size_t)(row - c * 6) * 6;
+  const double* bk = blocks + (size_t)c * 36 + (size_t)(row - c * 6) * 6;
   const double* vc = v + (size_t)c * 6;
   double a = 0;
 #pragma unroll
-  for (int k = 0; k < 6; k++) a += b[k] * vc[k];
+  for (int k = 0; k < 6; k++) a += bk[k] * vc[k];
   return a;
 }
 
-// initializeBundleCg (with dampBundleFocal and the rest of bundleFocalPoints on the shared route):
-// point and camera block checks, x = 0, r = b, z = M^-1 r, p = z, rho, |r0|^2.
+// initializeBundleCg (with dampBundleFocal and the rest of bundleFocalPoints on the shared route),
+// one workgroup per bundle: point and camera block checks, x = 0, r = b, z = M^-1 r, p = z, rho,
+// |r0|^2.
 template <bool SHARED>
 __global__ __launch_bounds__(CG_THREADS) void k_init_cg(Problem q) {
   __shared__ double lds[CG_THREADS / 64];
-  double* s = q.state;
+  const uint32_t bi = blockIdx.x;
+  const Bundle& g = q.bundles[bi];
+  double* s = bstate(q, bi);
   const uint32_t tid = threadIdx.x;
   const double active = s[S_ACTIVE], lambda = s[S_LAMBDA];
   __syncthreads();
@@ -638,16 +663,16 @@ __global__ __launch_bounds__(CG_THREADS) void k_init_cg(Problem q) {
     for (int j = S_RHO; j < 26; j++) s[j] = 0;
   if (active == 0) return;
   double fail = 0;
-  for (uint32_t t = tid; t < q.n_tiles; t += CG_THREADS) fail += q.tile_fail[t] ? 1.0 : 0.0;
-  for (uint32_t c = tid; c < q.C; c += CG_THREADS)
+  for (uint32_t t = g.tile0 + tid; t < g.tile1; t += CG_THREADS) fail += q.tile_fail[t] ? 1.0 : 0.0;
+  for (uint32_t c = g.cam0 + tid; c < g.cam1; c += CG_THREADS)
     if (!(q.camInv[(size_t)c * 36] > 0)) fail += 1;
   double hff = 0, bf = 0, tf = 0, tr = 0;
   if (SHARED) {
-    for (uint32_t c = tid; c < q.C; c += CG_THREADS) {
+    for (uint32_t c = g.cam0 + tid; c < g.cam1; c += CG_THREADS) {
       hff += q.camFocal[2 * (size_t)c];
       bf += q.camFocal[2 * (size_t)c + 1];
     }
-    for (uint32_t t = tid; t < q.n_tiles; t += CG_THREADS) {
+    for (uint32_t t = g.tile0 + tid; t < g.tile1; t += CG_THREADS) {
       tf += q.tile_focal[2 * (size_t)t];
       tr += q.tile_focal[2 * (size_t)t + 1];
     }
@@ -679,8 +704,7 @@ __global__ __launch_bounds__(CG_THREADS) void k_init_cg(Problem q) {
   double* r = vr(q);
   double* p = vp(q);
   double* z = vz(q);
-  const uint32_t n = q.C * 6;
-  for (uint32_t row = tid; row < n; row += CG_THREADS) {
+  for (uint32_t row = 6 * g.cam0 + tid; row < 6 * g.cam1; row += CG_THREADS) {
     const double rv = q.camRhs[row];
     const double zv = block_row(q.camInv, row, q.camRhs);
     x[row] = 0;
@@ -691,7 +715,7 @@ __global__ __launch_bounds__(CG_THREADS) void k_init_cg(Problem q) {
     res += rv * rv;
   }
   if (SHARED && tid == 0) {
-    const uint32_t f = q.m - 1;
+    const uint32_t f = g.focal_row;
     const double zf = finv * frhs;
     x[f] = 0;
     r[f] = frhs;
@@ -721,20 +745,23 @@ enum SweepMode { SWEEP_SCHUR = 0, SWEEP_UPDATE = 1 };
 template <class WT, bool SHARED, int MODE>
 __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_sweep(Problem q) {
   __shared__ double st_all[TILE_WAVES][TILE][3];
-  if (MODE == SWEEP_SCHUR && q.state[S_PCG_ACTIVE] == 0) return;
-  if (MODE == SWEEP_UPDATE && (q.state[S_ACTIVE] == 0 || q.state[S_PCG_FAILED] != 0)) return;
-  const double* v = MODE == SWEEP_SCHUR ? vp(q) : vx(q);
-  const uint32_t wave = threadIdx.x / TILE, lane = threadIdx.x % TILE;
+  // the wave's index is uniform across its lanes: keep the tile's loads and branches scalar
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / TILE), lane = threadIdx.x % TILE;
   const uint32_t t = blockIdx.x * TILE_WAVES + wave;
   if (t >= q.n_tiles) return;
+  // independent loads first: the tile's bounds and bundle, then the bundle's state word
+  const uint32_t fp = q.tiles[t], ep = q.tiles[t + 1], b = q.tile_bundle[t];
+  const double* sb = bstate(q, b);
+  if (MODE == SWEEP_SCHUR && sb[S_PCG_ACTIVE] == 0) return;
+  if (MODE == SWEEP_UPDATE && (sb[S_ACTIVE] == 0 || sb[S_PCG_FAILED] != 0)) return;
+  const double* v = MODE == SWEEP_SCHUR ? vp(q) : vx(q);
   auto& st = st_all[wave];
-  const uint32_t fp = q.tiles[t], ep = q.tiles[t + 1];
   const uint32_t* ps = point_start(q);
   const uint32_t s = ps[fp], e = ps[ep];
   const uint32_t my = fp + lane;
   const bool has = my < ep;
   const uint32_t mys = has ? ps[my] : 0, mye = has ? ps[my + 1] : 0;
-  const double vf = SHARED ? v[q.m - 1] : 0.0;
+  const double vf = SHARED ? v[6 * q.C + b] : 0.0;  // the bundle's focal row
   double acc[3] = {0, 0, 0};
   for (uint32_t base = s; base < e; base += TILE) {
     const uint32_t i = base + lane;
@@ -785,9 +812,9 @@ template <class WT, bool SHARED>
 __global__ __launch_bounds__(TASK_THREADS) void k_camera_sweep(Problem q) {
   constexpr int NV = SHARED ? 7 : 6;
   __shared__ double red[TASK_THREADS / 64][NV];
-  if (q.state[S_PCG_ACTIVE] == 0) return;
   const uint32_t t = blockIdx.x;
-  const uint32_t s = q.tasks[3 * t + 1], e = q.tasks[3 * t + 2];
+  const uint32_t s = q.tasks[4 * t + 1], e = q.tasks[4 * t + 2], b = q.tasks[4 * t + 3];
+  if (bstate(q, b)[S_PCG_ACTIVE] == 0) return;
   const double* y = vy(q);
   double acc[NV];
 #pragma unroll
@@ -815,15 +842,16 @@ __global__ __launch_bounds__(TASK_THREADS) void k_camera_sweep(Problem q) {
   }
 }
 
-// Finish S·v from the camera partials: (S v)_c = A_c v_c [+ H_cf v_f] - sum_tasks part_c, and the
-// focal row H_ff v_f + sum_c (H_cf . v_c - part_c[6]). Returns this thread's share of v . S v.
-// Uniform control flow (uses block_sum).
+// Finish S·v of bundle g from the camera partials: (S v)_c = A_c v_c [+ H_cf v_f] - sum_tasks
+// part_c, and the focal row H_ff v_f + sum_c H_cf . v_c - sum_tasks part[6]. Returns this thread's
+// share of v . S v. Uniform control flow (uses block_sum).
 template <bool SHARED>
-__device__ double schur_finish(const Problem& q, const double* v, double* sv, double* lds) {
-  const uint32_t tid = threadIdx.x, n = q.C * 6;
-  const double vf = SHARED ? v[q.m - 1] : 0.0;
+__device__ double schur_finish(const Problem& q, const Bundle& g, const double* s,
+                               const double* v, double* sv, double* lds) {
+  const uint32_t tid = threadIdx.x;
+  const double vf = SHARED ? v[g.focal_row] : 0.0;
   double dot = 0, fsum = 0;
-  for (uint32_t row = tid; row < n; row += CG_THREADS) {
+  for (uint32_t row = 6 * g.cam0 + tid; row < 6 * g.cam1; row += CG_THREADS) {
     const uint32_t c = row / 6, j = row - c * 6;
     double sum = 0;
     for (uint32_t t = q.task_start[c]; t < q.task_start[c + 1]; t++)
@@ -836,22 +864,26 @@ __device__ double schur_finish(const Problem& q, const double* v, double* sv, do
     if (SHARED) fsum += q.camHcf[row] * v[row];
   }
   if (SHARED) {
-    for (uint32_t t = tid; t < q.n_tasks; t += CG_THREADS) fsum -= q.task_part[(size_t)t * 8 + 6];
+    for (uint32_t t = g.task0 + tid; t < g.task1; t += CG_THREADS)
+      fsum -= q.task_part[(size_t)t * 8 + 6];
     fsum = block_sum(fsum, lds);
-    const double sf = q.state[S_FOCAL_DIAG] * vf + fsum;
+    const double sf = s[S_FOCAL_DIAG] * vf + fsum;
     if (tid == 0) {
-      sv[q.m - 1] = sf;
+      sv[g.focal_row] = sf;
       dot += vf * sf;
     }
   }
   return dot;
 }
 
-// advanceBundleCg b:883-929 / advancePcg(false) j:874-924, after finishing S p.
+// advanceBundleCg b:883-929 / advancePcg(false) j:874-924, one workgroup per bundle, after
+// finishing S p.
 template <bool SHARED>
 __global__ __launch_bounds__(CG_THREADS) void k_cg_advance(Problem q) {
   __shared__ double lds[CG_THREADS / 64];
-  double* s = q.state;
+  const uint32_t bi = blockIdx.x;
+  const Bundle& g = q.bundles[bi];
+  double* s = bstate(q, bi);
   if (s[S_PCG_ACTIVE] == 0) return;
   const uint32_t tid = threadIdx.x;
   const double rho0 = s[S_RHO], r0 = s[S_R0];
@@ -860,7 +892,7 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_advance(Problem q) {
   double* p = vp(q);
   double* S = vs(q);
   double* z = vz(q);
-  double den = block_sum(schur_finish<SHARED>(q, p, S, lds), lds);
+  double den = block_sum(schur_finish<SHARED>(q, g, s, p, S, lds), lds);
   if (tid == 0) s[S_MATVECS] += 1;
   if (!(den > 0)) {
     if (tid == 0) {
@@ -878,15 +910,15 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_advance(Problem q) {
     return;
   }
   double res = 0, rho = 0;
-  const uint32_t n = q.C * 6;
-  for (uint32_t i = tid; i < n; i += CG_THREADS) {
+  const uint32_t r_lo = 6 * g.cam0, r_hi = 6 * g.cam1;
+  for (uint32_t i = r_lo + tid; i < r_hi; i += CG_THREADS) {
     x[i] += alpha * p[i];
     const double ri = r[i] - alpha * S[i];
     r[i] = ri;
     res += ri * ri;
   }
   if (SHARED && tid == 0) {
-    const uint32_t f = q.m - 1;
+    const uint32_t f = g.focal_row;
     x[f] += alpha * p[f];
     r[f] -= alpha * S[f];
     res += r[f] * r[f];
@@ -894,7 +926,7 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_advance(Problem q) {
     rho += z[f] * r[f];
   }
   __syncthreads();  // every row of r before the block products
-  for (uint32_t i = tid; i < n; i += CG_THREADS) {
+  for (uint32_t i = r_lo + tid; i < r_hi; i += CG_THREADS) {
     const double zi = block_row(q.camInv, i, r);
     z[i] = zi;
     rho += zi * r[i];
@@ -921,24 +953,26 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_advance(Problem q) {
     return;
   }
   const double beta = rho / rho0;
-  for (uint32_t i = tid; i < n; i += CG_THREADS) p[i] = z[i] + beta * p[i];
-  if (SHARED && tid == 0) p[q.m - 1] = z[q.m - 1] + beta * p[q.m - 1];
+  for (uint32_t i = r_lo + tid; i < r_hi; i += CG_THREADS) p[i] = z[i] + beta * p[i];
+  if (SHARED && tid == 0) p[g.focal_row] = z[g.focal_row] + beta * p[g.focal_row];
   if (tid == 0) s[S_RHO] = rho;
 }
 
-// The linear-system probe: S v, M^-1 v and the RHS of the assembled system.
+// The linear-system probe (bundle 0): S v, M^-1 v and the RHS of the assembled system.
 template <bool SHARED>
 __global__ __launch_bounds__(CG_THREADS) void k_probe(Problem q, const double* v, double* sv,
                                                       double* zv, double* rhs) {
   __shared__ double lds[CG_THREADS / 64];
-  (void)schur_finish<SHARED>(q, v, sv, lds);
-  for (uint32_t row = threadIdx.x; row < q.C * 6; row += CG_THREADS) {
+  const Bundle& g = q.bundles[0];
+  const double* s = bstate(q, 0);
+  (void)schur_finish<SHARED>(q, g, s, v, sv, lds);
+  for (uint32_t row = 6 * g.cam0 + threadIdx.x; row < 6 * g.cam1; row += CG_THREADS) {
     zv[row] = block_row(q.camInv, row, v);
     rhs[row] = q.camRhs[row];
   }
   if (SHARED && threadIdx.x == 0) {
-    zv[q.m - 1] = q.state[S_FOCAL_INV] * v[q.m - 1];
-    rhs[q.m - 1] = q.state[S_FOCAL_RHS];
+    zv[g.focal_row] = s[S_FOCAL_INV] * v[g.focal_row];
+    rhs[g.focal_row] = s[S_FOCAL_RHS];
   }
 }
 
@@ -948,13 +982,20 @@ __global__ __launch_bounds__(CG_THREADS) void k_probe(Problem q, const double* v
 // from storage: (W^T v)_j = w (Jp_u[j] (Jc_u . v_c + Jf_u v_f) + Jp_v[j] (Jc_v . v_c + Jf_v v_f))
 // and (W y)_k = w (Jc_u[k] (Jp_u . y) + Jc_v[k] (Jp_v . y)). The mixed form computes its
 // refinement residual b - S x with this operator, so FP64 cross blocks never need storing.
+// `guard`: skip bundles whose PCG is inactive (the PCG use); the refinement runs regardless.
 template <bool SHARED>
 __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_sweep_implicit(Problem q,
-                                                                          const double* v) {
+                                                                          const double* v,
+                                                                          int guard) {
   __shared__ double st_all[TILE_WAVES][TILE][3];
-  const uint32_t wave = threadIdx.x / TILE, lane = threadIdx.x % TILE;
+  // the wave's index is uniform across its lanes: keep the tile's loads and branches scalar
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / TILE), lane = threadIdx.x % TILE;
   const uint32_t t = blockIdx.x * TILE_WAVES + wave;
   if (t >= q.n_tiles) return;
+  const uint32_t b = q.tile_bundle[t];
+  const double* sb = bstate(q, b);
+  if (guard && sb[S_PCG_ACTIVE] == 0) return;
+  const Bundle& g = q.bundles[b];
   auto& st = st_all[wave];
   const uint32_t fp = q.tiles[t], ep = q.tiles[t + 1];
   const uint32_t* ps = point_start(q);
@@ -962,8 +1003,8 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_sweep_implicit(Probl
   const uint32_t my = fp + lane;
   const bool has = my < ep;
   const uint32_t mys = has ? ps[my] : 0, mye = has ? ps[my + 1] : 0;
-  const double vf = SHARED ? v[q.m - 1] : 0.0;
-  const double fscale = SHARED ? focal_scale(q.state[S_LOG_FOCAL]) : 1.0;
+  const double vf = SHARED ? v[g.focal_row] : 0.0;
+  const double fscale = SHARED ? focal_scale(sb[S_LOG_FOCAL]) : 1.0;
   double acc[3] = {0, 0, 0};
   for (uint32_t base = s; base < e; base += TILE) {
     const uint32_t i = base + lane;
@@ -971,7 +1012,7 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_sweep_implicit(Probl
       const uint32_t c = obs_camera(q)[i], p = obs_point(q)[i];
       Lin L;
       double tj[3] = {0, 0, 0};
-      if (linearize(q, c, p, i, fscale, variable_camera(q, c), L)) {
+      if (linearize(q, c, p, i, fscale, variable_camera(q, c), directional_point(g, p), L)) {
         double a0 = 0, a1 = 0;
         for (int k = 0; k < 6; k++) {
           a0 += L.Jc[k] * v[c * 6 + k];
@@ -1005,20 +1046,24 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_sweep_implicit(Probl
 }
 
 template <bool SHARED>
-__global__ __launch_bounds__(TASK_THREADS) void k_camera_sweep_implicit(Problem q) {
+__global__ __launch_bounds__(TASK_THREADS) void k_camera_sweep_implicit(Problem q, int guard) {
   constexpr int NV = SHARED ? 7 : 6;
   __shared__ double red[TASK_THREADS / 64][NV];
   const uint32_t t = blockIdx.x;
-  const uint32_t c = q.tasks[3 * t], s = q.tasks[3 * t + 1], e = q.tasks[3 * t + 2];
+  const uint32_t c = q.tasks[4 * t], s = q.tasks[4 * t + 1], e = q.tasks[4 * t + 2];
+  const uint32_t b = q.camera_bundle[c];
+  const double* sb = bstate(q, b);
+  if (guard && sb[S_PCG_ACTIVE] == 0) return;
+  const Bundle& g = q.bundles[b];
   const bool var = variable_camera(q, c);
-  const double fscale = SHARED ? focal_scale(q.state[S_LOG_FOCAL]) : 1.0;
+  const double fscale = SHARED ? focal_scale(sb[S_LOG_FOCAL]) : 1.0;
   const double* y = vy(q);
   double acc[NV];
   for (int k = 0; k < NV; k++) acc[k] = 0;
   for (uint32_t n = s + threadIdx.x; n < e; n += TASK_THREADS) {
     const uint32_t i = camera_obs(q)[n], p = q.cam_point[n];
     Lin L;
-    if (!linearize(q, c, p, i, fscale, var, L)) continue;
+    if (!linearize(q, c, p, i, fscale, var, directional_point(g, p), L)) continue;
     const double y0 = y[(size_t)p * 3], y1 = y[(size_t)p * 3 + 1], y2 = y[(size_t)p * 3 + 2];
     const double u0 = L.w * (L.Jp[0] * y0 + L.Jp[1] * y1 + L.Jp[2] * y2);
     const double u1 = L.w * (L.Jp[3] * y0 + L.Jp[4] * y1 + L.Jp[5] * y2);
@@ -1038,33 +1083,36 @@ __global__ __launch_bounds__(TASK_THREADS) void k_camera_sweep_implicit(Problem 
   }
 }
 
-// The refinement step of the mixed form, after the implicit sweeps of x: the FP64 residual
-// r = b - S x replaces the recurrence residual. When |r|^2 <= refine_tol^2 |r0|^2 the solve is
-// done (words 22, 24); otherwise PCG restarts from x with z = M^-1 r, p = z (words 18, 19).
-// Runs only after a converged PCG pass (word 24 set, word 20 clear).
+// The refinement step of the mixed form, per bundle, after the implicit sweeps of x: the FP64
+// residual r = b - S x replaces the recurrence residual. When |r|^2 <= refine_tol^2 |r0|^2 the
+// solve is done (words 22, 24); otherwise PCG restarts from x with z = M^-1 r, p = z (18, 19).
+// Runs only for a bundle whose PCG pass converged (word 24 set, word 20 clear).
 template <bool SHARED>
 __global__ __launch_bounds__(CG_THREADS) void k_refine(Problem q) {
   __shared__ double lds[CG_THREADS / 64];
-  double* s = q.state;
+  const uint32_t bi = blockIdx.x;
+  const Bundle& g = q.bundles[bi];
+  double* s = bstate(q, bi);
   const double converged = s[S_CONVERGED], failed = s[S_PCG_FAILED], r0 = s[S_R0];
   const double active = s[S_ACTIVE];
   if (active == 0 || converged == 0 || failed != 0) return;
-  const uint32_t tid = threadIdx.x, n = q.C * 6;
+  const uint32_t tid = threadIdx.x;
   double* x = vx(q);
   double* r = vr(q);
   double* p = vp(q);
   double* S = vs(q);
   double* z = vz(q);
-  (void)schur_finish<SHARED>(q, x, S, lds);
+  (void)schur_finish<SHARED>(q, g, s, x, S, lds);
   __syncthreads();
+  const uint32_t r_lo = 6 * g.cam0, r_hi = 6 * g.cam1;
   double res = 0;
-  for (uint32_t i = tid; i < n; i += CG_THREADS) {
+  for (uint32_t i = r_lo + tid; i < r_hi; i += CG_THREADS) {
     const double ri = q.camRhs[i] - S[i];
     r[i] = ri;
     res += ri * ri;
   }
   if (SHARED && tid == 0) {
-    const uint32_t f = q.m - 1;
+    const uint32_t f = g.focal_row;
     r[f] = s[S_FOCAL_RHS] - S[f];
     res += r[f] * r[f];
   }
@@ -1073,15 +1121,16 @@ __global__ __launch_bounds__(CG_THREADS) void k_refine(Problem q) {
     if (tid == 0) s[S_R] = res;
     return;
   }
+  __syncthreads();  // every row of r before the block products
   double rho = 0;
-  for (uint32_t i = tid; i < n; i += CG_THREADS) {
+  for (uint32_t i = r_lo + tid; i < r_hi; i += CG_THREADS) {
     const double zi = block_row(q.camInv, i, r);
     z[i] = zi;
     p[i] = zi;
     rho += zi * r[i];
   }
   if (SHARED && tid == 0) {
-    const uint32_t f = q.m - 1;
+    const uint32_t f = g.focal_row;
     z[f] = s[S_FOCAL_INV] * r[f];
     p[f] = z[f];
     rho += z[f] * r[f];
@@ -1103,12 +1152,14 @@ __global__ __launch_bounds__(CG_THREADS) void k_refine(Problem q) {
 
 // Cameras of the candidate: movePose for variable cameras, the trial log focal on the shared route.
 __global__ void k_update_cameras(Problem q) {
-  const double* s = q.state;
-  if (s[S_ACTIVE] == 0 || s[S_PCG_FAILED] != 0) return;
   const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
-  const double* x = vx(q);
-  if (h == 0 && q.shared) q.state[S_TRIAL_FOCAL] = s[S_LOG_FOCAL] + x[q.m - 1];
   if (h >= q.C) return;
+  const uint32_t b = q.camera_bundle[h];
+  double* s = bstate(q, b);
+  if (s[S_ACTIVE] == 0 || s[S_PCG_FAILED] != 0) return;
+  const Bundle& g = q.bundles[b];
+  const double* x = vx(q);
+  if (h == g.cam0 && q.shared) s[S_TRIAL_FOCAL] = s[S_LOG_FOCAL] + x[g.focal_row];
   double pose[12], moved[12];
   for (int j = 0; j < 12; j++) pose[j] = q.scene[(size_t)h * 12 + j];
   if (variable_camera(q, h)) {
@@ -1121,11 +1172,14 @@ __global__ void k_update_cameras(Problem q) {
   for (int j = 0; j < 12; j++) q.trial[(size_t)h * 12 + j] = moved[j];
 }
 
-// bundleCandidateGauge: reject an unconverged solve, check the focal proposal, the baseline, the
-// scale and every trial pose; on success set the gauge scale and the trial-valid word.
+// bundleCandidateGauge, per bundle: reject an unconverged solve, check the focal proposal, the
+// baseline, the scale and every trial pose; on success set the gauge scale and the trial-valid
+// word.
 __global__ __launch_bounds__(256) void k_gauge(Problem q) {
   __shared__ double lds[4];
-  double* s = q.state;
+  const uint32_t bi = blockIdx.x;
+  const Bundle& g = q.bundles[bi];
+  double* s = bstate(q, bi);
   const uint32_t tid = threadIdx.x;
   const double active = s[S_ACTIVE], failed = s[S_PCG_FAILED], r0 = s[S_R0], r = s[S_R];
   const double trial_focal = s[S_TRIAL_FOCAL];
@@ -1140,12 +1194,12 @@ __global__ __launch_bounds__(256) void k_gauge(Problem q) {
   if (q.shared) {
     if (!(fabs(trial_focal) < 700)) return;
     const double fs = focal_scale(trial_focal);
-    for (uint32_t c = tid; c < q.C; c += 256) {
+    for (uint32_t c = g.cam0 + tid; c < g.cam1; c += 256) {
       const double* m = q.scene + (size_t)q.C * 12 + (size_t)q.P * 3 + (size_t)c * 6;
       if (!(m[0] * fs >= 1e-6 && m[1] * fs >= 1e-6 && m[0] * fs <= 1e6 && m[1] * fs <= 1e6)) bad += 1;
     }
   }
-  for (uint32_t c = tid; c < q.C; c += 256) {
+  for (uint32_t c = g.cam0 + tid; c < g.cam1; c += 256) {
     const double* p = q.trial + (size_t)c * 12;
     for (int j = 0; j < 12; j++)
       if (!(fabs(p[j]) <= 1e9)) bad += 1;
@@ -1160,12 +1214,12 @@ __global__ __launch_bounds__(256) void k_gauge(Problem q) {
   bad = block_sum(bad, lds);
   if (tid != 0) return;
   double c[3];
-  center_of(q.trial + (size_t)q.scale_view * 12, c);
+  center_of(q.trial + (size_t)g.scale_view * 12, c);
   const double d0 = s[S_ORIGIN] - c[0], d1 = s[S_ORIGIN + 1] - c[1], d2 = s[S_ORIGIN + 2] - c[2];
   const double baseline = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
   if (!(baseline > 1e-12)) return;
   double scale = 1;
-  if (q.normalize_scale) scale = s[S_BASELINE] / baseline;
+  if (g.normalize_scale) scale = s[S_BASELINE] / baseline;
   if (!(scale > 1e-9 && scale < 1e9)) return;
   if (bad > 0) return;
   s[S_GAUGE] = scale;
@@ -1173,62 +1227,79 @@ __global__ __launch_bounds__(256) void k_gauge(Problem q) {
 }
 
 // normalizeBundleCandidate: unit directional points; the similarity X' = o + s (X - o),
-// t' = s t + (s - 1) R o about the gauge origin when one camera is fixed.
+// t' = s t + (s - 1) R o about the gauge origin when one camera is fixed. Thread h handles point h
+// and camera h.
 __global__ void k_normalize(Problem q) {
-  const double* s = q.state;
-  if (s[S_ACTIVE] == 0 || s[S_TRIAL_VALID] == 0) return;
   const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
-  const double g = s[S_GAUGE];
-  const double o[3] = {s[S_ORIGIN], s[S_ORIGIN + 1], s[S_ORIGIN + 2]};
-  if (h < q.P && directional_point(q, h)) {
-    double* d = q.trial + (size_t)q.C * 12 + (size_t)h * 3;
-    const double norm = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-    if (norm > 1e-12) {
-      d[0] = d[0] / norm;
-      d[1] = d[1] / norm;
-      d[2] = d[2] / norm;
+  if (h < q.P) {
+    const uint32_t b = q.point_bundle[h];
+    const Bundle& g = q.bundles[b];
+    const double* s = bstate(q, b);
+    if (s[S_ACTIVE] != 0 && s[S_TRIAL_VALID] != 0) {
+      double* X = q.trial + (size_t)q.C * 12 + (size_t)h * 3;
+      if (directional_point(g, h)) {
+        const double norm = sqrt(X[0] * X[0] + X[1] * X[1] + X[2] * X[2]);
+        if (norm > 1e-12) {
+          X[0] = X[0] / norm;
+          X[1] = X[1] / norm;
+          X[2] = X[2] / norm;
+        }
+      } else if (g.normalize_scale) {
+        const double sc = s[S_GAUGE];
+        for (int j = 0; j < 3; j++) X[j] = s[S_ORIGIN + j] + sc * (X[j] - s[S_ORIGIN + j]);
+      }
     }
   }
-  if (!q.normalize_scale) return;
-  if (h < q.P && !directional_point(q, h)) {
-    double* X = q.trial + (size_t)q.C * 12 + (size_t)h * 3;
-    for (int j = 0; j < 3; j++) X[j] = o[j] + g * (X[j] - o[j]);
-  }
-  if (h < q.C && h != q.anchor) {
-    double* p = q.trial + (size_t)h * 12;
-    double rot[3];
-    for (int r = 0; r < 3; r++)
-      rot[r] = (p[r * 3] * o[0] + p[r * 3 + 1] * o[1] + p[r * 3 + 2] * o[2] + p[9 + r]) - p[9 + r];
-    const double t0 = g * p[9] + (g - 1) * rot[0];
-    const double t1 = g * p[10] + (g - 1) * rot[1];
-    const double t2 = g * p[11] + (g - 1) * rot[2];
-    p[9] = t0;
-    p[10] = t1;
-    p[11] = t2;
+  if (h < q.C) {
+    const uint32_t b = q.camera_bundle[h];
+    const Bundle& g = q.bundles[b];
+    const double* s = bstate(q, b);
+    if (s[S_ACTIVE] != 0 && s[S_TRIAL_VALID] != 0 && g.normalize_scale && h != g.anchor) {
+      const double sc = s[S_GAUGE];
+      const double o[3] = {s[S_ORIGIN], s[S_ORIGIN + 1], s[S_ORIGIN + 2]};
+      double* p = q.trial + (size_t)h * 12;
+      double rot[3];
+      for (int r = 0; r < 3; r++)
+        rot[r] = (p[r * 3] * o[0] + p[r * 3 + 1] * o[1] + p[r * 3 + 2] * o[2] + p[9 + r]) - p[9 + r];
+      const double t0 = sc * p[9] + (sc - 1) * rot[0];
+      const double t1 = sc * p[10] + (sc - 1) * rot[1];
+      const double t2 = sc * p[11] + (sc - 1) * rot[2];
+      p[9] = t0;
+      p[10] = t1;
+      p[11] = t2;
+    }
   }
 }
 
-// commitBundleCandidate: the trial becomes the scene when accepted.
+// commitBundleCandidate: the trial becomes the scene for every bundle that accepted it.
 __global__ void k_commit(Problem q) {
-  if (q.state[S_COMMIT] == 0) return;
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t total = (size_t)q.C * 12 + (size_t)q.P * 3;
-  if (i < total) q.scene[i] = q.trial[i];
-  if (i == 0 && q.shared) q.state[S_LOG_FOCAL] = q.state[S_TRIAL_FOCAL];
+  const size_t cams = (size_t)q.C * 12, total = cams + (size_t)q.P * 3;
+  if (i >= total) return;
+  const uint32_t b = i < cams ? q.camera_bundle[i / 12] : q.point_bundle[(i - cams) / 3];
+  double* s = bstate(q, b);
+  if (s[S_COMMIT] == 0) return;
+  q.scene[i] = q.trial[i];
+  if (q.shared && i == (size_t)q.bundles[b].cam0 * 12) s[S_LOG_FOCAL] = s[S_TRIAL_FOCAL];
 }
 
-// packBundleResult: coordinates, state words, RMSEs (and the log focal scale).
+// packBundleResult: coordinates of the whole batch, then per bundle its state words with the
+// RMSEs (and the log focal scale).
 __global__ void k_pack(Problem q, double* out) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t total = (size_t)q.C * 12 + (size_t)q.P * 3;
+  const uint32_t words = 26 + q.shared;
   if (i < total) out[i] = q.scene[i];
-  if (i < 26) {
-    double v = q.state[i];
-    if (i == 24) v = sqrt(q.state[S_SSE0] / (double)(float)q.M);
-    if (i == 25) v = sqrt(q.state[S_SSE] / (double)(float)q.M);
+  if (i < (size_t)q.B * words) {
+    const uint32_t b = (uint32_t)(i / words), w = (uint32_t)(i % words);
+    const Bundle& g = q.bundles[b];
+    const double* s = bstate(q, b);
+    const double M = (double)(float)(g.obs1 - g.obs0);
+    double v = w < 26 ? s[w] : s[S_LOG_FOCAL];
+    if (w == 24) v = sqrt(s[S_SSE0] / M);
+    if (w == 25) v = sqrt(s[S_SSE] / M);
     out[total + i] = v;
   }
-  if (i == 0 && q.shared) out[total + 26] = q.state[S_LOG_FOCAL];
 }
 
 }  // namespace tcvba

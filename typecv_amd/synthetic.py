@@ -405,3 +405,21 @@ def trajectory_scene(cameras: int = 500, points: int = 200_000, seed: int = 2,
     packed = pack_arrays(poses, models, Xp, lengths, obs_camera, pixels, active)
     return packed, {"truth": {"R": R, "t": t_true, "X": X, "focal": focal}, "focal_factor": factor,
                     "outliers": int(bad.sum()), "anchor": 0, "scale_view": 1}
+
+
+def local_bundles(count: int = 256, seed: int = 4, cameras=(5, 40), points=(500, 20_000),
+                  track=(3, 12), **scene_options):
+    """`count` independent local bundles of uneven size: cameras ~ U{cameras}, points ~ U{points}
+    per bundle, each point seen by 3–12 cameras of its bundle (at most the bundle's cameras), the
+    trajectory generator's noise, outliers and focal error otherwise. Returns a list of
+    (PackedScene, info)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for b in range(count):
+        C = int(rng.integers(cameras[0], cameras[1] + 1))
+        P = int(rng.integers(points[0], points[1] + 1))
+        hi = min(track[1], C)
+        lo = min(track[0], hi)
+        out.append(trajectory_scene(cameras=C, points=P, seed=int(rng.integers(1 << 30)),
+                                    track=(lo, hi), **scene_options))
+    return out
