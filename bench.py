@@ -26,6 +26,15 @@ sys.path.insert(0, str(ROOT))
 METRIC = ("LM iters/sec + obs/sec, mixed vs FP64 at ≤1e-6 rel cost; Schur matvec HBM GB/s")
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes of one Schur product (point sweep + camera sweep) measured with
+# `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE` over `bench.py --mode probe`
+# (profiles/schur_trajectory500_fp64_traffic.md): FETCH_SIZE x 2.00 and WRITE_SIZE x 1.00, the
+# factors that make a 1 GiB device copy read and write 1 GiB. Keyed by
+# (cameras, points, seed, precision, operator); other workloads report null.
+MEASURED_TRAFFIC = {
+    (500, 200_000, 2, "fp64", "stored"): 2 * (185315.7 + 200277.6) * 1024 + (4879.8 + 124.6) * 1024,
+}
+
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -152,7 +161,10 @@ def main() -> int:
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": MEASURED_TRAFFIC.get((C, P, args.seed, args.precision,
+                                             "implicit" if args.implicit else "stored")),
+            "algorithmic_bytes": bytes_per,
             "kernel": "Schur matvec: k_point_sweep + k_camera_sweep",
         },
     }
@@ -176,5 +188,117 @@ def main() -> int:
     return 0
 
 
+def batched(args) -> int:
+    """256 independent local bundles (cameras U{5..40}, points U{500..20000}, each point seen by
+    3-12 cameras of its bundle; the trajectory scene's noise, outliers and focal error), shared
+    focal, Huber 1 px, 30 LM iterations, mixed precision, solved together by one
+    tcv_ba_solve_batch call per step. Reports bundles/s over the batches' LM loops and the p50/p99
+    of the per-bundle LM-loop latency."""
+    import numpy as np
+
+    from typecv_amd import bundle as B
+    from typecv_amd import synthetic as S
+
+    t0 = time.time()
+    bundles = S.local_bundles(count=args.bundles, seed=args.seed)
+    packeds = [p for p, _ in bundles]
+    M = sum(p.M for p in packeds)
+    log(f"{len(packeds)} bundles, {sum(p.C for p in packeds)} cameras, "
+        f"{sum(p.P for p in packeds)} points, {M} observations in {time.time() - t0:.1f}s")
+    controls = [B.make_control(p.C, p.P, p.M, shared_focal=True, huber=1.0,
+                               iterations=args.iterations, max_cg=min(256, max(32, 12 * p.C + 1)),
+                               precision=args.precision) for p in packeds]
+    # the bundles' default budgets differ (12C+1); a batch shares one budget: the largest
+    budget = max(c.max_cg for c in controls)
+    for c in controls:
+        c.max_cg = budget
+    device = B.HipDevice(0)
+    for k in range(args.warmup):
+        B.solve_batch(device, controls, packeds)
+    loops, walls, latencies, lm = [], [], [], 0
+    for k in range(args.steps):
+        t = time.perf_counter()
+        outs, report, finish = B.solve_batch(device, controls, packeds)
+        walls.append(time.perf_counter() - t)
+        loops.append(float(finish.max()) * 1e-3)
+        latencies.extend(finish.tolist())
+        lm += report["lm_iterations"]
+        log(f"step {k}: wall {walls[-1] * 1e3:.1f} ms, LM loop {loops[-1] * 1e3:.1f} ms")
+    device.dispose()
+    lat = np.array(latencies)
+    result = {
+        "metric": "batched local bundles: bundles/s, p50/p99 per-bundle LM-loop latency",
+        "value": len(packeds) * args.steps / sum(loops),
+        "unit": "bundles/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * sum(loops) / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32-products/f64-accumulation" if args.precision == "mixed" else "f64",
+        "data": "synthetic (seeded local bundles)",
+        "config": {"workload": "batched local bundles", "bundles": len(packeds),
+                   "observations": M, "lm_iterations_max": args.iterations,
+                   "precision": args.precision, "max_cg": budget},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                       "max": float(lat.max())},
+        "end_to_end_bundles_per_s": len(packeds) * args.steps / sum(walls),
+        "lm_iterations_per_s": lm / sum(loops),
+        "observations_per_s": M * (lm / len(packeds)) / (sum(loops) / args.steps) / args.steps
+        if args.steps else 0.0,
+    }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+def probe(args) -> int:
+    """Counter run: a 1 GiB device copy (the calibration kernel of known size), then `--repeat`
+    Schur products of the workload through tcv_ba_schur_probe, for `rocprofv3 --pmc FETCH_SIZE` /
+    `WRITE_SIZE` runs. Prints the HIP-event time per sweep."""
+    import numpy as np
+    import torch
+
+    from typecv_amd import bundle as B
+    from typecv_amd import synthetic as S
+
+    packed, _ = S.trajectory_scene(cameras=args.cameras, points=args.points, seed=args.seed)
+    src = torch.empty(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
+    src.fill_(1.0)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    control = B.make_control(packed.C, packed.P, packed.M, shared_focal=True, huber=1.0,
+                             iterations=1, max_cg=256, precision=args.precision, timing=True)
+    m = 6 * packed.C + 1
+    vec = np.random.default_rng(1).normal(size=m)
+    with B.HipDevice(0) as device:
+        _, _, _, status, rep = B.schur_probe(device, control, packed, 1e-3, 0.0, vec,
+                                             repeat=args.repeat)
+    n = max(1, rep["matvecs"])
+    print(json.dumps({"mode": "probe", "observations": packed.M, "status": status,
+                      "point_sweep_us": rep["schur_point_ms"] / n * 1e3,
+                      "camera_sweep_us": rep["schur_camera_ms"] / n * 1e3,
+                      "copy_bytes": 2 * src.numel() * 8}), flush=True)
+    return 0
+
+
 if __name__ == "__main__":
+    if "--mode" in sys.argv and "probe" in sys.argv:
+        ap = argparse.ArgumentParser()
+        ap.add_argument("--mode")
+        ap.add_argument("--cameras", type=int, default=500)
+        ap.add_argument("--points", type=int, default=200_000)
+        ap.add_argument("--seed", type=int, default=2)
+        ap.add_argument("--repeat", type=int, default=10)
+        ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
+        sys.exit(probe(ap.parse_args()))
+    if "--workload" in sys.argv and "batched" in sys.argv:
+        ap = argparse.ArgumentParser()
+        ap.add_argument("--workload")
+        ap.add_argument("--gpus", type=int, default=1)
+        ap.add_argument("--steps", type=int, default=2)
+        ap.add_argument("--warmup", type=int, default=1)
+        ap.add_argument("--bundles", type=int, default=256)
+        ap.add_argument("--iterations", type=int, default=30)
+        ap.add_argument("--precision", choices=("fp64", "mixed"), default="mixed")
+        ap.add_argument("--seed", type=int, default=4)
+        sys.exit(batched(ap.parse_args()))
     sys.exit(main())
