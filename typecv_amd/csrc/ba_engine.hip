@@ -51,14 +51,15 @@ struct tcv_ba_context {
   std::vector<DevBuf*> all;
   DevBuf scene, trial, index, out, bundles, tiles, tile_bundle, tasks, task_start, cost_blocks,
       camera_bundle, point_bundle, cam_point, Wp, Wc, pblk, tile_fail, tile_focal, task_part, camA,
-      camInv, camRhs, camHcf, camFocal, vec, state, cost_part, error, probe;
+      camInv, camRhs, camHcf, camFocal, vec, state, cost_part, error, probe, groups, group_part;
   std::vector<hipEvent_t> events;
   tcv_ba_context() {
     all = {&scene,      &trial,        &index,     &out,       &bundles,    &tiles,
            &tile_bundle, &tasks,       &task_start, &cost_blocks, &camera_bundle,
            &point_bundle, &cam_point,  &Wp,        &Wc,        &pblk,       &tile_fail,
            &tile_focal, &task_part,    &camA,      &camInv,    &camRhs,     &camHcf,
-           &camFocal,   &vec,          &state,     &cost_part, &error,      &probe};
+           &camFocal,   &vec,          &state,     &cost_part, &error,      &probe,
+           &groups,     &group_part};
   }
 };
 
@@ -136,7 +137,7 @@ bool same_options(const tcv_ba_control& a, const tcv_ba_control& b) {
 struct Plan {
   std::vector<Bundle> bundles;
   std::vector<uint32_t> tiles, tile_bundle, tasks, task_start, cost_blocks, camera_bundle,
-      point_bundle;
+      point_bundle, groups;
 };
 
 constexpr uint32_t kTaskObs = 1024;
@@ -218,6 +219,14 @@ int build_plan(const std::vector<uint32_t>& ps, const std::vector<uint32_t>& cs,
       } while (at < e);
     }
     g.task1 = (uint32_t)(plan.tasks.size() / 4);
+    // PCG-step groups of at most GROUP_CAMS cameras
+    g.group0 = (uint32_t)(plan.groups.size() / 3);
+    for (uint32_t c = g.cam0; c < g.cam1; c += GROUP_CAMS) {
+      plan.groups.push_back(b);
+      plan.groups.push_back(c);
+      plan.groups.push_back(std::min<uint32_t>(c + GROUP_CAMS, g.cam1));
+    }
+    g.group1 = (uint32_t)(plan.groups.size() / 3);
     for (uint32_t p = g.pt0; p < g.pt1; p++) plan.point_bundle[p] = b;
     // cost blocks
     g.cost0 = (uint32_t)(plan.cost_blocks.size() / 3);
@@ -265,7 +274,9 @@ struct Launch {
     hipLaunchKernelGGL((k_camera_sweep<WT, SHARED>), dim3(q.n_tasks), dim3(TASK_THREADS), 0, s, q);
   }
   static void advance(const Problem& q, hipStream_t s) {
-    hipLaunchKernelGGL((k_cg_advance<SHARED>), dim3(q.B), dim3(CG_THREADS), 0, s, q);
+    hipLaunchKernelGGL((k_cg_product<SHARED>), dim3(q.n_groups), dim3(ADV_THREADS), 0, s, q);
+    hipLaunchKernelGGL((k_cg_step<SHARED>), dim3(q.n_groups), dim3(ADV_THREADS), 0, s, q);
+    hipLaunchKernelGGL((k_cg_direction<SHARED>), dim3(q.n_groups), dim3(ADV_THREADS), 0, s, q);
   }
   static void update_points(const Problem& q, hipStream_t s) {
     hipLaunchKernelGGL((k_point_sweep<WT, SHARED, SWEEP_UPDATE>), dim3(tile_blocks(q)),
@@ -373,6 +384,7 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
   const uint32_t n_tiles = (uint32_t)plan.tiles.size() - 1;
   const uint32_t n_tasks = (uint32_t)plan.tasks.size() / 4;
   const uint32_t n_cost = (uint32_t)plan.cost_blocks.size() / 3;
+  const uint32_t n_groups = (uint32_t)plan.groups.size() / 3;
   const size_t w_words = (size_t)K * (((size_t)M + 63) / 64) * 64;
   const size_t vec_words = (size_t)m * 5 + (size_t)P * 3;
   TRY(ensure(ctx->scene, scene_size * 8));
@@ -393,6 +405,7 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
   TRY(ensure(ctx->state, (size_t)B * S_WORDS * 8));
   TRY(ensure(ctx->cost_part, (size_t)n_cost * 3 * 8));
   TRY(ensure(ctx->error, 8));
+  TRY(ensure(ctx->group_part, (size_t)n_groups * 4 * 8));
   TRY(ensure_host_state(ctx, (size_t)B * S_WORDS));
   TRY(upload(ctx->bundles, plan.bundles, st));
   TRY(upload(ctx->tiles, plan.tiles, st));
@@ -402,6 +415,7 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
   TRY(upload(ctx->cost_blocks, plan.cost_blocks, st));
   TRY(upload(ctx->camera_bundle, plan.camera_bundle, st));
   TRY(upload(ctx->point_bundle, plan.point_bundle, st));
+  TRY(upload(ctx->groups, plan.groups, st));
   HIP_TRY(hipMemcpyAsync(ctx->scene.ptr, scene, scene_size * 8, hipMemcpyDeviceToDevice, st));
   HIP_TRY(hipMemsetAsync(ctx->trial.ptr, 0, trial_size * 8, st));
   HIP_TRY(hipMemsetAsync(ctx->vec.ptr, 0, vec_words * 8, st));
@@ -418,6 +432,7 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
   q.n_tiles = n_tiles;
   q.n_tasks = n_tasks;
   q.n_cost_blocks = n_cost;
+  q.n_groups = n_groups;
   q.huber = c->huber;
   q.tol = c->relative_tolerance;
   q.refine_tol = c->refinement_tolerance > 0 ? c->refinement_tolerance : c->relative_tolerance;
@@ -432,6 +447,8 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
   q.cost_blocks = (const uint32_t*)ctx->cost_blocks.ptr;
   q.camera_bundle = (const uint32_t*)ctx->camera_bundle.ptr;
   q.point_bundle = (const uint32_t*)ctx->point_bundle.ptr;
+  q.groups = (const uint32_t*)ctx->groups.ptr;
+  q.group_part = (double*)ctx->group_part.ptr;
   q.cam_point = (uint32_t*)ctx->cam_point.ptr;
   q.Wp = ctx->Wp.ptr;
   q.Wc = ctx->Wc.ptr;

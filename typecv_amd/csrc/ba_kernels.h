@@ -43,6 +43,7 @@ enum : int {
   S_FOCAL_RHS = 30,      // reduced focal RHS (focal[6C+2])
   S_MATVECS = 31,        // Schur products that ran while PCG was active (whole solve)
   S_REFINEMENTS = 32,    // mixed form: refinement restarts taken (whole solve)
+  S_RHO_NEXT = 33,       // rho of the next PCG iteration (becomes word 18 at its first launch)
   S_WORDS = 40
 };
 
@@ -58,6 +59,8 @@ constexpr int TILE = 64;  // observations per wave tile
 #endif
 constexpr int TILE_WAVES = TCV_TILE_WAVES;  // wave tiles per workgroup of the point-tile kernels
 constexpr int TASK_THREADS = 256;  // threads of a camera-task workgroup
+constexpr int ADV_THREADS = 256;   // threads of a PCG-step workgroup
+constexpr int GROUP_CAMS = 42;     // cameras of a PCG-step workgroup (6 rows each)
 constexpr int CG_THREADS = 1024;   // threads of the per-bundle PCG kernels
 constexpr int COST_OBS = 256;      // observations per cost block
 constexpr int ASM_WORDS = 56;      // per-task camera assembly sums (see k_camera_pass)
@@ -70,12 +73,13 @@ struct Bundle {
   uint32_t anchor, scale_view;  // global camera indices
   uint32_t normalize_scale;
   uint32_t focal_row;  // 6C + b
+  uint32_t group0, group1;  // the bundle's PCG-step groups
   uint32_t pad;
 };
 
 struct Problem {
   uint32_t C, P, M, B, m, K, shared;
-  uint32_t n_tiles, n_tasks, n_cost_blocks;
+  uint32_t n_tiles, n_tasks, n_cost_blocks, n_groups;
   double huber, tol;
   double refine_tol;  // mixed form: relative FP64 residual of the outer refinement
   double* scene;
@@ -89,6 +93,7 @@ struct Problem {
   const uint32_t* cost_blocks;    // 3 per block: bundle, first and end observation
   const uint32_t* camera_bundle;  // C
   const uint32_t* point_bundle;   // P
+  const uint32_t* groups;         // 3 per PCG-step group: bundle, first and end camera
   uint32_t* cam_point;
   void* Wp;
   void* Wc;
@@ -101,6 +106,7 @@ struct Problem {
   double* camRhs;      // 6 per camera
   double* camHcf;      // 6 per camera
   double* camFocal;    // 2 per camera: H_ff and b_f shares
+  double* group_part;  // 4 per PCG-step group: p . S p, focal share, |r|^2, r . z
   double* vec;         // x r p s z (m each), y (3P)
   double* state;       // S_WORDS per bundle
   double* cost_part;   // 3 per cost block

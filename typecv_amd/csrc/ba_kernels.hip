@@ -16,7 +16,7 @@
 //   k_init_cg (per bundle)           initializeBundleCg b:695-741 / j:646-700 with dampBundleFocal
 //   k_point_sweep<SCHUR>             bundleSchurPoint b:742-745 (body b:185-203, j:198-218)
 //   k_camera_sweep                   bundleSchurCamera* b:746-882 / j:705-870 (camera partials)
-//   k_cg_advance (per bundle)        finishes S·p (A p, focal row) then advanceBundleCg b:883-929
+//   k_cg_product / k_cg_step / k_cg_direction   finish S·p, then advanceBundleCg b:883-929
 //   k_update_cameras / k_point_sweep<UPDATE>   updateBundleCandidate b:930-961 / j:925-960
 //   k_gauge (per bundle)             bundleCandidateGauge b:962-993 / j:961-1000
 //   k_normalize                      normalizeBundleCandidate b:994-1023
@@ -59,6 +59,27 @@ __device__ __forceinline__ double block_sum(double v, double* lds) {
   double s = 0;
   for (int w = 0; w < waves; w++) s += lds[w];
   return s;
+}
+
+// block_sum of two values with one LDS round: the same sums, in the same order, as two block_sum
+// calls. `lds` holds two doubles per wave.
+__device__ __forceinline__ void block_sum2(double& a, double& b, double* lds) {
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_xor(a, o, 64);
+    b += __shfl_xor(b, o, 64);
+  }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, waves = blockDim.x >> 6;
+  __syncthreads();
+  if (lane == 0) {
+    lds[wave] = a;
+    lds[waves + wave] = b;
+  }
+  __syncthreads();
+  double sa = 0, sb = 0;
+  for (int w = 0; w < waves; w++) sa += lds[w];
+  for (int w = 0; w < waves; w++) sb += lds[waves + w];
+  a = sa;
+  b = sb;
 }
 
 // Orders one wave's LDS writes before its later LDS reads. The point-tile kernels give every wave
@@ -652,7 +673,7 @@ __device__ __forceinline__ double block_row(const double* blocks, uint32_t row, 
 // |r0|^2.
 template <bool SHARED>
 __global__ __launch_bounds__(CG_THREADS) void k_init_cg(Problem q) {
-  __shared__ double lds[CG_THREADS / 64];
+  __shared__ double lds[2 * (CG_THREADS / 64)];
   const uint32_t bi = blockIdx.x;
   const Bundle& g = q.bundles[bi];
   double* s = bstate(q, bi);
@@ -728,6 +749,7 @@ __global__ __launch_bounds__(CG_THREADS) void k_init_cg(Problem q) {
   res = block_sum(res, lds);
   if (tid == 0) {
     s[S_RHO] = rho;
+    s[S_RHO_NEXT] = rho;
     s[S_R0] = res;
     s[S_R] = res;
     if (res > 0 && rho > 0)
@@ -842,127 +864,233 @@ __global__ __launch_bounds__(TASK_THREADS) void k_camera_sweep(Problem q) {
   }
 }
 
-// Finish S·v of bundle g from the camera partials: (S v)_c = A_c v_c [+ H_cf v_f] - sum_tasks
-// part_c, and the focal row H_ff v_f + sum_c H_cf . v_c - sum_tasks part[6]. Returns this thread's
-// share of v . S v. Uniform control flow (uses block_sum).
+// The camera's sums of the camera sweep: sum over its tasks of part[0..NV-1], in task order.
+template <int NV>
+__device__ __forceinline__ void camera_sums(const Problem& q, uint32_t c, double part[NV]) {
+#pragma unroll
+  for (int j = 0; j < NV; j++) part[j] = 0;
+  for (uint32_t t = q.task_start[c]; t < q.task_start[c + 1]; t++)
+#pragma unroll
+    for (int j = 0; j < NV; j++) part[j] += q.task_part[(size_t)t * 8 + j];
+}
+
+// Finish S·v of bundle g from the camera-task partials, one thread per camera:
+//   (S v)_c    = A_c v_c [+ H_cf v_f] - sum of the camera's task partials [0..5]
+//   focal row  = H_ff v_f + sum_c (H_cf . v_c - sum of the camera's task partials [6])
+// Returns v . S v, the same value in every thread. Uniform control flow. Used outside the PCG loop
+// (the refinement residual and the probe).
 template <bool SHARED>
 __device__ double schur_finish(const Problem& q, const Bundle& g, const double* s,
                                const double* v, double* sv, double* lds) {
+  constexpr int NV = SHARED ? 7 : 6;
   const uint32_t tid = threadIdx.x;
   const double vf = SHARED ? v[g.focal_row] : 0.0;
   double dot = 0, fsum = 0;
-  for (uint32_t row = 6 * g.cam0 + tid; row < 6 * g.cam1; row += CG_THREADS) {
-    const uint32_t c = row / 6, j = row - c * 6;
-    double sum = 0;
-    for (uint32_t t = q.task_start[c]; t < q.task_start[c + 1]; t++)
-      sum += q.task_part[(size_t)t * 8 + j];
-    double a = SHARED ? q.camHcf[row] * vf : 0.0;
-    a += block_row(q.camA, row, v);
-    a -= sum;
-    sv[row] = a;
-    dot += v[row] * a;
-    if (SHARED) fsum += q.camHcf[row] * v[row];
-  }
-  if (SHARED) {
-    for (uint32_t t = g.task0 + tid; t < g.task1; t += CG_THREADS)
-      fsum -= q.task_part[(size_t)t * 8 + 6];
-    fsum = block_sum(fsum, lds);
-    const double sf = s[S_FOCAL_DIAG] * vf + fsum;
-    if (tid == 0) {
-      sv[g.focal_row] = sf;
-      dot += vf * sf;
+  for (uint32_t c = g.cam0 + tid; c < g.cam1; c += blockDim.x) {
+    const size_t o = 6 * (size_t)c;
+    double part[NV];
+    camera_sums<NV>(q, c, part);
+    const double* A = q.camA + (size_t)c * 36;
+    double fsh = 0;
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+      double a = SHARED ? q.camHcf[o + j] * vf : 0.0, av = 0;
+#pragma unroll
+      for (int k = 0; k < 6; k++) av += A[j * 6 + k] * v[o + k];
+      a += av;
+      a -= part[j];
+      sv[o + j] = a;
+      dot += v[o + j] * a;
+      if (SHARED) fsh += q.camHcf[o + j] * v[o + j];
     }
+    if (SHARED) fsum += fsh - part[NV - 1];
   }
-  return dot;
+  if (!SHARED) return block_sum(dot, lds);
+  block_sum2(dot, fsum, lds);
+  const double sf = s[S_FOCAL_DIAG] * vf + fsum;
+  if (tid == 0) sv[g.focal_row] = sf;
+  return dot + vf * sf;
 }
 
-// advanceBundleCg b:883-929 / advancePcg(false) j:874-924, one workgroup per bundle, after
-// finishing S p.
+// One PCG iteration after the camera sweep (the rest of bundleSchurCameraRow b:204-225 and
+// advanceBundleCg b:883-929 / advancePcg(false) j:874-924) in three launches over groups of at most
+// GROUP_CAMS cameras of one bundle, one thread per row. No workgroup waits for another: each launch
+// reduces the previous launch's per-group partials itself, in one fixed order, so every workgroup of
+// a bundle derives the same scalars and takes the same branch; the bundle's first group writes the
+// state words.
+//   k_cg_product    (S p)_c = A_c p_c [+ H_cf p_f] - the camera's task partials; per group p . S p
+//                   and the focal shares H_cf . p_c - part_c[6]
+//   k_cg_step       focal row, p . S p, alpha and its tests; x += alpha p, r -= alpha S p,
+//                   z = M^-1 r (the camera's 6x6 block); per group |r|^2 and r . z
+//   k_cg_direction  |r|^2 and r . z, the convergence and breakdown tests, beta, p = z + beta p
+// rho of the next iteration goes to word S_RHO_NEXT (the launch still reads word 18) and becomes
+// word 18 at the next k_cg_product.
+struct Group {
+  uint32_t bundle, c0, c1, row, rows;
+  bool has, first;
+};
+
+__device__ __forceinline__ Group group_of(const Problem& q) {
+  Group G;
+  const uint32_t gi = blockIdx.x;
+  G.bundle = q.groups[3 * gi];
+  G.c0 = q.groups[3 * gi + 1];
+  G.c1 = q.groups[3 * gi + 2];
+  G.rows = 6 * (G.c1 - G.c0);
+  G.row = 6 * G.c0 + threadIdx.x;
+  G.has = threadIdx.x < G.rows;
+  G.first = gi == q.bundles[G.bundle].group0;
+  return G;
+}
+
+// Sum over the bundle's groups of words w and w+1 of group_part (4 per group), in group order.
+__device__ __forceinline__ void group_sums(const Problem& q, const Bundle& g, int w, double& a,
+                                           double& b, double* lds) {
+  a = 0;
+  b = 0;
+  for (uint32_t k = g.group0 + threadIdx.x; k < g.group1; k += blockDim.x) {
+    a += q.group_part[4 * (size_t)k + w];
+    b += q.group_part[4 * (size_t)k + w + 1];
+  }
+  block_sum2(a, b, lds);
+}
+
 template <bool SHARED>
-__global__ __launch_bounds__(CG_THREADS) void k_cg_advance(Problem q) {
-  __shared__ double lds[CG_THREADS / 64];
-  const uint32_t bi = blockIdx.x;
-  const Bundle& g = q.bundles[bi];
-  double* s = bstate(q, bi);
+__global__ __launch_bounds__(ADV_THREADS) void k_cg_product(Problem q) {
+  constexpr int NV = SHARED ? 7 : 6;
+  __shared__ double lds[2 * (ADV_THREADS / 64)];
+  const Group G = group_of(q);
+  double* s = bstate(q, G.bundle);
   if (s[S_PCG_ACTIVE] == 0) return;
+  const Bundle& g = q.bundles[G.bundle];
+  const double* p = vp(q);
+  double d = 0, fs = 0;
+  if (G.has) {
+    const uint32_t c = G.row / 6, j = G.row - 6 * c;
+    double part[NV];
+    camera_sums<NV>(q, c, part);
+    const double pf = SHARED ? p[g.focal_row] : 0.0;
+    const double* A = q.camA + (size_t)c * 36 + j * 6;
+    double a = SHARED ? q.camHcf[G.row] * pf : 0.0, av = 0;
+#pragma unroll
+    for (int k = 0; k < 6; k++) av += A[k] * p[6 * (size_t)c + k];
+    a += av;
+    a -= part[j];
+    vs(q)[G.row] = a;
+    d = p[G.row] * a;
+    if (SHARED) fs = q.camHcf[G.row] * p[G.row] - (j == 0 ? part[6] : 0.0);
+  }
+  block_sum2(d, fs, lds);
+  if (threadIdx.x == 0) {
+    q.group_part[4 * (size_t)blockIdx.x] = d;
+    q.group_part[4 * (size_t)blockIdx.x + 1] = fs;
+    if (G.first) s[S_RHO] = s[S_RHO_NEXT];
+  }
+}
+
+template <bool SHARED>
+__global__ __launch_bounds__(ADV_THREADS) void k_cg_step(Problem q) {
+  __shared__ double rr[ADV_THREADS];
+  __shared__ double lds[2 * (ADV_THREADS / 64)];
+  const Group G = group_of(q);
+  double* s = bstate(q, G.bundle);
+  if (s[S_PCG_ACTIVE] == 0) return;
+  const Bundle& g = q.bundles[G.bundle];
   const uint32_t tid = threadIdx.x;
-  const double rho0 = s[S_RHO], r0 = s[S_R0];
+  double dot, fsum;
+  group_sums(q, g, 0, dot, fsum, lds);
+  const double* p = vp(q);
+  double den = dot, sf = 0;
+  if (SHARED) {
+    const double pf = p[g.focal_row];
+    sf = s[S_FOCAL_DIAG] * pf + fsum;
+    den += pf * sf;
+  }
+  const double alpha = s[S_RHO] / den;
+  const bool broke = !(den > 0) || !(fabs(alpha) < 1e20);
+  if (G.first && tid == 0) {
+    s[S_MATVECS] += 1;
+    if (SHARED) vs(q)[g.focal_row] = sf;
+    if (broke) {
+      s[S_PCG_ACTIVE] = 0;
+      s[S_PCG_FAILED] = 1;
+    }
+  }
+  if (broke) return;
   double* x = vx(q);
   double* r = vr(q);
-  double* p = vp(q);
-  double* S = vs(q);
   double* z = vz(q);
-  double den = block_sum(schur_finish<SHARED>(q, g, s, p, S, lds), lds);
-  if (tid == 0) s[S_MATVECS] += 1;
-  if (!(den > 0)) {
-    if (tid == 0) {
-      s[S_PCG_ACTIVE] = 0;
-      s[S_PCG_FAILED] = 1;
-    }
-    return;
+  double res = 0, rho = 0, ri = 0;
+  if (G.has) {
+    x[G.row] += alpha * p[G.row];
+    ri = r[G.row] - alpha * vs(q)[G.row];
+    r[G.row] = ri;
+    rr[tid] = ri;
+    res = ri * ri;
   }
-  const double alpha = rho0 / den;
-  if (!(fabs(alpha) < 1e20)) {
-    if (tid == 0) {
-      s[S_PCG_ACTIVE] = 0;
-      s[S_PCG_FAILED] = 1;
-    }
-    return;
+  __syncthreads();
+  if (G.has) {
+    const uint32_t j = tid % 6, base = tid - j;
+    const double* inv = q.camInv + (size_t)(G.row / 6) * 36 + j * 6;
+    double zi = 0;
+#pragma unroll
+    for (int k = 0; k < 6; k++) zi += inv[k] * rr[base + k];
+    z[G.row] = zi;
+    rho = zi * ri;
   }
-  double res = 0, rho = 0;
-  const uint32_t r_lo = 6 * g.cam0, r_hi = 6 * g.cam1;
-  for (uint32_t i = r_lo + tid; i < r_hi; i += CG_THREADS) {
-    x[i] += alpha * p[i];
-    const double ri = r[i] - alpha * S[i];
-    r[i] = ri;
-    res += ri * ri;
-  }
-  if (SHARED && tid == 0) {
+  if (SHARED && G.first && tid == 0) {
     const uint32_t f = g.focal_row;
     x[f] += alpha * p[f];
-    r[f] -= alpha * S[f];
+    r[f] -= alpha * sf;
     res += r[f] * r[f];
     z[f] = s[S_FOCAL_INV] * r[f];
     rho += z[f] * r[f];
   }
-  __syncthreads();  // every row of r before the block products
-  for (uint32_t i = r_lo + tid; i < r_hi; i += CG_THREADS) {
-    const double zi = block_row(q.camInv, i, r);
-    z[i] = zi;
-    rho += zi * r[i];
-  }
-  res = block_sum(res, lds);
-  rho = block_sum(rho, lds);
+  block_sum2(res, rho, lds);
   if (tid == 0) {
+    q.group_part[4 * (size_t)blockIdx.x + 2] = res;
+    q.group_part[4 * (size_t)blockIdx.x + 3] = rho;
+  }
+}
+
+template <bool SHARED>
+__global__ __launch_bounds__(ADV_THREADS) void k_cg_direction(Problem q) {
+  __shared__ double lds[2 * (ADV_THREADS / 64)];
+  const Group G = group_of(q);
+  double* s = bstate(q, G.bundle);
+  if (s[S_PCG_ACTIVE] == 0) return;
+  const Bundle& g = q.bundles[G.bundle];
+  double res, rho;
+  group_sums(q, g, 2, res, rho, lds);
+  const bool converged = res <= q.tol * q.tol * s[S_R0];
+  const bool broke = !converged && !(rho > 0);
+  const double beta = rho / s[S_RHO];
+  if (G.first && threadIdx.x == 0) {
     s[S_CG_STEPS] += 1;
     s[S_STEPS] += 1;
     s[S_R] = res;
-  }
-  if (res <= q.tol * q.tol * r0) {
-    if (tid == 0) {
+    if (converged) {
       s[S_PCG_ACTIVE] = 0;
       s[S_CONVERGED] = 1;
-    }
-    return;
-  }
-  if (!(rho > 0)) {
-    if (tid == 0) {
+    } else if (broke) {
       s[S_PCG_ACTIVE] = 0;
       s[S_PCG_FAILED] = 1;
+    } else {
+      s[S_RHO_NEXT] = rho;
     }
-    return;
   }
-  const double beta = rho / rho0;
-  for (uint32_t i = r_lo + tid; i < r_hi; i += CG_THREADS) p[i] = z[i] + beta * p[i];
-  if (SHARED && tid == 0) p[g.focal_row] = z[g.focal_row] + beta * p[g.focal_row];
-  if (tid == 0) s[S_RHO] = rho;
+  if (converged || broke) return;
+  double* p = vp(q);
+  const double* z = vz(q);
+  if (G.has) p[G.row] = z[G.row] + beta * p[G.row];
+  if (SHARED && G.first && threadIdx.x == 0) p[g.focal_row] = z[g.focal_row] + beta * p[g.focal_row];
 }
 
 // The linear-system probe (bundle 0): S v, M^-1 v and the RHS of the assembled system.
 template <bool SHARED>
 __global__ __launch_bounds__(CG_THREADS) void k_probe(Problem q, const double* v, double* sv,
                                                       double* zv, double* rhs) {
-  __shared__ double lds[CG_THREADS / 64];
+  __shared__ double lds[2 * (CG_THREADS / 64)];
   const Bundle& g = q.bundles[0];
   const double* s = bstate(q, 0);
   (void)schur_finish<SHARED>(q, g, s, v, sv, lds);
@@ -1045,6 +1173,8 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_sweep_implicit(Probl
   for (int j = 0; j < 3; j++) y[j] = pb[j * 3] * acc[0] + pb[j * 3 + 1] * acc[1] + pb[j * 3 + 2] * acc[2];
 }
 
+// The implicit camera sweep: relinearises each slot of the task instead of reading its stored
+// cross block. `guard` set: the PCG loop's sweep of p (skipped once word 19 is clear).
 template <bool SHARED>
 __global__ __launch_bounds__(TASK_THREADS) void k_camera_sweep_implicit(Problem q, int guard) {
   constexpr int NV = SHARED ? 7 : 6;
@@ -1089,7 +1219,7 @@ __global__ __launch_bounds__(TASK_THREADS) void k_camera_sweep_implicit(Problem 
 // Runs only for a bundle whose PCG pass converged (word 24 set, word 20 clear).
 template <bool SHARED>
 __global__ __launch_bounds__(CG_THREADS) void k_refine(Problem q) {
-  __shared__ double lds[CG_THREADS / 64];
+  __shared__ double lds[2 * (CG_THREADS / 64)];
   const uint32_t bi = blockIdx.x;
   const Bundle& g = q.bundles[bi];
   double* s = bstate(q, bi);
@@ -1139,6 +1269,7 @@ __global__ __launch_bounds__(CG_THREADS) void k_refine(Problem q) {
   if (tid == 0) {
     s[S_R] = res;
     s[S_RHO] = rho;
+    s[S_RHO_NEXT] = rho;
     s[S_CONVERGED] = 0;
     s[S_REFINEMENTS] += 1;
     if (rho > 0)
