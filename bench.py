@@ -280,7 +280,187 @@ def probe(args) -> int:
     return 0
 
 
+def grad(args) -> int:
+    """Differentiation on a shared-focal trajectory scene (FP64; the generator of the benchmark's
+    main workload, with its noise, outliers and focal error): the forward solve, the Jacobian
+    kernels against central differences (h = 1e-6) of the oracle's projection on a sample of
+    observations (`--jacobian-only` stops there: the 500-camera scene's check), a polish to a
+    stationary point (damped exact-Hessian LM, then Newton), the adjoint backward for a seeded
+    N(0, 1) cotangent on every output (poses, points, log focal), and central differences
+    (h = 1e-3 px) of re-polished solutions on 64 random single-observation pixel perturbations.
+    The default scene is 20 cameras / 3000 points: along longer trajectories the exact Hessian at
+    the solution has a bending mode whose eigenvalue falls with the camera count, so the derivative
+    grows without bound and re-solves of perturbed pixels leave the linear regime."""
+    import numpy as np
+
+    from oracle import oracle as O
+    from typecv_amd import bundle as B
+    from typecv_amd import synthetic as S
+
+    packed, info = S.trajectory_scene(cameras=args.cameras, points=args.points, seed=args.seed,
+                                      outliers=args.outliers)
+    C, P, M = packed.C, packed.P, packed.M
+    log(f"scene C={C} P={P} M={M}")
+    control = B.make_control(C, P, M, shared_focal=True, huber=args.huber,
+                             iterations=args.iterations, max_cg=min(256, max(32, 12 * C + 1)))
+    device = B.HipDevice(0)
+    t = time.perf_counter()
+    out, rep = B.solve_packed(device, control, packed)
+    forward_s = time.perf_counter() - t
+    at = B.at_solution(packed, out)
+    lf = B.output_log_focal(control, out)
+    log(f"forward {forward_s:.2f}s, {rep['lm_iterations']} LM, log focal {lf:.6f}")
+    # Jacobian kernels against central differences of the oracle's projection
+    J = B.jacobian(device, control, at, lf)
+    rng = np.random.default_rng(args.seed)
+    sample = rng.choice(M, size=args.jacobian_sample, replace=False)
+    cams = packed.index[:M].astype(np.int64)
+    pts = packed.index[M: 2 * M].astype(np.int64)
+    models = at.scene[12 * C + 3 * P: 18 * C + 3 * P].reshape(C, 6)
+    active = packed.index[3 * M + P + C + 2:].astype(bool)
+
+    def pixel(pose, X, model, lfs):
+        s = O.focal_scale(lfs)
+        k = model.copy()
+        k[0] *= s
+        k[1] *= s
+        R = pose[:9].reshape(3, 3)
+        ok, px, _, _ = O.project(k, R @ X + pose[9:])
+        return np.asarray(px)
+
+    h, worst = 1e-6, 0.0
+    for i in sample:
+        c, p = cams[i], pts[i]
+        pose = at.scene[12 * c: 12 * c + 12]
+        X = at.scene[12 * C + 3 * p: 12 * C + 3 * p + 3]
+        cols = []
+        for k in range(6):
+            if not active[c]:
+                cols.append(np.zeros(2))
+                continue
+            d = np.zeros(6)
+            d[k] = h
+            cols.append((pixel(O.move_pose(pose, d), X, models[c], lf)
+                         - pixel(O.move_pose(pose, -d), X, models[c], lf)) / (2 * h))
+        for k in range(3):
+            e = np.zeros(3)
+            e[k] = h
+            cols.append((pixel(pose, X + e, models[c], lf) - pixel(pose, X - e, models[c], lf))
+                        / (2 * h))
+        cols.append((pixel(pose, X, models[c], lf + h) - pixel(pose, X, models[c], lf - h)) / (2 * h))
+        fd = np.array(cols).T  # 2 x 10
+        an = np.concatenate([J[i, 2:14].reshape(2, 6), J[i, 14:20].reshape(2, 3),
+                             J[i, 20:22].reshape(2, 1)], axis=1)
+        worst = max(worst, float(np.max(np.abs(fd - an)) / max(1.0, np.max(np.abs(an)))))
+    log(f"jacobian vs central differences on {len(sample)} observations: max rel err {worst:.2e}")
+    if args.jacobian_only:
+        print(json.dumps({
+            "metric": "differentiation: Jacobian kernels vs central differences (h = 1e-6)",
+            "value": worst, "unit": "max relative error", "higher_is_better": False, "n_gpus": 1,
+            "dtype": "f64", "data": "synthetic (seeded trajectory scene)",
+            "config": {"workload": "shared-focal trajectory scene, Jacobian kernels", "cameras": C,
+                       "points": P, "observations": M, "sample": len(sample)},
+            "forward_s": forward_s}), flush=True)
+        device.dispose()
+        return 0
+    # Newton polish to a stationary point
+    t = time.perf_counter()
+    pol_out, pol_rep = B.polish(device, control, at, lf, iterations=4, tolerance=1e-12,
+                                lm_iterations=args.polish_lm, max_cg=args.max_cg)
+    polish_s = time.perf_counter() - t
+    pol = B.at_solution(at, pol_out)
+    lf = B.output_log_focal(control, pol_out)
+    g_before = np.linalg.norm(O.gradient(control, at.scene, packed.index, B.output_log_focal(
+        control, out))[0])
+    g_after = np.linalg.norm(O.gradient(control, pol.scene, packed.index, lf)[0])
+    log(f"polish {polish_s:.2f}s ({pol_rep['lm_iterations']} damped, "
+        f"{pol_rep['refinement_steps']:.0f} Newton steps, {pol_rep['cg_iterations']} CG): "
+        f"|g| {g_before:.3e} -> {g_after:.3e}")
+    result = {
+        "metric": "differentiation: Jacobian vs central differences, adjoint vs FD of re-solves",
+        "n_gpus": 1, "dtype": "f64", "data": "synthetic (seeded trajectory scene)",
+        "config": {"workload": "shared-focal trajectory scene, implicit differentiation",
+                   "cameras": C, "points": P, "observations": M, "outliers": info["outliers"]},
+        "forward_s": forward_s, "polish_s": polish_s,
+        "polish": {"damped_steps": pol_rep["lm_iterations"], "newton_steps": pol_rep["refinement_steps"],
+                   "cg_iterations": pol_rep["cg_iterations"]},
+        "gradient_norm": {"forward": g_before, "polished": g_after},
+        "jacobian": {"sample": len(sample), "h": h, "max_rel_err": worst},
+    }
+    if pol_rep["refinement_steps"] < 1:
+        result["backward"] = {"error": "the polish reached no stationary point (no Newton step "
+                                       "could be taken): the implicit derivative is undefined"}
+        print(json.dumps(result), flush=True)
+        device.dispose()
+        return 1
+    # the backward
+    n_out = 12 * C + 3 * P + 1
+    cot = np.random.default_rng(args.seed + 1).normal(size=n_out)
+    B.backward(device, control, pol, lf, cot, tolerance=args.tolerance, max_cg=args.max_cg)
+    t = time.perf_counter()
+    pixels, adjoint, back = B.backward(device, control, pol, lf, cot, tolerance=args.tolerance,
+                                       max_cg=args.max_cg)
+    backward_s = time.perf_counter() - t
+    log(f"backward {backward_s * 1e3:.1f} ms, {back['cg_iterations']} CG")
+    # central differences of re-polished solutions; a perturbation that can carry its observation's
+    # residual across the Huber radius (|r| within 2h of it at the solution) meets the kink of the
+    # robust cost, where the derivative is one-sided: reported apart
+    radius = np.hypot(*B.jacobian(device, control, pol, lf)[:, 0:2].T)
+    errs, kinks = [], []
+    pix = 18 * C + 3 * P
+    hp = 1e-3
+    for k in range(args.perturbations):
+        i = int(rng.integers(0, M))
+        d = rng.normal(size=2)
+        d /= np.linalg.norm(d)
+        outs = []
+        for sign in (1, -1):
+            scene = pol.scene.copy()
+            scene[pix + 2 * i: pix + 2 * i + 2] += sign * hp * d
+            o, _ = B.polish(device, control, B.dataclasses.replace(pol, scene=scene), lf,
+                            iterations=2, tolerance=1e-12, max_cg=args.max_cg)
+            outs.append(o)
+        fd = cot[:n_out - 1] @ ((outs[0][:n_out - 1] - outs[1][:n_out - 1]) / (2 * hp))
+        fd += cot[-1] * (B.output_log_focal(control, outs[0])
+                         - B.output_log_focal(control, outs[1])) / (2 * hp)
+        an = float(pixels[i] @ d)
+        err = abs(fd - an) / max(abs(fd), 1e-12)
+        kink = abs(radius[i] - control.huber) < 2 * hp
+        (kinks if kink else errs).append(err)
+        log(f"perturbation {k}: observation {i} (|r| {radius[i]:.4f}{', at the Huber radius' if kink else ''}),"
+            f" fd {fd:.6e}, adjoint {an:.6e}, rel {err:.1e}")
+    device.dispose()
+    result.update({
+        "value": backward_s * 1e3, "unit": "ms per backward", "higher_is_better": False,
+        "backward": {"ms": backward_s * 1e3, "cg_iterations": back["cg_iterations"],
+                     "tolerance": args.tolerance},
+        "finite_differences": {"perturbations": len(errs) + len(kinks), "h_px": hp,
+                               "max_rel_err": float(max(errs)) if errs else None,
+                               "median_rel_err": float(np.median(errs)) if errs else None,
+                               "at_huber_radius": len(kinks),
+                               "at_huber_radius_rel_err": [float(e) for e in kinks]},
+    })
+    print(json.dumps(result), flush=True)
+    return 0
+
+
 if __name__ == "__main__":
+    if "--mode" in sys.argv and "grad" in sys.argv:
+        ap = argparse.ArgumentParser()
+        ap.add_argument("--mode")
+        ap.add_argument("--cameras", type=int, default=20)
+        ap.add_argument("--points", type=int, default=3000)
+        ap.add_argument("--jacobian-only", action="store_true")
+        ap.add_argument("--seed", type=int, default=2)
+        ap.add_argument("--iterations", type=int, default=30)
+        ap.add_argument("--jacobian-sample", type=int, default=2000)
+        ap.add_argument("--perturbations", type=int, default=64)
+        ap.add_argument("--tolerance", type=float, default=1e-10)
+        ap.add_argument("--polish-lm", type=int, default=200)
+        ap.add_argument("--max-cg", type=int, default=20000)
+        ap.add_argument("--outliers", type=float, default=0.02)
+        ap.add_argument("--huber", type=float, default=1.0)
+        sys.exit(grad(ap.parse_args()))
     if "--mode" in sys.argv and "probe" in sys.argv:
         ap = argparse.ArgumentParser()
         ap.add_argument("--mode")

@@ -56,7 +56,8 @@ enum tcv_ba_status {
   TCV_BA_ERR_INDEX = 2,    /* the index array is not a packBundleScene layout of these sizes */
   TCV_BA_ERR_DEVICE = 3,   /* a HIP runtime call failed */
   TCV_BA_ERR_MEMORY = 4,   /* device allocation failed (the reference's budget refusal) */
-  TCV_BA_ERR_NO_DEVICE = 5 /* no gfx950 device is visible */
+  TCV_BA_ERR_NO_DEVICE = 5, /* no gfx950 device is visible */
+  TCV_BA_ERR_SOLVE = 6      /* the adjoint system was not solved (not a minimum, or PCG budget) */
 };
 
 /*
@@ -181,6 +182,61 @@ int tcv_ba_schur_probe(tcv_ba_context* context, const tcv_ba_control* control,
                        const double* scene, const uint32_t* index, double lambda,
                        double log_focal, const double* p, double* rhs, double* sp, double* z,
                        int32_t* status, uint32_t repeat, tcv_ba_report* report);
+
+/*
+ * Differentiation of the solve. The reference operator has no derivative entries: these extend
+ * `bundleAdjustGpu` (src/sfm/bundle/gpu/bundle.ts:803-839) with the pieces of a backward pass. All
+ * take a host scene/index in the packBundleScene layouts (reference bundle-host.ts:121-218) holding
+ * the poses and points to differentiate at, and `log_focal`, the shared log focal scale (ignored on
+ * the fixed-focal route). Derivatives are of the half robust cost E = 1/2 sum_i robustCost(|r_i|),
+ * r_i = pixel_i - u_i (numerics.shade.ts:141-187), in the left-perturbation chart of movePose
+ * (numerics.shade.ts:254-281): R' = exp([omega]x) R, t' = exp([omega]x) t + v, X' = X + dX,
+ * log f' = log f + df.
+ */
+
+/*
+ * The Jacobian kernels' output: 23 doubles per observation (packed observation order), r (2),
+ * d(u,v)/d(omega, v) (12: row u, then row v; zero for a fixed camera), d(u,v)/dX (6), d(u,v)/d log f
+ * (2) and the Huber weight w; all zero with w = -1 where the point does not project. The forward's
+ * linearizeBundle (bundle.shade.ts:251-297, joint-focal.shade.ts:287-336).
+ */
+int tcv_ba_jacobian(tcv_ba_context* context, const tcv_ba_control* control, const double* scene,
+                    const uint32_t* index, double log_focal, double* out, tcv_ba_report* report);
+
+/*
+ * Polish of a solve to a stationary point of E, on the exact Hessian (Gauss-Newton with the Huber
+ * curvature plus the residual-weighted second derivatives), each system solved by the forward's
+ * Schur/PCG to `tolerance` within `max_cg` iterations and every step mapped to the forward's gauge
+ * (normalizeBundleCandidate, bundle.shade.ts:994-1023):
+ *   1. up to `lm_iterations` damped steps with the forward's damping and accept/reject rule
+ *      (acceptBundleCandidate, bundle.shade.ts:1071-1096), until that rule stops;
+ *   2. up to `newton_steps` undamped Newton steps without a cost test (near the solution the cost
+ *      change falls below its own rounding), stopping at the first that cannot be solved.
+ * `out` has the tcv_ba_output_size layout of tcv_ba_solve (word 10 counts the accepted damped
+ * steps and Newton steps; words 8/9 hold the input's cost); report->refinement_steps counts the
+ * Newton steps taken. Makes the stationarity that implicit differentiation assumes hold to rounding.
+ */
+int tcv_ba_polish(tcv_ba_context* context, const tcv_ba_control* control, const double* scene,
+                  const uint32_t* index, double log_focal, uint32_t lm_iterations,
+                  uint32_t newton_steps, uint32_t max_cg, double tolerance, double* out,
+                  tcv_ba_report* report);
+
+/*
+ * The adjoint (backward) of the solve by implicit differentiation at a stationary scene:
+ * for a cotangent v on the outputs (12 per camera: R row-major, t; 3 per point; then, on the shared
+ * route, one for the log focal scale) returns pixel_cotangent[2M] = d<v, theta*>/d(observation
+ * pixels). Solves H lambda = (dN)^T v~ with the exact Hessian H by the forward's Schur/PCG (to
+ * `tolerance`, at most `max_cg` iterations), where v~ is v in the chart and N the gauge map
+ * (directional renormalisation, the similarity of the one-fixed-camera gauge), then
+ * pixel_cotangent_i = D_i J_i lambda with D_i = d(w r)/dr. Cross blocks are relinearised from the
+ * scene, not kept from the forward. `adjoint` (optional, 6C + 3P + shared doubles: chart rows of each
+ * camera, point rows, focal) receives lambda. TCV_BA_ERR_SOLVE when the system is not positive
+ * definite or PCG does not converge.
+ */
+int tcv_ba_backward(tcv_ba_context* context, const tcv_ba_control* control, const double* scene,
+                    const uint32_t* index, double log_focal, const double* cotangent,
+                    uint32_t max_cg, double tolerance, double* pixel_cotangent, double* adjoint,
+                    tcv_ba_report* report);
 
 #ifdef __cplusplus
 }

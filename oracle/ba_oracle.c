@@ -1176,3 +1176,44 @@ int tcv_oracle_solve3(const double* a, const double* b, double* x) { return solv
 int tcv_oracle_solve6(const double* a, const double* b, double* x) { return solve6(a, b, x); }
 void tcv_oracle_move_pose(const double* p, const double* d, double* out) { move_pose(p, d, out); }
 double tcv_oracle_focal_scale(double x) { return focal_scale(x); }
+
+/*
+ * The gradient of the half robust cost E = 1/2 sum_i robustCost(|r_i|) in the movePose chart at
+ * log focal scale `log_focal`: 6 per camera (omega, v; zero for a fixed camera), 3 per point, then
+ * the log focal scale on the shared route: g = sum_i J_i^T w_i r_i with the forward's Jacobians
+ * and Huber weights (linearizeBundle). `half_cost` (optional) receives E. Observations that do not
+ * project are skipped; returns 1 when there was one, else 0. A negative value: out of memory.
+ */
+int tcv_oracle_gradient(const tcv_ba_control* ctl, const double* scene, const uint32_t* index,
+                        double log_focal, double* grad, double* half_cost) {
+  Oracle o;
+  if (!oracle_init(&o, ctl, scene, index)) return -TCV_BA_ERR_MEMORY;
+  for (int j = 0; j < 26; j++) o.state[j] = 0;
+  o.state[1] = 1;
+  if (o.shared) o.focal[o.C * 6 + 3] = o.focal[o.C * 6 + 4] = log_focal;
+  linearize(&o);
+  const size_t n = (size_t)o.C * 6 + (size_t)o.P * 3 + o.shared;
+  for (size_t k = 0; k < n; k++) grad[k] = 0;
+  double cost = 0;
+  int skipped = 0;
+  for (uint32_t i = 0; i < o.M; i++) {
+    const double* b = o.J + (size_t)i * 24;
+    if (b[23] != 0) {
+      skipped = 1;
+      continue;
+    }
+    const uint32_t c = OBS_CAMERA(&o, i), p = OBS_POINT(&o, i);
+    const double wu = b[20] * b[18], wv = b[20] * b[19];
+    for (int k = 0; k < 6; k++) grad[(size_t)c * 6 + k] += b[k] * wu + b[6 + k] * wv;
+    for (int k = 0; k < 3; k++)
+      grad[(size_t)o.C * 6 + (size_t)p * 3 + k] += b[12 + k] * wu + b[15 + k] * wv;
+    if (o.shared) {
+      const double* f = o.focal + (size_t)o.C * 6 + 6 + (size_t)i * 2;
+      grad[n - 1] += f[0] * wu + f[1] * wv;
+    }
+    cost += 0.5 * b[21];
+  }
+  if (half_cost) *half_cost = cost;
+  oracle_free(&o);
+  return skipped;
+}

@@ -54,6 +54,7 @@ def library() -> ctypes.CDLL:
     lib.tcv_oracle_move_pose.restype = None
     lib.tcv_oracle_focal_scale.argtypes = [ctypes.c_double]
     lib.tcv_oracle_focal_scale.restype = ctypes.c_double
+    lib.tcv_oracle_gradient.argtypes = [P(_native.Control), vp, vp, ctypes.c_double, vp, vp]
     _lib = lib
     return lib
 
@@ -177,3 +178,18 @@ def move_pose(p, d):
 
 def focal_scale(x: float) -> float:
     return library().tcv_oracle_focal_scale(float(x))
+
+
+def gradient(control: _native.Control, scene: np.ndarray, index: np.ndarray,
+             log_focal: float = 0.0):
+    """(g, half cost, skipped): the chart gradient of 1/2 sum robustCost (6C + 3P + shared)."""
+    n = control.cameras * 6 + control.points * 3 + (1 if control.shared_focal else 0)
+    g = np.empty(n)
+    cost = np.zeros(1)
+    scene = np.ascontiguousarray(scene, dtype=np.float64)
+    index = np.ascontiguousarray(index, dtype=np.uint32)
+    r = library().tcv_oracle_gradient(ctypes.byref(control), _ptr(scene), _ptr(index),
+                                      float(log_focal), _ptr(g), _ptr(cost))
+    if r < 0:
+        raise RuntimeError(f"oracle gradient failed ({r})")
+    return g, float(cost[0]), bool(r)

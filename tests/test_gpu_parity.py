@@ -54,6 +54,10 @@ def packed_cases():
     traj, _ = S.trajectory_scene(cameras=40, points=3000, seed=5)
     yield "trajectory_shared", traj, B.make_control(
         traj.C, traj.P, traj.M, iterations=15, max_cg=256, shared_focal=True, huber=1.0)
+    # over 1024 observations per camera: every camera list is split into two tasks
+    dense, _ = S.trajectory_scene(cameras=20, points=3000, seed=2)
+    yield "trajectory_split_cameras", dense, B.make_control(
+        dense.C, dense.P, dense.M, iterations=30, max_cg=256, shared_focal=True, huber=1.0)
 
 
 PACKED = list(packed_cases())

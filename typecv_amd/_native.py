@@ -21,6 +21,7 @@ STATUS = {
     3: "device",
     4: "memory",
     5: "no-device",
+    6: "solve",
 }
 
 # The symbols include/typecv_ba.h declares, in declaration order.
@@ -38,6 +39,9 @@ EXPORTS = (
     "tcv_ba_solve_device",
     "tcv_ba_solve_batch",
     "tcv_ba_schur_probe",
+    "tcv_ba_jacobian",
+    "tcv_ba_polish",
+    "tcv_ba_backward",
 )
 
 
@@ -141,6 +145,15 @@ def library() -> ctypes.CDLL:
                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
                                        P(Report)]
     lib.tcv_ba_schur_probe.restype = ctypes.c_int
+    vp = ctypes.c_void_p
+    lib.tcv_ba_jacobian.argtypes = [vp, c_control, vp, vp, ctypes.c_double, vp, P(Report)]
+    lib.tcv_ba_jacobian.restype = ctypes.c_int
+    lib.tcv_ba_polish.argtypes = [vp, c_control, vp, vp, ctypes.c_double, ctypes.c_uint32,
+                                  ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, vp, P(Report)]
+    lib.tcv_ba_polish.restype = ctypes.c_int
+    lib.tcv_ba_backward.argtypes = [vp, c_control, vp, vp, ctypes.c_double, vp, ctypes.c_uint32,
+                                    ctypes.c_double, vp, vp, P(Report)]
+    lib.tcv_ba_backward.restype = ctypes.c_int
     _lib = lib
     return lib
 

@@ -20,6 +20,7 @@ before any device work, or after it for the device-detected input errors of `che
 from __future__ import annotations
 
 import ctypes
+import dataclasses
 import math
 import threading
 from typing import Any, Mapping
@@ -404,6 +405,90 @@ def schur_probe(device: HipDevice, control: _native.Control, packed: PackedScene
     return rhs, sp, z, int(status[0]), report.as_dict()
 
 
+# ------------------------------------------------------------------ derivatives
+
+
+def at_solution(packed: PackedScene, out: np.ndarray) -> PackedScene:
+    """The packed problem with its poses and points replaced by a solve's output coordinates."""
+    scene = packed.scene.copy()
+    n = packed.C * 12 + packed.P * 3
+    scene[:n] = out[:n]
+    return dataclasses.replace(packed, scene=scene)
+
+
+def output_log_focal(control: _native.Control, out: np.ndarray) -> float:
+    """The log focal scale word of a solve's output (0 on the fixed-focal route)."""
+    if not control.shared_focal:
+        return 0.0
+    return float(out[control.cameras * 12 + control.points * 3 + 26])
+
+
+def jacobian(device: HipDevice, control: _native.Control, packed: PackedScene,
+             log_focal: float = 0.0) -> np.ndarray:
+    """`tcv_ba_jacobian`: (M, 23) = r (2), d(u,v)/d(omega, v) (12), d(u,v)/dX (6),
+    d(u,v)/d log f (2), w per observation."""
+    lib = _native.library()
+    scene = np.ascontiguousarray(packed.scene, dtype=np.float64)
+    index = np.ascontiguousarray(packed.index, dtype=np.uint32)
+    out = np.empty((packed.M, 23))
+    report = _native.Report()
+
+    def job():
+        _native.check(lib.tcv_ba_jacobian(device.handle, ctypes.byref(control), scene.ctypes.data,
+                                          index.ctypes.data, float(log_focal), out.ctypes.data,
+                                          ctypes.byref(report)), "tcv_ba_jacobian")
+
+    device.enqueue(job)
+    return out
+
+
+def polish(device: HipDevice, control: _native.Control, packed: PackedScene,
+           log_focal: float = 0.0, iterations: int = 3, max_cg: int = 4000,
+           tolerance: float = 1e-12, lm_iterations: int = 0):
+    """`tcv_ba_polish`: up to `lm_iterations` damped steps, then up to `iterations` Newton steps,
+    on the exact Hessian from the packed scene. Returns (out in the solve layout, report)."""
+    lib = _native.library()
+    scene = np.ascontiguousarray(packed.scene, dtype=np.float64)
+    index = np.ascontiguousarray(packed.index, dtype=np.uint32)
+    out = np.empty(lib.tcv_ba_output_size(ctypes.byref(control)))
+    report = _native.Report()
+
+    def job():
+        _native.check(lib.tcv_ba_polish(device.handle, ctypes.byref(control), scene.ctypes.data,
+                                        index.ctypes.data, float(log_focal), int(lm_iterations),
+                                        int(iterations), int(max_cg), float(tolerance),
+                                        out.ctypes.data, ctypes.byref(report)), "tcv_ba_polish")
+
+    device.enqueue(job)
+    return out, report.as_dict()
+
+
+def backward(device: HipDevice, control: _native.Control, packed: PackedScene, log_focal: float,
+             cotangent: np.ndarray, max_cg: int = 4000, tolerance: float = 1e-10):
+    """`tcv_ba_backward`: the pixel cotangent (M, 2) of a cotangent on the outputs
+    (12C + 3P [+ 1]), the adjoint lambda (6C + 3P [+ 1], chart) and the report."""
+    lib = _native.library()
+    n_out = control.cameras * 12 + control.points * 3 + (1 if control.shared_focal else 0)
+    cot = np.ascontiguousarray(cotangent, dtype=np.float64)
+    if cot.shape != (n_out,):
+        raise ValueError(f"cotangent must have {n_out} entries")
+    scene = np.ascontiguousarray(packed.scene, dtype=np.float64)
+    index = np.ascontiguousarray(packed.index, dtype=np.uint32)
+    pixels = np.empty((packed.M, 2))
+    adjoint = np.empty(control.cameras * 6 + control.points * 3 + (1 if control.shared_focal else 0))
+    report = _native.Report()
+
+    def job():
+        _native.check(lib.tcv_ba_backward(device.handle, ctypes.byref(control), scene.ctypes.data,
+                                          index.ctypes.data, float(log_focal), cot.ctypes.data,
+                                          int(max_cg), float(tolerance), pixels.ctypes.data,
+                                          adjoint.ctypes.data, ctypes.byref(report)),
+                      "tcv_ba_backward")
+
+    device.enqueue(job)
+    return pixels, adjoint, report.as_dict()
+
+
 # ------------------------------------------------------------------ the operator
 
 
@@ -527,3 +612,76 @@ def bundle_adjust_gpu(device: HipDevice, scene: Mapping, options: Mapping | None
 
 # The reference's camelCase name.
 bundleAdjustGpu = bundle_adjust_gpu
+
+
+def bundle_adjust_gpu_backward(device: HipDevice, scene: Mapping, options: Mapping | None,
+                               result: Mapping, cotangent: Mapping, *, polish_steps: int = 3,
+                               tolerance: float = 1e-10, max_cg: int = 4000) -> dict:
+    """The backward entry of `bundle_adjust_gpu`: for a cotangent on its outputs,
+    {"views": [{"R": 3x3, "t": 3}] (result order), "landmarks": [{"xyz": 3}],
+     "directionalConstraints": [{"direction": 3}] (when the result has them),
+     "logFocal": d/d log f (shared-focal route)},
+    returns the cotangent of the observation pixels by implicit differentiation at the solution:
+    {"landmarks": [[2-vectors per observation]] (input landmark and observation order),
+     "directionalConstraints": [...] likewise, "pixels": (M, 2) in packed order, "report"}.
+    `polish_steps` Newton steps on the exact Hessian first make the returned solution stationary to
+    rounding (0 differentiates at the result as given). Entries the cotangent omits are zero."""
+    options = dict(options or {})
+    _check_option_enums(options)
+    shared = bool(options.get("refineSharedFocal"))
+    if shared:
+        default_max_cg = lambda C: min(256, max(32, C * 12 + 1))  # noqa: E731
+    else:
+        default_max_cg = lambda C: min(96, max(24, C * 12))  # noqa: E731
+    problem = bundle_problem(scene, options, default_max_cg)
+    model = problem["input"]
+    C, P, M = problem["C"], problem["P"], problem["M"]
+    finite = problem["finitePoints"]
+    packed = pack_bundle_scene(model, problem["byId"], problem["active"])
+    control = make_control(
+        C, P, M, directional=P - finite, anchor=problem["byId"][problem["anchorId"]],
+        scale_view=problem["byId"][problem["scaleId"]], normalize_scale=problem["fixed"] == 1,
+        shared_focal=shared, iterations=problem["iterations"], max_cg=problem["maxCg"],
+        huber=problem["huber"], tolerance=problem["tolerance"])
+    rec = result["reconstruction"]
+    coords = packed.scene.copy()
+    for c, v in enumerate(rec["views"]):
+        coords[12 * c: 12 * c + 9] = np.asarray(v["pose"]["R"], dtype=np.float64).reshape(9)
+        coords[12 * c + 9: 12 * c + 12] = np.asarray(v["pose"]["t"], dtype=np.float64).reshape(3)
+    directions = result.get("directionalConstraints", [])
+    for i, p in enumerate(list(rec["landmarks"]) + [{"xyz": d["direction"]} for d in directions]):
+        coords[12 * C + 3 * i: 12 * C + 3 * i + 3] = np.asarray(p["xyz"], dtype=np.float64)
+    at = dataclasses.replace(packed, scene=coords)
+    log_focal = 0.0
+    if shared:
+        log_focal = math.log(rec["views"][0]["camera"]["fx"] / model["views"][0]["camera"]["fx"])
+    report = {}
+    if polish_steps:
+        out, report = polish(device, control, at, log_focal, iterations=polish_steps,
+                             max_cg=max_cg, tolerance=min(1e-12, tolerance))
+        at = at_solution(at, out)
+        log_focal = output_log_focal(control, out)
+    cot = np.zeros(12 * C + 3 * P + (1 if shared else 0))
+    for c, v in enumerate(cotangent.get("views", [])):
+        if "R" in v:
+            cot[12 * c: 12 * c + 9] = np.asarray(v["R"], dtype=np.float64).reshape(9)
+        if "t" in v:
+            cot[12 * c + 9: 12 * c + 12] = np.asarray(v["t"], dtype=np.float64).reshape(3)
+    for i, p in enumerate(cotangent.get("landmarks", [])):
+        cot[12 * C + 3 * i: 12 * C + 3 * i + 3] = np.asarray(p["xyz"], dtype=np.float64)
+    for k, d in enumerate(cotangent.get("directionalConstraints", [])):
+        i = finite + k
+        cot[12 * C + 3 * i: 12 * C + 3 * i + 3] = np.asarray(d["direction"], dtype=np.float64)
+    if shared:
+        cot[-1] = float(cotangent.get("logFocal", 0.0))
+    pixels, adjoint, back = backward(device, control, at, log_focal, cot, max_cg=max_cg,
+                                     tolerance=tolerance)
+    lengths = [len(p["observations"]) for p in model["landmarks"]]
+    starts = np.concatenate([[0], np.cumsum(lengths)])
+    per_point = [[pixels[k].copy() for k in range(starts[i], starts[i + 1])] for i in range(P)]
+    return {"landmarks": per_point[:finite], "directionalConstraints": per_point[finite:],
+            "pixels": pixels, "adjoint": adjoint, "logFocal": log_focal,
+            "report": {"polish": report, "backward": back}}
+
+
+bundleAdjustGpuBackward = bundle_adjust_gpu_backward

@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -51,7 +52,8 @@ struct tcv_ba_context {
   std::vector<DevBuf*> all;
   DevBuf scene, trial, index, out, bundles, tiles, tile_bundle, tasks, task_start, cost_blocks,
       camera_bundle, point_bundle, cam_point, Wp, Wc, pblk, tile_fail, tile_focal, task_part, camA,
-      camInv, camRhs, camHcf, camFocal, vec, state, cost_part, error, probe, groups, group_part;
+      camInv, camRhs, camHcf, camFocal, vec, state, cost_part, error, probe, groups, group_part,
+      rhs, grad;
   std::vector<hipEvent_t> events;
   tcv_ba_context() {
     all = {&scene,      &trial,        &index,     &out,       &bundles,    &tiles,
@@ -59,7 +61,7 @@ struct tcv_ba_context {
            &point_bundle, &cam_point,  &Wp,        &Wc,        &pblk,       &tile_fail,
            &tile_focal, &task_part,    &camA,      &camInv,    &camRhs,     &camHcf,
            &camFocal,   &vec,          &state,     &cost_part, &error,      &probe,
-           &groups,     &group_part};
+           &groups,     &group_part,   &rhs,       &grad};
   }
 };
 
@@ -256,12 +258,32 @@ uint32_t tile_blocks(const Problem& q) { return blocks_for(q.n_tiles, TILE_WAVES
 template <class WT, bool SHARED>
 struct Launch {
   static void point_pass(const Problem& q, hipStream_t s) {
-    hipLaunchKernelGGL((k_point_pass<WT, SHARED>), dim3(tile_blocks(q)), dim3(TILE * TILE_WAVES),
-                       0, s, q);
+    hipLaunchKernelGGL((k_point_pass<WT, SHARED, false>), dim3(tile_blocks(q)),
+                       dim3(TILE * TILE_WAVES), 0, s, q);
   }
   static void camera_pass(const Problem& q, hipStream_t s) {
-    hipLaunchKernelGGL((k_camera_pass<WT, SHARED>), dim3(q.n_tasks), dim3(TASK_THREADS), 0, s, q);
+    hipLaunchKernelGGL((k_camera_pass<WT, SHARED, false>), dim3(q.n_tasks), dim3(TASK_THREADS), 0,
+                       s, q);
     hipLaunchKernelGGL((k_camera_finish<SHARED>), dim3(blocks_for(q.C, 64)), dim3(64), 0, s, q);
+  }
+  // The exact-Hessian assembly (FP64 blocks) of the Newton polish and the adjoint.
+  static void exact_passes(const Problem& q, hipStream_t s) {
+    hipLaunchKernelGGL((k_point_pass<double, SHARED, true>), dim3(tile_blocks(q)),
+                       dim3(TILE * TILE_WAVES), 0, s, q);
+    hipLaunchKernelGGL((k_camera_pass<double, SHARED, true>), dim3(q.n_tasks), dim3(TASK_THREADS),
+                       0, s, q);
+    hipLaunchKernelGGL((k_camera_finish<SHARED>), dim3(blocks_for(q.C, 64)), dim3(64), 0, s, q);
+  }
+  static void solve_points(const Problem& q, hipStream_t s) {
+    hipLaunchKernelGGL((k_point_sweep<WT, SHARED, SWEEP_SOLVE>), dim3(tile_blocks(q)),
+                       dim3(TILE * TILE_WAVES), 0, s, q);
+  }
+  static void pixel_cotangent(const Problem& q, double* out, hipStream_t s) {
+    hipLaunchKernelGGL((k_pixel_cotangent<SHARED>), dim3(blocks_for(q.M, 256)), dim3(256), 0, s,
+                       q, out);
+  }
+  static void jacobian(const Problem& q, double* out, hipStream_t s) {
+    hipLaunchKernelGGL((k_jacobian<SHARED>), dim3(blocks_for(q.M, 256)), dim3(256), 0, s, q, out);
   }
   static void init_cg(const Problem& q, hipStream_t s) {
     hipLaunchKernelGGL((k_init_cg<SHARED>), dim3(q.B), dim3(CG_THREADS), 0, s, q);
@@ -315,13 +337,19 @@ struct Ops {
   void (*update_points)(const Problem&, hipStream_t);
   void (*probe)(const Problem&, const double*, double*, double*, double*, hipStream_t);
   void (*refine)(const Problem&, hipStream_t);
+  void (*exact_passes)(const Problem&, hipStream_t);
+  void (*solve_points)(const Problem&, hipStream_t);
+  void (*pixel_cotangent)(const Problem&, double*, hipStream_t);
+  void (*jacobian)(const Problem&, double*, hipStream_t);
 };
 
 template <class WT, bool SHARED>
 Ops ops_for(bool implicit) {
   using L = Launch<WT, SHARED>;
-  Ops o{L::point_pass, L::camera_pass,   L::init_cg, L::point_sweep, L::camera_sweep,
-        L::advance,    L::update_points, L::probe,   L::refine};
+  Ops o{L::point_pass,   L::camera_pass,   L::init_cg,         L::point_sweep,
+        L::camera_sweep, L::advance,       L::update_points,   L::probe,
+        L::refine,       L::exact_passes,  L::solve_points,    L::pixel_cotangent,
+        L::jacobian};
   if (implicit) {
     o.point_sweep = L::implicit_p_sweep;
     o.camera_sweep = L::implicit_camera_sweep;
@@ -436,6 +464,7 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
   q.huber = c->huber;
   q.tol = c->relative_tolerance;
   q.refine_tol = c->refinement_tolerance > 0 ? c->refinement_tolerance : c->relative_tolerance;
+  q.stop_change = 1e-9;
   q.scene = (double*)ctx->scene.ptr;
   q.trial = (double*)ctx->trial.ptr;
   q.index = index;
@@ -701,6 +730,161 @@ int solve_device(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B
   return TCV_BA_OK;
 }
 
+// ------------------------------------------------------------------ Newton polish and adjoint
+
+// Upload a host scene and index and set up a one-bundle FP64 problem at log focal scale
+// `log_focal`, with the PCG tolerance `tolerance`; the scene's cost words are filled. Rejects the
+// inputs the forward rejects.
+int prepare_exact(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene,
+                  const uint32_t* index, double log_focal, double tolerance, hipStream_t st,
+                  tcv_ba_control& fc, Problem& q, Ops& ops, tcv_ba_report* rep) {
+  TRY(check_control(c));
+  if (!(tolerance >= 1e-14 && tolerance <= 1e-2) || !(std::fabs(log_focal) < 700)) {
+    set_error("tolerance outside [1e-14, 1e-2] or log focal scale outside (-700, 700)");
+    return TCV_BA_ERR_ARGUMENT;
+  }
+  HIP_TRY(hipSetDevice(ctx->device));
+  fc = *c;
+  fc.precision = TCV_BA_FP64;
+  fc.flags &= ~TCV_BA_FLAG_IMPLICIT;
+  const size_t ss = tcv_ba_scene_size(c), is = tcv_ba_index_size(c);
+  TRY(ensure(ctx->probe, ss * 8));
+  TRY(ensure(ctx->index, is * 4));
+  HIP_TRY(hipMemcpyAsync(ctx->probe.ptr, scene, ss * 8, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(ctx->index.ptr, index, is * 4, hipMemcpyHostToDevice, st));
+  TRY(setup(ctx, &fc, 1, (const double*)ctx->probe.ptr, (const uint32_t*)ctx->index.ptr, st, q,
+            rep));
+  q.tol = tolerance;
+  q.exact_gauge = 1;
+  ops = select_ops(&fc);
+  hipLaunchKernelGGL(k_init_bundle, dim3(1), dim3(64), 0, st, q);
+  const double lf = c->shared_focal ? log_focal : 0.0;
+  HIP_TRY(hipMemcpyAsync(q.state + S_LOG_FOCAL, &lf, 8, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_cost<false>, dim3(q.n_cost_blocks), dim3(COST_OBS), 0, st, q);
+  hipLaunchKernelGGL(k_finish_cost<false>, dim3(1), dim3(COST_OBS), 0, st, q);
+  HIP_TRY(hipGetLastError());
+  TRY(read_state(ctx, q, st, rep));
+  const double input_error = word(ctx, 0, S_INPUT_ERROR);
+  if (input_error != 0) {
+    set_error(input_error == 1 ? "GPU BA gauge requires a nonzero baseline"
+                               : "GPU BA observations must initially project in front of their cameras");
+    return TCV_BA_ERR_ARGUMENT;
+  }
+  return TCV_BA_OK;
+}
+
+// One undamped solve of the exact-Hessian system (the right-hand side is -gradient, or q.rhs_in):
+// assembly, PCG to q.tol within `max_cg` iterations. True when PCG converged (or the right-hand
+// side was zero).
+int exact_solve(tcv_ba_context* ctx, const tcv_ba_control* fc, const Problem& q, const Ops& ops,
+                hipStream_t st, uint32_t max_cg, tcv_ba_report* rep, bool& converged) {
+  hipLaunchKernelGGL(k_newton_begin, dim3(1), dim3(64), 0, st, q);
+  ops.exact_passes(q, st);
+  ops.init_cg(q, st);
+  HIP_TRY(hipGetLastError());
+  TRY(read_state(ctx, q, st, rep));
+  if (word(ctx, 0, S_PCG_FAILED) != 0) {
+    converged = false;
+    return TCV_BA_OK;
+  }
+  if (word(ctx, 0, S_PCG_ACTIVE) != 0) {
+    TRY(run_pcg(ctx, fc, q, ops, st, rep, 32, max_cg));
+    if (any_word(ctx, q, S_PCG_ACTIVE))
+      hipLaunchKernelGGL(k_stop_pcg, dim3(1), dim3(64), 0, st, q);
+    TRY(read_state(ctx, q, st, rep));
+    converged = word(ctx, 0, S_CONVERGED) != 0 && word(ctx, 0, S_PCG_FAILED) == 0;
+  } else {
+    converged = true;  // zero right-hand side: x = 0
+  }
+  if (rep) rep->cg_iterations += (uint64_t)word(ctx, 0, S_STEPS);
+  return TCV_BA_OK;
+}
+
+// The right-hand side of the adjoint solve (layout of Problem::rhs_in) from a cotangent on the
+// solve's outputs (R row-major and t per camera, X per point, then the log focal scale):
+//  * each variable camera's chart cotangent for R' = exp(omega) R, t' = exp(omega) t + v:
+//    omega: sum_j R_j x dR_j + t x dt (R_j, dR_j the columns), v: dt; fixed cameras do not move;
+//  * then the transpose of the derivative of the gauge map the forward applies to every candidate:
+//    directional points are renormalised (v_X - X (X . v_X), X unit), and with one fixed camera the
+//    similarity about the anchor centre o rescales the baseline b to the scale view, which adds
+//    -grad(b) (n . v) / b with n the similarity generator (X - o per finite point, -R (C - o) on each
+//    variable camera's v) and grad(b) = -R_s (C_s - o) / b on the scale view's v.
+std::vector<double> adjoint_rhs(const tcv_ba_control* c, const double* scene,
+                                const uint32_t* index, const double* cot) {
+  const size_t C = c->cameras, P = c->points, M = c->observations;
+  const size_t sh = c->shared_focal ? 1 : 0, m = 6 * C + sh;
+  const uint32_t* active = index + 3 * M + P + C + 2;
+  const double* X = scene + 12 * C;
+  std::vector<double> rhs(m + 3 * P, 0.0);
+  auto cross = [](const double* a, const double* b, double* o) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+  };
+  auto center = [&](size_t k, double* o) {
+    const double* p = scene + 12 * k;
+    for (int i = 0; i < 3; i++) o[i] = -(p[i] * p[9] + p[3 + i] * p[10] + p[6 + i] * p[11]);
+  };
+  for (size_t k = 0; k < C; k++) {
+    if (!active[k]) continue;
+    const double* p = scene + 12 * k;
+    const double* d = cot + 12 * k;
+    double w[3] = {0, 0, 0}, x[3];
+    for (int j = 0; j < 3; j++) {
+      const double Rj[3] = {p[j], p[3 + j], p[6 + j]}, dRj[3] = {d[j], d[3 + j], d[6 + j]};
+      cross(Rj, dRj, x);
+      for (int i = 0; i < 3; i++) w[i] += x[i];
+    }
+    cross(p + 9, d + 9, x);
+    for (int i = 0; i < 3; i++) {
+      rhs[6 * k + i] = w[i] + x[i];
+      rhs[6 * k + 3 + i] = d[9 + i];
+    }
+  }
+  if (sh) rhs[6 * C] = cot[12 * C + 3 * P];
+  const size_t first_dir = P - c->directional_points;
+  for (size_t p = 0; p < P; p++) {
+    double* r = &rhs[m + 3 * p];
+    for (int i = 0; i < 3; i++) r[i] = cot[12 * C + 3 * p + i];
+    if (p >= first_dir) {
+      const double* xp = X + 3 * p;
+      const double n2 = xp[0] * xp[0] + xp[1] * xp[1] + xp[2] * xp[2];
+      if (n2 > 0) {
+        const double dot = (xp[0] * r[0] + xp[1] * r[1] + xp[2] * r[2]) / n2;
+        for (int i = 0; i < 3; i++) r[i] -= xp[i] * dot;
+      }
+    }
+  }
+  if (c->normalize_scale) {
+    double o[3], cs[3];
+    center(c->anchor, o);
+    center(c->scale_view, cs);
+    const double d0 = cs[0] - o[0], d1 = cs[1] - o[1], d2 = cs[2] - o[2];
+    const double b = std::sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+    double nv = 0;
+    for (size_t p = 0; p < first_dir; p++)
+      for (int i = 0; i < 3; i++) nv += (X[3 * p + i] - o[i]) * rhs[m + 3 * p + i];
+    for (size_t k = 0; k < C; k++) {
+      if (!active[k]) continue;
+      const double* R = scene + 12 * k;
+      double ck[3];
+      center(k, ck);
+      for (int i = 0; i < 3; i++) {
+        const double n_i = -(R[i * 3] * (ck[0] - o[0]) + R[i * 3 + 1] * (ck[1] - o[1]) +
+                             R[i * 3 + 2] * (ck[2] - o[2]));
+        nv += n_i * rhs[6 * k + 3 + i];
+      }
+    }
+    const double* Rs = scene + 12 * (size_t)c->scale_view;
+    const double dd[3] = {d0, d1, d2};
+    for (int i = 0; i < 3; i++) {
+      const double grad = -(Rs[i * 3] * dd[0] + Rs[i * 3 + 1] * dd[1] + Rs[i * 3 + 2] * dd[2]) / b;
+      rhs[6 * (size_t)c->scale_view + 3 + i] -= grad * nv / b;
+    }
+  }
+  return rhs;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ C ABI
@@ -726,6 +910,7 @@ const char* tcv_ba_status_string(int status) {
     case TCV_BA_ERR_DEVICE: return "GPU BA device call failed";
     case TCV_BA_ERR_MEMORY: return "GPU BA needs more device memory than is free";
     case TCV_BA_ERR_NO_DEVICE: return "no gfx950 HIP device is visible";
+    case TCV_BA_ERR_SOLVE: return "GPU BA adjoint system was not solved";
     default: return "unknown status";
   }
 }
@@ -980,6 +1165,171 @@ int tcv_ba_schur_probe(tcv_ba_context* ctx, const tcv_ba_control* c, const doubl
   if (c->shared_focal && !(ctx->h_state[S_FOCAL_INV] > 0)) bad = 1;
   *status = bad;
   if (rep) rep->device_bytes = held_bytes(ctx);
+  return TCV_BA_OK;
+}
+
+int tcv_ba_polish(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene,
+                  const uint32_t* index, double log_focal, uint32_t lm_iterations,
+                  uint32_t newton_steps, uint32_t max_cg, double tolerance, double* out,
+                  tcv_ba_report* rep) {
+  if (!ctx || !c || !scene || !index || !out || lm_iterations > 1000 || newton_steps > 100 ||
+      max_cg < 1 || max_cg > 100000)
+    return TCV_BA_ERR_ARGUMENT;
+  if (rep) std::memset(rep, 0, sizeof *rep);
+  const double t0 = now_ms();
+  hipStream_t st = ctx->stream;
+  tcv_ba_control fc;
+  Problem q;
+  Ops ops;
+  TRY(prepare_exact(ctx, c, scene, index, log_focal, tolerance, st, fc, q, ops, rep));
+  const double cost0 = word(ctx, 0, S_COST0), sse0 = word(ctx, 0, S_SSE0);
+  const uint32_t update_blocks = blocks_for(q.C > q.P ? q.C : q.P, 256);
+  const uint32_t trial_blocks = blocks_for((size_t)q.C * 12 + (size_t)q.P * 3, 256);
+  auto candidate = [&]() {
+    hipLaunchKernelGGL(k_update_cameras, dim3(blocks_for(q.C, 64)), dim3(64), 0, st, q);
+    ops.update_points(q, st);
+    hipLaunchKernelGGL(k_gauge, dim3(1), dim3(256), 0, st, q);
+    hipLaunchKernelGGL(k_normalize, dim3(update_blocks), dim3(256), 0, st, q);
+  };
+  // 1. Levenberg-Marquardt on the exact Hessian with the forward's damping and accept/reject rule
+  //    (acceptBundleCandidate, bundle.shade.ts:1071-1096) and PCG tolerance, going on while the
+  //    cost still decreases (until the damping passes 1e12 or the iterations end)
+  Problem q1 = q;
+  q1.stop_change = 1e-15;
+  q1.tol = c->relative_tolerance;
+  for (uint32_t it = 0; it < lm_iterations && word(ctx, 0, S_ACTIVE) != 0; it++) {
+    ops.exact_passes(q1, st);
+    ops.init_cg(q1, st);
+    HIP_TRY(hipGetLastError());
+    TRY(read_state(ctx, q1, st, rep));
+    if (word(ctx, 0, S_PCG_ACTIVE) != 0) {
+      TRY(run_pcg(ctx, &fc, q1, ops, st, rep, 32, max_cg));
+      if (any_word(ctx, q1, S_PCG_ACTIVE))
+        hipLaunchKernelGGL(k_stop_pcg, dim3(1), dim3(64), 0, st, q1);
+    }
+    hipLaunchKernelGGL(k_update_cameras, dim3(blocks_for(q.C, 64)), dim3(64), 0, st, q1);
+    ops.update_points(q1, st);
+    hipLaunchKernelGGL(k_gauge, dim3(1), dim3(256), 0, st, q1);
+    hipLaunchKernelGGL(k_normalize, dim3(update_blocks), dim3(256), 0, st, q1);
+    hipLaunchKernelGGL(k_cost<true>, dim3(q.n_cost_blocks), dim3(COST_OBS), 0, st, q1);
+    hipLaunchKernelGGL(k_finish_cost<true>, dim3(1), dim3(COST_OBS), 0, st, q1);
+    hipLaunchKernelGGL(k_commit, dim3(trial_blocks), dim3(256), 0, st, q1);
+    HIP_TRY(hipGetLastError());
+    TRY(read_state(ctx, q, st, rep));
+    if (rep) {
+      rep->lm_iterations++;
+      rep->cg_iterations += (uint64_t)word(ctx, 0, S_STEPS);
+    }
+  }
+  // 2. undamped Newton steps without a cost test (near the solution the cost change falls below
+  //    the rounding of the cost itself)
+  double taken_newton = 0;
+  for (uint32_t it = 0; it < newton_steps; it++) {
+    bool converged = false;
+    TRY(exact_solve(ctx, &fc, q, ops, st, max_cg, rep, converged));
+    if (!converged) break;
+    candidate();
+    hipLaunchKernelGGL(k_newton_accept, dim3(1), dim3(64), 0, st, q);
+    hipLaunchKernelGGL(k_commit, dim3(trial_blocks), dim3(256), 0, st, q);
+    HIP_TRY(hipGetLastError());
+    TRY(read_state(ctx, q, st, rep));
+    if (word(ctx, 0, S_COMMIT) == 0) break;
+    taken_newton += 1;
+  }
+  if (rep) rep->refinement_steps = taken_newton;
+  // the final cost; the initial words keep the polish input's
+  hipLaunchKernelGGL(k_cost<false>, dim3(q.n_cost_blocks), dim3(COST_OBS), 0, st, q);
+  hipLaunchKernelGGL(k_finish_cost<false>, dim3(1), dim3(COST_OBS), 0, st, q);
+  const double initial[2] = {cost0, sse0};
+  HIP_TRY(hipMemcpyAsync(q.state + S_COST0, initial, 16, hipMemcpyHostToDevice, st));
+  const size_t os = tcv_ba_output_size(c);
+  TRY(ensure(ctx->out, os * 8));
+  hipLaunchKernelGGL(k_pack, dim3(blocks_for(os, 256)), dim3(256), 0, st, q, (double*)ctx->out.ptr);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out, ctx->out.ptr, os * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if (rep) {
+    rep->solve_ms = now_ms() - t0;
+    rep->device_bytes = held_bytes(ctx);
+  }
+  return TCV_BA_OK;
+}
+
+int tcv_ba_backward(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene,
+                    const uint32_t* index, double log_focal, const double* cotangent,
+                    uint32_t max_cg, double tolerance, double* pixel_cotangent, double* adjoint,
+                    tcv_ba_report* rep) {
+  if (!ctx || !c || !scene || !index || !cotangent || !pixel_cotangent || max_cg < 1 ||
+      max_cg > 100000)
+    return TCV_BA_ERR_ARGUMENT;
+  if (rep) std::memset(rep, 0, sizeof *rep);
+  const double t0 = now_ms();
+  hipStream_t st = ctx->stream;
+  tcv_ba_control fc;
+  Problem q;
+  Ops ops;
+  TRY(prepare_exact(ctx, c, scene, index, log_focal, tolerance, st, fc, q, ops, rep));
+  const size_t C = q.C, P = q.P, M = q.M, m = q.m;
+  for (size_t k = 0; k < 12 * C + 3 * P + q.shared; k++)
+    if (!std::isfinite(cotangent[k])) {
+      set_error("cotangent has a non-finite entry");
+      return TCV_BA_ERR_ARGUMENT;
+    }
+  const std::vector<double> rhs = adjoint_rhs(c, scene, index, cotangent);
+  TRY(ensure(ctx->rhs, rhs.size() * 8));
+  TRY(ensure(ctx->grad, 2 * M * 8));
+  HIP_TRY(hipMemcpyAsync(ctx->rhs.ptr, rhs.data(), rhs.size() * 8, hipMemcpyHostToDevice, st));
+  q.rhs_in = (const double*)ctx->rhs.ptr;
+  const double ts = now_ms();
+  bool converged = false;
+  TRY(exact_solve(ctx, &fc, q, ops, st, max_cg, rep, converged));
+  if (!converged) {
+    char text[256];
+    if (word(ctx, 0, S_PCG_FAILED) != 0)
+      std::snprintf(text, sizeof text,
+                    "the adjoint system is not positive definite at this scene (not a minimum?): "
+                    "%.0f point tiles, %.0f camera blocks, focal %.0f failed; %.0f PCG steps",
+                    word(ctx, 0, S_FAIL_TILES), word(ctx, 0, S_FAIL_CAMERAS),
+                    word(ctx, 0, S_FAIL_FOCAL), word(ctx, 0, S_STEPS));
+    else
+      std::snprintf(text, sizeof text,
+                    "the adjoint PCG did not reach its tolerance within %u iterations", max_cg);
+    set_error(text);
+    return TCV_BA_ERR_SOLVE;
+  }
+  ops.solve_points(q, st);
+  ops.pixel_cotangent(q, (double*)ctx->grad.ptr, st);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(pixel_cotangent, ctx->grad.ptr, 2 * M * 8, hipMemcpyDeviceToHost, st));
+  if (adjoint) {
+    HIP_TRY(hipMemcpyAsync(adjoint, q.vec, 6 * C * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(adjoint + 6 * C, q.vec + 5 * m, 3 * P * 8, hipMemcpyDeviceToHost, st));
+    if (q.shared)
+      HIP_TRY(hipMemcpyAsync(adjoint + 6 * C + 3 * P, q.vec + 6 * C, 8, hipMemcpyDeviceToHost, st));
+  }
+  HIP_TRY(hipStreamSynchronize(st));
+  if (rep) {
+    rep->solve_ms = now_ms() - t0;
+    rep->schur_ms = now_ms() - ts;
+    rep->device_bytes = held_bytes(ctx);
+  }
+  return TCV_BA_OK;
+}
+
+int tcv_ba_jacobian(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene,
+                    const uint32_t* index, double log_focal, double* out, tcv_ba_report* rep) {
+  if (!ctx || !c || !scene || !index || !out) return TCV_BA_ERR_ARGUMENT;
+  if (rep) std::memset(rep, 0, sizeof *rep);
+  hipStream_t st = ctx->stream;
+  tcv_ba_control fc;
+  Problem q;
+  Ops ops;
+  TRY(prepare_exact(ctx, c, scene, index, log_focal, c->relative_tolerance, st, fc, q, ops, rep));
+  TRY(ensure(ctx->grad, (size_t)q.M * 23 * 8));
+  ops.jacobian(q, (double*)ctx->grad.ptr, st);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out, ctx->grad.ptr, (size_t)q.M * 23 * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
   return TCV_BA_OK;
 }
 

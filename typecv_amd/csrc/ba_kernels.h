@@ -44,6 +44,9 @@ enum : int {
   S_MATVECS = 31,        // Schur products that ran while PCG was active (whole solve)
   S_REFINEMENTS = 32,    // mixed form: refinement restarts taken (whole solve)
   S_RHO_NEXT = 33,       // rho of the next PCG iteration (becomes word 18 at its first launch)
+  S_FAIL_TILES = 34,     // point tiles with a failed point block at the last PCG set-up
+  S_FAIL_CAMERAS = 35,   // cameras with a failed preconditioner block at the last PCG set-up
+  S_FAIL_FOCAL = 36,     // 1 when the reduced focal diagonal was not positive
   S_WORDS = 40
 };
 
@@ -82,6 +85,8 @@ struct Problem {
   uint32_t n_tiles, n_tasks, n_cost_blocks, n_groups;
   double huber, tol;
   double refine_tol;  // mixed form: relative FP64 residual of the outer refinement
+  double stop_change; // an accepted step whose relative cost change is below this ends the LM loop
+                      // (1e-9, acceptBundleCandidate b:1082-1095)
   double* scene;
   double* trial;
   const uint32_t* index;
@@ -111,6 +116,11 @@ struct Problem {
   double* state;       // S_WORDS per bundle
   double* cost_part;   // 3 per cost block
   uint32_t* error;     // index validation flag
+  // a given right-hand side (the adjoint solve): b_c (6C), b_f (one per bundle), then b_p (3P);
+  // null: the Gauss-Newton gradient -J^T w r
+  const double* rhs_in;
+  // exact-Hessian systems (polish, adjoint): regularise the one-fixed-camera similarity gauge
+  uint32_t exact_gauge;
 };
 
 __host__ __device__ inline const uint32_t* obs_camera(const Problem& q) { return q.index; }

@@ -109,6 +109,11 @@ struct Lin {
   double r[2];    // projected - observed pixel
   double w;       // Huber weight
   double Jf[2];   // d(u,v)/d(log f) = pixel - principal point
+  // the intermediates the exact Hessian reuses (dead, and removed, where unused)
+  double qc[3];   // camera-frame point
+  double du[3], dv[3];  // d(u,v)/dq
+  double R[9];    // rotation, row-major
+  Model k;        // camera model at the focal scale
 };
 
 __device__ __forceinline__ bool linearize(const Problem& q, uint32_t c, uint32_t p, uint32_t i,
@@ -159,7 +164,84 @@ __device__ __forceinline__ bool linearize(const Problem& q, uint32_t c, uint32_t
   }
   L.Jf[0] = px - k.cx;
   L.Jf[1] = py - k.cy;
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    L.qc[j] = qc[j];
+    L.du[j] = du[j];
+    L.dv[j] = dv[j];
+  }
+#pragma unroll
+  for (int j = 0; j < 9; j++) L.R[j] = R[j];
+  L.k = k;
   return true;
+}
+
+// Packed upper triangle of a symmetric 10x10 block over (omega 3, v 3, X 3, log f).
+__host__ __device__ constexpr int up10(int j, int l) { return j * 10 - j * (j - 1) / 2 + (l - j); }
+
+// The exact Hessian of one observation's half robust cost minus its Gauss-Newton block w J^T J:
+//   C = J^T (D - w I) J + sum_k rho_k d2(pixel_k)/dtheta2,   rho = w r,
+// D = d(w r)/dr (I inside the Huber radius, w (I - r r^T / |r|^2) outside), in the left-perturbation
+// chart q = exp(omega)(R X + t) + v of movePose (q = exp(omega) R X for a directional point). Its
+// parts: -w a a^T with a = J^T r/|r| outside the radius; Q^T G Q with Q = dq/dtheta and G the
+// weighted curvature of the projection; g . d2q/dtheta2 with g = rho_u du + rho_v dv, i.e.
+// 1/2 (g q^T + q g^T) - (g . q) I on omega-omega and (R_j x g) on omega-X_j; and rho . J on the
+// focal row and column (d pixel/d log f = pixel - c). Rows of a fixed camera stay zero.
+__host__ __device__ void exact_correction(const Lin& L, bool var, bool dir, double huber, double C[55]) {
+  for (int k = 0; k < 55; k++) C[k] = 0;
+  double J[2][10];
+  for (int k = 0; k < 6; k++) {
+    J[0][k] = L.Jc[k];
+    J[1][k] = L.Jc[6 + k];
+  }
+  for (int k = 0; k < 3; k++) {
+    J[0][6 + k] = L.Jp[k];
+    J[1][6 + k] = L.Jp[3 + k];
+  }
+  J[0][9] = L.Jf[0];
+  J[1][9] = L.Jf[1];
+  const double rho0 = L.w * L.r[0], rho1 = L.w * L.r[1];
+  const double e = sqrt(L.r[0] * L.r[0] + L.r[1] * L.r[1]);
+  if (e > huber) {
+    const double h0 = L.r[0] / e, h1 = L.r[1] / e;
+    double a[10];
+    for (int k = 0; k < 10; k++) a[k] = J[0][k] * h0 + J[1][k] * h1;
+    for (int j = 0; j < 10; j++)
+      for (int l = j; l < 10; l++) C[up10(j, l)] -= L.w * a[j] * a[l];
+  }
+  double G[9];
+  projection_curvature(L.k, L.qc, rho0, rho1, G);
+  // Q = dq/dtheta over omega, v, X (columns)
+  double Q[3][9];
+  const double* q = L.qc;
+  const double cross[3][3] = {{0, -q[2], q[1]}, {q[2], 0, -q[0]}, {-q[1], q[0], 0}};  // e_a x q
+  for (int a = 0; a < 3; a++)
+    for (int m = 0; m < 3; m++) {
+      Q[m][a] = var ? cross[a][m] : 0.0;
+      Q[m][3 + a] = (var && !dir && m == a) ? 1.0 : 0.0;
+      Q[m][6 + a] = L.R[m * 3 + a];
+    }
+  double GQ[3][9];
+  for (int m = 0; m < 3; m++)
+    for (int j = 0; j < 9; j++) GQ[m][j] = G[m * 3] * Q[0][j] + G[m * 3 + 1] * Q[1][j] + G[m * 3 + 2] * Q[2][j];
+  for (int i = 0; i < 9; i++)
+    for (int j = i; j < 9; j++) C[up10(i, j)] += Q[0][i] * GQ[0][j] + Q[1][i] * GQ[1][j] + Q[2][i] * GQ[2][j];
+  const double g[3] = {rho0 * L.du[0] + rho1 * L.dv[0], rho0 * L.du[1] + rho1 * L.dv[1],
+                       rho0 * L.du[2] + rho1 * L.dv[2]};
+  if (var) {
+    const double gq = g[0] * q[0] + g[1] * q[1] + g[2] * q[2];
+    for (int a = 0; a < 3; a++)
+      for (int b = a; b < 3; b++)
+        C[up10(a, b)] += 0.5 * (g[a] * q[b] + q[a] * g[b]) - (a == b ? gq : 0.0);
+    for (int j = 0; j < 3; j++) {
+      const double Rj[3] = {L.R[j], L.R[3 + j], L.R[6 + j]};
+      const double x[3] = {Rj[1] * g[2] - Rj[2] * g[1], Rj[2] * g[0] - Rj[0] * g[2],
+                           Rj[0] * g[1] - Rj[1] * g[0]};
+      for (int a = 0; a < 3; a++) C[up10(a, 6 + j)] += x[a];
+    }
+  }
+  for (int k = 0; k < 9; k++) C[up10(k, 9)] += rho0 * J[0][k] + rho1 * J[1][k];
+  C[up10(9, 9)] += rho0 * J[0][9] + rho1 * J[1][9];
 }
 
 // Cross block W = w Jc^T Jp (18 words, [k*3+j]) and the focal row w Jf^T Jp (3 words).
@@ -364,7 +446,7 @@ __global__ __launch_bounds__(COST_OBS) void k_finish_cost(Problem q) {
     s[S_ACCEPTED] += 1;
     s[S_COMMIT] = 1;
     s[S_LAMBDA] = fmax(1e-10, s[S_LAMBDA] / 3);
-    if (change < 1e-9 * fmax(1.0, cost)) s[S_ACTIVE] = 0;
+    if (change < q.stop_change * fmax(1.0, cost)) s[S_ACTIVE] = 0;
   } else {
     s[S_REJECTED] += 1;
     s[S_LAMBDA] *= 10;
@@ -379,7 +461,7 @@ __global__ __launch_bounds__(COST_OBS) void k_finish_cost(Problem q) {
 // (point-major), stage the point terms in LDS; then one lane per point sums its observations in
 // order, damps V, inverts it and writes the point block. The shared route adds the point's share of
 // the reduced focal diagonal and RHS (bundleFocalPoints), summed per tile in point order.
-template <class WT, bool SHARED>
+template <class WT, bool SHARED, bool EXACT>
 __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_pass(Problem q) {
   constexpr int NS = SHARED ? 13 : 10;  // staged words per observation: V 6, b 3, [fc 3], bad
   __shared__ double st_all[TILE_WAVES][TILE][NS];
@@ -409,12 +491,20 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_pass(Problem q) {
     const uint32_t i = base + lane;
     if (i < e) {
       const uint32_t c = obs_camera(q)[i], p = obs_point(q)[i];
+      const bool var = variable_camera(q, c), dir = directional_point(g, p);
       Lin L;
-      const bool ok =
-          linearize(q, c, p, i, fscale, variable_camera(q, c), directional_point(g, p), L);
+      const bool ok = linearize(q, c, p, i, fscale, var, dir, L);
       double W[21];
+      double Cx[EXACT ? 55 : 1];
       if (ok) {
         cross_block(L, SHARED, W);
+        if (EXACT) {
+          exact_correction(L, var, dir, q.huber, Cx);
+          for (int k = 0; k < 6; k++)
+            for (int j = 0; j < 3; j++) W[k * 3 + j] += Cx[up10(k, 6 + j)];
+          if (SHARED)
+            for (int j = 0; j < 3; j++) W[18 + j] += Cx[up10(6 + j, 9)];
+        }
       } else {
         for (int k = 0; k < 21; k++) W[k] = 0;
       }
@@ -433,6 +523,15 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_pass(Problem q) {
         o[6] = -(a0 * L.r[0]) - c0 * L.r[1];
         o[7] = -(a1 * L.r[0]) - c1 * L.r[1];
         o[8] = -(a2 * L.r[0]) - c2 * L.r[1];
+        if (EXACT) {
+          o[0] += Cx[up10(6, 6)];
+          o[1] += Cx[up10(6, 7)];
+          o[2] += Cx[up10(6, 8)];
+          o[3] += Cx[up10(7, 7)];
+          o[4] += Cx[up10(7, 8)];
+          o[5] += Cx[up10(8, 8)];
+          if (q.rhs_in) o[6] = o[7] = o[8] = 0;
+        }
         if (SHARED) {
           o[9] = W[18];
           o[10] = W[19];
@@ -463,11 +562,22 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_pass(Problem q) {
   if (has) {
     double inv[9];
     failed = bad;
+    if (EXACT && q.rhs_in)
+      for (int k = 0; k < 3; k++) bp[k] = q.rhs_in[(size_t)q.m + (size_t)my * 3 + k];
     if (!failed) {
       double A[9] = {V[0], V[1], V[2], V[1], V[3], V[4], V[2], V[4], V[5]};
       A[0] += lambda * fmax(A[0], 1e-10);
       A[4] += lambda * fmax(A[4], 1e-10);
       A[8] += lambda * fmax(A[8], 1e-10);
+      if (q.exact_gauge && directional_point(g, my)) {
+        // the radial direction of a directional point leaves every projection unchanged: a zero
+        // row and column of the Hessian. Fix it with mu X X^T (X is unit after normalisation);
+        // the right-hand sides carry no radial component, so the solution keeps none.
+        const double* X = q.scene + (size_t)q.C * 12 + (size_t)my * 3;
+        const double mu = fmax(fmax(A[0], A[4]), fmax(A[8], 1e-10));
+        for (int j = 0; j < 3; j++)
+          for (int k = 0; k < 3; k++) A[j * 3 + k] += mu * X[j] * X[k];
+      }
       failed = !inverse3(A, inv);
     }
     double* out = q.pblk + (size_t)my * 12;
@@ -514,7 +624,7 @@ __host__ __device__ constexpr int up6(int j, int l) { return j * 6 - j * (j - 1)
 //   U = sum w Jc^T Jc (21), Q = sum W V^-1 W^T (21), rhs = -sum w Jc^T r - W V^-1 b_p (6)
 //   [shared] H_cf = sum w Jc^T Jf (6), H_ff = sum w Jf^T Jf, b_f = -sum w Jf^T r
 // into ASM_WORDS per task, in a fixed order.
-template <class WT, bool SHARED>
+template <class WT, bool SHARED, bool EXACT>
 __global__ __launch_bounds__(TASK_THREADS) void k_camera_pass(Problem q) {
   constexpr int NV = SHARED ? 56 : 48;
   __shared__ double red[TASK_THREADS / 64][NV];
@@ -532,17 +642,28 @@ __global__ __launch_bounds__(TASK_THREADS) void k_camera_pass(Problem q) {
   for (int k = 0; k < NV; k++) acc[k] = 0;
   for (uint32_t n = s + threadIdx.x; n < e; n += TASK_THREADS) {
     const uint32_t i = camera_obs(q)[n], p = q.cam_point[n];
+    const bool dir = directional_point(g, p);
     Lin L;
-    const bool ok = linearize(q, c, p, i, fscale, var, directional_point(g, p), L);
+    const bool ok = linearize(q, c, p, i, fscale, var, dir, L);
     double W[21];
+    double Cx[EXACT ? 55 : 1];
     if (ok) {
       cross_block(L, SHARED, W);
+      if (EXACT) {
+        exact_correction(L, var, dir, q.huber, Cx);
+        for (int k = 0; k < 6; k++)
+          for (int j = 0; j < 3; j++) W[k * 3 + j] += Cx[up10(k, 6 + j)];
+        if (SHARED)
+          for (int j = 0; j < 3; j++) W[18 + j] += Cx[up10(6 + j, 9)];
+      }
     } else {
       for (int k = 0; k < 21; k++) W[k] = 0;
     }
     for (uint32_t k = 0; k < K; k++) *wptr<WT, SHARED>(q.Wc, k, n, q.M) = (WT)W[k];
     if (!ok) continue;
     const double w = L.w;
+    // with a right-hand side given, the camera's own b_c enters once, in k_camera_finish
+    const double rw = (EXACT && q.rhs_in) ? 0.0 : 1.0;
     if (var) {
       const double* pb = q.pblk + (size_t)p * 12;
       double Vi[9], bpv[3];
@@ -560,17 +681,20 @@ __global__ __launch_bounds__(TASK_THREADS) void k_camera_pass(Problem q) {
 #pragma unroll
         for (int l = j; l < 6; l++) {
           acc[up6(j, l)] += wj0 * L.Jc[l] + wj1 * L.Jc[6 + l];
+          if (EXACT) acc[up6(j, l)] += Cx[up10(j, l)];
           acc[21 + up6(j, l)] +=
               T[j * 3] * W[l * 3] + T[j * 3 + 1] * W[l * 3 + 1] + T[j * 3 + 2] * W[l * 3 + 2];
         }
-        acc[42 + j] += -(wj0 * L.r[0]) - wj1 * L.r[1] -
+        acc[42 + j] += rw * (-(wj0 * L.r[0]) - wj1 * L.r[1]) -
                        (T[j * 3] * bpv[0] + T[j * 3 + 1] * bpv[1] + T[j * 3 + 2] * bpv[2]);
         if (SHARED) acc[48 + j] += wj0 * L.Jf[0] + wj1 * L.Jf[1];
+        if (SHARED && EXACT) acc[48 + j] += Cx[up10(j, 9)];
       }
     }
     if (SHARED) {
       acc[54] += w * (L.Jf[0] * L.Jf[0] + L.Jf[1] * L.Jf[1]);
-      acc[55] += -(w * (L.Jf[0] * L.r[0] + L.Jf[1] * L.r[1]));
+      if (EXACT) acc[54] += Cx[up10(9, 9)];
+      acc[55] += rw * -(w * (L.Jf[0] * L.r[0] + L.Jf[1] * L.r[1]));
     }
   }
 #pragma unroll
@@ -624,12 +748,36 @@ __global__ void k_camera_finish(Problem q) {
       S[j * 6 + l] = S[l * 6 + j] = acc[21 + up6(j, l)];
     }
   for (int j = 0; j < 6; j++) a[j * 6 + j] += lambda * fmax(a[j * 6 + j], 1e-10);
+  if (q.exact_gauge) {
+    // The exact Hessian of a one-fixed-camera bundle is singular along the similarity about the
+    // anchor centre o (the gauge the forward fixes by rescaling the baseline b = |C_s - o|). Adding
+    // mu a a^T to the scale view's translation block, a = R_s (C_s - o) / b (unit; the baseline's
+    // gradient up to sign), makes it definite without changing the solution: for a right-hand side
+    // orthogonal to the gauge direction n (a gradient, or a gauge-projected cotangent) and a.n != 0,
+    // the solution of H x = b with a.x = 0 also solves (H + mu a a^T) x = b, for every mu > 0.
+    const uint32_t b = q.camera_bundle[c];
+    const Bundle& g = q.bundles[b];
+    if (g.normalize_scale && c == g.scale_view) {
+      const double* pose = q.scene + (size_t)c * 12;
+      const double* o = bstate(q, b) + S_ORIGIN;
+      double C[3], d[3], av[3];
+      center_of(pose, C);
+      for (int i = 0; i < 3; i++) d[i] = C[i] - o[i];
+      const double base = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+      for (int i = 0; i < 3; i++)
+        av[i] = (pose[i * 3] * d[0] + pose[i * 3 + 1] * d[1] + pose[i * 3 + 2] * d[2]) / base;
+      const double mu = fmax(fmax(a[21], a[28]), fmax(a[35], 1e-10));
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) a[(3 + i) * 6 + 3 + j] += mu * av[i] * av[j];
+    }
+  }
   for (int j = 0; j < 36; j++) {
     A[j] = a[j];
     S[j] = a[j] - S[j];
   }
   for (int j = 0; j < 6; j++) {
     rhs[j] = acc[42 + j];
+    if (q.rhs_in) rhs[j] += q.rhs_in[(size_t)c * 6 + j];
     if (SHARED) q.camHcf[(size_t)c * 6 + j] = acc[48 + j];
   }
   Chol6 f;
@@ -683,10 +831,18 @@ __global__ __launch_bounds__(CG_THREADS) void k_init_cg(Problem q) {
   if (tid == 0)
     for (int j = S_RHO; j < 26; j++) s[j] = 0;
   if (active == 0) return;
-  double fail = 0;
-  for (uint32_t t = g.tile0 + tid; t < g.tile1; t += CG_THREADS) fail += q.tile_fail[t] ? 1.0 : 0.0;
+  double fail_tiles = 0, fail_cameras = 0;
+  for (uint32_t t = g.tile0 + tid; t < g.tile1; t += CG_THREADS)
+    fail_tiles += q.tile_fail[t] ? 1.0 : 0.0;
   for (uint32_t c = g.cam0 + tid; c < g.cam1; c += CG_THREADS)
-    if (!(q.camInv[(size_t)c * 36] > 0)) fail += 1;
+    if (!(q.camInv[(size_t)c * 36] > 0)) fail_cameras += 1;
+  block_sum2(fail_tiles, fail_cameras, lds);
+  double fail = fail_tiles + fail_cameras;
+  if (tid == 0) {
+    s[S_FAIL_TILES] = fail_tiles;
+    s[S_FAIL_CAMERAS] = fail_cameras;
+    s[S_FAIL_FOCAL] = 0;
+  }
   double hff = 0, bf = 0, tf = 0, tr = 0;
   if (SHARED) {
     for (uint32_t c = g.cam0 + tid; c < g.cam1; c += CG_THREADS) {
@@ -703,6 +859,7 @@ __global__ __launch_bounds__(CG_THREADS) void k_init_cg(Problem q) {
     tr = block_sum(tr, lds);
   }
   double finv = 0, frhs = 0;
+  if (SHARED && q.rhs_in) bf = q.rhs_in[g.focal_row];
   if (SHARED) {
     const double damped = hff + lambda * fmax(hff, 1e-10);
     const double diag = damped - tf;
@@ -713,9 +870,11 @@ __global__ __launch_bounds__(CG_THREADS) void k_init_cg(Problem q) {
       s[S_FOCAL_INV] = finv;
       s[S_FOCAL_RHS] = frhs;
     }
-    if (!(finv > 0)) fail += 1;
+    if (!(finv > 0)) {
+      fail += 1;
+      if (tid == 0) s[S_FAIL_FOCAL] = 1;
+    }
   }
-  fail = block_sum(fail, lds);
   if (fail > 0) {
     if (tid == 0) s[S_PCG_FAILED] = 1;
     return;
@@ -762,7 +921,8 @@ __global__ __launch_bounds__(CG_THREADS) void k_init_cg(Problem q) {
 // The point sweep over tiles of whole points, one lane per observation.
 //   SCHUR:  y_p = V_p^-1 sum_i (W_i^T v_c(i) + w_fi v_f)                (bundleSchurPointAt)
 //   UPDATE: X_p' = X_p + V_p^-1 (b_p - sum_i (W_i^T x_c(i) + w_fi x_f))  (updateBundleCandidate)
-enum SweepMode { SWEEP_SCHUR = 0, SWEEP_UPDATE = 1 };
+//   SOLVE:  y_p = V_p^-1 (b_p - sum_i (W_i^T x_c(i) + w_fi x_f))  (the point part of a solve)
+enum SweepMode { SWEEP_SCHUR = 0, SWEEP_UPDATE = 1, SWEEP_SOLVE = 2 };
 
 template <class WT, bool SHARED, int MODE>
 __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_sweep(Problem q) {
@@ -775,7 +935,7 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_sweep(Problem q) {
   const uint32_t fp = q.tiles[t], ep = q.tiles[t + 1], b = q.tile_bundle[t];
   const double* sb = bstate(q, b);
   if (MODE == SWEEP_SCHUR && sb[S_PCG_ACTIVE] == 0) return;
-  if (MODE == SWEEP_UPDATE && (sb[S_ACTIVE] == 0 || sb[S_PCG_FAILED] != 0)) return;
+  if (MODE != SWEEP_SCHUR && (sb[S_ACTIVE] == 0 || sb[S_PCG_FAILED] != 0)) return;
   const double* v = MODE == SWEEP_SCHUR ? vp(q) : vx(q);
   auto& st = st_all[wave];
   const uint32_t* ps = point_start(q);
@@ -820,11 +980,15 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_sweep(Problem q) {
   if (MODE == SWEEP_SCHUR) {
     double* y = vy(q) + (size_t)my * 3;
     for (int j = 0; j < 3; j++) y[j] = pb[j * 3] * acc[0] + pb[j * 3 + 1] * acc[1] + pb[j * 3 + 2] * acc[2];
-  } else {
+  } else if (MODE == SWEEP_UPDATE) {
     const double r0 = pb[9] - acc[0], r1 = pb[10] - acc[1], r2 = pb[11] - acc[2];
     const size_t at = (size_t)q.C * 12 + (size_t)my * 3;
     for (int j = 0; j < 3; j++)
       q.trial[at + j] = q.scene[at + j] + (pb[j * 3] * r0 + pb[j * 3 + 1] * r1 + pb[j * 3 + 2] * r2);
+  } else {
+    const double r0 = pb[9] - acc[0], r1 = pb[10] - acc[1], r2 = pb[11] - acc[2];
+    double* y = vy(q) + (size_t)my * 3;
+    for (int j = 0; j < 3; j++) y[j] = pb[j * 3] * r0 + pb[j * 3 + 1] * r1 + pb[j * 3 + 2] * r2;
   }
 }
 
@@ -1412,6 +1576,99 @@ __global__ void k_commit(Problem q) {
   if (s[S_COMMIT] == 0) return;
   q.scene[i] = q.trial[i];
   if (q.shared && i == (size_t)q.bundles[b].cam0 * 12) s[S_LOG_FOCAL] = s[S_TRIAL_FOCAL];
+}
+
+// ------------------------------------------------------------------ Newton polish and adjoint
+
+// A Newton iteration of the polish, per bundle: undamped (lambda 0) on the exact Hessian.
+__global__ void k_newton_begin(Problem q) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= q.B) return;
+  double* s = bstate(q, b);
+  s[S_LAMBDA] = 0;
+  s[S_ACTIVE] = s[S_INPUT_ERROR] == 0 ? 1.0 : 0.0;
+  s[S_COMMIT] = 0;
+}
+
+// The Newton step is kept whenever the gauge admitted it: there is no cost test, since near the
+// solution the cost change falls below the rounding of the cost itself.
+__global__ void k_newton_accept(Problem q) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= q.B) return;
+  double* s = bstate(q, b);
+  const bool take = s[S_ACTIVE] != 0 && s[S_TRIAL_VALID] != 0;
+  s[S_COMMIT] = take ? 1.0 : 0.0;
+  if (take) s[S_ACCEPTED] += 1;
+}
+
+// The pixel cotangent of the adjoint, one thread per observation: with lambda the solution of
+// H lambda = v (cameras in x, the focal row in x, points in y),
+//   d<v, theta*>/d u_i = D_i J_i lambda,
+// D_i = d(w r)/dr at the observation (the implicit function theorem on sum_i J_i^T w_i r_i = 0 with
+// r_i = pixel_i(theta) - u_i). An observation that does not project gets 0.
+template <bool SHARED>
+__global__ void k_pixel_cotangent(Problem q, double* out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= q.M) return;
+  const uint32_t c = obs_camera(q)[i], p = obs_point(q)[i];
+  const uint32_t b = q.camera_bundle[c];
+  const double* sb = bstate(q, b);
+  const Bundle& g = q.bundles[b];
+  const double fscale = SHARED ? focal_scale(sb[S_LOG_FOCAL]) : 1.0;
+  Lin L;
+  double t0 = 0, t1 = 0;
+  if (linearize(q, c, p, i, fscale, variable_camera(q, c), directional_point(g, p), L)) {
+    const double* x = vx(q);
+    const double* y = vy(q) + (size_t)p * 3;
+    for (int k = 0; k < 6; k++) {
+      t0 += L.Jc[k] * x[(size_t)c * 6 + k];
+      t1 += L.Jc[6 + k] * x[(size_t)c * 6 + k];
+    }
+    for (int k = 0; k < 3; k++) {
+      t0 += L.Jp[k] * y[k];
+      t1 += L.Jp[3 + k] * y[k];
+    }
+    if (SHARED) {
+      t0 += L.Jf[0] * x[g.focal_row];
+      t1 += L.Jf[1] * x[g.focal_row];
+    }
+    const double e = sqrt(L.r[0] * L.r[0] + L.r[1] * L.r[1]);
+    if (e > q.huber) {
+      const double h0 = L.r[0] / e, h1 = L.r[1] / e, d = h0 * t0 + h1 * t1;
+      t0 = L.w * (t0 - h0 * d);
+      t1 = L.w * (t1 - h1 * d);
+    }
+  }
+  out[2 * (size_t)i] = t0;
+  out[2 * (size_t)i + 1] = t1;
+}
+
+// The per-observation linearisation (the Jacobian kernels' output): 23 words per observation,
+// r (2), d(u,v)/d(omega, v) (12, row u then row v), d(u,v)/dX (6), d(u,v)/d log f (2), w; all
+// zero, w = -1, for an observation that does not project.
+template <bool SHARED>
+__global__ void k_jacobian(Problem q, double* out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= q.M) return;
+  const uint32_t c = obs_camera(q)[i], p = obs_point(q)[i];
+  const uint32_t b = q.camera_bundle[c];
+  const double* sb = bstate(q, b);
+  const Bundle& g = q.bundles[b];
+  const double fscale = SHARED ? focal_scale(sb[S_LOG_FOCAL]) : 1.0;
+  Lin L;
+  double* o = out + (size_t)i * 23;
+  if (!linearize(q, c, p, i, fscale, variable_camera(q, c), directional_point(g, p), L)) {
+    for (int k = 0; k < 22; k++) o[k] = 0;
+    o[22] = -1;
+    return;
+  }
+  o[0] = L.r[0];
+  o[1] = L.r[1];
+  for (int k = 0; k < 12; k++) o[2 + k] = L.Jc[k];
+  for (int k = 0; k < 6; k++) o[14 + k] = L.Jp[k];
+  o[20] = L.Jf[0];
+  o[21] = L.Jf[1];
+  o[22] = L.w;
 }
 
 // packBundleResult: coordinates of the whole batch, then per bundle its state words with the

@@ -16,7 +16,7 @@ struct Model {
 
 // radialLimit, numerics.shade.ts:79-99: the squared radius where the radial map stops being
 // injective (the first zero of d(r·D(r))/dr), or 1e30.
-__device__ __forceinline__ double radial_limit(const Model& k) {
+__host__ __device__ __forceinline__ double radial_limit(const Model& k) {
   if (k.k2 == 0) {
     if (k.k1 < 0) return -1 / (3 * k.k1);
     return 1e30;
@@ -37,7 +37,7 @@ __device__ __forceinline__ double radial_limit(const Model& k) {
 }
 
 // project, numerics.shade.ts:141-173: pixel and the rows d(pixel)/d(q) of camera-frame point q.
-__device__ __forceinline__ bool project(const Model& k, const double q[3], double& px, double& py,
+__host__ __device__ __forceinline__ bool project(const Model& k, const double q[3], double& px, double& py,
                                         double du[3], double dv[3]) {
   if (!(q[2] > 1e-12)) return false;
   const double x = q[0] / q[2], y = q[1] / q[2], r2 = x * x + y * y;
@@ -59,8 +59,41 @@ __device__ __forceinline__ bool project(const Model& k, const double q[3], doubl
   return mp < 1e12 && mu < 1e12 && mv < 1e12;
 }
 
+// The residual-weighted second derivative of the projection: G = a0 d2u/dq2 + a1 d2v/dq2 (3x3,
+// row-major, symmetric) at a camera-frame point q that `project` accepted. With x = q0/q2,
+// y = q1/q2, r2 = x^2 + y^2, D = 1 + k1 r2 + k2 r2^2, D' = k1 + 2 k2 r2, D'' = 2 k2, the image-plane
+// map m = (x D, y D) has
+//   d2 m_x/dx2 = 6x D' + 4x^3 D'',  d2 m_x/dxdy = 2y D' + 4x^2 y D'',  d2 m_x/dy2 = 2x D' + 4x y^2 D''
+// (m_y by exchanging x and y), and the chain through (x, y)(q) adds the curvature of x = q0/q2 and
+// y = q1/q2: d2x/dq0dq2 = -1/q2^2, d2x/dq2^2 = 2x/q2^2 (likewise y).
+__host__ __device__ __forceinline__ void projection_curvature(const Model& k, const double q[3], double a0,
+                                                     double a1, double G[9]) {
+  const double s = 1 / q[2], x = q[0] * s, y = q[1] * s, r2 = x * x + y * y;
+  const double D = 1 + k.k1 * r2 + k.k2 * r2 * r2, D1 = k.k1 + 2 * k.k2 * r2, D2 = 2 * k.k2;
+  const double a = a0 * k.fx, c = a1 * k.fy;
+  // first derivatives of m in (x, y), weighted
+  const double gx = a * (D + 2 * x * x * D1) + c * (2 * x * y * D1);
+  const double gy = a * (2 * x * y * D1) + c * (D + 2 * y * y * D1);
+  // second derivatives of m in (x, y), weighted
+  const double mxx = a * (6 * x * D1 + 4 * x * x * x * D2) + c * (2 * y * D1 + 4 * x * x * y * D2);
+  const double mxy = a * (2 * y * D1 + 4 * x * x * y * D2) + c * (2 * x * D1 + 4 * x * y * y * D2);
+  const double myy = a * (2 * x * D1 + 4 * x * y * y * D2) + c * (6 * y * D1 + 4 * y * y * y * D2);
+  // d(x, y)/dq = s [[1, 0, -x], [0, 1, -y]]
+  const double Jx[3] = {s, 0, -x * s}, Jy[3] = {0, s, -y * s};
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++)
+      G[i * 3 + j] = Jx[i] * (mxx * Jx[j] + mxy * Jy[j]) + Jy[i] * (mxy * Jx[j] + myy * Jy[j]);
+  // curvature of x(q), y(q)
+  const double s2 = s * s;
+  G[0 * 3 + 2] += -gx * s2;
+  G[2 * 3 + 0] += -gx * s2;
+  G[1 * 3 + 2] += -gy * s2;
+  G[2 * 3 + 1] += -gy * s2;
+  G[2 * 3 + 2] += 2 * (gx * x + gy * y) * s2;
+}
+
 // robustCost, numerics.shade.ts:184-187: Huber cost of a pixel error.
-__device__ __forceinline__ double robust_cost(double error, double huber) {
+__host__ __device__ __forceinline__ double robust_cost(double error, double huber) {
   if (error <= huber) return error * error;
   return 2 * huber * error - huber * huber;
 }
@@ -148,7 +181,7 @@ __device__ __forceinline__ void chol6_solve(const Chol6& f, const double rhs[6],
 }
 
 // movePose, numerics.shade.ts:254-281: R' = exp([w]x) R, t' = exp([w]x) t + v.
-__device__ __forceinline__ void move_pose(const double p[12], const double delta[6], double out[12]) {
+__host__ __device__ __forceinline__ void move_pose(const double p[12], const double delta[6], double out[12]) {
   const double x = delta[0], y = delta[1], z = delta[2];
   const double theta = sqrt(x * x + y * y + z * z);
   double a = 1 - (theta * theta) / 6, b = 0.5 - (theta * theta) / 24;
