@@ -1217,3 +1217,383 @@ int tcv_oracle_gradient(const tcv_ba_control* ctl, const double* scene, const ui
   oracle_free(&o);
   return skipped;
 }
+
+/* ---------------------------------------------------------------- the dense CPU reference */
+
+/* inverse3, numerics.ts:100-131: closed-form adjugate inverse; 0 for a singular block. */
+static int adjugate_inverse3(const double a[9], double out[9]) {
+  const double a00 = a[0], a01 = a[1], a02 = a[2], a10 = a[3], a11 = a[4], a12 = a[5], a20 = a[6],
+               a21 = a[7], a22 = a[8];
+  const double c00 = a11 * a22 - a12 * a21, c01 = a02 * a21 - a01 * a22, c02 = a01 * a12 - a02 * a11;
+  const double det = a00 * c00 + a10 * c01 + a20 * c02;
+  if (!isfinite(det) || det == 0) return 0;
+  out[0] = c00 / det;
+  out[1] = c01 / det;
+  out[2] = c02 / det;
+  out[3] = (a12 * a20 - a10 * a22) / det;
+  out[4] = (a00 * a22 - a02 * a20) / det;
+  out[5] = (a02 * a10 - a00 * a12) / det;
+  out[6] = (a10 * a21 - a11 * a20) / det;
+  out[7] = (a01 * a20 - a00 * a21) / det;
+  out[8] = (a00 * a11 - a01 * a10) / det;
+  for (int k = 0; k < 9; k++)
+    if (!isfinite(out[k])) return 0;
+  return 1;
+}
+
+/* solveSystem, numerics.ts:70-98: Gaussian elimination with partial pivoting on [A | b]; 0 when a
+ * pivot falls below 1e-14 times the largest |a_ij| or the system is not finite. */
+static int solve_system(const double* A, const double* b, uint32_t n, double* out) {
+  double* a = malloc((size_t)n * (n + 1) * sizeof(double));
+  if (!a) return 0;
+  double scale = 0;
+  for (uint32_t i = 0; i < n; i++) {
+    for (uint32_t j = 0; j < n; j++) {
+      const double v = A[(size_t)i * n + j];
+      if (!isfinite(v)) {
+        free(a);
+        return 0;
+      }
+      a[(size_t)i * (n + 1) + j] = v;
+      if (fabs(v) > scale) scale = fabs(v);
+    }
+    a[(size_t)i * (n + 1) + n] = b[i];
+    if (!isfinite(b[i])) {
+      free(a);
+      return 0;
+    }
+  }
+  if (!(scale > 0)) {
+    free(a);
+    return n == 0;
+  }
+  for (uint32_t j = 0; j < n; j++) {
+    uint32_t p = j;
+    for (uint32_t i = j + 1; i < n; i++)
+      if (fabs(a[(size_t)i * (n + 1) + j]) > fabs(a[(size_t)p * (n + 1) + j])) p = i;
+    if (fabs(a[(size_t)p * (n + 1) + j]) < 1e-14 * scale) {
+      free(a);
+      return 0;
+    }
+    if (p != j)
+      for (uint32_t k = 0; k <= n; k++) {
+        const double t = a[(size_t)p * (n + 1) + k];
+        a[(size_t)p * (n + 1) + k] = a[(size_t)j * (n + 1) + k];
+        a[(size_t)j * (n + 1) + k] = t;
+      }
+    const double* rj = a + (size_t)j * (n + 1);
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = j + 1; ii < (int64_t)n; ii++) {
+      double* ri = a + (size_t)ii * (n + 1);
+      const double f = ri[j] / rj[j];
+      for (uint32_t k = j; k <= n; k++) ri[k] -= f * rj[k];
+    }
+  }
+  for (int64_t i = (int64_t)n - 1; i >= 0; i--) {
+    double v = a[(size_t)i * (n + 1) + n];
+    for (uint32_t j = (uint32_t)i + 1; j < n; j++) v -= a[(size_t)i * (n + 1) + j] * out[j];
+    out[i] = v / a[(size_t)i * (n + 1) + i];
+  }
+  free(a);
+  for (uint32_t i = 0; i < n; i++)
+    if (!isfinite(out[i])) return 0;
+  return 1;
+}
+
+/* bundleMetrics (bundle-shared.ts:32-54) of a scene at focal scale `fscale`: robust cost and sum of
+ * squares; 0 when an observation does not project. */
+static int dense_metrics(const tcv_ba_control* ctl, const double* scene, const uint32_t* index,
+                         double fscale, double* cost, double* squares) {
+  const uint32_t C = ctl->cameras, P = ctl->points, M = ctl->observations;
+  const double* pixels = scene + (size_t)C * 18 + (size_t)P * 3;
+  double c = 0, s = 0;
+  for (uint32_t i = 0; i < M; i++) {
+    const uint32_t cam = index[i], p = index[M + i];
+    const double* pose = scene + (size_t)cam * 12;
+    const double* X = scene + (size_t)C * 12 + (size_t)p * 3;
+    const double* mo = scene + (size_t)C * 12 + (size_t)P * 3 + (size_t)cam * 6;
+    Model k = {mo[0] * fscale, mo[1] * fscale, mo[2], mo[3], mo[4], mo[5]};
+    const int dir = p >= P - ctl->directional_points;
+    double q[3];
+    for (int r = 0; r < 3; r++)
+      q[r] = pose[r * 3] * X[0] + pose[r * 3 + 1] * X[1] + pose[r * 3 + 2] * X[2] + (dir ? 0 : pose[9 + r]);
+    Projection pr = project(&k, q);
+    if (!pr.ok) return 0;
+    const double e = hypot(pr.px - pixels[2 * (size_t)i], pr.py - pixels[2 * (size_t)i + 1]);
+    c += e <= ctl->huber ? e * e : 2 * ctl->huber * e - ctl->huber * ctl->huber;
+    s += e * e;
+  }
+  *cost = c;
+  *squares = s;
+  return 1;
+}
+
+/*
+ * The reference's dense CPU bundle adjustment, `bundleAdjust` (bundle-adjustment.ts:80-291, fixed
+ * intrinsics; joint-focal.ts:45-287 with one shared log focal scale): LM with point-block Schur
+ * elimination into a dense reduced system over the variable cameras' 6 rows (and the focal row),
+ * solved by solveSystem; damping lambda * max(diag, 1e-10), adjugate point inverses, the
+ * one-fixed-camera similarity normalisation, accept on a lower robust cost, stop on the stall rule
+ * (bundleStalled) or lambda > 1e12. `out` has the packBundleResult layout with words 6 (cost),
+ * 7 (sum of squares), 8/9 (initial), 10/11 (accepted/rejected), 24/25 (RMSE) and, on the shared
+ * route, 26 = log of the focal scale. Returns 0, or TCV_BA_ERR_MEMORY / TCV_BA_ERR_ARGUMENT.
+ */
+int tcv_oracle_dense_solve(const tcv_ba_control* ctl, const double* scene_in, const uint32_t* index,
+                           double* out) {
+  const uint32_t C = ctl->cameras, P = ctl->points, M = ctl->observations;
+  const int shared = ctl->shared_focal ? 1 : 0;
+  const uint32_t* point_start = index + 2 * (size_t)M;
+  const uint32_t* active = index + 3 * (size_t)M + P + C + 2;
+  int32_t* row_of = malloc(C * sizeof(int32_t));
+  uint32_t n_active = 0, fixed = 0;
+  for (uint32_t c = 0; c < C; c++) {
+    row_of[c] = active[c] ? (int32_t)(6 * n_active++) : -1;
+    if (!active[c]) fixed++;
+  }
+  const uint32_t focal_row = 6 * n_active, n = 6 * n_active + shared;
+  if ((uint64_t)n * n > 16ull * 1024 * 1024) {
+    free(row_of);
+    return TCV_BA_ERR_ARGUMENT;
+  }
+  const size_t scene_size = (size_t)C * 18 + (size_t)P * 3 + (size_t)M * 2, coords = (size_t)C * 12 + (size_t)P * 3;
+  double* model = malloc(scene_size * sizeof(double));
+  double* cand = malloc(scene_size * sizeof(double));
+  double* A = malloc((size_t)n * n * sizeof(double));
+  double* rhs = malloc(n * sizeof(double));
+  double* step = malloc(n * sizeof(double));
+  double* Vinv = malloc((size_t)P * 9 * sizeof(double));
+  double* bp = malloc((size_t)P * 3 * sizeof(double));
+  double* Wobs = malloc((size_t)M * 18 * sizeof(double)); /* camera rows x point, per observation */
+  double* Wf = malloc((size_t)P * 3 * sizeof(double));    /* focal row x point, per point */
+  if (!model || !cand || !A || !rhs || !step || !Vinv || !bp || !Wobs || !Wf) {
+    free(row_of); free(model); free(cand); free(A); free(rhs); free(step); free(Vinv); free(bp);
+    free(Wobs); free(Wf);
+    return TCV_BA_ERR_MEMORY;
+  }
+  memcpy(model, scene_in, scene_size * sizeof(double));
+  double origin[3], scale_center[3];
+  {
+    const double* a = model + (size_t)ctl->anchor * 12;
+    const double* s = model + (size_t)ctl->scale_view * 12;
+    for (int i = 0; i < 3; i++) {
+      origin[i] = -(a[i] * a[9] + a[3 + i] * a[10] + a[6 + i] * a[11]);
+      scale_center[i] = -(s[i] * s[9] + s[3 + i] * s[10] + s[6 + i] * s[11]);
+    }
+  }
+  const double baseline = hypot(hypot(origin[0] - scale_center[0], origin[1] - scale_center[1]),
+                                origin[2] - scale_center[2]);
+  double fscale = 1, log_focal = 0;
+  double cost0 = 0, sq0 = 0;
+  if (!dense_metrics(ctl, model, index, fscale, &cost0, &sq0)) {
+    free(row_of); free(model); free(cand); free(A); free(rhs); free(step); free(Vinv); free(bp);
+    free(Wobs); free(Wf);
+    return TCV_BA_ERR_ARGUMENT;
+  }
+  double cost = cost0, sq = sq0, lambda = 1e-3;
+  uint32_t accepted = 0, rejected = 0;
+  const double* pixels = model + (size_t)C * 18 + (size_t)P * 3;
+  for (uint32_t it = 0; it < ctl->iterations; it++) {
+    memset(A, 0, (size_t)n * n * sizeof(double));
+    memset(rhs, 0, n * sizeof(double));
+    int bad_point = 0;
+    for (uint32_t p = 0; p < P; p++) {
+      double V[9] = {0}, b[3] = {0}, wf[3] = {0};
+      const int dir = p >= P - ctl->directional_points;
+      const double* X = model + (size_t)C * 12 + (size_t)p * 3;
+      for (uint32_t i = point_start[p]; i < point_start[p + 1]; i++) {
+        const uint32_t c = index[i];
+        const double* pose = model + (size_t)c * 12;
+        const double* mo = model + (size_t)C * 12 + (size_t)P * 3 + (size_t)c * 6;
+        Model k = {mo[0] * fscale, mo[1] * fscale, mo[2], mo[3], mo[4], mo[5]};
+        double q[3];
+        for (int r = 0; r < 3; r++)
+          q[r] = pose[r * 3] * X[0] + pose[r * 3 + 1] * X[1] + pose[r * 3 + 2] * X[2] + (dir ? 0 : pose[9 + r]);
+        Projection pr = project(&k, q);
+        double jp[6], jc[12];
+        for (int a = 0; a < 3; a++) {
+          jp[a] = pr.du[0] * pose[a] + pr.du[1] * pose[3 + a] + pr.du[2] * pose[6 + a];
+          jp[3 + a] = pr.dv[0] * pose[a] + pr.dv[1] * pose[3 + a] + pr.dv[2] * pose[6 + a];
+        }
+        pose_row(q, pr.du, jc);
+        pose_row(q, pr.dv, jc + 6);
+        if (dir)
+          for (int a = 3; a < 6; a++) jc[a] = jc[6 + a] = 0;
+        const double r[2] = {pr.px - pixels[2 * (size_t)i], pr.py - pixels[2 * (size_t)i + 1]};
+        const double e = hypot(r[0], r[1]), w = e <= ctl->huber ? 1 : ctl->huber / e;
+        const double jf[2] = {pr.px - k.cx, pr.py - k.cy};
+        const int base = row_of[c];
+        double* W = Wobs + (size_t)i * 18;
+        for (int a = 0; a < 18; a++) W[a] = 0;
+        for (int axis = 0; axis < 2; axis++) {
+          for (int a = 0; a < 3; a++) {
+            b[a] -= w * jp[axis * 3 + a] * r[axis];
+            for (int l = 0; l < 3; l++) V[a * 3 + l] += w * jp[axis * 3 + a] * jp[axis * 3 + l];
+          }
+          /* the observation's rows: its camera's 6 (when variable) and the focal row */
+          int rows[7], nr = 0;
+          double J[7];
+          if (base >= 0)
+            for (int a = 0; a < 6; a++) {
+              rows[nr] = base + a;
+              J[nr++] = jc[axis * 6 + a];
+            }
+          if (shared) {
+            rows[nr] = (int)focal_row;
+            J[nr++] = jf[axis];
+          }
+          for (int a = 0; a < nr; a++) {
+            rhs[rows[a]] -= w * J[a] * r[axis];
+            for (int l = 0; l < nr; l++) A[(size_t)rows[a] * n + rows[l]] += w * J[a] * J[l];
+            for (int l = 0; l < 3; l++) {
+              const double v = w * J[a] * jp[axis * 3 + l];
+              if (rows[a] == (int)focal_row && shared) wf[l] += v;
+              else W[(rows[a] - base) * 3 + l] += v;
+            }
+          }
+        }
+      }
+      for (int a = 0; a < 3; a++) V[a * 3 + a] += lambda * fmax(V[a * 3 + a], 1e-10);
+      if (!adjugate_inverse3(V, Vinv + (size_t)p * 9)) bad_point = 1;
+      for (int a = 0; a < 3; a++) {
+        bp[(size_t)p * 3 + a] = b[a];
+        Wf[(size_t)p * 3 + a] = wf[a];
+      }
+    }
+    if (bad_point) {
+      lambda *= 10;
+      rejected++;
+      continue;
+    }
+    for (uint32_t a = 0; a < n; a++) A[(size_t)a * n + a] += lambda * fmax(A[(size_t)a * n + a], 1e-10);
+    /* S = A - W V^-1 W^T, reduced rhs = b - W V^-1 b_p, point by point */
+    for (uint32_t p = 0; p < P; p++) {
+      const double* inv = Vinv + (size_t)p * 9;
+      const uint32_t i0 = point_start[p], i1 = point_start[p + 1];
+      /* the point's rows and their cross vectors */
+      const uint32_t nrows = 6 * (i1 - i0) + (uint32_t)shared;
+      int* rows = malloc(nrows * sizeof(int));
+      double* Wr = malloc((size_t)nrows * 3 * sizeof(double));
+      double* Tr = malloc((size_t)nrows * 3 * sizeof(double));
+      uint32_t cnt = 0;
+      for (uint32_t i = i0; i < i1; i++) {
+        const int base = row_of[index[i]];
+        if (base < 0) continue;
+        for (int a = 0; a < 6; a++) {
+          rows[cnt] = base + a;
+          for (int l = 0; l < 3; l++) Wr[cnt * 3 + l] = Wobs[(size_t)i * 18 + a * 3 + l];
+          cnt++;
+        }
+      }
+      if (shared) {
+        rows[cnt] = (int)focal_row;
+        for (int l = 0; l < 3; l++) Wr[cnt * 3 + l] = Wf[(size_t)p * 3 + l];
+        cnt++;
+      }
+      for (uint32_t a = 0; a < cnt; a++) {
+        for (int l = 0; l < 3; l++) {
+          double t = 0;
+          for (int k = 0; k < 3; k++) t += Wr[a * 3 + k] * inv[k * 3 + l];
+          Tr[a * 3 + l] = t;
+        }
+        for (int l = 0; l < 3; l++) rhs[rows[a]] -= Tr[a * 3 + l] * bp[(size_t)p * 3 + l];
+      }
+      for (uint32_t a = 0; a < cnt; a++)
+        for (uint32_t l = 0; l < cnt; l++) {
+          double v = 0;
+          for (int k = 0; k < 3; k++) v += Tr[a * 3 + k] * Wr[l * 3 + k];
+          A[(size_t)rows[a] * n + rows[l]] -= v;
+        }
+      free(rows);
+      free(Wr);
+      free(Tr);
+    }
+    if (!solve_system(A, rhs, n, step)) {
+      lambda *= 10;
+      rejected++;
+      continue;
+    }
+    const double new_scale = shared ? exp(step[focal_row]) : 1.0;
+    if (!(isfinite(new_scale) && new_scale > 0)) {
+      lambda *= 10;
+      rejected++;
+      continue;
+    }
+    memcpy(cand, model, scene_size * sizeof(double));
+    for (uint32_t c = 0; c < C; c++)
+      if (row_of[c] >= 0) move_pose(model + (size_t)c * 12, step + row_of[c], cand + (size_t)c * 12);
+    for (uint32_t p = 0; p < P; p++) {
+      double b[3] = {bp[(size_t)p * 3], bp[(size_t)p * 3 + 1], bp[(size_t)p * 3 + 2]};
+      for (uint32_t i = point_start[p]; i < point_start[p + 1]; i++) {
+        const int base = row_of[index[i]];
+        if (base < 0) continue;
+        for (int j = 0; j < 3; j++)
+          for (int a = 0; a < 6; a++) b[j] -= Wobs[(size_t)i * 18 + a * 3 + j] * step[base + a];
+      }
+      if (shared)
+        for (int j = 0; j < 3; j++) b[j] -= Wf[(size_t)p * 3 + j] * step[focal_row];
+      const double* inv = Vinv + (size_t)p * 9;
+      for (int j = 0; j < 3; j++)
+        cand[(size_t)C * 12 + (size_t)p * 3 + j] +=
+            inv[j * 3] * b[0] + inv[j * 3 + 1] * b[1] + inv[j * 3 + 2] * b[2];
+    }
+    const double* sv = cand + (size_t)ctl->scale_view * 12;
+    double cs[3];
+    for (int i = 0; i < 3; i++) cs[i] = -(sv[i] * sv[9] + sv[3 + i] * sv[10] + sv[6 + i] * sv[11]);
+    const double nb = hypot(hypot(origin[0] - cs[0], origin[1] - cs[1]), origin[2] - cs[2]);
+    if (!(nb > 1e-12)) {
+      lambda *= 10;
+      rejected++;
+      continue;
+    }
+    const double s = fixed == 1 ? baseline / nb : 1.0;
+    for (uint32_t c = 0; c < C; c++) {
+      if (c == ctl->anchor) continue;
+      double* pc = cand + (size_t)c * 12;
+      for (int i = 0; i < 3; i++) {
+        const double ro = pc[i * 3] * origin[0] + pc[i * 3 + 1] * origin[1] + pc[i * 3 + 2] * origin[2];
+        pc[9 + i] = s * pc[9 + i] + (s - 1) * ro;
+      }
+    }
+    for (uint32_t p = 0; p < P; p++) {
+      double* X = cand + (size_t)C * 12 + (size_t)p * 3;
+      if (p >= P - ctl->directional_points) {
+        const double nx = hypot(hypot(X[0], X[1]), X[2]);
+        for (int j = 0; j < 3; j++) X[j] /= nx;
+      } else {
+        for (int j = 0; j < 3; j++) X[j] = origin[j] + s * (X[j] - origin[j]);
+      }
+    }
+    double next_cost, next_sq;
+    const int ok = dense_metrics(ctl, cand, index, fscale * new_scale, &next_cost, &next_sq);
+    if (ok && next_cost < cost) {
+      const double improvement = cost - next_cost;
+      memcpy(model, cand, coords * sizeof(double));
+      fscale *= new_scale;
+      if (shared) log_focal += step[focal_row];
+      cost = next_cost;
+      sq = next_sq;
+      lambda = fmax(1e-10, lambda / 3);
+      accepted++;
+      if (improvement < ctl->stall_tolerance * fmax(1, cost)) break;
+    } else {
+      lambda *= 10;
+      rejected++;
+      if (lambda > 1e12) break;
+    }
+  }
+  memcpy(out, model, coords * sizeof(double));
+  for (int i = 0; i < 26; i++) out[coords + i] = 0;
+  out[coords + 2] = baseline;
+  out[coords + 6] = cost;
+  out[coords + 7] = sq;
+  out[coords + 8] = cost0;
+  out[coords + 9] = sq0;
+  out[coords + 10] = accepted;
+  out[coords + 11] = rejected;
+  out[coords + 24] = sqrt(sq0 / M);
+  out[coords + 25] = sqrt(sq / M);
+  if (shared) out[coords + 26] = log_focal;
+  free(row_of); free(model); free(cand); free(A); free(rhs); free(step); free(Vinv); free(bp);
+  free(Wobs); free(Wf);
+  return 0;
+}

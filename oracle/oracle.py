@@ -55,6 +55,7 @@ def library() -> ctypes.CDLL:
     lib.tcv_oracle_focal_scale.argtypes = [ctypes.c_double]
     lib.tcv_oracle_focal_scale.restype = ctypes.c_double
     lib.tcv_oracle_gradient.argtypes = [P(_native.Control), vp, vp, ctypes.c_double, vp, vp]
+    lib.tcv_oracle_dense_solve.argtypes = [P(_native.Control), vp, vp, vp]
     _lib = lib
     return lib
 
@@ -193,3 +194,15 @@ def gradient(control: _native.Control, scene: np.ndarray, index: np.ndarray,
     if r < 0:
         raise RuntimeError(f"oracle gradient failed ({r})")
     return g, float(cost[0]), bool(r)
+
+
+def dense_solve(control: _native.Control, packed: PackedScene) -> np.ndarray:
+    """The reference's dense CPU `bundleAdjust` (packBundleResult layout, see the C header)."""
+    size = control.cameras * 12 + control.points * 3 + 26 + (1 if control.shared_focal else 0)
+    out = np.empty(size)
+    scene = np.ascontiguousarray(packed.scene, dtype=np.float64)
+    index = np.ascontiguousarray(packed.index, dtype=np.uint32)
+    r = library().tcv_oracle_dense_solve(ctypes.byref(control), _ptr(scene), _ptr(index), _ptr(out))
+    if r != 0:
+        raise RuntimeError(f"oracle dense solve failed ({r})")
+    return out

@@ -115,3 +115,17 @@ def test_solve_is_deterministic(device):
     a, _ = B.solve_packed(device, control, packed)
     b, _ = B.solve_packed(device, control, packed)
     assert np.array_equal(a, b)
+
+
+def test_cfg1_ring_matches_the_dense_cpu_solver(device):
+    """The GPU engine on the 10-camera ring scene (2,000 points, ~20k observations, 20 LM
+    iterations, PCG <= 50) against the reference's dense CPU solver restated in the oracle
+    (bundle-adjustment.ts:80-291): final cost within 1e-7 relative (focal.test.mjs:38)."""
+    print("\n[parity] cfg1 ring vs dense CPU solver", flush=True)
+    ring, _ = S.ring_scene(seed=1)
+    control = B.make_control(ring.C, ring.P, ring.M, iterations=20, max_cg=50)
+    got, _ = B.solve_packed(device, control, ring)
+    dense = O.dense_solve(control, ring)
+    base = ring.C * 12 + ring.P * 3
+    assert abs(got[base + 6] - dense[base + 6]) <= 1e-7 * dense[base + 6]
+    assert np.max(np.abs(got[:base] - dense[:base]) / np.maximum(1.0, np.abs(dense[:base]))) <= 1e-7

@@ -5,6 +5,7 @@ tests/test_gpu_operator.py."""
 import math
 
 import numpy as np
+import pytest
 
 from oracle import oracle as O
 from typecv_amd import bundle as B
@@ -112,3 +113,35 @@ def test_stall_tolerance_ends_the_loop_early():
     _, a, base = run(S.bundle_fixture(), {})
     _, b, _ = run(S.bundle_fixture(), {"stallTolerance": 0.5})
     assert b[base + 10] < a[base + 10]
+
+
+def _dense_cases():
+    from problems import packed_problem
+
+    yield "bundle_fixture", packed_problem(S.bundle_fixture(), {"iterations": 20})
+    yield "bundle_fixture_outlier", packed_problem(S.bundle_fixture(outlier=True),
+                                                   {"iterations": 20, "huberPx": 1})
+    yield "joint_focal", packed_problem(
+        S.joint_focal_fixture()["input"],
+        {"iterations": 40, "refineSharedFocal": True, "maxCgIterations": 256,
+         "cgRelativeTolerance": 1e-10})
+    ring, _ = S.ring_scene(seed=1)
+    yield "ring_cfg1", (ring, B.make_control(ring.C, ring.P, ring.M, iterations=20, max_cg=50))
+
+
+DENSE = list(_dense_cases())
+
+
+@pytest.mark.parametrize("name,case", DENSE, ids=[c[0] for c in DENSE])
+def test_dense_reduced_camera_solver_agrees_with_matrix_free(name, case):
+    """The reference's dense CPU `bundleAdjust` (bundle-adjustment.ts / joint-focal.ts, restated as
+    tcv_oracle_dense_solve: dense reduced system and pivoted elimination, adjugate point inverses)
+    against the matrix-free Schur/PCG route: final cost within 1e-7 (focal.test.mjs:38) and the same
+    coordinates."""
+    packed, control = case
+    dense = O.dense_solve(control, packed)
+    free, _ = O.solve(control, packed)
+    base = packed.C * 12 + packed.P * 3
+    assert abs(dense[base + 6] - free[base + 6]) <= 1e-7 * max(1.0, free[base + 6])
+    assert np.max(np.abs(dense[:base] - free[:base]) / np.maximum(1.0, np.abs(free[:base]))) <= 1e-9
+    assert dense[base + 6] < dense[base + 8]
