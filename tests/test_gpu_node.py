@@ -1,0 +1,53 @@
+"""The Node-API addon end to end on the GPU: a packed solve through `typecv_amd/node/run.js`
+(Node, napi_async_work, tcv_ba_solve) returns exactly the array the Python binding returns for the
+same inputs, and its diagnostics."""
+import json
+import shutil
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from typecv_amd import bundle as B
+from typecv_amd import synthetic as S
+from problems import packed_problem
+
+pytestmark = pytest.mark.gpu
+
+NODE_DIR = Path(__file__).resolve().parent.parent / "typecv_amd" / "node"
+
+
+def control_vector(c) -> np.ndarray:
+    return np.array([c.cameras, c.points, c.directional_points, c.observations, c.anchor,
+                     c.scale_view, c.normalize_scale, c.shared_focal, c.iterations, c.max_cg,
+                     c.precision, c.flags, c.huber, c.relative_tolerance, c.stall_tolerance,
+                     c.refinement_tolerance], dtype=np.float64)
+
+
+@pytest.mark.parametrize("shared", [False, True])
+def test_node_solve_matches_python(device, tmp_path, shared):
+    node = shutil.which("node")
+    if node is None or not (NODE_DIR / "typecv_ba.node").exists():
+        pytest.skip("no Node or no built addon")
+    print(f"\n[node] shared={shared}", flush=True)
+    if shared:
+        model = S.joint_focal_fixture()["input"]
+        options = {"iterations": 40, "refineSharedFocal": True, "maxCgIterations": 256,
+                   "cgRelativeTolerance": 1e-10}
+    else:
+        model, options = S.bundle_fixture(outlier=True), {"iterations": 20, "huberPx": 1}
+    packed, control = packed_problem(model, options)
+    control_vector(control).tofile(tmp_path / "control.f64")
+    packed.scene.astype(np.float64).tofile(tmp_path / "scene.f64")
+    packed.index.astype(np.uint32).tofile(tmp_path / "index.u32")
+    run = subprocess.run([node, str(NODE_DIR / "run.js"), str(tmp_path / "control.f64"),
+                          str(tmp_path / "scene.f64"), str(tmp_path / "index.u32"),
+                          str(tmp_path / "out.f64")], capture_output=True, text=True, timeout=120)
+    assert run.returncode == 0, run.stderr
+    got = np.fromfile(tmp_path / "out.f64", dtype=np.float64)
+    expected, _ = B.solve_packed(device, control, packed)
+    assert np.array_equal(got, expected)
+    diag = json.loads(run.stdout.strip().splitlines()[-1])["diagnostics"]
+    base = packed.C * 12 + packed.P * 3
+    assert diag["finalCost"] == expected[base + 6] and diag["observations"] == packed.M

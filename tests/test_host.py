@@ -121,3 +121,22 @@ def test_c_abi_exports_every_declared_symbol():
     assert lib.tcv_ba_scene_size(ctypes.byref(control)) == 180 + 300 + 2000
     assert lib.tcv_ba_index_size(ctypes.byref(control)) == 3000 + 100 + 22
     assert lib.tcv_ba_status_string(2).decode().startswith("GPU BA index")
+
+
+def test_node_addon_loads_and_exports_the_operator():
+    """The Node-API addon over the C ABI loads in the installed Node (no device touched) and
+    exports the operator's entry points."""
+    import json
+    import shutil
+    import subprocess
+    from pathlib import Path
+
+    node = shutil.which("node")
+    addon = Path(__file__).resolve().parent.parent / "typecv_amd" / "node" / "typecv_ba.node"
+    if node is None or not addon.exists():
+        pytest.skip("no Node or no built addon (build() builds it where Node-API headers exist)")
+    out = subprocess.run([node, str(addon.parent / "run.js"), "--exports"], capture_output=True,
+                         text=True, timeout=60)
+    assert out.returncode == 0, out.stderr
+    assert json.loads(out.stdout) == ["backward", "deviceBytes", "open", "outputSize", "release",
+                                      "solve"]

@@ -1,0 +1,89 @@
+'use strict';
+/*
+ * The TypeScript operator's side of the engine, in JavaScript for the installed Node: a device
+ * with the reference's per-device FIFO of whole jobs (src/device/gpu/device.ts:592-598), the
+ * control vector of tcv_ba_control, and the packed solve/backward as Promises, as
+ * bundleAdjustGpu's enqueue body (src/sfm/bundle/gpu/bundle.ts:178-799) would call them.
+ */
+const path = require('path');
+const addon = require(path.join(__dirname, 'typecv_ba.node'));
+
+// BUNDLE_STATE, src/sfm/bundle/gpu/bundle-shared.ts:38-54 (the result words after the coordinates)
+const BUNDLE_STATE = Object.freeze({
+  lambda: 0, active: 1, baseline: 2, cost: 6, sse: 7, initialCost: 8, initialSse: 9,
+  accepted: 10, rejected: 11, inputError: 14, cgSteps: 16, failedSolves: 17,
+  initialRmsePx: 24, finalRmsePx: 25, focalScale: 26,
+});
+
+class HipDevice {
+  constructor(index = 0) {
+    this.native = addon.open(index);
+    this.tail = Promise.resolve();
+    this.closed = false;
+  }
+  /* Jobs run one after another on this device, in the order they were enqueued. */
+  enqueue(job) {
+    if (this.closed) return Promise.reject(new Error('device is closed'));
+    const run = this.tail.then(job, job);
+    this.tail = run.catch(() => undefined);
+    return run;
+  }
+  get statistics() {
+    return { liveBytes: addon.deviceBytes(this.native) };
+  }
+  drop() {
+    addon.release(this.native);
+  }
+  dispose() {
+    this.closed = true;
+    addon.release(this.native);
+  }
+}
+
+/* tcv_ba_control as the addon's Float64Array(16). */
+function makeControl({ cameras, points, observations, directionalPoints = 0, anchor = 0,
+  scaleView = 1, normalizeScale = true, sharedFocal = false, iterations = 20, maxCg = 24,
+  precision = 'fp64', flags = 0, huber = 2, tolerance = 1e-6, stallTolerance = 1e-9,
+  refinementTolerance = 0 }) {
+  return Float64Array.from([cameras, points, directionalPoints, observations, anchor, scaleView,
+    normalizeScale ? 1 : 0, sharedFocal ? 1 : 0, iterations, maxCg,
+    precision === 'mixed' ? 1 : 0, flags, huber, tolerance, stallTolerance,
+    refinementTolerance]);
+}
+
+/* One packed solve: the packBundleResult array and its diagnostics, with checkBundleOutput's
+ * errors (bundle-host.ts:278-283). */
+function solvePacked(device, control, scene, index) {
+  return device.enqueue(async () => {
+    const values = await addon.solve(device.native, control, scene, index);
+    const base = control[0] * 12 + control[1] * 3;
+    if (values[base + BUNDLE_STATE.inputError] === 1)
+      throw new RangeError('GPU BA gauge requires a nonzero baseline');
+    if (values[base + BUNDLE_STATE.inputError] === 2)
+      throw new RangeError('GPU BA observations must initially project in front of their cameras');
+    return {
+      values,
+      diagnostics: {
+        initialCost: values[base + BUNDLE_STATE.initialCost],
+        finalCost: values[base + BUNDLE_STATE.cost],
+        initialRmsePx: values[base + BUNDLE_STATE.initialRmsePx],
+        finalRmsePx: values[base + BUNDLE_STATE.finalRmsePx],
+        acceptedSteps: values[base + BUNDLE_STATE.accepted],
+        rejectedSteps: values[base + BUNDLE_STATE.rejected],
+        observations: control[3],
+        baseline: values[base + BUNDLE_STATE.baseline],
+      },
+      cgIterations: values[base + BUNDLE_STATE.cgSteps],
+      linearSolverFailures: values[base + BUNDLE_STATE.failedSolves],
+    };
+  });
+}
+
+/* The backward entry: pixel cotangents (2M) of a cotangent on the solve's outputs. */
+function backwardPacked(device, control, scene, index, logFocal, cotangent,
+  maxCg = 4000, tolerance = 1e-10) {
+  return device.enqueue(() => addon.backward(device.native, control, scene, index, logFocal,
+    cotangent, maxCg, tolerance));
+}
+
+module.exports = { HipDevice, makeControl, solvePacked, backwardPacked, BUNDLE_STATE, addon };

@@ -1,0 +1,39 @@
+'use strict';
+/*
+ * Command-line runner of the packed solve through the Node-API addon:
+ *   node run.js <control.f64> <scene.f64> <index.u32> <out.f64>
+ * reads the raw little-endian arrays, solves on device 0 and writes the packBundleResult array.
+ * `node run.js --exports` prints the addon's exported names (no device is touched).
+ */
+const fs = require('fs');
+const { HipDevice, solvePacked, addon } = require('./bundle.js');
+
+function read(file, Type) {
+  const bytes = fs.readFileSync(file);
+  return new Type(bytes.buffer.slice(bytes.byteOffset, bytes.byteOffset + bytes.byteLength));
+}
+
+async function main(argv) {
+  if (argv[0] === '--exports') {
+    console.log(JSON.stringify(Object.keys(addon).sort()));
+    return 0;
+  }
+  const [controlFile, sceneFile, indexFile, outFile] = argv;
+  const control = read(controlFile, Float64Array);
+  const scene = read(sceneFile, Float64Array);
+  const index = read(indexFile, Uint32Array);
+  const device = new HipDevice(0);
+  try {
+    const result = await solvePacked(device, control, scene, index);
+    fs.writeFileSync(outFile, Buffer.from(result.values.buffer));
+    console.log(JSON.stringify({ diagnostics: result.diagnostics, cgIterations: result.cgIterations }));
+  } finally {
+    device.dispose();
+  }
+  return 0;
+}
+
+main(process.argv.slice(2)).then((code) => process.exit(code), (error) => {
+  console.error(String(error && error.stack || error));
+  process.exit(1);
+});

@@ -1,0 +1,309 @@
+/*
+ * typecv_ba_napi.c — the Node-API addon over the engine's C ABI (include/typecv_ba.h).
+ *
+ * The reference operator `bundleAdjustGpu` (src/sfm/bundle/gpu/bundle.ts:803-839) is a Promise
+ * that enqueues one whole job per device (src/device/gpu/device.ts:592-598). This addon gives its
+ * TypeScript host the same shape: every solve runs on a libuv worker thread (napi_async_work) and
+ * resolves a Promise with the packBundleResult array; errors reject it with the engine's status
+ * string and detail. The typed arrays passed in stay referenced until the job completes.
+ *
+ * Exports:
+ *   open(device)                                   -> context (External; destroyed when collected)
+ *   outputSize(control)                            -> number of doubles of solve's result
+ *   deviceBytes(context), release(context)         -> tcv_ba_device_bytes, tcv_ba_release
+ *   solve(context, control, scene, index)          -> Promise<Float64Array>
+ *   backward(context, control, scene, index, logFocal, cotangent, maxCg, tolerance)
+ *                                                  -> Promise<Float64Array(2M)>
+ * `control` is a Float64Array(16): the tcv_ba_control fields in declaration order (12 integers,
+ * then huber, relative_tolerance, stall_tolerance, refinement_tolerance); `scene` a Float64Array
+ * and `index` a Uint32Array in the packBundleScene layouts (bundle-host.ts:121-218).
+ */
+#include <node_api.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "typecv_ba.h"
+
+#define CHECK(env, call)                                             \
+  do {                                                               \
+    if ((call) != napi_ok) {                                         \
+      napi_throw_error((env), NULL, "typecv_ba addon: " #call);      \
+      return NULL;                                                   \
+    }                                                                \
+  } while (0)
+
+static void finalize_context(napi_env env, void* data, void* hint) {
+  (void)env;
+  (void)hint;
+  tcv_ba_destroy((tcv_ba_context*)data);
+}
+
+static tcv_ba_context* context_of(napi_env env, napi_value v) {
+  void* p = NULL;
+  if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
+    napi_throw_type_error(env, NULL, "expected a context from open()");
+    return NULL;
+  }
+  return (tcv_ba_context*)p;
+}
+
+/* A typed array's data and length, checked against the expected element type. */
+static void* typed_data(napi_env env, napi_value v, napi_typedarray_type want, size_t* length) {
+  bool is = false;
+  napi_is_typedarray(env, v, &is);
+  if (!is) {
+    napi_throw_type_error(env, NULL, "expected a typed array");
+    return NULL;
+  }
+  napi_typedarray_type type;
+  void* data = NULL;
+  napi_value buffer;
+  size_t offset = 0;
+  napi_get_typedarray_info(env, v, &type, length, &data, &buffer, &offset);
+  if (type != want) {
+    napi_throw_type_error(env, NULL,
+                          want == napi_float64_array ? "expected a Float64Array" : "expected a Uint32Array");
+    return NULL;
+  }
+  return data;
+}
+
+static int control_of(napi_env env, napi_value v, tcv_ba_control* c) {
+  size_t n = 0;
+  const double* f = typed_data(env, v, napi_float64_array, &n);
+  if (!f) return 0;
+  if (n != 16) {
+    napi_throw_range_error(env, NULL, "control must be a Float64Array(16)");
+    return 0;
+  }
+  uint32_t* u = &c->cameras;
+  for (int k = 0; k < 12; k++) {
+    if (!(f[k] >= 0 && f[k] <= 4294967295.0) || f[k] != (double)(uint32_t)f[k]) {
+      napi_throw_range_error(env, NULL, "control integer field out of range");
+      return 0;
+    }
+    u[k] = (uint32_t)f[k];
+  }
+  c->huber = f[12];
+  c->relative_tolerance = f[13];
+  c->stall_tolerance = f[14];
+  c->refinement_tolerance = f[15];
+  return 1;
+}
+
+static napi_value open_device(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  int32_t device = 0;
+  if (argc > 0) CHECK(env, napi_get_value_int32(env, argv[0], &device));
+  tcv_ba_context* ctx = NULL;
+  const int status = tcv_ba_create(device, &ctx);
+  if (status != TCV_BA_OK) {
+    char text[512];
+    snprintf(text, sizeof text, "%s (%s)", tcv_ba_status_string(status), tcv_ba_last_error());
+    napi_throw_error(env, NULL, text);
+    return NULL;
+  }
+  napi_value out;
+  CHECK(env, napi_create_external(env, ctx, finalize_context, NULL, &out));
+  return out;
+}
+
+static napi_value output_size(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  tcv_ba_control c;
+  if (argc < 1 || !control_of(env, argv[0], &c)) return NULL;
+  napi_value out;
+  CHECK(env, napi_create_double(env, (double)tcv_ba_output_size(&c), &out));
+  return out;
+}
+
+static napi_value device_bytes(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  tcv_ba_context* ctx = argc > 0 ? context_of(env, argv[0]) : NULL;
+  if (!ctx) return NULL;
+  napi_value out;
+  CHECK(env, napi_create_double(env, (double)tcv_ba_device_bytes(ctx), &out));
+  return out;
+}
+
+static napi_value release(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  tcv_ba_context* ctx = argc > 0 ? context_of(env, argv[0]) : NULL;
+  if (!ctx) return NULL;
+  tcv_ba_release(ctx);
+  return NULL;
+}
+
+/* ---------------------------------------------------------------- asynchronous jobs */
+
+enum { JOB_SOLVE = 0, JOB_BACKWARD = 1 };
+
+typedef struct {
+  int kind;
+  napi_async_work work;
+  napi_deferred deferred;
+  tcv_ba_context* ctx;
+  tcv_ba_control control;
+  napi_ref refs[4]; /* context, scene, index, cotangent */
+  int nrefs;
+  const double* scene;
+  const uint32_t* index;
+  const double* cotangent;
+  double log_focal, tolerance;
+  uint32_t max_cg;
+  double* out;
+  size_t out_len;
+  tcv_ba_report report;
+  int status;
+  char detail[256];
+} Job;
+
+static void execute(napi_env env, void* data) {
+  (void)env;
+  Job* j = data;
+  if (j->kind == JOB_SOLVE)
+    j->status = tcv_ba_solve(j->ctx, &j->control, j->scene, j->index, j->out, &j->report);
+  else
+    j->status = tcv_ba_backward(j->ctx, &j->control, j->scene, j->index, j->log_focal,
+                                j->cotangent, j->max_cg, j->tolerance, j->out, NULL, &j->report);
+  if (j->status != TCV_BA_OK) snprintf(j->detail, sizeof j->detail, "%s", tcv_ba_last_error());
+}
+
+static void complete(napi_env env, napi_status s, void* data) {
+  Job* j = data;
+  if (s == napi_ok && j->status == TCV_BA_OK) {
+    napi_value buffer, array;
+    void* bytes = NULL;
+    napi_create_arraybuffer(env, j->out_len * sizeof(double), &bytes, &buffer);
+    memcpy(bytes, j->out, j->out_len * sizeof(double));
+    napi_create_typedarray(env, napi_float64_array, j->out_len, buffer, 0, &array);
+    napi_resolve_deferred(env, j->deferred, array);
+  } else {
+    char text[512];
+    snprintf(text, sizeof text, "%s (%s)",
+             s == napi_ok ? tcv_ba_status_string(j->status) : "job cancelled", j->detail);
+    napi_value msg, err;
+    napi_create_string_utf8(env, text, NAPI_AUTO_LENGTH, &msg);
+    if (j->status == TCV_BA_ERR_ARGUMENT)
+      napi_create_range_error(env, NULL, msg, &err);
+    else
+      napi_create_error(env, NULL, msg, &err);
+    napi_reject_deferred(env, j->deferred, err);
+  }
+  for (int k = 0; k < j->nrefs; k++) napi_delete_reference(env, j->refs[k]);
+  napi_delete_async_work(env, j->work);
+  free(j->out);
+  free(j);
+}
+
+static napi_value queue(napi_env env, Job* j, const char* name) {
+  napi_value promise, resource;
+  if (napi_create_promise(env, &j->deferred, &promise) != napi_ok ||
+      napi_create_string_utf8(env, name, NAPI_AUTO_LENGTH, &resource) != napi_ok ||
+      napi_create_async_work(env, NULL, resource, execute, complete, j, &j->work) != napi_ok ||
+      napi_queue_async_work(env, j->work) != napi_ok) {
+    for (int k = 0; k < j->nrefs; k++) napi_delete_reference(env, j->refs[k]);
+    free(j->out);
+    free(j);
+    napi_throw_error(env, NULL, "typecv_ba addon: could not queue the job");
+    return NULL;
+  }
+  return promise;
+}
+
+/* Common arguments: context, control, scene, index, with the sizes the control implies. */
+static Job* job_of(napi_env env, napi_value* argv, size_t argc, size_t need, int kind) {
+  if (argc < need) {
+    napi_throw_type_error(env, NULL, "missing arguments");
+    return NULL;
+  }
+  Job* j = calloc(1, sizeof(Job));
+  if (!j) {
+    napi_throw_error(env, NULL, "out of host memory");
+    return NULL;
+  }
+  j->kind = kind;
+  j->ctx = context_of(env, argv[0]);
+  size_t ns = 0, ni = 0;
+  if (!j->ctx || !control_of(env, argv[1], &j->control) ||
+      !(j->scene = typed_data(env, argv[2], napi_float64_array, &ns)) ||
+      !(j->index = typed_data(env, argv[3], napi_uint32_array, &ni))) {
+    free(j);
+    return NULL;
+  }
+  if (ns != tcv_ba_scene_size(&j->control) || ni != tcv_ba_index_size(&j->control)) {
+    napi_throw_range_error(env, NULL, "scene/index lengths do not match the control sizes");
+    free(j);
+    return NULL;
+  }
+  napi_create_reference(env, argv[0], 1, &j->refs[j->nrefs++]);
+  napi_create_reference(env, argv[2], 1, &j->refs[j->nrefs++]);
+  napi_create_reference(env, argv[3], 1, &j->refs[j->nrefs++]);
+  return j;
+}
+
+static napi_value solve(napi_env env, napi_callback_info info) {
+  size_t argc = 4;
+  napi_value argv[4];
+  CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  Job* j = job_of(env, argv, argc, 4, JOB_SOLVE);
+  if (!j) return NULL;
+  j->out_len = tcv_ba_output_size(&j->control);
+  j->out = malloc(j->out_len * sizeof(double));
+  return queue(env, j, "typecv_ba.solve");
+}
+
+static napi_value backward(napi_env env, napi_callback_info info) {
+  size_t argc = 8;
+  napi_value argv[8];
+  CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  Job* j = job_of(env, argv, argc, 8, JOB_BACKWARD);
+  if (!j) return NULL;
+  size_t nc = 0;
+  double max_cg = 0;
+  if (napi_get_value_double(env, argv[4], &j->log_focal) != napi_ok ||
+      !(j->cotangent = typed_data(env, argv[5], napi_float64_array, &nc)) ||
+      napi_get_value_double(env, argv[6], &max_cg) != napi_ok ||
+      napi_get_value_double(env, argv[7], &j->tolerance) != napi_ok) {
+    for (int k = 0; k < j->nrefs; k++) napi_delete_reference(env, j->refs[k]);
+    free(j);
+    return NULL;
+  }
+  const size_t want = (size_t)j->control.cameras * 12 + (size_t)j->control.points * 3 +
+                      (j->control.shared_focal ? 1 : 0);
+  if (nc != want || !(max_cg >= 1 && max_cg <= 100000)) {
+    for (int k = 0; k < j->nrefs; k++) napi_delete_reference(env, j->refs[k]);
+    free(j);
+    napi_throw_range_error(env, NULL, "cotangent length or maxCg out of range");
+    return NULL;
+  }
+  j->max_cg = (uint32_t)max_cg;
+  napi_create_reference(env, argv[5], 1, &j->refs[j->nrefs++]);
+  j->out_len = 2 * (size_t)j->control.observations;
+  j->out = malloc(j->out_len * sizeof(double));
+  return queue(env, j, "typecv_ba.backward");
+}
+
+static napi_value init(napi_env env, napi_value exports) {
+  const napi_property_descriptor props[] = {
+      {"open", NULL, open_device, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"outputSize", NULL, output_size, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"deviceBytes", NULL, device_bytes, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"release", NULL, release, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"solve", NULL, solve, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"backward", NULL, backward, NULL, NULL, NULL, napi_enumerable, NULL},
+  };
+  napi_define_properties(env, exports, sizeof props / sizeof props[0], props);
+  return exports;
+}
+
+NAPI_MODULE(NODE_GYP_MODULE_NAME, init)
