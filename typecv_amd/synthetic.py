@@ -24,7 +24,8 @@ from .packing import PackedScene, pack_arrays
 
 
 def _xorshift(seed: int):
-    state = [seed & 0xFFFFFFFF]
+    """`seededRandom`, src/geometry/numerics.ts:133-141 (a zero seed takes 0x9e3779b9)."""
+    state = [(seed & 0xFFFFFFFF) or 0x9E3779B9]
 
     def next_value() -> float:
         s = state[0]
@@ -68,6 +69,55 @@ def pose_fixture(count=97, noise=0.3, outliers=0.25, planar=False, scale=1, radi
 
 def _pose(R, t):
     return {"R": np.array(R, dtype=np.float64), "t": np.array(t, dtype=np.float64)}
+
+
+def update_pose(pose, delta) -> dict:
+    """`updatePose`, src/geometry/pose.ts:53-57: R' = exp(w) R, t' = exp(w) t + v."""
+    dR = _rotation_exp(delta[:3]).reshape(3, 3)
+    R = np.asarray(pose["R"], dtype=np.float64).reshape(3, 3)
+    t = np.asarray(pose["t"], dtype=np.float64)
+    return _pose((dR @ R).reshape(9), dR @ t + np.asarray(delta[3:6], dtype=np.float64))
+
+
+def _rotation_exp(w) -> np.ndarray:
+    """`rotationExp`, src/geometry/pose.ts:40-51: the left SO(3) exponential, row-major."""
+    x, y, z = (float(v) for v in w)
+    theta = math.sqrt(x * x + y * y + z * z)
+    a = 1 - theta * theta / 6 if theta < 1e-7 else math.sin(theta) / theta
+    b = 0.5 - theta * theta / 24 if theta < 1e-7 else (1 - math.cos(theta)) / (theta * theta)
+    W = np.array([[0, -z, y], [z, 0, -x], [-y, x, 0]])
+    return (np.eye(3) + a * W + b * (W @ W)).reshape(9)
+
+
+def reference_scene(count=90, views=6, noise=0.0, seed=731, camera=None):
+    """`scene`, tests/scene.ts:10-61: `count` points in a box in front of `views` cameras stepping
+    along x (ids 10, 20, ...), every point observed by every view with uniform pixel noise of
+    half-width `noise`. Returns (reconstruction, rng) with the generator left where the
+    reference's is."""
+    camera = dict(camera or {"fx": 850, "fy": 870, "cx": 480, "cy": 270})
+    camera.setdefault("k1", 0.0)
+    camera.setdefault("k2", 0.0)
+    rng = _xorshift(seed)
+    points = [np.array([(rng() - 0.5) * 4, (rng() - 0.5) * 3, 4 + rng() * 5]) for _ in range(count)]
+    registered = []
+    for i in range(views):
+        R = _rotation_exp([0.008 * i, -0.018 * i, 0.003 * i])
+        C = np.array([i * 0.32, math.sin(i) * 0.04, 0.02 * i])
+        t = -(R.reshape(3, 3) @ C)
+        registered.append({"id": i * 10 + 10, "camera": dict(camera), "pose": _pose(R, t)})
+    keypoints = []
+    for v in registered:
+        kp = []
+        for X in points:
+            u, w = _project_pinhole_radial(camera, v["pose"]["R"], v["pose"]["t"], X)
+            kp.append((u + (rng() - 0.5) * noise * 2, w + (rng() - 0.5) * noise * 2))
+        keypoints.append(kp)
+    landmarks = [{"trackId": i, "xyz": X.copy(),
+                  "observations": [{"viewId": v["id"], "featureId": i,
+                                    "pixel": [keypoints[j][i][0], keypoints[j][i][1]]}
+                                   for j, v in enumerate(registered)]}
+                 for i, X in enumerate(points)]
+    return {"views": registered, "landmarks": landmarks}, rng
 
 
 def _project_pinhole_radial(camera, R, t, X):
