@@ -207,7 +207,8 @@ def batched(args) -> int:
         f"{sum(p.P for p in packeds)} points, {M} observations in {time.time() - t0:.1f}s")
     controls = [B.make_control(p.C, p.P, p.M, shared_focal=True, huber=1.0,
                                iterations=args.iterations, max_cg=min(256, max(32, 12 * p.C + 1)),
-                               precision=args.precision) for p in packeds]
+                               precision=args.precision, schur_cache=args.schur_cache)
+                for p in packeds]
     # the bundles' default budgets differ (12C+1); a batch shares one budget: the largest
     budget = max(c.max_cg for c in controls)
     for c in controls:
@@ -215,7 +216,7 @@ def batched(args) -> int:
     device = B.HipDevice(0)
     for k in range(args.warmup):
         B.solve_batch(device, controls, packeds)
-    loops, walls, latencies, lm = [], [], [], 0
+    loops, walls, latencies, lm, cg, direct = [], [], [], 0, 0, 0
     for k in range(args.steps):
         t = time.perf_counter()
         outs, report, finish = B.solve_batch(device, controls, packeds)
@@ -223,7 +224,11 @@ def batched(args) -> int:
         loops.append(float(finish.max()) * 1e-3)
         latencies.extend(finish.tolist())
         lm += report["lm_iterations"]
-        log(f"step {k}: wall {walls[-1] * 1e3:.1f} ms, LM loop {loops[-1] * 1e3:.1f} ms")
+        cg += report["cg_iterations"]
+        direct += report["direct_solves"]
+        log(f"step {k}: wall {walls[-1] * 1e3:.1f} ms, LM loop {loops[-1] * 1e3:.1f} ms, "
+            f"{report['lm_iterations']} LM, {report['cg_iterations']} PCG, "
+            f"{report['direct_solves']} direct")
     device.dispose()
     lat = np.array(latencies)
     result = {
@@ -237,11 +242,14 @@ def batched(args) -> int:
         "data": "synthetic (seeded local bundles)",
         "config": {"workload": "batched local bundles", "bundles": len(packeds),
                    "observations": M, "lm_iterations_max": args.iterations,
-                   "precision": args.precision, "max_cg": budget},
+                   "precision": args.precision, "max_cg": budget,
+                   "linear_solver": "direct (schurCache)" if args.schur_cache else "PCG"},
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                        "max": float(lat.max())},
         "end_to_end_bundles_per_s": len(packeds) * args.steps / sum(walls),
         "lm_iterations_per_s": lm / sum(loops),
+        "pcg_iterations_per_step": cg / max(1, args.steps),
+        "direct_solves_per_step": direct / max(1, args.steps),
         "observations_per_s": M * (lm / len(packeds)) / (sum(loops) / args.steps) / args.steps
         if args.steps else 0.0,
     }
@@ -480,5 +488,7 @@ if __name__ == "__main__":
         ap.add_argument("--iterations", type=int, default=30)
         ap.add_argument("--precision", choices=("fp64", "mixed"), default="mixed")
         ap.add_argument("--seed", type=int, default=4)
+        ap.add_argument("--schur-cache", action="store_true",
+                        help="the direct route: dense reduced camera system, Cholesky, certificate")
         sys.exit(batched(ap.parse_args()))
     sys.exit(main())

@@ -89,6 +89,12 @@ typedef struct tcv_ba_control {
 /* PCG (and the probe) use the implicit FP64 Schur operator: every cross block is relinearised from
  * the scene in each sweep instead of streamed from storage. Same results to rounding. */
 #define TCV_BA_FLAG_IMPLICIT 2u
+/* The direct route, the reference's opt-in `schurCache` (src/sfm/bundle/gpu/joint-focal.ts:102-108,
+ * joint-focal.shade.ts:1206-1383): each bundle of at most 128 cameras assembles its reduced camera
+ * system densely, factors it by Cholesky and solves it in place of the PCG loop; the matrix-free
+ * residual certifies the solution (PCG restarts from it when it misses the tolerance). Larger
+ * bundles, and batches in which a camera observes one point twice, keep the PCG route. */
+#define TCV_BA_FLAG_SCHUR_CACHE 4u
 
 /* What one solve did. Times are milliseconds of host wall clock unless named otherwise. */
 typedef struct tcv_ba_report {
@@ -106,6 +112,7 @@ typedef struct tcv_ba_report {
   uint64_t device_bytes;    /* device memory held by the context after the solve */
   uint64_t point_tiles;     /* wave tiles of whole points */
   uint64_t camera_tasks;    /* camera-list tasks */
+  uint64_t direct_solves;   /* certified direct solves (TCV_BA_FLAG_SCHUR_CACHE) */
 } tcv_ba_report;
 
 typedef struct tcv_ba_context tcv_ba_context;

@@ -1,0 +1,146 @@
+"""The direct route (TCV_BA_FLAG_SCHUR_CACHE, the reference's `schurCache` option): the reduced
+camera system of each bundle of at most 128 cameras is assembled densely, factored by Cholesky and
+certified by the matrix-free residual.
+
+Bars (stated here):
+* against the oracle's matrix-free solve with PCG tightened to 1e-12 (the same LM schedule, a
+  near-exact linear solve per iteration): final cost within 1e-9 relative, every pose/point
+  coordinate x within 1e-6·max(1, |x|), accepted/rejected counts equal;
+* against the reference's dense CPU solver restated in the oracle (bundle-adjustment.ts:80-291) on
+  the 10-camera ring: final cost within 1e-7 relative (focal.test.mjs:38);
+* the mixed form against the FP64 oracle: final cost within 1e-6 relative;
+* a batch equals its bundles solved alone, bit for bit, also when one bundle is too large for the
+  direct route and keeps PCG.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from typecv_amd import bundle as B
+from typecv_amd import synthetic as S
+
+from test_gpu_parity import CASES, packed_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def variant(control, **kw):
+    base = dict(directional=control.directional_points, anchor=control.anchor,
+                scale_view=control.scale_view, normalize_scale=bool(control.normalize_scale),
+                shared_focal=bool(control.shared_focal), iterations=control.iterations,
+                max_cg=control.max_cg, huber=control.huber,
+                tolerance=control.relative_tolerance)
+    base.update(kw)
+    return B.make_control(control.cameras, control.points, control.observations, **base)
+
+
+def problems():
+    for name, model, options in CASES:
+        if name == "ring_320":
+            continue
+        packed, control = packed_problem(model, options)
+        yield name, packed, control
+    ring, _ = S.ring_scene(seed=3, cameras=10, points=400)
+    yield "ring_scene", ring, B.make_control(ring.C, ring.P, ring.M, iterations=20, max_cg=50)
+    traj, _ = S.trajectory_scene(cameras=20, points=3000, seed=2)
+    yield "trajectory_shared", traj, B.make_control(
+        traj.C, traj.P, traj.M, iterations=20, max_cg=256, shared_focal=True, huber=1.0)
+    # more points than one pass of the row workgroups' lookup table (8,192)
+    wide, _ = S.trajectory_scene(cameras=12, points=20000, seed=7)
+    yield "trajectory_wide", wide, B.make_control(
+        wide.C, wide.P, wide.M, iterations=10, max_cg=256, shared_focal=True, huber=1.0)
+
+
+PROBLEMS = list(problems())
+
+
+def compare(got, expected, base, cost_bar, coord_bar):
+    cost = expected[base + 6]
+    rel = abs(got[base + 6] - cost) / max(1.0, abs(cost))
+    coord = np.max(np.abs(got[:base] - expected[:base]) / np.maximum(1.0, np.abs(expected[:base])))
+    print(f"  final cost {got[base + 6]:.12g} vs {cost:.12g} (rel {rel:.2e}); "
+          f"max coordinate error {coord:.2e}", flush=True)
+    assert rel <= cost_bar
+    assert coord <= coord_bar
+
+
+@pytest.mark.parametrize("name,packed,control", PROBLEMS, ids=[p[0] for p in PROBLEMS])
+def test_direct_matches_tight_pcg_oracle(device, name, packed, control):
+    print(f"\n[direct] {name} C={packed.C} P={packed.P} M={packed.M}", flush=True)
+    tight = variant(control, tolerance=1e-12, max_cg=256)
+    expected, stats = O.solve(tight, packed)
+    got, report = B.solve_packed(device, variant(control, schur_cache=True), packed)
+    base = packed.C * 12 + packed.P * 3
+    print(f"  direct solves {report['direct_solves']}, PCG {report['cg_iterations']} "
+          f"(oracle PCG {stats['cg_iterations']:.0f})", flush=True)
+    assert report["direct_solves"] > 0
+    compare(got, expected, base, 1e-9, 1e-6)
+    for word in (10, 11, 14):
+        assert got[base + word] == expected[base + word], word
+    if control.shared_focal:
+        assert abs(got[base + 26] - expected[base + 26]) <= 1e-8
+
+
+def test_direct_needs_no_pcg_when_the_factor_certifies(device):
+    _, packed, control = PROBLEMS[-2]
+    got, report = B.solve_packed(device, variant(control, schur_cache=True), packed)
+    base = packed.C * 12 + packed.P * 3
+    steps = got[base + 10] + got[base + 11]
+    print(f"\n[direct] certified {report['direct_solves']} of {steps:.0f} LM steps, "
+          f"PCG {report['cg_iterations']}", flush=True)
+    assert report["direct_solves"] == steps
+    assert report["cg_iterations"] == 0
+
+
+def test_direct_ring_matches_the_dense_cpu_solver(device):
+    print("\n[direct] cfg1 ring vs dense CPU solver", flush=True)
+    ring, _ = S.ring_scene(seed=1)
+    control = B.make_control(ring.C, ring.P, ring.M, iterations=20, max_cg=50, schur_cache=True)
+    got, report = B.solve_packed(device, control, ring)
+    dense = O.dense_solve(control, ring)
+    base = ring.C * 12 + ring.P * 3
+    assert report["direct_solves"] > 0
+    compare(got, dense, base, 1e-7, 1e-7)
+
+
+@pytest.mark.parametrize("name,packed,control", PROBLEMS[-3:], ids=[p[0] for p in PROBLEMS[-3:]])
+def test_direct_mixed_against_fp64_oracle(device, name, packed, control):
+    """The factor of the FP32 cross blocks' system, certified and corrected in FP64: the final cost
+    within the north star's 1e-6 of the FP64 oracle. Coordinates are printed, not bounded: a step
+    is only as exact as its certified residual, and the far points of the wide scene move by the
+    conditioning times it (1.5e-4 relative at a certified 1e-10, against 4e-9 for the FP64
+    factor)."""
+    print(f"\n[direct mixed] {name}", flush=True)
+    expected, _ = O.solve(variant(control, tolerance=1e-12, max_cg=256), packed)
+    for refine in (0.0, 1e-10):
+        got, report = B.solve_packed(device, variant(control, precision="mixed", schur_cache=True,
+                                                     refinement_tolerance=refine), packed)
+        base = packed.C * 12 + packed.P * 3
+        print(f"  refine {refine:g}: direct {report['direct_solves']}, "
+              f"PCG {report['cg_iterations']}", flush=True)
+        compare(got, expected, base, 1e-6, np.inf)
+
+
+def test_direct_is_deterministic(device):
+    print("\n[direct] determinism", flush=True)
+    _, packed, control = PROBLEMS[-2]
+    c = variant(control, schur_cache=True)
+    a, _ = B.solve_packed(device, c, packed)
+    b, _ = B.solve_packed(device, c, packed)
+    assert np.array_equal(a, b)
+
+
+def test_direct_batch_equals_single_solves(device):
+    print("\n[direct] batch with a bundle above the direct route's size", flush=True)
+    scenes = [S.trajectory_scene(cameras=c, points=p, seed=s, track=(3, min(c, 12)))[0]
+              for c, p, s in ((6, 800, 1), (130, 2000, 2), (25, 4000, 3))]
+    controls = [B.make_control(s.C, s.P, s.M, iterations=8, max_cg=256, shared_focal=True,
+                               huber=1.0, schur_cache=True) for s in scenes]
+    outs, report, _ = B.solve_batch(device, controls, scenes)
+    singles = 0
+    for scene, control, out in zip(scenes, controls, outs):
+        alone, rep = B.solve_packed(device, control, scene)
+        singles += rep["direct_solves"]
+        assert np.array_equal(out, alone)
+    assert report["direct_solves"] == singles
+    assert singles > 0

@@ -13,6 +13,7 @@ FP64 = 0
 MIXED = 1
 FLAG_TIMING = 1
 FLAG_IMPLICIT = 2
+FLAG_SCHUR_CACHE = 4
 
 STATUS = {
     0: "ok",
@@ -86,6 +87,7 @@ class Report(ctypes.Structure):
         ("device_bytes", ctypes.c_uint64),
         ("point_tiles", ctypes.c_uint64),
         ("camera_tasks", ctypes.c_uint64),
+        ("direct_solves", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:

@@ -317,9 +317,11 @@ def make_control(C: int, P: int, M: int, *, directional: int = 0, anchor: int = 
                  iterations: int = 20, max_cg: int = 24, precision: str = "fp64",
                  huber: float = 2.0, tolerance: float = 1e-6, stall_tolerance: float = 1e-9,
                  refinement_tolerance: float = 0.0, timing: bool = False,
-                 implicit: bool = False) -> _native.Control:
-    """The `tcv_ba_control` of a problem."""
-    flags = (_native.FLAG_TIMING if timing else 0) | (_native.FLAG_IMPLICIT if implicit else 0)
+                 implicit: bool = False, schur_cache: bool = False) -> _native.Control:
+    """The `tcv_ba_control` of a problem. `schur_cache`: the direct route (dense reduced camera
+    system, Cholesky, matrix-free certificate) for bundles of at most 128 cameras."""
+    flags = ((_native.FLAG_TIMING if timing else 0) | (_native.FLAG_IMPLICIT if implicit else 0)
+             | (_native.FLAG_SCHUR_CACHE if schur_cache else 0))
     return _native.Control(
         cameras=C, points=P, directional_points=directional, observations=M, anchor=anchor,
         scale_view=scale_view, normalize_scale=int(normalize_scale),
@@ -550,7 +552,8 @@ def bundle_adjust_gpu(device: HipDevice, scene: Mapping, options: Mapping | None
         scale_view=problem["byId"][problem["scaleId"]], normalize_scale=problem["fixed"] == 1,
         shared_focal=shared, iterations=problem["iterations"], max_cg=problem["maxCg"],
         precision=options.get("precision", "fp64"), huber=problem["huber"],
-        tolerance=problem["tolerance"], stall_tolerance=stall)
+        tolerance=problem["tolerance"], stall_tolerance=stall,
+        schur_cache=options.get("schurCache") is True)
     before = device.statistics
     values, report = solve_packed(device, control, packed)
     after = device.statistics

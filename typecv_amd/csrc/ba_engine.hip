@@ -21,6 +21,7 @@
 
 #include "../../include/typecv_ba.h"
 #include "ba_kernels.hip"
+#include "ba_dense.hip"
 
 using namespace tcvba;
 
@@ -53,7 +54,7 @@ struct tcv_ba_context {
   DevBuf scene, trial, index, out, bundles, tiles, tile_bundle, tasks, task_start, cost_blocks,
       camera_bundle, point_bundle, cam_point, Wp, Wc, pblk, tile_fail, tile_focal, task_part, camA,
       camInv, camRhs, camHcf, camFocal, vec, state, cost_part, error, probe, groups, group_part,
-      rhs, grad;
+      rhs, grad, camS, camG, pfc, dense, direct_bundles, direct_cams;
   std::vector<hipEvent_t> events;
   tcv_ba_context() {
     all = {&scene,      &trial,        &index,     &out,       &bundles,    &tiles,
@@ -61,7 +62,8 @@ struct tcv_ba_context {
            &point_bundle, &cam_point,  &Wp,        &Wc,        &pblk,       &tile_fail,
            &tile_focal, &task_part,    &camA,      &camInv,    &camRhs,     &camHcf,
            &camFocal,   &vec,          &state,     &cost_part, &error,      &probe,
-           &groups,     &group_part,   &rhs,       &grad};
+           &groups,     &group_part,   &rhs,       &grad,      &camS,       &camG,   &pfc,
+           &dense,      &direct_bundles, &direct_cams};
   }
 };
 
@@ -119,7 +121,8 @@ int check_control(const tcv_ba_control* c) {
                   c->stall_tolerance >= 0 && c->stall_tolerance < 1 &&
                   (c->refinement_tolerance == 0 ||
                    (c->refinement_tolerance >= 1e-14 && c->refinement_tolerance <= 1e-2)) &&
-                  (c->flags & ~(TCV_BA_FLAG_TIMING | TCV_BA_FLAG_IMPLICIT)) == 0;
+                  (c->flags & ~(TCV_BA_FLAG_TIMING | TCV_BA_FLAG_IMPLICIT |
+                                TCV_BA_FLAG_SCHUR_CACHE)) == 0;
   if (!ok) {
     set_error("control outside the admitted ranges (bundleProblem)");
     return TCV_BA_ERR_ARGUMENT;
@@ -139,12 +142,37 @@ bool same_options(const tcv_ba_control& a, const tcv_ba_control& b) {
 struct Plan {
   std::vector<Bundle> bundles;
   std::vector<uint32_t> tiles, tile_bundle, tasks, task_start, cost_blocks, camera_bundle,
-      point_bundle, groups;
+      point_bundle, groups, direct_bundles, direct_cams;
+  uint64_t dense_words = 0;
 };
+
+// The direct route for the bundles of at most DENSE_MAX_CAMERAS cameras (all others stay on PCG).
+void plan_direct(Plan& plan, bool want, uint32_t shared) {
+  plan.direct_bundles.clear();
+  plan.direct_cams.clear();
+  plan.dense_words = 0;
+  for (uint32_t b = 0; b < plan.bundles.size(); b++) {
+    Bundle& g = plan.bundles[b];
+    const uint32_t cams = g.cam1 - g.cam0;
+    g.direct = want && cams <= DENSE_MAX_CAMERAS ? 1u : 0u;
+    // m_b unknowns padded to a multiple of DENSE_NB, plus the right-hand-side row
+    g.dense_n = g.direct ? (6 * cams + shared + DENSE_NB - 1) / DENSE_NB * DENSE_NB + 1 : 0;
+    g.dense_off = plan.dense_words;
+    if (!g.direct) continue;
+    plan.dense_words += (uint64_t)g.dense_n * g.dense_n;
+    plan.direct_bundles.push_back(b);
+    for (uint32_t c = g.cam0; c < g.cam1; c++) plan.direct_cams.push_back(c);
+  }
+}
 
 constexpr uint32_t kTaskObs = 1024;
 constexpr uint32_t kChunk = 8;  // PCG iterations between reads of the state words
 constexpr uint32_t kMaxRefinements = 4;  // mixed form: refinement restarts per LM iteration
+#ifndef TCV_DIRECT_PASSES
+#define TCV_DIRECT_PASSES 3
+#endif
+constexpr uint32_t kDirectPasses = TCV_DIRECT_PASSES;  // certificates of a direct solution per LM
+                                                        // iteration (corrections in between)
 
 // Bundle ranges, wave tiles of whole points of one bundle (at most 64 observations and 64 points;
 // a longer point alone), camera-list tasks of at most kTaskObs slots (each camera has at least
@@ -312,17 +340,34 @@ struct Launch {
   static void implicit_p_sweep(const Problem& q, hipStream_t s) {
     hipLaunchKernelGGL((k_point_sweep_implicit<SHARED>), dim3(tile_blocks(q)),
                        dim3(TILE * TILE_WAVES), 0, s, q, (const double*)(q.vec + 2 * (size_t)q.m),
-                       1);
+                       (int)GUARD_PCG);
   }
   static void implicit_camera_sweep(const Problem& q, hipStream_t s) {
     hipLaunchKernelGGL((k_camera_sweep_implicit<SHARED>), dim3(q.n_tasks), dim3(TASK_THREADS), 0,
-                       s, q, 1);
+                       s, q, (int)GUARD_PCG);
+  }
+  // The direct route: assembly and factorisation, the certificate of x, a correction from the factor.
+  static void dense_solve(const Problem& q, hipStream_t s) {
+    hipLaunchKernelGGL((k_dense_rows<SHARED>), dim3(q.n_direct_cams), dim3(DENSE_THREADS), 0,
+                       s, q);
+    hipLaunchKernelGGL((k_dense_factor<SHARED>), dim3(q.n_direct), dim3(DENSE_THREADS), 0, s, q);
+  }
+  static void certify(const Problem& q, int final_pass, double tol, hipStream_t s) {
+    hipLaunchKernelGGL((k_point_sweep_implicit<SHARED>), dim3(tile_blocks(q)),
+                       dim3(TILE * TILE_WAVES), 0, s, q, (const double*)q.vec, (int)GUARD_DIRECT);
+    hipLaunchKernelGGL((k_camera_sweep_implicit<SHARED>), dim3(q.n_tasks), dim3(TASK_THREADS), 0,
+                       s, q, (int)GUARD_DIRECT);
+    hipLaunchKernelGGL((k_certify<SHARED>), dim3(q.n_direct), dim3(CG_THREADS), 0, s, q,
+                       final_pass, tol);
+  }
+  static void dense_correct(const Problem& q, hipStream_t s) {
+    hipLaunchKernelGGL((k_dense_correct<SHARED>), dim3(q.n_direct), dim3(DENSE_THREADS), 0, s, q);
   }
   static void refine(const Problem& q, hipStream_t s) {
     hipLaunchKernelGGL((k_point_sweep_implicit<SHARED>), dim3(tile_blocks(q)),
-                       dim3(TILE * TILE_WAVES), 0, s, q, (const double*)q.vec, 0);
+                       dim3(TILE * TILE_WAVES), 0, s, q, (const double*)q.vec, (int)GUARD_NONE);
     hipLaunchKernelGGL((k_camera_sweep_implicit<SHARED>), dim3(q.n_tasks), dim3(TASK_THREADS), 0,
-                       s, q, 0);
+                       s, q, (int)GUARD_NONE);
     hipLaunchKernelGGL((k_refine<SHARED>), dim3(q.B), dim3(CG_THREADS), 0, s, q);
   }
 };
@@ -341,6 +386,9 @@ struct Ops {
   void (*solve_points)(const Problem&, hipStream_t);
   void (*pixel_cotangent)(const Problem&, double*, hipStream_t);
   void (*jacobian)(const Problem&, double*, hipStream_t);
+  void (*dense_solve)(const Problem&, hipStream_t);
+  void (*certify)(const Problem&, int, double, hipStream_t);
+  void (*dense_correct)(const Problem&, hipStream_t);
 };
 
 template <class WT, bool SHARED>
@@ -349,7 +397,7 @@ Ops ops_for(bool implicit) {
   Ops o{L::point_pass,   L::camera_pass,   L::init_cg,         L::point_sweep,
         L::camera_sweep, L::advance,       L::update_points,   L::probe,
         L::refine,       L::exact_passes,  L::solve_points,    L::pixel_cotangent,
-        L::jacobian};
+        L::jacobian,     L::dense_solve,   L::certify,         L::dense_correct};
   if (implicit) {
     o.point_sweep = L::implicit_p_sweep;
     o.camera_sweep = L::implicit_camera_sweep;
@@ -409,6 +457,8 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
   HIP_TRY(hipStreamSynchronize(st));
   Plan plan;
   TRY(build_plan(ps, cs, controls, B, C, P, M, plan));
+  plan_direct(plan, (c->flags & TCV_BA_FLAG_SCHUR_CACHE) != 0, shared);
+  const bool direct = !plan.direct_bundles.empty();
   const uint32_t n_tiles = (uint32_t)plan.tiles.size() - 1;
   const uint32_t n_tasks = (uint32_t)plan.tasks.size() / 4;
   const uint32_t n_cost = (uint32_t)plan.cost_blocks.size() / 3;
@@ -435,6 +485,14 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
   TRY(ensure(ctx->error, 8));
   TRY(ensure(ctx->group_part, (size_t)n_groups * 4 * 8));
   TRY(ensure_host_state(ctx, (size_t)B * S_WORDS));
+  if (direct) {
+    TRY(ensure(ctx->camS, (size_t)C * 36 * 8));
+    TRY(ensure(ctx->camG, (size_t)M * 18 * 8));
+    if (shared) TRY(ensure(ctx->pfc, (size_t)P * 3 * 8));
+    TRY(ensure(ctx->dense, plan.dense_words * 8));
+    TRY(upload(ctx->direct_bundles, plan.direct_bundles, st));
+    TRY(upload(ctx->direct_cams, plan.direct_cams, st));
+  }
   TRY(upload(ctx->bundles, plan.bundles, st));
   TRY(upload(ctx->tiles, plan.tiles, st));
   TRY(upload(ctx->tile_bundle, plan.tile_bundle, st));
@@ -494,15 +552,33 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
   q.state = (double*)ctx->state.ptr;
   q.cost_part = (double*)ctx->cost_part.ptr;
   q.error = (uint32_t*)ctx->error.ptr;
+  if (direct) {
+    q.n_direct = (uint32_t)plan.direct_bundles.size();
+    q.n_direct_cams = (uint32_t)plan.direct_cams.size();
+    q.direct_bundles = (const uint32_t*)ctx->direct_bundles.ptr;
+    q.direct_cams = (const uint32_t*)ctx->direct_cams.ptr;
+    q.camS = (double*)ctx->camS.ptr;
+    q.camG = (double*)ctx->camG.ptr;
+    q.pfc = shared ? (double*)ctx->pfc.ptr : nullptr;
+    q.dense = (double*)ctx->dense.ptr;
+  }
   hipLaunchKernelGGL(k_check_points, dim3(blocks_for(P, 256)), dim3(256), 0, st, q);
   hipLaunchKernelGGL(k_check_tasks, dim3(n_tasks), dim3(256), 0, st, q);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(ctx->h_error, ctx->error.ptr, 4, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
-  if (*ctx->h_error != 0) {
+  if ((*ctx->h_error & 3u) != 0) {
     set_error(*ctx->h_error & 1 ? "observation point/camera ids do not match the point offsets"
                                 : "camera lists do not match the observation cameras");
     return TCV_BA_ERR_INDEX;
+  }
+  if (direct && (*ctx->h_error & 4u) != 0) {
+    // a camera observes one point twice: its point list is not a set, so the batch keeps PCG
+    plan_direct(plan, false, shared);
+    TRY(upload(ctx->bundles, plan.bundles, st));
+    q.n_direct = q.n_direct_cams = 0;
+    q.direct_bundles = q.direct_cams = nullptr;
+    q.camS = q.camG = q.pfc = q.dense = nullptr;
   }
   if (rep) {
     rep->setup_ms = now_ms() - t0;
@@ -629,7 +705,26 @@ int run_lm(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, hipSt
     ops.camera_pass(q, st);
     ops.init_cg(q, st);
     HIP_TRY(hipGetLastError());
-    TRY(run_pcg(ctx, c, q, ops, st, rep, hint, c->max_cg));
+    if (q.n_direct) {
+      // The direct route: factor and solve, then certify x with the matrix-free residual (the
+      // tolerance of the form's final residual), correcting from the factor in between; a
+      // solution that still misses it hands over to PCG, which also runs the bundles that are not
+      // direct and those whose factorisation failed.
+      const double tol = c->precision == TCV_BA_MIXED ? q.refine_tol : q.tol;
+      ops.dense_solve(q, st);
+      for (uint32_t k = 0; k < kDirectPasses; k++) {
+        ops.certify(q, k + 1 == kDirectPasses, tol, st);
+        HIP_TRY(hipGetLastError());
+        TRY(read_state(ctx, q, st, rep));
+        bool pending = false;
+        for (uint32_t b = 0; b < q.B; b++) pending |= word(ctx, b, S_DIRECT) == 1;
+        if (!pending) break;
+        ops.dense_correct(q, st);
+      }
+      if (any_word(ctx, q, S_PCG_ACTIVE)) TRY(run_pcg(ctx, c, q, ops, st, rep, hint, c->max_cg));
+    } else {
+      TRY(run_pcg(ctx, c, q, ops, st, rep, hint, c->max_cg));
+    }
     // The mixed form: replace the FP32-operator residual by the FP64 one and restart PCG from x
     // until the FP64 residual meets the refinement tolerance (each restart has its own budget).
     if (c->precision == TCV_BA_MIXED) {
@@ -639,7 +734,7 @@ int run_lm(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, hipSt
         bool pending = false;
         for (uint32_t b = 0; b < q.B; b++)
           pending |= word(ctx, b, S_CONVERGED) != 0 && word(ctx, b, S_PCG_FAILED) == 0 &&
-                     word(ctx, b, S_ACTIVE) != 0;
+                     word(ctx, b, S_ACTIVE) != 0 && word(ctx, b, S_DIRECT) != 2;
         if (!pending) break;
         ops.refine(q, st);
         HIP_TRY(hipGetLastError());
@@ -692,13 +787,15 @@ int run_lm(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, hipSt
   if (finish_ms)
     for (uint32_t b = 0; b < q.B; b++) finish_ms[b] = finish[b] < 0 ? end : finish[b];
   if (rep) {
-    double cg = 0, lm = 0;
+    double cg = 0, lm = 0, direct = 0;
     for (uint32_t b = 0; b < q.B; b++) {
       cg += word(ctx, b, S_CG_STEPS);
       lm += word(ctx, b, S_ACCEPTED) + word(ctx, b, S_REJECTED);
+      direct += word(ctx, b, S_DIRECT_SOLVES);
     }
     rep->cg_iterations = (uint64_t)cg;
     rep->lm_iterations = (uint64_t)lm;
+    rep->direct_solves = (uint64_t)direct;
   }
   return TCV_BA_OK;
 }

@@ -47,6 +47,10 @@ enum : int {
   S_FAIL_TILES = 34,     // point tiles with a failed point block at the last PCG set-up
   S_FAIL_CAMERAS = 35,   // cameras with a failed preconditioner block at the last PCG set-up
   S_FAIL_FOCAL = 36,     // 1 when the reduced focal diagonal was not positive
+  S_FOCAL_SCHUR = 37,    // reduced focal diagonal H_ff(damped) - sum_p fc_p^T V_p^-1 fc_p
+  S_DIRECT = 38,         // direct route this LM iteration: 0 none, 1 solved and not yet
+                         // certified, 2 certified by the matrix-free residual
+  S_DIRECT_SOLVES = 39,  // certified direct solves (whole solve)
   S_WORDS = 40
 };
 
@@ -67,6 +71,10 @@ constexpr int GROUP_CAMS = 42;     // cameras of a PCG-step workgroup (6 rows ea
 constexpr int CG_THREADS = 1024;   // threads of the per-bundle PCG kernels
 constexpr int COST_OBS = 256;      // observations per cost block
 constexpr int ASM_WORDS = 56;      // per-task camera assembly sums (see k_camera_pass)
+constexpr int DENSE_THREADS = 256; // threads of the direct route's row and factor workgroups
+constexpr int DENSE_TABLE = 8192;  // point ids of one pass of a row workgroup's lookup table
+constexpr int DENSE_NB = 32;       // block size of the dense Cholesky factorisation
+constexpr uint32_t DENSE_MAX_CAMERAS = 128;  // largest bundle the direct route factors
 
 // One bundle of the batch: its global ranges and gauge.
 struct Bundle {
@@ -77,7 +85,10 @@ struct Bundle {
   uint32_t normalize_scale;
   uint32_t focal_row;  // 6C + b
   uint32_t group0, group1;  // the bundle's PCG-step groups
+  uint32_t direct;          // 1: the reduced camera system is assembled and factored
+  uint32_t dense_n;         // rows of the bundle's dense array: m_b = 6 C_b [+ 1] unknowns + 1
   uint32_t pad;
+  uint64_t dense_off;       // first double of the bundle's dense array
 };
 
 struct Problem {
@@ -121,6 +132,15 @@ struct Problem {
   const double* rhs_in;
   // exact-Hessian systems (polish, adjoint): regularise the one-fixed-camera similarity gauge
   uint32_t exact_gauge;
+  // the direct route (null / 0 when no bundle takes it)
+  uint32_t n_direct, n_direct_cams;
+  const uint32_t* direct_bundles;  // n_direct bundle ids
+  const uint32_t* direct_cams;     // n_direct_cams camera ids, the cameras of those bundles
+  double* camS;    // 36 per camera: the Schur diagonal block S_cc = A_c - sum W V^-1 W^T
+  double* camG;    // 18 per camera-major slot, contiguous: G = W L_p with V_p^-1 = L_p L_p^T
+  double* pfc;     // 3 per point (shared focal): L_p^T fc_p, fc_p = sum_i w Jp_i^T Jf_i
+  double* dense;   // per direct bundle dense_n x dense_n row-major; the lower triangle holds S,
+                   // then its Cholesky factor; row m_b holds the reduced right-hand side
 };
 
 __host__ __device__ inline const uint32_t* obs_camera(const Problem& q) { return q.index; }

@@ -293,7 +293,8 @@ __global__ void k_check_points(Problem q) {
 }
 
 // Every slot of a camera task names an observation of that camera, ascending within the camera's
-// list; writes the camera-major point ids.
+// list; writes the camera-major point ids. Flag 4 (not an error) marks a camera that observes one
+// point twice: the direct route's point lookup needs each camera's points strictly ascending.
 __global__ void k_check_tasks(Problem q) {
   const uint32_t t = blockIdx.x;
   const uint32_t c = q.tasks[4 * t], s = q.tasks[4 * t + 1], e = q.tasks[4 * t + 2];
@@ -302,7 +303,11 @@ __global__ void k_check_tasks(Problem q) {
     const uint32_t i = camera_obs(q)[n];
     bool bad = i >= q.M;
     if (!bad) bad = obs_camera(q)[i] != c;
-    if (!bad && n > list) bad = camera_obs(q)[n - 1] >= i;
+    if (!bad && n > list) {
+      const uint32_t before = camera_obs(q)[n - 1];
+      bad = before >= i;
+      if (!bad && obs_point(q)[before] == obs_point(q)[i]) atomicOr(q.error, 4u);
+    }
     if (bad) {
       atomicOr(q.error, 2u);
       q.cam_point[n] = 0;
@@ -581,8 +586,10 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_pass(Problem q) {
       failed = !inverse3(A, inv);
     }
     double* out = q.pblk + (size_t)my * 12;
+    double* pf = (SHARED && q.pfc) ? q.pfc + (size_t)my * 3 : nullptr;
     if (failed) {
       for (int k = 0; k < 12; k++) out[k] = 0;
+      if (pf) pf[0] = pf[1] = pf[2] = 0;
     } else {
       for (int k = 0; k < 9; k++) out[k] = inv[k];
       for (int k = 0; k < 3; k++) out[9 + k] = bp[k];
@@ -592,6 +599,13 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_pass(Problem q) {
           for (int k = 0; k < 3; k++) tj += fc[k] * inv[k * 3 + j];
           dF += tj * fc[j];
           dR += tj * bp[j];
+        }
+        if (pf) {
+          double L[6];
+          lower_cholesky3(inv, L);
+          pf[0] = L[0] * fc[0] + L[1] * fc[1] + L[3] * fc[2];
+          pf[1] = L[2] * fc[1] + L[4] * fc[2];
+          pf[2] = L[5] * fc[2];
         }
       }
     }
@@ -675,6 +689,18 @@ __global__ __launch_bounds__(TASK_THREADS) void k_camera_pass(Problem q) {
 #pragma unroll
         for (int l = 0; l < 3; l++)
           T[j * 3 + l] = W[j * 3] * Vi[l] + W[j * 3 + 1] * Vi[3 + l] + W[j * 3 + 2] * Vi[6 + l];
+      if (q.camG) {
+        // the direct route's factor of this slot's share of W V^-1 W^T
+        double Lf[6];
+        lower_cholesky3(Vi, Lf);
+        double* G = q.camG + (size_t)n * 18;
+#pragma unroll
+        for (int j = 0; j < 6; j++) {
+          G[j * 3] = W[j * 3] * Lf[0] + W[j * 3 + 1] * Lf[1] + W[j * 3 + 2] * Lf[3];
+          G[j * 3 + 1] = W[j * 3 + 1] * Lf[2] + W[j * 3 + 2] * Lf[4];
+          G[j * 3 + 2] = W[j * 3 + 2] * Lf[5];
+        }
+      }
 #pragma unroll
       for (int j = 0; j < 6; j++) {
         const double wj0 = w * L.Jc[j], wj1 = w * L.Jc[6 + j];
@@ -734,6 +760,8 @@ __global__ void k_camera_finish(Problem q) {
   }
   if (!variable_camera(q, c)) {
     for (int j = 0; j < 36; j++) A[j] = inv[j] = (j % 7 == 0) ? 1.0 : 0.0;
+    if (q.camS)
+      for (int j = 0; j < 36; j++) q.camS[(size_t)c * 36 + j] = (j % 7 == 0) ? 1.0 : 0.0;
     for (int j = 0; j < 6; j++) {
       rhs[j] = 0;
       if (SHARED) q.camHcf[(size_t)c * 6 + j] = 0;
@@ -775,6 +803,8 @@ __global__ void k_camera_finish(Problem q) {
     A[j] = a[j];
     S[j] = a[j] - S[j];
   }
+  if (q.camS)
+    for (int j = 0; j < 36; j++) q.camS[(size_t)c * 36 + j] = S[j];
   for (int j = 0; j < 6; j++) {
     rhs[j] = acc[42 + j];
     if (q.rhs_in) rhs[j] += q.rhs_in[(size_t)c * 6 + j];
@@ -828,8 +858,10 @@ __global__ __launch_bounds__(CG_THREADS) void k_init_cg(Problem q) {
   const uint32_t tid = threadIdx.x;
   const double active = s[S_ACTIVE], lambda = s[S_LAMBDA];
   __syncthreads();
-  if (tid == 0)
+  if (tid == 0) {
     for (int j = S_RHO; j < 26; j++) s[j] = 0;
+    s[S_DIRECT] = 0;
+  }
   if (active == 0) return;
   double fail_tiles = 0, fail_cameras = 0;
   for (uint32_t t = g.tile0 + tid; t < g.tile1; t += CG_THREADS)
@@ -869,6 +901,7 @@ __global__ __launch_bounds__(CG_THREADS) void k_init_cg(Problem q) {
       s[S_FOCAL_DIAG] = damped;
       s[S_FOCAL_INV] = finv;
       s[S_FOCAL_RHS] = frhs;
+      s[S_FOCAL_SCHUR] = diag;
     }
     if (!(finv > 0)) {
       fail += 1;
@@ -1270,11 +1303,21 @@ __global__ __launch_bounds__(CG_THREADS) void k_probe(Problem q, const double* v
 
 // ------------------------------------------------------------------ implicit FP64 operator
 
+// Which bundles a sweep of the implicit operator covers.
+enum SweepGuard { GUARD_NONE = 0, GUARD_PCG = 1, GUARD_DIRECT = 2 };
+
+__device__ __forceinline__ bool skip_bundle(const double* sb, int guard) {
+  if (guard == GUARD_PCG) return sb[S_PCG_ACTIVE] == 0;
+  if (guard == GUARD_DIRECT) return sb[S_DIRECT] != 1;
+  return false;
+}
+
 // The two Schur sweeps with every cross block relinearised from the scene in FP64 instead of read
 // from storage: (W^T v)_j = w (Jp_u[j] (Jc_u . v_c + Jf_u v_f) + Jp_v[j] (Jc_v . v_c + Jf_v v_f))
 // and (W y)_k = w (Jc_u[k] (Jp_u . y) + Jc_v[k] (Jp_v . y)). The mixed form computes its
 // refinement residual b - S x with this operator, so FP64 cross blocks never need storing.
-// `guard`: skip bundles whose PCG is inactive (the PCG use); the refinement runs regardless.
+// `guard`: GUARD_PCG skips bundles whose PCG is inactive (the PCG use), GUARD_DIRECT those without
+// an uncertified direct solution; GUARD_NONE (the refinement) runs every bundle.
 template <bool SHARED>
 __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_sweep_implicit(Problem q,
                                                                           const double* v,
@@ -1286,7 +1329,7 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_sweep_implicit(Probl
   if (t >= q.n_tiles) return;
   const uint32_t b = q.tile_bundle[t];
   const double* sb = bstate(q, b);
-  if (guard && sb[S_PCG_ACTIVE] == 0) return;
+  if (skip_bundle(sb, guard)) return;
   const Bundle& g = q.bundles[b];
   auto& st = st_all[wave];
   const uint32_t fp = q.tiles[t], ep = q.tiles[t + 1];
@@ -1338,7 +1381,7 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_sweep_implicit(Probl
 }
 
 // The implicit camera sweep: relinearises each slot of the task instead of reading its stored
-// cross block. `guard` set: the PCG loop's sweep of p (skipped once word 19 is clear).
+// cross block. `guard` as for the implicit point sweep.
 template <bool SHARED>
 __global__ __launch_bounds__(TASK_THREADS) void k_camera_sweep_implicit(Problem q, int guard) {
   constexpr int NV = SHARED ? 7 : 6;
@@ -1347,7 +1390,7 @@ __global__ __launch_bounds__(TASK_THREADS) void k_camera_sweep_implicit(Problem 
   const uint32_t c = q.tasks[4 * t], s = q.tasks[4 * t + 1], e = q.tasks[4 * t + 2];
   const uint32_t b = q.camera_bundle[c];
   const double* sb = bstate(q, b);
-  if (guard && sb[S_PCG_ACTIVE] == 0) return;
+  if (skip_bundle(sb, guard)) return;
   const Bundle& g = q.bundles[b];
   const bool var = variable_camera(q, c);
   const double fscale = SHARED ? focal_scale(sb[S_LOG_FOCAL]) : 1.0;
@@ -1389,7 +1432,7 @@ __global__ __launch_bounds__(CG_THREADS) void k_refine(Problem q) {
   double* s = bstate(q, bi);
   const double converged = s[S_CONVERGED], failed = s[S_PCG_FAILED], r0 = s[S_R0];
   const double active = s[S_ACTIVE];
-  if (active == 0 || converged == 0 || failed != 0) return;
+  if (active == 0 || converged == 0 || failed != 0 || s[S_DIRECT] == 2) return;
   const uint32_t tid = threadIdx.x;
   double* x = vx(q);
   double* r = vr(q);

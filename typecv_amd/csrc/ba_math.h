@@ -137,6 +137,18 @@ __device__ __forceinline__ bool inverse3(const double V[9], double inv[9]) {
   return true;
 }
 
+// Lower Cholesky factor of a symmetric positive definite 3x3 block (row-major V): V = L L^T with
+// L packed as l00, l10, l11, l20, l21, l22. The direct route splits V_p^-1 = L L^T so that each
+// observation's cross block becomes one factor G = W L of the product W V^-1 W^T = G G^T.
+__device__ __forceinline__ void lower_cholesky3(const double V[9], double L[6]) {
+  L[0] = sqrt(V[0]);
+  L[1] = V[3] / L[0];
+  L[2] = sqrt(V[4] - L[1] * L[1]);
+  L[3] = V[6] / L[0];
+  L[4] = (V[7] - L[3] * L[1]) / L[2];
+  L[5] = sqrt(V[8] - L[3] * L[3] - L[4] * L[4]);
+}
+
 // The equilibrated Cholesky factor of numerics.shade.ts:221-253 (solve6), computed once.
 struct Chol6 {
   double L[21];  // packed lower triangle, row i holds L[i][0..i]

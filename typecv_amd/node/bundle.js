@@ -40,15 +40,16 @@ class HipDevice {
   }
 }
 
-/* tcv_ba_control as the addon's Float64Array(16). */
+/* tcv_ba_control as the addon's Float64Array(16). `schurCache` sets TCV_BA_FLAG_SCHUR_CACHE (4),
+ * the direct route of the reference's option of that name. */
 function makeControl({ cameras, points, observations, directionalPoints = 0, anchor = 0,
   scaleView = 1, normalizeScale = true, sharedFocal = false, iterations = 20, maxCg = 24,
   precision = 'fp64', flags = 0, huber = 2, tolerance = 1e-6, stallTolerance = 1e-9,
-  refinementTolerance = 0 }) {
+  refinementTolerance = 0, schurCache = false }) {
   return Float64Array.from([cameras, points, directionalPoints, observations, anchor, scaleView,
     normalizeScale ? 1 : 0, sharedFocal ? 1 : 0, iterations, maxCg,
-    precision === 'mixed' ? 1 : 0, flags, huber, tolerance, stallTolerance,
-    refinementTolerance]);
+    precision === 'mixed' ? 1 : 0, schurCache ? flags | 4 : flags, huber, tolerance,
+    stallTolerance, refinementTolerance]);
 }
 
 /* One packed solve: the packBundleResult array and its diagnostics, with checkBundleOutput's
