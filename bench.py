@@ -188,12 +188,35 @@ def main() -> int:
     return 0
 
 
+def run_batch(B, device, controls, packeds, steps, warmup):
+    """Times `steps` batched solves after `warmup`: (LM-loop seconds per step, wall seconds per
+    step, per-bundle LM-loop latencies in ms, summed report counters)."""
+    for _ in range(warmup):
+        B.solve_batch(device, controls, packeds)
+    loops, walls, latencies = [], [], []
+    totals = {"lm_iterations": 0, "cg_iterations": 0, "direct_solves": 0}
+    for k in range(steps):
+        t = time.perf_counter()
+        outs, report, finish = B.solve_batch(device, controls, packeds)
+        walls.append(time.perf_counter() - t)
+        loops.append(float(finish.max()) * 1e-3)
+        latencies.extend(finish.tolist())
+        for key in totals:
+            totals[key] += report[key]
+        log(f"step {k}: wall {walls[-1] * 1e3:.1f} ms, LM loop {loops[-1] * 1e3:.1f} ms, "
+            f"{report['lm_iterations']} LM, {report['cg_iterations']} PCG, "
+            f"{report['direct_solves']} direct")
+    return loops, walls, latencies, totals
+
+
 def batched(args) -> int:
     """256 independent local bundles (cameras U{5..40}, points U{500..20000}, each point seen by
     3-12 cameras of its bundle; the trajectory scene's noise, outliers and focal error), shared
     focal, Huber 1 px, 30 LM iterations, mixed precision, solved together by one
-    tcv_ba_solve_batch call per step. Reports bundles/s over the batches' LM loops and the p50/p99
-    of the per-bundle LM-loop latency."""
+    tcv_ba_solve_batch call per step. The value is the direct route (schurCache: each bundle's
+    reduced camera system factored, certified by the matrix-free residual); the PCG route is timed
+    beside it unless --no-pcg. Reports bundles/s over the batches' LM loops and the p50/p99 of the
+    per-bundle LM-loop latency."""
     import numpy as np
 
     from typecv_amd import bundle as B
@@ -205,32 +228,38 @@ def batched(args) -> int:
     M = sum(p.M for p in packeds)
     log(f"{len(packeds)} bundles, {sum(p.C for p in packeds)} cameras, "
         f"{sum(p.P for p in packeds)} points, {M} observations in {time.time() - t0:.1f}s")
-    controls = [B.make_control(p.C, p.P, p.M, shared_focal=True, huber=1.0,
-                               iterations=args.iterations, max_cg=min(256, max(32, 12 * p.C + 1)),
-                               precision=args.precision, schur_cache=args.schur_cache)
-                for p in packeds]
-    # the bundles' default budgets differ (12C+1); a batch shares one budget: the largest
-    budget = max(c.max_cg for c in controls)
-    for c in controls:
-        c.max_cg = budget
+
+    def controls_for(direct):
+        cs = [B.make_control(p.C, p.P, p.M, shared_focal=True, huber=1.0,
+                             iterations=args.iterations, max_cg=min(256, max(32, 12 * p.C + 1)),
+                             precision=args.precision, schur_cache=direct) for p in packeds]
+        # the bundles' default budgets differ (12C+1); a batch shares one budget: the largest
+        budget = max(c.max_cg for c in cs)
+        for c in cs:
+            c.max_cg = budget
+        return cs, budget
+
     device = B.HipDevice(0)
-    for k in range(args.warmup):
-        B.solve_batch(device, controls, packeds)
-    loops, walls, latencies, lm, cg, direct = [], [], [], 0, 0, 0
-    for k in range(args.steps):
-        t = time.perf_counter()
-        outs, report, finish = B.solve_batch(device, controls, packeds)
-        walls.append(time.perf_counter() - t)
-        loops.append(float(finish.max()) * 1e-3)
-        latencies.extend(finish.tolist())
-        lm += report["lm_iterations"]
-        cg += report["cg_iterations"]
-        direct += report["direct_solves"]
-        log(f"step {k}: wall {walls[-1] * 1e3:.1f} ms, LM loop {loops[-1] * 1e3:.1f} ms, "
-            f"{report['lm_iterations']} LM, {report['cg_iterations']} PCG, "
-            f"{report['direct_solves']} direct")
+    controls, budget = controls_for(True)
+    log("direct route")
+    loops, walls, latencies, totals = run_batch(B, device, controls, packeds, args.steps,
+                                                args.warmup)
+    pcg = None
+    if not args.no_pcg:
+        log("PCG route")
+        pcg_controls, _ = controls_for(False)
+        p_loops, p_walls, p_lat, p_totals = run_batch(B, device, pcg_controls, packeds,
+                                                      args.pcg_steps, args.warmup)
+        p_lat = np.array(p_lat)
+        pcg = {"bundles_per_s": len(packeds) * args.pcg_steps / sum(p_loops),
+               "ms_per_step": 1e3 * sum(p_loops) / args.pcg_steps,
+               "steps": args.pcg_steps,
+               "latency_ms": {"p50": float(np.percentile(p_lat, 50)),
+                              "p99": float(np.percentile(p_lat, 99))},
+               "pcg_iterations_per_step": p_totals["cg_iterations"] / args.pcg_steps}
     device.dispose()
     lat = np.array(latencies)
+    lm = totals["lm_iterations"]
     result = {
         "metric": "batched local bundles: bundles/s, p50/p99 per-bundle LM-loop latency",
         "value": len(packeds) * args.steps / sum(loops),
@@ -243,15 +272,16 @@ def batched(args) -> int:
         "config": {"workload": "batched local bundles", "bundles": len(packeds),
                    "observations": M, "lm_iterations_max": args.iterations,
                    "precision": args.precision, "max_cg": budget,
-                   "linear_solver": "direct (schurCache)" if args.schur_cache else "PCG"},
+                   "linear_solver": "direct (schurCache), certified by the matrix-free residual"},
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                        "max": float(lat.max())},
         "end_to_end_bundles_per_s": len(packeds) * args.steps / sum(walls),
         "lm_iterations_per_s": lm / sum(loops),
-        "pcg_iterations_per_step": cg / max(1, args.steps),
-        "direct_solves_per_step": direct / max(1, args.steps),
+        "pcg_iterations_per_step": totals["cg_iterations"] / max(1, args.steps),
+        "direct_solves_per_step": totals["direct_solves"] / max(1, args.steps),
         "observations_per_s": M * (lm / len(packeds)) / (sum(loops) / args.steps) / args.steps
         if args.steps else 0.0,
+        "pcg_route": pcg,
     }
     print(json.dumps(result), flush=True)
     return 0
@@ -488,7 +518,7 @@ if __name__ == "__main__":
         ap.add_argument("--iterations", type=int, default=30)
         ap.add_argument("--precision", choices=("fp64", "mixed"), default="mixed")
         ap.add_argument("--seed", type=int, default=4)
-        ap.add_argument("--schur-cache", action="store_true",
-                        help="the direct route: dense reduced camera system, Cholesky, certificate")
+        ap.add_argument("--no-pcg", action="store_true", help="skip timing the PCG route")
+        ap.add_argument("--pcg-steps", type=int, default=1)
         sys.exit(batched(ap.parse_args()))
     sys.exit(main())

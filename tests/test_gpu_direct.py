@@ -144,3 +144,37 @@ def test_direct_batch_equals_single_solves(device):
         assert np.array_equal(out, alone)
     assert report["direct_solves"] == singles
     assert singles > 0
+
+
+@pytest.mark.parametrize("case", ["joint_one_fixed", "joint_two_fixed", "directional"])
+def test_direct_polish_and_backward_match_pcg(device, case):
+    """The exact-Hessian solves of the Newton polish and of the adjoint backward through the direct
+    route (the gauge regularisation enters the dense system through S_cc and V^-1): the polished
+    solution and the pixel cotangent equal the PCG route's within 1e-8 relative."""
+    import test_gpu_adjoint as T
+    print(f"\n[direct] polish and backward ({case})", flush=True)
+    if case == "directional":
+        packed, control = T.directional_problem()
+    else:
+        packed, control = T.joint_problem(fixed_two=case == "joint_two_fixed")
+    at, lf = T.solved(device, packed, control)
+    direct = variant(control, schur_cache=True)
+    pcg_out, _ = B.polish(device, control, at, lf, iterations=4, tolerance=1e-13)
+    dir_out, prep = B.polish(device, direct, at, lf, iterations=4, tolerance=1e-13)
+    base = packed.C * 12 + packed.P * 3
+    scale = np.maximum(1.0, np.abs(pcg_out[:base]))
+    print(f"  polish: direct solves {prep['direct_solves']}, max coordinate difference "
+          f"{np.max(np.abs(dir_out[:base] - pcg_out[:base]) / scale):.2e}", flush=True)
+    assert prep["direct_solves"] > 0
+    assert np.max(np.abs(dir_out[:base] - pcg_out[:base]) / scale) <= 1e-8
+    pol = B.at_solution(at, pcg_out)
+    lf = B.output_log_focal(control, pcg_out)
+    sh = 1 if control.shared_focal else 0
+    cot = np.random.default_rng(5).normal(size=12 * packed.C + 3 * packed.P + sh)
+    want, _, _ = B.backward(device, control, pol, lf, cot, tolerance=1e-12)
+    got, _, brep = B.backward(device, direct, pol, lf, cot, tolerance=1e-12)
+    rel = np.max(np.abs(got - want)) / np.max(np.abs(want))
+    print(f"  backward: direct solves {brep['direct_solves']}, relative difference {rel:.2e}",
+          flush=True)
+    assert brep["direct_solves"] == 1
+    assert rel <= 1e-8

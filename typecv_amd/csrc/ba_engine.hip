@@ -352,11 +352,21 @@ struct Launch {
                        s, q);
     hipLaunchKernelGGL((k_dense_factor<SHARED>), dim3(q.n_direct), dim3(DENSE_THREADS), 0, s, q);
   }
+  // The certificate's product S x: the implicit FP64 operator, or the stored FP64 cross blocks of an
+  // exact-Hessian system.
   static void certify(const Problem& q, int final_pass, double tol, hipStream_t s) {
-    hipLaunchKernelGGL((k_point_sweep_implicit<SHARED>), dim3(tile_blocks(q)),
-                       dim3(TILE * TILE_WAVES), 0, s, q, (const double*)q.vec, (int)GUARD_DIRECT);
-    hipLaunchKernelGGL((k_camera_sweep_implicit<SHARED>), dim3(q.n_tasks), dim3(TASK_THREADS), 0,
-                       s, q, (int)GUARD_DIRECT);
+    if (q.exact) {
+      hipLaunchKernelGGL((k_point_sweep<WT, SHARED, SWEEP_CERTIFY>), dim3(tile_blocks(q)),
+                         dim3(TILE * TILE_WAVES), 0, s, q);
+      hipLaunchKernelGGL((k_camera_sweep<WT, SHARED, true>), dim3(q.n_tasks), dim3(TASK_THREADS),
+                         0, s, q);
+    } else {
+      hipLaunchKernelGGL((k_point_sweep_implicit<SHARED>), dim3(tile_blocks(q)),
+                         dim3(TILE * TILE_WAVES), 0, s, q, (const double*)q.vec,
+                         (int)GUARD_DIRECT);
+      hipLaunchKernelGGL((k_camera_sweep_implicit<SHARED>), dim3(q.n_tasks), dim3(TASK_THREADS),
+                         0, s, q, (int)GUARD_DIRECT);
+    }
     hipLaunchKernelGGL((k_certify<SHARED>), dim3(q.n_direct), dim3(CG_THREADS), 0, s, q,
                        final_pass, tol);
   }
@@ -664,6 +674,27 @@ int run_pcg(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, cons
   return TCV_BA_OK;
 }
 
+// The linear solve of a step, after init_cg. The direct route first: factor and solve, then
+// certify x with the matrix-free residual at `tol` (the form's final residual), correcting from the
+// factor in between; a solution that still misses it hands over to PCG, which also runs the
+// bundles that are not direct and those whose factorisation failed.
+int solve_linear(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, const Ops& ops,
+                 hipStream_t st, tcv_ba_report* rep, uint32_t first, uint32_t budget, double tol) {
+  if (!q.n_direct) return run_pcg(ctx, c, q, ops, st, rep, first, budget);
+  ops.dense_solve(q, st);
+  for (uint32_t k = 0; k < kDirectPasses; k++) {
+    ops.certify(q, k + 1 == kDirectPasses, tol, st);
+    HIP_TRY(hipGetLastError());
+    TRY(read_state(ctx, q, st, rep));
+    bool pending = false;
+    for (uint32_t b = 0; b < q.B; b++) pending |= word(ctx, b, S_DIRECT) == 1;
+    if (!pending) break;
+    ops.dense_correct(q, st);
+  }
+  if (!any_word(ctx, q, S_PCG_ACTIVE)) return TCV_BA_OK;
+  return run_pcg(ctx, c, q, ops, st, rep, first, budget);
+}
+
 // Ends the PCG of every bundle whose budget ran out (the reference stops at maxCgIterations);
 // its candidate is then rejected by the gauge's convergence check, as in the reference.
 __global__ void k_stop_pcg(Problem q) {
@@ -705,26 +736,8 @@ int run_lm(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, hipSt
     ops.camera_pass(q, st);
     ops.init_cg(q, st);
     HIP_TRY(hipGetLastError());
-    if (q.n_direct) {
-      // The direct route: factor and solve, then certify x with the matrix-free residual (the
-      // tolerance of the form's final residual), correcting from the factor in between; a
-      // solution that still misses it hands over to PCG, which also runs the bundles that are not
-      // direct and those whose factorisation failed.
-      const double tol = c->precision == TCV_BA_MIXED ? q.refine_tol : q.tol;
-      ops.dense_solve(q, st);
-      for (uint32_t k = 0; k < kDirectPasses; k++) {
-        ops.certify(q, k + 1 == kDirectPasses, tol, st);
-        HIP_TRY(hipGetLastError());
-        TRY(read_state(ctx, q, st, rep));
-        bool pending = false;
-        for (uint32_t b = 0; b < q.B; b++) pending |= word(ctx, b, S_DIRECT) == 1;
-        if (!pending) break;
-        ops.dense_correct(q, st);
-      }
-      if (any_word(ctx, q, S_PCG_ACTIVE)) TRY(run_pcg(ctx, c, q, ops, st, rep, hint, c->max_cg));
-    } else {
-      TRY(run_pcg(ctx, c, q, ops, st, rep, hint, c->max_cg));
-    }
+    TRY(solve_linear(ctx, c, q, ops, st, rep, hint, c->max_cg,
+                     c->precision == TCV_BA_MIXED ? q.refine_tol : q.tol));
     // The mixed form: replace the FP32-operator residual by the FP64 one and restart PCG from x
     // until the FP64 residual meets the refinement tolerance (each restart has its own budget).
     if (c->precision == TCV_BA_MIXED) {
@@ -853,6 +866,7 @@ int prepare_exact(tcv_ba_context* ctx, const tcv_ba_control* c, const double* sc
             rep));
   q.tol = tolerance;
   q.exact_gauge = 1;
+  q.exact = 1;
   ops = select_ops(&fc);
   hipLaunchKernelGGL(k_init_bundle, dim3(1), dim3(64), 0, st, q);
   const double lf = c->shared_focal ? log_focal : 0.0;
@@ -871,8 +885,8 @@ int prepare_exact(tcv_ba_context* ctx, const tcv_ba_control* c, const double* sc
 }
 
 // One undamped solve of the exact-Hessian system (the right-hand side is -gradient, or q.rhs_in):
-// assembly, PCG to q.tol within `max_cg` iterations. True when PCG converged (or the right-hand
-// side was zero).
+// assembly, then the direct route (TCV_BA_FLAG_SCHUR_CACHE) and/or PCG to q.tol within `max_cg`
+// iterations. True when the solve converged (or the right-hand side was zero).
 int exact_solve(tcv_ba_context* ctx, const tcv_ba_control* fc, const Problem& q, const Ops& ops,
                 hipStream_t st, uint32_t max_cg, tcv_ba_report* rep, bool& converged) {
   hipLaunchKernelGGL(k_newton_begin, dim3(1), dim3(64), 0, st, q);
@@ -885,7 +899,7 @@ int exact_solve(tcv_ba_context* ctx, const tcv_ba_control* fc, const Problem& q,
     return TCV_BA_OK;
   }
   if (word(ctx, 0, S_PCG_ACTIVE) != 0) {
-    TRY(run_pcg(ctx, fc, q, ops, st, rep, 32, max_cg));
+    TRY(solve_linear(ctx, fc, q, ops, st, rep, 32, max_cg, q.tol));
     if (any_word(ctx, q, S_PCG_ACTIVE))
       hipLaunchKernelGGL(k_stop_pcg, dim3(1), dim3(64), 0, st, q);
     TRY(read_state(ctx, q, st, rep));
@@ -1300,7 +1314,7 @@ int tcv_ba_polish(tcv_ba_context* ctx, const tcv_ba_control* c, const double* sc
     HIP_TRY(hipGetLastError());
     TRY(read_state(ctx, q1, st, rep));
     if (word(ctx, 0, S_PCG_ACTIVE) != 0) {
-      TRY(run_pcg(ctx, &fc, q1, ops, st, rep, 32, max_cg));
+      TRY(solve_linear(ctx, &fc, q1, ops, st, rep, 32, max_cg, q1.tol));
       if (any_word(ctx, q1, S_PCG_ACTIVE))
         hipLaunchKernelGGL(k_stop_pcg, dim3(1), dim3(64), 0, st, q1);
     }
@@ -1333,7 +1347,10 @@ int tcv_ba_polish(tcv_ba_context* ctx, const tcv_ba_control* c, const double* sc
     if (word(ctx, 0, S_COMMIT) == 0) break;
     taken_newton += 1;
   }
-  if (rep) rep->refinement_steps = taken_newton;
+  if (rep) {
+    rep->refinement_steps = taken_newton;
+    rep->direct_solves = (uint64_t)word(ctx, 0, S_DIRECT_SOLVES);
+  }
   // the final cost; the initial words keep the polish input's
   hipLaunchKernelGGL(k_cost<false>, dim3(q.n_cost_blocks), dim3(COST_OBS), 0, st, q);
   hipLaunchKernelGGL(k_finish_cost<false>, dim3(1), dim3(COST_OBS), 0, st, q);
@@ -1409,6 +1426,7 @@ int tcv_ba_backward(tcv_ba_context* ctx, const tcv_ba_control* c, const double* 
     rep->solve_ms = now_ms() - t0;
     rep->schur_ms = now_ms() - ts;
     rep->device_bytes = held_bytes(ctx);
+    rep->direct_solves = (uint64_t)word(ctx, 0, S_DIRECT_SOLVES);
   }
   return TCV_BA_OK;
 }

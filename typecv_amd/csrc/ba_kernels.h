@@ -132,6 +132,9 @@ struct Problem {
   const double* rhs_in;
   // exact-Hessian systems (polish, adjoint): regularise the one-fixed-camera similarity gauge
   uint32_t exact_gauge;
+  // the stored cross blocks hold the exact Hessian's (the implicit operator relinearises the
+  // Gauss-Newton blocks only, so it cannot certify such a system)
+  uint32_t exact;
   // the direct route (null / 0 when no bundle takes it)
   uint32_t n_direct, n_direct_cams;
   const uint32_t* direct_bundles;  // n_direct bundle ids

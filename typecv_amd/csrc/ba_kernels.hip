@@ -955,7 +955,8 @@ __global__ __launch_bounds__(CG_THREADS) void k_init_cg(Problem q) {
 //   SCHUR:  y_p = V_p^-1 sum_i (W_i^T v_c(i) + w_fi v_f)                (bundleSchurPointAt)
 //   UPDATE: X_p' = X_p + V_p^-1 (b_p - sum_i (W_i^T x_c(i) + w_fi x_f))  (updateBundleCandidate)
 //   SOLVE:  y_p = V_p^-1 (b_p - sum_i (W_i^T x_c(i) + w_fi x_f))  (the point part of a solve)
-enum SweepMode { SWEEP_SCHUR = 0, SWEEP_UPDATE = 1, SWEEP_SOLVE = 2 };
+//   CERTIFY: as SCHUR on x, for the bundles with an uncertified direct solution
+enum SweepMode { SWEEP_SCHUR = 0, SWEEP_UPDATE = 1, SWEEP_SOLVE = 2, SWEEP_CERTIFY = 3 };
 
 template <class WT, bool SHARED, int MODE>
 __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_sweep(Problem q) {
@@ -968,7 +969,9 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_sweep(Problem q) {
   const uint32_t fp = q.tiles[t], ep = q.tiles[t + 1], b = q.tile_bundle[t];
   const double* sb = bstate(q, b);
   if (MODE == SWEEP_SCHUR && sb[S_PCG_ACTIVE] == 0) return;
-  if (MODE != SWEEP_SCHUR && (sb[S_ACTIVE] == 0 || sb[S_PCG_FAILED] != 0)) return;
+  if (MODE == SWEEP_CERTIFY && sb[S_DIRECT] != 1) return;
+  if ((MODE == SWEEP_UPDATE || MODE == SWEEP_SOLVE) && (sb[S_ACTIVE] == 0 || sb[S_PCG_FAILED] != 0))
+    return;
   const double* v = MODE == SWEEP_SCHUR ? vp(q) : vx(q);
   auto& st = st_all[wave];
   const uint32_t* ps = point_start(q);
@@ -1010,7 +1013,7 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_sweep(Problem q) {
   }
   if (!has) return;
   const double* pb = q.pblk + (size_t)my * 12;
-  if (MODE == SWEEP_SCHUR) {
+  if (MODE == SWEEP_SCHUR || MODE == SWEEP_CERTIFY) {
     double* y = vy(q) + (size_t)my * 3;
     for (int j = 0; j < 3; j++) y[j] = pb[j * 3] * acc[0] + pb[j * 3 + 1] * acc[1] + pb[j * 3 + 2] * acc[2];
   } else if (MODE == SWEEP_UPDATE) {
@@ -1027,13 +1030,14 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_sweep(Problem q) {
 
 // The camera sweep: one workgroup per camera-list task streams the camera-major cross blocks and
 // gathers y of each slot's point: part[t] = sum_n W_n y_p(n) (6) and sum_n w_fn . y_p(n) (focal).
-template <class WT, bool SHARED>
+// CERTIFY: for the bundles with an uncertified direct solution instead of those running PCG.
+template <class WT, bool SHARED, bool CERTIFY = false>
 __global__ __launch_bounds__(TASK_THREADS) void k_camera_sweep(Problem q) {
   constexpr int NV = SHARED ? 7 : 6;
   __shared__ double red[TASK_THREADS / 64][NV];
   const uint32_t t = blockIdx.x;
   const uint32_t s = q.tasks[4 * t + 1], e = q.tasks[4 * t + 2], b = q.tasks[4 * t + 3];
-  if (bstate(q, b)[S_PCG_ACTIVE] == 0) return;
+  if (CERTIFY ? bstate(q, b)[S_DIRECT] != 1 : bstate(q, b)[S_PCG_ACTIVE] == 0) return;
   const double* y = vy(q);
   double acc[NV];
 #pragma unroll
