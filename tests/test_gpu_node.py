@@ -1,6 +1,7 @@
 """The Node-API addon end to end on the GPU: a packed solve through `typecv_amd/node/run.js`
 (Node, napi_async_work, tcv_ba_solve) returns exactly the array the Python binding returns for the
-same inputs, and its diagnostics."""
+same inputs, and its diagnostics; a batch through `solveBatch` (tcv_ba_solve_batch) returns each
+problem's array exactly as the Python batch does."""
 import json
 import shutil
 import subprocess
@@ -51,3 +52,26 @@ def test_node_solve_matches_python(device, tmp_path, shared):
     diag = json.loads(run.stdout.strip().splitlines()[-1])["diagnostics"]
     base = packed.C * 12 + packed.P * 3
     assert diag["finalCost"] == expected[base + 6] and diag["observations"] == packed.M
+
+
+def test_node_batch_matches_python(device, tmp_path):
+    node = shutil.which("node")
+    if node is None or not (NODE_DIR / "typecv_ba.node").exists():
+        pytest.skip("no Node or no built addon")
+    print("\n[node] batch", flush=True)
+    scenes = [S.trajectory_scene(cameras=c, points=p, seed=s, track=(3, min(c, 8)))[0]
+              for c, p, s in ((6, 600, 1), (9, 900, 2))]
+    controls = [B.make_control(x.C, x.P, x.M, iterations=6, max_cg=128, shared_focal=True,
+                               huber=1.0, schur_cache=True) for x in scenes]
+    args = [node, str(NODE_DIR / "run.js"), "--batch", str(tmp_path / "out")]
+    for b, (x, c) in enumerate(zip(scenes, controls)):
+        control_vector(c).tofile(tmp_path / f"c{b}.f64")
+        x.scene.astype(np.float64).tofile(tmp_path / f"s{b}.f64")
+        x.index.astype(np.uint32).tofile(tmp_path / f"i{b}.u32")
+        args += [str(tmp_path / f"c{b}.f64"), str(tmp_path / f"s{b}.f64"), str(tmp_path / f"i{b}.u32")]
+    run = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert run.returncode == 0, run.stderr
+    expected, _, _ = B.solve_batch(device, controls, scenes)
+    for b, want in enumerate(expected):
+        got = np.fromfile(tmp_path / f"out.{b}.f64", dtype=np.float64)
+        assert np.array_equal(got, want)

@@ -139,4 +139,4 @@ def test_node_addon_loads_and_exports_the_operator():
                          text=True, timeout=60)
     assert out.returncode == 0, out.stderr
     assert json.loads(out.stdout) == ["backward", "deviceBytes", "open", "outputSize", "release",
-                                      "solve"]
+                                      "solve", "solveBatch"]

@@ -52,31 +52,43 @@ function makeControl({ cameras, points, observations, directionalPoints = 0, anc
     stallTolerance, refinementTolerance]);
 }
 
-/* One packed solve: the packBundleResult array and its diagnostics, with checkBundleOutput's
- * errors (bundle-host.ts:278-283). */
+/* A packBundleResult array and its diagnostics, with checkBundleOutput's errors
+ * (bundle-host.ts:278-283). */
+function resultOf(control, values) {
+  const base = control[0] * 12 + control[1] * 3;
+  if (values[base + BUNDLE_STATE.inputError] === 1)
+    throw new RangeError('GPU BA gauge requires a nonzero baseline');
+  if (values[base + BUNDLE_STATE.inputError] === 2)
+    throw new RangeError('GPU BA observations must initially project in front of their cameras');
+  return {
+    values,
+    diagnostics: {
+      initialCost: values[base + BUNDLE_STATE.initialCost],
+      finalCost: values[base + BUNDLE_STATE.cost],
+      initialRmsePx: values[base + BUNDLE_STATE.initialRmsePx],
+      finalRmsePx: values[base + BUNDLE_STATE.finalRmsePx],
+      acceptedSteps: values[base + BUNDLE_STATE.accepted],
+      rejectedSteps: values[base + BUNDLE_STATE.rejected],
+      observations: control[3],
+      baseline: values[base + BUNDLE_STATE.baseline],
+    },
+    cgIterations: values[base + BUNDLE_STATE.cgSteps],
+    linearSolverFailures: values[base + BUNDLE_STATE.failedSolves],
+  };
+}
+
+/* One packed solve. */
 function solvePacked(device, control, scene, index) {
+  return device.enqueue(async () =>
+    resultOf(control, await addon.solve(device.native, control, scene, index)));
+}
+
+/* Many independent packed problems in one batched engine call (tcv_ba_solve_batch), as one device
+ * job: the results in order. The controls must share every option but sizes and gauge. */
+function solveBatchPacked(device, controls, scenes, indices) {
   return device.enqueue(async () => {
-    const values = await addon.solve(device.native, control, scene, index);
-    const base = control[0] * 12 + control[1] * 3;
-    if (values[base + BUNDLE_STATE.inputError] === 1)
-      throw new RangeError('GPU BA gauge requires a nonzero baseline');
-    if (values[base + BUNDLE_STATE.inputError] === 2)
-      throw new RangeError('GPU BA observations must initially project in front of their cameras');
-    return {
-      values,
-      diagnostics: {
-        initialCost: values[base + BUNDLE_STATE.initialCost],
-        finalCost: values[base + BUNDLE_STATE.cost],
-        initialRmsePx: values[base + BUNDLE_STATE.initialRmsePx],
-        finalRmsePx: values[base + BUNDLE_STATE.finalRmsePx],
-        acceptedSteps: values[base + BUNDLE_STATE.accepted],
-        rejectedSteps: values[base + BUNDLE_STATE.rejected],
-        observations: control[3],
-        baseline: values[base + BUNDLE_STATE.baseline],
-      },
-      cgIterations: values[base + BUNDLE_STATE.cgSteps],
-      linearSolverFailures: values[base + BUNDLE_STATE.failedSolves],
-    };
+    const values = await addon.solveBatch(device.native, controls, scenes, indices);
+    return values.map((v, b) => resultOf(controls[b], v));
   });
 }
 
@@ -87,4 +99,6 @@ function backwardPacked(device, control, scene, index, logFocal, cotangent,
     cotangent, maxCg, tolerance));
 }
 
-module.exports = { HipDevice, makeControl, solvePacked, backwardPacked, BUNDLE_STATE, addon };
+module.exports = {
+  HipDevice, makeControl, solvePacked, solveBatchPacked, backwardPacked, BUNDLE_STATE, addon,
+};

@@ -2,11 +2,13 @@
 /*
  * Command-line runner of the packed solve through the Node-API addon:
  *   node run.js <control.f64> <scene.f64> <index.u32> <out.f64>
- * reads the raw little-endian arrays, solves on device 0 and writes the packBundleResult array.
+ * reads the raw little-endian arrays, solves on device 0 and writes the packBundleResult array;
+ *   node run.js --batch <out prefix> (<control.f64> <scene.f64> <index.u32>)...
+ * solves the problems in one batched call and writes <out prefix>.<b>.f64 for each.
  * `node run.js --exports` prints the addon's exported names (no device is touched).
  */
 const fs = require('fs');
-const { HipDevice, solvePacked, addon } = require('./bundle.js');
+const { HipDevice, solvePacked, solveBatchPacked, addon } = require('./bundle.js');
 
 function read(file, Type) {
   const bytes = fs.readFileSync(file);
@@ -16,6 +18,24 @@ function read(file, Type) {
 async function main(argv) {
   if (argv[0] === '--exports') {
     console.log(JSON.stringify(Object.keys(addon).sort()));
+    return 0;
+  }
+  if (argv[0] === '--batch') {
+    const prefix = argv[1], files = argv.slice(2);
+    const controls = [], scenes = [], indices = [];
+    for (let k = 0; k + 2 < files.length; k += 3) {
+      controls.push(read(files[k], Float64Array));
+      scenes.push(read(files[k + 1], Float64Array));
+      indices.push(read(files[k + 2], Uint32Array));
+    }
+    const device = new HipDevice(0);
+    try {
+      const results = await solveBatchPacked(device, controls, scenes, indices);
+      results.forEach((r, b) => fs.writeFileSync(`${prefix}.${b}.f64`, Buffer.from(r.values.buffer)));
+      console.log(JSON.stringify(results.map((r) => r.diagnostics)));
+    } finally {
+      device.dispose();
+    }
     return 0;
   }
   const [controlFile, sceneFile, indexFile, outFile] = argv;

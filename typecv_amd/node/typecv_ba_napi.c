@@ -14,6 +14,8 @@
  *   solve(context, control, scene, index)          -> Promise<Float64Array>
  *   backward(context, control, scene, index, logFocal, cotangent, maxCg, tolerance)
  *                                                  -> Promise<Float64Array(2M)>
+ *   solveBatch(context, controls[], scenes[], indices[])
+ *                                                  -> Promise<Float64Array[]> (tcv_ba_solve_batch)
  * `control` is a Float64Array(16): the tcv_ba_control fields in declaration order (12 integers,
  * then huber, relative_tolerance, stall_tolerance, refinement_tolerance); `scene` a Float64Array
  * and `index` a Uint32Array in the packBundleScene layouts (bundle-host.ts:121-218).
@@ -293,6 +295,164 @@ static napi_value backward(napi_env env, napi_callback_info info) {
   return queue(env, j, "typecv_ba.backward");
 }
 
+/* ---------------------------------------------------------------- batches */
+
+typedef struct {
+  napi_async_work work;
+  napi_deferred deferred;
+  tcv_ba_context* ctx;
+  uint32_t count;
+  tcv_ba_control* controls;
+  const double** scenes;
+  const uint32_t** indices;
+  double** outs;
+  size_t* out_lens;
+  napi_ref* refs; /* context, then scene and index of each bundle */
+  uint32_t nrefs;
+  tcv_ba_report report;
+  int status;
+  char detail[256];
+} BatchJob;
+
+static void free_batch(napi_env env, BatchJob* j) {
+  for (uint32_t k = 0; k < j->nrefs; k++) napi_delete_reference(env, j->refs[k]);
+  if (j->outs)
+    for (uint32_t b = 0; b < j->count; b++) free(j->outs[b]);
+  free(j->controls);
+  free(j->scenes);
+  free(j->indices);
+  free(j->outs);
+  free(j->out_lens);
+  free(j->refs);
+  free(j);
+}
+
+static void execute_batch(napi_env env, void* data) {
+  (void)env;
+  BatchJob* j = data;
+  j->status = tcv_ba_solve_batch(j->ctx, j->count, j->controls, j->scenes, j->indices, j->outs,
+                                 &j->report, NULL);
+  if (j->status != TCV_BA_OK) snprintf(j->detail, sizeof j->detail, "%s", tcv_ba_last_error());
+}
+
+static void complete_batch(napi_env env, napi_status s, void* data) {
+  BatchJob* j = data;
+  if (s == napi_ok && j->status == TCV_BA_OK) {
+    napi_value list;
+    napi_create_array_with_length(env, j->count, &list);
+    for (uint32_t b = 0; b < j->count; b++) {
+      napi_value buffer, array;
+      void* bytes = NULL;
+      napi_create_arraybuffer(env, j->out_lens[b] * sizeof(double), &bytes, &buffer);
+      memcpy(bytes, j->outs[b], j->out_lens[b] * sizeof(double));
+      napi_create_typedarray(env, napi_float64_array, j->out_lens[b], buffer, 0, &array);
+      napi_set_element(env, list, b, array);
+    }
+    napi_resolve_deferred(env, j->deferred, list);
+  } else {
+    char text[512];
+    snprintf(text, sizeof text, "%s (%s)",
+             s == napi_ok ? tcv_ba_status_string(j->status) : "job cancelled", j->detail);
+    napi_value msg, err;
+    napi_create_string_utf8(env, text, NAPI_AUTO_LENGTH, &msg);
+    if (j->status == TCV_BA_ERR_ARGUMENT)
+      napi_create_range_error(env, NULL, msg, &err);
+    else
+      napi_create_error(env, NULL, msg, &err);
+    napi_reject_deferred(env, j->deferred, err);
+  }
+  napi_delete_async_work(env, j->work);
+  free_batch(env, j);
+}
+
+/* Element b of a JavaScript array, or NULL with a pending TypeError. */
+static napi_value element_of(napi_env env, napi_value list, uint32_t b) {
+  napi_value v = NULL;
+  if (napi_get_element(env, list, b, &v) != napi_ok) {
+    napi_throw_type_error(env, NULL, "expected arrays of typed arrays");
+    return NULL;
+  }
+  return v;
+}
+
+static napi_value solve_batch(napi_env env, napi_callback_info info) {
+  size_t argc = 4;
+  napi_value argv[4];
+  CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  if (argc < 4) {
+    napi_throw_type_error(env, NULL, "missing arguments");
+    return NULL;
+  }
+  tcv_ba_context* ctx = context_of(env, argv[0]);
+  if (!ctx) return NULL;
+  uint32_t count = 0, ns = 0, ni = 0;
+  bool arrays[3] = {false, false, false};
+  for (int k = 0; k < 3; k++) napi_is_array(env, argv[1 + k], &arrays[k]);
+  if (!arrays[0] || !arrays[1] || !arrays[2] ||
+      napi_get_array_length(env, argv[1], &count) != napi_ok ||
+      napi_get_array_length(env, argv[2], &ns) != napi_ok ||
+      napi_get_array_length(env, argv[3], &ni) != napi_ok || count < 1 || ns != count ||
+      ni != count) {
+    napi_throw_range_error(env, NULL, "controls, scenes and indices must be arrays of one length");
+    return NULL;
+  }
+  BatchJob* j = calloc(1, sizeof(BatchJob));
+  if (!j) {
+    napi_throw_error(env, NULL, "out of host memory");
+    return NULL;
+  }
+  j->ctx = ctx;
+  j->count = count;
+  j->controls = calloc(count, sizeof(tcv_ba_control));
+  j->scenes = calloc(count, sizeof(double*));
+  j->indices = calloc(count, sizeof(uint32_t*));
+  j->outs = calloc(count, sizeof(double*));
+  j->out_lens = calloc(count, sizeof(size_t));
+  j->refs = calloc(1 + 2 * (size_t)count, sizeof(napi_ref));
+  if (!j->controls || !j->scenes || !j->indices || !j->outs || !j->out_lens || !j->refs) {
+    free_batch(env, j);
+    napi_throw_error(env, NULL, "out of host memory");
+    return NULL;
+  }
+  napi_create_reference(env, argv[0], 1, &j->refs[j->nrefs++]);
+  for (uint32_t b = 0; b < count; b++) {
+    napi_value c = element_of(env, argv[1], b), sv = element_of(env, argv[2], b),
+               iv = element_of(env, argv[3], b);
+    size_t nsv = 0, niv = 0;
+    if (!c || !sv || !iv || !control_of(env, c, &j->controls[b]) ||
+        !(j->scenes[b] = typed_data(env, sv, napi_float64_array, &nsv)) ||
+        !(j->indices[b] = typed_data(env, iv, napi_uint32_array, &niv))) {
+      free_batch(env, j);
+      return NULL;
+    }
+    if (nsv != tcv_ba_scene_size(&j->controls[b]) || niv != tcv_ba_index_size(&j->controls[b])) {
+      free_batch(env, j);
+      napi_throw_range_error(env, NULL, "scene/index lengths do not match the control sizes");
+      return NULL;
+    }
+    napi_create_reference(env, sv, 1, &j->refs[j->nrefs++]);
+    napi_create_reference(env, iv, 1, &j->refs[j->nrefs++]);
+    j->out_lens[b] = tcv_ba_output_size(&j->controls[b]);
+    j->outs[b] = malloc(j->out_lens[b] * sizeof(double));
+    if (!j->outs[b]) {
+      free_batch(env, j);
+      napi_throw_error(env, NULL, "out of host memory");
+      return NULL;
+    }
+  }
+  napi_value promise, resource;
+  if (napi_create_promise(env, &j->deferred, &promise) != napi_ok ||
+      napi_create_string_utf8(env, "typecv_ba.solveBatch", NAPI_AUTO_LENGTH, &resource) != napi_ok ||
+      napi_create_async_work(env, NULL, resource, execute_batch, complete_batch, j, &j->work) !=
+          napi_ok ||
+      napi_queue_async_work(env, j->work) != napi_ok) {
+    free_batch(env, j);
+    napi_throw_error(env, NULL, "typecv_ba addon: could not queue the job");
+    return NULL;
+  }
+  return promise;
+}
+
 static napi_value init(napi_env env, napi_value exports) {
   const napi_property_descriptor props[] = {
       {"open", NULL, open_device, NULL, NULL, NULL, napi_enumerable, NULL},
@@ -301,6 +461,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"release", NULL, release, NULL, NULL, NULL, napi_enumerable, NULL},
       {"solve", NULL, solve, NULL, NULL, NULL, napi_enumerable, NULL},
       {"backward", NULL, backward, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"solveBatch", NULL, solve_batch, NULL, NULL, NULL, napi_enumerable, NULL},
   };
   napi_define_properties(env, exports, sizeof props / sizeof props[0], props);
   return exports;
