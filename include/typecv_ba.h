@@ -100,7 +100,7 @@ typedef struct tcv_ba_control {
 typedef struct tcv_ba_report {
   double solve_ms;          /* device-resident solve: setup + LM loop, inputs already in HBM */
   double setup_ms;          /* index checks, tile tables, camera-major permutation */
-  double schur_ms;          /* HIP-event time of the active Schur sweeps (TCV_BA_FLAG_TIMING) */
+  double schur_ms;          /* HIP-event time of the timed Schur sweeps (TCV_BA_FLAG_TIMING) */
   double schur_point_ms;    /* of which the point sweep */
   double schur_camera_ms;   /* of which the camera sweep */
   double refinement_steps;  /* outer refinement corrections taken (mixed form) */
@@ -113,6 +113,7 @@ typedef struct tcv_ba_report {
   uint64_t point_tiles;     /* wave tiles of whole points */
   uint64_t camera_tasks;    /* camera-list tasks */
   uint64_t direct_solves;   /* certified direct solves (TCV_BA_FLAG_SCHUR_CACHE) */
+  uint64_t timed_matvecs;   /* Schur products timed into schur_ms (every 8th, TCV_BA_FLAG_TIMING) */
 } tcv_ba_report;
 
 typedef struct tcv_ba_context tcv_ba_context;

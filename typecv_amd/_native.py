@@ -88,6 +88,7 @@ class Report(ctypes.Structure):
         ("point_tiles", ctypes.c_uint64),
         ("camera_tasks", ctypes.c_uint64),
         ("direct_solves", ctypes.c_uint64),
+        ("timed_matvecs", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:
