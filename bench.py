@@ -289,6 +289,53 @@ def batched(args) -> int:
     return 0
 
 
+def real_scene(args) -> int:
+    """The reference's real-data BA measurement (south-building-24: 24 views, 3,000 landmarks,
+    12,706 observations; 4 LM iterations, Huber 1 px; tests/golden/south_building_24.npz) through
+    the operator on the packed problem, fixed and shared focal, PCG and direct routes: wall time of
+    the device-resident LM loop (upload excluded) and the final RMSE against the reference's record
+    (docs/experiments/gpu-budget/south-building-24-ba-*.json, 6,741 / 8,938 ms on its WebGPU tier)."""
+    import numpy as np
+
+    sys.path.insert(0, str(Path(__file__).resolve().parent / "tests"))
+    import real_scene as R
+    from typecv_amd import bundle as B
+
+    model = R.reconstruction()
+    rows = []
+    with B.HipDevice(0) as device:
+        for focal in ("fixed", "shared"):
+            for cache in (False, True):
+                options = {**R.OPTIONS, "refineSharedFocal": focal == "shared",
+                           "schurCache": cache}
+                problem = B.bundle_problem(model, options, (lambda C: min(256, max(32, 12 * C + 1)))
+                                           if focal == "shared" else
+                                           (lambda C: min(96, max(24, 12 * C))))
+                packed = B.pack_bundle_scene(problem["input"], problem["byId"], problem["active"])
+                control = B.make_control(
+                    packed.C, packed.P, packed.M, directional=packed.P - problem["finitePoints"],
+                    anchor=problem["byId"][problem["anchorId"]],
+                    scale_view=problem["byId"][problem["scaleId"]],
+                    normalize_scale=problem["fixed"] == 1, shared_focal=focal == "shared",
+                    iterations=problem["iterations"], max_cg=problem["maxCg"],
+                    huber=problem["huber"], tolerance=problem["tolerance"], schur_cache=cache)
+                times, out, rep = [], None, None
+                for _ in range(args.warmup + args.steps):
+                    out, rep = B.solve_packed(device, control, packed)
+                    times.append(rep["solve_ms"] - rep["setup_ms"])
+                base = packed.C * 12 + packed.P * 3
+                rows.append({"focal": focal, "route": "direct" if cache else "PCG",
+                             "lm_loop_ms": float(np.median(times[args.warmup:])),
+                             "final_rmse_px": float(out[base + 25]),
+                             "record_final_rmse_px": R.RECORD[focal]["finalRmsePx"],
+                             "pcg_iterations": int(rep["cg_iterations"]),
+                             "direct_solves": int(rep["direct_solves"])})
+                log(str(rows[-1]))
+    print(json.dumps({"mode": "south-building-24", "cameras": 24, "points": 3000,
+                      "observations": 12706, "lm_iterations": 4, "runs": rows}), flush=True)
+    return 0
+
+
 def probe(args) -> int:
     """Counter run: a 1 GiB device copy (the calibration kernel of known size), then `--repeat`
     Schur products of the workload through tcv_ba_schur_probe, for `rocprofv3 --pmc FETCH_SIZE` /
@@ -510,6 +557,12 @@ if __name__ == "__main__":
         ap.add_argument("--repeat", type=int, default=10)
         ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
         sys.exit(probe(ap.parse_args()))
+    if "--workload" in sys.argv and "south-building-24" in sys.argv:
+        ap = argparse.ArgumentParser()
+        ap.add_argument("--workload")
+        ap.add_argument("--steps", type=int, default=5)
+        ap.add_argument("--warmup", type=int, default=1)
+        sys.exit(real_scene(ap.parse_args()))
     if "--workload" in sys.argv and "batched" in sys.argv:
         ap = argparse.ArgumentParser()
         ap.add_argument("--workload")
