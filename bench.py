@@ -269,7 +269,8 @@ def batched(args) -> int:
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": 1e3 * sum(loops) / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32-products/f64-accumulation" if args.precision == "mixed" else "f64",
+        # the direct route stores no FP32 cross blocks: it computes in FP64 in either form
+        "dtype": "f64",
         "data": "synthetic (seeded local bundles)",
         "config": {"workload": "batched local bundles", "bundles": len(packeds),
                    "observations": M, "lm_iterations_max": args.iterations,

@@ -146,15 +146,20 @@ struct Plan {
   uint64_t dense_words = 0;
 };
 
-// The direct route for the bundles of at most DENSE_MAX_CAMERAS cameras (all others stay on PCG).
-void plan_direct(Plan& plan, bool want, uint32_t shared) {
+// The direct route for the bundles of at most DENSE_MAX_CAMERAS cameras (all others stay on PCG),
+// except those marked in `excluded`. Outside the exact systems a direct bundle stores no cross
+// blocks.
+void plan_direct(Plan& plan, bool want, uint32_t shared, bool exact,
+                 const std::vector<char>& excluded = {}) {
   plan.direct_bundles.clear();
   plan.direct_cams.clear();
   plan.dense_words = 0;
   for (uint32_t b = 0; b < plan.bundles.size(); b++) {
     Bundle& g = plan.bundles[b];
     const uint32_t cams = g.cam1 - g.cam0;
-    g.direct = want && cams <= DENSE_MAX_CAMERAS ? 1u : 0u;
+    const bool out = b < excluded.size() && excluded[b];
+    g.direct = want && cams <= DENSE_MAX_CAMERAS && !out ? 1u : 0u;
+    g.unstored = g.direct && !exact ? 1u : 0u;
     // m_b unknowns padded to a multiple of DENSE_NB, plus the right-hand-side row
     g.dense_n = g.direct ? (6 * cams + shared + DENSE_NB - 1) / DENSE_NB * DENSE_NB + 1 : 0;
     g.dense_off = plan.dense_words;
@@ -343,6 +348,19 @@ struct Launch {
                        dim3(TILE * TILE_WAVES), 0, s, q, (const double*)(q.vec + 2 * (size_t)q.m),
                        (int)GUARD_PCG);
   }
+  static void unstored_p_sweep(const Problem& q, hipStream_t s) {
+    hipLaunchKernelGGL((k_point_sweep_implicit<SHARED>), dim3(tile_blocks(q)),
+                       dim3(TILE * TILE_WAVES), 0, s, q, (const double*)(q.vec + 2 * (size_t)q.m),
+                       (int)GUARD_PCG_UNSTORED);
+  }
+  static void unstored_camera_sweep(const Problem& q, hipStream_t s) {
+    hipLaunchKernelGGL((k_camera_sweep_implicit<SHARED>), dim3(q.n_tasks), dim3(TASK_THREADS), 0,
+                       s, q, (int)GUARD_PCG_UNSTORED);
+  }
+  static void implicit_update(const Problem& q, hipStream_t s) {
+    hipLaunchKernelGGL((k_point_sweep_implicit<SHARED>), dim3(tile_blocks(q)),
+                       dim3(TILE * TILE_WAVES), 0, s, q, (const double*)q.vec, (int)GUARD_UPDATE);
+  }
   static void implicit_camera_sweep(const Problem& q, hipStream_t s) {
     hipLaunchKernelGGL((k_camera_sweep_implicit<SHARED>), dim3(q.n_tasks), dim3(TASK_THREADS), 0,
                        s, q, (int)GUARD_PCG);
@@ -400,7 +418,26 @@ struct Ops {
   void (*dense_solve)(const Problem&, hipStream_t);
   void (*certify)(const Problem&, int, double, hipStream_t);
   void (*dense_correct)(const Problem&, hipStream_t);
+  // the bundles whose cross blocks are not stored: their PCG sweeps (null when point_sweep and
+  // camera_sweep are the implicit operator, which covers every bundle) and their update
+  void (*unstored_point_sweep)(const Problem&, hipStream_t);
+  void (*unstored_camera_sweep)(const Problem&, hipStream_t);
+  void (*unstored_update)(const Problem&, hipStream_t);
 };
+
+// The PCG sweeps of one iteration over the stored and the unstored bundles.
+void point_sweeps(const Problem& q, const Ops& ops, hipStream_t st) {
+  if (!ops.unstored_point_sweep || q.n_unstored < q.B) ops.point_sweep(q, st);
+  if (ops.unstored_point_sweep && q.n_unstored) ops.unstored_point_sweep(q, st);
+}
+void camera_sweeps(const Problem& q, const Ops& ops, hipStream_t st) {
+  if (!ops.unstored_camera_sweep || q.n_unstored < q.B) ops.camera_sweep(q, st);
+  if (ops.unstored_camera_sweep && q.n_unstored) ops.unstored_camera_sweep(q, st);
+}
+void update_points(const Problem& q, const Ops& ops, hipStream_t st) {
+  if (q.n_unstored < q.B) ops.update_points(q, st);
+  if (q.n_unstored) ops.unstored_update(q, st);
+}
 
 template <class WT, bool SHARED>
 Ops ops_for(bool implicit) {
@@ -408,10 +445,13 @@ Ops ops_for(bool implicit) {
   Ops o{L::point_pass,   L::camera_pass,   L::init_cg,         L::point_sweep,
         L::camera_sweep, L::advance,       L::update_points,   L::probe,
         L::refine,       L::exact_passes,  L::solve_points,    L::pixel_cotangent,
-        L::jacobian,     L::dense_solve,   L::certify,         L::dense_correct};
+        L::jacobian,     L::dense_solve,   L::certify,         L::dense_correct,
+        L::unstored_p_sweep, L::unstored_camera_sweep, L::implicit_update};
   if (implicit) {
     o.point_sweep = L::implicit_p_sweep;
     o.camera_sweep = L::implicit_camera_sweep;
+    o.unstored_point_sweep = nullptr;
+    o.unstored_camera_sweep = nullptr;
   }
   return o;
 }
@@ -432,10 +472,35 @@ int upload(DevBuf& buf, const std::vector<T>& v, hipStream_t st) {
   return TCV_BA_OK;
 }
 
+int read_state(tcv_ba_context* ctx, const Problem& q, hipStream_t st, tcv_ba_report* rep) {
+  HIP_TRY(hipMemcpyAsync(ctx->h_state, q.state, (size_t)q.B * S_WORDS * 8, hipMemcpyDeviceToHost,
+                         st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if (rep) rep->flag_reads++;
+  return TCV_BA_OK;
+}
+
+double word(const tcv_ba_context* ctx, uint32_t b, int w) {
+  return ctx->h_state[(size_t)b * S_WORDS + w];
+}
+
+bool any_word(const tcv_ba_context* ctx, const Problem& q, int w) {
+  for (uint32_t b = 0; b < q.B; b++)
+    if (word(ctx, b, w) != 0) return true;
+  return false;
+}
+
+double max_word(const tcv_ba_context* ctx, const Problem& q, int w) {
+  double v = 0;
+  for (uint32_t b = 0; b < q.B; b++) v = std::max(v, word(ctx, b, w));
+  return v;
+}
+
 // Allocate, validate the index and build the plan of a batch of B bundles whose packed arrays
 // `scene` / `index` (device pointers) hold the whole batch.
 int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const double* scene,
-          const uint32_t* index, hipStream_t st, Problem& q, tcv_ba_report* rep) {
+          const uint32_t* index, hipStream_t st, Problem& q, tcv_ba_report* rep,
+          bool exact = false) {
   const double t0 = now_ms();
   if (B < 1) return TCV_BA_ERR_ARGUMENT;
   uint64_t C64 = 0, P64 = 0, M64 = 0;
@@ -468,8 +533,10 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
   HIP_TRY(hipStreamSynchronize(st));
   Plan plan;
   TRY(build_plan(ps, cs, controls, B, C, P, M, plan));
-  plan_direct(plan, (c->flags & TCV_BA_FLAG_SCHUR_CACHE) != 0, shared);
+  const bool want_direct = (c->flags & TCV_BA_FLAG_SCHUR_CACHE) != 0;
+  plan_direct(plan, want_direct, shared, exact);
   const bool direct = !plan.direct_bundles.empty();
+  const uint32_t n_unstored = exact ? 0u : (uint32_t)plan.direct_bundles.size();
   const uint32_t n_tiles = (uint32_t)plan.tiles.size() - 1;
   const uint32_t n_tasks = (uint32_t)plan.tasks.size() / 4;
   const uint32_t n_cost = (uint32_t)plan.cost_blocks.size() / 3;
@@ -479,8 +546,10 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
   TRY(ensure(ctx->scene, scene_size * 8));
   TRY(ensure(ctx->trial, trial_size * 8));
   TRY(ensure(ctx->cam_point, (size_t)M * 4));
-  TRY(ensure(ctx->Wp, w_words * ws));
-  TRY(ensure(ctx->Wc, w_words * ws));
+  if (n_unstored < B) {
+    TRY(ensure(ctx->Wp, w_words * ws));
+    TRY(ensure(ctx->Wc, w_words * ws));
+  }
   TRY(ensure(ctx->pblk, (size_t)P * 12 * 8));
   TRY(ensure(ctx->tile_fail, (size_t)n_tiles * 4));
   TRY(ensure(ctx->tile_focal, (size_t)n_tiles * 16));
@@ -548,8 +617,10 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
   q.groups = (const uint32_t*)ctx->groups.ptr;
   q.group_part = (double*)ctx->group_part.ptr;
   q.cam_point = (uint32_t*)ctx->cam_point.ptr;
-  q.Wp = ctx->Wp.ptr;
-  q.Wc = ctx->Wc.ptr;
+  q.Wp = n_unstored < B ? ctx->Wp.ptr : nullptr;
+  q.Wc = n_unstored < B ? ctx->Wc.ptr : nullptr;
+  q.exact = exact ? 1u : 0u;
+  q.n_unstored = n_unstored;
   q.pblk = (double*)ctx->pblk.ptr;
   q.tile_fail = (uint32_t*)ctx->tile_fail.ptr;
   q.tile_focal = (double*)ctx->tile_focal.ptr;
@@ -584,12 +655,24 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
     return TCV_BA_ERR_INDEX;
   }
   if (direct && (*ctx->h_error & 4u) != 0) {
-    // a camera observes one point twice: its point list is not a set, so the batch keeps PCG
-    plan_direct(plan, false, shared);
+    // a camera observes one point twice: that bundle's camera lists are not sets of points, so it
+    // keeps PCG (word 40 marks it); the others keep the direct route
+    TRY(read_state(ctx, q, st, nullptr));
+    std::vector<char> excluded(B);
+    for (uint32_t b = 0; b < B; b++) excluded[b] = word(ctx, b, S_DUPLICATE) != 0;
+    plan_direct(plan, true, shared, exact, excluded);
     TRY(upload(ctx->bundles, plan.bundles, st));
-    q.n_direct = q.n_direct_cams = 0;
-    q.direct_bundles = q.direct_cams = nullptr;
-    q.camS = q.camG = q.pfc = q.dense = nullptr;
+    TRY(upload(ctx->direct_bundles, plan.direct_bundles, st));
+    TRY(upload(ctx->direct_cams, plan.direct_cams, st));
+    q.n_direct = (uint32_t)plan.direct_bundles.size();
+    q.n_direct_cams = (uint32_t)plan.direct_cams.size();
+    q.n_unstored = exact ? 0u : q.n_direct;
+    if (q.n_unstored < B) {
+      TRY(ensure(ctx->Wp, w_words * ws));
+      TRY(ensure(ctx->Wc, w_words * ws));
+      q.Wp = ctx->Wp.ptr;
+      q.Wc = ctx->Wc.ptr;
+    }
   }
   if (rep) {
     rep->setup_ms = now_ms() - t0;
@@ -597,30 +680,6 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
     rep->camera_tasks = n_tasks;
   }
   return TCV_BA_OK;
-}
-
-int read_state(tcv_ba_context* ctx, const Problem& q, hipStream_t st, tcv_ba_report* rep) {
-  HIP_TRY(hipMemcpyAsync(ctx->h_state, q.state, (size_t)q.B * S_WORDS * 8, hipMemcpyDeviceToHost,
-                         st));
-  HIP_TRY(hipStreamSynchronize(st));
-  if (rep) rep->flag_reads++;
-  return TCV_BA_OK;
-}
-
-double word(const tcv_ba_context* ctx, uint32_t b, int w) {
-  return ctx->h_state[(size_t)b * S_WORDS + w];
-}
-
-bool any_word(const tcv_ba_context* ctx, const Problem& q, int w) {
-  for (uint32_t b = 0; b < q.B; b++)
-    if (word(ctx, b, w) != 0) return true;
-  return false;
-}
-
-double max_word(const tcv_ba_context* ctx, const Problem& q, int w) {
-  double v = 0;
-  for (uint32_t b = 0; b < q.B; b++) v = std::max(v, word(ctx, b, w));
-  return v;
 }
 
 int ensure_events(tcv_ba_context* ctx, size_t n) {
@@ -648,9 +707,9 @@ int run_pcg(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, cons
     for (uint32_t k = 0; k < n; k++) {
       const bool timed = timing && k % kTimingStride == 0;
       if (timed) HIP_TRY(hipEventRecord(ctx->events[3 * k], st));
-      ops.point_sweep(q, st);
+      point_sweeps(q, ops, st);
       if (timed) HIP_TRY(hipEventRecord(ctx->events[3 * k + 1], st));
-      ops.camera_sweep(q, st);
+      camera_sweeps(q, ops, st);
       if (timed) HIP_TRY(hipEventRecord(ctx->events[3 * k + 2], st));
       ops.advance(q, st);
     }
@@ -728,6 +787,7 @@ int run_lm(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, hipSt
     if (!active[b]) finish[b] = now_ms() - t0;
   }
   uint32_t hint = 16;
+  std::vector<char> unstored;  // the mixed form's refinement skips these (their PCG is FP64)
   const uint32_t update_blocks = blocks_for(q.C > q.P ? q.C : q.P, 256);
   const uint32_t trial_blocks = blocks_for((size_t)q.C * 12 + (size_t)q.P * 3, 256);
   if (timing) TRY(ensure_events(ctx, 3 * 256 + 3));
@@ -746,13 +806,20 @@ int run_lm(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, hipSt
     // The mixed form: replace the FP32-operator residual by the FP64 one and restart PCG from x
     // until the FP64 residual meets the refinement tolerance (each restart has its own budget).
     if (c->precision == TCV_BA_MIXED) {
+      if (unstored.empty()) {
+        unstored.resize(q.B);
+        std::vector<Bundle> bundles(q.B);
+        HIP_TRY(hipMemcpy(bundles.data(), q.bundles, q.B * sizeof(Bundle), hipMemcpyDeviceToHost));
+        for (uint32_t b = 0; b < q.B; b++) unstored[b] = bundles[b].unstored != 0;
+      }
       if (any_word(ctx, q, S_PCG_ACTIVE))
         hipLaunchKernelGGL(k_stop_pcg, dim3(blocks_for(q.B, 64)), dim3(64), 0, st, q);
       for (uint32_t k = 0; k < kMaxRefinements; k++) {
         bool pending = false;
         for (uint32_t b = 0; b < q.B; b++)
           pending |= word(ctx, b, S_CONVERGED) != 0 && word(ctx, b, S_PCG_FAILED) == 0 &&
-                     word(ctx, b, S_ACTIVE) != 0 && word(ctx, b, S_DIRECT) != 2;
+                     word(ctx, b, S_ACTIVE) != 0 && word(ctx, b, S_DIRECT) != 2 &&
+                     !unstored[b];
         if (!pending) break;
         ops.refine(q, st);
         HIP_TRY(hipGetLastError());
@@ -771,7 +838,7 @@ int run_lm(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, hipSt
     const uint32_t steps = (uint32_t)max_word(ctx, q, S_STEPS);
     hint = steps * 3 / 4 < kChunk ? kChunk : steps * 3 / 4;
     hipLaunchKernelGGL(k_update_cameras, dim3(blocks_for(q.C, 64)), dim3(64), 0, st, q);
-    ops.update_points(q, st);
+    update_points(q, ops, st);
     hipLaunchKernelGGL(k_gauge, dim3(q.B), dim3(256), 0, st, q);
     hipLaunchKernelGGL(k_normalize, dim3(update_blocks), dim3(256), 0, st, q);
     hipLaunchKernelGGL(k_cost<true>, dim3(q.n_cost_blocks), dim3(COST_OBS), 0, st, q);
@@ -868,7 +935,7 @@ int prepare_exact(tcv_ba_context* ctx, const tcv_ba_control* c, const double* sc
   HIP_TRY(hipMemcpyAsync(ctx->probe.ptr, scene, ss * 8, hipMemcpyHostToDevice, st));
   HIP_TRY(hipMemcpyAsync(ctx->index.ptr, index, is * 4, hipMemcpyHostToDevice, st));
   TRY(setup(ctx, &fc, 1, (const double*)ctx->probe.ptr, (const uint32_t*)ctx->index.ptr, st, q,
-            rep));
+            rep, true));
   q.tol = tolerance;
   q.exact_gauge = 1;
   q.exact = 1;
@@ -1248,9 +1315,9 @@ int tcv_ba_schur_probe(tcv_ba_context* ctx, const tcv_ba_control* c, const doubl
   TRY(ensure_events(ctx, 3));
   for (uint32_t r = 0; r < repeat; r++) {
     if (timing) HIP_TRY(hipEventRecord(ctx->events[0], st));
-    ops.point_sweep(q, st);
+    point_sweeps(q, ops, st);
     if (timing) HIP_TRY(hipEventRecord(ctx->events[1], st));
-    ops.camera_sweep(q, st);
+    camera_sweeps(q, ops, st);
     if (timing) HIP_TRY(hipEventRecord(ctx->events[2], st));
     if (timing) {
       HIP_TRY(hipEventSynchronize(ctx->events[2]));
@@ -1304,7 +1371,7 @@ int tcv_ba_polish(tcv_ba_context* ctx, const tcv_ba_control* c, const double* sc
   const uint32_t trial_blocks = blocks_for((size_t)q.C * 12 + (size_t)q.P * 3, 256);
   auto candidate = [&]() {
     hipLaunchKernelGGL(k_update_cameras, dim3(blocks_for(q.C, 64)), dim3(64), 0, st, q);
-    ops.update_points(q, st);
+    update_points(q, ops, st);
     hipLaunchKernelGGL(k_gauge, dim3(1), dim3(256), 0, st, q);
     hipLaunchKernelGGL(k_normalize, dim3(update_blocks), dim3(256), 0, st, q);
   };
@@ -1325,7 +1392,7 @@ int tcv_ba_polish(tcv_ba_context* ctx, const tcv_ba_control* c, const double* sc
         hipLaunchKernelGGL(k_stop_pcg, dim3(1), dim3(64), 0, st, q1);
     }
     hipLaunchKernelGGL(k_update_cameras, dim3(blocks_for(q.C, 64)), dim3(64), 0, st, q1);
-    ops.update_points(q1, st);
+    update_points(q1, ops, st);
     hipLaunchKernelGGL(k_gauge, dim3(1), dim3(256), 0, st, q1);
     hipLaunchKernelGGL(k_normalize, dim3(update_blocks), dim3(256), 0, st, q1);
     hipLaunchKernelGGL(k_cost<true>, dim3(q.n_cost_blocks), dim3(COST_OBS), 0, st, q1);

@@ -51,7 +51,8 @@ enum : int {
   S_DIRECT = 38,         // direct route this LM iteration: 0 none, 1 solved and not yet
                          // certified, 2 certified by the matrix-free residual
   S_DIRECT_SOLVES = 39,  // certified direct solves (whole solve)
-  S_WORDS = 40
+  S_DUPLICATE = 40,      // set at set-up when a camera of the bundle observes one point twice
+  S_WORDS = 48
 };
 
 constexpr int TILE = 64;  // observations per wave tile
@@ -87,7 +88,8 @@ struct Bundle {
   uint32_t group0, group1;  // the bundle's PCG-step groups
   uint32_t direct;          // 1: the reduced camera system is assembled and factored
   uint32_t dense_n;         // rows of the bundle's dense array: m_b = 6 C_b [+ 1] unknowns + 1
-  uint32_t pad;
+  uint32_t unstored;        // 1: the passes store no cross blocks of this bundle (a direct bundle
+                            // outside the exact systems); its sweeps and update relinearise
   uint64_t dense_off;       // first double of the bundle's dense array
 };
 
@@ -135,6 +137,7 @@ struct Problem {
   // the stored cross blocks hold the exact Hessian's (the implicit operator relinearises the
   // Gauss-Newton blocks only, so it cannot certify such a system)
   uint32_t exact;
+  uint32_t n_unstored;  // bundles whose cross blocks are not stored (Bundle::unstored)
   // the direct route (null / 0 when no bundle takes it)
   uint32_t n_direct, n_direct_cams;
   const uint32_t* direct_bundles;  // n_direct bundle ids

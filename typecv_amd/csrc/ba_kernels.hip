@@ -33,6 +33,8 @@ __device__ __forceinline__ double* bstate(const Problem& q, uint32_t b) {
 __device__ __forceinline__ bool variable_camera(const Problem& q, uint32_t c) {
   return camera_active(q)[c] != 0;
 }
+// The passes store the bundle's cross blocks, and the stored-block sweeps cover it.
+__device__ __forceinline__ bool stores_w(const Bundle& g) { return g.unstored == 0; }
 __device__ __forceinline__ bool directional_point(const Bundle& g, uint32_t p) {
   return p >= g.first_directional;
 }
@@ -293,8 +295,9 @@ __global__ void k_check_points(Problem q) {
 }
 
 // Every slot of a camera task names an observation of that camera, ascending within the camera's
-// list; writes the camera-major point ids. Flag 4 (not an error) marks a camera that observes one
-// point twice: the direct route's point lookup needs each camera's points strictly ascending.
+// list; writes the camera-major point ids. Flag 4 (not an error) and the bundle's word 40 mark a
+// camera that observes one point twice: the direct route's point lookup needs each camera's points
+// strictly ascending, so that bundle keeps PCG.
 __global__ void k_check_tasks(Problem q) {
   const uint32_t t = blockIdx.x;
   const uint32_t c = q.tasks[4 * t], s = q.tasks[4 * t + 1], e = q.tasks[4 * t + 2];
@@ -306,7 +309,10 @@ __global__ void k_check_tasks(Problem q) {
     if (!bad && n > list) {
       const uint32_t before = camera_obs(q)[n - 1];
       bad = before >= i;
-      if (!bad && obs_point(q)[before] == obs_point(q)[i]) atomicOr(q.error, 4u);
+      if (!bad && obs_point(q)[before] == obs_point(q)[i]) {
+        atomicOr(q.error, 4u);
+        bstate(q, q.tasks[4 * t + 3])[S_DUPLICATE] = 1;
+      }
     }
     if (bad) {
       atomicOr(q.error, 2u);
@@ -513,7 +519,8 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_pass(Problem q) {
       } else {
         for (int k = 0; k < 21; k++) W[k] = 0;
       }
-      for (uint32_t k = 0; k < K; k++) *wptr<WT, SHARED>(q.Wp, k, i, q.M) = (WT)W[k];
+      if (stores_w(g))
+        for (uint32_t k = 0; k < K; k++) *wptr<WT, SHARED>(q.Wp, k, i, q.M) = (WT)W[k];
       double* o = st[lane];
       if (ok) {
         const double w = L.w;
@@ -673,7 +680,8 @@ __global__ __launch_bounds__(TASK_THREADS) void k_camera_pass(Problem q) {
     } else {
       for (int k = 0; k < 21; k++) W[k] = 0;
     }
-    for (uint32_t k = 0; k < K; k++) *wptr<WT, SHARED>(q.Wc, k, n, q.M) = (WT)W[k];
+    if (stores_w(g))
+      for (uint32_t k = 0; k < K; k++) *wptr<WT, SHARED>(q.Wc, k, n, q.M) = (WT)W[k];
     if (!ok) continue;
     const double w = L.w;
     // with a right-hand side given, the camera's own b_c enters once, in k_camera_finish
@@ -972,6 +980,7 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_sweep(Problem q) {
   if (MODE == SWEEP_CERTIFY && sb[S_DIRECT] != 1) return;
   if ((MODE == SWEEP_UPDATE || MODE == SWEEP_SOLVE) && (sb[S_ACTIVE] == 0 || sb[S_PCG_FAILED] != 0))
     return;
+  if (!stores_w(q.bundles[b])) return;
   const double* v = MODE == SWEEP_SCHUR ? vp(q) : vx(q);
   auto& st = st_all[wave];
   const uint32_t* ps = point_start(q);
@@ -1038,6 +1047,7 @@ __global__ __launch_bounds__(TASK_THREADS) void k_camera_sweep(Problem q) {
   const uint32_t t = blockIdx.x;
   const uint32_t s = q.tasks[4 * t + 1], e = q.tasks[4 * t + 2], b = q.tasks[4 * t + 3];
   if (CERTIFY ? bstate(q, b)[S_DIRECT] != 1 : bstate(q, b)[S_PCG_ACTIVE] == 0) return;
+  if (!stores_w(q.bundles[b])) return;
   const double* y = vy(q);
   double acc[NV];
 #pragma unroll
@@ -1308,11 +1318,18 @@ __global__ __launch_bounds__(CG_THREADS) void k_probe(Problem q, const double* v
 // ------------------------------------------------------------------ implicit FP64 operator
 
 // Which bundles a sweep of the implicit operator covers.
-enum SweepGuard { GUARD_NONE = 0, GUARD_PCG = 1, GUARD_DIRECT = 2 };
+// GUARD_PCG_UNSTORED: the PCG sweeps of the bundles whose cross blocks are not stored;
+// GUARD_UPDATE (point sweep only): the candidate's points of those bundles, as
+// k_point_sweep<SWEEP_UPDATE> does for the others.
+enum SweepGuard {
+  GUARD_NONE = 0, GUARD_PCG = 1, GUARD_DIRECT = 2, GUARD_PCG_UNSTORED = 3, GUARD_UPDATE = 4
+};
 
-__device__ __forceinline__ bool skip_bundle(const double* sb, int guard) {
+__device__ __forceinline__ bool skip_bundle(const Bundle& g, const double* sb, int guard) {
   if (guard == GUARD_PCG) return sb[S_PCG_ACTIVE] == 0;
   if (guard == GUARD_DIRECT) return sb[S_DIRECT] != 1;
+  if (guard == GUARD_PCG_UNSTORED) return sb[S_PCG_ACTIVE] == 0 || stores_w(g);
+  if (guard == GUARD_UPDATE) return sb[S_ACTIVE] == 0 || sb[S_PCG_FAILED] != 0 || stores_w(g);
   return false;
 }
 
@@ -1333,8 +1350,8 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_sweep_implicit(Probl
   if (t >= q.n_tiles) return;
   const uint32_t b = q.tile_bundle[t];
   const double* sb = bstate(q, b);
-  if (skip_bundle(sb, guard)) return;
   const Bundle& g = q.bundles[b];
+  if (skip_bundle(g, sb, guard)) return;
   auto& st = st_all[wave];
   const uint32_t fp = q.tiles[t], ep = q.tiles[t + 1];
   const uint32_t* ps = point_start(q);
@@ -1380,6 +1397,13 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_sweep_implicit(Probl
   }
   if (!has) return;
   const double* pb = q.pblk + (size_t)my * 12;
+  if (guard == GUARD_UPDATE) {
+    const double r0 = pb[9] - acc[0], r1 = pb[10] - acc[1], r2 = pb[11] - acc[2];
+    const size_t at = (size_t)q.C * 12 + (size_t)my * 3;
+    for (int j = 0; j < 3; j++)
+      q.trial[at + j] = q.scene[at + j] + (pb[j * 3] * r0 + pb[j * 3 + 1] * r1 + pb[j * 3 + 2] * r2);
+    return;
+  }
   double* y = vy(q) + (size_t)my * 3;
   for (int j = 0; j < 3; j++) y[j] = pb[j * 3] * acc[0] + pb[j * 3 + 1] * acc[1] + pb[j * 3 + 2] * acc[2];
 }
@@ -1394,8 +1418,8 @@ __global__ __launch_bounds__(TASK_THREADS) void k_camera_sweep_implicit(Problem 
   const uint32_t c = q.tasks[4 * t], s = q.tasks[4 * t + 1], e = q.tasks[4 * t + 2];
   const uint32_t b = q.camera_bundle[c];
   const double* sb = bstate(q, b);
-  if (skip_bundle(sb, guard)) return;
   const Bundle& g = q.bundles[b];
+  if (skip_bundle(g, sb, guard)) return;
   const bool var = variable_camera(q, c);
   const double fscale = SHARED ? focal_scale(sb[S_LOG_FOCAL]) : 1.0;
   const double* y = vy(q);
@@ -1436,7 +1460,9 @@ __global__ __launch_bounds__(CG_THREADS) void k_refine(Problem q) {
   double* s = bstate(q, bi);
   const double converged = s[S_CONVERGED], failed = s[S_PCG_FAILED], r0 = s[S_R0];
   const double active = s[S_ACTIVE];
-  if (active == 0 || converged == 0 || failed != 0 || s[S_DIRECT] == 2) return;
+  // a certified direct solution, and a bundle whose PCG ran on the FP64 implicit operator (no stored
+  // cross blocks), have an FP64 residual already
+  if (active == 0 || converged == 0 || failed != 0 || s[S_DIRECT] == 2 || !stores_w(g)) return;
   const uint32_t tid = threadIdx.x;
   double* x = vx(q);
   double* r = vr(q);
