@@ -64,6 +64,8 @@ def main() -> int:
                     help="relinearise cross blocks in every Schur sweep instead of streaming them")
     ap.add_argument("--refine", type=float, default=0.0,
                     help="mixed form: FP64 residual target of the refinement (0: the PCG tolerance)")
+    ap.add_argument("--schur-cache", action="store_true",
+                    help="the schurCache option: a dense Cholesky solve per LM iteration")
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-lm", type=int, default=2, help="LM iterations of the CPU sample (0: skip)")
     args = ap.parse_args()
@@ -87,7 +89,7 @@ def main() -> int:
     control = B.make_control(C, P, M, shared_focal=True, huber=1.0, iterations=args.iterations,
                              max_cg=min(256, max(32, 12 * C + 1)), precision=args.precision,
                              refinement_tolerance=args.refine, timing=True,
-                             implicit=args.implicit)
+                             implicit=args.implicit, schur_cache=args.schur_cache)
     lib = _native.library()
     device = B.HipDevice(0)
     scene = torch.from_numpy(packed.scene).cuda()
@@ -146,7 +148,9 @@ def main() -> int:
             "max_cg": control.max_cg, "huber_px": 1.0, "precision": args.precision,
             "outliers": info["outliers"], "focal_factor": info["focal_factor"],
             "operator": "implicit" if args.implicit else "stored",
+            "linear_solver": "direct (schurCache)" if args.schur_cache else "matrix-free PCG",
         },
+        "direct_solves_per_step": sum(r["direct_solves"] for r in reports) / args.steps,
         "refinements_per_step": sum(r["refinement_steps"] for r in reports) / args.steps,
         "observations_per_s": M * lm / elapsed,
         "lm_iterations_per_step": lm / args.steps,

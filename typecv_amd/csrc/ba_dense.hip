@@ -101,10 +101,38 @@ __device__ __forceinline__ uint32_t xcd_contiguous(uint32_t i, uint32_t n) {
 #endif
 }
 
+// Whether cameras lc and ld of bundle g (local indices) observe a common point: bit ld of row lc of
+// the bundle's covisibility bitmap (k_covisibility).
+__device__ __forceinline__ bool covisible(const Problem& q, const Bundle& g, uint32_t lc,
+                                          uint32_t ld) {
+  const uint32_t words = (g.cam1 - g.cam0 + 31) / 32;
+  return (q.covis[g.covis_off + (size_t)lc * words + ld / 32] >> (ld % 32)) & 1u;
+}
+
+// One thread per point of a direct bundle: mark every pair of its observing cameras in the bundle's
+// covisibility bitmap (integer OR: the result does not depend on the order). Once per solve.
+__global__ void k_covisibility(Problem q) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= q.P) return;
+  const Bundle& g = q.bundles[q.point_bundle[p]];
+  if (!g.direct) return;
+  const uint32_t words = (g.cam1 - g.cam0 + 31) / 32;
+  const uint32_t* ps = point_start(q);
+  for (uint32_t i = ps[p]; i < ps[p + 1]; i++) {
+    const uint32_t a = obs_camera(q)[i] - g.cam0;
+    for (uint32_t j = ps[p]; j < ps[p + 1]; j++) {
+      const uint32_t d = obs_camera(q)[j] - g.cam0;
+      if (d < a) atomicOr(&q.covis[g.covis_off + (size_t)a * words + d / 32], 1u << (d % 32));
+    }
+  }
+}
+
 template <bool SHARED>
 __global__ __launch_bounds__(DENSE_THREADS) void k_dense_rows(Problem q) {
+  // camera c's slots of one chunk: a table indexed by point id when their points span at most
+  // DENSE_TABLE ids, else their (ascending) point ids, searched by bisection
   __shared__ uint32_t table[DENSE_TABLE];
-  __shared__ uint32_t span[2 * DENSE_MAX_CAMERAS];
+  __shared__ uint32_t span[2 * DENSE_LARGE_CAMERAS];
   __shared__ double red[DENSE_THREADS / 64][36];
   const uint32_t c = q.direct_cams[xcd_contiguous(blockIdx.x, gridDim.x)];
   const uint32_t b = q.camera_bundle[c];
@@ -147,35 +175,58 @@ __global__ __launch_bounds__(DENSE_THREADS) void k_dense_rows(Problem q) {
     }
   }
   if (!var || lc == 0) return;
-  for (uint32_t base = g.pt0; base < g.pt1; base += DENSE_TABLE) {
-    const uint32_t top = base + DENSE_TABLE < g.pt1 ? base + DENSE_TABLE : g.pt1;
-    __syncthreads();  // the previous pass's reads of the table and spans
-    // each list's slots whose points lie in [base, top)
-    for (uint32_t ld = tid; ld <= lc; ld += DENSE_THREADS) {
+  const uint32_t cend = cs[c + 1];
+  for (uint32_t cc = cs[c]; cc < cend;) {
+    // the chunk [cc, ce) of camera c's list and the point ids [p_lo, p_hi) it spans
+    const uint32_t p_lo = cp[cc];
+    uint32_t ce = first_slot(cp, cc, cend, p_lo + DENSE_TABLE);
+    const bool tabled = ce - cc >= DENSE_CHUNK_MIN || ce == cend;
+    if (!tabled) ce = cend - cc > DENSE_TABLE ? cc + DENSE_TABLE : cend;
+    const uint32_t p_hi = cp[ce - 1] + 1;
+    __syncthreads();  // the previous chunk's reads of the table and spans
+    // each covisible list's slots whose points lie in [p_lo, p_hi)
+    for (uint32_t ld = tid; ld < lc; ld += DENSE_THREADS) {
       const uint32_t d = g.cam0 + ld;
-      uint32_t lo = cs[d], hi = cs[d + 1];
-      if (base > g.pt0) lo = first_slot(cp, lo, hi, base);
-      if (top < g.pt1) hi = first_slot(cp, lo, hi, top);
+      uint32_t lo = 0, hi = 0;
+      if (covisible(q, g, lc, ld) && variable_camera(q, d)) {
+        lo = first_slot(cp, cs[d], cs[d + 1], p_lo);
+        hi = first_slot(cp, lo, cs[d + 1], p_hi);
+      }
       span[2 * ld] = lo;
       span[2 * ld + 1] = hi;
     }
-    for (uint32_t k = tid; k < top - base; k += DENSE_THREADS) table[k] = DENSE_NONE;
+    if (tabled) {
+      for (uint32_t k = tid; k < p_hi - p_lo; k += DENSE_THREADS) table[k] = DENSE_NONE;
+      __syncthreads();
+      for (uint32_t s = cc + tid; s < ce; s += DENSE_THREADS) table[cp[s] - p_lo] = s;
+    } else {
+      for (uint32_t s = cc + tid; s < ce; s += DENSE_THREADS) table[s - cc] = cp[s];
+    }
     __syncthreads();
-    const uint32_t clo = span[2 * lc], chi = span[2 * lc + 1];
-    if (clo == chi) continue;
-    for (uint32_t s = clo + tid; s < chi; s += DENSE_THREADS) table[cp[s] - base] = s;
-    __syncthreads();
+    const uint32_t nc = ce - cc;
     for (uint32_t ld = 0; ld < lc; ld++) {
-      const uint32_t d = g.cam0 + ld;
       const uint32_t dlo = span[2 * ld], dhi = span[2 * ld + 1];
-      if (dlo == dhi || !variable_camera(q, d)) continue;
+      if (dlo == dhi) continue;
       double acc[36];
 #pragma unroll
       for (int k = 0; k < 36; k++) acc[k] = 0;
       int hit = 0;
       for (uint32_t s = dlo + tid; s < dhi; s += DENSE_THREADS) {
         const uint32_t p = cp[s];
-        const uint32_t sc = table[p - base];
+        uint32_t sc = DENSE_NONE;
+        if (tabled) {
+          sc = table[p - p_lo];
+        } else {
+          uint32_t lo = 0, hi = nc;
+          while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (table[mid] < p)
+              lo = mid + 1;
+            else
+              hi = mid;
+          }
+          if (lo < nc && table[lo] == p) sc = cc + lo;
+        }
         if (sc == DENSE_NONE) continue;
         hit = 1;
         pair_block(q.camG, sc, s, acc);
@@ -184,6 +235,7 @@ __global__ __launch_bounds__(DENSE_THREADS) void k_dense_rows(Problem q) {
       const double v = dense_block_sum<36>(acc, red);
       if (tid < 36) D[(size_t)(r0 + tid / 6) * n + 6 * ld + tid % 6] -= v;
     }
+    cc = ce;
   }
 }
 
@@ -267,7 +319,7 @@ __global__ __launch_bounds__(DENSE_THREADS) void k_dense_factor(Problem q) {
   __shared__ double Pj[NB][NB + 1];
   __shared__ double y[DENSE_MAX_ROWS];
   __shared__ int bad;
-  const uint32_t b = q.direct_bundles[blockIdx.x];
+  const uint32_t b = q.small_bundles[blockIdx.x];
   double* s = bstate(q, b);
   if (s[S_ACTIVE] == 0 || s[S_PCG_ACTIVE] == 0) return;
   const Bundle& g = q.bundles[b];
@@ -366,7 +418,7 @@ template <bool SHARED>
 __global__ __launch_bounds__(DENSE_THREADS) void k_dense_correct(Problem q) {
   __shared__ double L[NB][NB + 1];
   __shared__ double y[DENSE_MAX_ROWS];
-  const uint32_t b = q.direct_bundles[blockIdx.x];
+  const uint32_t b = q.small_bundles[blockIdx.x];
   const double* s = bstate(q, b);
   if (s[S_ACTIVE] == 0 || s[S_DIRECT] != 1) return;
   const Bundle& g = q.bundles[b];
@@ -377,6 +429,275 @@ __global__ __launch_bounds__(DENSE_THREADS) void k_dense_correct(Problem q) {
   dense_substitute(A, n, u, y, L);
   double* x = vx(q);
   for (uint32_t i = tid; i < mb; i += DENSE_THREADS) x[dense_row(g, i)] += y[i];
+}
+
+// ------------------------------------------------------------------ large bundles
+//
+// A bundle of more than DENSE_MAX_CAMERAS cameras is factored by a sequence of launches, one
+// 64-column panel at a time: k_lfact_diag factors the diagonal block (one wave, lane per row),
+// k_lfact_panel solves the rows below against it (a thread per row), k_lfact_update subtracts the
+// panel's outer products from the trailing lower triangle (a workgroup per 64x64 tile, 4x4 outputs
+// per thread). The substitution runs a launch per block and direction: every workgroup of a step
+// solves the step's 64x64 triangular system itself (the same sums everywhere), workgroup 0 stores
+// the result, and each workgroup then updates its own rows. The dense array keeps u + 1 more words
+// after its (u + 1)^2 for the forward-substituted vector.
+
+constexpr int LNB = DENSE_LNB;
+
+// Word 41 (failed pivot) is set by k_lfact_diag; mode 0 is the first solve (the bundle's PCG set-up
+// still pending), mode 1 a correction (an uncertified direct solution).
+__device__ __forceinline__ bool large_skip(const double* s, int mode) {
+  if (s[S_ACTIVE] == 0) return true;
+  if (mode == 0) return s[S_PCG_ACTIVE] == 0 || s[S_DENSE_BAD] != 0;
+  return s[S_DIRECT] != 1;
+}
+
+// Identity padding rows, a zero right-hand side beyond the unknowns, the failure word cleared.
+template <bool SHARED>
+__global__ __launch_bounds__(DENSE_THREADS) void k_lfact_begin(Problem q, uint32_t b) {
+  double* s = bstate(q, b);
+  if (s[S_ACTIVE] == 0 || s[S_PCG_ACTIVE] == 0) return;
+  const Bundle& g = q.bundles[b];
+  const uint32_t n = g.dense_n, u = n - 1, mb = dense_unknowns<SHARED>(g);
+  double* A = q.dense + g.dense_off;
+  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+  for (uint32_t i = mb; i < u; i++)
+    for (uint32_t j = gid; j <= i; j += stride) A[(size_t)i * n + j] = j == i ? 1.0 : 0.0;
+  for (uint32_t j = mb + gid; j < u; j += stride) A[(size_t)u * n + j] = 0;
+  if (gid == 0) s[S_DENSE_BAD] = 0;
+}
+
+template <bool SHARED>
+__global__ __launch_bounds__(64) void k_lfact_diag(Problem q, uint32_t b, uint32_t kb) {
+  __shared__ double L[LNB][LNB + 1];
+  double* s = bstate(q, b);
+  if (large_skip(s, 0)) return;
+  const Bundle& g = q.bundles[b];
+  const uint32_t n = g.dense_n, lane = threadIdx.x;
+  double* A = q.dense + g.dense_off;
+  for (uint32_t e = lane; e < LNB * LNB; e += 64) {
+    const uint32_t i = e / LNB, j = e % LNB;
+    L[i][j] = j <= i ? A[(size_t)(kb + i) * n + kb + j] : 0.0;
+  }
+  wave_sync();
+  bool bad = false;
+  for (uint32_t k = 0; k < (uint32_t)LNB; k++) {
+    if (lane == k) {
+      const double d = L[k][k];
+      bad = !(d > 0) || !(d < INFINITY);
+      L[k][k] = sqrt(fabs(d));
+    }
+    wave_sync();
+    if (lane > k) L[lane][k] /= L[k][k];
+    wave_sync();
+    if (lane > k) {
+      const double lik = L[lane][k];
+      for (uint32_t j = k + 1; j <= lane; j++) L[lane][j] -= lik * L[j][k];
+    }
+    wave_sync();
+  }
+  if (__any(bad) && lane == 0) s[S_DENSE_BAD] = 1;
+  for (uint32_t e = lane; e < LNB * LNB; e += 64) {
+    const uint32_t i = e / LNB, j = e % LNB;
+    if (j <= i) A[(size_t)(kb + i) * n + kb + j] = L[i][j];
+  }
+}
+
+template <bool SHARED>
+__global__ __launch_bounds__(DENSE_THREADS) void k_lfact_panel(Problem q, uint32_t b, uint32_t kb) {
+  __shared__ double L[LNB][LNB + 1];
+  const double* s = bstate(q, b);
+  if (large_skip(s, 0)) return;
+  const Bundle& g = q.bundles[b];
+  const uint32_t n = g.dense_n, u = n - 1;
+  double* A = q.dense + g.dense_off;
+  for (uint32_t e = threadIdx.x; e < LNB * LNB; e += DENSE_THREADS) {
+    const uint32_t i = e / LNB, j = e % LNB;
+    L[i][j] = j <= i ? A[(size_t)(kb + i) * n + kb + j] : 0.0;
+  }
+  __syncthreads();
+  const uint32_t i = kb + LNB + blockIdx.x * DENSE_THREADS + threadIdx.x;
+  if (i >= u) return;
+  double* row = A + (size_t)i * n + kb;
+  double a[LNB];
+#pragma unroll
+  for (int j = 0; j < LNB; j++) a[j] = row[j];
+#pragma unroll
+  for (int j = 0; j < LNB; j++) {
+    double v = a[j];
+#pragma unroll
+    for (int t = 0; t < j; t++) v -= a[t] * L[j][t];
+    a[j] = v / L[j][j];
+    // keep the scheduler from hoisting the LDS reads of later rows
+    __builtin_amdgcn_sched_barrier(0);
+  }
+#pragma unroll
+  for (int j = 0; j < LNB; j++) row[j] = a[j];
+}
+
+// Tile index t of the lower triangle of T x T tiles: (row, column) with column <= row.
+__device__ __forceinline__ void lower_tile(uint32_t t, uint32_t& ti, uint32_t& tj) {
+  uint32_t r = (uint32_t)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while (r * (r + 1) / 2 > t) r--;
+  while ((r + 1) * (r + 2) / 2 <= t) r++;
+  ti = r;
+  tj = t - r * (r + 1) / 2;
+}
+
+template <bool SHARED>
+__global__ __launch_bounds__(DENSE_THREADS) void k_lfact_update(Problem q, uint32_t b,
+                                                                 uint32_t kb) {
+  __shared__ double Pi[LNB][LNB + 1];
+  __shared__ double Pj[LNB][LNB + 1];
+  const double* s = bstate(q, b);
+  if (large_skip(s, 0)) return;
+  const Bundle& g = q.bundles[b];
+  const uint32_t n = g.dense_n;
+  double* A = q.dense + g.dense_off;
+  uint32_t ti, tj;
+  lower_tile(blockIdx.x, ti, tj);
+  const uint32_t ib = kb + LNB + LNB * ti, jb = kb + LNB + LNB * tj;
+  for (uint32_t e = threadIdx.x; e < LNB * LNB; e += DENSE_THREADS) {
+    const uint32_t i = e / LNB, t = e % LNB;
+    Pi[i][t] = A[(size_t)(ib + i) * n + kb + t];
+    Pj[i][t] = A[(size_t)(jb + i) * n + kb + t];
+  }
+  __syncthreads();
+  const uint32_t i0 = (threadIdx.x / 16) * 4, j0 = (threadIdx.x % 16) * 4;
+  double acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int c = 0; c < 4; c++) acc[a][c] = 0;
+  for (int t = 0; t < LNB; t++) {
+    double x[4], y[4];
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      x[a] = Pi[i0 + a][t];
+      y[a] = Pj[j0 + a][t];
+    }
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+      for (int c = 0; c < 4; c++) acc[a][c] += x[a] * y[c];
+  }
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+      if (ti != tj || j0 + c <= i0 + a) A[(size_t)(ib + i0 + a) * n + jb + j0 + c] -= acc[a][c];
+}
+
+// Forward step of block kb: y_kb = L_kk^-1 w_kb into the vector after the array (workgroup 0),
+// and w_i -= L_i,kb y_kb for the rows i below, w the array's last row.
+template <bool SHARED>
+__global__ __launch_bounds__(DENSE_THREADS) void k_lsolve_fwd(Problem q, uint32_t b, uint32_t kb,
+                                                               int mode) {
+  __shared__ double L[LNB][LNB + 1];
+  __shared__ double y[LNB];
+  const double* s = bstate(q, b);
+  if (large_skip(s, mode)) return;
+  const Bundle& g = q.bundles[b];
+  const uint32_t n = g.dense_n, u = n - 1, tid = threadIdx.x;
+  double* A = q.dense + g.dense_off;
+  double* w = A + (size_t)u * n;
+  double* yv = A + (size_t)n * n;
+  for (uint32_t e = tid; e < LNB * LNB; e += DENSE_THREADS) {
+    const uint32_t i = e / LNB, j = e % LNB;
+    L[i][j] = j <= i ? A[(size_t)(kb + i) * n + kb + j] : 0.0;
+  }
+  if (tid < (uint32_t)LNB) y[tid] = w[kb + tid];
+  __syncthreads();
+  if (tid < 64) {
+    for (uint32_t j = 0; j < (uint32_t)LNB; j++) {
+      if (tid == j) y[j] /= L[j][j];
+      wave_sync();
+      if (tid > j) y[tid] -= L[tid][j] * y[j];
+      wave_sync();
+    }
+  }
+  __syncthreads();
+  if (blockIdx.x == 0 && tid < (uint32_t)LNB) yv[kb + tid] = y[tid];
+  const uint32_t i = kb + LNB + blockIdx.x * DENSE_THREADS + tid;
+  if (i >= u) return;
+  const double* row = A + (size_t)i * n + kb;
+  double sum = 0;
+#pragma unroll 8
+  for (int t = 0; t < LNB; t++) sum += row[t] * y[t];
+  w[i] -= sum;
+}
+
+// Backward step of block kb: x_kb = L_kk^-T y_kb into w (workgroup 0), and y_i -= L_kb,i^T x_kb for
+// the rows i above.
+template <bool SHARED>
+__global__ __launch_bounds__(DENSE_THREADS) void k_lsolve_bwd(Problem q, uint32_t b, uint32_t kb,
+                                                               int mode) {
+  __shared__ double L[LNB][LNB + 1];
+  __shared__ double x[LNB];
+  const double* s = bstate(q, b);
+  if (large_skip(s, mode)) return;
+  const Bundle& g = q.bundles[b];
+  const uint32_t n = g.dense_n, u = n - 1, tid = threadIdx.x;
+  double* A = q.dense + g.dense_off;
+  double* w = A + (size_t)u * n;
+  double* yv = A + (size_t)n * n;
+  for (uint32_t e = tid; e < LNB * LNB; e += DENSE_THREADS) {
+    const uint32_t i = e / LNB, j = e % LNB;
+    L[i][j] = j <= i ? A[(size_t)(kb + i) * n + kb + j] : 0.0;
+  }
+  if (tid < (uint32_t)LNB) x[tid] = yv[kb + tid];
+  __syncthreads();
+  if (tid < 64) {
+    for (int j = LNB - 1; j >= 0; j--) {
+      if (tid == (uint32_t)j) x[j] /= L[j][j];
+      wave_sync();
+      if (tid < (uint32_t)j) x[tid] -= L[j][tid] * x[j];
+      wave_sync();
+    }
+  }
+  __syncthreads();
+  if (blockIdx.x == 0 && tid < (uint32_t)LNB) w[kb + tid] = x[tid];
+  const uint32_t i = blockIdx.x * DENSE_THREADS + tid;
+  if (i >= kb) return;
+  double sum = 0;
+#pragma unroll 8
+  for (int t = 0; t < LNB; t++) sum += A[(size_t)(kb + t) * n + i] * x[t];
+  yv[i] -= sum;
+}
+
+// The residual r into w before a correction (zero in the padding).
+template <bool SHARED>
+__global__ __launch_bounds__(DENSE_THREADS) void k_lsolve_in(Problem q, uint32_t b) {
+  const double* s = bstate(q, b);
+  if (large_skip(s, 1)) return;
+  const Bundle& g = q.bundles[b];
+  const uint32_t n = g.dense_n, u = n - 1, mb = dense_unknowns<SHARED>(g);
+  double* w = q.dense + g.dense_off + (size_t)u * n;
+  const double* r = vr(q);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < u; i += gridDim.x * blockDim.x)
+    w[i] = i < mb ? r[dense_row(g, i)] : 0.0;
+}
+
+// x = w (mode 0: the solution, the bundle's PCG set aside) or x += w (mode 1: a correction).
+template <bool SHARED>
+__global__ __launch_bounds__(DENSE_THREADS) void k_lsolve_out(Problem q, uint32_t b, int mode) {
+  double* s = bstate(q, b);
+  if (large_skip(s, mode)) return;
+  const Bundle& g = q.bundles[b];
+  const uint32_t n = g.dense_n, u = n - 1, mb = dense_unknowns<SHARED>(g);
+  const double* w = q.dense + g.dense_off + (size_t)u * n;
+  double* x = vx(q);
+  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  for (uint32_t i = gid; i < mb; i += gridDim.x * blockDim.x) {
+    if (mode == 0)
+      x[dense_row(g, i)] = w[i];
+    else
+      x[dense_row(g, i)] += w[i];
+  }
+  if (mode == 0 && gid == 0) {
+    s[S_PCG_ACTIVE] = 0;
+    s[S_DIRECT] = 1;
+  }
 }
 
 // The certificate of a direct solution, per bundle with word 38 = 1, after the implicit sweeps of

@@ -54,7 +54,9 @@ struct tcv_ba_context {
   DevBuf scene, trial, index, out, bundles, tiles, tile_bundle, tasks, task_start, cost_blocks,
       camera_bundle, point_bundle, cam_point, Wp, Wc, pblk, tile_fail, tile_focal, task_part, camA,
       camInv, camRhs, camHcf, camFocal, vec, state, cost_part, error, probe, groups, group_part,
-      rhs, grad, camS, camG, pfc, dense, direct_bundles, direct_cams;
+      rhs, grad, camS, camG, pfc, dense, direct_bundles, direct_cams, covis, small_bundles;
+  // the direct bundles factored by a sequence of launches, and their dense array sizes
+  std::vector<uint32_t> large_bundles, large_n;
   std::vector<hipEvent_t> events;
   tcv_ba_context() {
     all = {&scene,      &trial,        &index,     &out,       &bundles,    &tiles,
@@ -63,7 +65,7 @@ struct tcv_ba_context {
            &tile_focal, &task_part,    &camA,      &camInv,    &camRhs,     &camHcf,
            &camFocal,   &vec,          &state,     &cost_part, &error,      &probe,
            &groups,     &group_part,   &rhs,       &grad,      &camS,       &camG,   &pfc,
-           &dense,      &direct_bundles, &direct_cams};
+           &dense,      &direct_bundles, &direct_cams, &covis, &small_bundles};
   }
 };
 
@@ -142,8 +144,9 @@ bool same_options(const tcv_ba_control& a, const tcv_ba_control& b) {
 struct Plan {
   std::vector<Bundle> bundles;
   std::vector<uint32_t> tiles, tile_bundle, tasks, task_start, cost_blocks, camera_bundle,
-      point_bundle, groups, direct_bundles, direct_cams;
+      point_bundle, groups, direct_bundles, direct_cams, small_bundles, large_bundles;
   uint64_t dense_words = 0;
+  uint32_t covis_words = 0;
 };
 
 // The direct route for the bundles of at most DENSE_MAX_CAMERAS cameras (all others stay on PCG),
@@ -153,19 +156,28 @@ void plan_direct(Plan& plan, bool want, uint32_t shared, bool exact,
                  const std::vector<char>& excluded = {}) {
   plan.direct_bundles.clear();
   plan.direct_cams.clear();
+  plan.small_bundles.clear();
+  plan.large_bundles.clear();
   plan.dense_words = 0;
+  plan.covis_words = 0;
   for (uint32_t b = 0; b < plan.bundles.size(); b++) {
     Bundle& g = plan.bundles[b];
     const uint32_t cams = g.cam1 - g.cam0;
     const bool out = b < excluded.size() && excluded[b];
-    g.direct = want && cams <= DENSE_MAX_CAMERAS && !out ? 1u : 0u;
+    g.direct = want && cams <= DENSE_LARGE_CAMERAS && !out ? 1u : 0u;
+    g.large = g.direct && cams > DENSE_MAX_CAMERAS ? 1u : 0u;
     g.unstored = g.direct && !exact ? 1u : 0u;
-    // m_b unknowns padded to a multiple of DENSE_NB, plus the right-hand-side row
-    g.dense_n = g.direct ? (6 * cams + shared + DENSE_NB - 1) / DENSE_NB * DENSE_NB + 1 : 0;
+    // m_b unknowns padded to a multiple of the factorisation's block, plus the right-hand-side
+    // row; the multi-launch route keeps a vector after the array
+    const uint32_t nb = g.large ? DENSE_LNB : DENSE_NB;
+    g.dense_n = g.direct ? (6 * cams + shared + nb - 1) / nb * nb + 1 : 0;
     g.dense_off = plan.dense_words;
+    g.covis_off = plan.covis_words;
     if (!g.direct) continue;
-    plan.dense_words += (uint64_t)g.dense_n * g.dense_n;
+    plan.dense_words += (uint64_t)g.dense_n * g.dense_n + g.dense_n;
+    plan.covis_words += cams * ((cams + 31) / 32);
     plan.direct_bundles.push_back(b);
+    (g.large ? plan.large_bundles : plan.small_bundles).push_back(b);
     for (uint32_t c = g.cam0; c < g.cam1; c++) plan.direct_cams.push_back(c);
   }
 }
@@ -369,7 +381,44 @@ struct Launch {
   static void dense_solve(const Problem& q, hipStream_t s) {
     hipLaunchKernelGGL((k_dense_rows<SHARED>), dim3(q.n_direct_cams), dim3(DENSE_THREADS), 0,
                        s, q);
-    hipLaunchKernelGGL((k_dense_factor<SHARED>), dim3(q.n_direct), dim3(DENSE_THREADS), 0, s, q);
+    if (q.n_small)
+      hipLaunchKernelGGL((k_dense_factor<SHARED>), dim3(q.n_small), dim3(DENSE_THREADS), 0, s, q);
+  }
+  // A large bundle (dense array n x n): the factorisation, panel by panel.
+  static void large_factor(const Problem& q, uint32_t b, uint32_t n, hipStream_t s) {
+    const uint32_t u = n - 1;
+    hipLaunchKernelGGL((k_lfact_begin<SHARED>), dim3(64), dim3(DENSE_THREADS), 0, s, q, b);
+    for (uint32_t kb = 0; kb < u; kb += DENSE_LNB) {
+      hipLaunchKernelGGL((k_lfact_diag<SHARED>), dim3(1), dim3(64), 0, s, q, b, kb);
+      const uint32_t rows = u - kb - DENSE_LNB;
+      if (!rows) continue;
+      hipLaunchKernelGGL((k_lfact_panel<SHARED>), dim3(blocks_for(rows, DENSE_THREADS)),
+                         dim3(DENSE_THREADS), 0, s, q, b, kb);
+      const uint32_t t = rows / DENSE_LNB;
+      hipLaunchKernelGGL((k_lfact_update<SHARED>), dim3(t * (t + 1) / 2), dim3(DENSE_THREADS), 0,
+                         s, q, b, kb);
+    }
+  }
+  // The substitutions of a large bundle: mode 0 solves the assembled right-hand side into x, mode
+  // 1 adds the correction S^-1 r.
+  static void large_solve(const Problem& q, uint32_t b, uint32_t n, int mode, hipStream_t s) {
+    const uint32_t u = n - 1;
+    if (mode == 1)
+      hipLaunchKernelGGL((k_lsolve_in<SHARED>), dim3(blocks_for(u, DENSE_THREADS)),
+                         dim3(DENSE_THREADS), 0, s, q, b);
+    for (uint32_t kb = 0; kb < u; kb += DENSE_LNB) {
+      const uint32_t rows = u - kb - DENSE_LNB;
+      hipLaunchKernelGGL((k_lsolve_fwd<SHARED>), dim3(rows ? blocks_for(rows, DENSE_THREADS) : 1),
+                         dim3(DENSE_THREADS), 0, s, q, b, kb, mode);
+    }
+    for (uint32_t kb = u; kb > 0;) {
+      kb -= DENSE_LNB;
+      hipLaunchKernelGGL((k_lsolve_bwd<SHARED>), dim3(kb ? blocks_for(kb, DENSE_THREADS) : 1),
+                         dim3(DENSE_THREADS), 0, s, q, b, kb, mode);
+    }
+    // mode 0 clears word 19, which every workgroup of the launch tests first: one workgroup
+    hipLaunchKernelGGL((k_lsolve_out<SHARED>), dim3(mode == 0 ? 1 : blocks_for(u, DENSE_THREADS)),
+                       dim3(DENSE_THREADS), 0, s, q, b, mode);
   }
   // The certificate's product S x: the implicit FP64 operator, or the stored FP64 cross blocks of an
   // exact-Hessian system.
@@ -390,7 +439,8 @@ struct Launch {
                        final_pass, tol);
   }
   static void dense_correct(const Problem& q, hipStream_t s) {
-    hipLaunchKernelGGL((k_dense_correct<SHARED>), dim3(q.n_direct), dim3(DENSE_THREADS), 0, s, q);
+    if (q.n_small)
+      hipLaunchKernelGGL((k_dense_correct<SHARED>), dim3(q.n_small), dim3(DENSE_THREADS), 0, s, q);
   }
   static void refine(const Problem& q, hipStream_t s) {
     hipLaunchKernelGGL((k_point_sweep_implicit<SHARED>), dim3(tile_blocks(q)),
@@ -423,6 +473,8 @@ struct Ops {
   void (*unstored_point_sweep)(const Problem&, hipStream_t);
   void (*unstored_camera_sweep)(const Problem&, hipStream_t);
   void (*unstored_update)(const Problem&, hipStream_t);
+  void (*large_factor)(const Problem&, uint32_t, uint32_t, hipStream_t);
+  void (*large_solve)(const Problem&, uint32_t, uint32_t, int, hipStream_t);
 };
 
 // The PCG sweeps of one iteration over the stored and the unstored bundles.
@@ -446,7 +498,8 @@ Ops ops_for(bool implicit) {
         L::camera_sweep, L::advance,       L::update_points,   L::probe,
         L::refine,       L::exact_passes,  L::solve_points,    L::pixel_cotangent,
         L::jacobian,     L::dense_solve,   L::certify,         L::dense_correct,
-        L::unstored_p_sweep, L::unstored_camera_sweep, L::implicit_update};
+        L::unstored_p_sweep, L::unstored_camera_sweep, L::implicit_update,
+        L::large_factor, L::large_solve};
   if (implicit) {
     o.point_sweep = L::implicit_p_sweep;
     o.camera_sweep = L::implicit_camera_sweep;
@@ -570,8 +623,10 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
     TRY(ensure(ctx->camG, (size_t)M * 18 * 8));
     if (shared) TRY(ensure(ctx->pfc, (size_t)P * 3 * 8));
     TRY(ensure(ctx->dense, plan.dense_words * 8));
+    TRY(ensure(ctx->covis, (size_t)plan.covis_words * 4));
     TRY(upload(ctx->direct_bundles, plan.direct_bundles, st));
     TRY(upload(ctx->direct_cams, plan.direct_cams, st));
+    TRY(upload(ctx->small_bundles, plan.small_bundles, st));
   }
   TRY(upload(ctx->bundles, plan.bundles, st));
   TRY(upload(ctx->tiles, plan.tiles, st));
@@ -643,6 +698,9 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
     q.camG = (double*)ctx->camG.ptr;
     q.pfc = shared ? (double*)ctx->pfc.ptr : nullptr;
     q.dense = (double*)ctx->dense.ptr;
+    q.covis = (uint32_t*)ctx->covis.ptr;
+    q.n_small = (uint32_t)plan.small_bundles.size();
+    q.small_bundles = (const uint32_t*)ctx->small_bundles.ptr;
   }
   hipLaunchKernelGGL(k_check_points, dim3(blocks_for(P, 256)), dim3(256), 0, st, q);
   hipLaunchKernelGGL(k_check_tasks, dim3(n_tasks), dim3(256), 0, st, q);
@@ -664,8 +722,10 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
     TRY(upload(ctx->bundles, plan.bundles, st));
     TRY(upload(ctx->direct_bundles, plan.direct_bundles, st));
     TRY(upload(ctx->direct_cams, plan.direct_cams, st));
+    TRY(upload(ctx->small_bundles, plan.small_bundles, st));
     q.n_direct = (uint32_t)plan.direct_bundles.size();
     q.n_direct_cams = (uint32_t)plan.direct_cams.size();
+    q.n_small = (uint32_t)plan.small_bundles.size();
     q.n_unstored = exact ? 0u : q.n_direct;
     if (q.n_unstored < B) {
       TRY(ensure(ctx->Wp, w_words * ws));
@@ -673,6 +733,15 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
       q.Wp = ctx->Wp.ptr;
       q.Wc = ctx->Wc.ptr;
     }
+  }
+  ctx->large_bundles = plan.large_bundles;
+  ctx->large_n.clear();
+  for (uint32_t b : plan.large_bundles) ctx->large_n.push_back(plan.bundles[b].dense_n);
+  if (q.n_direct) {
+    // which cameras of a direct bundle share points: the assembly skips the pairs that do not
+    HIP_TRY(hipMemsetAsync(q.covis, 0, (size_t)plan.covis_words * 4, st));
+    hipLaunchKernelGGL(k_covisibility, dim3(blocks_for(P, 256)), dim3(256), 0, st, q);
+    HIP_TRY(hipGetLastError());
   }
   if (rep) {
     rep->setup_ms = now_ms() - t0;
@@ -746,6 +815,10 @@ int solve_linear(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q,
                  hipStream_t st, tcv_ba_report* rep, uint32_t first, uint32_t budget, double tol) {
   if (!q.n_direct) return run_pcg(ctx, c, q, ops, st, rep, first, budget);
   ops.dense_solve(q, st);
+  for (size_t k = 0; k < ctx->large_bundles.size(); k++) {
+    ops.large_factor(q, ctx->large_bundles[k], ctx->large_n[k], st);
+    ops.large_solve(q, ctx->large_bundles[k], ctx->large_n[k], 0, st);
+  }
   for (uint32_t k = 0; k < kDirectPasses; k++) {
     ops.certify(q, k + 1 == kDirectPasses, tol, st);
     HIP_TRY(hipGetLastError());
@@ -754,6 +827,9 @@ int solve_linear(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q,
     for (uint32_t b = 0; b < q.B; b++) pending |= word(ctx, b, S_DIRECT) == 1;
     if (!pending) break;
     ops.dense_correct(q, st);
+    for (size_t j = 0; j < ctx->large_bundles.size(); j++)
+      if (word(ctx, ctx->large_bundles[j], S_DIRECT) == 1)
+        ops.large_solve(q, ctx->large_bundles[j], ctx->large_n[j], 1, st);
   }
   if (!any_word(ctx, q, S_PCG_ACTIVE)) return TCV_BA_OK;
   return run_pcg(ctx, c, q, ops, st, rep, first, budget);

@@ -52,6 +52,7 @@ enum : int {
                          // certified, 2 certified by the matrix-free residual
   S_DIRECT_SOLVES = 39,  // certified direct solves (whole solve)
   S_DUPLICATE = 40,      // set at set-up when a camera of the bundle observes one point twice
+  S_DENSE_BAD = 41,      // a pivot of the multi-launch factorisation failed this LM iteration
   S_WORDS = 48
 };
 
@@ -75,7 +76,12 @@ constexpr int ASM_WORDS = 56;      // per-task camera assembly sums (see k_camer
 constexpr int DENSE_THREADS = 256; // threads of the direct route's row and factor workgroups
 constexpr int DENSE_TABLE = 8192;  // point ids of one pass of a row workgroup's lookup table
 constexpr int DENSE_NB = 32;       // block size of the dense Cholesky factorisation
-constexpr uint32_t DENSE_MAX_CAMERAS = 128;  // largest bundle the direct route factors
+constexpr uint32_t DENSE_MAX_CAMERAS = 128;  // largest bundle factored by one workgroup
+constexpr uint32_t DENSE_LARGE_CAMERAS = 2048;  // largest bundle the direct route factors (the
+                                                // larger ones by a sequence of launches)
+constexpr int DENSE_LNB = 64;          // block size of the multi-launch factorisation
+constexpr uint32_t DENSE_CHUNK_MIN = 256;  // fewest slots a table-indexed chunk of a camera's list
+                                           // holds (sparser lists are searched by bisection)
 
 // One bundle of the batch: its global ranges and gauge.
 struct Bundle {
@@ -90,6 +96,8 @@ struct Bundle {
   uint32_t dense_n;         // rows of the bundle's dense array: m_b = 6 C_b [+ 1] unknowns + 1
   uint32_t unstored;        // 1: the passes store no cross blocks of this bundle (a direct bundle
                             // outside the exact systems); its sweeps and update relinearise
+  uint32_t large;           // 1: a direct bundle of more than DENSE_MAX_CAMERAS cameras
+  uint32_t covis_off;       // first word of the bundle's covisibility bitmap (direct bundles)
   uint64_t dense_off;       // first double of the bundle's dense array
 };
 
@@ -146,7 +154,11 @@ struct Problem {
   double* camG;    // 18 per camera-major slot, contiguous: G = W L_p with V_p^-1 = L_p L_p^T
   double* pfc;     // 3 per point (shared focal): L_p^T fc_p, fc_p = sum_i w Jp_i^T Jf_i
   double* dense;   // per direct bundle dense_n x dense_n row-major; the lower triangle holds S,
-                   // then its Cholesky factor; row m_b holds the reduced right-hand side
+                   // then its Cholesky factor; the last row holds the reduced right-hand side
+  uint32_t* covis; // per direct bundle C_b rows of ceil(C_b / 32) words: bit d of row c set when
+                   // cameras c > d observe a common point
+  uint32_t n_small;                // direct bundles factored by one workgroup each
+  const uint32_t* small_bundles;
 };
 
 __host__ __device__ inline const uint32_t* obs_camera(const Problem& q) { return q.index; }
