@@ -90,10 +90,11 @@ typedef struct tcv_ba_control {
  * the scene in each sweep instead of streamed from storage. Same results to rounding. */
 #define TCV_BA_FLAG_IMPLICIT 2u
 /* The direct route, the reference's opt-in `schurCache` (src/sfm/bundle/gpu/joint-focal.ts:102-108,
- * joint-focal.shade.ts:1206-1383): each bundle of at most 128 cameras assembles its reduced camera
- * system densely, factors it by Cholesky and solves it in place of the PCG loop; the matrix-free
- * residual certifies the solution (PCG restarts from it when it misses the tolerance). Larger
- * bundles, and batches in which a camera observes one point twice, keep the PCG route. */
+ * joint-focal.shade.ts:1206-1383): each bundle of at most 2,048 cameras assembles its reduced
+ * camera system densely, factors it by Cholesky (one workgroup per bundle up to 128 cameras, a
+ * sequence of launches above) and solves it in place of the PCG loop; the matrix-free residual
+ * certifies the solution (PCG restarts from it when it misses the tolerance). Larger bundles, and
+ * bundles in which a camera observes one point twice, keep the PCG route. */
 #define TCV_BA_FLAG_SCHUR_CACHE 4u
 
 /* What one solve did. Times are milliseconds of host wall clock unless named otherwise. */

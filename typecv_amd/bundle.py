@@ -319,7 +319,7 @@ def make_control(C: int, P: int, M: int, *, directional: int = 0, anchor: int = 
                  refinement_tolerance: float = 0.0, timing: bool = False,
                  implicit: bool = False, schur_cache: bool = False) -> _native.Control:
     """The `tcv_ba_control` of a problem. `schur_cache`: the direct route (dense reduced camera
-    system, Cholesky, matrix-free certificate) for bundles of at most 128 cameras."""
+    system, Cholesky, matrix-free certificate) for bundles of at most 2,048 cameras."""
     flags = ((_native.FLAG_TIMING if timing else 0) | (_native.FLAG_IMPLICIT if implicit else 0)
              | (_native.FLAG_SCHUR_CACHE if schur_cache else 0))
     return _native.Control(

@@ -434,23 +434,54 @@ __global__ __launch_bounds__(DENSE_THREADS) void k_dense_correct(Problem q) {
 // ------------------------------------------------------------------ large bundles
 //
 // A bundle of more than DENSE_MAX_CAMERAS cameras is factored by a sequence of launches, one
-// 64-column panel at a time: k_lfact_diag factors the diagonal block (one wave, lane per row),
-// k_lfact_panel solves the rows below against it (a thread per row), k_lfact_update subtracts the
-// panel's outer products from the trailing lower triangle (a workgroup per 64x64 tile, 4x4 outputs
-// per thread). The substitution runs a launch per block and direction: every workgroup of a step
-// solves the step's 64x64 triangular system itself (the same sums everywhere), workgroup 0 stores
-// the result, and each workgroup then updates its own rows. The dense array keeps u + 1 more words
-// after its (u + 1)^2 for the forward-substituted vector.
+// 64-column panel at a time:
+//   k_lfact_diag    the diagonal block L_kk (one workgroup: a step per column, the block's entries
+//                   held in registers by fixed owner threads, one barrier per step) and its inverse
+//                   X_kk = L_kk^-1 (the same scheme, a step per row), stored beside the array;
+//   k_lfact_panel   the rows below as a product with the inverse, P = A X_kk^T (a workgroup per 64
+//                   rows, 4x4 outputs per thread), and with it the forward substitution of the
+//                   right-hand side, which is the array's last row: y_kb = X_kk w_kb, then
+//                   w_i -= P_i . y_kb for the workgroup's own rows;
+//   k_lfact_update  the panel's outer products subtracted from the trailing lower triangle (a
+//                   workgroup per 64x64 tile, 4x4 outputs per thread).
+// The back substitution runs a launch per block (k_lsolve_bwd): x_kb = X_kk^T y_kb, then
+// y_i -= L_kb,i^T x_kb for the rows above. A correction (iterative refinement) also runs the
+// forward substitution block by block (k_lsolve_fwd). Beside a large bundle's (u + 1)^2 array the
+// dense words hold the forward-substituted vector (u + 1 words) and the u / 64 inverses X_kk.
 
 constexpr int LNB = DENSE_LNB;
 
+// A large bundle's dense array as the multi-launch kernels address it: A (n x n, row-major), the
+// forward vector after it (n words), then the u / 64 inverses X_kk (64 x 64, row-major), and the
+// bundle's state words. The host passes the pointers, so that a kernel's first loads of the array
+// go out together with its load of the state words.
+struct LargeDense {
+  double* A;
+  double* state;
+  uint32_t n;
+};
+
+__device__ __forceinline__ double* large_vector(const LargeDense& d) {
+  return d.A + (size_t)d.n * d.n;
+}
+__device__ __forceinline__ double* large_inverse(const LargeDense& d, uint32_t kb) {
+  return large_vector(d) + d.n + (size_t)kb * LNB;
+}
+
 // Word 41 (failed pivot) is set by k_lfact_diag; mode 0 is the first solve (the bundle's PCG set-up
 // still pending), mode 1 a correction (an uncertified direct solution).
+// The words are read together (no short-circuit), in one round trip.
 __device__ __forceinline__ bool large_skip(const double* s, int mode) {
-  if (s[S_ACTIVE] == 0) return true;
-  if (mode == 0) return s[S_PCG_ACTIVE] == 0 || s[S_DENSE_BAD] != 0;
-  return s[S_DIRECT] != 1;
+  const double active = s[S_ACTIVE], pcg = s[S_PCG_ACTIVE], bad = s[S_DENSE_BAD],
+               direct = s[S_DIRECT];
+  if (mode == 0) return (active == 0) | (pcg == 0) | (bad != 0);
+  return (active == 0) | (direct != 1);
 }
+
+// Marks a loaded value as used here: the load is issued (and waited for) before what follows, so a
+// kernel's array loads go out together with its state-word loads instead of after the test of
+// those words. An empty asm statement: no instruction is emitted.
+__device__ __forceinline__ void settle(double& v) { asm volatile("" : "+v"(v)); }
 
 // Identity padding rows, a zero right-hand side beyond the unknowns, the failure word cleared.
 template <bool SHARED>
@@ -467,108 +498,357 @@ __global__ __launch_bounds__(DENSE_THREADS) void k_lfact_begin(Problem q, uint32
   if (gid == 0) s[S_DENSE_BAD] = 0;
 }
 
-template <bool SHARED>
-__global__ __launch_bounds__(64) void k_lfact_diag(Problem q, uint32_t b, uint32_t kb) {
-  __shared__ double L[LNB][LNB + 1];
-  double* s = bstate(q, b);
-  if (large_skip(s, 0)) return;
-  const Bundle& g = q.bundles[b];
-  const uint32_t n = g.dense_n, lane = threadIdx.x;
-  double* A = q.dense + g.dense_off;
-  for (uint32_t e = lane; e < LNB * LNB; e += 64) {
-    const uint32_t i = e / LNB, j = e % LNB;
-    L[i][j] = j <= i ? A[(size_t)(kb + i) * n + kb + j] : 0.0;
-  }
-  wave_sync();
-  bool bad = false;
-  for (uint32_t k = 0; k < (uint32_t)LNB; k++) {
-    if (lane == k) {
-      const double d = L[k][k];
-      bad = !(d > 0) || !(d < INFINITY);
-      L[k][k] = sqrt(fabs(d));
-    }
-    wave_sync();
-    if (lane > k) L[lane][k] /= L[k][k];
-    wave_sync();
-    if (lane > k) {
-      const double lik = L[lane][k];
-      for (uint32_t j = k + 1; j <= lane; j++) L[lane][j] -= lik * L[j][k];
-    }
-    wave_sync();
-  }
-  if (__any(bad) && lane == 0) s[S_DENSE_BAD] = 1;
-  for (uint32_t e = lane; e < LNB * LNB; e += 64) {
-    const uint32_t i = e / LNB, j = e % LNB;
-    if (j <= i) A[(size_t)(kb + i) * n + kb + j] = L[i][j];
-  }
+// (row, column) of entry e of a lower triangle enumerated row by row (e = i (i + 1) / 2 + j).
+__device__ __forceinline__ void lower_entry(uint32_t e, uint32_t& i, uint32_t& j) {
+  uint32_t r = (uint32_t)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+  while (r * (r + 1) / 2 > e) r--;
+  while ((r + 1) * (r + 2) / 2 <= e) r++;
+  i = r;
+  j = e - r * (r + 1) / 2;
 }
 
-template <bool SHARED>
-__global__ __launch_bounds__(DENSE_THREADS) void k_lfact_panel(Problem q, uint32_t b, uint32_t kb) {
-  __shared__ double L[LNB][LNB + 1];
-  const double* s = bstate(q, b);
-  if (large_skip(s, 0)) return;
-  const Bundle& g = q.bundles[b];
-  const uint32_t n = g.dense_n, u = n - 1;
-  double* A = q.dense + g.dense_off;
-  for (uint32_t e = threadIdx.x; e < LNB * LNB; e += DENSE_THREADS) {
-    const uint32_t i = e / LNB, j = e % LNB;
-    L[i][j] = j <= i ? A[(size_t)(kb + i) * n + kb + j] : 0.0;
+// One doubling level of the inverse of the 64 x 64 lower-triangular L (in C, zero above the
+// diagonal) into X (zero above the diagonal; its diagonal blocks of size H already inverted): for
+// each pair of diagonal H x H blocks at p = 2 H b, X21 = -X22 (L21 X11). T = L21 X11 is kept in C's
+// upper triangle (T_rc at C[p + c][p + H + r]), which nothing else reads. The 32 H outputs of a
+// phase are spread over the 256 threads, H / 8 per thread from H = 8 up, their sums interleaved.
+template <int H>
+__device__ __forceinline__ void inverse_level(double (*C)[LNB + 1], double (*X)[LNB + 1]) {
+  constexpr uint32_t OUT = 32 * H, Q = OUT > DENSE_THREADS ? OUT / DENSE_THREADS : 1;
+  const uint32_t tid = threadIdx.x;
+  const bool on = OUT >= DENSE_THREADS || tid < OUT;
+  uint32_t r[Q], c[Q], p[Q];
+#pragma unroll
+  for (uint32_t q = 0; q < Q; q++) {
+    const uint32_t o = tid + DENSE_THREADS * q;
+    r[q] = o % (H * H) / H;
+    c[q] = o % H;
+    p[q] = 2 * H * (o / (H * H));
+  }
+  double sum[Q];
+  if (on) {
+#pragma unroll
+    for (uint32_t q = 0; q < Q; q++) sum[q] = 0;
+#pragma unroll
+    for (int m = 0; m < H; m++)
+#pragma unroll
+      for (uint32_t q = 0; q < Q; q++)
+        sum[q] += C[p[q] + H + r[q]][p[q] + m] * X[p[q] + m][p[q] + c[q]];
+#pragma unroll
+    for (uint32_t q = 0; q < Q; q++) C[p[q] + c[q]][p[q] + H + r[q]] = sum[q];
   }
   __syncthreads();
-  const uint32_t i = kb + LNB + blockIdx.x * DENSE_THREADS + threadIdx.x;
-  if (i >= u) return;
-  double* row = A + (size_t)i * n + kb;
-  double a[LNB];
+  if (on) {
 #pragma unroll
-  for (int j = 0; j < LNB; j++) a[j] = row[j];
+    for (uint32_t q = 0; q < Q; q++) sum[q] = 0;
 #pragma unroll
-  for (int j = 0; j < LNB; j++) {
-    double v = a[j];
+    for (int m = 0; m < H; m++)
 #pragma unroll
-    for (int t = 0; t < j; t++) v -= a[t] * L[j][t];
-    a[j] = v / L[j][j];
-    // keep the scheduler from hoisting the LDS reads of later rows
-    __builtin_amdgcn_sched_barrier(0);
-  }
+      for (uint32_t q = 0; q < Q; q++)
+        sum[q] += X[p[q] + H + r[q]][p[q] + H + m] * C[p[q] + c[q]][p[q] + H + m];
 #pragma unroll
-  for (int j = 0; j < LNB; j++) row[j] = a[j];
-}
-
-// Tile index t of the lower triangle of T x T tiles: (row, column) with column <= row.
-__device__ __forceinline__ void lower_tile(uint32_t t, uint32_t& ti, uint32_t& tj) {
-  uint32_t r = (uint32_t)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-  while (r * (r + 1) / 2 > t) r--;
-  while ((r + 1) * (r + 2) / 2 <= t) r++;
-  ti = r;
-  tj = t - r * (r + 1) / 2;
-}
-
-template <bool SHARED>
-__global__ __launch_bounds__(DENSE_THREADS) void k_lfact_update(Problem q, uint32_t b,
-                                                                 uint32_t kb) {
-  __shared__ double Pi[LNB][LNB + 1];
-  __shared__ double Pj[LNB][LNB + 1];
-  const double* s = bstate(q, b);
-  if (large_skip(s, 0)) return;
-  const Bundle& g = q.bundles[b];
-  const uint32_t n = g.dense_n;
-  double* A = q.dense + g.dense_off;
-  uint32_t ti, tj;
-  lower_tile(blockIdx.x, ti, tj);
-  const uint32_t ib = kb + LNB + LNB * ti, jb = kb + LNB + LNB * tj;
-  for (uint32_t e = threadIdx.x; e < LNB * LNB; e += DENSE_THREADS) {
-    const uint32_t i = e / LNB, t = e % LNB;
-    Pi[i][t] = A[(size_t)(ib + i) * n + kb + t];
-    Pj[i][t] = A[(size_t)(jb + i) * n + kb + t];
+    for (uint32_t q = 0; q < Q; q++) X[p[q] + H + r[q]][p[q] + c[q]] = -sum[q];
   }
   __syncthreads();
-  const uint32_t i0 = (threadIdx.x / 16) * 4, j0 = (threadIdx.x % 16) * 4;
+}
+
+// 1 / v: the hardware estimate and one Newton step (relative error a few ulp).
+__device__ __forceinline__ double recip(double v) {
+  const double r = __builtin_amdgcn_rcp(v);
+  return fma(r, fma(-v, r, 1.0), r);
+}
+
+// LDS words of the diagonal-block routine (two 64 x 65 arrays, the published columns of two block
+// steps, their pivot coefficients, 1 / sqrt of the pivots); the update kernel's two panel tiles
+// share the same buffer.
+constexpr int DIAG_LDS = 2 * LNB * (LNB + 1) + 2 * 4 * LNB + 2 * 16 + LNB;
+
+// The diagonal block of panel kb, L_kk and X_kk = L_kk^-1, by the 256 threads of one workgroup.
+// Thread t holds the 4x4 block a (rows 4 (t / 16).., columns 4 (t % 16)..) of the whole symmetric
+// 64 x 64 block in registers. Cholesky in 16 block steps of 4 columns: the threads owning columns
+// k..k+3 publish them (zero above row k), the owner of the 4x4 pivot block publishes its
+// elimination coefficients; after one barrier every thread eliminates its own rows' and columns'
+// 4 values and subtracts the rank-4 product from its 16 entries. The eliminated columns scaled by
+// 1 / sqrt(pivot) are L. The inverse is built by doubling (inverse_level), six levels of two
+// products each.
+__device__ void diag_block(const LargeDense& d, uint32_t kb, double a[4][4], double* smem) {
+  auto C = reinterpret_cast<double(*)[LNB + 1]>(smem);  // row k: L's column k, then L
+  auto X = reinterpret_cast<double(*)[LNB + 1]>(smem + LNB * (LNB + 1));
+  double* pre = smem + 2 * LNB * (LNB + 1);  // [2][4][64]: the published columns of a block step
+  double* coef = pre + 2 * 4 * LNB;          // [2][16]: f10 f20 f21 f30 f31 f32, then 1 / pivot
+  double* rroot = coef + 2 * 16;
+  const uint32_t n = d.n, tid = threadIdx.x, i0 = (tid / 16) * 4, j0 = (tid % 16) * 4;
+  double* A = d.A + (size_t)kb * n + kb;
+  for (uint32_t k = 0; k < (uint32_t)LNB; k += 4) {
+    double* P = pre + ((k / 4) & 1) * 4 * LNB;
+    double* F = coef + ((k / 4) & 1) * 16;
+    if (j0 == k) {
+#pragma unroll
+      for (int x = 0; x < 4; x++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) P[c * LNB + i0 + x] = i0 + x >= k ? a[x][c] : 0.0;
+      if (i0 == k) {
+        // the pivot block's elimination: e[m][l] = a_ml - sum_{l' < l} e[m][l'] e[l][l'] / e[l'][l']
+        double e[4][4], g[4];
+#pragma unroll
+        for (int l = 0; l < 4; l++) {
+#pragma unroll
+          for (int m = l; m < 4; m++) {
+            double v = a[m][l];
+#pragma unroll
+            for (int t = 0; t < l; t++) v -= e[m][t] * e[l][t] * g[t];
+            e[m][l] = v;
+          }
+          g[l] = recip(e[l][l]);
+        }
+        F[0] = e[1][0] * g[0];
+        F[1] = e[2][0] * g[0];
+        F[2] = e[2][1] * g[1];
+        F[3] = e[3][0] * g[0];
+        F[4] = e[3][1] * g[1];
+        F[5] = e[3][2] * g[2];
+#pragma unroll
+        for (int l = 0; l < 4; l++) F[6 + l] = g[l];
+      }
+    }
+    __syncthreads();
+    const double f10 = F[0], f20 = F[1], f21 = F[2], f30 = F[3], f31 = F[4], f32 = F[5];
+    // the 8 vectors (4 rows, 4 columns) eliminated level by level, so that each level's 8 chains
+    // interleave; then the rank-4 product, one column level at a time (16 independent updates)
+    double v[8][4];
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+#pragma unroll
+      for (int x = 0; x < 4; x++) {
+        v[x][c] = P[c * LNB + i0 + x];
+        v[4 + x][c] = P[c * LNB + j0 + x];
+      }
+#pragma unroll
+    for (int e = 0; e < 8; e++) v[e][1] -= v[e][0] * f10;
+#pragma unroll
+    for (int e = 0; e < 8; e++) v[e][2] -= v[e][0] * f20 + v[e][1] * f21;
+#pragma unroll
+    for (int e = 0; e < 8; e++) v[e][3] -= v[e][0] * f30 + v[e][1] * f31 + v[e][2] * f32;
+#pragma unroll
+    for (int l = 0; l < 4; l++) {
+      const double g = F[6 + l];
+      double cj[4];
+#pragma unroll
+      for (int y = 0; y < 4; y++) cj[y] = v[4 + y][l] * g;
+#pragma unroll
+      for (int x = 0; x < 4; x++)
+#pragma unroll
+        for (int y = 0; y < 4; y++) a[x][y] -= v[x][l] * cj[y];
+    }
+    if (j0 == k) {
+#pragma unroll
+      for (int x = 0; x < 4; x++)
+#pragma unroll
+        for (int l = 0; l < 4; l++) C[k + l][i0 + x] = i0 + x >= k + l ? v[x][l] : 0.0;
+    }
+  }
+  __syncthreads();
+  if (tid < (uint32_t)LNB) {
+    const double piv = C[tid][tid];
+    rroot[tid] = 1.0 / sqrt(fabs(piv));
+    if (!(piv > 0) || !(piv < INFINITY)) d.state[S_DENSE_BAD] = 1;
+  }
+  __syncthreads();
+  double l[4][4];
+#pragma unroll
+  for (int x = 0; x < 4; x++)
+#pragma unroll
+    for (int y = 0; y < 4; y++) l[x][y] = C[j0 + y][i0 + x] * rroot[j0 + y];
+#pragma unroll
+  for (int x = 0; x < 4; x++)
+#pragma unroll
+    for (int y = 0; y < 4; y++)
+      if (i0 + x >= j0 + y) A[(size_t)(i0 + x) * n + j0 + y] = l[x][y];
+  __syncthreads();  // C is read: it takes L (row-major, zero above the diagonal)
+#pragma unroll
+  for (int x = 0; x < 4; x++)
+#pragma unroll
+    for (int y = 0; y < 4; y++) {
+      C[i0 + x][j0 + y] = l[x][y];
+      X[i0 + x][j0 + y] = i0 + x == j0 + y ? 1.0 / l[x][y] : 0.0;
+    }
+  __syncthreads();
+  inverse_level<1>(C, X);
+  inverse_level<2>(C, X);
+  inverse_level<4>(C, X);
+  inverse_level<8>(C, X);
+  inverse_level<16>(C, X);
+  inverse_level<32>(C, X);
+  double* Xg = large_inverse(d, kb);
+  for (uint32_t e = tid; e < (uint32_t)(LNB * LNB); e += DENSE_THREADS) {
+    const uint32_t i = e / LNB, j = e % LNB;
+    Xg[e] = j <= i ? X[i][j] : 0.0;
+  }
+}
+
+// The first diagonal block (the later ones are factored by the update launch before them).
+__global__ __launch_bounds__(DENSE_THREADS) void k_lfact_diag(LargeDense d, uint32_t kb) {
+  __shared__ double smem[DIAG_LDS];
+  const bool skip = large_skip(d.state, 0);
+  const uint32_t n = d.n, tid = threadIdx.x, i0 = (tid / 16) * 4, j0 = (tid % 16) * 4;
+  const double* A = d.A + (size_t)kb * n + kb;
+  double a[4][4];
+#pragma unroll
+  for (int x = 0; x < 4; x++)
+#pragma unroll
+    for (int y = 0; y < 4; y++) {
+      const uint32_t i = i0 + x, j = j0 + y;
+      a[x][y] = A[(size_t)max(i, j) * n + min(i, j)];
+    }
+#pragma unroll
+  for (int x = 0; x < 4; x++)
+#pragma unroll
+    for (int y = 0; y < 4; y++) settle(a[x][y]);
+  if (skip) return;
+  diag_block(d, kb, a, smem);
+}
+
+// The 64 values X v (TRANSPOSE: X^T v) for a 64 x 64 X with row stride LD and v in LDS, into out
+// (LDS): four threads per output, partial sums of 16 terms combined by shuffles. 256 threads.
+template <bool TRANSPOSE, int LD>
+__device__ __forceinline__ void block_matvec(const double* X, const double* v, double* out) {
+  const uint32_t tid = threadIdx.x, o = tid / 4, part = tid % 4;
+  double sum = 0;
+#pragma unroll
+  for (int k = 0; k < LNB / 4; k++) {
+    const uint32_t c = part * (LNB / 4) + k;
+    sum += (TRANSPOSE ? X[c * LD + o] : X[o * LD + c]) * v[c];
+  }
+  sum += __shfl_xor(sum, 1, 64);
+  sum += __shfl_xor(sum, 2, 64);
+  if (part == 0) out[o] = sum;
+}
+
+// The panel below block kb as P = A X_kk^T (a workgroup per 64 rows, 4x4 outputs per thread), and
+// the forward substitution of block kb: y_kb = X_kk w_kb (workgroup 0 stores it; the last block
+// has no rows below and runs only this), w_i -= P_i . y_kb for the workgroup's rows.
+__global__ __launch_bounds__(DENSE_THREADS) void k_lfact_panel(LargeDense d, uint32_t kb) {
+  __shared__ double Xs[LNB][LNB + 1];
+  __shared__ double Ps[LNB][LNB + 1];
+  __shared__ double wk[LNB], y[LNB];
+  const bool skip = large_skip(d.state, 0);
+  const uint32_t n = d.n, u = n - 1, tid = threadIdx.x;
+  double* A = d.A;
+  double* w = A + (size_t)u * n;
+  const double* Xg = large_inverse(d, kb);
+  const uint32_t r0 = kb + LNB + LNB * blockIdx.x;
+  const bool rows = r0 < u;
+  // every load issued before the first wait: X_kk, w_kb and the 64 panel rows (thread tid: column
+  // tid % 64 of rows tid / 64 + 4 r)
+  const uint32_t ti = tid / LNB, tc = tid % LNB;
+  double xv[LNB / 4], pv[LNB / 4];
+#pragma unroll
+  for (int r = 0; r < LNB / 4; r++) {
+    xv[r] = Xg[(ti + 4 * r) * LNB + tc];
+    pv[r] = rows ? A[(size_t)(r0 + ti + 4 * r) * n + kb + tc] : 0.0;
+  }
+  double wv = w[kb + tc];
+#pragma unroll
+  for (int r = 0; r < LNB / 4; r++) {
+    settle(xv[r]);
+    settle(pv[r]);
+  }
+  settle(wv);
+  if (skip) return;
+  if (tid < (uint32_t)LNB) wk[tid] = wv;
+#pragma unroll
+  for (int r = 0; r < LNB / 4; r++) {
+    Xs[ti + 4 * r][tc] = xv[r];
+    Ps[ti + 4 * r][tc] = pv[r];
+  }
+  __syncthreads();
+  block_matvec<false, LNB + 1>(&Xs[0][0], wk, y);  // y_kb = X_kk w_kb
+  __syncthreads();
+  if (blockIdx.x == 0 && tid < (uint32_t)LNB) large_vector(d)[kb + tid] = y[tid];
+  if (!rows) return;
+  const uint32_t i0 = (tid / 16) * 4, j0 = (tid % 16) * 4;
   double acc[4][4];
 #pragma unroll
   for (int a = 0; a < 4; a++)
 #pragma unroll
     for (int c = 0; c < 4; c++) acc[a][c] = 0;
+#pragma unroll 4
+  for (int t = 0; t < LNB; t++) {
+    double xa[4], xb[4];
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      xa[a] = Ps[i0 + a][t];
+      xb[a] = Xs[j0 + a][t];
+    }
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+      for (int c = 0; c < 4; c++) acc[a][c] += xa[a] * xb[c];
+  }
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    double part = 0;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      A[(size_t)(r0 + i0 + a) * n + kb + j0 + c] = acc[a][c];
+      part += acc[a][c] * y[j0 + c];
+    }
+    // the 16 threads of row i0 + a hold its 64 columns: lanes tid % 16 of one 16-lane group
+    for (int o = 8; o > 0; o >>= 1) part += __shfl_xor(part, o, 64);
+    if (j0 == 0) w[r0 + i0 + a] -= part;
+  }
+}
+
+// The trailing lower triangle less the panel's outer products: a workgroup per 64x64 tile
+// (blockIdx.x enumerates the tiles of the lower triangle row by row), 4x4 outputs per thread.
+// Workgroup 0, whose tile is the next diagonal block, goes on to factor it.
+__global__ __launch_bounds__(DENSE_THREADS) void k_lfact_update(LargeDense d, uint32_t kb) {
+  __shared__ double smem[DIAG_LDS];
+  auto Pi = reinterpret_cast<double(*)[LNB + 1]>(smem);
+  auto Pj = reinterpret_cast<double(*)[LNB + 1]>(smem + LNB * (LNB + 1));
+  const bool skip = large_skip(d.state, 0);
+  const uint32_t n = d.n, tid = threadIdx.x;
+  double* A = d.A;
+  uint32_t ti, tj;
+  lower_entry(blockIdx.x, ti, tj);
+  const uint32_t ib = kb + LNB + LNB * ti, jb = kb + LNB + LNB * tj;
+  const uint32_t i0 = (tid / 16) * 4, j0 = (tid % 16) * 4;
+  // all loads in flight at once: the two panel tiles (thread tid: column tid % 64 of rows
+  // tid / 64 + 4 r) and the thread's 4x4 outputs (entries above the diagonal of a diagonal tile
+  // are read and left unchanged)
+  double pa[LNB / 4], pb[LNB / 4], out[4][4];
+  const uint32_t sr = tid / LNB, sc = tid % LNB;
+#pragma unroll
+  for (int r = 0; r < LNB / 4; r++) {
+    pa[r] = A[(size_t)(ib + sr + 4 * r) * n + kb + sc];
+    pb[r] = A[(size_t)(jb + sr + 4 * r) * n + kb + sc];
+  }
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int c = 0; c < 4; c++) out[a][c] = A[(size_t)(ib + i0 + a) * n + jb + j0 + c];
+#pragma unroll
+  for (int r = 0; r < LNB / 4; r++) {
+    settle(pa[r]);
+    settle(pb[r]);
+  }
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int c = 0; c < 4; c++) settle(out[a][c]);
+  if (skip) return;
+#pragma unroll
+  for (int r = 0; r < LNB / 4; r++) {
+    Pi[sr + 4 * r][sc] = pa[r];
+    Pj[sr + 4 * r][sc] = pb[r];
+  }
+  __syncthreads();
+  double acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int c = 0; c < 4; c++) acc[a][c] = 0;
+#pragma unroll 4
   for (int t = 0; t < LNB; t++) {
     double x[4], y[4];
 #pragma unroll
@@ -584,40 +864,44 @@ __global__ __launch_bounds__(DENSE_THREADS) void k_lfact_update(Problem q, uint3
 #pragma unroll
   for (int a = 0; a < 4; a++)
 #pragma unroll
+    for (int c = 0; c < 4; c++) {
+      out[a][c] -= acc[a][c];
+      if (ti != tj || j0 + c <= i0 + a) A[(size_t)(ib + i0 + a) * n + jb + j0 + c] = out[a][c];
+    }
+  if (blockIdx.x != 0) return;
+  // tile (0, 0) is the next panel's diagonal block, final now: this workgroup factors it, its lower
+  // triangle mirrored through LDS into the whole symmetric block the routine takes
+  __syncthreads();  // the panel tiles are read
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
     for (int c = 0; c < 4; c++)
-      if (ti != tj || j0 + c <= i0 + a) A[(size_t)(ib + i0 + a) * n + jb + j0 + c] -= acc[a][c];
+      if (j0 + c <= i0 + a) Pi[i0 + a][j0 + c] = Pi[j0 + c][i0 + a] = out[a][c];
+  __syncthreads();
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int c = 0; c < 4; c++) out[a][c] = Pi[i0 + a][j0 + c];
+  __syncthreads();
+  diag_block(d, kb + LNB, out, smem);
 }
 
-// Forward step of block kb: y_kb = L_kk^-1 w_kb into the vector after the array (workgroup 0),
+// Forward step of block kb of a correction: y_kb = X_kk w_kb into the forward vector (workgroup 0),
 // and w_i -= L_i,kb y_kb for the rows i below, w the array's last row.
-template <bool SHARED>
-__global__ __launch_bounds__(DENSE_THREADS) void k_lsolve_fwd(Problem q, uint32_t b, uint32_t kb,
-                                                               int mode) {
-  __shared__ double L[LNB][LNB + 1];
-  __shared__ double y[LNB];
-  const double* s = bstate(q, b);
-  if (large_skip(s, mode)) return;
-  const Bundle& g = q.bundles[b];
-  const uint32_t n = g.dense_n, u = n - 1, tid = threadIdx.x;
-  double* A = q.dense + g.dense_off;
+__global__ __launch_bounds__(DENSE_THREADS) void k_lsolve_fwd(LargeDense d, uint32_t kb) {
+  __shared__ double wk[LNB], y[LNB];
+  const bool skip = large_skip(d.state, 1);
+  const uint32_t n = d.n, u = n - 1, tid = threadIdx.x;
+  double* A = d.A;
   double* w = A + (size_t)u * n;
-  double* yv = A + (size_t)n * n;
-  for (uint32_t e = tid; e < LNB * LNB; e += DENSE_THREADS) {
-    const uint32_t i = e / LNB, j = e % LNB;
-    L[i][j] = j <= i ? A[(size_t)(kb + i) * n + kb + j] : 0.0;
-  }
-  if (tid < (uint32_t)LNB) y[tid] = w[kb + tid];
+  double wv = w[kb + tid % LNB];
+  settle(wv);
+  if (skip) return;
+  if (tid < (uint32_t)LNB) wk[tid] = wv;
   __syncthreads();
-  if (tid < 64) {
-    for (uint32_t j = 0; j < (uint32_t)LNB; j++) {
-      if (tid == j) y[j] /= L[j][j];
-      wave_sync();
-      if (tid > j) y[tid] -= L[tid][j] * y[j];
-      wave_sync();
-    }
-  }
+  block_matvec<false, LNB>(large_inverse(d, kb), wk, y);
   __syncthreads();
-  if (blockIdx.x == 0 && tid < (uint32_t)LNB) yv[kb + tid] = y[tid];
+  if (blockIdx.x == 0 && tid < (uint32_t)LNB) large_vector(d)[kb + tid] = y[tid];
   const uint32_t i = kb + LNB + blockIdx.x * DENSE_THREADS + tid;
   if (i >= u) return;
   const double* row = A + (size_t)i * n + kb;
@@ -627,41 +911,36 @@ __global__ __launch_bounds__(DENSE_THREADS) void k_lsolve_fwd(Problem q, uint32_
   w[i] -= sum;
 }
 
-// Backward step of block kb: x_kb = L_kk^-T y_kb into w (workgroup 0), and y_i -= L_kb,i^T x_kb for
-// the rows i above.
-template <bool SHARED>
-__global__ __launch_bounds__(DENSE_THREADS) void k_lsolve_bwd(Problem q, uint32_t b, uint32_t kb,
+// Backward step of block kb: x_kb = X_kk^T y_kb into w (workgroup 0), and y_i -= L_kb,i^T x_kb for
+// the rows i above, y the forward vector.
+__global__ __launch_bounds__(DENSE_THREADS) void k_lsolve_bwd(LargeDense d, uint32_t kb,
                                                                int mode) {
-  __shared__ double L[LNB][LNB + 1];
-  __shared__ double x[LNB];
-  const double* s = bstate(q, b);
-  if (large_skip(s, mode)) return;
-  const Bundle& g = q.bundles[b];
-  const uint32_t n = g.dense_n, u = n - 1, tid = threadIdx.x;
-  double* A = q.dense + g.dense_off;
+  __shared__ double yk[LNB], x[LNB];
+  const bool skip = large_skip(d.state, mode);
+  const uint32_t n = d.n, u = n - 1, tid = threadIdx.x;
+  double* A = d.A;
   double* w = A + (size_t)u * n;
-  double* yv = A + (size_t)n * n;
-  for (uint32_t e = tid; e < LNB * LNB; e += DENSE_THREADS) {
-    const uint32_t i = e / LNB, j = e % LNB;
-    L[i][j] = j <= i ? A[(size_t)(kb + i) * n + kb + j] : 0.0;
-  }
-  if (tid < (uint32_t)LNB) x[tid] = yv[kb + tid];
+  double* yv = large_vector(d);
+  // the thread's column of the block row, loaded with the rest before the first wait
+  const uint32_t i = blockIdx.x * DENSE_THREADS + tid;
+  const uint32_t ic = min(i, kb ? kb - 1 : 0u);
+  double col[LNB];
+#pragma unroll
+  for (int t = 0; t < LNB; t++) col[t] = A[(size_t)(kb + t) * n + ic];
+  double yvk = yv[kb + tid % LNB];
+#pragma unroll
+  for (int t = 0; t < LNB; t++) settle(col[t]);
+  settle(yvk);
+  if (skip) return;
+  if (tid < (uint32_t)LNB) yk[tid] = yvk;
   __syncthreads();
-  if (tid < 64) {
-    for (int j = LNB - 1; j >= 0; j--) {
-      if (tid == (uint32_t)j) x[j] /= L[j][j];
-      wave_sync();
-      if (tid < (uint32_t)j) x[tid] -= L[j][tid] * x[j];
-      wave_sync();
-    }
-  }
+  block_matvec<true, LNB>(large_inverse(d, kb), yk, x);
   __syncthreads();
   if (blockIdx.x == 0 && tid < (uint32_t)LNB) w[kb + tid] = x[tid];
-  const uint32_t i = blockIdx.x * DENSE_THREADS + tid;
   if (i >= kb) return;
   double sum = 0;
-#pragma unroll 8
-  for (int t = 0; t < LNB; t++) sum += A[(size_t)(kb + t) * n + i] * x[t];
+#pragma unroll
+  for (int t = 0; t < LNB; t++) sum += col[t] * x[t];
   yv[i] -= sum;
 }
 

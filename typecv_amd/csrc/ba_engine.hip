@@ -57,6 +57,7 @@ struct tcv_ba_context {
       rhs, grad, camS, camG, pfc, dense, direct_bundles, direct_cams, covis, small_bundles;
   // the direct bundles factored by a sequence of launches, and their dense array sizes
   std::vector<uint32_t> large_bundles, large_n;
+  std::vector<uint64_t> large_off;
   std::vector<hipEvent_t> events;
   tcv_ba_context() {
     all = {&scene,      &trial,        &index,     &out,       &bundles,    &tiles,
@@ -175,6 +176,7 @@ void plan_direct(Plan& plan, bool want, uint32_t shared, bool exact,
     g.covis_off = plan.covis_words;
     if (!g.direct) continue;
     plan.dense_words += (uint64_t)g.dense_n * g.dense_n + g.dense_n;
+    if (g.large) plan.dense_words += (uint64_t)(g.dense_n - 1) * DENSE_LNB;  // the X_kk blocks
     plan.covis_words += cams * ((cams + 31) / 32);
     plan.direct_bundles.push_back(b);
     (g.large ? plan.large_bundles : plan.small_bundles).push_back(b);
@@ -384,37 +386,41 @@ struct Launch {
     if (q.n_small)
       hipLaunchKernelGGL((k_dense_factor<SHARED>), dim3(q.n_small), dim3(DENSE_THREADS), 0, s, q);
   }
-  // A large bundle (dense array n x n): the factorisation, panel by panel.
-  static void large_factor(const Problem& q, uint32_t b, uint32_t n, hipStream_t s) {
+  // A large bundle (dense array n x n at word `off`): the factorisation, panel by panel.
+  static void large_factor(const Problem& q, uint32_t b, uint32_t n, uint64_t off, hipStream_t s) {
     const uint32_t u = n - 1;
+    const LargeDense d{q.dense + off, q.state + (size_t)b * S_WORDS, n};
     hipLaunchKernelGGL((k_lfact_begin<SHARED>), dim3(64), dim3(DENSE_THREADS), 0, s, q, b);
+    // the first diagonal block; each later one is factored by the update launch before it
+    hipLaunchKernelGGL(k_lfact_diag, dim3(1), dim3(DENSE_THREADS), 0, s, d, 0u);
     for (uint32_t kb = 0; kb < u; kb += DENSE_LNB) {
-      hipLaunchKernelGGL((k_lfact_diag<SHARED>), dim3(1), dim3(64), 0, s, q, b, kb);
-      const uint32_t rows = u - kb - DENSE_LNB;
+      // the panel's launch also carries the forward substitution of block kb (the last one too)
+      const uint32_t rows = u - kb - DENSE_LNB, t = rows / DENSE_LNB;
+      hipLaunchKernelGGL(k_lfact_panel, dim3(t ? t : 1), dim3(DENSE_THREADS), 0, s, d, kb);
       if (!rows) continue;
-      hipLaunchKernelGGL((k_lfact_panel<SHARED>), dim3(blocks_for(rows, DENSE_THREADS)),
-                         dim3(DENSE_THREADS), 0, s, q, b, kb);
-      const uint32_t t = rows / DENSE_LNB;
-      hipLaunchKernelGGL((k_lfact_update<SHARED>), dim3(t * (t + 1) / 2), dim3(DENSE_THREADS), 0,
-                         s, q, b, kb);
+      hipLaunchKernelGGL(k_lfact_update, dim3(t * (t + 1) / 2), dim3(DENSE_THREADS), 0, s, d, kb);
     }
   }
-  // The substitutions of a large bundle: mode 0 solves the assembled right-hand side into x, mode
-  // 1 adds the correction S^-1 r.
-  static void large_solve(const Problem& q, uint32_t b, uint32_t n, int mode, hipStream_t s) {
+  // The substitutions of a large bundle: mode 0 finishes the solve of the assembled right-hand
+  // side (its forward substitution ran with the factorisation) into x, mode 1 adds the correction
+  // S^-1 r.
+  static void large_solve(const Problem& q, uint32_t b, uint32_t n, uint64_t off, int mode,
+                          hipStream_t s) {
     const uint32_t u = n - 1;
-    if (mode == 1)
+    const LargeDense d{q.dense + off, q.state + (size_t)b * S_WORDS, n};
+    if (mode == 1) {
       hipLaunchKernelGGL((k_lsolve_in<SHARED>), dim3(blocks_for(u, DENSE_THREADS)),
                          dim3(DENSE_THREADS), 0, s, q, b);
-    for (uint32_t kb = 0; kb < u; kb += DENSE_LNB) {
-      const uint32_t rows = u - kb - DENSE_LNB;
-      hipLaunchKernelGGL((k_lsolve_fwd<SHARED>), dim3(rows ? blocks_for(rows, DENSE_THREADS) : 1),
-                         dim3(DENSE_THREADS), 0, s, q, b, kb, mode);
+      for (uint32_t kb = 0; kb < u; kb += DENSE_LNB) {
+        const uint32_t rows = u - kb - DENSE_LNB;
+        hipLaunchKernelGGL(k_lsolve_fwd, dim3(rows ? blocks_for(rows, DENSE_THREADS) : 1),
+                           dim3(DENSE_THREADS), 0, s, d, kb);
+      }
     }
     for (uint32_t kb = u; kb > 0;) {
       kb -= DENSE_LNB;
-      hipLaunchKernelGGL((k_lsolve_bwd<SHARED>), dim3(kb ? blocks_for(kb, DENSE_THREADS) : 1),
-                         dim3(DENSE_THREADS), 0, s, q, b, kb, mode);
+      hipLaunchKernelGGL(k_lsolve_bwd, dim3(kb ? blocks_for(kb, DENSE_THREADS) : 1),
+                         dim3(DENSE_THREADS), 0, s, d, kb, mode);
     }
     // mode 0 clears word 19, which every workgroup of the launch tests first: one workgroup
     hipLaunchKernelGGL((k_lsolve_out<SHARED>), dim3(mode == 0 ? 1 : blocks_for(u, DENSE_THREADS)),
@@ -473,8 +479,8 @@ struct Ops {
   void (*unstored_point_sweep)(const Problem&, hipStream_t);
   void (*unstored_camera_sweep)(const Problem&, hipStream_t);
   void (*unstored_update)(const Problem&, hipStream_t);
-  void (*large_factor)(const Problem&, uint32_t, uint32_t, hipStream_t);
-  void (*large_solve)(const Problem&, uint32_t, uint32_t, int, hipStream_t);
+  void (*large_factor)(const Problem&, uint32_t, uint32_t, uint64_t, hipStream_t);
+  void (*large_solve)(const Problem&, uint32_t, uint32_t, uint64_t, int, hipStream_t);
 };
 
 // The PCG sweeps of one iteration over the stored and the unstored bundles.
@@ -736,7 +742,11 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
   }
   ctx->large_bundles = plan.large_bundles;
   ctx->large_n.clear();
-  for (uint32_t b : plan.large_bundles) ctx->large_n.push_back(plan.bundles[b].dense_n);
+  ctx->large_off.clear();
+  for (uint32_t b : plan.large_bundles) {
+    ctx->large_n.push_back(plan.bundles[b].dense_n);
+    ctx->large_off.push_back(plan.bundles[b].dense_off);
+  }
   if (q.n_direct) {
     // which cameras of a direct bundle share points: the assembly skips the pairs that do not
     HIP_TRY(hipMemsetAsync(q.covis, 0, (size_t)plan.covis_words * 4, st));
@@ -816,8 +826,8 @@ int solve_linear(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q,
   if (!q.n_direct) return run_pcg(ctx, c, q, ops, st, rep, first, budget);
   ops.dense_solve(q, st);
   for (size_t k = 0; k < ctx->large_bundles.size(); k++) {
-    ops.large_factor(q, ctx->large_bundles[k], ctx->large_n[k], st);
-    ops.large_solve(q, ctx->large_bundles[k], ctx->large_n[k], 0, st);
+    ops.large_factor(q, ctx->large_bundles[k], ctx->large_n[k], ctx->large_off[k], st);
+    ops.large_solve(q, ctx->large_bundles[k], ctx->large_n[k], ctx->large_off[k], 0, st);
   }
   for (uint32_t k = 0; k < kDirectPasses; k++) {
     ops.certify(q, k + 1 == kDirectPasses, tol, st);
@@ -829,7 +839,7 @@ int solve_linear(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q,
     ops.dense_correct(q, st);
     for (size_t j = 0; j < ctx->large_bundles.size(); j++)
       if (word(ctx, ctx->large_bundles[j], S_DIRECT) == 1)
-        ops.large_solve(q, ctx->large_bundles[j], ctx->large_n[j], 1, st);
+        ops.large_solve(q, ctx->large_bundles[j], ctx->large_n[j], ctx->large_off[j], 1, st);
   }
   if (!any_word(ctx, q, S_PCG_ACTIVE)) return TCV_BA_OK;
   return run_pcg(ctx, c, q, ops, st, rep, first, budget);
