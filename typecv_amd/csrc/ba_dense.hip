@@ -85,22 +85,6 @@ __device__ __forceinline__ void pair_block(const double* camG, uint32_t a, uint3
       acc[j * 6 + l] += x[j * 3] * y[l * 3] + x[j * 3 + 1] * y[l * 3 + 1] + x[j * 3 + 2] * y[l * 3 + 2];
 }
 
-// The workgroup's logical index such that each of the 8 XCDs (which receive workgroups round-robin
-// in dispatch order) runs one contiguous range of the grid: the rows of one bundle then share that
-// XCD's L2, where the bundle's G rows are read once per row camera.
-__device__ __forceinline__ uint32_t xcd_contiguous(uint32_t i, uint32_t n) {
-#ifndef TCV_XCD_REMAP
-#define TCV_XCD_REMAP 1
-#endif
-#if TCV_XCD_REMAP
-  const uint32_t q = n / 8, r = n % 8, x = i % 8, k = i / 8;
-  return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
-#else
-  (void)n;
-  return i;
-#endif
-}
-
 // Whether cameras lc and ld of bundle g (local indices) observe a common point: bit ld of row lc of
 // the bundle's covisibility bitmap (k_covisibility).
 __device__ __forceinline__ bool covisible(const Problem& q, const Bundle& g, uint32_t lc,
@@ -724,7 +708,28 @@ __device__ __forceinline__ void block_matvec(const double* X, const double* v, d
   if (part == 0) out[o] = sum;
 }
 
-// The panel below block kb as P = A X_kk^T (a workgroup per 64 rows, 4x4 outputs per thread), and
+typedef double dense_d4 __attribute__((ext_vector_type(4)));
+
+// acc[b] += the 16 x 16 block (rows 16 w.., columns 16 b..) of Am Bm^T over K = 64, for wave w of
+// the workgroup, by FP64 matrix-core products (16x16x4): Am, Bm are 64 x 64 in LDS (row stride
+// LNB + 1). Operand lane l holds row l % 16, column l / 16 of each 16 x 4 slice; register e of
+// acc[b] in lane l is entry (16 w + l / 16 + 4 e, 16 b + l % 16).
+__device__ __forceinline__ void strip_product(const double (*Am)[LNB + 1],
+                                              const double (*Bm)[LNB + 1], dense_d4 acc[4]) {
+  const uint32_t lane = threadIdx.x % 64, w = threadIdx.x / 64, r = lane % 16, kq = lane / 16;
+#pragma unroll 4
+  for (int kk = 0; kk < LNB / 4; kk++) {
+    const double a = Am[16 * w + r][4 * kk + kq];
+    double b[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) b[c] = Bm[16 * c + r][4 * kk + kq];
+#pragma unroll
+    for (int c = 0; c < 4; c++) acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[c], acc[c], 0, 0, 0);
+  }
+}
+
+// The panel below block kb as P = A X_kk^T (a workgroup per 64 rows, a 16-row strip per wave on
+// the matrix cores), and
 // the forward substitution of block kb: y_kb = X_kk w_kb (workgroup 0 stores it; the last block
 // has no rows below and runs only this), w_i -= P_i . y_kb for the workgroup's rows.
 __global__ __launch_bounds__(DENSE_THREADS) void k_lfact_panel(LargeDense d, uint32_t kb) {
@@ -766,41 +771,29 @@ __global__ __launch_bounds__(DENSE_THREADS) void k_lfact_panel(LargeDense d, uin
   __syncthreads();
   if (blockIdx.x == 0 && tid < (uint32_t)LNB) large_vector(d)[kb + tid] = y[tid];
   if (!rows) return;
-  const uint32_t i0 = (tid / 16) * 4, j0 = (tid % 16) * 4;
-  double acc[4][4];
+  dense_d4 acc[4];
 #pragma unroll
-  for (int a = 0; a < 4; a++)
+  for (int c = 0; c < 4; c++) acc[c] = dense_d4{0, 0, 0, 0};
+  strip_product(Ps, Xs, acc);
+  const uint32_t lane = tid % 64, row0 = 16 * (tid / 64) + lane / 16, col = lane % 16;
 #pragma unroll
-    for (int c = 0; c < 4; c++) acc[a][c] = 0;
-#pragma unroll 4
-  for (int t = 0; t < LNB; t++) {
-    double xa[4], xb[4];
-#pragma unroll
-    for (int a = 0; a < 4; a++) {
-      xa[a] = Ps[i0 + a][t];
-      xb[a] = Xs[j0 + a][t];
-    }
-#pragma unroll
-    for (int a = 0; a < 4; a++)
-#pragma unroll
-      for (int c = 0; c < 4; c++) acc[a][c] += xa[a] * xb[c];
-  }
-#pragma unroll
-  for (int a = 0; a < 4; a++) {
+  for (int e = 0; e < 4; e++) {
+    const uint32_t i = row0 + 4 * e;
     double part = 0;
 #pragma unroll
     for (int c = 0; c < 4; c++) {
-      A[(size_t)(r0 + i0 + a) * n + kb + j0 + c] = acc[a][c];
-      part += acc[a][c] * y[j0 + c];
+      A[(size_t)(r0 + i) * n + kb + 16 * c + col] = acc[c][e];
+      part += acc[c][e] * y[16 * c + col];
     }
-    // the 16 threads of row i0 + a hold its 64 columns: lanes tid % 16 of one 16-lane group
+    // the 16 lanes of row i (one 16-lane group) hold its 64 columns
     for (int o = 8; o > 0; o >>= 1) part += __shfl_xor(part, o, 64);
-    if (j0 == 0) w[r0 + i0 + a] -= part;
+    if (col == 0) w[r0 + i] -= part;
   }
 }
 
 // The trailing lower triangle less the panel's outer products: a workgroup per 64x64 tile
-// (blockIdx.x enumerates the tiles of the lower triangle row by row), 4x4 outputs per thread.
+// (blockIdx.x enumerates the tiles of the lower triangle row by row), a 16-row strip per wave on
+// the matrix cores.
 // Workgroup 0, whose tile is the next diagonal block, goes on to factor it.
 __global__ __launch_bounds__(DENSE_THREADS) void k_lfact_update(LargeDense d, uint32_t kb) {
   __shared__ double smem[DIAG_LDS];
@@ -812,9 +805,10 @@ __global__ __launch_bounds__(DENSE_THREADS) void k_lfact_update(LargeDense d, ui
   uint32_t ti, tj;
   lower_entry(blockIdx.x, ti, tj);
   const uint32_t ib = kb + LNB + LNB * ti, jb = kb + LNB + LNB * tj;
-  const uint32_t i0 = (tid / 16) * 4, j0 = (tid % 16) * 4;
+  // the thread's outputs in the matrix-core layout: out[c][e] is entry (row0 + 4 e, 16 c + col)
+  const uint32_t lane = tid % 64, row0 = 16 * (tid / 64) + lane / 16, col = lane % 16;
   // all loads in flight at once: the two panel tiles (thread tid: column tid % 64 of rows
-  // tid / 64 + 4 r) and the thread's 4x4 outputs (entries above the diagonal of a diagonal tile
+  // tid / 64 + 4 r) and the thread's 16 outputs (entries above the diagonal of a diagonal tile
   // are read and left unchanged)
   double pa[LNB / 4], pb[LNB / 4], out[4][4];
   const uint32_t sr = tid / LNB, sc = tid % LNB;
@@ -824,18 +818,18 @@ __global__ __launch_bounds__(DENSE_THREADS) void k_lfact_update(LargeDense d, ui
     pb[r] = A[(size_t)(jb + sr + 4 * r) * n + kb + sc];
   }
 #pragma unroll
-  for (int a = 0; a < 4; a++)
+  for (int c = 0; c < 4; c++)
 #pragma unroll
-    for (int c = 0; c < 4; c++) out[a][c] = A[(size_t)(ib + i0 + a) * n + jb + j0 + c];
+    for (int e = 0; e < 4; e++) out[c][e] = A[(size_t)(ib + row0 + 4 * e) * n + jb + 16 * c + col];
 #pragma unroll
   for (int r = 0; r < LNB / 4; r++) {
     settle(pa[r]);
     settle(pb[r]);
   }
 #pragma unroll
-  for (int a = 0; a < 4; a++)
+  for (int c = 0; c < 4; c++)
 #pragma unroll
-    for (int c = 0; c < 4; c++) settle(out[a][c]);
+    for (int e = 0; e < 4; e++) settle(out[c][e]);
   if (skip) return;
 #pragma unroll
   for (int r = 0; r < LNB / 4; r++) {
@@ -843,47 +837,38 @@ __global__ __launch_bounds__(DENSE_THREADS) void k_lfact_update(LargeDense d, ui
     Pj[sr + 4 * r][sc] = pb[r];
   }
   __syncthreads();
-  double acc[4][4];
+  dense_d4 acc[4];
 #pragma unroll
-  for (int a = 0; a < 4; a++)
+  for (int c = 0; c < 4; c++) acc[c] = dense_d4{0, 0, 0, 0};
+  strip_product(Pi, Pj, acc);
 #pragma unroll
-    for (int c = 0; c < 4; c++) acc[a][c] = 0;
-#pragma unroll 4
-  for (int t = 0; t < LNB; t++) {
-    double x[4], y[4];
+  for (int c = 0; c < 4; c++)
 #pragma unroll
-    for (int a = 0; a < 4; a++) {
-      x[a] = Pi[i0 + a][t];
-      y[a] = Pj[j0 + a][t];
-    }
-#pragma unroll
-    for (int a = 0; a < 4; a++)
-#pragma unroll
-      for (int c = 0; c < 4; c++) acc[a][c] += x[a] * y[c];
-  }
-#pragma unroll
-  for (int a = 0; a < 4; a++)
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-      out[a][c] -= acc[a][c];
-      if (ti != tj || j0 + c <= i0 + a) A[(size_t)(ib + i0 + a) * n + jb + j0 + c] = out[a][c];
+    for (int e = 0; e < 4; e++) {
+      const uint32_t i = row0 + 4 * e, j = 16 * c + col;
+      out[c][e] -= acc[c][e];
+      if (ti != tj || j <= i) A[(size_t)(ib + i) * n + jb + j] = out[c][e];
     }
   if (blockIdx.x != 0) return;
   // tile (0, 0) is the next panel's diagonal block, final now: this workgroup factors it, its lower
-  // triangle mirrored through LDS into the whole symmetric block the routine takes
+  // triangle mirrored through LDS into the whole symmetric block in the routine's layout
   __syncthreads();  // the panel tiles are read
 #pragma unroll
-  for (int a = 0; a < 4; a++)
+  for (int c = 0; c < 4; c++)
 #pragma unroll
-    for (int c = 0; c < 4; c++)
-      if (j0 + c <= i0 + a) Pi[i0 + a][j0 + c] = Pi[j0 + c][i0 + a] = out[a][c];
+    for (int e = 0; e < 4; e++) {
+      const uint32_t i = row0 + 4 * e, j = 16 * c + col;
+      if (j <= i) Pi[i][j] = Pi[j][i] = out[c][e];
+    }
   __syncthreads();
+  const uint32_t i0 = (tid / 16) * 4, j0 = (tid % 16) * 4;
+  double a4[4][4];
 #pragma unroll
-  for (int a = 0; a < 4; a++)
+  for (int x = 0; x < 4; x++)
 #pragma unroll
-    for (int c = 0; c < 4; c++) out[a][c] = Pi[i0 + a][j0 + c];
+    for (int y = 0; y < 4; y++) a4[x][y] = Pi[i0 + x][j0 + y];
   __syncthreads();
-  diag_block(d, kb + LNB, out, smem);
+  diag_block(d, kb + LNB, a4, smem);
 }
 
 // Forward step of block kb of a correction: y_kb = X_kk w_kb into the forward vector (workgroup 0),

@@ -184,7 +184,7 @@ void plan_direct(Plan& plan, bool want, uint32_t shared, bool exact,
   }
 }
 
-constexpr uint32_t kTaskObs = 1024;
+constexpr uint32_t kTaskObs = TASK_OBS;
 constexpr uint32_t kChunk = 8;  // PCG iterations between reads of the state words
 constexpr uint32_t kTimingStride = 8;  // TCV_BA_FLAG_TIMING times every 8th Schur product
 constexpr uint32_t kMaxRefinements = 4;  // mixed form: refinement restarts per LM iteration

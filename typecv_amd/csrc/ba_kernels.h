@@ -68,6 +68,7 @@ constexpr int TILE = 64;  // observations per wave tile
 #endif
 constexpr int TILE_WAVES = TCV_TILE_WAVES;  // wave tiles per workgroup of the point-tile kernels
 constexpr int TASK_THREADS = 256;  // threads of a camera-task workgroup
+constexpr int TASK_OBS = 1024;     // most slots of a camera-list task (4 per thread)
 constexpr int ADV_THREADS = 256;   // threads of a PCG-step workgroup
 constexpr int GROUP_CAMS = 42;     // cameras of a PCG-step workgroup (6 rows each)
 constexpr int CG_THREADS = 1024;   // threads of the per-bundle PCG kernels

@@ -84,6 +84,23 @@ __device__ __forceinline__ void block_sum2(double& a, double& b, double* lds) {
   b = sb;
 }
 
+// The workgroup's logical index such that each of the 8 XCDs (which receive workgroups round-robin
+// in dispatch order) runs one contiguous range of the grid: workgroups of neighbouring cameras then
+// share that XCD's L2, where the vectors they gather (y by point in the camera sweep, a bundle's G
+// rows in the dense assembly) stay resident.
+__device__ __forceinline__ uint32_t xcd_contiguous(uint32_t i, uint32_t n) {
+#ifndef TCV_XCD_REMAP
+#define TCV_XCD_REMAP 1
+#endif
+#if TCV_XCD_REMAP
+  const uint32_t q = n / 8, r = n % 8, x = i % 8, k = i / 8;
+  return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+#else
+  (void)n;
+  return i;
+#endif
+}
+
 // Orders one wave's LDS writes before its later LDS reads. The point-tile kernels give every wave
 // its own LDS slice and their loops have per-wave trip counts, so they synchronise per wave, never
 // per workgroup.
@@ -1044,7 +1061,7 @@ template <class WT, bool SHARED, bool CERTIFY = false>
 __global__ __launch_bounds__(TASK_THREADS) void k_camera_sweep(Problem q) {
   constexpr int NV = SHARED ? 7 : 6;
   __shared__ double red[TASK_THREADS / 64][NV];
-  const uint32_t t = blockIdx.x;
+  const uint32_t t = xcd_contiguous(blockIdx.x, gridDim.x);
   const uint32_t s = q.tasks[4 * t + 1], e = q.tasks[4 * t + 2], b = q.tasks[4 * t + 3];
   if (CERTIFY ? bstate(q, b)[S_DIRECT] != 1 : bstate(q, b)[S_PCG_ACTIVE] == 0) return;
   if (!stores_w(q.bundles[b])) return;
@@ -1052,14 +1069,34 @@ __global__ __launch_bounds__(TASK_THREADS) void k_camera_sweep(Problem q) {
   double acc[NV];
 #pragma unroll
   for (int k = 0; k < NV; k++) acc[k] = 0;
-  for (uint32_t n = s + threadIdx.x; n < e; n += TASK_THREADS) {
-    const uint32_t p = q.cam_point[n];
-    const double y0 = y[(size_t)p * 3], y1 = y[(size_t)p * 3 + 1], y2 = y[(size_t)p * 3 + 2];
+  // the thread's (at most) 4 slots with no branch around a load: the point ids first, then the
+  // gathered y, so that each round of loads is in flight at once; a slot past the task's end
+  // repeats its last slot with y scaled to zero (an exact zero term). A camera without
+  // observations has one empty task.
+  constexpr int PER = TASK_OBS / TASK_THREADS;
+  if (e > s) {
+    uint32_t slot[PER], pt[PER];
 #pragma unroll
-    for (int j = 0; j < NV; j++)
-      acc[j] += wload(wptr<WT, SHARED>(q.Wc, j * 3, n, q.M)) * y0 +
-                wload(wptr<WT, SHARED>(q.Wc, j * 3 + 1, n, q.M)) * y1 +
-                wload(wptr<WT, SHARED>(q.Wc, j * 3 + 2, n, q.M)) * y2;
+    for (int k = 0; k < PER; k++) {
+      slot[k] = min(s + threadIdx.x + TASK_THREADS * k, e - 1);
+      pt[k] = q.cam_point[slot[k]];
+    }
+    double yv[PER][3];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      const double f = s + threadIdx.x + TASK_THREADS * k < e ? 1.0 : 0.0;
+#pragma unroll
+      for (int c = 0; c < 3; c++) yv[k][c] = y[(size_t)pt[k] * 3 + c] * f;
+    }
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      const uint32_t n = slot[k];
+#pragma unroll
+      for (int j = 0; j < NV; j++)
+        acc[j] += wload(wptr<WT, SHARED>(q.Wc, j * 3, n, q.M)) * yv[k][0] +
+                  wload(wptr<WT, SHARED>(q.Wc, j * 3 + 1, n, q.M)) * yv[k][1] +
+                  wload(wptr<WT, SHARED>(q.Wc, j * 3 + 2, n, q.M)) * yv[k][2];
+    }
   }
 #pragma unroll
   for (int k = 0; k < NV; k++)
