@@ -791,10 +791,36 @@ __global__ __launch_bounds__(DENSE_THREADS) void k_lfact_panel(LargeDense d, uin
   }
 }
 
+// Tile (ti, tj) of logical index L over the T x T lower triangle of tiles, in the order of 16 x 16
+// super-tiles (row-major over the lower triangle of super-tiles, row-major inside each): a run of
+// consecutive indices, which one XCD receives (xcd_contiguous), works through one super-tile at a
+// time and finds its 16 + 16 panel tiles in that XCD's L2.
+constexpr uint32_t SUPER_TILE = 16;
+__device__ __forceinline__ void super_tile(uint32_t L, uint32_t T, uint32_t& ti, uint32_t& tj) {
+  for (uint32_t I = 0;; I++) {
+    const uint32_t r0 = I * SUPER_TILE, h = min(SUPER_TILE, T - r0);
+    const uint32_t full = h * SUPER_TILE * I, row = full + h * (h + 1) / 2;
+    if (L < row) {
+      if (L < full) {
+        const uint32_t rem = L % (h * SUPER_TILE);
+        ti = r0 + rem / SUPER_TILE;
+        tj = L / (h * SUPER_TILE) * SUPER_TILE + rem % SUPER_TILE;
+      } else {
+        uint32_t r, c;
+        lower_entry(L - full, r, c);
+        ti = r0 + r;
+        tj = r0 + c;
+      }
+      return;
+    }
+    L -= row;
+  }
+}
+
 // The trailing lower triangle less the panel's outer products: a workgroup per 64x64 tile
-// (blockIdx.x enumerates the tiles of the lower triangle row by row), a 16-row strip per wave on
-// the matrix cores.
-// Workgroup 0, whose tile is the next diagonal block, goes on to factor it.
+// (super_tile order), a 16-row strip per wave on the matrix cores.
+// The workgroup of tile (0, 0), the next diagonal block (blockIdx 0, dispatched first), goes on to
+// factor it.
 __global__ __launch_bounds__(DENSE_THREADS) void k_lfact_update(LargeDense d, uint32_t kb) {
   __shared__ double smem[DIAG_LDS];
   auto Pi = reinterpret_cast<double(*)[LNB + 1]>(smem);
@@ -803,7 +829,7 @@ __global__ __launch_bounds__(DENSE_THREADS) void k_lfact_update(LargeDense d, ui
   const uint32_t n = d.n, tid = threadIdx.x;
   double* A = d.A;
   uint32_t ti, tj;
-  lower_entry(blockIdx.x, ti, tj);
+  super_tile(xcd_contiguous(blockIdx.x, gridDim.x), (n - 1 - kb - LNB) / LNB, ti, tj);
   const uint32_t ib = kb + LNB + LNB * ti, jb = kb + LNB + LNB * tj;
   // the thread's outputs in the matrix-core layout: out[c][e] is entry (row0 + 4 e, 16 c + col)
   const uint32_t lane = tid % 64, row0 = 16 * (tid / 64) + lane / 16, col = lane % 16;
@@ -849,7 +875,7 @@ __global__ __launch_bounds__(DENSE_THREADS) void k_lfact_update(LargeDense d, ui
       out[c][e] -= acc[c][e];
       if (ti != tj || j <= i) A[(size_t)(ib + i) * n + jb + j] = out[c][e];
     }
-  if (blockIdx.x != 0) return;
+  if (ti != 0 || tj != 0) return;
   // tile (0, 0) is the next panel's diagonal block, final now: this workgroup factors it, its lower
   // triangle mirrored through LDS into the whole symmetric block in the routine's layout
   __syncthreads();  // the panel tiles are read
