@@ -817,23 +817,32 @@ __device__ __forceinline__ void super_tile(uint32_t L, uint32_t T, uint32_t& ti,
   }
 }
 
-// The trailing lower triangle less the panel's outer products: a workgroup per 64x64 tile
-// (super_tile order), a 16-row strip per wave on the matrix cores.
-// The workgroup of tile (0, 0), the next diagonal block (blockIdx 0, dispatched first), goes on to
+// The trailing lower triangle from row and column base = kb + 64 panels less the outer products
+// of the `panels` (1 or 2) solved panels at columns kb..base-1: a workgroup per 64x64 tile
+// (super_tile order), a 16-row strip per wave on the matrix cores, the panels one after the
+// other into the same accumulators, so that each output tile is read and written once for both.
+// `column`: only the first tile column (rows base.., columns base..base+63), one panel. The
+// workgroup of tile (0, 0), the next diagonal block (blockIdx 0, dispatched first), goes on to
 // factor it.
-__global__ __launch_bounds__(DENSE_THREADS) void k_lfact_update(LargeDense d, uint32_t kb) {
+__global__ __launch_bounds__(DENSE_THREADS) void k_lfact_update(LargeDense d, uint32_t kb,
+                                                                 uint32_t panels, uint32_t column) {
   __shared__ double smem[DIAG_LDS];
   auto Pi = reinterpret_cast<double(*)[LNB + 1]>(smem);
   auto Pj = reinterpret_cast<double(*)[LNB + 1]>(smem + LNB * (LNB + 1));
   const bool skip = large_skip(d.state, 0);
-  const uint32_t n = d.n, tid = threadIdx.x;
+  const uint32_t n = d.n, tid = threadIdx.x, base = kb + LNB * panels;
   double* A = d.A;
   uint32_t ti, tj;
-  super_tile(xcd_contiguous(blockIdx.x, gridDim.x), (n - 1 - kb - LNB) / LNB, ti, tj);
-  const uint32_t ib = kb + LNB + LNB * ti, jb = kb + LNB + LNB * tj;
+  if (column) {
+    ti = xcd_contiguous(blockIdx.x, gridDim.x);
+    tj = 0;
+  } else {
+    super_tile(xcd_contiguous(blockIdx.x, gridDim.x), (n - 1 - base) / LNB, ti, tj);
+  }
+  const uint32_t ib = base + LNB * ti, jb = base + LNB * tj;
   // the thread's outputs in the matrix-core layout: out[c][e] is entry (row0 + 4 e, 16 c + col)
   const uint32_t lane = tid % 64, row0 = 16 * (tid / 64) + lane / 16, col = lane % 16;
-  // all loads in flight at once: the two panel tiles (thread tid: column tid % 64 of rows
+  // all loads in flight at once: the first panel's two tiles (thread tid: column tid % 64 of rows
   // tid / 64 + 4 r) and the thread's 16 outputs (entries above the diagonal of a diagonal tile
   // are read and left unchanged)
   double pa[LNB / 4], pb[LNB / 4], out[4][4];
@@ -857,16 +866,28 @@ __global__ __launch_bounds__(DENSE_THREADS) void k_lfact_update(LargeDense d, ui
 #pragma unroll
     for (int e = 0; e < 4; e++) settle(out[c][e]);
   if (skip) return;
-#pragma unroll
-  for (int r = 0; r < LNB / 4; r++) {
-    Pi[sr + 4 * r][sc] = pa[r];
-    Pj[sr + 4 * r][sc] = pb[r];
-  }
-  __syncthreads();
   dense_d4 acc[4];
 #pragma unroll
   for (int c = 0; c < 4; c++) acc[c] = dense_d4{0, 0, 0, 0};
-  strip_product(Pi, Pj, acc);
+  for (uint32_t p = 0; p < panels; p++) {
+    if (p) __syncthreads();  // the previous panel's tiles are read
+#pragma unroll
+    for (int r = 0; r < LNB / 4; r++) {
+      Pi[sr + 4 * r][sc] = pa[r];
+      Pj[sr + 4 * r][sc] = pb[r];
+    }
+    if (p + 1 < panels) {
+      // the next panel's tiles, in flight during this panel's products
+      const uint32_t pc = kb + LNB * (p + 1);
+#pragma unroll
+      for (int r = 0; r < LNB / 4; r++) {
+        pa[r] = A[(size_t)(ib + sr + 4 * r) * n + pc + sc];
+        pb[r] = A[(size_t)(jb + sr + 4 * r) * n + pc + sc];
+      }
+    }
+    __syncthreads();
+    strip_product(Pi, Pj, acc);
+  }
 #pragma unroll
   for (int c = 0; c < 4; c++)
 #pragma unroll
@@ -894,7 +915,7 @@ __global__ __launch_bounds__(DENSE_THREADS) void k_lfact_update(LargeDense d, ui
 #pragma unroll
     for (int y = 0; y < 4; y++) a4[x][y] = Pi[i0 + x][j0 + y];
   __syncthreads();
-  diag_block(d, kb + LNB, a4, smem);
+  diag_block(d, base, a4, smem);
 }
 
 // Forward step of block kb of a correction: y_kb = X_kk w_kb into the forward vector (workgroup 0),

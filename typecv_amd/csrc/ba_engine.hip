@@ -393,12 +393,24 @@ struct Launch {
     hipLaunchKernelGGL((k_lfact_begin<SHARED>), dim3(64), dim3(DENSE_THREADS), 0, s, q, b);
     // the first diagonal block; each later one is factored by the update launch before it
     hipLaunchKernelGGL(k_lfact_diag, dim3(1), dim3(DENSE_THREADS), 0, s, d, 0u);
-    for (uint32_t kb = 0; kb < u; kb += DENSE_LNB) {
+    // panels in pairs: the first panel updates only the second's column, and the trailing matrix
+    // takes both panels' products in one pass over its tiles
+    for (uint32_t kb = 0;;) {
+      const uint32_t t = (u - kb - DENSE_LNB) / DENSE_LNB;  // blocks below panel kb
       // the panel's launch also carries the forward substitution of block kb (the last one too)
-      const uint32_t rows = u - kb - DENSE_LNB, t = rows / DENSE_LNB;
       hipLaunchKernelGGL(k_lfact_panel, dim3(t ? t : 1), dim3(DENSE_THREADS), 0, s, d, kb);
-      if (!rows) continue;
-      hipLaunchKernelGGL(k_lfact_update, dim3(t * (t + 1) / 2), dim3(DENSE_THREADS), 0, s, d, kb);
+      if (!t) break;
+      if (t == 1) {
+        hipLaunchKernelGGL(k_lfact_update, dim3(1), dim3(DENSE_THREADS), 0, s, d, kb, 1u, 0u);
+        kb += DENSE_LNB;
+        continue;
+      }
+      hipLaunchKernelGGL(k_lfact_update, dim3(t), dim3(DENSE_THREADS), 0, s, d, kb, 1u, 1u);
+      hipLaunchKernelGGL(k_lfact_panel, dim3(t - 1), dim3(DENSE_THREADS), 0, s, d,
+                         kb + DENSE_LNB);
+      hipLaunchKernelGGL(k_lfact_update, dim3((t - 1) * t / 2), dim3(DENSE_THREADS), 0, s, d, kb,
+                         2u, 0u);
+      kb += 2 * DENSE_LNB;
     }
   }
   // The substitutions of a large bundle: mode 0 finishes the solve of the assembled right-hand
