@@ -187,15 +187,17 @@ __global__ __launch_bounds__(DENSE_THREADS) void k_dense_rows(Problem q) {
       for (uint32_t s = cc + tid; s < ce; s += DENSE_THREADS) table[s - cc] = cp[s];
     }
     __syncthreads();
-    const uint32_t nc = ce - cc;
-    for (uint32_t ld = 0; ld < lc; ld++) {
+    // wave w takes the earlier cameras ld = w, w + 4, ...: each block (c, d) is summed by one
+    // wave (lane-strided matches, then a butterfly), with no workgroup barrier per block
+    const uint32_t nc = ce - cc, wave = tid / 64, lane = tid % 64;
+    for (uint32_t ld = wave; ld < lc; ld += DENSE_THREADS / 64) {
       const uint32_t dlo = span[2 * ld], dhi = span[2 * ld + 1];
       if (dlo == dhi) continue;
       double acc[36];
 #pragma unroll
       for (int k = 0; k < 36; k++) acc[k] = 0;
-      int hit = 0;
-      for (uint32_t s = dlo + tid; s < dhi; s += DENSE_THREADS) {
+      bool hit = false;
+      for (uint32_t s = dlo + lane; s < dhi; s += 64) {
         const uint32_t p = cp[s];
         uint32_t sc = DENSE_NONE;
         if (tabled) {
@@ -212,12 +214,17 @@ __global__ __launch_bounds__(DENSE_THREADS) void k_dense_rows(Problem q) {
           if (lo < nc && table[lo] == p) sc = cc + lo;
         }
         if (sc == DENSE_NONE) continue;
-        hit = 1;
+        hit = true;
         pair_block(q.camG, sc, s, acc);
       }
-      if (!__syncthreads_or(hit)) continue;
-      const double v = dense_block_sum<36>(acc, red);
-      if (tid < 36) D[(size_t)(r0 + tid / 6) * n + 6 * ld + tid % 6] -= v;
+      if (!__any(hit)) continue;
+#pragma unroll
+      for (int k = 0; k < 36; k++)
+        for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_xor(acc[k], o, 64);
+      double* blk = D + (size_t)r0 * n + 6 * ld;
+#pragma unroll
+      for (int k = 0; k < 36; k++)
+        if (lane == (uint32_t)k) blk[(size_t)(k / 6) * n + k % 6] -= acc[k];
     }
     cc = ce;
   }
