@@ -115,6 +115,8 @@ typedef struct tcv_ba_report {
   uint64_t camera_tasks;    /* camera-list tasks */
   uint64_t direct_solves;   /* certified direct solves (TCV_BA_FLAG_SCHUR_CACHE) */
   uint64_t timed_matvecs;   /* Schur products timed into schur_ms (every 8th, TCV_BA_FLAG_TIMING) */
+  uint64_t direct_corrections; /* refinement corrections of direct solutions from their factor
+                                  (one per bundle whose certificate missed, at most two per solve) */
 } tcv_ba_report;
 
 typedef struct tcv_ba_context tcv_ba_context;

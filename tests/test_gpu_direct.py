@@ -221,3 +221,23 @@ def test_large_direct_polish_matches_pcg(device):
           f"coordinate difference {diff:.2e}", flush=True)
     assert prep["direct_solves"] > 0
     assert diff <= 1e-8
+
+
+def test_large_direct_correction_from_the_factor(device):
+    """A 160-camera bundle (961 unknowns) at a 1e-12 certificate: the first certificate of one LM
+    step misses and one correction x += S^-1 r from the stored multi-launch factor (the forward and
+    back substitutions of a residual) passes it; every step stays on the direct route and the
+    result matches the oracle with PCG at 1e-12 (cost 1e-9 relative, coordinates 1e-6)."""
+    print("\n[direct] correction from the large factor", flush=True)
+    scene, _ = S.trajectory_scene(cameras=160, points=6000, seed=4)
+    control = B.make_control(scene.C, scene.P, scene.M, iterations=8, max_cg=256,
+                             shared_focal=True, huber=1.0, tolerance=1e-12, schur_cache=True)
+    got, report = B.solve_packed(device, control, scene)
+    base = scene.C * 12 + scene.P * 3
+    steps = got[base + 10] + got[base + 11]
+    print(f"  direct solves {report['direct_solves']} of {steps:.0f}, corrections "
+          f"{report['direct_corrections']}, PCG {report['cg_iterations']}", flush=True)
+    assert report["direct_corrections"] >= 1
+    assert report["direct_solves"] == steps
+    expected, _ = O.solve(variant(control, schur_cache=False, max_cg=TIGHT_CG), scene)
+    compare(got, expected, base, 1e-9, 1e-6)

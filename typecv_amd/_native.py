@@ -89,6 +89,7 @@ class Report(ctypes.Structure):
         ("camera_tasks", ctypes.c_uint64),
         ("direct_solves", ctypes.c_uint64),
         ("timed_matvecs", ctypes.c_uint64),
+        ("direct_corrections", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:

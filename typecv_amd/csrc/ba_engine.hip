@@ -845,9 +845,10 @@ int solve_linear(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q,
     ops.certify(q, k + 1 == kDirectPasses, tol, st);
     HIP_TRY(hipGetLastError());
     TRY(read_state(ctx, q, st, rep));
-    bool pending = false;
-    for (uint32_t b = 0; b < q.B; b++) pending |= word(ctx, b, S_DIRECT) == 1;
+    uint32_t pending = 0;
+    for (uint32_t b = 0; b < q.B; b++) pending += word(ctx, b, S_DIRECT) == 1 ? 1u : 0u;
     if (!pending) break;
+    if (rep && k + 1 < kDirectPasses) rep->direct_corrections += pending;
     ops.dense_correct(q, st);
     for (size_t j = 0; j < ctx->large_bundles.size(); j++)
       if (word(ctx, ctx->large_bundles[j], S_DIRECT) == 1)
