@@ -323,6 +323,72 @@ def batched(args) -> int:
     return 0
 
 
+def observations(args) -> int:
+    """Fixed-scene observation evaluation (`evaluateObservationsGpu`, the mapper's observation
+    job) over the trajectory scene's observations at its perturbed starting poses: projection and
+    threshold of every observation, the optional depth guard per landmark, the inlier summary.
+    A step is one `tcv_obs_evaluate` call from host arrays (upload and readback of the flags
+    included; the per-landmark masks of the Python mirror are not built); the CPU baseline is the
+    oracle's restatement of the reference's CPU tier on the same arrays, one thread."""
+    import numpy as np
+
+    from oracle import oracle as O
+    from typecv_amd import bundle as B
+    from typecv_amd import observations as OB
+    from typecv_amd import synthetic as S
+
+    packed, _ = S.trajectory_scene(cameras=args.cameras, points=args.points, seed=2)
+    C, P, M = packed.C, packed.P, packed.M
+    sc, ix = packed.scene, packed.index
+    poses, pts = sc[:12 * C].reshape(C, 12), sc[12 * C:12 * C + 3 * P]
+    models = sc[12 * C + 3 * P:12 * C + 3 * P + 6 * C].reshape(C, 6)
+    pixels = sc[12 * C + 3 * P + 6 * C:]
+    guarded = args.max_relative_depth_std > 0
+    request = {
+        "depth": {"maxRelativeDepthStdDev": args.max_relative_depth_std, "pixelNoiseStdDevPx": 1.0},
+        "guarded": guarded, "lengths": [], "count": M, "views": C, "points": P,
+        "cameras": np.ascontiguousarray(np.concatenate([poses, models], axis=1).reshape(-1)),
+        "points_xyz": np.ascontiguousarray(pts), "pixels": np.ascontiguousarray(pixels),
+        "indices": np.ascontiguousarray(np.stack([ix[:M], ix[M:2 * M]], axis=1).reshape(-1)),
+        "offsets": np.ascontiguousarray(ix[2 * M:2 * M + P + 1]) if guarded else None,
+    }
+    if guarded:
+        request["lengths"] = np.diff(request["offsets"]).tolist()
+    device = B.HipDevice(0)
+    for _ in range(args.warmup):
+        OB.evaluate_arrays(device, request, args.max_error)
+    times = []
+    for k in range(args.steps):
+        g_flags, g_summary, elapsed = OB.evaluate_arrays(device, request, args.max_error)
+        times.append(elapsed * 1e-3)
+        log(f"step {k}: {elapsed:.2f} ms, {g_summary[1]:.0f} inliers")
+    device.dispose()
+    result = {"inlierCount": float(g_summary[1]),
+              "meanErrorPx": float(g_summary[0] / g_summary[1]) if g_summary[1] else None}
+    t = time.perf_counter()
+    flags, _, summary = O.observations(request["cameras"], request["points_xyz"],
+                                       request["pixels"], request["indices"], request["offsets"],
+                                       args.max_error, args.max_relative_depth_std, 1.0)
+    cpu = time.perf_counter() - t
+    ms = float(np.median(times)) * 1e3
+    print(json.dumps({
+        "metric": "fixed-scene observation evaluation: observations/s", "value": M / (ms * 1e-3),
+        "unit": "observations/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded trajectory scene at its starting poses)",
+        "config": {"workload": "evaluateObservationsGpu", "cameras": C, "points": P,
+                   "observations": M, "max_error_px": args.max_error,
+                   "max_relative_depth_std_dev": args.max_relative_depth_std},
+        "note": "host arrays in and flags out: the timed step includes the PCIe transfers",
+        "inliers": result["inlierCount"], "mean_error_px": result["meanErrorPx"],
+        "matches_oracle": bool(np.array_equal(flags, g_flags[:len(flags)])
+                               and np.array_equal(summary, g_summary)),
+        "cpu_baseline": {"value": M / cpu, "unit": "observations/s", "cores": 1, "kind": "port",
+                         "sample": f"the whole evaluation ({M} observations), C oracle -O3"},
+    }), flush=True)
+    return 0
+
+
 def real_scene(args) -> int:
     """The reference's real-data BA measurement (south-building-24: 24 views, 3,000 landmarks,
     12,706 observations; 4 LM iterations, Huber 1 px; tests/golden/south_building_24.npz) through
@@ -591,6 +657,16 @@ if __name__ == "__main__":
         ap.add_argument("--repeat", type=int, default=10)
         ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
         sys.exit(probe(ap.parse_args()))
+    if "--workload" in sys.argv and "observations" in sys.argv:
+        ap = argparse.ArgumentParser()
+        ap.add_argument("--workload")
+        ap.add_argument("--steps", type=int, default=5)
+        ap.add_argument("--warmup", type=int, default=1)
+        ap.add_argument("--cameras", type=int, default=500)
+        ap.add_argument("--points", type=int, default=200_000)
+        ap.add_argument("--max-error", type=float, default=3.0)
+        ap.add_argument("--max-relative-depth-std", type=float, default=0.0)
+        sys.exit(observations(ap.parse_args()))
     if "--workload" in sys.argv and "south-building-24" in sys.argv:
         ap = argparse.ArgumentParser()
         ap.add_argument("--workload")

@@ -195,6 +195,37 @@ int tcv_ba_schur_probe(tcv_ba_context* context, const tcv_ba_control* control,
                        int32_t* status, uint32_t repeat, tcv_ba_report* report);
 
 /*
+ * Fixed-scene observation evaluation: evaluateObservationsGpu / evaluateContinuedObservationsGpu
+ * (src/geometry/gpu/observations.ts:34-127; kernels observation-evaluation.shade.ts:44-137 and
+ * observationSummary, triangulation.shade.ts:134-143). The host packs the request as the
+ * reference's observationRequest does (observations.ts:132-197):
+ *   cameras  18 doubles per referenced view: the pose record (R row-major, t), then fx fy cx cy k1 k2
+ *   points   3 doubles per landmark
+ *   pixels   2 doubles per observation, observations in landmark order
+ *   indices  2 words per observation: the view's record, the landmark
+ *   offsets  (guarded runs: max_relative_depth_std_dev > 0) tracks + 1 words, each track's first
+ *            observation; NULL otherwise
+ * flags (out) receives `observations` words, 1 for an inlier (projected in front of the camera
+ * within max_error_px), then on guarded runs `tracks` words, 1 for a track whose depth was not
+ * stable (its observations' flags are then 0); summary (out) receives the inliers' error sum and
+ * their count. All arrays are host arrays.
+ */
+typedef struct tcv_obs_control {
+  uint32_t views;         /* camera records */
+  uint32_t points;        /* point records */
+  uint32_t observations;  /* observations (may be 0) */
+  uint32_t tracks;        /* tracks of a guarded run (offsets holds tracks + 1), else 0 */
+  double max_error_px;    /* inlier threshold, [1e-12, 1e6] */
+  double max_relative_depth_std_dev;  /* > 0: guarded run; 0: no depth filter */
+  double pixel_noise_std_dev_px;      /* > 0 */
+} tcv_obs_control;
+
+int tcv_obs_evaluate(tcv_ba_context* context, const tcv_obs_control* control,
+                     const double* cameras, const double* points, const double* pixels,
+                     const uint32_t* indices, const uint32_t* offsets, uint32_t* flags,
+                     double* summary);
+
+/*
  * Differentiation of the solve. The reference operator has no derivative entries: these extend
  * `bundleAdjustGpu` (src/sfm/bundle/gpu/bundle.ts:803-839) with the pieces of a backward pass. All
  * take a host scene/index in the packBundleScene layouts (reference bundle-host.ts:121-218) holding

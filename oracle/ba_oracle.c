@@ -1177,6 +1177,126 @@ int tcv_oracle_solve6(const double* a, const double* b, double* x) { return solv
 void tcv_oracle_move_pose(const double* p, const double* d, double* out) { move_pose(p, d, out); }
 double tcv_oracle_focal_scale(double x) { return focal_scale(x); }
 
+/* ------------------------------------------------ fixed-scene observation evaluation
+ *
+ * evaluateObservationsGpu (src/geometry/gpu/observations.ts:34-127) as its CPU tier runs the
+ * kernels of observation-evaluation.shade.ts, in their order:
+ *   projectIndexedObservations      :44-54   flag and error of every observation
+ *   filterIndexedObservationDepth   :85-97   (guarded) tracks whose depth is not stable lose all
+ *                                            their flags; flag count + h marks track h rejected
+ *   indexedObservationPartialsOracle :128-137 per 64 observations, errors and counts summed in lane
+ *                                            order (the CPU tier's partials kernel)
+ *   observationSummary  triangulation.shade.ts:134-143  the partials summed in block order
+ * cameras: 18 per view (pose record R row-major, t, then fx fy cx cy k1 k2); points: 3 per
+ * landmark; pixels: 2 per observation; indices: (view, landmark) per observation; offsets: the
+ * tracks' first observations (tracks + 1), guarded runs only. flags: count (+ tracks) words;
+ * errors: count; summary: [error sum, inlier count].
+ */
+static void obs_transform(const double* p, const double* X, double q[3]) {
+  q[0] = p[0] * X[0] + p[1] * X[1] + p[2] * X[2] + p[9];
+  q[1] = p[3] * X[0] + p[4] * X[1] + p[5] * X[2] + p[10];
+  q[2] = p[6] * X[0] + p[7] * X[1] + p[8] * X[2] + p[11];
+}
+
+static Projection obs_project(const double* cameras, const uint32_t* indices, const double* points,
+                              uint32_t i, const double* X) {
+  const double* rec = cameras + (size_t)indices[2 * i] * 18;
+  const Model k = {rec[12], rec[13], rec[14], rec[15], rec[16], rec[17]};
+  double q[3];
+  obs_transform(rec, X ? X : points + 3 * (size_t)indices[2 * i + 1], q);
+  return project(&k, q);
+}
+
+/* stableDepth, observation-evaluation.shade.ts:57-83 */
+static int obs_stable_depth(const double* cameras, const uint32_t* indices, const uint32_t* flags,
+                            uint32_t start, uint32_t end, const double* X, double limit,
+                            double noise) {
+  double A[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  uint32_t count = 0;
+  for (uint32_t i = start; i < end; i++) {
+    if (flags[i] == 0) continue;
+    count++;
+    const double* p = cameras + (size_t)indices[2 * i] * 18;
+    Projection pr = obs_project(cameras, indices, NULL, i, X);
+    if (!pr.ok) return 0;
+    for (int row = 0; row < 2; row++) {
+      const double* g = row == 0 ? pr.du : pr.dv;
+      const double J[3] = {g[0] * p[0] + g[1] * p[3] + g[2] * p[6],
+                           g[0] * p[1] + g[1] * p[4] + g[2] * p[7],
+                           g[0] * p[2] + g[1] * p[5] + g[2] * p[8]};
+      for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) A[3 * r + c] += J[r] * J[c];
+    }
+  }
+  if (count < 2) return 0;
+  for (uint32_t i = start; i < end; i++) {
+    if (flags[i] == 0) continue;
+    const double* p = cameras + (size_t)indices[2 * i] * 18;
+    const double axis[3] = {p[6], p[7], p[8]};
+    double q[3], solved[3];
+    obs_transform(p, X, q);
+    const int ok = solve3(A, axis, solved);
+    if (!ok || !(q[2] > 0)) return 0;
+    const double variance = axis[0] * solved[0] + axis[1] * solved[1] + axis[2] * solved[2];
+    if (!(variance >= 0)) return 0;
+    const double relative = noise * sqrt(variance) / q[2];
+    if (!(relative <= limit)) return 0;
+  }
+  return 1;
+}
+
+void tcv_oracle_observations(const double* cameras, const double* points, const double* pixels,
+                             const uint32_t* indices, uint32_t count, const uint32_t* offsets,
+                             uint32_t tracks, double max_error, double max_relative_depth_std,
+                             double pixel_noise_std, uint32_t* flags, double* errors,
+                             double* summary) {
+  for (uint32_t i = 0; i < count; i++) {
+    flags[i] = 0;
+    errors[i] = 0;
+    Projection pr = obs_project(cameras, indices, points, i, NULL);
+    if (!pr.ok) continue;
+    const double x = pr.px - pixels[2 * (size_t)i], y = pr.py - pixels[2 * (size_t)i + 1];
+    const double e = sqrt(x * x + y * y);
+    if (e <= max_error) {
+      flags[i] = 1;
+      errors[i] = e;
+    }
+  }
+  if (offsets)
+    for (uint32_t h = 0; h < tracks; h++) {
+      const uint32_t start = offsets[h], end = offsets[h + 1];
+      flags[count + h] = 0;
+      if (end <= start) continue;
+      uint32_t accepted = 0;
+      for (uint32_t i = start; i < end; i++) accepted += flags[i] != 0;
+      if (accepted < 2) continue;
+      const double* X = points + 3 * (size_t)indices[2 * start + 1];
+      if (!obs_stable_depth(cameras, indices, flags, start, end, X, max_relative_depth_std,
+                            pixel_noise_std)) {
+        flags[count + h] = 1;
+        for (uint32_t i = start; i < end; i++) {
+          flags[i] = 0;
+          errors[i] = 0;
+        }
+      }
+    }
+  const uint32_t blocks = (count > 0 ? count : 1) / 64 + ((count > 0 ? count : 1) % 64 != 0);
+  double error = 0, inliers = 0;
+  for (uint32_t b = 0; b < blocks; b++) {
+    double e = 0, n = 0;
+    for (uint32_t lane = 0; lane < 64; lane++) {
+      const uint32_t i = b * 64 + lane;
+      if (i >= count || flags[i] == 0) continue;
+      e += errors[i];
+      n += 1;
+    }
+    error += e;
+    inliers += n;
+  }
+  summary[0] = error;
+  summary[1] = inliers;
+}
+
 /*
  * The gradient of the half robust cost E = 1/2 sum_i robustCost(|r_i|) in the movePose chart at
  * log focal scale `log_focal`: 6 per camera (omega, v; zero for a fixed camera), 3 per point, then

@@ -56,6 +56,10 @@ def library() -> ctypes.CDLL:
     lib.tcv_oracle_focal_scale.restype = ctypes.c_double
     lib.tcv_oracle_gradient.argtypes = [P(_native.Control), vp, vp, ctypes.c_double, vp, vp]
     lib.tcv_oracle_dense_solve.argtypes = [P(_native.Control), vp, vp, vp]
+    lib.tcv_oracle_observations.argtypes = [vp, vp, vp, vp, ctypes.c_uint32, vp, ctypes.c_uint32,
+                                            ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                            vp, vp, vp]
+    lib.tcv_oracle_observations.restype = None
     _lib = lib
     return lib
 
@@ -206,3 +210,25 @@ def dense_solve(control: _native.Control, packed: PackedScene) -> np.ndarray:
     if r != 0:
         raise RuntimeError(f"oracle dense solve failed ({r})")
     return out
+
+
+def observations(cameras: np.ndarray, points: np.ndarray, pixels: np.ndarray,
+                 indices: np.ndarray, offsets: np.ndarray | None, max_error: float,
+                 max_relative_depth_std: float = 0.0, pixel_noise_std: float = 1.0):
+    """The fixed-scene observation evaluation (`evaluateObservationsGpu`'s kernels, see the C
+    source): (flags: count [+ tracks] u32, errors: count f64, summary: [error sum, inliers])."""
+    count = len(indices) // 2
+    tracks = 0 if offsets is None else len(offsets) - 1
+    cameras = np.ascontiguousarray(cameras, dtype=np.float64)
+    points = np.ascontiguousarray(points, dtype=np.float64)
+    pixels = np.ascontiguousarray(pixels, dtype=np.float64)
+    indices = np.ascontiguousarray(indices, dtype=np.uint32)
+    off = None if offsets is None else np.ascontiguousarray(offsets, dtype=np.uint32)
+    flags = np.zeros(max(1, count + tracks), dtype=np.uint32)
+    errors = np.zeros(max(1, count))
+    summary = np.zeros(2)
+    library().tcv_oracle_observations(_ptr(cameras), _ptr(points), _ptr(pixels), _ptr(indices),
+                                      count, _ptr(off), tracks, float(max_error),
+                                      float(max_relative_depth_std), float(pixel_noise_std),
+                                      _ptr(flags), _ptr(errors), _ptr(summary))
+    return flags[:count + tracks], errors[:count], summary

@@ -105,7 +105,7 @@ def test_directional_constraints_append_unit_bearings():
 def declared_functions():
     text = (ROOT / "include" / "typecv_ba.h").read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return re.findall(r"^\s*[a-z_0-9 ]+\*?\s+\**(tcv_ba_[a-z_]+)\s*\(", text, flags=re.M)
+    return re.findall(r"^\s*[a-z_0-9 ]+\*?\s+\**(tcv_[a-z_]+)\s*\(", text, flags=re.M)
 
 
 def test_c_abi_exports_every_declared_symbol():

@@ -43,6 +43,7 @@ EXPORTS = (
     "tcv_ba_jacobian",
     "tcv_ba_polish",
     "tcv_ba_backward",
+    "tcv_obs_evaluate",
 )
 
 
@@ -94,6 +95,19 @@ class Report(ctypes.Structure):
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class ObsControl(ctypes.Structure):
+    """`tcv_obs_control` (include/typecv_ba.h)."""
+    _fields_ = [
+        ("views", ctypes.c_uint32),
+        ("points", ctypes.c_uint32),
+        ("observations", ctypes.c_uint32),
+        ("tracks", ctypes.c_uint32),
+        ("max_error_px", ctypes.c_double),
+        ("max_relative_depth_std_dev", ctypes.c_double),
+        ("pixel_noise_std_dev_px", ctypes.c_double),
+    ]
 
 
 _lib = None
@@ -158,6 +172,8 @@ def library() -> ctypes.CDLL:
     lib.tcv_ba_backward.argtypes = [vp, c_control, vp, vp, ctypes.c_double, vp, ctypes.c_uint32,
                                     ctypes.c_double, vp, vp, P(Report)]
     lib.tcv_ba_backward.restype = ctypes.c_int
+    lib.tcv_obs_evaluate.argtypes = [vp, P(ObsControl), vp, vp, vp, vp, vp, vp, vp]
+    lib.tcv_obs_evaluate.restype = ctypes.c_int
     _lib = lib
     return lib
 

@@ -22,6 +22,7 @@
 #include "../../include/typecv_ba.h"
 #include "ba_kernels.hip"
 #include "ba_dense.hip"
+#include "obs_eval.hip"
 
 using namespace tcvba;
 
@@ -54,7 +55,9 @@ struct tcv_ba_context {
   DevBuf scene, trial, index, out, bundles, tiles, tile_bundle, tasks, task_start, cost_blocks,
       camera_bundle, point_bundle, cam_point, Wp, Wc, pblk, tile_fail, tile_focal, task_part, camA,
       camInv, camRhs, camHcf, camFocal, vec, state, cost_part, error, probe, groups, group_part,
-      rhs, grad, camS, camG, pfc, dense, direct_bundles, direct_cams, covis, small_bundles;
+      rhs, grad, camS, camG, pfc, dense, direct_bundles, direct_cams, covis, small_bundles,
+      obs_cameras, obs_points, obs_pixels, obs_indices, obs_offsets, obs_flags, obs_errors,
+      obs_partials, obs_summary;
   // the direct bundles factored by a sequence of launches, and their dense array sizes
   std::vector<uint32_t> large_bundles, large_n;
   std::vector<uint64_t> large_off;
@@ -66,7 +69,9 @@ struct tcv_ba_context {
            &tile_focal, &task_part,    &camA,      &camInv,    &camRhs,     &camHcf,
            &camFocal,   &vec,          &state,     &cost_part, &error,      &probe,
            &groups,     &group_part,   &rhs,       &grad,      &camS,       &camG,   &pfc,
-           &dense,      &direct_bundles, &direct_cams, &covis, &small_bundles};
+           &dense,      &direct_bundles, &direct_cams, &covis, &small_bundles,
+           &obs_cameras, &obs_points, &obs_pixels, &obs_indices, &obs_offsets, &obs_flags,
+           &obs_errors, &obs_partials, &obs_summary};
   }
 };
 
@@ -1600,6 +1605,97 @@ int tcv_ba_backward(tcv_ba_context* ctx, const tcv_ba_control* c, const double* 
     rep->device_bytes = held_bytes(ctx);
     rep->direct_solves = (uint64_t)word(ctx, 0, S_DIRECT_SOLVES);
   }
+  return TCV_BA_OK;
+}
+
+int tcv_obs_evaluate(tcv_ba_context* ctx, const tcv_obs_control* c, const double* cameras,
+                     const double* points, const double* pixels, const uint32_t* indices,
+                     const uint32_t* offsets, uint32_t* flags, double* summary) {
+  if (!ctx || !c || !flags || !summary) return TCV_BA_ERR_ARGUMENT;
+  const uint32_t M = c->observations, T = c->tracks;
+  const bool guarded = c->max_relative_depth_std_dev > 0;
+  // the reference's request checks (observations.ts:141-150, multiview.ts:26-36)
+  if (!(c->max_error_px >= 1e-12 && c->max_error_px <= 1e6) ||
+      !(c->pixel_noise_std_dev_px > 0 && c->pixel_noise_std_dev_px < INFINITY) ||
+      !(c->max_relative_depth_std_dev >= 0 && c->max_relative_depth_std_dev < INFINITY)) {
+    set_error("Invalid GPU observation threshold");
+    return TCV_BA_ERR_ARGUMENT;
+  }
+  if ((M && (!cameras || !points || !pixels || !indices || !c->views || !c->points)) ||
+      (guarded && T && !offsets) || (!guarded && T) || (guarded && T != c->points)) {
+    set_error("observation arrays do not match the control");
+    return TCV_BA_ERR_ARGUMENT;
+  }
+  // every index in range, offsets ascending and covering the observations: checked on the host,
+  // as the arrays are host arrays here
+  for (uint32_t i = 0; i < M; i++)
+    if (indices[2 * (size_t)i] >= c->views || indices[2 * (size_t)i + 1] >= c->points) {
+      set_error("observation index outside the camera or point records");
+      return TCV_BA_ERR_INDEX;
+    }
+  if (guarded && T) {
+    if (offsets[0] != 0 || offsets[T] != M) {
+      set_error("track offsets do not cover the observations");
+      return TCV_BA_ERR_INDEX;
+    }
+    for (uint32_t h = 0; h < T; h++)
+      if (offsets[h + 1] < offsets[h]) {
+        set_error("track offsets decrease");
+        return TCV_BA_ERR_INDEX;
+      }
+  }
+  HIP_TRY(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const uint32_t blocks = (M ? M : 1) / 64 + ((M ? M : 1) % 64 != 0);
+  const size_t nflags = (size_t)M + (guarded ? T : 0);
+  TRY(ensure(ctx->obs_cameras, (size_t)c->views * 18 * 8 + 8));
+  TRY(ensure(ctx->obs_points, (size_t)c->points * 3 * 8 + 8));
+  TRY(ensure(ctx->obs_pixels, (size_t)M * 2 * 8 + 8));
+  TRY(ensure(ctx->obs_indices, (size_t)M * 2 * 4 + 8));
+  TRY(ensure(ctx->obs_offsets, (size_t)(T + 1) * 4 + 8));
+  TRY(ensure(ctx->obs_flags, nflags * 4 + 8));
+  TRY(ensure(ctx->obs_errors, (size_t)M * 8 + 8));
+  TRY(ensure(ctx->obs_partials, (size_t)blocks * 2 * 8));
+  TRY(ensure(ctx->obs_summary, 2 * 8));
+  if (M) {
+    HIP_TRY(hipMemcpyAsync(ctx->obs_cameras.ptr, cameras, (size_t)c->views * 18 * 8,
+                           hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(ctx->obs_points.ptr, points, (size_t)c->points * 3 * 8,
+                           hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(ctx->obs_pixels.ptr, pixels, (size_t)M * 2 * 8, hipMemcpyHostToDevice,
+                           st));
+    HIP_TRY(hipMemcpyAsync(ctx->obs_indices.ptr, indices, (size_t)M * 2 * 4,
+                           hipMemcpyHostToDevice, st));
+  }
+  if (guarded && T)
+    HIP_TRY(hipMemcpyAsync(ctx->obs_offsets.ptr, offsets, (size_t)(T + 1) * 4,
+                           hipMemcpyHostToDevice, st));
+  ObsProblem o;
+  o.cameras = (const double*)ctx->obs_cameras.ptr;
+  o.points = (const double*)ctx->obs_points.ptr;
+  o.pixels = (const double*)ctx->obs_pixels.ptr;
+  o.indices = (const uint32_t*)ctx->obs_indices.ptr;
+  o.offsets = (const uint32_t*)ctx->obs_offsets.ptr;
+  o.flags = (uint32_t*)ctx->obs_flags.ptr;
+  o.errors = (double*)ctx->obs_errors.ptr;
+  o.partials = (double*)ctx->obs_partials.ptr;
+  o.summary = (double*)ctx->obs_summary.ptr;
+  o.count = M;
+  o.tracks = guarded ? T : 0;
+  o.blocks = blocks;
+  o.max_error = c->max_error_px;
+  o.max_relative_depth_std = c->max_relative_depth_std_dev;
+  o.pixel_noise_std = c->pixel_noise_std_dev_px;
+  if (M) hipLaunchKernelGGL(k_obs_project, dim3(blocks_for(M, 256)), dim3(256), 0, st, o);
+  if (o.tracks) hipLaunchKernelGGL(k_obs_depth, dim3(blocks_for(o.tracks, 256)), dim3(256), 0, st, o);
+  if (M) hipLaunchKernelGGL(k_obs_partials, dim3(blocks_for(blocks, 256)), dim3(256), 0, st, o);
+  else HIP_TRY(hipMemsetAsync(o.partials, 0, (size_t)blocks * 2 * 8, st));
+  hipLaunchKernelGGL(k_obs_summary, dim3(1), dim3(64), 0, st, o);
+  HIP_TRY(hipGetLastError());
+  if (nflags)
+    HIP_TRY(hipMemcpyAsync(flags, o.flags, nflags * 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(summary, o.summary, 2 * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
   return TCV_BA_OK;
 }
 
