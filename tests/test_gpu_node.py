@@ -1,7 +1,8 @@
 """The Node-API addon end to end on the GPU: a packed solve through `typecv_amd/node/run.js`
 (Node, napi_async_work, tcv_ba_solve) returns exactly the array the Python binding returns for the
 same inputs, and its diagnostics; a batch through `solveBatch` (tcv_ba_solve_batch) returns each
-problem's array exactly as the Python batch does."""
+problem's array exactly as the Python batch does; an observation evaluation through `evaluate`
+(tcv_obs_evaluate) returns the Python path's flags and summary."""
 import json
 import shutil
 import subprocess
@@ -75,3 +76,42 @@ def test_node_batch_matches_python(device, tmp_path):
     for b, want in enumerate(expected):
         got = np.fromfile(tmp_path / f"out.{b}.f64", dtype=np.float64)
         assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("guarded", [False, True])
+def test_node_observations_match_python(device, tmp_path, guarded):
+    node = shutil.which("node")
+    if node is None or not (NODE_DIR / "typecv_ba.node").exists():
+        pytest.skip("no Node or no built addon")
+    import sys
+    sys.path.insert(0, str(Path(__file__).resolve().parent))
+    from obs_cases import observation_model
+    from typecv_amd import observations as OB
+    print(f"\n[node] observations guarded={guarded}", flush=True)
+    model = observation_model(points=300, views=5, baseline=0.2)
+    model["landmarks"][7]["observations"][1]["pixel"][0] += 9.0
+    options = {"maxRelativeDepthStdDev": 0.02} if guarded else {}
+    req = OB.observation_request(model, 3.0, options, False)
+    tracks = len(req["lengths"]) if guarded else 0
+    np.array([req["views"], req["points"], req["count"], tracks, 3.0,
+              req["depth"]["maxRelativeDepthStdDev"], req["depth"]["pixelNoiseStdDevPx"]],
+             dtype=np.float64).tofile(tmp_path / "c.f64")
+    req["cameras"].tofile(tmp_path / "cam.f64")
+    req["points_xyz"].tofile(tmp_path / "pts.f64")
+    req["pixels"].tofile(tmp_path / "px.f64")
+    req["indices"].tofile(tmp_path / "ix.u32")
+    offsets = "-"
+    if guarded:
+        req["offsets"].tofile(tmp_path / "off.u32")
+        offsets = str(tmp_path / "off.u32")
+    run = subprocess.run([node, str(NODE_DIR / "run.js"), "--observations",
+                          str(tmp_path / "c.f64"), str(tmp_path / "cam.f64"),
+                          str(tmp_path / "pts.f64"), str(tmp_path / "px.f64"),
+                          str(tmp_path / "ix.u32"), offsets, str(tmp_path / "out")],
+                         capture_output=True, text=True, timeout=120)
+    assert run.returncode == 0, run.stderr
+    flags, summary, _ = OB.evaluate_arrays(device, req, 3.0)
+    got_flags = np.fromfile(tmp_path / "out.flags.u32", dtype=np.uint32)
+    got_summary = np.fromfile(tmp_path / "out.summary.f64", dtype=np.float64)
+    assert np.array_equal(got_flags, flags[:req["count"] + tracks])
+    assert np.array_equal(got_summary, summary)

@@ -138,5 +138,5 @@ def test_node_addon_loads_and_exports_the_operator():
     out = subprocess.run([node, str(addon.parent / "run.js"), "--exports"], capture_output=True,
                          text=True, timeout=60)
     assert out.returncode == 0, out.stderr
-    assert json.loads(out.stdout) == ["backward", "deviceBytes", "open", "outputSize", "release",
-                                      "solve", "solveBatch"]
+    assert json.loads(out.stdout) == ["backward", "deviceBytes", "evaluate", "open", "outputSize",
+                                       "release", "solve", "solveBatch"]

@@ -99,6 +99,19 @@ function backwardPacked(device, control, scene, index, logFocal, cotangent,
     cotangent, maxCg, tolerance));
 }
 
+/* The fixed-scene observation evaluation (evaluateObservationsGpu's device part) of a request
+ * packed as observationRequest packs it (src/geometry/gpu/observations.ts:132-197): the flags
+ * (observations [+ tracks]) and the summary [error sum, inlier count]. */
+function evaluatePacked(device, { views, points, observations, tracks = 0, maxErrorPx = 3,
+  maxRelativeDepthStdDev = 0, pixelNoiseStdDevPx = 1 }, cameras, xyz, pixels, indices,
+offsets = null) {
+  const control = Float64Array.from([views, points, observations, tracks, maxErrorPx,
+    maxRelativeDepthStdDev, pixelNoiseStdDevPx]);
+  return device.enqueue(() =>
+    addon.evaluate(device.native, control, cameras, xyz, pixels, indices, offsets));
+}
+
 module.exports = {
-  HipDevice, makeControl, solvePacked, solveBatchPacked, backwardPacked, BUNDLE_STATE, addon,
+  HipDevice, makeControl, solvePacked, solveBatchPacked, backwardPacked, evaluatePacked,
+  BUNDLE_STATE, addon,
 };

@@ -5,10 +5,15 @@
  * reads the raw little-endian arrays, solves on device 0 and writes the packBundleResult array;
  *   node run.js --batch <out prefix> (<control.f64> <scene.f64> <index.u32>)...
  * solves the problems in one batched call and writes <out prefix>.<b>.f64 for each.
+ *   node run.js --observations <control.f64> <cameras.f64> <points.f64> <pixels.f64>
+ *       <indices.u32> <offsets.u32 | -> <out prefix>
+ * evaluates a packed observation request (control: views, points, observations, tracks,
+ * maxErrorPx, maxRelativeDepthStdDev, pixelNoiseStdDevPx) and writes <out prefix>.flags.u32 and
+ * <out prefix>.summary.f64.
  * `node run.js --exports` prints the addon's exported names (no device is touched).
  */
 const fs = require('fs');
-const { HipDevice, solvePacked, solveBatchPacked, addon } = require('./bundle.js');
+const { HipDevice, solvePacked, solveBatchPacked, evaluatePacked, addon } = require('./bundle.js');
 
 function read(file, Type) {
   const bytes = fs.readFileSync(file);
@@ -18,6 +23,26 @@ function read(file, Type) {
 async function main(argv) {
   if (argv[0] === '--exports') {
     console.log(JSON.stringify(Object.keys(addon).sort()));
+    return 0;
+  }
+  if (argv[0] === '--observations') {
+    const [, controlFile, camerasFile, pointsFile, pixelsFile, indicesFile, offsetsFile, prefix] =
+      argv;
+    const c = read(controlFile, Float64Array);
+    const offsets = offsetsFile === '-' ? null : read(offsetsFile, Uint32Array);
+    const device = new HipDevice(0);
+    try {
+      const r = await evaluatePacked(device, {
+        views: c[0], points: c[1], observations: c[2], tracks: c[3], maxErrorPx: c[4],
+        maxRelativeDepthStdDev: c[5], pixelNoiseStdDevPx: c[6],
+      }, read(camerasFile, Float64Array), read(pointsFile, Float64Array),
+      read(pixelsFile, Float64Array), read(indicesFile, Uint32Array), offsets);
+      fs.writeFileSync(`${prefix}.flags.u32`, Buffer.from(r.flags.buffer));
+      fs.writeFileSync(`${prefix}.summary.f64`, Buffer.from(r.summary.buffer));
+      console.log(JSON.stringify({ inliers: r.summary[1] }));
+    } finally {
+      device.dispose();
+    }
     return 0;
   }
   if (argv[0] === '--batch') {

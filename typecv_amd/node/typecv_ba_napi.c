@@ -16,6 +16,11 @@
  *                                                  -> Promise<Float64Array(2M)>
  *   solveBatch(context, controls[], scenes[], indices[])
  *                                                  -> Promise<Float64Array[]> (tcv_ba_solve_batch)
+ *   evaluate(context, control, cameras, points, pixels, indices, offsets|null)
+ *                                                  -> Promise<{flags: Uint32Array,
+ *                                                     summary: Float64Array(2)}> (tcv_obs_evaluate)
+ *     control: Float64Array(7) = views, points, observations, tracks, maxErrorPx,
+ *     maxRelativeDepthStdDev, pixelNoiseStdDevPx (the tcv_obs_control fields)
  * `control` is a Float64Array(16): the tcv_ba_control fields in declaration order (12 integers,
  * then huber, relative_tolerance, stall_tolerance, refinement_tolerance); `scene` a Float64Array
  * and `index` a Uint32Array in the packBundleScene layouts (bundle-host.ts:121-218).
@@ -143,6 +148,145 @@ static napi_value release(napi_env env, napi_callback_info info) {
   if (!ctx) return NULL;
   tcv_ba_release(ctx);
   return NULL;
+}
+
+/* ---------------------------------------------------------------- observation evaluation */
+
+typedef struct {
+  napi_async_work work;
+  napi_deferred deferred;
+  tcv_ba_context* ctx;
+  tcv_obs_control control;
+  napi_ref refs[7];
+  int nrefs;
+  const double *cameras, *points, *pixels;
+  const uint32_t *indices, *offsets;
+  uint32_t* flags;
+  size_t nflags;
+  double summary[2];
+  int status;
+  char detail[256];
+} ObsJob;
+
+static void obs_execute(napi_env env, void* data) {
+  (void)env;
+  ObsJob* j = data;
+  j->status = tcv_obs_evaluate(j->ctx, &j->control, j->cameras, j->points, j->pixels, j->indices,
+                               j->offsets, j->flags, j->summary);
+  if (j->status != TCV_BA_OK) snprintf(j->detail, sizeof j->detail, "%s", tcv_ba_last_error());
+}
+
+static void obs_complete(napi_env env, napi_status s, void* data) {
+  ObsJob* j = data;
+  if (s == napi_ok && j->status == TCV_BA_OK) {
+    napi_value result, fbuf, flags, sbuf, summary;
+    void* bytes = NULL;
+    napi_create_object(env, &result);
+    napi_create_arraybuffer(env, j->nflags * sizeof(uint32_t), &bytes, &fbuf);
+    if (j->nflags) memcpy(bytes, j->flags, j->nflags * sizeof(uint32_t));
+    napi_create_typedarray(env, napi_uint32_array, j->nflags, fbuf, 0, &flags);
+    napi_create_arraybuffer(env, 2 * sizeof(double), &bytes, &sbuf);
+    memcpy(bytes, j->summary, 2 * sizeof(double));
+    napi_create_typedarray(env, napi_float64_array, 2, sbuf, 0, &summary);
+    napi_set_named_property(env, result, "flags", flags);
+    napi_set_named_property(env, result, "summary", summary);
+    napi_resolve_deferred(env, j->deferred, result);
+  } else {
+    char text[512];
+    snprintf(text, sizeof text, "%s (%s)",
+             s == napi_ok ? tcv_ba_status_string(j->status) : "job cancelled", j->detail);
+    napi_value msg, err;
+    napi_create_string_utf8(env, text, NAPI_AUTO_LENGTH, &msg);
+    if (j->status == TCV_BA_ERR_ARGUMENT)
+      napi_create_range_error(env, NULL, msg, &err);
+    else
+      napi_create_error(env, NULL, msg, &err);
+    napi_reject_deferred(env, j->deferred, err);
+  }
+  for (int k = 0; k < j->nrefs; k++) napi_delete_reference(env, j->refs[k]);
+  napi_delete_async_work(env, j->work);
+  free(j->flags);
+  free(j);
+}
+
+static napi_value evaluate(napi_env env, napi_callback_info info) {
+  size_t argc = 7;
+  napi_value argv[7];
+  CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  if (argc < 6) {
+    napi_throw_type_error(env, NULL, "missing arguments");
+    return NULL;
+  }
+  ObsJob* j = calloc(1, sizeof(ObsJob));
+  if (!j) {
+    napi_throw_error(env, NULL, "out of host memory");
+    return NULL;
+  }
+  size_t nc = 0, ncam = 0, npt = 0, npx = 0, nix = 0, noff = 0;
+  const double* c = NULL;
+  j->ctx = context_of(env, argv[0]);
+  if (!j->ctx || !(c = typed_data(env, argv[1], napi_float64_array, &nc)) ||
+      !(j->cameras = typed_data(env, argv[2], napi_float64_array, &ncam)) ||
+      !(j->points = typed_data(env, argv[3], napi_float64_array, &npt)) ||
+      !(j->pixels = typed_data(env, argv[4], napi_float64_array, &npx)) ||
+      !(j->indices = typed_data(env, argv[5], napi_uint32_array, &nix))) {
+    free(j);
+    return NULL;
+  }
+  napi_valuetype t = napi_null;
+  if (argc > 6) napi_typeof(env, argv[6], &t);
+  if (t != napi_null && t != napi_undefined &&
+      !(j->offsets = typed_data(env, argv[6], napi_uint32_array, &noff))) {
+    free(j);
+    return NULL;
+  }
+  if (nc != 7) {
+    napi_throw_range_error(env, NULL, "control must be a Float64Array(7)");
+    free(j);
+    return NULL;
+  }
+  for (int k = 0; k < 4; k++)
+    if (!(c[k] >= 0 && c[k] <= 4294967295.0) || c[k] != (double)(uint32_t)c[k]) {
+      napi_throw_range_error(env, NULL, "control integer field out of range");
+      free(j);
+      return NULL;
+    }
+  j->control.views = (uint32_t)c[0];
+  j->control.points = (uint32_t)c[1];
+  j->control.observations = (uint32_t)c[2];
+  j->control.tracks = (uint32_t)c[3];
+  j->control.max_error_px = c[4];
+  j->control.max_relative_depth_std_dev = c[5];
+  j->control.pixel_noise_std_dev_px = c[6];
+  const size_t M = j->control.observations, T = j->control.tracks;
+  if (ncam < 18 * (size_t)j->control.views || npt < 3 * (size_t)j->control.points ||
+      npx < 2 * M || nix < 2 * M || (j->offsets && noff < T + 1) || (T && !j->offsets)) {
+    napi_throw_range_error(env, NULL, "observation arrays are shorter than the control implies");
+    free(j);
+    return NULL;
+  }
+  j->nflags = M + T;
+  j->flags = malloc((j->nflags ? j->nflags : 1) * sizeof(uint32_t));
+  if (!j->flags) {
+    free(j);
+    napi_throw_error(env, NULL, "out of host memory");
+    return NULL;
+  }
+  for (size_t k = 0; k < (j->offsets ? 7u : 6u); k++)
+    if (k != 1) napi_create_reference(env, argv[k], 1, &j->refs[j->nrefs++]);
+  napi_value promise, resource;
+  if (napi_create_promise(env, &j->deferred, &promise) != napi_ok ||
+      napi_create_string_utf8(env, "tcv_obs_evaluate", NAPI_AUTO_LENGTH, &resource) != napi_ok ||
+      napi_create_async_work(env, NULL, resource, obs_execute, obs_complete, j, &j->work) !=
+          napi_ok ||
+      napi_queue_async_work(env, j->work) != napi_ok) {
+    for (int k = 0; k < j->nrefs; k++) napi_delete_reference(env, j->refs[k]);
+    free(j->flags);
+    free(j);
+    napi_throw_error(env, NULL, "typecv_ba addon: could not queue the job");
+    return NULL;
+  }
+  return promise;
 }
 
 /* ---------------------------------------------------------------- asynchronous jobs */
@@ -462,6 +606,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"solve", NULL, solve, NULL, NULL, NULL, napi_enumerable, NULL},
       {"backward", NULL, backward, NULL, NULL, NULL, napi_enumerable, NULL},
       {"solveBatch", NULL, solve_batch, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"evaluate", NULL, evaluate, NULL, NULL, NULL, napi_enumerable, NULL},
   };
   napi_define_properties(env, exports, sizeof props / sizeof props[0], props);
   return exports;
