@@ -469,11 +469,6 @@ __device__ __forceinline__ bool large_skip(const double* s, int mode) {
   return (active == 0) | (direct != 1);
 }
 
-// Marks a loaded value as used here: the load is issued (and waited for) before what follows, so a
-// kernel's array loads go out together with its state-word loads instead of after the test of
-// those words. An empty asm statement: no instruction is emitted.
-__device__ __forceinline__ void settle(double& v) { asm volatile("" : "+v"(v)); }
-
 // Identity padding rows, a zero right-hand side beyond the unknowns, the failure word cleared.
 template <bool SHARED>
 __global__ __launch_bounds__(DENSE_THREADS) void k_lfact_begin(Problem q, uint32_t b) {

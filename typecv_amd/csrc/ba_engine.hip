@@ -275,13 +275,13 @@ int build_plan(const std::vector<uint32_t>& ps, const std::vector<uint32_t>& cs,
     }
     g.task1 = (uint32_t)(plan.tasks.size() / 4);
     // PCG-step groups of at most GROUP_CAMS cameras
-    g.group0 = (uint32_t)(plan.groups.size() / 3);
+    g.group0 = (uint32_t)(plan.groups.size() / GROUP_WORDS);
+    g.group1 = g.group0 + (g.cam1 - g.cam0 + GROUP_CAMS - 1) / GROUP_CAMS;
     for (uint32_t c = g.cam0; c < g.cam1; c += GROUP_CAMS) {
-      plan.groups.push_back(b);
-      plan.groups.push_back(c);
-      plan.groups.push_back(std::min<uint32_t>(c + GROUP_CAMS, g.cam1));
+      const uint32_t record[GROUP_WORDS] = {b, c, std::min<uint32_t>(c + GROUP_CAMS, g.cam1),
+                                            g.group0, g.group1, 0, 0, 0};
+      plan.groups.insert(plan.groups.end(), record, record + GROUP_WORDS);
     }
-    g.group1 = (uint32_t)(plan.groups.size() / 3);
     for (uint32_t p = g.pt0; p < g.pt1; p++) plan.point_bundle[p] = b;
     // cost blocks
     g.cost0 = (uint32_t)(plan.cost_blocks.size() / 3);
@@ -616,7 +616,7 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
   const uint32_t n_tiles = (uint32_t)plan.tiles.size() - 1;
   const uint32_t n_tasks = (uint32_t)plan.tasks.size() / 4;
   const uint32_t n_cost = (uint32_t)plan.cost_blocks.size() / 3;
-  const uint32_t n_groups = (uint32_t)plan.groups.size() / 3;
+  const uint32_t n_groups = (uint32_t)plan.groups.size() / GROUP_WORDS;
   const size_t w_words = (size_t)K * (((size_t)M + 63) / 64) * 64;
   const size_t vec_words = (size_t)m * 5 + (size_t)P * 3;
   TRY(ensure(ctx->scene, scene_size * 8));

@@ -71,6 +71,7 @@ constexpr int TASK_THREADS = 256;  // threads of a camera-task workgroup
 constexpr int TASK_OBS = 1024;     // most slots of a camera-list task (4 per thread)
 constexpr int ADV_THREADS = 256;   // threads of a PCG-step workgroup
 constexpr int GROUP_CAMS = 42;     // cameras of a PCG-step workgroup (6 rows each)
+constexpr int GROUP_WORDS = 8;     // words of a PCG-step group record
 constexpr int CG_THREADS = 1024;   // threads of the per-bundle PCG kernels
 constexpr int COST_OBS = 256;      // observations per cost block
 constexpr int ASM_WORDS = 56;      // per-task camera assembly sums (see k_camera_pass)
@@ -120,7 +121,8 @@ struct Problem {
   const uint32_t* cost_blocks;    // 3 per block: bundle, first and end observation
   const uint32_t* camera_bundle;  // C
   const uint32_t* point_bundle;   // P
-  const uint32_t* groups;         // 3 per PCG-step group: bundle, first and end camera
+  const uint32_t* groups;         // GROUP_WORDS per PCG-step group: bundle, first and end camera,
+                                  // the bundle's first and end group, 3 unused
   uint32_t* cam_point;
   void* Wp;
   void* Wc;

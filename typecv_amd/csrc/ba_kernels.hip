@@ -101,6 +101,26 @@ __device__ __forceinline__ uint32_t xcd_contiguous(uint32_t i, uint32_t n) {
 #endif
 }
 
+// The wave tile of a wave of a point-tile kernel (TILE_WAVES tiles per workgroup).
+__device__ __forceinline__ uint32_t point_tile(uint32_t wave) {
+#ifndef TCV_POINT_XCD
+#define TCV_POINT_XCD 1
+#endif
+#if TCV_POINT_XCD
+  return xcd_contiguous(blockIdx.x, gridDim.x) * TILE_WAVES + wave;
+#else
+  return blockIdx.x * TILE_WAVES + wave;
+#endif
+}
+
+// Marks a loaded value as used here: the load is issued (and waited for) before what follows, so a
+// kernel's array loads go out together with its state-word loads instead of after the test of
+// those words. An empty asm statement: no instruction is emitted.
+__device__ __forceinline__ void settle(double& v) { asm volatile("" : "+v"(v)); }
+__device__ __forceinline__ void settle(uint32_t& v) { asm volatile("" : "+v"(v)); }
+// The same for a value that is uniform across the wave (kept in scalar registers).
+__device__ __forceinline__ void settle_uniform(double& v) { asm volatile("" : "+s"(v)); }
+
 // Orders one wave's LDS writes before its later LDS reads. The point-tile kernels give every wave
 // its own LDS slice and their loops have per-wave trip counts, so they synchronise per wave, never
 // per workgroup.
@@ -496,7 +516,7 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_pass(Problem q) {
   __shared__ double fsum_all[TILE_WAVES][TILE][2];
   // the wave's index is uniform across its lanes: keep the tile's loads and branches scalar
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / TILE), lane = threadIdx.x % TILE;
-  const uint32_t t = blockIdx.x * TILE_WAVES + wave;
+  const uint32_t t = point_tile(wave);
   if (t >= q.n_tiles) return;
   const uint32_t b = q.tile_bundle[t];
   const double* sb = bstate(q, b);
@@ -988,7 +1008,7 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_sweep(Problem q) {
   __shared__ double st_all[TILE_WAVES][TILE][3];
   // the wave's index is uniform across its lanes: keep the tile's loads and branches scalar
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / TILE), lane = threadIdx.x % TILE;
-  const uint32_t t = blockIdx.x * TILE_WAVES + wave;
+  const uint32_t t = point_tile(wave);
   if (t >= q.n_tiles) return;
   // independent loads first: the tile's bounds and bundle, then the bundle's state word
   const uint32_t fp = q.tiles[t], ep = q.tiles[t + 1], b = q.tile_bundle[t];
@@ -997,7 +1017,7 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_sweep(Problem q) {
   if (MODE == SWEEP_CERTIFY && sb[S_DIRECT] != 1) return;
   if ((MODE == SWEEP_UPDATE || MODE == SWEEP_SOLVE) && (sb[S_ACTIVE] == 0 || sb[S_PCG_FAILED] != 0))
     return;
-  if (!stores_w(q.bundles[b])) return;
+  if (q.n_unstored != 0 && !stores_w(q.bundles[b])) return;
   const double* v = MODE == SWEEP_SCHUR ? vp(q) : vx(q);
   auto& st = st_all[wave];
   const uint32_t* ps = point_start(q);
@@ -1063,24 +1083,29 @@ __global__ __launch_bounds__(TASK_THREADS) void k_camera_sweep(Problem q) {
   __shared__ double red[TASK_THREADS / 64][NV];
   const uint32_t t = xcd_contiguous(blockIdx.x, gridDim.x);
   const uint32_t s = q.tasks[4 * t + 1], e = q.tasks[4 * t + 2], b = q.tasks[4 * t + 3];
-  if (CERTIFY ? bstate(q, b)[S_DIRECT] != 1 : bstate(q, b)[S_PCG_ACTIVE] == 0) return;
-  if (!stores_w(q.bundles[b])) return;
+  // the thread's (at most) 4 slots with no branch around a load: the point ids (loaded with the
+  // bundle's state word), then the gathered y, so that each round of loads is in flight at once; a
+  // slot past the task's end repeats its last slot with y scaled to zero (an exact zero term). A
+  // camera without observations has one empty task.
+  constexpr int PER = TASK_OBS / TASK_THREADS;
+  double state = bstate(q, b)[CERTIFY ? S_DIRECT : S_PCG_ACTIVE];
+  uint32_t slot[PER], pt[PER];
+  const uint32_t last = e > s ? e - 1 : 0;
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    slot[k] = min(s + threadIdx.x + TASK_THREADS * k, last);
+    pt[k] = e > s ? q.cam_point[slot[k]] : 0;
+  }
+  settle_uniform(state);
+#pragma unroll
+  for (int k = 0; k < PER; k++) settle(pt[k]);
+  if (CERTIFY ? state != 1 : state == 0) return;
+  if (q.n_unstored != 0 && !stores_w(q.bundles[b])) return;
   const double* y = vy(q);
   double acc[NV];
 #pragma unroll
   for (int k = 0; k < NV; k++) acc[k] = 0;
-  // the thread's (at most) 4 slots with no branch around a load: the point ids first, then the
-  // gathered y, so that each round of loads is in flight at once; a slot past the task's end
-  // repeats its last slot with y scaled to zero (an exact zero term). A camera without
-  // observations has one empty task.
-  constexpr int PER = TASK_OBS / TASK_THREADS;
   if (e > s) {
-    uint32_t slot[PER], pt[PER];
-#pragma unroll
-    for (int k = 0; k < PER; k++) {
-      slot[k] = min(s + threadIdx.x + TASK_THREADS * k, e - 1);
-      pt[k] = q.cam_point[slot[k]];
-    }
     double yv[PER][3];
 #pragma unroll
     for (int k = 0; k < PER; k++) {
@@ -1174,65 +1199,90 @@ __device__ double schur_finish(const Problem& q, const Bundle& g, const double* 
 // rho of the next iteration goes to word S_RHO_NEXT (the launch still reads word 18) and becomes
 // word 18 at the next k_cg_product.
 struct Group {
-  uint32_t bundle, c0, c1, row, rows;
+  uint32_t bundle, c0, c1, g0, g1, row, rows;
   bool has, first;
 };
 
+// The workgroup's group: one record of GROUP_WORDS words (bundle, first and end camera, the
+// bundle's first and end group), loaded as one scalar block.
 __device__ __forceinline__ Group group_of(const Problem& q) {
   Group G;
-  const uint32_t gi = blockIdx.x;
-  G.bundle = q.groups[3 * gi];
-  G.c0 = q.groups[3 * gi + 1];
-  G.c1 = q.groups[3 * gi + 2];
+  const uint32_t* w = q.groups + (size_t)GROUP_WORDS * blockIdx.x;
+  G.bundle = w[0];
+  G.c0 = w[1];
+  G.c1 = w[2];
+  G.g0 = w[3];
+  G.g1 = w[4];
   G.rows = 6 * (G.c1 - G.c0);
   G.row = 6 * G.c0 + threadIdx.x;
   G.has = threadIdx.x < G.rows;
-  G.first = gi == q.bundles[G.bundle].group0;
+  G.first = blockIdx.x == G.g0;
   return G;
 }
 
-// Sum over the bundle's groups of words w and w+1 of group_part (4 per group), in group order.
-__device__ __forceinline__ void group_sums(const Problem& q, const Bundle& g, int w, double& a,
-                                           double& b, double* lds) {
+// This thread's share of the sums over the bundle's groups of words w and w+1 of group_part (4 per
+// group); block_sum2 completes them, in group order.
+__device__ __forceinline__ void group_loads(const Problem& q, const Group& G, int w, double& a,
+                                            double& b) {
   a = 0;
   b = 0;
-  for (uint32_t k = g.group0 + threadIdx.x; k < g.group1; k += blockDim.x) {
+  for (uint32_t k = G.g0 + threadIdx.x; k < G.g1; k += blockDim.x) {
     a += q.group_part[4 * (size_t)k + w];
     b += q.group_part[4 * (size_t)k + w + 1];
   }
-  block_sum2(a, b, lds);
 }
 
+// Each launch below issues all of its loads (state words, partials, its rows) before it tests the
+// bundle's state: the chain of dependent loads from the group record to the first store is then
+// two loads deep (three in k_cg_product, whose task partials follow the camera's task range)
+// instead of one per test. Threads past the group's rows load row 6 c0 and store nothing.
 template <bool SHARED>
 __global__ __launch_bounds__(ADV_THREADS) void k_cg_product(Problem q) {
   constexpr int NV = SHARED ? 7 : 6;
   __shared__ double lds[2 * (ADV_THREADS / 64)];
   const Group G = group_of(q);
   double* s = bstate(q, G.bundle);
-  if (s[S_PCG_ACTIVE] == 0) return;
-  const Bundle& g = q.bundles[G.bundle];
   const double* p = vp(q);
+  const uint32_t row = G.has ? G.row : 6 * G.c0, c = row / 6, j = row - 6 * c;
+  double active = s[S_PCG_ACTIVE], rho_next = s[S_RHO_NEXT];
+  double pf = SHARED ? p[6 * q.C + G.bundle] : 0.0, hcf = SHARED ? q.camHcf[row] : 0.0;
+  double pr = p[row];
+  double A[6], pc[6], part[NV];
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    A[k] = q.camA[(size_t)c * 36 + j * 6 + k];
+    pc[k] = p[6 * (size_t)c + k];
+  }
+  camera_sums<NV>(q, c, part);
+  settle(active);
+  settle(rho_next);
+  settle(pf);
+  settle(hcf);
+  settle(pr);
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    settle(A[k]);
+    settle(pc[k]);
+  }
+#pragma unroll
+  for (int k = 0; k < NV; k++) settle(part[k]);
+  if (active == 0) return;
   double d = 0, fs = 0;
   if (G.has) {
-    const uint32_t c = G.row / 6, j = G.row - 6 * c;
-    double part[NV];
-    camera_sums<NV>(q, c, part);
-    const double pf = SHARED ? p[g.focal_row] : 0.0;
-    const double* A = q.camA + (size_t)c * 36 + j * 6;
-    double a = SHARED ? q.camHcf[G.row] * pf : 0.0, av = 0;
+    double a = SHARED ? hcf * pf : 0.0, av = 0;
 #pragma unroll
-    for (int k = 0; k < 6; k++) av += A[k] * p[6 * (size_t)c + k];
+    for (int k = 0; k < 6; k++) av += A[k] * pc[k];
     a += av;
     a -= part[j];
     vs(q)[G.row] = a;
-    d = p[G.row] * a;
-    if (SHARED) fs = q.camHcf[G.row] * p[G.row] - (j == 0 ? part[6] : 0.0);
+    d = pr * a;
+    if (SHARED) fs = hcf * pr - (j == 0 ? part[6] : 0.0);
   }
   block_sum2(d, fs, lds);
   if (threadIdx.x == 0) {
     q.group_part[4 * (size_t)blockIdx.x] = d;
     q.group_part[4 * (size_t)blockIdx.x + 1] = fs;
-    if (G.first) s[S_RHO] = s[S_RHO_NEXT];
+    if (G.first) s[S_RHO] = rho_next;
   }
 }
 
@@ -1242,62 +1292,85 @@ __global__ __launch_bounds__(ADV_THREADS) void k_cg_step(Problem q) {
   __shared__ double lds[2 * (ADV_THREADS / 64)];
   const Group G = group_of(q);
   double* s = bstate(q, G.bundle);
-  if (s[S_PCG_ACTIVE] == 0) return;
-  const Bundle& g = q.bundles[G.bundle];
-  const uint32_t tid = threadIdx.x;
-  double dot, fsum;
-  group_sums(q, g, 0, dot, fsum, lds);
+  const uint32_t tid = threadIdx.x, f = 6 * q.C + G.bundle;
+  const uint32_t row = G.has ? G.row : 6 * G.c0, c = row / 6, j = row - 6 * c;
+  double* x = vx(q);
+  double* r = vr(q);
+  double* z = vz(q);
+  double* sv = vs(q);
   const double* p = vp(q);
+  double active = s[S_PCG_ACTIVE], rho = s[S_RHO];
+  double fdiag = SHARED ? s[S_FOCAL_DIAG] : 0.0, finv = SHARED ? s[S_FOCAL_INV] : 0.0;
+  double dot, fsum;
+  group_loads(q, G, 0, dot, fsum);
+  double xr = x[row], pr = p[row], rv = r[row], svr = sv[row];
+  double pf = SHARED ? p[f] : 0.0, xf = SHARED ? x[f] : 0.0, rf = SHARED ? r[f] : 0.0;
+  double inv[6];
+#pragma unroll
+  for (int k = 0; k < 6; k++) inv[k] = q.camInv[(size_t)c * 36 + j * 6 + k];
+  settle(active);
+  settle(rho);
+  settle(fdiag);
+  settle(finv);
+  settle(dot);
+  settle(fsum);
+  settle(xr);
+  settle(pr);
+  settle(rv);
+  settle(svr);
+  settle(pf);
+  settle(xf);
+  settle(rf);
+#pragma unroll
+  for (int k = 0; k < 6; k++) settle(inv[k]);
+  if (active == 0) return;
+  block_sum2(dot, fsum, lds);
   double den = dot, sf = 0;
   if (SHARED) {
-    const double pf = p[g.focal_row];
-    sf = s[S_FOCAL_DIAG] * pf + fsum;
+    sf = fdiag * pf + fsum;
     den += pf * sf;
   }
-  const double alpha = s[S_RHO] / den;
+  const double alpha = rho / den;
   const bool broke = !(den > 0) || !(fabs(alpha) < 1e20);
   if (G.first && tid == 0) {
     s[S_MATVECS] += 1;
-    if (SHARED) vs(q)[g.focal_row] = sf;
+    if (SHARED) sv[f] = sf;
     if (broke) {
       s[S_PCG_ACTIVE] = 0;
       s[S_PCG_FAILED] = 1;
     }
   }
   if (broke) return;
-  double* x = vx(q);
-  double* r = vr(q);
-  double* z = vz(q);
-  double res = 0, rho = 0, ri = 0;
+  double res = 0, rz = 0, ri = 0;
   if (G.has) {
-    x[G.row] += alpha * p[G.row];
-    ri = r[G.row] - alpha * vs(q)[G.row];
+    x[G.row] = xr + alpha * pr;
+    ri = rv - alpha * svr;
     r[G.row] = ri;
     rr[tid] = ri;
     res = ri * ri;
   }
   __syncthreads();
   if (G.has) {
-    const uint32_t j = tid % 6, base = tid - j;
-    const double* inv = q.camInv + (size_t)(G.row / 6) * 36 + j * 6;
+    const uint32_t base = tid - j;
     double zi = 0;
 #pragma unroll
     for (int k = 0; k < 6; k++) zi += inv[k] * rr[base + k];
     z[G.row] = zi;
-    rho = zi * ri;
+    rz = zi * ri;
   }
   if (SHARED && G.first && tid == 0) {
-    const uint32_t f = g.focal_row;
-    x[f] += alpha * p[f];
-    r[f] -= alpha * sf;
-    res += r[f] * r[f];
-    z[f] = s[S_FOCAL_INV] * r[f];
-    rho += z[f] * r[f];
+    x[f] = xf + alpha * pf;
+    const double rn = rf - alpha * sf;
+    r[f] = rn;
+    res += rn * rn;
+    const double zf = finv * rn;
+    z[f] = zf;
+    rz += zf * rn;
   }
-  block_sum2(res, rho, lds);
+  block_sum2(res, rz, lds);
   if (tid == 0) {
     q.group_part[4 * (size_t)blockIdx.x + 2] = res;
-    q.group_part[4 * (size_t)blockIdx.x + 3] = rho;
+    q.group_part[4 * (size_t)blockIdx.x + 3] = rz;
   }
 }
 
@@ -1306,13 +1379,28 @@ __global__ __launch_bounds__(ADV_THREADS) void k_cg_direction(Problem q) {
   __shared__ double lds[2 * (ADV_THREADS / 64)];
   const Group G = group_of(q);
   double* s = bstate(q, G.bundle);
-  if (s[S_PCG_ACTIVE] == 0) return;
-  const Bundle& g = q.bundles[G.bundle];
+  const uint32_t f = 6 * q.C + G.bundle, row = G.has ? G.row : 6 * G.c0;
+  double* p = vp(q);
+  const double* z = vz(q);
+  double active = s[S_PCG_ACTIVE], rho_old = s[S_RHO], r0 = s[S_R0];
   double res, rho;
-  group_sums(q, g, 2, res, rho, lds);
-  const bool converged = res <= q.tol * q.tol * s[S_R0];
+  group_loads(q, G, 2, res, rho);
+  double pr = p[row], zr = z[row];
+  double pf = SHARED ? p[f] : 0.0, zf = SHARED ? z[f] : 0.0;
+  settle(active);
+  settle(rho_old);
+  settle(r0);
+  settle(res);
+  settle(rho);
+  settle(pr);
+  settle(zr);
+  settle(pf);
+  settle(zf);
+  if (active == 0) return;
+  block_sum2(res, rho, lds);
+  const bool converged = res <= q.tol * q.tol * r0;
   const bool broke = !converged && !(rho > 0);
-  const double beta = rho / s[S_RHO];
+  const double beta = rho / rho_old;
   if (G.first && threadIdx.x == 0) {
     s[S_CG_STEPS] += 1;
     s[S_STEPS] += 1;
@@ -1328,10 +1416,8 @@ __global__ __launch_bounds__(ADV_THREADS) void k_cg_direction(Problem q) {
     }
   }
   if (converged || broke) return;
-  double* p = vp(q);
-  const double* z = vz(q);
-  if (G.has) p[G.row] = z[G.row] + beta * p[G.row];
-  if (SHARED && G.first && threadIdx.x == 0) p[g.focal_row] = z[g.focal_row] + beta * p[g.focal_row];
+  if (G.has) p[G.row] = zr + beta * pr;
+  if (SHARED && G.first && threadIdx.x == 0) p[f] = zf + beta * pf;
 }
 
 // The linear-system probe (bundle 0): S v, M^-1 v and the RHS of the assembled system.
@@ -1383,7 +1469,7 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_sweep_implicit(Probl
   __shared__ double st_all[TILE_WAVES][TILE][3];
   // the wave's index is uniform across its lanes: keep the tile's loads and branches scalar
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / TILE), lane = threadIdx.x % TILE;
-  const uint32_t t = blockIdx.x * TILE_WAVES + wave;
+  const uint32_t t = point_tile(wave);
   if (t >= q.n_tiles) return;
   const uint32_t b = q.tile_bundle[t];
   const double* sb = bstate(q, b);
