@@ -1077,8 +1077,15 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_sweep(Problem q) {
 // The camera sweep: one workgroup per camera-list task streams the camera-major cross blocks and
 // gathers y of each slot's point: part[t] = sum_n W_n y_p(n) (6) and sum_n w_fn . y_p(n) (focal).
 // CERTIFY: for the bundles with an uncertified direct solution instead of those running PCG.
+// With FP32 cross blocks the sweep is held to exactly five waves per SIMD (81 registers a lane):
+// its loads are half as large, and more waves in flight beat more loads per thread (40.0 against
+// 42.4 us per sweep at 500 cameras; five as a lower bound only, the compiler takes 96 registers
+// and the sweep 51 us). With FP64 ones the compiler's own choice (four waves, 100 registers) is
+// the faster (62 against 82 us when held to five).
 template <class WT, bool SHARED, bool CERTIFY = false>
-__global__ __launch_bounds__(TASK_THREADS) void k_camera_sweep(Problem q) {
+__global__ __launch_bounds__(TASK_THREADS)
+__attribute__((amdgpu_waves_per_eu(sizeof(WT) == 4 ? 5 : 1, sizeof(WT) == 4 ? 5 : 8)))
+void k_camera_sweep(Problem q) {
   constexpr int NV = SHARED ? 7 : 6;
   __shared__ double red[TASK_THREADS / 64][NV];
   const uint32_t t = xcd_contiguous(blockIdx.x, gridDim.x);
