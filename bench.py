@@ -166,6 +166,8 @@ def main() -> int:
             "camera_sweep_ms": camera_ms / max(1, timed), "bytes": bytes_per,
             "timed_products": timed,
             "share_of_solve": mv_ms * matvecs / max(1e-9, sum(r["solve_ms"] for r in reports)),
+            # camera-sweep workgroups per CU picked by the context's schedule tuner (0: no cap)
+            "camera_sweep_cap": reports[-1]["camera_sweep_cap"],
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -117,6 +117,9 @@ typedef struct tcv_ba_report {
   uint64_t timed_matvecs;   /* Schur products timed into schur_ms (every 8th, TCV_BA_FLAG_TIMING) */
   uint64_t direct_corrections; /* refinement corrections of direct solutions from their factor
                                   (one per bundle whose certificate missed, at most two per solve) */
+  uint64_t camera_sweep_cap;   /* camera-sweep workgroups per CU that the PCG launches used, as
+                                  picked by the context's schedule tuner for this problem shape
+                                  (0: as many as the sweep's registers allow) */
 } tcv_ba_report;
 
 typedef struct tcv_ba_context tcv_ba_context;

@@ -1,0 +1,35 @@
+"""The camera sweep's schedule tuner changes speed only.
+
+A context's first PCG chunk on a new problem shape runs iterations 1..6 with the two candidate
+residencies in turn (no cap, and three workgroups per CU by LDS reservation) and keeps the
+faster; the partition into camera tasks and every sum stay the same. So a solve that tunes and a
+solve of the same problem that runs with the picked cap return the same bits, in both arithmetic
+forms.
+"""
+import numpy as np
+import pytest
+
+from typecv_amd import bundle as B
+from typecv_amd import synthetic as S
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_tuned_schedule_keeps_the_bits(precision):
+    packed, _ = S.trajectory_scene(cameras=60, points=8000, seed=5)
+    control = B.make_control(packed.C, packed.P, packed.M, shared_focal=True, huber=1.0,
+                             iterations=6, max_cg=64, precision=precision)
+    print(f"\n[schedule] {precision} C={packed.C} P={packed.P} M={packed.M}", flush=True)
+    device = B.HipDevice(0)  # a fresh context: its first solve tunes
+    try:
+        first, r1 = B.solve_packed(device, control, packed)
+        second, r2 = B.solve_packed(device, control, packed)
+    finally:
+        device.dispose()
+    print(f"[schedule] cap {r1['camera_sweep_cap']} / {r2['camera_sweep_cap']}, "
+          f"PCG {r1['cg_iterations']}", flush=True)
+    assert r1["cg_iterations"] > 6
+    assert r1["camera_sweep_cap"] in (0, 3)
+    assert r2["camera_sweep_cap"] == r1["camera_sweep_cap"]
+    assert np.array_equal(first, second)

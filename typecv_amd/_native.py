@@ -91,6 +91,7 @@ class Report(ctypes.Structure):
         ("direct_solves", ctypes.c_uint64),
         ("timed_matvecs", ctypes.c_uint64),
         ("direct_corrections", ctypes.c_uint64),
+        ("camera_sweep_cap", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:

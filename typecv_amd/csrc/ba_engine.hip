@@ -62,6 +62,11 @@ struct tcv_ba_context {
   std::vector<uint32_t> large_bundles, large_n;
   std::vector<uint64_t> large_off;
   std::vector<hipEvent_t> events;
+  // the schedule tuner (run_pcg): the shape of the prepared problem, the shape the camera sweep's
+  // residency was last tuned for and the workgroups per CU picked for it, and its events
+  std::vector<uint64_t> shape, tuned_shape;
+  uint32_t camera_cap = 0;
+  std::vector<hipEvent_t> tune_events;
   tcv_ba_context() {
     all = {&scene,      &trial,        &index,     &out,       &bundles,    &tiles,
            &tile_bundle, &tasks,       &task_start, &cost_blocks, &camera_bundle,
@@ -346,7 +351,8 @@ struct Launch {
                        dim3(TILE * TILE_WAVES), 0, s, q);
   }
   static void camera_sweep(const Problem& q, hipStream_t s) {
-    hipLaunchKernelGGL((k_camera_sweep<WT, SHARED>), dim3(q.n_tasks), dim3(TASK_THREADS), 0, s, q);
+    hipLaunchKernelGGL((k_camera_sweep<WT, SHARED>), dim3(q.n_tasks), dim3(TASK_THREADS),
+                       q.camera_lds, s, q);
   }
   static void advance(const Problem& q, hipStream_t s) {
     hipLaunchKernelGGL((k_cg_product<SHARED>), dim3(q.n_groups), dim3(ADV_THREADS), 0, s, q);
@@ -770,6 +776,8 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
     hipLaunchKernelGGL(k_covisibility, dim3(blocks_for(P, 256)), dim3(256), 0, st, q);
     HIP_TRY(hipGetLastError());
   }
+  // the shape that the camera sweep's schedule is tuned for
+  ctx->shape = {C, P, M, B, n_tasks, ws, shared, q.n_unstored};
   if (rep) {
     rep->setup_ms = now_ms() - t0;
     rep->point_tiles = n_tiles;
@@ -787,6 +795,44 @@ int ensure_events(tcv_ba_context* ctx, size_t n) {
   return TCV_BA_OK;
 }
 
+// The camera sweep's schedule tuner. Its candidate caps the workgroups a CU holds at once by
+// reserving LDS per workgroup (160 KiB per CU; at most 63 KiB, which needs no opt-in); 0 leaves
+// the count to the sweep's registers. The cap changes neither the partition into tasks nor any
+// sum, so both candidates compute the same bits; only their speed differs (FP64 cross blocks on
+// the 500-camera scene: 61.6 us with no cap, 60.2 us at three workgroups per CU, 60.3 at two;
+// FP32: 39.9 us with no cap, 41.8 at three). The first PCG chunk of a problem shape that has not
+// been tuned runs its iterations 1..6 with the candidates in turn (each iteration's one camera
+// sweep, so nothing is added), times them with HIP events and keeps the cap when its three
+// sweeps beat no cap's by 1.5 %.
+constexpr uint32_t kTuneCaps[2] = {0, 3};
+constexpr uint32_t kTuneFirst = 1, kTuneRuns = 6;  // iterations [1, 7) of the first chunk
+
+uint32_t camera_lds(uint32_t cap) {
+  return cap ? std::min<uint32_t>(163840 / cap - 1024, 64512) : 0;
+}
+
+int ensure_tune_events(tcv_ba_context* ctx) {
+  while (ctx->tune_events.size() < 2 * kTuneRuns) {
+    hipEvent_t e;
+    HIP_TRY(hipEventCreate(&e));
+    ctx->tune_events.push_back(e);
+  }
+  return TCV_BA_OK;
+}
+
+// After the tuned chunk has been read back: the candidates' summed times, the pick.
+int tune_pick(tcv_ba_context* ctx) {
+  double total[2] = {0, 0};
+  for (uint32_t k = 0; k < kTuneRuns; k++) {
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->tune_events[2 * k], ctx->tune_events[2 * k + 1]));
+    total[k % 2] += ms;
+  }
+  ctx->camera_cap = total[1] < 0.985 * total[0] ? kTuneCaps[1] : kTuneCaps[0];
+  ctx->tuned_shape = ctx->shape;
+  return TCV_BA_OK;
+}
+
 // PCG in chunks with one read of the state words after each: the first chunk `first` iterations,
 // later ones kChunk, at most `budget` in all. A chunk that outlives convergence costs only guarded
 // no-op launches (state word 19), as the reference's chunks do. With TCV_BA_FLAG_TIMING the two
@@ -798,16 +844,35 @@ int run_pcg(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, cons
   const bool timing = (c->flags & TCV_BA_FLAG_TIMING) != 0 && rep;
   uint32_t done = 0, chunk = first;
   double matvecs_before = max_word(ctx, q, S_MATVECS);
+  // the camera sweep's schedule: tuned in the first chunk when this shape has not been (the
+  // stored-block sweep only; the tuning iterations are not multiples of kTimingStride)
+  Problem qt = q;
+  const bool tuned = ctx->tuned_shape == ctx->shape;
+  bool tune = !tuned && ops.unstored_camera_sweep != nullptr && q.n_unstored < q.B &&
+              std::min(first, budget) >= kTuneFirst + kTuneRuns;
+  static_assert(kTuneFirst > 0 && kTuneFirst + kTuneRuns <= kTimingStride, "tuning iterations");
+  if (tune) TRY(ensure_tune_events(ctx));
+  qt.camera_lds = tuned ? camera_lds(ctx->camera_cap) : 0;
   while (done < budget) {
     const uint32_t n = chunk < budget - done ? chunk : budget - done;
     for (uint32_t k = 0; k < n; k++) {
       const bool timed = timing && k % kTimingStride == 0;
+      const bool trial = tune && k >= kTuneFirst && k < kTuneFirst + kTuneRuns;
       if (timed) HIP_TRY(hipEventRecord(ctx->events[3 * k], st));
-      point_sweeps(q, ops, st);
+      point_sweeps(qt, ops, st);
       if (timed) HIP_TRY(hipEventRecord(ctx->events[3 * k + 1], st));
-      camera_sweeps(q, ops, st);
+      if (trial) {
+        const uint32_t r = k - kTuneFirst;
+        Problem qc = qt;
+        qc.camera_lds = camera_lds(kTuneCaps[r % 2]);
+        HIP_TRY(hipEventRecord(ctx->tune_events[2 * r], st));
+        camera_sweeps(qc, ops, st);
+        HIP_TRY(hipEventRecord(ctx->tune_events[2 * r + 1], st));
+      } else {
+        camera_sweeps(qt, ops, st);
+      }
       if (timed) HIP_TRY(hipEventRecord(ctx->events[3 * k + 2], st));
-      ops.advance(q, st);
+      ops.advance(qt, st);
     }
     HIP_TRY(hipGetLastError());
     if (rep) rep->matvec_launches += n;
@@ -828,9 +893,15 @@ int run_pcg(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, cons
         rep->timed_matvecs++;
       }
     }
+    if (tune) {
+      TRY(tune_pick(ctx));
+      qt.camera_lds = camera_lds(ctx->camera_cap);
+      tune = false;
+    }
     if (!any_word(ctx, q, S_PCG_ACTIVE)) break;
     chunk = kChunk;
   }
+  if (rep) rep->camera_sweep_cap = qt.camera_lds ? ctx->camera_cap : 0;
   return TCV_BA_OK;
 }
 
@@ -1253,6 +1324,7 @@ void tcv_ba_destroy(tcv_ba_context* ctx) {
   (void)hipSetDevice(ctx->device);
   tcv_ba_release(ctx);
   for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
+  for (hipEvent_t e : ctx->tune_events) (void)hipEventDestroy(e);
   if (ctx->h_state) (void)hipHostFree(ctx->h_state);
   if (ctx->h_error) (void)hipHostFree(ctx->h_error);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

@@ -162,6 +162,9 @@ struct Problem {
                    // cameras c > d observe a common point
   uint32_t n_small;                // direct bundles factored by one workgroup each
   const uint32_t* small_bundles;
+  // launch schedule, read by the host only: dynamic LDS bytes reserved per camera-sweep workgroup,
+  // which caps the workgroups a CU holds at once (0: no reservation)
+  uint32_t camera_lds;
 };
 
 __host__ __device__ inline const uint32_t* obs_camera(const Problem& q) { return q.index; }
