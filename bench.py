@@ -32,7 +32,7 @@ HBM_PEAK_GBS = 8000.0
 # factors that make a 1 GiB device copy read and write 1 GiB. Keyed by
 # (cameras, points, seed, precision, operator); other workloads report null.
 MEASURED_TRAFFIC = {
-    (500, 200_000, 2, "fp64", "stored"): 2 * (185204.6 + 178822.0) * 1024 + (4879.8 + 125.1) * 1024,
+    (500, 200_000, 2, "fp64", "stored"): 2 * (175330.7 + 175545.0) * 1024 + (4688.1 + 125.1) * 1024,
 }
 
 
