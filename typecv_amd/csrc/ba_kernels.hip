@@ -996,6 +996,27 @@ __global__ __launch_bounds__(CG_THREADS) void k_init_cg(Problem q) {
   }
 }
 
+// acc += st[j - base] for j in [lo, hi), in that order: four rows' LDS reads are issued together
+// and waited for once, the adds stay one after another, so the sums are the plain loop's.
+__device__ __forceinline__ void segment_sum(const double (*st)[3], uint32_t base, uint32_t lo,
+                                            uint32_t hi, double acc[3]) {
+  uint32_t j = lo;
+  for (; j + 4 <= hi; j += 4) {
+    double r[4][3];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int k = 0; k < 3; k++) r[u][k] = st[j + u - base][k];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int k = 0; k < 3; k++) acc[k] += r[u][k];
+  }
+  for (; j < hi; j++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) acc[k] += st[j - base][k];
+}
+
 // The point sweep over tiles of whole points, one lane per observation.
 //   SCHUR:  y_p = V_p^-1 sum_i (W_i^T v_c(i) + w_fi v_f)                (bundleSchurPointAt)
 //   UPDATE: X_p' = X_p + V_p^-1 (b_p - sum_i (W_i^T x_c(i) + w_fi x_f))  (updateBundleCandidate)
@@ -1003,13 +1024,10 @@ __global__ __launch_bounds__(CG_THREADS) void k_init_cg(Problem q) {
 //   CERTIFY: as SCHUR on x, for the bundles with an uncertified direct solution
 enum SweepMode { SWEEP_SCHUR = 0, SWEEP_UPDATE = 1, SWEEP_SOLVE = 2, SWEEP_CERTIFY = 3 };
 
+// One tile t of the point sweep by one wave (st: the wave's LDS slice).
 template <class WT, bool SHARED, int MODE>
-__global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_sweep(Problem q) {
-  __shared__ double st_all[TILE_WAVES][TILE][3];
-  // the wave's index is uniform across its lanes: keep the tile's loads and branches scalar
-  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / TILE), lane = threadIdx.x % TILE;
-  const uint32_t t = point_tile(wave);
-  if (t >= q.n_tiles) return;
+__device__ __forceinline__ void sweep_tile(const Problem& q, uint32_t t, uint32_t lane,
+                                           double (*st)[3]) {
   // independent loads first: the tile's bounds and bundle, then the bundle's state word
   const uint32_t fp = q.tiles[t], ep = q.tiles[t + 1], b = q.tile_bundle[t];
   const double* sb = bstate(q, b);
@@ -1019,7 +1037,6 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_sweep(Problem q) {
     return;
   if (q.n_unstored != 0 && !stores_w(q.bundles[b])) return;
   const double* v = MODE == SWEEP_SCHUR ? vp(q) : vx(q);
-  auto& st = st_all[wave];
   const uint32_t* ps = point_start(q);
   const uint32_t s = ps[fp], e = ps[ep];
   const uint32_t my = fp + lane;
@@ -1049,11 +1066,7 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_sweep(Problem q) {
     wave_sync();
     if (has) {
       const uint32_t lo = mys > base ? mys : base, hi = mye < base + TILE ? mye : base + TILE;
-      for (uint32_t j = lo; j < hi; j++) {
-        acc[0] += st[j - base][0];
-        acc[1] += st[j - base][1];
-        acc[2] += st[j - base][2];
-      }
+      segment_sum(st, base, lo, hi, acc);
     }
     wave_sync();
   }
@@ -1072,6 +1085,16 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_sweep(Problem q) {
     double* y = vy(q) + (size_t)my * 3;
     for (int j = 0; j < 3; j++) y[j] = pb[j * 3] * r0 + pb[j * 3 + 1] * r1 + pb[j * 3 + 2] * r2;
   }
+}
+
+template <class WT, bool SHARED, int MODE>
+__global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_sweep(Problem q) {
+  __shared__ double st_all[TILE_WAVES][TILE][3];
+  // the wave's index is uniform across its lanes: keep the tile's loads and branches scalar
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / TILE), lane = threadIdx.x % TILE;
+  const uint32_t t = point_tile(wave);
+  if (t >= q.n_tiles) return;
+  sweep_tile<WT, SHARED, MODE>(q, t, lane, st_all[wave]);
 }
 
 // The camera sweep: one workgroup per camera-list task streams the camera-major cross blocks and
@@ -1517,11 +1540,7 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_sweep_implicit(Probl
     wave_sync();
     if (has) {
       const uint32_t lo = mys > base ? mys : base, hi = mye < base + TILE ? mye : base + TILE;
-      for (uint32_t j = lo; j < hi; j++) {
-        acc[0] += st[j - base][0];
-        acc[1] += st[j - base][1];
-        acc[2] += st[j - base][2];
-      }
+      segment_sum(st, base, lo, hi, acc);
     }
     wave_sync();
   }
