@@ -168,6 +168,8 @@ def main() -> int:
             "share_of_solve": mv_ms * matvecs / max(1e-9, sum(r["solve_ms"] for r in reports)),
             # camera-sweep workgroups per CU picked by the context's schedule tuner (0: no cap)
             "camera_sweep_cap": reports[-1]["camera_sweep_cap"],
+            # 1: the tuner picked the fused FP64 product (k_point_sweep_fused + k_camera_tsum)
+            "fused_product": reports[-1]["fused_product"],
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -120,6 +120,9 @@ typedef struct tcv_ba_report {
   uint64_t camera_sweep_cap;   /* camera-sweep workgroups per CU that the PCG launches used, as
                                   picked by the context's schedule tuner for this problem shape
                                   (0: as many as the sweep's registers allow) */
+  uint64_t fused_product;      /* 1: the PCG launches used the fused FP64 Schur product (one read
+                                  of the cross blocks, a row of products per observation), as
+                                  picked by the same tuner; 0: the two plain sweeps */
 } tcv_ba_report;
 
 typedef struct tcv_ba_context tcv_ba_context;

@@ -1,10 +1,10 @@
-"""The camera sweep's schedule tuner changes speed only.
+"""The Schur product's schedule tuner changes speed only.
 
-A context's first PCG chunk on a new problem shape runs iterations 1..6 with the two candidate
-residencies in turn (no cap, and three workgroups per CU by LDS reservation) and keeps the
-faster; the partition into camera tasks and every sum stay the same. So a solve that tunes and a
-solve of the same problem that runs with the picked cap return the same bits, in both arithmetic
-forms.
+A context's first PCG chunk on a new problem shape runs iterations 1..6 with the three candidate
+schedules in turn (the plain sweeps with no cap, with the camera sweep capped at three workgroups
+per CU by LDS reservation, and the fused FP64 product) and keeps the fastest; no schedule changes
+the task partition or any sum. So a solve that tunes (and so mixes all three) and a later solve of
+the same problem that runs only the picked one return the same bits, in both arithmetic forms.
 """
 import numpy as np
 import pytest
@@ -27,9 +27,12 @@ def test_tuned_schedule_keeps_the_bits(precision):
         second, r2 = B.solve_packed(device, control, packed)
     finally:
         device.dispose()
-    print(f"[schedule] cap {r1['camera_sweep_cap']} / {r2['camera_sweep_cap']}, "
-          f"PCG {r1['cg_iterations']}", flush=True)
+    print(f"[schedule] cap {r1['camera_sweep_cap']} / {r2['camera_sweep_cap']}, fused "
+          f"{r1['fused_product']} / {r2['fused_product']}, PCG {r1['cg_iterations']}", flush=True)
     assert r1["cg_iterations"] > 6
     assert r1["camera_sweep_cap"] in (0, 3)
     assert r2["camera_sweep_cap"] == r1["camera_sweep_cap"]
+    assert r1["fused_product"] in (0, 1) and r2["fused_product"] == r1["fused_product"]
+    if precision == "mixed":
+        assert r1["fused_product"] == 0
     assert np.array_equal(first, second)

@@ -92,6 +92,7 @@ class Report(ctypes.Structure):
         ("timed_matvecs", ctypes.c_uint64),
         ("direct_corrections", ctypes.c_uint64),
         ("camera_sweep_cap", ctypes.c_uint64),
+        ("fused_product", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:

@@ -56,6 +56,7 @@ struct tcv_ba_context {
       camera_bundle, point_bundle, cam_point, Wp, Wc, pblk, tile_fail, tile_focal, task_part, camA,
       camInv, camRhs, camHcf, camFocal, vec, state, cost_part, error, probe, groups, group_part,
       rhs, grad, camS, camG, pfc, dense, direct_bundles, direct_cams, covis, small_bundles,
+      tprod,
       obs_cameras, obs_points, obs_pixels, obs_indices, obs_offsets, obs_flags, obs_errors,
       obs_partials, obs_summary;
   // the direct bundles factored by a sequence of launches, and their dense array sizes
@@ -66,6 +67,7 @@ struct tcv_ba_context {
   // residency was last tuned for and the workgroups per CU picked for it, and its events
   std::vector<uint64_t> shape, tuned_shape;
   uint32_t camera_cap = 0;
+  bool tuned_fused = false;
   std::vector<hipEvent_t> tune_events;
   tcv_ba_context() {
     all = {&scene,      &trial,        &index,     &out,       &bundles,    &tiles,
@@ -74,7 +76,7 @@ struct tcv_ba_context {
            &tile_focal, &task_part,    &camA,      &camInv,    &camRhs,     &camHcf,
            &camFocal,   &vec,          &state,     &cost_part, &error,      &probe,
            &groups,     &group_part,   &rhs,       &grad,      &camS,       &camG,   &pfc,
-           &dense,      &direct_bundles, &direct_cams, &covis, &small_bundles,
+           &dense,      &direct_bundles, &direct_cams, &covis, &small_bundles, &tprod,
            &obs_cameras, &obs_points, &obs_pixels, &obs_indices, &obs_offsets, &obs_flags,
            &obs_errors, &obs_partials, &obs_summary};
   }
@@ -347,12 +349,20 @@ struct Launch {
     hipLaunchKernelGGL((k_init_cg<SHARED>), dim3(q.B), dim3(CG_THREADS), 0, s, q);
   }
   static void point_sweep(const Problem& q, hipStream_t s) {
-    hipLaunchKernelGGL((k_point_sweep<WT, SHARED, SWEEP_SCHUR>), dim3(tile_blocks(q)),
-                       dim3(TILE * TILE_WAVES), 0, s, q);
+    if (sizeof(WT) == 8 && q.tprod)
+      hipLaunchKernelGGL((k_point_sweep_fused<SHARED>), dim3(tile_blocks(q)),
+                         dim3(TILE * TILE_WAVES), 0, s, q);
+    else
+      hipLaunchKernelGGL((k_point_sweep<WT, SHARED, SWEEP_SCHUR>), dim3(tile_blocks(q)),
+                         dim3(TILE * TILE_WAVES), 0, s, q);
   }
   static void camera_sweep(const Problem& q, hipStream_t s) {
-    hipLaunchKernelGGL((k_camera_sweep<WT, SHARED>), dim3(q.n_tasks), dim3(TASK_THREADS),
-                       q.camera_lds, s, q);
+    if (sizeof(WT) == 8 && q.tprod)
+      hipLaunchKernelGGL((k_camera_tsum<SHARED>), dim3(q.n_tasks), dim3(TASK_THREADS),
+                         q.camera_lds, s, q);
+    else
+      hipLaunchKernelGGL((k_camera_sweep<WT, SHARED>), dim3(q.n_tasks), dim3(TASK_THREADS),
+                         q.camera_lds, s, q);
   }
   static void advance(const Problem& q, hipStream_t s) {
     hipLaunchKernelGGL((k_cg_product<SHARED>), dim3(q.n_groups), dim3(ADV_THREADS), 0, s, q);
@@ -632,6 +642,13 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
     TRY(ensure(ctx->Wp, w_words * ws));
     TRY(ensure(ctx->Wc, w_words * ws));
   }
+#ifndef TCV_FUSED
+#define TCV_FUSED 1
+#endif
+  // the fused FP64 product's rows (FP32 blocks keep the two plain sweeps: t stays FP64, so its
+  // round trip would outweigh the halved cross-block reads)
+  const bool fused = TCV_FUSED && ws == 8 && n_unstored < B;
+  if (fused) TRY(ensure(ctx->tprod, (size_t)M * 64));
   TRY(ensure(ctx->pblk, (size_t)P * 12 * 8));
   TRY(ensure(ctx->tile_fail, (size_t)n_tiles * 4));
   TRY(ensure(ctx->tile_focal, (size_t)n_tiles * 16));
@@ -718,6 +735,7 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
   q.state = (double*)ctx->state.ptr;
   q.cost_part = (double*)ctx->cost_part.ptr;
   q.error = (uint32_t*)ctx->error.ptr;
+  q.tprod = fused ? (double*)ctx->tprod.ptr : nullptr;
   if (direct) {
     q.n_direct = (uint32_t)plan.direct_bundles.size();
     q.n_direct_cams = (uint32_t)plan.direct_cams.size();
@@ -795,20 +813,34 @@ int ensure_events(tcv_ba_context* ctx, size_t n) {
   return TCV_BA_OK;
 }
 
-// The camera sweep's schedule tuner. Its candidate caps the workgroups a CU holds at once by
-// reserving LDS per workgroup (160 KiB per CU; at most 63 KiB, which needs no opt-in); 0 leaves
-// the count to the sweep's registers. The cap changes neither the partition into tasks nor any
-// sum, so both candidates compute the same bits; only their speed differs (FP64 cross blocks on
-// the 500-camera scene: 61.6 us with no cap, 60.2 us at three workgroups per CU, 60.3 at two;
-// FP32: 39.9 us with no cap, 41.8 at three). The first PCG chunk of a problem shape that has not
-// been tuned runs its iterations 1..6 with the candidates in turn (each iteration's one camera
-// sweep, so nothing is added), times them with HIP events and keeps the cap when its three
-// sweeps beat no cap's by 1.5 %.
-constexpr uint32_t kTuneCaps[2] = {0, 3};
+// The Schur product's schedule tuner. Its candidates differ in speed only, never in a bit: the
+// two plain sweeps with the camera sweep's residency left to its registers or capped at three
+// workgroups per CU (an LDS reservation per workgroup: 160 KiB per CU, at most 63 KiB, which
+// needs no opt-in), and, with FP64 cross blocks, the fused product (k_point_sweep_fused +
+// k_camera_tsum, whose 64-byte rows of t per observation pay off while the device's last-level
+// cache holds them). Neither the task partition nor any sum changes. Measured at 500 cameras
+// (FP64): plain 61.6 + 66.1 us, capped camera sweep 60.2 us, fused 95.0 + 27.6 us (-3.5 %); at
+// 2,000 cameras the fused product is 10 % slower (its 621 MB of rows miss that cache). The first
+// PCG chunk of a problem shape not yet tuned runs its iterations 1..6 with the candidates in turn
+// (each iteration's one product, so nothing is added), times each product with HIP events and
+// keeps the fastest candidate.
+struct TuneCandidate {
+  bool fused;
+  uint32_t cap;
+};
+constexpr TuneCandidate kTune[3] = {{false, 0}, {false, 3}, {true, 0}};
 constexpr uint32_t kTuneFirst = 1, kTuneRuns = 6;  // iterations [1, 7) of the first chunk
 
 uint32_t camera_lds(uint32_t cap) {
   return cap ? std::min<uint32_t>(163840 / cap - 1024, 64512) : 0;
+}
+
+// The schedule `pick` applied to the problem q: its residency cap and product.
+Problem scheduled(const Problem& q, const TuneCandidate& pick) {
+  Problem s = q;
+  s.camera_lds = camera_lds(pick.cap);
+  if (!pick.fused) s.tprod = nullptr;
+  return s;
 }
 
 int ensure_tune_events(tcv_ba_context* ctx) {
@@ -821,14 +853,18 @@ int ensure_tune_events(tcv_ba_context* ctx) {
 }
 
 // After the tuned chunk has been read back: the candidates' summed times, the pick.
-int tune_pick(tcv_ba_context* ctx) {
-  double total[2] = {0, 0};
+int tune_pick(tcv_ba_context* ctx, bool fused_available) {
+  double total[3] = {0, 0, 0};
   for (uint32_t k = 0; k < kTuneRuns; k++) {
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, ctx->tune_events[2 * k], ctx->tune_events[2 * k + 1]));
-    total[k % 2] += ms;
+    total[k % 3] += ms;
   }
-  ctx->camera_cap = total[1] < 0.985 * total[0] ? kTuneCaps[1] : kTuneCaps[0];
+  uint32_t best = 0;
+  for (uint32_t j = 1; j < 3; j++)
+    if ((fused_available || !kTune[j].fused) && total[j] < total[best]) best = j;
+  ctx->tuned_fused = kTune[best].fused;
+  ctx->camera_cap = kTune[best].cap;
   ctx->tuned_shape = ctx->shape;
   return TCV_BA_OK;
 }
@@ -844,34 +880,34 @@ int run_pcg(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, cons
   const bool timing = (c->flags & TCV_BA_FLAG_TIMING) != 0 && rep;
   uint32_t done = 0, chunk = first;
   double matvecs_before = max_word(ctx, q, S_MATVECS);
-  // the camera sweep's schedule: tuned in the first chunk when this shape has not been (the
-  // stored-block sweep only; the tuning iterations are not multiples of kTimingStride)
-  Problem qt = q;
+  // the product's schedule: tuned in the first chunk when this shape has not been (the stored-block
+  // sweeps only; the tuning iterations are not multiples of kTimingStride); until then the plain
+  // sweeps with no cap
   const bool tuned = ctx->tuned_shape == ctx->shape;
   bool tune = !tuned && ops.unstored_camera_sweep != nullptr && q.n_unstored < q.B &&
               std::min(first, budget) >= kTuneFirst + kTuneRuns;
   static_assert(kTuneFirst > 0 && kTuneFirst + kTuneRuns <= kTimingStride, "tuning iterations");
   if (tune) TRY(ensure_tune_events(ctx));
-  qt.camera_lds = tuned ? camera_lds(ctx->camera_cap) : 0;
+  Problem qt = scheduled(q, tuned ? TuneCandidate{ctx->tuned_fused, ctx->camera_cap} : kTune[0]);
   while (done < budget) {
     const uint32_t n = chunk < budget - done ? chunk : budget - done;
     for (uint32_t k = 0; k < n; k++) {
       const bool timed = timing && k % kTimingStride == 0;
       const bool trial = tune && k >= kTuneFirst && k < kTuneFirst + kTuneRuns;
-      if (timed) HIP_TRY(hipEventRecord(ctx->events[3 * k], st));
-      point_sweeps(qt, ops, st);
-      if (timed) HIP_TRY(hipEventRecord(ctx->events[3 * k + 1], st));
       if (trial) {
         const uint32_t r = k - kTuneFirst;
-        Problem qc = qt;
-        qc.camera_lds = camera_lds(kTuneCaps[r % 2]);
+        const Problem qc = scheduled(q, kTune[r % 3]);
         HIP_TRY(hipEventRecord(ctx->tune_events[2 * r], st));
+        point_sweeps(qc, ops, st);
         camera_sweeps(qc, ops, st);
         HIP_TRY(hipEventRecord(ctx->tune_events[2 * r + 1], st));
       } else {
+        if (timed) HIP_TRY(hipEventRecord(ctx->events[3 * k], st));
+        point_sweeps(qt, ops, st);
+        if (timed) HIP_TRY(hipEventRecord(ctx->events[3 * k + 1], st));
         camera_sweeps(qt, ops, st);
+        if (timed) HIP_TRY(hipEventRecord(ctx->events[3 * k + 2], st));
       }
-      if (timed) HIP_TRY(hipEventRecord(ctx->events[3 * k + 2], st));
       ops.advance(qt, st);
     }
     HIP_TRY(hipGetLastError());
@@ -894,14 +930,17 @@ int run_pcg(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, cons
       }
     }
     if (tune) {
-      TRY(tune_pick(ctx));
-      qt.camera_lds = camera_lds(ctx->camera_cap);
+      TRY(tune_pick(ctx, q.tprod != nullptr));
+      qt = scheduled(q, TuneCandidate{ctx->tuned_fused, ctx->camera_cap});
       tune = false;
     }
     if (!any_word(ctx, q, S_PCG_ACTIVE)) break;
     chunk = kChunk;
   }
-  if (rep) rep->camera_sweep_cap = qt.camera_lds ? ctx->camera_cap : 0;
+  if (rep) {
+    rep->camera_sweep_cap = qt.camera_lds ? ctx->camera_cap : 0;
+    rep->fused_product = qt.tprod != nullptr;
+  }
   return TCV_BA_OK;
 }
 

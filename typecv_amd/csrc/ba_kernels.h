@@ -162,6 +162,9 @@ struct Problem {
                    // cameras c > d observe a common point
   uint32_t n_small;                // direct bundles factored by one workgroup each
   const uint32_t* small_bundles;
+  // the fused FP64 product (k_point_sweep_fused / k_camera_tsum; null: the two plain sweeps):
+  // 8 per observation, t_i = W_i y_p(i) (6 or 7 words used, the rest zero)
+  double* tprod;
   // launch schedule, read by the host only: dynamic LDS bytes reserved per camera-sweep workgroup,
   // which caps the workgroups a CU holds at once (0: no reservation)
   uint32_t camera_lds;

@@ -1087,6 +1087,126 @@ __device__ __forceinline__ void sweep_tile(const Problem& q, uint32_t t, uint32_
   }
 }
 
+// The FP64 Schur product's point sweep with the camera products fused in: a wave that has summed
+// y_p for the points of its tile still holds each observation's cross block, so it forms
+// t_i = W_i y_p (6 words, 7 with the focal row) at once and writes it as the observation's 64-byte
+// row of tprod (point-major, so a wave's rows are one contiguous 4 KiB store); k_camera_tsum then
+// gathers the rows of each camera task instead of streaming Wc and gathering y. The cross blocks
+// are read once per product instead of twice (W 168 B per observation; t 64 B written and read
+// back, from the device's last-level cache where it still holds them). Each t_i and each sum is
+// the expression and order of k_point_sweep + k_camera_sweep, and the products are the same bits.
+// A tile of more than 64 observations (one point with more) reads its cross blocks a second time
+// for its products.
+template <bool SHARED>
+__device__ __forceinline__ void sweep_tile_fused(const Problem& q, uint32_t t, uint32_t lane,
+                                                 double (*st)[3], double (*ys)[3],
+                                                 uint32_t* own) {
+  constexpr int K = SHARED ? 21 : 18, NV = SHARED ? 7 : 6;
+  const uint32_t fp = q.tiles[t], ep = q.tiles[t + 1], b = q.tile_bundle[t];
+  if (bstate(q, b)[S_PCG_ACTIVE] == 0) return;
+  if (q.n_unstored != 0 && !stores_w(q.bundles[b])) return;
+  const double* v = vp(q);
+  const uint32_t* ps = point_start(q);
+  const uint32_t s = ps[fp], e = ps[ep];
+  const uint32_t my = fp + lane;
+  const bool has = my < ep;
+  const uint32_t mys = has ? ps[my] : 0, mye = has ? ps[my + 1] : 0;
+  const double vf = SHARED ? v[6 * q.C + b] : 0.0;
+  double acc[3] = {0, 0, 0};
+  if (e - s <= TILE) {
+    const uint32_t i = s + lane;
+    const bool live = i < e;
+    double w[K];
+    if (live) {
+      const uint32_t c = obs_camera(q)[i];
+      double vc[6];
+      for (int k = 0; k < 6; k++) vc[k] = v[c * 6 + k];
+#pragma unroll
+      for (int k = 0; k < K; k++) w[k] = wload(wptr<double, SHARED>(q.Wp, k, i, q.M));
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        double a = 0;
+#pragma unroll
+        for (int k = 0; k < 6; k++) a += w[k * 3 + j] * vc[k];
+        if (SHARED) a += w[18 + j] * vf;
+        st[lane][j] = a;
+      }
+    }
+    wave_sync();
+    if (has) {
+      segment_sum(st, s, mys, mye, acc);
+      const double* pb = q.pblk + (size_t)my * 12;
+      for (int j = 0; j < 3; j++)
+        ys[lane][j] = pb[j * 3] * acc[0] + pb[j * 3 + 1] * acc[1] + pb[j * 3 + 2] * acc[2];
+      for (uint32_t j = mys; j < mye; j++) own[j - s] = lane;
+    }
+    wave_sync();
+    if (live) {
+      const uint32_t o = own[lane];
+      const double y0 = ys[o][0], y1 = ys[o][1], y2 = ys[o][2];
+      double* out = q.tprod + (size_t)i * 8;
+#pragma unroll
+      for (int j = 0; j < NV; j++) out[j] = w[j * 3] * y0 + w[j * 3 + 1] * y1 + w[j * 3 + 2] * y2;
+#pragma unroll
+      for (int j = NV; j < 8; j++) out[j] = 0.0;  // whole 64-byte rows: full cache lines
+    }
+    return;
+  }
+  // one point (lane 0's): its sums chunk by chunk as k_point_sweep's, then its products
+  for (uint32_t base = s; base < e; base += TILE) {
+    const uint32_t i = base + lane;
+    if (i < e) {
+      const uint32_t c = obs_camera(q)[i];
+      double vc[6];
+      for (int k = 0; k < 6; k++) vc[k] = v[c * 6 + k];
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        double a = 0;
+#pragma unroll
+        for (int k = 0; k < 6; k++) a += wload(wptr<double, SHARED>(q.Wp, k * 3 + j, i, q.M)) * vc[k];
+        if (SHARED) a += wload(wptr<double, SHARED>(q.Wp, 18 + j, i, q.M)) * vf;
+        st[lane][j] = a;
+      }
+    }
+    wave_sync();
+    if (has) {
+      const uint32_t lo = mys > base ? mys : base, hi = mye < base + TILE ? mye : base + TILE;
+      segment_sum(st, base, lo, hi, acc);
+    }
+    wave_sync();
+  }
+  if (has) {
+    const double* pb = q.pblk + (size_t)my * 12;
+    for (int j = 0; j < 3; j++)
+      ys[0][j] = pb[j * 3] * acc[0] + pb[j * 3 + 1] * acc[1] + pb[j * 3 + 2] * acc[2];
+  }
+  wave_sync();
+  const double y0 = ys[0][0], y1 = ys[0][1], y2 = ys[0][2];
+  for (uint32_t base = s; base < e; base += TILE) {
+    const uint32_t i = base + lane;
+    if (i >= e) continue;
+    double* out = q.tprod + (size_t)i * 8;
+#pragma unroll
+    for (int j = 0; j < NV; j++)
+      out[j] = wload(wptr<double, SHARED>(q.Wp, j * 3, i, q.M)) * y0 +
+               wload(wptr<double, SHARED>(q.Wp, j * 3 + 1, i, q.M)) * y1 +
+               wload(wptr<double, SHARED>(q.Wp, j * 3 + 2, i, q.M)) * y2;
+#pragma unroll
+    for (int j = NV; j < 8; j++) out[j] = 0.0;
+  }
+}
+
+template <bool SHARED>
+__global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_sweep_fused(Problem q) {
+  __shared__ double st_all[TILE_WAVES][TILE][3];
+  __shared__ double ys_all[TILE_WAVES][TILE][3];
+  __shared__ uint32_t own_all[TILE_WAVES][TILE];
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / TILE), lane = threadIdx.x % TILE;
+  const uint32_t t = point_tile(wave);
+  if (t >= q.n_tiles) return;
+  sweep_tile_fused<SHARED>(q, t, lane, st_all[wave], ys_all[wave], own_all[wave]);
+}
+
 template <class WT, bool SHARED, int MODE>
 __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_sweep(Problem q) {
   __shared__ double st_all[TILE_WAVES][TILE][3];
@@ -1151,6 +1271,52 @@ void k_camera_sweep(Problem q) {
         acc[j] += wload(wptr<WT, SHARED>(q.Wc, j * 3, n, q.M)) * yv[k][0] +
                   wload(wptr<WT, SHARED>(q.Wc, j * 3 + 1, n, q.M)) * yv[k][1] +
                   wload(wptr<WT, SHARED>(q.Wc, j * 3 + 2, n, q.M)) * yv[k][2];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NV; k++)
+    for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_xor(acc[k], o, 64);
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+    for (int k = 0; k < NV; k++) red[wave][k] = acc[k];
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double v = 0;
+    for (int w = 0; w < TASK_THREADS / 64; w++) v += red[w][threadIdx.x];
+    q.task_part[(size_t)t * 8 + threadIdx.x] = v;
+  }
+}
+
+// The camera side of the fused FP64 product: one workgroup per camera task sums the rows of tprod
+// of its slots' observations (t_i = W_i y_p(i), written by k_point_sweep_fused) in k_camera_sweep's
+// slot order and reduction, so part[t] is the same as k_camera_sweep's.
+template <bool SHARED>
+__global__ __launch_bounds__(TASK_THREADS) void k_camera_tsum(Problem q) {
+  constexpr int NV = SHARED ? 7 : 6;
+  __shared__ double red[TASK_THREADS / 64][NV];
+  const uint32_t t = xcd_contiguous(blockIdx.x, gridDim.x);
+  const uint32_t s = q.tasks[4 * t + 1], e = q.tasks[4 * t + 2], b = q.tasks[4 * t + 3];
+  if (bstate(q, b)[S_PCG_ACTIVE] == 0) return;
+  if (q.n_unstored != 0 && !stores_w(q.bundles[b])) return;
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) acc[k] = 0;
+  constexpr int PER = TASK_OBS / TASK_THREADS;
+  if (e > s) {
+    double tv[PER][NV];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      const uint32_t n = min(s + threadIdx.x + TASK_THREADS * k, e - 1);
+      const double* row = q.tprod + (size_t)camera_obs(q)[n] * 8;
+#pragma unroll
+      for (int j = 0; j < NV; j++) tv[k][j] = row[j];
+    }
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      if (s + threadIdx.x + TASK_THREADS * k < e) {
+#pragma unroll
+        for (int j = 0; j < NV; j++) acc[j] += tv[k][j];
+      }
     }
   }
 #pragma unroll
