@@ -32,7 +32,8 @@ HBM_PEAK_GBS = 8000.0
 # factors that make a 1 GiB device copy read and write 1 GiB. Keyed by
 # (cameras, points, seed, precision, operator); other workloads report null.
 MEASURED_TRAFFIC = {
-    (500, 200_000, 2, "fp64", "stored"): 2 * (175330.7 + 175545.0) * 1024 + (4688.1 + 125.1) * 1024,
+    # the fused product (k_point_sweep_fused + k_camera_tsum), the schedule tuner's pick here
+    (500, 200_000, 2, "fp64", "stored"): 2 * (180051.5 + 71652.3) * 1024 + (118175.7 + 142.2) * 1024,
 }
 
 
@@ -177,7 +178,9 @@ def main() -> int:
             "traffic": MEASURED_TRAFFIC.get((C, P, args.seed, args.precision,
                                              "implicit" if args.implicit else "stored")),
             "algorithmic_bytes": bytes_per,
-            "kernel": "Schur matvec: k_point_sweep + k_camera_sweep",
+            "kernel": ("Schur matvec: k_point_sweep_fused + k_camera_tsum"
+                       if reports[-1]["fused_product"] else
+                       "Schur matvec: k_point_sweep + k_camera_sweep"),
         },
     }
     if not (args.schur_cache or args.implicit or args.no_direct_line or args.precision != "fp64"):
