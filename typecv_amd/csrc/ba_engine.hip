@@ -349,15 +349,15 @@ struct Launch {
     hipLaunchKernelGGL((k_init_cg<SHARED>), dim3(q.B), dim3(CG_THREADS), 0, s, q);
   }
   static void point_sweep(const Problem& q, hipStream_t s) {
-    if (sizeof(WT) == 8 && q.tprod)
-      hipLaunchKernelGGL((k_point_sweep_fused<SHARED>), dim3(tile_blocks(q)),
+    if (q.tprod)
+      hipLaunchKernelGGL((k_point_sweep_fused<WT, SHARED>), dim3(tile_blocks(q)),
                          dim3(TILE * TILE_WAVES), 0, s, q);
     else
       hipLaunchKernelGGL((k_point_sweep<WT, SHARED, SWEEP_SCHUR>), dim3(tile_blocks(q)),
                          dim3(TILE * TILE_WAVES), 0, s, q);
   }
   static void camera_sweep(const Problem& q, hipStream_t s) {
-    if (sizeof(WT) == 8 && q.tprod)
+    if (q.tprod)
       hipLaunchKernelGGL((k_camera_tsum<SHARED>), dim3(q.n_tasks), dim3(TASK_THREADS),
                          q.camera_lds, s, q);
     else
@@ -647,6 +647,9 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
 #endif
   // the fused FP64 product's rows (FP32 blocks keep the two plain sweeps: t stays FP64, so its
   // round trip would outweigh the halved cross-block reads)
+  // the fused product's rows (the tuner times it against the plain sweeps, which compute the same
+  // bits); FP64 blocks only: over FP32 ones it is slower (70.8 + 27.4 against 42.6 + 40.0 us at
+  // 500 cameras), since the rows stay FP64 while the blocks it saves reading are halved
   const bool fused = TCV_FUSED && ws == 8 && n_unstored < B;
   if (fused) TRY(ensure(ctx->tprod, (size_t)M * 64));
   TRY(ensure(ctx->pblk, (size_t)P * 12 * 8));
@@ -816,14 +819,13 @@ int ensure_events(tcv_ba_context* ctx, size_t n) {
 // The Schur product's schedule tuner. Its candidates differ in speed only, never in a bit: the
 // two plain sweeps with the camera sweep's residency left to its registers or capped at three
 // workgroups per CU (an LDS reservation per workgroup: 160 KiB per CU, at most 63 KiB, which
-// needs no opt-in), and, with FP64 cross blocks, the fused product (k_point_sweep_fused +
-// k_camera_tsum, whose 64-byte rows of t per observation pay off while the device's last-level
-// cache holds them). Neither the task partition nor any sum changes. Measured at 500 cameras
+// needs no opt-in), and the fused product (k_point_sweep_fused + k_camera_tsum, whose 64-byte
+// rows of t per observation pay off while the device's last-level cache holds them). Neither the task partition nor any sum changes. Measured at 500 cameras
 // (FP64): plain 61.6 + 66.1 us, capped camera sweep 60.2 us, fused 95.0 + 27.6 us (-3.5 %); at
 // 2,000 cameras the fused product is 10 % slower (its 621 MB of rows miss that cache). The first
 // PCG chunk of a problem shape not yet tuned runs its iterations 1..6 with the candidates in turn
 // (each iteration's one product, so nothing is added), times each product with HIP events and
-// keeps the fastest candidate.
+// keeps the fastest candidate, if it beats the plain sweeps by 2 % (timing noise aside).
 struct TuneCandidate {
   bool fused;
   uint32_t cap;
@@ -863,6 +865,7 @@ int tune_pick(tcv_ba_context* ctx, bool fused_available) {
   uint32_t best = 0;
   for (uint32_t j = 1; j < 3; j++)
     if ((fused_available || !kTune[j].fused) && total[j] < total[best]) best = j;
+  if (total[best] > 0.98 * total[0]) best = 0;  // a schedule other than the plain one must win by 2 %
   ctx->tuned_fused = kTune[best].fused;
   ctx->camera_cap = kTune[best].cap;
   ctx->tuned_shape = ctx->shape;

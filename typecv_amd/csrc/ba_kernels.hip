@@ -1087,17 +1087,18 @@ __device__ __forceinline__ void sweep_tile(const Problem& q, uint32_t t, uint32_
   }
 }
 
-// The FP64 Schur product's point sweep with the camera products fused in: a wave that has summed
-// y_p for the points of its tile still holds each observation's cross block, so it forms
-// t_i = W_i y_p (6 words, 7 with the focal row) at once and writes it as the observation's 64-byte
-// row of tprod (point-major, so a wave's rows are one contiguous 4 KiB store); k_camera_tsum then
-// gathers the rows of each camera task instead of streaming Wc and gathering y. The cross blocks
-// are read once per product instead of twice (W 168 B per observation; t 64 B written and read
-// back, from the device's last-level cache where it still holds them). Each t_i and each sum is
-// the expression and order of k_point_sweep + k_camera_sweep, and the products are the same bits.
-// A tile of more than 64 observations (one point with more) reads its cross blocks a second time
-// for its products.
-template <bool SHARED>
+// The Schur product's point sweep with the camera products fused in: a wave that has summed y_p
+// for the points of its tile still holds each observation's cross block, so it forms
+// t_i = W_i y_p (6 words, 7 with the focal row) at once, in FP64, and writes it as the
+// observation's 64-byte row of tprod (point-major, so a wave's rows are one contiguous 4 KiB
+// store); k_camera_tsum then gathers the rows of each camera task instead of streaming Wc and
+// gathering y. The cross blocks (FP64, or the mixed form's FP32) are read once per product instead
+// of twice (t written and read back, from the device's last-level cache where it still holds
+// them). Each t_i and each sum is the expression and order of k_point_sweep + k_camera_sweep, so
+// the products are the same bits. (FP32 rows, tried for the mixed form, cost the refinement its
+// 1e-6 bar on a 40-camera scene.) A tile of more than 64 observations (one point with more) reads
+// its cross blocks a second time for its products.
+template <class WT, bool SHARED>
 __device__ __forceinline__ void sweep_tile_fused(const Problem& q, uint32_t t, uint32_t lane,
                                                  double (*st)[3], double (*ys)[3],
                                                  uint32_t* own) {
@@ -1122,7 +1123,7 @@ __device__ __forceinline__ void sweep_tile_fused(const Problem& q, uint32_t t, u
       double vc[6];
       for (int k = 0; k < 6; k++) vc[k] = v[c * 6 + k];
 #pragma unroll
-      for (int k = 0; k < K; k++) w[k] = wload(wptr<double, SHARED>(q.Wp, k, i, q.M));
+      for (int k = 0; k < K; k++) w[k] = wload(wptr<WT, SHARED>(q.Wp, k, i, q.M));
 #pragma unroll
       for (int j = 0; j < 3; j++) {
         double a = 0;
@@ -1163,8 +1164,8 @@ __device__ __forceinline__ void sweep_tile_fused(const Problem& q, uint32_t t, u
       for (int j = 0; j < 3; j++) {
         double a = 0;
 #pragma unroll
-        for (int k = 0; k < 6; k++) a += wload(wptr<double, SHARED>(q.Wp, k * 3 + j, i, q.M)) * vc[k];
-        if (SHARED) a += wload(wptr<double, SHARED>(q.Wp, 18 + j, i, q.M)) * vf;
+        for (int k = 0; k < 6; k++) a += wload(wptr<WT, SHARED>(q.Wp, k * 3 + j, i, q.M)) * vc[k];
+        if (SHARED) a += wload(wptr<WT, SHARED>(q.Wp, 18 + j, i, q.M)) * vf;
         st[lane][j] = a;
       }
     }
@@ -1188,15 +1189,15 @@ __device__ __forceinline__ void sweep_tile_fused(const Problem& q, uint32_t t, u
     double* out = q.tprod + (size_t)i * 8;
 #pragma unroll
     for (int j = 0; j < NV; j++)
-      out[j] = wload(wptr<double, SHARED>(q.Wp, j * 3, i, q.M)) * y0 +
-               wload(wptr<double, SHARED>(q.Wp, j * 3 + 1, i, q.M)) * y1 +
-               wload(wptr<double, SHARED>(q.Wp, j * 3 + 2, i, q.M)) * y2;
+      out[j] = wload(wptr<WT, SHARED>(q.Wp, j * 3, i, q.M)) * y0 +
+               wload(wptr<WT, SHARED>(q.Wp, j * 3 + 1, i, q.M)) * y1 +
+               wload(wptr<WT, SHARED>(q.Wp, j * 3 + 2, i, q.M)) * y2;
 #pragma unroll
     for (int j = NV; j < 8; j++) out[j] = 0.0;
   }
 }
 
-template <bool SHARED>
+template <class WT, bool SHARED>
 __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_sweep_fused(Problem q) {
   __shared__ double st_all[TILE_WAVES][TILE][3];
   __shared__ double ys_all[TILE_WAVES][TILE][3];
@@ -1204,7 +1205,7 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_sweep_fused(Problem 
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / TILE), lane = threadIdx.x % TILE;
   const uint32_t t = point_tile(wave);
   if (t >= q.n_tiles) return;
-  sweep_tile_fused<SHARED>(q, t, lane, st_all[wave], ys_all[wave], own_all[wave]);
+  sweep_tile_fused<WT, SHARED>(q, t, lane, st_all[wave], ys_all[wave], own_all[wave]);
 }
 
 template <class WT, bool SHARED, int MODE>
@@ -1287,9 +1288,9 @@ void k_camera_sweep(Problem q) {
   }
 }
 
-// The camera side of the fused FP64 product: one workgroup per camera task sums the rows of tprod
-// of its slots' observations (t_i = W_i y_p(i), written by k_point_sweep_fused) in k_camera_sweep's
-// slot order and reduction, so part[t] is the same as k_camera_sweep's.
+// The camera side of the fused product: one workgroup per camera task sums the rows of tprod of
+// its slots' observations (t_i = W_i y_p(i), written by k_point_sweep_fused) in k_camera_sweep's
+// slot order and reduction, so part[t] is k_camera_sweep's.
 template <bool SHARED>
 __global__ __launch_bounds__(TASK_THREADS) void k_camera_tsum(Problem q) {
   constexpr int NV = SHARED ? 7 : 6;
