@@ -16,6 +16,8 @@
 //   k_init_cg (per bundle)           initializeBundleCg b:695-741 / j:646-700 with dampBundleFocal
 //   k_point_sweep<SCHUR>             bundleSchurPoint b:742-745 (body b:185-203, j:198-218)
 //   k_camera_sweep                   bundleSchurCamera* b:746-882 / j:705-870 (camera partials)
+//   k_point_sweep_fused + k_camera_tsum   the same product with the cross blocks read once (a row
+//                                    of W y per observation); the same sums, the same bits
 //   k_cg_product / k_cg_step / k_cg_direction   finish S·p, then advanceBundleCg b:883-929
 //   k_update_cameras / k_point_sweep<UPDATE>   updateBundleCandidate b:930-961 / j:925-960
 //   k_gauge (per bundle)             bundleCandidateGauge b:962-993 / j:961-1000
