@@ -820,12 +820,13 @@ int ensure_events(tcv_ba_context* ctx, size_t n) {
 // two plain sweeps with the camera sweep's residency left to its registers or capped at three
 // workgroups per CU (an LDS reservation per workgroup: 160 KiB per CU, at most 63 KiB, which
 // needs no opt-in), and the fused product (k_point_sweep_fused + k_camera_tsum, whose 64-byte
-// rows of t per observation pay off while the device's last-level cache holds them). Neither the task partition nor any sum changes. Measured at 500 cameras
-// (FP64): plain 61.6 + 66.1 us, capped camera sweep 60.2 us, fused 95.0 + 27.6 us (-3.5 %); at
-// 2,000 cameras the fused product is 10 % slower (its 621 MB of rows miss that cache). The first
-// PCG chunk of a problem shape not yet tuned runs its iterations 1..6 with the candidates in turn
-// (each iteration's one product, so nothing is added), times each product with HIP events and
-// keeps the fastest candidate, if it beats the plain sweeps by 2 % (timing noise aside).
+// rows of t per observation pay off while the device's last-level cache holds them). Neither the
+// task partition nor any sum changes. Measured at 500 cameras (FP64): plain 66.1 + 61.6 us (point
+// + camera sweep), capped camera sweep 60.2 us, fused 95.0 + 27.6 us (-3.5 %); at 2,000 cameras
+// the fused product is 13 % slower (its 621 MB of rows miss that cache). The first PCG chunk of
+// a problem shape not yet tuned runs its iterations 1..6 with the candidates in turn (each
+// iteration's one product, so nothing is added), times each product with HIP events and keeps
+// the fastest candidate, if it beats the plain sweeps by 2 % (timing noise aside).
 struct TuneCandidate {
   bool fused;
   uint32_t cap;
