@@ -1,14 +1,18 @@
 """Benchmark of the MI355X bundle-adjustment engine on the shared-focal trajectory scene.
 
-Workload (one GPU): 500 cameras on a smooth closed trajectory, 200,000 points each seen by 5-20
-cameras (~1.9M observations), N(0, 0.7 px) noise, 2 % uniform outliers, common focal error
+Workload (one GPU): 2,000 cameras on a smooth closed trajectory, 1,000,000 points each seen by
+5-20 cameras (~9.7M observations), N(0, 0.7 px) noise, 2 % uniform outliers, common focal error
 U(0.95, 1.05); joint pose/point/shared-focal LM with Huber delta = 1 px, 30 LM iterations, PCG up
-to 256 iterations (the route's default), FP64. A step is one whole solve from the device-resident
-packed scene (tcv_ba_solve_device); the scene is generated and uploaded before timing.
+to 256 iterations (the route's default). A step is one whole solve from the device-resident packed
+scene (tcv_ba_solve_device); the scene is generated and uploaded before timing.
 
-Prints one JSON line: LM iterations/s over the timed steps (`value`), observations/s, the Schur
-matvec roofline (algorithmic bytes of the two sweeps / their HIP-event time), and the CPU oracle
-timed on a bounded sample of the same solve on the host cores.
+Prints one JSON line: the plain FP64 form's LM iterations/s over the timed steps (`value`),
+observations/s and the Schur matvec roofline (algorithmic bytes of the two sweeps / their HIP-event
+time); the mixed form on the same scene in the same process (`mixed`: FP32 cross blocks and
+products with FP64 accumulation, refinement of the FP64 residual to 1e-10; its speed-up and its
+final-cost, pose and point differences from the FP64 run); the directSolve route; the 500-camera
+scene and the 256 local bundles as sub-records; and the CPU oracle timed on bounded samples of the
+same solve on the host cores.
 """
 from __future__ import annotations
 
@@ -52,31 +56,111 @@ def matvec_bytes(C: int, P: int, M: int, shared: bool, s_w: int = 8, s_v: int = 
     return total
 
 
+def solve_forms(args, B, _native, lib, device, packed, scene, index, out, precision, steps,
+                warmup, refine=0.0, direct=False, label=""):
+    """`warmup` untimed and `steps` timed device-resident solves of one form of the workload:
+    (reports, elapsed seconds of the timed solves, final output array)."""
+    import torch
+
+    C, P, M = packed.C, packed.P, packed.M
+    control = B.make_control(C, P, M, shared_focal=True, huber=1.0, iterations=args.iterations,
+                             max_cg=min(256, max(32, 12 * C + 1)), precision=precision,
+                             refinement_tolerance=refine, timing=True, direct=direct)
+
+    def solve() -> dict:
+        report = _native.Report()
+        status = lib.tcv_ba_solve_device(device.handle, ctypes.byref(control), scene.data_ptr(),
+                                         index.data_ptr(), out.data_ptr(), ctypes.byref(report),
+                                         None)
+        _native.check(status, "tcv_ba_solve_device")
+        return report.as_dict()
+
+    for k in range(warmup):
+        r = solve()
+        log(f"{label} warmup {k}: {r['solve_ms']:.1f} ms, {r['lm_iterations']} LM, "
+            f"{r['cg_iterations']} CG, {r['refinement_steps']:.0f} refinements")
+    torch.cuda.synchronize()
+    reports = []
+    start = time.perf_counter()
+    for k in range(steps):
+        reports.append(solve())
+        log(f"{label} step {k}: {reports[-1]['solve_ms']:.1f} ms")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - start
+    return control, reports, elapsed, out.cpu().numpy()
+
+
+def form_record(packed, seed, precision, operator, reports, elapsed, values):
+    """LM iterations/s, observations/s and the Schur product's roofline of timed solves."""
+    C, P, M = packed.C, packed.P, packed.M
+    steps = len(reports)
+    lm = sum(r["lm_iterations"] for r in reports)
+    cg = sum(r["cg_iterations"] for r in reports)
+    timed = sum(r["timed_matvecs"] for r in reports)  # the engine times every 8th product
+    s_w = 4 if precision == "mixed" else 8
+    bytes_per = matvec_bytes(C, P, M, True, s_w=s_w)
+    mv_ms = sum(r["schur_ms"] for r in reports) / max(1, timed)
+    achieved = bytes_per / (mv_ms * 1e-3) / 1e9 if mv_ms > 0 else 0.0
+    base = C * 12 + P * 3
+    fused = reports[-1]["fused_product"]
+    return {
+        "value": lm / elapsed, "unit": "LM iterations/s", "steps": steps,
+        "ms_per_step": elapsed * 1e3 / steps,
+        "observations_per_s": M * lm / elapsed,
+        "pcg_iterations_per_s": cg / elapsed,
+        "lm_iterations_per_step": lm / steps, "cg_iterations_per_step": cg / steps,
+        "matvecs_per_step": sum(r["matvecs"] for r in reports) / steps,
+        "refinements_per_step": sum(r["refinement_steps"] for r in reports) / steps,
+        "direct_solves_per_step": sum(r["direct_solves"] for r in reports) / steps,
+        "final_cost": float(values[base + 6]), "initial_cost": float(values[base + 8]),
+        "final_rmse_px": float(values[base + 25]),
+        "schur_matvec": {
+            "ms": mv_ms, "point_sweep_ms": sum(r["schur_point_ms"] for r in reports) / max(1, timed),
+            "camera_sweep_ms": sum(r["schur_camera_ms"] for r in reports) / max(1, timed),
+            "bytes": bytes_per, "timed_products": timed,
+            "share_of_solve": mv_ms * sum(r["matvecs"] for r in reports)
+            / max(1e-9, sum(r["solve_ms"] for r in reports)),
+            # camera-sweep workgroups per CU picked by the context's schedule tuner (0: no cap)
+            "camera_sweep_cap": reports[-1]["camera_sweep_cap"],
+            # 1: the tuner picked the fused FP64 product (k_point_sweep_fused + k_camera_tsum)
+            "fused_product": fused,
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": MEASURED_TRAFFIC.get((C, P, seed, precision, operator)),
+            "algorithmic_bytes": bytes_per,
+            "kernel": ("Schur matvec: k_point_sweep_fused + k_camera_tsum" if fused else
+                       "Schur matvec: k_point_sweep + k_camera_sweep"),
+        },
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cameras", type=int, default=500)
-    ap.add_argument("--points", type=int, default=200_000)
+    ap.add_argument("--cameras", type=int, default=2000)
+    ap.add_argument("--points", type=int, default=1_000_000)
     ap.add_argument("--iterations", type=int, default=30)
-    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
-    ap.add_argument("--implicit", action="store_true",
-                    help="relinearise cross blocks in every Schur sweep instead of streaming them")
-    ap.add_argument("--refine", type=float, default=0.0,
-                    help="mixed form: FP64 residual target of the refinement (0: the PCG tolerance)")
-    ap.add_argument("--schur-cache", action="store_true",
-                    help="the schurCache option: a dense Cholesky solve per LM iteration")
-    ap.add_argument("--no-direct-line", action="store_true",
-                    help="skip the second measurement of the same workload by the direct route")
+    ap.add_argument("--refine", type=float, default=1e-10,
+                    help="mixed form: FP64 residual target of the refinement")
+    ap.add_argument("--sub-steps", type=int, default=0,
+                    help="timed solves of the mixed, direct and sub-workload records "
+                         "(0: min(steps, 5))")
     ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--cpu-lm", type=int, default=2, help="LM iterations of the CPU sample (0: skip)")
+    ap.add_argument("--no-subworkloads", action="store_true",
+                    help="skip the 500-camera and 256-bundle records")
+    ap.add_argument("--cpu", action=argparse.BooleanOptionalAction, default=True,
+                    help="time the CPU oracle baselines")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(the matvec would shard by points with one all-reduce of 6C+1 doubles per PCG "
               "iteration); run it with --gpus 1", file=sys.stderr)
         return 2
+    sub_steps = args.sub_steps or min(args.steps, 5)
 
     import numpy as np
     import torch
@@ -89,145 +173,223 @@ def main() -> int:
     packed, info = S.trajectory_scene(cameras=args.cameras, points=args.points, seed=args.seed)
     C, P, M = packed.C, packed.P, packed.M
     log(f"scene C={C} P={P} M={M} generated in {time.time() - t0:.1f}s")
-    control = B.make_control(C, P, M, shared_focal=True, huber=1.0, iterations=args.iterations,
-                             max_cg=min(256, max(32, 12 * C + 1)), precision=args.precision,
-                             refinement_tolerance=args.refine, timing=True,
-                             implicit=args.implicit, schur_cache=args.schur_cache)
     lib = _native.library()
     device = B.HipDevice(0)
     scene = torch.from_numpy(packed.scene).cuda()
     index = torch.from_numpy(packed.index.view(np.int32)).cuda()
-    out = torch.empty(lib.tcv_ba_output_size(ctypes.byref(control)), dtype=torch.float64,
+    control0 = B.make_control(C, P, M, shared_focal=True)
+    out = torch.empty(lib.tcv_ba_output_size(ctypes.byref(control0)), dtype=torch.float64,
                       device="cuda")
     torch.cuda.synchronize()
-
-    def solve() -> dict:
-        report = _native.Report()
-        status = lib.tcv_ba_solve_device(device.handle, ctypes.byref(control), scene.data_ptr(),
-                                         index.data_ptr(), out.data_ptr(), ctypes.byref(report),
-                                         None)
-        _native.check(status, "tcv_ba_solve_device")
-        return report.as_dict()
-
-    for k in range(args.warmup):
-        r = solve()
-        log(f"warmup {k}: {r['solve_ms']:.1f} ms, {r['lm_iterations']} LM, {r['cg_iterations']} CG")
-    torch.cuda.synchronize()
-    reports = []
-    start = time.perf_counter()
-    for k in range(args.steps):
-        reports.append(solve())
-        log(f"step {k}: {reports[-1]['solve_ms']:.1f} ms")
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - start
-    values = out.cpu().numpy()
     base = C * 12 + P * 3
-    lm = sum(r["lm_iterations"] for r in reports)
-    matvecs = sum(r["matvecs"] for r in reports)
-    timed = sum(r["timed_matvecs"] for r in reports)  # the engine times every 8th product
-    schur_ms = sum(r["schur_ms"] for r in reports)
-    point_ms = sum(r["schur_point_ms"] for r in reports)
-    camera_ms = sum(r["schur_camera_ms"] for r in reports)
-    s_w = 4 if args.precision == "mixed" else 8
-    bytes_per = matvec_bytes(C, P, M, True, s_w=s_w)
-    mv_ms = schur_ms / max(1, timed)
-    achieved = bytes_per / (mv_ms * 1e-3) / 1e9 if mv_ms > 0 else 0.0
+
+    # the value: the plain FP64 form, exactly `steps` timed solves
+    control, reports, elapsed, f64 = solve_forms(args, B, _native, lib, device, packed, scene,
+                                                 index, out, "fp64", args.steps, args.warmup,
+                                                 label="fp64")
+    plain = form_record(packed, args.seed, "fp64", "stored", reports, elapsed, f64)
     result = {
         "metric": METRIC,
-        "value": lm / elapsed,
+        "value": plain["value"],
         "unit": "LM iterations/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed * 1e3 / args.steps,
+        "ms_per_step": plain["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32-products/f64-accumulation",
+        "dtype": "f64",
         "data": "synthetic (seeded trajectory scene, generated on the host, resident in HBM)",
         "config": {
             "workload": "shared-focal LM bundle adjustment, trajectory scene",
             "cameras": C, "points": P, "observations": M, "lm_iterations_max": args.iterations,
-            "max_cg": control.max_cg, "huber_px": 1.0, "precision": args.precision,
+            "max_cg": control.max_cg, "huber_px": 1.0, "precision": "fp64",
             "outliers": info["outliers"], "focal_factor": info["focal_factor"],
-            "operator": "implicit" if args.implicit else "stored",
-            "linear_solver": "direct (schurCache)" if args.schur_cache else "matrix-free PCG",
-        },
-        "direct_solves_per_step": sum(r["direct_solves"] for r in reports) / args.steps,
-        "refinements_per_step": sum(r["refinement_steps"] for r in reports) / args.steps,
-        "observations_per_s": M * lm / elapsed,
-        "lm_iterations_per_step": lm / args.steps,
-        "cg_iterations_per_step": sum(r["cg_iterations"] for r in reports) / args.steps,
-        "matvecs_per_step": matvecs / args.steps,
-        "final_cost": float(values[base + 6]),
-        "initial_cost": float(values[base + 8]),
-        "final_rmse_px": float(values[base + 25]),
-        "schur_matvec": {
-            "ms": mv_ms, "point_sweep_ms": point_ms / max(1, timed),
-            "camera_sweep_ms": camera_ms / max(1, timed), "bytes": bytes_per,
-            "timed_products": timed,
-            "share_of_solve": mv_ms * matvecs / max(1e-9, sum(r["solve_ms"] for r in reports)),
-            # camera-sweep workgroups per CU picked by the context's schedule tuner (0: no cap)
-            "camera_sweep_cap": reports[-1]["camera_sweep_cap"],
-            # 1: the tuner picked the fused FP64 product (k_point_sweep_fused + k_camera_tsum)
-            "fused_product": reports[-1]["fused_product"],
-        },
-        "roofline": {
-            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": MEASURED_TRAFFIC.get((C, P, args.seed, args.precision,
-                                             "implicit" if args.implicit else "stored")),
-            "algorithmic_bytes": bytes_per,
-            "kernel": ("Schur matvec: k_point_sweep_fused + k_camera_tsum"
-                       if reports[-1]["fused_product"] else
-                       "Schur matvec: k_point_sweep + k_camera_sweep"),
+            "operator": "stored", "linear_solver": "matrix-free PCG",
         },
     }
-    if not (args.schur_cache or args.implicit or args.no_direct_line or args.precision != "fp64"):
-        # the same workload with the schurCache option: a dense Cholesky solve of each LM
-        # iteration's reduced camera system instead of PCG (a separate line, not the value)
-        control = B.make_control(C, P, M, shared_focal=True, huber=1.0,
-                                 iterations=args.iterations, max_cg=control.max_cg,
-                                 schur_cache=True)
-        solve()
-        torch.cuda.synchronize()
-        direct = []
-        start = time.perf_counter()
-        for k in range(args.steps):
-            direct.append(solve())
-        torch.cuda.synchronize()
-        d_elapsed = time.perf_counter() - start
-        d_values = out.cpu().numpy()
-        d_lm = sum(r["lm_iterations"] for r in direct)
-        log(f"direct route: {d_elapsed * 1e3 / args.steps:.1f} ms per solve")
-        result["direct_route"] = {
-            "value": d_lm / d_elapsed, "unit": "LM iterations/s",
-            "ms_per_step": d_elapsed * 1e3 / args.steps,
-            "direct_solves_per_step": sum(r["direct_solves"] for r in direct) / args.steps,
-            "cg_iterations_per_step": sum(r["cg_iterations"] for r in direct) / args.steps,
-            "final_cost": float(d_values[base + 6]),
-            "final_rmse_px": float(d_values[base + 25]),
-            "linear_solver": "schurCache: dense Cholesky (64-column panels), certified by the "
-                             "matrix-free residual",
-        }
-    device.dispose()
-    if args.cpu_lm > 0:
-        from oracle import oracle as O
+    for key in ("observations_per_s", "pcg_iterations_per_s", "lm_iterations_per_step",
+                "cg_iterations_per_step", "matvecs_per_step", "final_cost", "initial_cost",
+                "final_rmse_px", "schur_matvec", "roofline"):
+        result[key] = plain[key]
 
-        threads = O.set_threads(0)
-        cpu_control = B.make_control(C, P, M, shared_focal=True, huber=1.0,
-                                     iterations=args.iterations, max_cg=control.max_cg)
-        log(f"CPU oracle sample: {args.cpu_lm} LM iterations on {threads} threads")
-        _, st = O.solve(cpu_control, packed, max_lm=args.cpu_lm)
-        result["cpu_baseline"] = {
-            "value": st["lm_iterations"] / (st["lm_ms"] * 1e-3),
-            "unit": "LM iterations/s", "cores": threads, "kind": "port",
-            "sample": f"first {int(st['lm_iterations'])} LM iterations of the same solve "
-                      f"({int(st['cg_iterations'])} PCG iterations, {st['lm_ms'] / 1e3:.1f} s), "
-                      "C oracle -O3 OpenMP",
-        }
+    # the mixed form on the same scene, in the same process: FP32 cross blocks and Schur products,
+    # FP64 accumulation, V^-1, preconditioner, dots and LM; refinement of the FP64 residual
+    _, reports, elapsed, mx = solve_forms(args, B, _native, lib, device, packed, scene, index,
+                                          out, "mixed", sub_steps, 1, refine=args.refine,
+                                          label="mixed")
+    mixed = form_record(packed, args.seed, "mixed", "stored", reports, elapsed, mx)
+    cost = f64[base + 6]
+    mixed.update({
+        "dtype": "f32-products/f64-accumulation", "refinement_tolerance": args.refine,
+        "speedup_vs_fp64": mixed["value"] / plain["value"],
+        "final_cost_rel_err_vs_fp64": abs(mx[base + 6] - cost) / cost,
+        "max_pose_abs_diff_vs_fp64": float(np.abs(mx[:12 * C] - f64[:12 * C]).max()),
+        "max_point_abs_diff_vs_fp64": float(np.abs(mx[12 * C:base] - f64[12 * C:base]).max()),
+        "log_focal_diff_vs_fp64": float(abs(mx[base + 26] - f64[base + 26])),
+    })
+    result["mixed"] = mixed
+    log(f"mixed: {mixed['value']:.2f} LM it/s ({mixed['speedup_vs_fp64']:.2f}x FP64), cost "
+        f"{mixed['final_cost_rel_err_vs_fp64']:.1e} from FP64")
+
+    # the directSolve option on the same scene: a dense Cholesky solve per LM iteration
+    _, reports, elapsed, dv = solve_forms(args, B, _native, lib, device, packed, scene, index,
+                                          out, "fp64", sub_steps, 1, direct=True, label="direct")
+    d_lm = sum(r["lm_iterations"] for r in reports)
+    result["direct_route"] = {
+        "value": d_lm / elapsed, "unit": "LM iterations/s", "steps": len(reports),
+        "ms_per_step": elapsed * 1e3 / len(reports),
+        "direct_solves_per_step": sum(r["direct_solves"] for r in reports) / len(reports),
+        "cg_iterations_per_step": sum(r["cg_iterations"] for r in reports) / len(reports),
+        "final_cost": float(dv[base + 6]), "final_rmse_px": float(dv[base + 25]),
+        "linear_solver": "directSolve: dense Cholesky (64-column panels), certified by the "
+                         "matrix-free residual",
+    }
+    del scene, index, out
+    device.dispose()
+    torch.cuda.empty_cache()
+
+    if not args.no_subworkloads:
+        result["trajectory_500"] = trajectory_500(args, sub_steps)
+        result["batched_256"] = batched_256(args, sub_steps)
+
+    if args.cpu:
+        result["cpu_baseline"] = cpu_baseline(packed, control, plain)
+        result["cpu_dense_ring_10"] = cpu_dense_ring()
     print(json.dumps(result), flush=True)
     return 0
+
+
+def trajectory_500(args, steps) -> dict:
+    """The 500-camera / 200,000-point trajectory scene in both forms (a sub-record)."""
+    import numpy as np
+    import torch
+
+    from typecv_amd import _native
+    from typecv_amd import bundle as B
+    from typecv_amd import synthetic as S
+
+    packed, _ = S.trajectory_scene(cameras=500, points=200_000, seed=args.seed)
+    log(f"sub-workload C={packed.C} P={packed.P} M={packed.M}")
+    lib = _native.library()
+    scene = torch.from_numpy(packed.scene).cuda()
+    index = torch.from_numpy(packed.index.view(np.int32)).cuda()
+    control0 = B.make_control(packed.C, packed.P, packed.M, shared_focal=True)
+    out = torch.empty(lib.tcv_ba_output_size(ctypes.byref(control0)), dtype=torch.float64,
+                      device="cuda")
+    record = {"config": {"cameras": packed.C, "points": packed.P, "observations": packed.M}}
+    with B.HipDevice(0) as device:
+        for precision in ("fp64", "mixed"):
+            _, reports, elapsed, values = solve_forms(
+                args, B, _native, lib, device, packed, scene, index, out, precision, steps, 1,
+                refine=args.refine if precision == "mixed" else 0.0, label=f"500 {precision}")
+            record[precision] = form_record(packed, args.seed, precision, "stored", reports,
+                                            elapsed, values)
+    record["mixed"]["speedup_vs_fp64"] = record["mixed"]["value"] / record["fp64"]["value"]
+    return record
+
+
+def batched_256(args, steps) -> dict:
+    """256 local bundles in one tcv_ba_solve_batch per step (a sub-record): the directSolve route
+    and the mixed PCG route, bundles/s and per-bundle LM-loop latency."""
+    import numpy as np
+
+    from typecv_amd import bundle as B
+    from typecv_amd import synthetic as S
+
+    bundles = S.local_bundles(count=256, seed=4)
+    packeds = [p for p, _ in bundles]
+    record = {"config": {"bundles": len(packeds), "observations": sum(p.M for p in packeds),
+                         "cameras": "U{5..40}", "points": "U{500..20000}"}}
+    with B.HipDevice(0) as device:
+        for name, precision, direct, n in (("direct_fp64", "fp64", True, steps),
+                                           ("pcg_mixed", "mixed", False, 1),
+                                           ("pcg_fp64", "fp64", False, 1)):
+            controls = [B.make_control(p.C, p.P, p.M, shared_focal=True, huber=1.0,
+                                       iterations=30, max_cg=min(256, max(32, 12 * p.C + 1)),
+                                       precision=precision, direct=direct) for p in packeds]
+            loops, walls, latencies, totals = run_batch(B, device, controls, packeds, n, 1)
+            lat = np.array(latencies)
+            record[name] = {
+                "bundles_per_s": len(packeds) * n / sum(loops), "steps": n,
+                "ms_per_step": 1e3 * sum(loops) / n,
+                "latency_ms": {"p50": float(np.percentile(lat, 50)),
+                               "p99": float(np.percentile(lat, 99))},
+                "end_to_end_bundles_per_s": len(packeds) * n / sum(walls),
+                "pcg_iterations_per_step": totals["cg_iterations"] / n,
+                "direct_solves_per_step": totals["direct_solves"] / n,
+            }
+    return record
+
+
+def cpu_baseline(packed, control, plain) -> dict:
+    """The C oracle (the reference's CPU-tier algorithm restated, -O3, OpenMP) on the host cores,
+    on bounded samples of the same solve: the first LM iteration on every host thread, and the
+    first LM iteration with its PCG capped at 4 iterations on one thread."""
+    from oracle import oracle as O
+    from typecv_amd import bundle as B
+
+    threads = O.set_threads(0)
+    C, P, M = packed.C, packed.P, packed.M
+    log(f"CPU oracle sample: 1 LM iteration on {threads} threads")
+    _, st = O.solve(control, packed, max_lm=1)
+    lm_s = st["lm_ms"] * 1e-3
+    one = B.make_control(C, P, M, shared_focal=True, huber=1.0, iterations=1, max_cg=4)
+    O.set_threads(1)
+    log("CPU oracle sample: 1 LM iteration, PCG <= 4, on 1 thread")
+    _, st1 = O.solve(one, packed, max_lm=1)
+    O.set_threads(0)
+    return {
+        "value": st["lm_iterations"] / lm_s, "unit": "LM iterations/s", "cores": threads,
+        "kind": "port",
+        "sample": f"first {int(st['lm_iterations'])} LM iteration of the same solve "
+                  f"({int(st['cg_iterations'])} PCG iterations, {lm_s:.1f} s), C oracle -O3 "
+                  f"OpenMP on {threads} threads",
+        "pcg_iterations_per_s": st["cg_iterations"] / lm_s,
+        "gpu_vs_cpu": {"lm_iterations": plain["value"] / (st["lm_iterations"] / lm_s),
+                       "pcg_iterations": plain["pcg_iterations_per_s"]
+                       / (st["cg_iterations"] / lm_s)},
+        "single_thread": {
+            "sample": f"first LM iteration with PCG capped at 4 ({int(st1['cg_iterations'])} "
+                      f"PCG iterations, {st1['lm_ms'] * 1e-3:.1f} s), 1 thread",
+            "seconds": st1["lm_ms"] * 1e-3, "cores": 1,
+        },
+    }
+
+
+def cpu_dense_ring() -> dict:
+    """The reference's dense reduced-camera CPU solver (bundleAdjust, bundle-adjustment.ts:80-291,
+    restated in the oracle) on the 10-camera ring (2,000 points, fixed focal, 20 LM iterations,
+    PCG <= 50 on the matrix-free routes), one thread, beside the GPU engine and the matrix-free
+    oracle on the same problem."""
+    from oracle import oracle as O
+    from typecv_amd import bundle as B
+    from typecv_amd import synthetic as S
+
+    ring, _ = S.ring_scene(seed=1)
+    control = B.make_control(ring.C, ring.P, ring.M, iterations=20, max_cg=50)
+    O.set_threads(1)
+    t = time.perf_counter()
+    dense = O.dense_solve(control, ring)
+    dense_s = time.perf_counter() - t
+    _, st = O.solve(control, ring)
+    O.set_threads(0)
+    base = ring.C * 12 + ring.P * 3
+    lm = dense[base + 10] + dense[base + 11]
+    with B.HipDevice(0) as device:
+        B.solve_packed(device, control, ring)
+        t = time.perf_counter()
+        got, rep = B.solve_packed(device, control, ring)
+        gpu_s = time.perf_counter() - t
+    return {
+        "value": lm / dense_s, "unit": "LM iterations/s", "cores": 1, "kind": "port",
+        "sample": f"the whole dense solve of the 10-camera ring ({ring.M} observations, "
+                  f"{lm:.0f} LM iterations, {dense_s:.2f} s)",
+        "matrix_free_oracle_lm_per_s": st["lm_iterations"] / (st["lm_ms"] * 1e-3),
+        "gpu_lm_per_s_host_arrays": rep["lm_iterations"] / gpu_s,
+        "final_cost": {"dense_cpu": float(dense[base + 6]), "gpu": float(got[base + 6])},
+    }
 
 
 def run_batch(B, device, controls, packeds, steps, warmup):
@@ -255,7 +417,7 @@ def batched(args) -> int:
     """256 independent local bundles (cameras U{5..40}, points U{500..20000}, each point seen by
     3-12 cameras of its bundle; the trajectory scene's noise, outliers and focal error), shared
     focal, Huber 1 px, 30 LM iterations, mixed precision, solved together by one
-    tcv_ba_solve_batch call per step. The value is the direct route (schurCache: each bundle's
+    tcv_ba_solve_batch call per step. The value is the direct route (directSolve: each bundle's
     reduced camera system factored, certified by the matrix-free residual); the PCG route is timed
     beside it unless --no-pcg. Reports bundles/s over the batches' LM loops and the p50/p99 of the
     per-bundle LM-loop latency."""
@@ -274,7 +436,7 @@ def batched(args) -> int:
     def controls_for(direct):
         cs = [B.make_control(p.C, p.P, p.M, shared_focal=True, huber=1.0,
                              iterations=args.iterations, max_cg=min(256, max(32, 12 * p.C + 1)),
-                             precision=args.precision, schur_cache=direct) for p in packeds]
+                             precision=args.precision, direct=direct) for p in packeds]
         # the bundles' default budgets differ (12C+1); a batch shares one budget: the largest
         budget = max(c.max_cg for c in cs)
         for c in cs:
@@ -315,7 +477,7 @@ def batched(args) -> int:
         "config": {"workload": "batched local bundles", "bundles": len(packeds),
                    "observations": M, "lm_iterations_max": args.iterations,
                    "precision": args.precision, "max_cg": budget,
-                   "linear_solver": "direct (schurCache), certified by the matrix-free residual"},
+                   "linear_solver": "direct (directSolve), certified by the matrix-free residual"},
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                        "max": float(lat.max())},
         "end_to_end_bundles_per_s": len(packeds) * args.steps / sum(walls),
@@ -414,7 +576,7 @@ def real_scene(args) -> int:
         for focal in ("fixed", "shared"):
             for cache in (False, True):
                 options = {**R.OPTIONS, "refineSharedFocal": focal == "shared",
-                           "schurCache": cache}
+                           "directSolve": cache}
                 problem = B.bundle_problem(model, options, (lambda C: min(256, max(32, 12 * C + 1)))
                                            if focal == "shared" else
                                            (lambda C: min(96, max(24, 12 * C))))
@@ -425,7 +587,7 @@ def real_scene(args) -> int:
                     scale_view=problem["byId"][problem["scaleId"]],
                     normalize_scale=problem["fixed"] == 1, shared_focal=focal == "shared",
                     iterations=problem["iterations"], max_cg=problem["maxCg"],
-                    huber=problem["huber"], tolerance=problem["tolerance"], schur_cache=cache)
+                    huber=problem["huber"], tolerance=problem["tolerance"], direct=cache)
                 times, out, rep = [], None, None
                 for _ in range(args.warmup + args.steps):
                     out, rep = B.solve_packed(device, control, packed)

@@ -27,8 +27,10 @@
  * reports them; the host turns them into its errors (reference `checkBundleOutput`,
  * src/sfm/bundle/gpu/bundle-host.ts:278-283).
  *
- * Thread safety: one context serialises its own calls; distinct contexts may be used from
- * distinct threads.
+ * Thread safety: every entry point that takes a context holds that context's lock for the whole
+ * call, so calls on one context from several threads run one after another (the reference's
+ * per-device FIFO of whole jobs, src/device/gpu/device.ts:592-598); distinct contexts run
+ * concurrently.
  */
 #ifndef TYPECV_BA_H
 #define TYPECV_BA_H
@@ -75,27 +77,37 @@ typedef struct tcv_ba_control {
   uint32_t normalize_scale;    /* 1 when exactly one camera is fixed */
   uint32_t shared_focal;       /* 0: fixed intrinsics (bundle.shade.ts); 1: shared log focal */
   uint32_t iterations;         /* LM iterations, 0..100 */
-  uint32_t max_cg;             /* PCG iterations per LM iteration, 1..256 */
+  uint32_t max_cg;             /* PCG iterations per LM iteration, 1..256 (per bundle in a batch;
+                                  each refinement restart has the same budget again) */
   uint32_t precision;          /* enum tcv_ba_precision */
   uint32_t flags;              /* 0; TCV_BA_FLAG_TIMING times the Schur sweeps with HIP events */
   double huber;                /* Huber threshold in pixels, [1e-12, 1e6] */
   double relative_tolerance;   /* PCG relative residual tolerance, [1e-12, 1e-2] */
   double stall_tolerance;      /* host LM stall rule; <= 1e-9 leaves only the device rule */
   double refinement_tolerance; /* mixed form: relative FP64 normal-equation residual that the
-                                  outer refinement reaches (0: relative_tolerance) */
+                                  outer refinement reaches (0: relative_tolerance). A step is
+                                  accepted when its final residual meets the larger of the two
+                                  tolerances (the one its solution was certified at) */
 } tcv_ba_control;
 
 #define TCV_BA_FLAG_TIMING 1u
 /* PCG (and the probe) use the implicit FP64 Schur operator: every cross block is relinearised from
  * the scene in each sweep instead of streamed from storage. Same results to rounding. */
 #define TCV_BA_FLAG_IMPLICIT 2u
-/* The direct route, the reference's opt-in `schurCache` (src/sfm/bundle/gpu/joint-focal.ts:102-108,
- * joint-focal.shade.ts:1206-1383): each bundle of at most 2,048 cameras assembles its reduced
- * camera system densely, factors it by Cholesky (one workgroup per bundle up to 128 cameras, a
- * sequence of launches above) and solves it in place of the PCG loop; the matrix-free residual
- * certifies the solution (PCG restarts from it when it misses the tolerance). Larger bundles, and
- * bundles in which a camera observes one point twice, keep the PCG route. */
+/* The reference's opt-in `schurCache` (src/sfm/bundle/gpu/joint-focal.ts:102-108). It is honoured
+ * only where the reference honours it: on the shared-focal route, for a bundle of at most 16
+ * cameras with M (6C + 1) <= 262,144. Such a bundle takes the direct route below (the reference runs
+ * PCG on the cached dense matrix instead; both are certified by the same matrix-free residual).
+ * Everywhere else the flag is validated and has no effect, as on the reference's fixed route
+ * (bundle.ts:825-826): the solve returns the matrix-free route's bits. */
 #define TCV_BA_FLAG_SCHUR_CACHE 4u
+/* The direct route, this engine's option `directSolve` (no counterpart in the reference): each
+ * bundle of at most 2,048 cameras assembles its reduced camera system densely, factors it by
+ * Cholesky (one workgroup per bundle up to 128 cameras, a sequence of launches above) and solves it
+ * in place of the PCG loop; the matrix-free residual certifies the solution (PCG restarts from it
+ * when it misses the tolerance), as joint-focal.shade.ts:1310-1383 certifies a cached solve.
+ * Larger bundles, and bundles in which a camera observes one point twice, keep the PCG route. */
+#define TCV_BA_FLAG_DIRECT 8u
 
 /* What one solve did. Times are milliseconds of host wall clock unless named otherwise. */
 typedef struct tcv_ba_report {
@@ -113,7 +125,7 @@ typedef struct tcv_ba_report {
   uint64_t device_bytes;    /* device memory held by the context after the solve */
   uint64_t point_tiles;     /* wave tiles of whole points */
   uint64_t camera_tasks;    /* camera-list tasks */
-  uint64_t direct_solves;   /* certified direct solves (TCV_BA_FLAG_SCHUR_CACHE) */
+  uint64_t direct_solves;   /* certified direct solves (TCV_BA_FLAG_DIRECT / SCHUR_CACHE) */
   uint64_t timed_matvecs;   /* Schur products timed into schur_ms (every 8th, TCV_BA_FLAG_TIMING) */
   uint64_t direct_corrections; /* refinement corrections of direct solutions from their factor
                                   (one per bundle whose certificate missed, at most two per solve) */
@@ -123,6 +135,8 @@ typedef struct tcv_ba_report {
   uint64_t fused_product;      /* 1: the PCG launches used the fused FP64 Schur product (one read
                                   of the cross blocks, a row of products per observation), as
                                   picked by the same tuner; 0: the two plain sweeps */
+  uint64_t streamed;           /* 1: the solve ran streamed under the context's working budget
+                                  (no stored cross blocks: every sweep relinearises them) */
 } tcv_ba_report;
 
 typedef struct tcv_ba_context tcv_ba_context;
@@ -147,8 +161,30 @@ int tcv_ba_create(int device, tcv_ba_context** out);
 void tcv_ba_destroy(tcv_ba_context* context);
 /* Device bytes currently held by the context (the reference's `statistics.liveBytes`). */
 uint64_t tcv_ba_device_bytes(const tcv_ba_context* context);
-/* Release every device buffer of the context (the reference's `drop`). */
+/* Release every device buffer of the context (the reference's `drop`). A call that fails past its
+ * argument checks releases them too, as the reference drops a failed job's buffers. */
 int tcv_ba_release(tcv_ba_context* context);
+
+/*
+ * The working budget of the solve entry points (tcv_ba_solve, tcv_ba_solve_device,
+ * tcv_ba_solve_batch) in bytes of device memory, staging of host arrays included; 0 (the default)
+ * sets none. The reference's `workingBytes` device option and residency plan
+ * (src/device/gpu/device.ts:360, src/device/gpu/budget-plan.ts:114-188, bundle.ts:195-242): a solve
+ * whose arrays all fit runs resident; otherwise it runs streamed when the arrays without the
+ * stored cross blocks fit (every Schur sweep then relinearises each block in registers, with the
+ * same results bit for bit); otherwise it is refused with TCV_BA_ERR_MEMORY and the reference's
+ * "GPU BA needs N bytes ... more than the working budget of B bytes" text before anything is
+ * allocated. A context whose held buffers would pass the budget releases them before allocating.
+ */
+int tcv_ba_set_working_bytes(tcv_ba_context* context, uint64_t bytes);
+
+/*
+ * Failure injection for tests of the error path: the n-th next read of the LM state words on this
+ * context fails with TCV_BA_ERR_DEVICE ("injected read failure"), as the reference's tests replace
+ * the device's read (tests/typeshade/cpu.test.mjs:508-531); 0 turns it off. No counterpart in the
+ * reference's operator API.
+ */
+int tcv_ba_fail_reads(tcv_ba_context* context, uint32_t n);
 
 /*
  * Solve from host arrays: upload, run the LM loop, read back `out`. The call that the Node-API

@@ -1,4 +1,4 @@
-"""The direct route (TCV_BA_FLAG_SCHUR_CACHE, the reference's `schurCache` option): the reduced
+"""The direct route (TCV_BA_FLAG_DIRECT, the option `directSolve`): the reduced
 camera system of each bundle of at most 2,048 cameras is assembled densely, factored by Cholesky
 (one workgroup per bundle up to 128 cameras, a sequence of launches above) and certified by the
 matrix-free residual.
@@ -78,7 +78,7 @@ def test_direct_matches_tight_pcg_oracle(device, name, packed, control):
     print(f"\n[direct] {name} C={packed.C} P={packed.P} M={packed.M}", flush=True)
     tight = variant(control, tolerance=1e-12, max_cg=TIGHT_CG)
     expected, stats = O.solve(tight, packed)
-    got, report = B.solve_packed(device, variant(control, schur_cache=True), packed)
+    got, report = B.solve_packed(device, variant(control, direct=True), packed)
     base = packed.C * 12 + packed.P * 3
     print(f"  direct solves {report['direct_solves']}, PCG {report['cg_iterations']} "
           f"(oracle PCG {stats['cg_iterations']:.0f})", flush=True)
@@ -93,7 +93,7 @@ def test_direct_matches_tight_pcg_oracle(device, name, packed, control):
 @pytest.mark.parametrize("name", ["trajectory_shared", "trajectory_large"])
 def test_direct_needs_no_pcg_when_the_factor_certifies(device, name):
     _, packed, control = BY_NAME[name]
-    got, report = B.solve_packed(device, variant(control, schur_cache=True), packed)
+    got, report = B.solve_packed(device, variant(control, direct=True), packed)
     base = packed.C * 12 + packed.P * 3
     steps = got[base + 10] + got[base + 11]
     print(f"\n[direct] certified {report['direct_solves']} of {steps:.0f} LM steps, "
@@ -105,7 +105,7 @@ def test_direct_needs_no_pcg_when_the_factor_certifies(device, name):
 def test_direct_ring_matches_the_dense_cpu_solver(device):
     print("\n[direct] cfg1 ring vs dense CPU solver", flush=True)
     ring, _ = S.ring_scene(seed=1)
-    control = B.make_control(ring.C, ring.P, ring.M, iterations=20, max_cg=50, schur_cache=True)
+    control = B.make_control(ring.C, ring.P, ring.M, iterations=20, max_cg=50, direct=True)
     got, report = B.solve_packed(device, control, ring)
     dense = O.dense_solve(control, ring)
     base = ring.C * 12 + ring.P * 3
@@ -127,7 +127,7 @@ def test_direct_mixed_against_fp64_oracle(device, name, packed, control):
     print(f"\n[direct mixed] {name}", flush=True)
     expected, _ = O.solve(variant(control, tolerance=1e-12, max_cg=TIGHT_CG), packed)
     for refine in (0.0, 1e-10):
-        got, report = B.solve_packed(device, variant(control, precision="mixed", schur_cache=True,
+        got, report = B.solve_packed(device, variant(control, precision="mixed", direct=True,
                                                      refinement_tolerance=refine), packed)
         base = packed.C * 12 + packed.P * 3
         print(f"  refine {refine:g}: direct {report['direct_solves']}, "
@@ -138,7 +138,7 @@ def test_direct_mixed_against_fp64_oracle(device, name, packed, control):
 def test_direct_is_deterministic(device):
     print("\n[direct] determinism", flush=True)
     _, packed, control = BY_NAME["trajectory_large"]
-    c = variant(control, schur_cache=True)
+    c = variant(control, direct=True)
     a, _ = B.solve_packed(device, c, packed)
     b, _ = B.solve_packed(device, c, packed)
     assert np.array_equal(a, b)
@@ -149,7 +149,7 @@ def test_direct_batch_equals_single_solves(device):
     scenes = [S.trajectory_scene(cameras=c, points=p, seed=s, track=(3, min(c, 12)))[0]
               for c, p, s in ((6, 800, 1), (130, 2000, 2), (25, 4000, 3), (2050, 6000, 4))]
     controls = [B.make_control(s.C, s.P, s.M, iterations=8, max_cg=256, shared_focal=True,
-                               huber=1.0, schur_cache=True) for s in scenes]
+                               huber=1.0, direct=True) for s in scenes]
     outs, report, _ = B.solve_batch(device, controls, scenes)
     singles = 0
     for scene, control, out in zip(scenes, controls, outs):
@@ -172,7 +172,7 @@ def test_direct_polish_and_backward_match_pcg(device, case):
     else:
         packed, control = T.joint_problem(fixed_two=case == "joint_two_fixed")
     at, lf = T.solved(device, packed, control)
-    direct = variant(control, schur_cache=True)
+    direct = variant(control, direct=True)
     pcg_out, _ = B.polish(device, control, at, lf, iterations=4, tolerance=1e-13)
     dir_out, prep = B.polish(device, direct, at, lf, iterations=4, tolerance=1e-13)
     base = packed.C * 12 + packed.P * 3
@@ -211,7 +211,7 @@ def test_large_direct_polish_matches_pcg(device):
                              shared_focal=True, huber=8.0, tolerance=1e-10)
     out, _ = B.solve_packed(device, control, scene)
     at, lf = B.at_solution(scene, out), B.output_log_focal(control, out)
-    direct = variant(control, schur_cache=True)
+    direct = variant(control, direct=True)
     pcg_out, _ = B.polish(device, control, at, lf, iterations=4, tolerance=1e-9)
     dir_out, prep = B.polish(device, direct, at, lf, iterations=4, tolerance=1e-9)
     base = scene.C * 12 + scene.P * 3
@@ -231,7 +231,7 @@ def test_large_direct_correction_from_the_factor(device):
     print("\n[direct] correction from the large factor", flush=True)
     scene, _ = S.trajectory_scene(cameras=160, points=6000, seed=4)
     control = B.make_control(scene.C, scene.P, scene.M, iterations=8, max_cg=256,
-                             shared_focal=True, huber=1.0, tolerance=1e-12, schur_cache=True)
+                             shared_focal=True, huber=1.0, tolerance=1e-12, direct=True)
     got, report = B.solve_packed(device, control, scene)
     base = scene.C * 12 + scene.P * 3
     steps = got[base + 10] + got[base + 11]
@@ -239,5 +239,5 @@ def test_large_direct_correction_from_the_factor(device):
           f"{report['direct_corrections']}, PCG {report['cg_iterations']}", flush=True)
     assert report["direct_corrections"] >= 1
     assert report["direct_solves"] == steps
-    expected, _ = O.solve(variant(control, schur_cache=False, max_cg=TIGHT_CG), scene)
+    expected, _ = O.solve(variant(control, direct=False, max_cg=TIGHT_CG), scene)
     compare(got, expected, base, 1e-9, 1e-6)

@@ -63,7 +63,7 @@ def test_node_batch_matches_python(device, tmp_path):
     scenes = [S.trajectory_scene(cameras=c, points=p, seed=s, track=(3, min(c, 8)))[0]
               for c, p, s in ((6, 600, 1), (9, 900, 2))]
     controls = [B.make_control(x.C, x.P, x.M, iterations=6, max_cg=128, shared_focal=True,
-                               huber=1.0, schur_cache=True) for x in scenes]
+                               huber=1.0, direct=True) for x in scenes]
     args = [node, str(NODE_DIR / "run.js"), "--batch", str(tmp_path / "out")]
     for b, (x, c) in enumerate(zip(scenes, controls)):
         control_vector(c).tofile(tmp_path / f"c{b}.f64")
@@ -115,3 +115,57 @@ def test_node_observations_match_python(device, tmp_path, guarded):
     got_summary = np.fromfile(tmp_path / "out.summary.f64", dtype=np.float64)
     assert np.array_equal(got_flags, flags[:req["count"] + tracks])
     assert np.array_equal(got_summary, summary)
+
+
+def test_node_concurrent_solves_on_one_context(device, tmp_path):
+    """Two un-awaited `addon.solve` calls on one context (libuv workers, no FIFO in between) and a
+    third after them return the Python binding's array, bit for bit: the context's lock runs them
+    one after another."""
+    node = shutil.which("node")
+    if node is None or not (NODE_DIR / "typecv_ba.node").exists():
+        pytest.skip("no Node or no built addon")
+    print("\n[node] concurrent solves", flush=True)
+    packed, _ = S.trajectory_scene(cameras=20, points=3000, seed=2)
+    control = B.make_control(packed.C, packed.P, packed.M, iterations=10, max_cg=241,
+                             shared_focal=True, huber=1.0)
+    control_vector(control).tofile(tmp_path / "control.f64")
+    packed.scene.astype(np.float64).tofile(tmp_path / "scene.f64")
+    packed.index.astype(np.uint32).tofile(tmp_path / "index.u32")
+    run = subprocess.run([node, str(NODE_DIR / "run.js"), "--concurrent",
+                          str(tmp_path / "control.f64"), str(tmp_path / "scene.f64"),
+                          str(tmp_path / "index.u32"), str(tmp_path / "out")],
+                         capture_output=True, text=True, timeout=120)
+    assert run.returncode == 0, run.stderr
+    expected, _ = B.solve_packed(device, control, packed)
+    for name in ("0", "1", "seq"):
+        got = np.fromfile(tmp_path / f"out.{name}.f64", dtype=np.float64)
+        assert np.array_equal(got, expected), name
+
+
+@pytest.mark.parametrize("mode", ["polish", "jacobian"])
+def test_node_derivative_entries_match_python(device, tmp_path, mode):
+    node = shutil.which("node")
+    if node is None or not (NODE_DIR / "typecv_ba.node").exists():
+        pytest.skip("no Node or no built addon")
+    print(f"\n[node] {mode}", flush=True)
+    packed, _ = S.trajectory_scene(cameras=10, points=600, seed=6)
+    control = B.make_control(packed.C, packed.P, packed.M, iterations=15, max_cg=121,
+                             shared_focal=True, huber=1.0)
+    out, _ = B.solve_packed(device, control, packed)
+    at = B.at_solution(packed, out)
+    lf = B.output_log_focal(control, out)
+    control_vector(control).tofile(tmp_path / "control.f64")
+    at.scene.astype(np.float64).tofile(tmp_path / "scene.f64")
+    at.index.astype(np.uint32).tofile(tmp_path / "index.u32")
+    run = subprocess.run([node, str(NODE_DIR / "run.js"), f"--{mode}",
+                          str(tmp_path / "control.f64"), str(tmp_path / "scene.f64"),
+                          str(tmp_path / "index.u32"), repr(lf), str(tmp_path / "out.f64")],
+                         capture_output=True, text=True, timeout=120)
+    assert run.returncode == 0, run.stderr
+    got = np.fromfile(tmp_path / "out.f64", dtype=np.float64)
+    if mode == "polish":
+        expected, _ = B.polish(device, control, at, lf, iterations=3, max_cg=4000,
+                               tolerance=1e-12, lm_iterations=30)
+    else:
+        expected = B.jacobian(device, control, at, lf).reshape(-1)
+    assert np.array_equal(got, expected)

@@ -2,7 +2,7 @@
 observations of COLMAP's SIFT matches; tests/real_scene.py) against the reference's recorded runs
 and the oracle: final RMSE within 1e-9 relative of the record (the reference's WebGPU tier rounds
 differently), 4 accepted steps, and the oracle's parity bars (final cost within 1e-9 relative,
-coordinates within 1e-7·max(1, |x|)). The direct route (`schurCache`) solves each step exactly
+coordinates within 1e-7·max(1, |x|)). The direct route (`directSolve`) solves each step exactly
 where the record's PCG stops at a relative residual of 1e-6: its final RMSE is within 1e-7."""
 import numpy as np
 import pytest
@@ -18,9 +18,9 @@ pytestmark = pytest.mark.gpu
 @pytest.mark.parametrize("focal", ["fixed", "shared"])
 @pytest.mark.parametrize("cache", [False, True])
 def test_operator_reproduces_the_recorded_run(device, focal, cache):
-    print(f"\n[south-building-24] {focal} schurCache={cache}", flush=True)
+    print(f"\n[south-building-24] {focal} directSolve={cache}", flush=True)
     model = R.reconstruction()
-    options = {**R.OPTIONS, "refineSharedFocal": focal == "shared", "schurCache": cache}
+    options = {**R.OPTIONS, "refineSharedFocal": focal == "shared", "directSolve": cache}
     result = B.bundle_adjust_gpu(device, model, options)
     d = result["diagnostics"]
     record = R.RECORD[focal]

@@ -1,0 +1,116 @@
+"""Parity at the benchmark configurations' real sizes.
+
+* The 10-camera ring (2,000 points, ~12k observations, 20 LM iterations, PCG <= 50) against the
+  matrix-free oracle at the parity bars: final cost within 1e-9 relative, every coordinate x within
+  1e-7·max(1, |x|), the accept/reject/failure/input-error words equal.
+* The 500-camera trajectory scene (200,000 points, 1.89 M observations, shared focal, Huber 1 px,
+  PCG <= 256): its first 3 LM iterations against the oracle at the same bars.
+* The 2,000-camera scene (1 M points, 9.7 M observations, 30 LM iterations): the mixed form with
+  FP64 refinement to 1e-10 against the FP64 form on the GPU: final cost within 1e-6 relative.
+* All 256 local bundles (cameras 5-40, points 500-20,000) in one mixed batch: a sample of bundles
+  equals their single solves bit for bit; the smallest bundles of the FP64 batch against the oracle.
+
+Each prints the largest absolute coordinate difference beside the relative bar.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from typecv_amd import bundle as B
+from typecv_amd import synthetic as S
+
+pytestmark = pytest.mark.gpu
+
+SHARED_CG = lambda C: min(256, max(32, 12 * C + 1))  # noqa: E731
+
+
+def compare(got, expected, base, label):
+    """The parity bars; returns the largest absolute and relative coordinate differences."""
+    cost = expected[base + 6]
+    diff = np.abs(got[:base] - expected[:base])
+    rel = diff / np.maximum(1.0, np.abs(expected[:base]))
+    print(f"  {label}: cost {got[base + 6]:.15g} vs {cost:.15g} "
+          f"({abs(got[base + 6] - cost) / max(1.0, abs(cost)):.2e}); coordinates max abs "
+          f"{diff.max():.2e}, max rel {rel.max():.2e}; words 10/11/17 "
+          f"{got[base + 10]:.0f}/{got[base + 11]:.0f}/{got[base + 17]:.0f}", flush=True)
+    assert abs(got[base + 6] - cost) <= 1e-9 * max(1.0, abs(cost))
+    assert rel.max() <= 1e-7
+    for word in (10, 11, 14, 17):
+        assert got[base + word] == expected[base + word], word
+    return diff.max(), rel.max()
+
+
+def test_ring_10_cameras_matches_the_oracle(device):
+    print("\n[scale] 10-camera ring", flush=True)
+    ring, _ = S.ring_scene(seed=1)
+    control = B.make_control(ring.C, ring.P, ring.M, iterations=20, max_cg=50)
+    got, _ = B.solve_packed(device, control, ring)
+    expected, _ = O.solve(control, ring)
+    compare(got, expected, ring.C * 12 + ring.P * 3, f"C={ring.C} M={ring.M}")
+
+
+def test_trajectory_500_cameras_first_lm_iterations_match_the_oracle(device):
+    print("\n[scale] 500-camera trajectory, 3 LM iterations", flush=True)
+    packed, _ = S.trajectory_scene(cameras=500, points=200_000, seed=2)
+    control = B.make_control(packed.C, packed.P, packed.M, shared_focal=True, huber=1.0,
+                             iterations=3, max_cg=SHARED_CG(packed.C))
+    got, report = B.solve_packed(device, control, packed)
+    expected, stats = O.solve(control, packed)
+    base = packed.C * 12 + packed.P * 3
+    print(f"  PCG iterations: GPU {report['cg_iterations']}, oracle {stats['cg_iterations']:.0f}",
+          flush=True)
+    assert got[base + 10] + got[base + 11] == 3
+    compare(got, expected, base, f"C={packed.C} P={packed.P} M={packed.M}")
+    assert abs(got[base + 26] - expected[base + 26]) <= 1e-9
+
+
+def test_trajectory_2000_cameras_mixed_matches_fp64(device):
+    print("\n[scale] 2,000-camera trajectory, mixed (refinement 1e-10) vs FP64", flush=True)
+    packed, _ = S.trajectory_scene(cameras=2000, points=1_000_000, seed=2)
+    kw = dict(shared_focal=True, huber=1.0, iterations=30, max_cg=SHARED_CG(packed.C))
+    fp64, r64 = B.solve_packed(device, B.make_control(packed.C, packed.P, packed.M, **kw), packed)
+    mixed, rmx = B.solve_packed(device, B.make_control(packed.C, packed.P, packed.M,
+                                                       precision="mixed",
+                                                       refinement_tolerance=1e-10, **kw), packed)
+    C, P = packed.C, packed.P
+    base = C * 12 + P * 3
+    cost = fp64[base + 6]
+    rel_cost = abs(mixed[base + 6] - cost) / cost
+    pose = np.abs(mixed[:12 * C] - fp64[:12 * C]).max()
+    point = np.abs(mixed[12 * C:base] - fp64[12 * C:base]).max()
+    print(f"  FP64 {r64['solve_ms']:.0f} ms, {r64['cg_iterations']} PCG; mixed "
+          f"{rmx['solve_ms']:.0f} ms, {rmx['cg_iterations']} PCG, "
+          f"{rmx['refinement_steps']:.0f} refinements", flush=True)
+    print(f"  final cost {mixed[base + 6]:.15g} vs {cost:.15g} ({rel_cost:.2e}); max pose "
+          f"difference {pose:.2e}, max point difference {point:.2e}; log focal "
+          f"{mixed[base + 26]:.12g} vs {fp64[base + 26]:.12g}", flush=True)
+    assert mixed[base + 6] < mixed[base + 8]
+    assert rel_cost <= 1e-6
+
+
+def test_all_256_local_bundles(device):
+    print("\n[scale] 256 local bundles", flush=True)
+    bundles = S.local_bundles(count=256, seed=4)
+    packeds = [p for p, _ in bundles]
+
+    def controls(precision):
+        return [B.make_control(p.C, p.P, p.M, shared_focal=True, huber=1.0, iterations=30,
+                               max_cg=SHARED_CG(p.C), precision=precision) for p in packeds]
+
+    mixed = controls("mixed")
+    outs, report, finish = B.solve_batch(device, mixed, packeds)
+    print(f"  mixed batch: {report['lm_iterations']} LM, {report['cg_iterations']} PCG, "
+          f"{report['solve_ms']:.0f} ms", flush=True)
+    assert np.all(finish > 0)
+    rng = np.random.default_rng(11)
+    for b in sorted(rng.choice(len(packeds), size=6, replace=False)):
+        alone, _ = B.solve_packed(device, mixed[b], packeds[b])
+        assert np.array_equal(outs[b], alone), b
+    fp64 = controls("fp64")
+    smallest = sorted(range(len(packeds)), key=lambda b: packeds[b].M)[:3]
+    outs, _, _ = B.solve_batch(device, [fp64[b] for b in smallest], [packeds[b] for b in smallest])
+    for got, b in zip(outs, smallest):
+        expected, _ = O.solve(fp64[b], packeds[b])
+        packed = packeds[b]
+        compare(got, expected, packed.C * 12 + packed.P * 3, f"bundle {b} C={packed.C} "
+                f"M={packed.M}")

@@ -85,6 +85,8 @@ def test_option_enum_errors_are_synchronous():
         B._check_option_enums({"schurReduction": "fast"})
     with pytest.raises(B.RangeError, match="schurCache"):
         B._check_option_enums({"schurCache": 1})
+    with pytest.raises(B.RangeError, match="directSolve"):
+        B._check_option_enums({"directSolve": "yes"})
     with pytest.raises(B.RangeError, match="stallTolerance"):
         B.bundle_stall_tolerance({"stallTolerance": 1.0})
 
@@ -138,5 +140,6 @@ def test_node_addon_loads_and_exports_the_operator():
     out = subprocess.run([node, str(addon.parent / "run.js"), "--exports"], capture_output=True,
                          text=True, timeout=60)
     assert out.returncode == 0, out.stderr
-    assert json.loads(out.stdout) == ["backward", "deviceBytes", "evaluate", "open", "outputSize",
-                                       "release", "solve", "solveBatch"]
+    assert json.loads(out.stdout) == ["backward", "deviceBytes", "evaluate", "jacobian", "open",
+                                       "outputSize", "polish", "release", "setWorkingBytes",
+                                       "solve", "solveBatch"]

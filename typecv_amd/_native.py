@@ -14,6 +14,7 @@ MIXED = 1
 FLAG_TIMING = 1
 FLAG_IMPLICIT = 2
 FLAG_SCHUR_CACHE = 4
+FLAG_DIRECT = 8
 
 STATUS = {
     0: "ok",
@@ -36,6 +37,8 @@ EXPORTS = (
     "tcv_ba_destroy",
     "tcv_ba_device_bytes",
     "tcv_ba_release",
+    "tcv_ba_set_working_bytes",
+    "tcv_ba_fail_reads",
     "tcv_ba_solve",
     "tcv_ba_solve_device",
     "tcv_ba_solve_batch",
@@ -93,6 +96,7 @@ class Report(ctypes.Structure):
         ("direct_corrections", ctypes.c_uint64),
         ("camera_sweep_cap", ctypes.c_uint64),
         ("fused_product", ctypes.c_uint64),
+        ("streamed", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -148,6 +152,10 @@ def library() -> ctypes.CDLL:
     lib.tcv_ba_device_bytes.restype = ctypes.c_uint64
     lib.tcv_ba_release.argtypes = [ctypes.c_void_p]
     lib.tcv_ba_release.restype = ctypes.c_int
+    lib.tcv_ba_set_working_bytes.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
+    lib.tcv_ba_set_working_bytes.restype = ctypes.c_int
+    lib.tcv_ba_fail_reads.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
+    lib.tcv_ba_fail_reads.restype = ctypes.c_int
     lib.tcv_ba_solve.argtypes = [ctypes.c_void_p, c_control, ctypes.c_void_p, ctypes.c_void_p,
                                  ctypes.c_void_p, P(Report)]
     lib.tcv_ba_solve.restype = ctypes.c_int

@@ -256,7 +256,7 @@ class HipDevice:
 
     backend = "hip"
 
-    def __init__(self, device: int = 0):
+    def __init__(self, device: int = 0, working_bytes: int = 0):
         lib = _native.library()
         handle = ctypes.c_void_p()
         _native.check(lib.tcv_ba_create(device, ctypes.byref(handle)), "tcv_ba_create")
@@ -267,10 +267,22 @@ class HipDevice:
         self.upload_bytes = 0
         self.readback_bytes = 0
         self.jobs = 0
+        self.working_bytes = 0
+        if working_bytes:
+            self.set_working_bytes(working_bytes)
 
     @classmethod
-    def open(cls, device: int = 0) -> "HipDevice":
-        return cls(device)
+    def open(cls, device: int = 0, working_bytes: int = 0) -> "HipDevice":
+        """`ShadeDevice.open({workingBytes})` (device.ts:357-360): `working_bytes` bounds the device
+        memory of one bundle adjustment (0: no bound); a problem above it streams or is refused."""
+        return cls(device, working_bytes)
+
+    def set_working_bytes(self, working_bytes: int) -> None:
+        if not _safe_int(working_bytes) or working_bytes < 0:
+            raise RangeError("workingBytes must be a non-negative integer")
+        _native.check(self._lib.tcv_ba_set_working_bytes(self.handle, int(working_bytes)),
+                      "tcv_ba_set_working_bytes")
+        self.working_bytes = int(working_bytes)
 
     @property
     def handle(self) -> ctypes.c_void_p:
@@ -317,11 +329,15 @@ def make_control(C: int, P: int, M: int, *, directional: int = 0, anchor: int = 
                  iterations: int = 20, max_cg: int = 24, precision: str = "fp64",
                  huber: float = 2.0, tolerance: float = 1e-6, stall_tolerance: float = 1e-9,
                  refinement_tolerance: float = 0.0, timing: bool = False,
-                 implicit: bool = False, schur_cache: bool = False) -> _native.Control:
-    """The `tcv_ba_control` of a problem. `schur_cache`: the direct route (dense reduced camera
-    system, Cholesky, matrix-free certificate) for bundles of at most 2,048 cameras."""
+                 implicit: bool = False, schur_cache: bool = False,
+                 direct: bool = False) -> _native.Control:
+    """The `tcv_ba_control` of a problem. `direct`: the direct route (dense reduced camera system,
+    Cholesky, matrix-free certificate) for bundles of at most 2,048 cameras; `schur_cache`: the
+    reference's `schurCache`, which takes the direct route only inside its guard (shared focal,
+    at most 16 cameras, M (6C + 1) <= 262,144) and is ignored elsewhere."""
     flags = ((_native.FLAG_TIMING if timing else 0) | (_native.FLAG_IMPLICIT if implicit else 0)
-             | (_native.FLAG_SCHUR_CACHE if schur_cache else 0))
+             | (_native.FLAG_SCHUR_CACHE if schur_cache else 0)
+             | (_native.FLAG_DIRECT if direct else 0))
     return _native.Control(
         cameras=C, points=P, directional_points=directional, observations=M, anchor=anchor,
         scale_view=scale_view, normalize_scale=int(normalize_scale),
@@ -508,6 +524,9 @@ def _check_option_enums(options: Mapping) -> None:
     cache = options.get("schurCache")
     if cache is not None and not isinstance(cache, bool):
         raise RangeError("schurCache must be a boolean")
+    direct = options.get("directSolve")
+    if direct is not None and not isinstance(direct, bool):
+        raise RangeError("directSolve must be a boolean")
     reduction = options.get("schurReduction")
     if reduction is not None and reduction not in ("auto", "serial", "cooperative"):
         raise RangeError("Invalid Schur reduction mode")
@@ -553,7 +572,8 @@ def bundle_adjust_gpu(device: HipDevice, scene: Mapping, options: Mapping | None
         shared_focal=shared, iterations=problem["iterations"], max_cg=problem["maxCg"],
         precision=options.get("precision", "fp64"), huber=problem["huber"],
         tolerance=problem["tolerance"], stall_tolerance=stall,
-        schur_cache=options.get("schurCache") is True)
+        schur_cache=options.get("schurCache") is True,
+        direct=options.get("directSolve") is True)
     before = device.statistics
     values, report = solve_packed(device, control, packed)
     after = device.statistics

@@ -1093,6 +1093,7 @@ __global__ __launch_bounds__(CG_THREADS) void k_certify(Problem q, int final_pas
     s[S_RHO] = rho;
     s[S_RHO_NEXT] = rho;
     s[S_DIRECT] = 0;
+    s[S_STEP_LIMIT] = s[S_STEPS] + pcg_budget(q, g);  // PCG from x with the bundle's budget
     if (rho > 0)
       s[S_PCG_ACTIVE] = 1;
     else

@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -46,6 +47,8 @@ double now_ms() {
 }  // namespace
 
 struct tcv_ba_context {
+  // held by every C ABI entry for the whole call: calls on one context run one after another
+  mutable std::mutex lock;
   int device = 0;
   hipStream_t stream = nullptr;
   double* h_state = nullptr;  // pinned copy of the state words of every bundle
@@ -69,6 +72,10 @@ struct tcv_ba_context {
   uint32_t camera_cap = 0;
   bool tuned_fused = false;
   std::vector<hipEvent_t> tune_events;
+  // the working budget of a solve in bytes (0: none; tcv_ba_set_working_bytes) and the read
+  // failure injected by tcv_ba_fail_reads (the n-th next read of the state words fails; 0: none)
+  uint64_t budget = 0;
+  uint32_t fail_read_in = 0;
   tcv_ba_context() {
     all = {&scene,      &trial,        &index,     &out,       &bundles,    &tiles,
            &tile_bundle, &tasks,       &task_start, &cost_blocks, &camera_bundle,
@@ -137,7 +144,7 @@ int check_control(const tcv_ba_control* c) {
                   (c->refinement_tolerance == 0 ||
                    (c->refinement_tolerance >= 1e-14 && c->refinement_tolerance <= 1e-2)) &&
                   (c->flags & ~(TCV_BA_FLAG_TIMING | TCV_BA_FLAG_IMPLICIT |
-                                TCV_BA_FLAG_SCHUR_CACHE)) == 0;
+                                TCV_BA_FLAG_SCHUR_CACHE | TCV_BA_FLAG_DIRECT)) == 0;
   if (!ok) {
     set_error("control outside the admitted ranges (bundleProblem)");
     return TCV_BA_ERR_ARGUMENT;
@@ -145,10 +152,10 @@ int check_control(const tcv_ba_control* c) {
   return TCV_BA_OK;
 }
 
-// The options every bundle of a batch shares.
+// The options every bundle of a batch shares (each keeps its own sizes, gauge and PCG budget).
 bool same_options(const tcv_ba_control& a, const tcv_ba_control& b) {
   return a.shared_focal == b.shared_focal && a.iterations == b.iterations &&
-         a.max_cg == b.max_cg && a.precision == b.precision && a.flags == b.flags &&
+         a.precision == b.precision && a.flags == b.flags &&
          a.huber == b.huber && a.relative_tolerance == b.relative_tolerance &&
          a.stall_tolerance == b.stall_tolerance &&
          a.refinement_tolerance == b.refinement_tolerance;
@@ -162,10 +169,17 @@ struct Plan {
   uint32_t covis_words = 0;
 };
 
-// The direct route for the bundles of at most DENSE_MAX_CAMERAS cameras (all others stay on PCG),
-// except those marked in `excluded`. Outside the exact systems a direct bundle stores no cross
-// blocks.
-void plan_direct(Plan& plan, bool want, uint32_t shared, bool exact,
+// The reference's guard of `schurCache` (joint-focal.ts:102-108): the shared-focal route, at most
+// 16 cameras and M (6C + 1) within its work limit of 262,144.
+bool schur_cache_admits(uint32_t shared, uint32_t cams, uint32_t obs) {
+  return shared != 0 && cams <= 16 && (uint64_t)obs * (6 * cams + 1) <= 262144;
+}
+
+// The direct route for the bundles that ask for it (TCV_BA_FLAG_DIRECT: every bundle of at most
+// DENSE_LARGE_CAMERAS cameras; TCV_BA_FLAG_SCHUR_CACHE: those inside the reference's guard), except
+// those marked in `excluded`; all others stay on PCG. Outside the exact systems a direct bundle
+// stores no cross blocks.
+void plan_direct(Plan& plan, uint32_t flags, uint32_t shared, bool exact,
                  const std::vector<char>& excluded = {}) {
   plan.direct_bundles.clear();
   plan.direct_cams.clear();
@@ -177,6 +191,9 @@ void plan_direct(Plan& plan, bool want, uint32_t shared, bool exact,
     Bundle& g = plan.bundles[b];
     const uint32_t cams = g.cam1 - g.cam0;
     const bool out = b < excluded.size() && excluded[b];
+    const bool want = (flags & TCV_BA_FLAG_DIRECT) != 0 ||
+                      ((flags & TCV_BA_FLAG_SCHUR_CACHE) != 0 &&
+                       schur_cache_admits(shared, cams, g.obs1 - g.obs0));
     g.direct = want && cams <= DENSE_LARGE_CAMERAS && !out ? 1u : 0u;
     g.large = g.direct && cams > DENSE_MAX_CAMERAS ? 1u : 0u;
     g.unstored = g.direct && !exact ? 1u : 0u;
@@ -250,6 +267,7 @@ int build_plan(const std::vector<uint32_t>& ps, const std::vector<uint32_t>& cs,
     g.scale_view = cam + k.scale_view;
     g.normalize_scale = k.normalize_scale;
     g.focal_row = 6 * C + b;
+    g.max_cg = k.max_cg;
     // tiles of whole points of this bundle
     g.tile0 = (uint32_t)plan.tiles.size() - 1;
     uint32_t first = g.pt0;
@@ -371,6 +389,19 @@ struct Launch {
   }
   static void update_points(const Problem& q, hipStream_t s) {
     hipLaunchKernelGGL((k_point_sweep<WT, SHARED, SWEEP_UPDATE>), dim3(tile_blocks(q)),
+                       dim3(TILE * TILE_WAVES), 0, s, q);
+  }
+  // The streamed route: the same sweeps with every cross block relinearised in registers.
+  static void streamed_point_sweep(const Problem& q, hipStream_t s) {
+    hipLaunchKernelGGL((k_point_sweep<WT, SHARED, SWEEP_SCHUR, true>), dim3(tile_blocks(q)),
+                       dim3(TILE * TILE_WAVES), 0, s, q);
+  }
+  static void streamed_camera_sweep(const Problem& q, hipStream_t s) {
+    hipLaunchKernelGGL((k_camera_sweep<WT, SHARED, false, true>), dim3(q.n_tasks),
+                       dim3(TASK_THREADS), 0, s, q);
+  }
+  static void streamed_update(const Problem& q, hipStream_t s) {
+    hipLaunchKernelGGL((k_point_sweep<WT, SHARED, SWEEP_UPDATE, true>), dim3(tile_blocks(q)),
                        dim3(TILE * TILE_WAVES), 0, s, q);
   }
   static void probe(const Problem& q, const double* v, double* sv, double* zv, double* rhs,
@@ -531,7 +562,7 @@ void update_points(const Problem& q, const Ops& ops, hipStream_t st) {
 }
 
 template <class WT, bool SHARED>
-Ops ops_for(bool implicit) {
+Ops ops_for(bool implicit, bool streamed) {
   using L = Launch<WT, SHARED>;
   Ops o{L::point_pass,   L::camera_pass,   L::init_cg,         L::point_sweep,
         L::camera_sweep, L::advance,       L::update_points,   L::probe,
@@ -544,16 +575,22 @@ Ops ops_for(bool implicit) {
     o.camera_sweep = L::implicit_camera_sweep;
     o.unstored_point_sweep = nullptr;
     o.unstored_camera_sweep = nullptr;
+  } else if (streamed) {
+    o.point_sweep = L::streamed_point_sweep;
+    o.camera_sweep = L::streamed_camera_sweep;
+    o.update_points = L::streamed_update;
   }
   return o;
 }
 
-Ops select_ops(const tcv_ba_control* c) {
+Ops select_ops(const tcv_ba_control* c, bool streamed = false) {
   const bool shared = c->shared_focal != 0;
   const bool implicit = (c->flags & TCV_BA_FLAG_IMPLICIT) != 0;
   if (c->precision == TCV_BA_MIXED)
-    return shared ? ops_for<float, true>(implicit) : ops_for<float, false>(implicit);
-  return shared ? ops_for<double, true>(implicit) : ops_for<double, false>(implicit);
+    return shared ? ops_for<float, true>(implicit, streamed)
+                  : ops_for<float, false>(implicit, streamed);
+  return shared ? ops_for<double, true>(implicit, streamed)
+                : ops_for<double, false>(implicit, streamed);
 }
 
 template <class T>
@@ -565,6 +602,10 @@ int upload(DevBuf& buf, const std::vector<T>& v, hipStream_t st) {
 }
 
 int read_state(tcv_ba_context* ctx, const Problem& q, hipStream_t st, tcv_ba_report* rep) {
+  if (ctx->fail_read_in && --ctx->fail_read_in == 0) {
+    set_error("injected read failure of the state words");
+    return TCV_BA_ERR_DEVICE;
+  }
   HIP_TRY(hipMemcpyAsync(ctx->h_state, q.state, (size_t)q.B * S_WORDS * 8, hipMemcpyDeviceToHost,
                          st));
   HIP_TRY(hipStreamSynchronize(st));
@@ -588,11 +629,142 @@ double max_word(const tcv_ba_context* ctx, const Problem& q, int w) {
   return v;
 }
 
+// A device array of a solve and the bytes the solve needs in it.
+struct Need {
+  DevBuf* buf;
+  size_t bytes;
+};
+
+// Every device array a solve of `plan` allocates (besides the caller's staging): the resident
+// form stores the cross blocks (and, with FP64 blocks, the fused product's rows); the streamed
+// form relinearises them in every sweep and stores none.
+std::vector<Need> solve_needs(tcv_ba_context* ctx, const tcv_ba_control* c, uint32_t B,
+                              uint32_t C, uint32_t P, uint32_t M, const Plan& plan, bool exact,
+                              bool streamed) {
+  const uint32_t shared = c->shared_focal ? 1u : 0u;
+  const size_t ws = c->precision == TCV_BA_MIXED ? 4 : 8;
+  const uint32_t K = shared ? 21 : 18;
+  const size_t m = (size_t)C * 6 + shared * B;
+  const uint32_t n_unstored = exact ? 0u : (uint32_t)plan.direct_bundles.size();
+  const size_t n_tiles = plan.tiles.size() - 1, n_tasks = plan.tasks.size() / 4;
+  const size_t n_cost = plan.cost_blocks.size() / 3, n_groups = plan.groups.size() / GROUP_WORDS;
+  const size_t w_words = (size_t)K * (((size_t)M + 63) / 64) * 64;
+  std::vector<Need> n = {
+      {&ctx->scene, ((size_t)C * 18 + (size_t)P * 3 + (size_t)M * 2) * 8},
+      {&ctx->trial, ((size_t)C * 12 + (size_t)P * 3) * 8},
+      {&ctx->cam_point, (size_t)M * 4},
+      {&ctx->pblk, (size_t)P * 12 * 8},
+      {&ctx->tile_fail, n_tiles * 4},
+      {&ctx->tile_focal, n_tiles * 16},
+      {&ctx->task_part, n_tasks * ASM_WORDS * 8},
+      {&ctx->camA, (size_t)C * 36 * 8},
+      {&ctx->camInv, (size_t)C * 36 * 8},
+      {&ctx->camRhs, (size_t)C * 6 * 8},
+      {&ctx->camHcf, (size_t)C * 6 * 8},
+      {&ctx->camFocal, (size_t)C * 2 * 8},
+      {&ctx->vec, (m * 5 + (size_t)P * 3) * 8},
+      {&ctx->state, (size_t)B * S_WORDS * 8},
+      {&ctx->cost_part, n_cost * 3 * 8},
+      {&ctx->error, 8},
+      {&ctx->group_part, n_groups * 4 * 8},
+      {&ctx->bundles, plan.bundles.size() * sizeof(Bundle)},
+      {&ctx->tiles, plan.tiles.size() * 4},
+      {&ctx->tile_bundle, plan.tile_bundle.size() * 4},
+      {&ctx->tasks, plan.tasks.size() * 4},
+      {&ctx->task_start, plan.task_start.size() * 4},
+      {&ctx->cost_blocks, plan.cost_blocks.size() * 4},
+      {&ctx->camera_bundle, plan.camera_bundle.size() * 4},
+      {&ctx->point_bundle, plan.point_bundle.size() * 4},
+      {&ctx->groups, plan.groups.size() * 4},
+  };
+  if (n_unstored < B && !streamed) {
+    n.push_back({&ctx->Wp, w_words * ws});
+    n.push_back({&ctx->Wc, w_words * ws});
+    // the fused FP64 product's rows (the tuner times it against the plain sweeps, which compute
+    // the same bits); FP64 blocks only: over FP32 ones it is slower (70.8 + 27.4 against 42.6 +
+    // 40.0 us at 500 cameras), since the rows stay FP64 while the blocks it saves are halved
+    if (ws == 8) n.push_back({&ctx->tprod, (size_t)M * 64});
+  }
+  if (!plan.direct_bundles.empty()) {
+    n.push_back({&ctx->camS, (size_t)C * 36 * 8});
+    n.push_back({&ctx->camG, (size_t)M * 18 * 8});
+    if (shared) n.push_back({&ctx->pfc, (size_t)P * 3 * 8});
+    n.push_back({&ctx->dense, plan.dense_words * 8});
+    n.push_back({&ctx->covis, (size_t)plan.covis_words * 4});
+    n.push_back({&ctx->direct_bundles, plan.direct_bundles.size() * 4});
+    n.push_back({&ctx->direct_cams, plan.direct_cams.size() * 4});
+    n.push_back({&ctx->small_bundles, plan.small_bundles.size() * 4});
+  }
+  for (Need& x : n) x.bytes = x.bytes ? x.bytes : 8;
+  return n;
+}
+
+size_t need_bytes(const std::vector<Need>& needs) {
+  size_t total = 0;
+  for (const Need& x : needs) total += x.bytes;
+  return total;
+}
+
+// The bytes the context holds once every need is allocated (buffers grow and are never shrunk).
+size_t held_after(const tcv_ba_context* ctx, const std::vector<Need>& needs) {
+  size_t total = 0;
+  for (const DevBuf* b : ctx->all) {
+    size_t bytes = b->bytes;
+    for (const Need& x : needs)
+      if (x.buf == b) bytes = std::max(bytes, x.bytes);
+    total += bytes;
+  }
+  return total;
+}
+
+int release_buffers(tcv_ba_context* ctx);
+
+// The residency plan under the context's working budget (reference planBundleResidency,
+// src/device/gpu/budget-plan.ts:114-188, and its refusal, src/sfm/bundle/gpu/bundle.ts:214-242):
+// resident when every array of the solve, the caller's `staging` included, fits the budget;
+// otherwise streamed when the arrays without the cross blocks fit; otherwise refused before any
+// allocation. With a budget, a context whose held buffers would pass it releases them first.
+// No budget (0): resident. `streamed` receives the decision.
+int admit(tcv_ba_context* ctx, const tcv_ba_control* c, uint32_t B, uint32_t C, uint32_t P,
+          uint32_t M, const Plan& plan, bool exact, const std::vector<Need>& staging,
+          bool& streamed) {
+  streamed = false;
+  if (ctx->budget == 0) return TCV_BA_OK;
+  std::vector<Need> resident = solve_needs(ctx, c, B, C, P, M, plan, exact, false);
+  std::vector<Need> floor = solve_needs(ctx, c, B, C, P, M, plan, exact, true);
+  resident.insert(resident.end(), staging.begin(), staging.end());
+  floor.insert(floor.end(), staging.begin(), staging.end());
+  const size_t need = need_bytes(floor);
+  if (need > ctx->budget) {
+    set_error("GPU BA needs " + std::to_string(need) +
+              " bytes for the arrays that stay resident and a window of one track, more than the "
+              "working budget of " + std::to_string(ctx->budget) + " bytes");
+    return TCV_BA_ERR_MEMORY;
+  }
+  streamed = need_bytes(resident) > ctx->budget;
+  if (held_after(ctx, streamed ? floor : resident) > ctx->budget) TRY(release_buffers(ctx));
+  return TCV_BA_OK;
+}
+
+// The point and camera offsets of a packed index (host array) and the plan of its batch.
+int host_plan(const tcv_ba_control* controls, uint32_t B, const uint32_t* index, uint32_t C,
+              uint32_t P, uint32_t M, bool exact, Plan& plan) {
+  std::vector<uint32_t> ps(index + 2 * (size_t)M, index + 2 * (size_t)M + P + 1);
+  std::vector<uint32_t> cs(index + 2 * (size_t)M + P + 1, index + 2 * (size_t)M + P + C + 2);
+  TRY(build_plan(ps, cs, controls, B, C, P, M, plan));
+  plan_direct(plan, controls[0].flags, controls[0].shared_focal ? 1u : 0u, exact);
+  return TCV_BA_OK;
+}
+
+// Residency of a setup call: decided by setup against the budget, or given by a caller that
+// admitted its staging with the plan already.
+enum Residency { DECIDE = -1, RESIDENT = 0, STREAMED = 1 };
+
 // Allocate, validate the index and build the plan of a batch of B bundles whose packed arrays
 // `scene` / `index` (device pointers) hold the whole batch.
 int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const double* scene,
           const uint32_t* index, hipStream_t st, Problem& q, tcv_ba_report* rep,
-          bool exact = false) {
+          bool exact = false, int residency = DECIDE) {
   const double t0 = now_ms();
   if (B < 1) return TCV_BA_ERR_ARGUMENT;
   uint64_t C64 = 0, P64 = 0, M64 = 0;
@@ -625,8 +797,7 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
   HIP_TRY(hipStreamSynchronize(st));
   Plan plan;
   TRY(build_plan(ps, cs, controls, B, C, P, M, plan));
-  const bool want_direct = (c->flags & TCV_BA_FLAG_SCHUR_CACHE) != 0;
-  plan_direct(plan, want_direct, shared, exact);
+  plan_direct(plan, c->flags, shared, exact);
   const bool direct = !plan.direct_bundles.empty();
   const uint32_t n_unstored = exact ? 0u : (uint32_t)plan.direct_bundles.size();
   const uint32_t n_tiles = (uint32_t)plan.tiles.size() - 1;
@@ -635,44 +806,13 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
   const uint32_t n_groups = (uint32_t)plan.groups.size() / GROUP_WORDS;
   const size_t w_words = (size_t)K * (((size_t)M + 63) / 64) * 64;
   const size_t vec_words = (size_t)m * 5 + (size_t)P * 3;
-  TRY(ensure(ctx->scene, scene_size * 8));
-  TRY(ensure(ctx->trial, trial_size * 8));
-  TRY(ensure(ctx->cam_point, (size_t)M * 4));
-  if (n_unstored < B) {
-    TRY(ensure(ctx->Wp, w_words * ws));
-    TRY(ensure(ctx->Wc, w_words * ws));
-  }
-#ifndef TCV_FUSED
-#define TCV_FUSED 1
-#endif
-  // the fused FP64 product's rows (FP32 blocks keep the two plain sweeps: t stays FP64, so its
-  // round trip would outweigh the halved cross-block reads)
-  // the fused product's rows (the tuner times it against the plain sweeps, which compute the same
-  // bits); FP64 blocks only: over FP32 ones it is slower (70.8 + 27.4 against 42.6 + 40.0 us at
-  // 500 cameras), since the rows stay FP64 while the blocks it saves reading are halved
-  const bool fused = TCV_FUSED && ws == 8 && n_unstored < B;
-  if (fused) TRY(ensure(ctx->tprod, (size_t)M * 64));
-  TRY(ensure(ctx->pblk, (size_t)P * 12 * 8));
-  TRY(ensure(ctx->tile_fail, (size_t)n_tiles * 4));
-  TRY(ensure(ctx->tile_focal, (size_t)n_tiles * 16));
-  TRY(ensure(ctx->task_part, (size_t)n_tasks * ASM_WORDS * 8));
-  TRY(ensure(ctx->camA, (size_t)C * 36 * 8));
-  TRY(ensure(ctx->camInv, (size_t)C * 36 * 8));
-  TRY(ensure(ctx->camRhs, (size_t)C * 6 * 8));
-  TRY(ensure(ctx->camHcf, (size_t)C * 6 * 8));
-  TRY(ensure(ctx->camFocal, (size_t)C * 2 * 8));
-  TRY(ensure(ctx->vec, vec_words * 8));
-  TRY(ensure(ctx->state, (size_t)B * S_WORDS * 8));
-  TRY(ensure(ctx->cost_part, (size_t)n_cost * 3 * 8));
-  TRY(ensure(ctx->error, 8));
-  TRY(ensure(ctx->group_part, (size_t)n_groups * 4 * 8));
+  bool streamed = residency == STREAMED;
+  if (residency == DECIDE) TRY(admit(ctx, c, B, C, P, M, plan, exact, {}, streamed));
+  for (const Need& x : solve_needs(ctx, c, B, C, P, M, plan, exact, streamed))
+    TRY(ensure(*x.buf, x.bytes));
+  const bool fused = !streamed && ws == 8 && n_unstored < B;
   TRY(ensure_host_state(ctx, (size_t)B * S_WORDS));
   if (direct) {
-    TRY(ensure(ctx->camS, (size_t)C * 36 * 8));
-    TRY(ensure(ctx->camG, (size_t)M * 18 * 8));
-    if (shared) TRY(ensure(ctx->pfc, (size_t)P * 3 * 8));
-    TRY(ensure(ctx->dense, plan.dense_words * 8));
-    TRY(ensure(ctx->covis, (size_t)plan.covis_words * 4));
     TRY(upload(ctx->direct_bundles, plan.direct_bundles, st));
     TRY(upload(ctx->direct_cams, plan.direct_cams, st));
     TRY(upload(ctx->small_bundles, plan.small_bundles, st));
@@ -706,6 +846,10 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
   q.huber = c->huber;
   q.tol = c->relative_tolerance;
   q.refine_tol = c->refinement_tolerance > 0 ? c->refinement_tolerance : c->relative_tolerance;
+  q.gauge_tol = c->precision == TCV_BA_MIXED ? std::max(q.tol, q.refine_tol) : q.tol;
+  q.cg_budget = 0;
+  q.cg_max = 0;
+  for (uint32_t b = 0; b < B; b++) q.cg_max = std::max(q.cg_max, controls[b].max_cg);
   q.stop_change = 1e-9;
   q.scene = (double*)ctx->scene.ptr;
   q.trial = (double*)ctx->trial.ptr;
@@ -721,8 +865,9 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
   q.groups = (const uint32_t*)ctx->groups.ptr;
   q.group_part = (double*)ctx->group_part.ptr;
   q.cam_point = (uint32_t*)ctx->cam_point.ptr;
-  q.Wp = n_unstored < B ? ctx->Wp.ptr : nullptr;
-  q.Wc = n_unstored < B ? ctx->Wc.ptr : nullptr;
+  q.Wp = n_unstored < B && !streamed ? ctx->Wp.ptr : nullptr;
+  q.Wc = n_unstored < B && !streamed ? ctx->Wc.ptr : nullptr;
+  q.streamed = streamed ? 1u : 0u;
   q.exact = exact ? 1u : 0u;
   q.n_unstored = n_unstored;
   q.pblk = (double*)ctx->pblk.ptr;
@@ -768,7 +913,7 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
     TRY(read_state(ctx, q, st, nullptr));
     std::vector<char> excluded(B);
     for (uint32_t b = 0; b < B; b++) excluded[b] = word(ctx, b, S_DUPLICATE) != 0;
-    plan_direct(plan, true, shared, exact, excluded);
+    plan_direct(plan, c->flags, shared, exact, excluded);
     TRY(upload(ctx->bundles, plan.bundles, st));
     TRY(upload(ctx->direct_bundles, plan.direct_bundles, st));
     TRY(upload(ctx->direct_cams, plan.direct_cams, st));
@@ -777,7 +922,10 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
     q.n_direct_cams = (uint32_t)plan.direct_cams.size();
     q.n_small = (uint32_t)plan.small_bundles.size();
     q.n_unstored = exact ? 0u : q.n_direct;
-    if (q.n_unstored < B) {
+    // the excluded bundles need cross blocks: stored, or under a working budget relinearised in
+    // every sweep (the same bits)
+    if (q.n_unstored < B && ctx->budget) q.streamed = 1;
+    if (q.n_unstored < B && !q.streamed) {
       TRY(ensure(ctx->Wp, w_words * ws));
       TRY(ensure(ctx->Wc, w_words * ws));
       q.Wp = ctx->Wp.ptr;
@@ -798,7 +946,7 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
     HIP_TRY(hipGetLastError());
   }
   // the shape that the camera sweep's schedule is tuned for
-  ctx->shape = {C, P, M, B, n_tasks, ws, shared, q.n_unstored};
+  ctx->shape = {C, P, M, B, n_tasks, ws, shared, q.n_unstored, q.streamed};
   if (rep) {
     rep->setup_ms = now_ms() - t0;
     rep->point_tiles = n_tiles;
@@ -888,7 +1036,7 @@ int run_pcg(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, cons
   // sweeps only; the tuning iterations are not multiples of kTimingStride); until then the plain
   // sweeps with no cap
   const bool tuned = ctx->tuned_shape == ctx->shape;
-  bool tune = !tuned && ops.unstored_camera_sweep != nullptr && q.n_unstored < q.B &&
+  bool tune = !tuned && !q.streamed && ops.unstored_camera_sweep != nullptr && q.n_unstored < q.B &&
               std::min(first, budget) >= kTuneFirst + kTuneRuns;
   static_assert(kTuneFirst > 0 && kTuneFirst + kTuneRuns <= kTimingStride, "tuning iterations");
   if (tune) TRY(ensure_tune_events(ctx));
@@ -988,7 +1136,7 @@ __global__ void k_stop_pcg(Problem q) {
 // from the start of the loop until its LM loop ended.
 int run_lm(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, hipStream_t st,
            tcv_ba_report* rep, double* finish_ms) {
-  const Ops ops = select_ops(c);
+  const Ops ops = select_ops(c, q.streamed != 0);
   const bool timing = (c->flags & TCV_BA_FLAG_TIMING) != 0 && rep;
   const double t0 = now_ms();
   hipLaunchKernelGGL(k_init_bundle, dim3(q.B), dim3(64), 0, st, q);
@@ -1019,7 +1167,7 @@ int run_lm(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, hipSt
     ops.camera_pass(q, st);
     ops.init_cg(q, st);
     HIP_TRY(hipGetLastError());
-    TRY(solve_linear(ctx, c, q, ops, st, rep, hint, c->max_cg,
+    TRY(solve_linear(ctx, c, q, ops, st, rep, hint, q.cg_max,
                      c->precision == TCV_BA_MIXED ? q.refine_tol : q.tol));
     // The mixed form: replace the FP32-operator residual by the FP64 one and restart PCG from x
     // until the FP64 residual meets the refinement tolerance (each restart has its own budget).
@@ -1043,7 +1191,7 @@ int run_lm(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, hipSt
         HIP_TRY(hipGetLastError());
         TRY(read_state(ctx, q, st, rep));
         if (!any_word(ctx, q, S_PCG_ACTIVE)) break;
-        TRY(run_pcg(ctx, c, q, ops, st, rep, kChunk, c->max_cg));
+        TRY(run_pcg(ctx, c, q, ops, st, rep, kChunk, q.cg_max));
         if (any_word(ctx, q, S_PCG_ACTIVE))
           hipLaunchKernelGGL(k_stop_pcg, dim3(blocks_for(q.B, 64)), dim3(64), 0, st, q);
       }
@@ -1106,6 +1254,19 @@ int run_lm(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, hipSt
   return TCV_BA_OK;
 }
 
+// Free every device buffer of the context (the reference's `drop`; also what a failed call does,
+// as the reference drops a failed job's buffers, device.ts:592-598).
+int release_buffers(tcv_ba_context* ctx) {
+  HIP_TRY(hipSetDevice(ctx->device));
+  if (ctx->stream) HIP_TRY(hipStreamSynchronize(ctx->stream));
+  for (DevBuf* b : ctx->all) {
+    if (b->ptr) HIP_TRY(hipFree(b->ptr));
+    b->ptr = nullptr;
+    b->bytes = 0;
+  }
+  return TCV_BA_OK;
+}
+
 uint64_t held_bytes(const tcv_ba_context* ctx) {
   uint64_t total = 0;
   for (const DevBuf* b : ctx->all) total += b->bytes;
@@ -1116,11 +1277,12 @@ uint64_t held_bytes(const tcv_ba_context* ctx) {
 // then (26 + shared) state words per bundle.
 int solve_device(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B,
                  const double* scene, const uint32_t* index, double* out, tcv_ba_report* rep,
-                 double* finish_ms, hipStream_t st) {
+                 double* finish_ms, hipStream_t st, int residency = DECIDE) {
   const double t0 = now_ms();
   HIP_TRY(hipSetDevice(ctx->device));
   Problem q;
-  TRY(setup(ctx, controls, B, scene, index, st, q, rep));
+  TRY(setup(ctx, controls, B, scene, index, st, q, rep, false, residency));
+  if (rep) rep->streamed = q.streamed;
   TRY(run_lm(ctx, &controls[0], q, st, rep, finish_ms));
   const size_t total = (size_t)q.C * 12 + (size_t)q.P * 3 + (size_t)B * (26 + q.shared);
   hipLaunchKernelGGL(k_pack, dim3(blocks_for(total, 256)), dim3(256), 0, st, q, out);
@@ -1355,45 +1517,60 @@ int tcv_ba_create(int device, tcv_ba_context** out) {
 
 int tcv_ba_release(tcv_ba_context* ctx) {
   if (!ctx) return TCV_BA_ERR_ARGUMENT;
-  HIP_TRY(hipSetDevice(ctx->device));
-  if (ctx->stream) HIP_TRY(hipStreamSynchronize(ctx->stream));
-  for (DevBuf* b : ctx->all) {
-    if (b->ptr) HIP_TRY(hipFree(b->ptr));
-    b->ptr = nullptr;
-    b->bytes = 0;
-  }
-  return TCV_BA_OK;
+  std::lock_guard<std::mutex> guard(ctx->lock);
+  return release_buffers(ctx);
 }
 
 void tcv_ba_destroy(tcv_ba_context* ctx) {
   if (!ctx) return;
-  (void)hipSetDevice(ctx->device);
-  tcv_ba_release(ctx);
-  for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
-  for (hipEvent_t e : ctx->tune_events) (void)hipEventDestroy(e);
-  if (ctx->h_state) (void)hipHostFree(ctx->h_state);
-  if (ctx->h_error) (void)hipHostFree(ctx->h_error);
-  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  {
+    // wait for a call still running on another thread
+    std::lock_guard<std::mutex> guard(ctx->lock);
+    (void)hipSetDevice(ctx->device);
+    release_buffers(ctx);
+    for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ctx->tune_events) (void)hipEventDestroy(e);
+    if (ctx->h_state) (void)hipHostFree(ctx->h_state);
+    if (ctx->h_error) (void)hipHostFree(ctx->h_error);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  }
   delete ctx;
 }
 
-uint64_t tcv_ba_device_bytes(const tcv_ba_context* ctx) { return ctx ? held_bytes(ctx) : 0; }
+uint64_t tcv_ba_device_bytes(const tcv_ba_context* ctx) {
+  if (!ctx) return 0;
+  std::lock_guard<std::mutex> guard(ctx->lock);
+  return held_bytes(ctx);
+}
 
-int tcv_ba_solve_device(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene,
-                        const uint32_t* index, double* out, tcv_ba_report* rep, void* stream) {
+
+static int body_ba_solve_device(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene,
+                                const uint32_t* index, double* out, tcv_ba_report* rep,
+                                void* stream) {
   if (!ctx || !c || !scene || !index || !out) return TCV_BA_ERR_ARGUMENT;
   if (rep) std::memset(rep, 0, sizeof *rep);
   hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
   return solve_device(ctx, c, 1, scene, index, out, rep, nullptr, st);
 }
 
-int tcv_ba_solve(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene,
-                 const uint32_t* index, double* out, tcv_ba_report* rep) {
+static int body_ba_solve(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene,
+                         const uint32_t* index, double* out, tcv_ba_report* rep) {
   if (!ctx || !c || !scene || !index || !out) return TCV_BA_ERR_ARGUMENT;
   if (rep) std::memset(rep, 0, sizeof *rep);
   TRY(check_control(c));
   HIP_TRY(hipSetDevice(ctx->device));
   const size_t ss = tcv_ba_scene_size(c), is = tcv_ba_index_size(c), os = tcv_ba_output_size(c);
+  // the residency plan under the working budget, with the staging of the host arrays, before
+  // anything is allocated
+  int residency = RESIDENT;
+  if (ctx->budget) {
+    Plan plan;
+    bool streamed = false;
+    TRY(host_plan(c, 1, index, c->cameras, c->points, c->observations, false, plan));
+    TRY(admit(ctx, c, 1, c->cameras, c->points, c->observations, plan, false,
+              {{&ctx->probe, ss * 8}, {&ctx->index, is * 4}, {&ctx->out, os * 8}}, streamed));
+    residency = streamed ? STREAMED : RESIDENT;
+  }
   TRY(ensure(ctx->probe, ss * 8));  // staging of the host scene
   TRY(ensure(ctx->index, is * 4));
   TRY(ensure(ctx->out, os * 8));
@@ -1401,16 +1578,16 @@ int tcv_ba_solve(tcv_ba_context* ctx, const tcv_ba_control* c, const double* sce
   HIP_TRY(hipMemcpyAsync(ctx->probe.ptr, scene, ss * 8, hipMemcpyHostToDevice, st));
   HIP_TRY(hipMemcpyAsync(ctx->index.ptr, index, is * 4, hipMemcpyHostToDevice, st));
   TRY(solve_device(ctx, c, 1, (const double*)ctx->probe.ptr, (const uint32_t*)ctx->index.ptr,
-                   (double*)ctx->out.ptr, rep, nullptr, st));
+                   (double*)ctx->out.ptr, rep, nullptr, st, residency));
   HIP_TRY(hipMemcpyAsync(out, ctx->out.ptr, os * 8, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
   if (rep) rep->device_bytes = held_bytes(ctx);
   return TCV_BA_OK;
 }
 
-int tcv_ba_solve_batch(tcv_ba_context* ctx, uint32_t count, const tcv_ba_control* controls,
-                       const double* const* scenes, const uint32_t* const* indices,
-                       double* const* outs, tcv_ba_report* rep, double* finish_ms) {
+static int body_ba_solve_batch(tcv_ba_context* ctx, uint32_t count, const tcv_ba_control* controls,
+                               const double* const* scenes, const uint32_t* const* indices,
+                               double* const* outs, tcv_ba_report* rep, double* finish_ms) {
   if (!ctx || count < 1 || !controls || !scenes || !indices || !outs) return TCV_BA_ERR_ARGUMENT;
   if (rep) std::memset(rep, 0, sizeof *rep);
   for (uint32_t b = 0; b < count; b++) {
@@ -1464,6 +1641,18 @@ int tcv_ba_solve_batch(tcv_ba_context* ctx, uint32_t count, const tcv_ba_control
   }
   const size_t words = 26 + (controls[0].shared_focal ? 1 : 0);
   const size_t out_size = 12 * C + 3 * P + count * words;
+  int residency = RESIDENT;
+  if (ctx->budget) {
+    Plan plan;
+    bool streamed = false;
+    TRY(host_plan(controls, count, index.data(), (uint32_t)C, (uint32_t)P, (uint32_t)M, false,
+                  plan));
+    TRY(admit(ctx, controls, count, (uint32_t)C, (uint32_t)P, (uint32_t)M, plan, false,
+              {{&ctx->probe, scene.size() * 8}, {&ctx->index, index.size() * 4},
+               {&ctx->out, out_size * 8}},
+              streamed));
+    residency = streamed ? STREAMED : RESIDENT;
+  }
   TRY(ensure(ctx->probe, scene.size() * 8));
   TRY(ensure(ctx->index, index.size() * 4));
   TRY(ensure(ctx->out, out_size * 8));
@@ -1471,7 +1660,8 @@ int tcv_ba_solve_batch(tcv_ba_context* ctx, uint32_t count, const tcv_ba_control
   HIP_TRY(hipMemcpyAsync(ctx->probe.ptr, scene.data(), scene.size() * 8, hipMemcpyHostToDevice, st));
   HIP_TRY(hipMemcpyAsync(ctx->index.ptr, index.data(), index.size() * 4, hipMemcpyHostToDevice, st));
   TRY(solve_device(ctx, controls, count, (const double*)ctx->probe.ptr,
-                   (const uint32_t*)ctx->index.ptr, (double*)ctx->out.ptr, rep, finish_ms, st));
+                   (const uint32_t*)ctx->index.ptr, (double*)ctx->out.ptr, rep, finish_ms, st,
+                   residency));
   std::vector<double> all(out_size);
   HIP_TRY(hipMemcpyAsync(all.data(), ctx->out.ptr, out_size * 8, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
@@ -1492,10 +1682,10 @@ int tcv_ba_solve_batch(tcv_ba_context* ctx, uint32_t count, const tcv_ba_control
   return TCV_BA_OK;
 }
 
-int tcv_ba_schur_probe(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene,
-                       const uint32_t* index, double lambda, double log_focal, const double* p,
-                       double* rhs, double* sp, double* z, int32_t* status, uint32_t repeat,
-                       tcv_ba_report* rep) {
+static int body_ba_schur_probe(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene,
+                               const uint32_t* index, double lambda, double log_focal,
+                               const double* p, double* rhs, double* sp, double* z, int32_t* status,
+                               uint32_t repeat, tcv_ba_report* rep) {
   if (!ctx || !c || !scene || !index || !p || !rhs || !sp || !z || !status)
     return TCV_BA_ERR_ARGUMENT;
   if (rep) std::memset(rep, 0, sizeof *rep);
@@ -1574,10 +1764,10 @@ int tcv_ba_schur_probe(tcv_ba_context* ctx, const tcv_ba_control* c, const doubl
   return TCV_BA_OK;
 }
 
-int tcv_ba_polish(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene,
-                  const uint32_t* index, double log_focal, uint32_t lm_iterations,
-                  uint32_t newton_steps, uint32_t max_cg, double tolerance, double* out,
-                  tcv_ba_report* rep) {
+static int body_ba_polish(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene,
+                          const uint32_t* index, double log_focal, uint32_t lm_iterations,
+                          uint32_t newton_steps, uint32_t max_cg, double tolerance, double* out,
+                          tcv_ba_report* rep) {
   if (!ctx || !c || !scene || !index || !out || lm_iterations > 1000 || newton_steps > 100 ||
       max_cg < 1 || max_cg > 100000)
     return TCV_BA_ERR_ARGUMENT;
@@ -1588,6 +1778,8 @@ int tcv_ba_polish(tcv_ba_context* ctx, const tcv_ba_control* c, const double* sc
   Problem q;
   Ops ops;
   TRY(prepare_exact(ctx, c, scene, index, log_focal, tolerance, st, fc, q, ops, rep));
+  q.cg_budget = max_cg;
+  q.cg_max = max_cg;
   const double cost0 = word(ctx, 0, S_COST0), sse0 = word(ctx, 0, S_SSE0);
   const uint32_t update_blocks = blocks_for(q.C > q.P ? q.C : q.P, 256);
   const uint32_t trial_blocks = blocks_for((size_t)q.C * 12 + (size_t)q.P * 3, 256);
@@ -1664,10 +1856,10 @@ int tcv_ba_polish(tcv_ba_context* ctx, const tcv_ba_control* c, const double* sc
   return TCV_BA_OK;
 }
 
-int tcv_ba_backward(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene,
-                    const uint32_t* index, double log_focal, const double* cotangent,
-                    uint32_t max_cg, double tolerance, double* pixel_cotangent, double* adjoint,
-                    tcv_ba_report* rep) {
+static int body_ba_backward(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene,
+                            const uint32_t* index, double log_focal, const double* cotangent,
+                            uint32_t max_cg, double tolerance, double* pixel_cotangent,
+                            double* adjoint, tcv_ba_report* rep) {
   if (!ctx || !c || !scene || !index || !cotangent || !pixel_cotangent || max_cg < 1 ||
       max_cg > 100000)
     return TCV_BA_ERR_ARGUMENT;
@@ -1678,6 +1870,8 @@ int tcv_ba_backward(tcv_ba_context* ctx, const tcv_ba_control* c, const double* 
   Problem q;
   Ops ops;
   TRY(prepare_exact(ctx, c, scene, index, log_focal, tolerance, st, fc, q, ops, rep));
+  q.cg_budget = max_cg;
+  q.cg_max = max_cg;
   const size_t C = q.C, P = q.P, M = q.M, m = q.m;
   for (size_t k = 0; k < 12 * C + 3 * P + q.shared; k++)
     if (!std::isfinite(cotangent[k])) {
@@ -1726,9 +1920,9 @@ int tcv_ba_backward(tcv_ba_context* ctx, const tcv_ba_control* c, const double* 
   return TCV_BA_OK;
 }
 
-int tcv_obs_evaluate(tcv_ba_context* ctx, const tcv_obs_control* c, const double* cameras,
-                     const double* points, const double* pixels, const uint32_t* indices,
-                     const uint32_t* offsets, uint32_t* flags, double* summary) {
+static int body_obs_evaluate(tcv_ba_context* ctx, const tcv_obs_control* c, const double* cameras,
+                             const double* points, const double* pixels, const uint32_t* indices,
+                             const uint32_t* offsets, uint32_t* flags, double* summary) {
   if (!ctx || !c || !flags || !summary) return TCV_BA_ERR_ARGUMENT;
   const uint32_t M = c->observations, T = c->tracks;
   const bool guarded = c->max_relative_depth_std_dev > 0;
@@ -1817,8 +2011,9 @@ int tcv_obs_evaluate(tcv_ba_context* ctx, const tcv_obs_control* c, const double
   return TCV_BA_OK;
 }
 
-int tcv_ba_jacobian(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene,
-                    const uint32_t* index, double log_focal, double* out, tcv_ba_report* rep) {
+static int body_ba_jacobian(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene,
+                            const uint32_t* index, double log_focal, double* out,
+                            tcv_ba_report* rep) {
   if (!ctx || !c || !scene || !index || !out) return TCV_BA_ERR_ARGUMENT;
   if (rep) std::memset(rep, 0, sizeof *rep);
   hipStream_t st = ctx->stream;
@@ -1831,6 +2026,107 @@ int tcv_ba_jacobian(tcv_ba_context* ctx, const tcv_ba_control* c, const double* 
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(out, ctx->grad.ptr, (size_t)q.M * 23 * 8, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
+  return TCV_BA_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// An entry point's call with the context's lock held for its whole duration. A call that fails
+// past its argument checks drops every device buffer of the context, as the reference drops a
+// failed job's buffers (the pending work is waited for first); the error text is the call's.
+template <class F>
+int locked(tcv_ba_context* ctx, F&& body) {
+  if (!ctx) return TCV_BA_ERR_ARGUMENT;
+  std::lock_guard<std::mutex> guard(ctx->lock);
+  const int status = body();
+  if (status != TCV_BA_OK && status != TCV_BA_ERR_ARGUMENT) {
+    const std::string text = g_last_error;
+    (void)hipSetDevice(ctx->device);
+    (void)hipDeviceSynchronize();
+    (void)release_buffers(ctx);
+    g_last_error = text;
+  }
+  return status;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tcv_ba_solve_device(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene,
+                        const uint32_t* index, double* out, tcv_ba_report* rep, void* stream) {
+  return locked(ctx, [&] { return body_ba_solve_device(ctx, c, scene, index, out, rep, stream); });
+}
+
+int tcv_ba_solve(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene,
+                 const uint32_t* index, double* out, tcv_ba_report* rep) {
+  return locked(ctx, [&] { return body_ba_solve(ctx, c, scene, index, out, rep); });
+}
+
+int tcv_ba_solve_batch(tcv_ba_context* ctx, uint32_t count, const tcv_ba_control* controls,
+                       const double* const* scenes, const uint32_t* const* indices,
+                       double* const* outs, tcv_ba_report* report, double* finish_ms) {
+  return locked(ctx, [&] {
+    return body_ba_solve_batch(ctx, count, controls, scenes, indices, outs, report, finish_ms);
+  });
+}
+
+int tcv_ba_schur_probe(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene,
+                       const uint32_t* index, double lambda, double log_focal, const double* p,
+                       double* rhs, double* sp, double* z, int32_t* status, uint32_t repeat,
+                       tcv_ba_report* rep) {
+  return locked(ctx, [&] {
+    return body_ba_schur_probe(ctx, c, scene, index, lambda, log_focal, p, rhs, sp, z, status,
+                               repeat, rep);
+  });
+}
+
+int tcv_ba_polish(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene,
+                  const uint32_t* index, double log_focal, uint32_t lm_iterations,
+                  uint32_t newton_steps, uint32_t max_cg, double tolerance, double* out,
+                  tcv_ba_report* rep) {
+  return locked(ctx, [&] {
+    return body_ba_polish(ctx, c, scene, index, log_focal, lm_iterations, newton_steps, max_cg,
+                          tolerance, out, rep);
+  });
+}
+
+int tcv_ba_backward(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene,
+                    const uint32_t* index, double log_focal, const double* cotangent,
+                    uint32_t max_cg, double tolerance, double* pixel_cotangent, double* adjoint,
+                    tcv_ba_report* rep) {
+  return locked(ctx, [&] {
+    return body_ba_backward(ctx, c, scene, index, log_focal, cotangent, max_cg, tolerance,
+                            pixel_cotangent, adjoint, rep);
+  });
+}
+
+int tcv_obs_evaluate(tcv_ba_context* ctx, const tcv_obs_control* c, const double* cameras,
+                     const double* points, const double* pixels, const uint32_t* indices,
+                     const uint32_t* offsets, uint32_t* flags, double* summary) {
+  return locked(ctx, [&] {
+    return body_obs_evaluate(ctx, c, cameras, points, pixels, indices, offsets, flags, summary);
+  });
+}
+
+int tcv_ba_jacobian(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene,
+                    const uint32_t* index, double log_focal, double* out, tcv_ba_report* rep) {
+  return locked(ctx, [&] { return body_ba_jacobian(ctx, c, scene, index, log_focal, out, rep); });
+}
+
+int tcv_ba_set_working_bytes(tcv_ba_context* ctx, uint64_t bytes) {
+  if (!ctx) return TCV_BA_ERR_ARGUMENT;
+  std::lock_guard<std::mutex> guard(ctx->lock);
+  ctx->budget = bytes;
+  return TCV_BA_OK;
+}
+
+int tcv_ba_fail_reads(tcv_ba_context* ctx, uint32_t n) {
+  if (!ctx) return TCV_BA_ERR_ARGUMENT;
+  std::lock_guard<std::mutex> guard(ctx->lock);
+  ctx->fail_read_in = n;
   return TCV_BA_OK;
 }
 

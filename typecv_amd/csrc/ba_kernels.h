@@ -53,6 +53,7 @@ enum : int {
   S_DIRECT_SOLVES = 39,  // certified direct solves (whole solve)
   S_DUPLICATE = 40,      // set at set-up when a camera of the bundle observes one point twice
   S_DENSE_BAD = 41,      // a pivot of the multi-launch factorisation failed this LM iteration
+  S_STEP_LIMIT = 42,     // word 23 at which the running PCG pass stops (its start + the budget)
   S_WORDS = 48
 };
 
@@ -100,6 +101,8 @@ struct Bundle {
                             // outside the exact systems); its sweeps and update relinearise
   uint32_t large;           // 1: a direct bundle of more than DENSE_MAX_CAMERAS cameras
   uint32_t covis_off;       // first word of the bundle's covisibility bitmap (direct bundles)
+  uint32_t max_cg;          // PCG iterations of one pass (the bundle's maxCgIterations)
+  uint32_t pad_;
   uint64_t dense_off;       // first double of the bundle's dense array
 };
 
@@ -108,6 +111,11 @@ struct Problem {
   uint32_t n_tiles, n_tasks, n_cost_blocks, n_groups;
   double huber, tol;
   double refine_tol;  // mixed form: relative FP64 residual of the outer refinement
+  double gauge_tol;   // relative residual a step's solution must meet to be proposed: the larger of
+                      // tol and (mixed form) refine_tol, the tolerance it was certified at
+  uint32_t cg_budget; // PCG iterations of one pass for every bundle (the exact-Hessian solves of the
+                      // polish and the adjoint); 0: each bundle's own Bundle::max_cg
+  uint32_t cg_max;    // the largest PCG budget of the batch's bundles (the host's launch budget)
   double stop_change; // an accepted step whose relative cost change is below this ends the LM loop
                       // (1e-9, acceptBundleCandidate b:1082-1095)
   double* scene;
@@ -149,6 +157,9 @@ struct Problem {
   // Gauss-Newton blocks only, so it cannot certify such a system)
   uint32_t exact;
   uint32_t n_unstored;  // bundles whose cross blocks are not stored (Bundle::unstored)
+  // the streamed route (the working budget holds no cross blocks): Wp and Wc are null and the
+  // sweeps relinearise every block in registers, in the stored sweeps' expressions
+  uint32_t streamed;
   // the direct route (null / 0 when no bundle takes it)
   uint32_t n_direct, n_direct_cams;
   const uint32_t* direct_bundles;  // n_direct bundle ids
@@ -169,6 +180,11 @@ struct Problem {
   // which caps the workgroups a CU holds at once (0: no reservation)
   uint32_t camera_lds;
 };
+
+// PCG iterations of one pass of bundle g.
+__host__ __device__ inline double pcg_budget(const Problem& q, const Bundle& g) {
+  return q.cg_budget ? q.cg_budget : g.max_cg;
+}
 
 __host__ __device__ inline const uint32_t* obs_camera(const Problem& q) { return q.index; }
 __host__ __device__ inline const uint32_t* obs_point(const Problem& q) { return q.index + q.M; }

@@ -298,6 +298,24 @@ __device__ __forceinline__ void cross_block(const Lin& L, bool shared, double W[
   }
 }
 
+// The cross block of observation i (camera c, point p) as the passes store it, relinearised from
+// the scene: the words of cross_block rounded to the storage type WT and widened back (zero where
+// the point does not project). The streamed route's sweeps use these in place of stored words, in
+// the same expressions, so a streamed run gives the stored run's bits.
+template <class WT, bool SHARED>
+__device__ __forceinline__ void relinearised_w(const Problem& q, const Bundle& g, uint32_t c,
+                                               uint32_t p, uint32_t i, double fscale, double* w) {
+  constexpr int K = SHARED ? 21 : 18;
+  Lin L;
+  double W[21];
+  if (linearize(q, c, p, i, fscale, variable_camera(q, c), directional_point(g, p), L))
+    cross_block(L, SHARED, W);
+  else
+    for (int k = 0; k < 21; k++) W[k] = 0;
+#pragma unroll
+  for (int k = 0; k < K; k++) w[k] = (double)(WT)W[k];
+}
+
 // Word k of the cross block of observation (or camera slot) i. Blocked layout [M/64][K][64]: the 64
 // observations of a wave read K runs of 64 consecutive words that lie next to each other, one
 // contiguous stretch of HBM per wave instead of K distant streams.
@@ -558,7 +576,7 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_pass(Problem q) {
       } else {
         for (int k = 0; k < 21; k++) W[k] = 0;
       }
-      if (stores_w(g))
+      if (stores_w(g) && q.Wp)
         for (uint32_t k = 0; k < K; k++) *wptr<WT, SHARED>(q.Wp, k, i, q.M) = (WT)W[k];
       double* o = st[lane];
       if (ok) {
@@ -719,7 +737,7 @@ __global__ __launch_bounds__(TASK_THREADS) void k_camera_pass(Problem q) {
     } else {
       for (int k = 0; k < 21; k++) W[k] = 0;
     }
-    if (stores_w(g))
+    if (stores_w(g) && q.Wc)
       for (uint32_t k = 0; k < K; k++) *wptr<WT, SHARED>(q.Wc, k, n, q.M) = (WT)W[k];
     if (!ok) continue;
     const double w = L.w;
@@ -908,6 +926,7 @@ __global__ __launch_bounds__(CG_THREADS) void k_init_cg(Problem q) {
   if (tid == 0) {
     for (int j = S_RHO; j < 26; j++) s[j] = 0;
     s[S_DIRECT] = 0;
+    s[S_STEP_LIMIT] = pcg_budget(q, g);
   }
   if (active == 0) return;
   double fail_tiles = 0, fail_cameras = 0;
@@ -1026,8 +1045,9 @@ __device__ __forceinline__ void segment_sum(const double (*st)[3], uint32_t base
 //   CERTIFY: as SCHUR on x, for the bundles with an uncertified direct solution
 enum SweepMode { SWEEP_SCHUR = 0, SWEEP_UPDATE = 1, SWEEP_SOLVE = 2, SWEEP_CERTIFY = 3 };
 
-// One tile t of the point sweep by one wave (st: the wave's LDS slice).
-template <class WT, bool SHARED, int MODE>
+// One tile t of the point sweep by one wave (st: the wave's LDS slice). RECOMPUTE: the streamed
+// route, whose cross blocks are relinearised (relinearised_w) instead of read from Wp.
+template <class WT, bool SHARED, int MODE, bool RECOMPUTE = false>
 __device__ __forceinline__ void sweep_tile(const Problem& q, uint32_t t, uint32_t lane,
                                            double (*st)[3]) {
   // independent loads first: the tile's bounds and bundle, then the bundle's state word
@@ -1045,6 +1065,7 @@ __device__ __forceinline__ void sweep_tile(const Problem& q, uint32_t t, uint32_
   const bool has = my < ep;
   const uint32_t mys = has ? ps[my] : 0, mye = has ? ps[my + 1] : 0;
   const double vf = SHARED ? v[6 * q.C + b] : 0.0;  // the bundle's focal row
+  const double fscale = RECOMPUTE && SHARED ? focal_scale(sb[S_LOG_FOCAL]) : 1.0;
   double acc[3] = {0, 0, 0};
   for (uint32_t base = s; base < e; base += TILE) {
     const uint32_t i = base + lane;
@@ -1052,13 +1073,19 @@ __device__ __forceinline__ void sweep_tile(const Problem& q, uint32_t t, uint32_
       const uint32_t c = obs_camera(q)[i];
       double vc[6];
       for (int k = 0; k < 6; k++) vc[k] = v[c * 6 + k];
+      double wr[RECOMPUTE ? (SHARED ? 21 : 18) : 1];
+      if (RECOMPUTE) relinearised_w<WT, SHARED>(q, q.bundles[b], c, obs_point(q)[i], i, fscale, wr);
       double tj[3];
 #pragma unroll
       for (int j = 0; j < 3; j++) {
         double a = 0;
 #pragma unroll
-        for (int k = 0; k < 6; k++) a += wload(wptr<WT, SHARED>(q.Wp, k * 3 + j, i, q.M)) * vc[k];
-        if (SHARED) a += wload(wptr<WT, SHARED>(q.Wp, 18 + j, i, q.M)) * vf;
+        for (int k = 0; k < 6; k++)
+          a += (RECOMPUTE ? wr[k * 3 + j] : wload(wptr<WT, SHARED>(q.Wp, k * 3 + j, i, q.M))) *
+               vc[k];
+        if (SHARED)
+          a += (RECOMPUTE ? wr[(18 + j) % (SHARED ? 21 : 18)]
+                          : wload(wptr<WT, SHARED>(q.Wp, 18 + j, i, q.M))) * vf;
         tj[j] = a;
       }
       st[lane][0] = tj[0];
@@ -1210,14 +1237,14 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_sweep_fused(Problem 
   sweep_tile_fused<WT, SHARED>(q, t, lane, st_all[wave], ys_all[wave], own_all[wave]);
 }
 
-template <class WT, bool SHARED, int MODE>
+template <class WT, bool SHARED, int MODE, bool RECOMPUTE = false>
 __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_sweep(Problem q) {
   __shared__ double st_all[TILE_WAVES][TILE][3];
   // the wave's index is uniform across its lanes: keep the tile's loads and branches scalar
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / TILE), lane = threadIdx.x % TILE;
   const uint32_t t = point_tile(wave);
   if (t >= q.n_tiles) return;
-  sweep_tile<WT, SHARED, MODE>(q, t, lane, st_all[wave]);
+  sweep_tile<WT, SHARED, MODE, RECOMPUTE>(q, t, lane, st_all[wave]);
 }
 
 // The camera sweep: one workgroup per camera-list task streams the camera-major cross blocks and
@@ -1228,9 +1255,12 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_sweep(Problem q) {
 // 42.4 us per sweep at 500 cameras; five as a lower bound only, the compiler takes 96 registers
 // and the sweep 51 us). With FP64 ones the compiler's own choice (four waves, 100 registers) is
 // the faster (62 against 82 us when held to five).
-template <class WT, bool SHARED, bool CERTIFY = false>
+// RECOMPUTE: the streamed route (each slot's cross block relinearised, relinearised_w, in place of
+// the Wc words; the same slot order and sums).
+template <class WT, bool SHARED, bool CERTIFY = false, bool RECOMPUTE = false>
 __global__ __launch_bounds__(TASK_THREADS)
-__attribute__((amdgpu_waves_per_eu(sizeof(WT) == 4 ? 5 : 1, sizeof(WT) == 4 ? 5 : 8)))
+__attribute__((amdgpu_waves_per_eu(sizeof(WT) == 4 && !RECOMPUTE ? 5 : 1,
+                                   sizeof(WT) == 4 && !RECOMPUTE ? 5 : 8)))
 void k_camera_sweep(Problem q) {
   constexpr int NV = SHARED ? 7 : 6;
   __shared__ double red[TASK_THREADS / 64][NV];
@@ -1266,14 +1296,28 @@ void k_camera_sweep(Problem q) {
 #pragma unroll
       for (int c = 0; c < 3; c++) yv[k][c] = y[(size_t)pt[k] * 3 + c] * f;
     }
+    if (RECOMPUTE) {
+      const uint32_t c = q.tasks[4 * t];
+      const Bundle& g = q.bundles[b];
+      const double fscale = SHARED ? focal_scale(bstate(q, b)[S_LOG_FOCAL]) : 1.0;
+      for (int k = 0; k < PER; k++) {
+        const uint32_t n = slot[k];
+        double wr[SHARED ? 21 : 18];
+        relinearised_w<WT, SHARED>(q, g, c, pt[k], camera_obs(q)[n], fscale, wr);
 #pragma unroll
-    for (int k = 0; k < PER; k++) {
-      const uint32_t n = slot[k];
+        for (int j = 0; j < NV; j++)
+          acc[j] += wr[j * 3] * yv[k][0] + wr[j * 3 + 1] * yv[k][1] + wr[j * 3 + 2] * yv[k][2];
+      }
+    } else {
 #pragma unroll
-      for (int j = 0; j < NV; j++)
-        acc[j] += wload(wptr<WT, SHARED>(q.Wc, j * 3, n, q.M)) * yv[k][0] +
-                  wload(wptr<WT, SHARED>(q.Wc, j * 3 + 1, n, q.M)) * yv[k][1] +
-                  wload(wptr<WT, SHARED>(q.Wc, j * 3 + 2, n, q.M)) * yv[k][2];
+      for (int k = 0; k < PER; k++) {
+        const uint32_t n = slot[k];
+#pragma unroll
+        for (int j = 0; j < NV; j++)
+          acc[j] += wload(wptr<WT, SHARED>(q.Wc, j * 3, n, q.M)) * yv[k][0] +
+                    wload(wptr<WT, SHARED>(q.Wc, j * 3 + 1, n, q.M)) * yv[k][1] +
+                    wload(wptr<WT, SHARED>(q.Wc, j * 3 + 2, n, q.M)) * yv[k][2];
+      }
     }
   }
 #pragma unroll
@@ -1582,6 +1626,7 @@ __global__ __launch_bounds__(ADV_THREADS) void k_cg_direction(Problem q) {
   double* p = vp(q);
   const double* z = vz(q);
   double active = s[S_PCG_ACTIVE], rho_old = s[S_RHO], r0 = s[S_R0];
+  double steps = s[S_STEPS], limit = s[S_STEP_LIMIT];
   double res, rho;
   group_loads(q, G, 2, res, rho);
   double pr = p[row], zr = z[row];
@@ -1589,6 +1634,8 @@ __global__ __launch_bounds__(ADV_THREADS) void k_cg_direction(Problem q) {
   settle(active);
   settle(rho_old);
   settle(r0);
+  settle(steps);
+  settle(limit);
   settle(res);
   settle(rho);
   settle(pr);
@@ -1599,10 +1646,13 @@ __global__ __launch_bounds__(ADV_THREADS) void k_cg_direction(Problem q) {
   block_sum2(res, rho, lds);
   const bool converged = res <= q.tol * q.tol * r0;
   const bool broke = !converged && !(rho > 0);
+  // the pass's budget: the bundle stops unconverged, as the reference's host loop stops launching
+  // at maxCgIterations (the gauge then rejects the step)
+  const bool spent = !converged && !broke && steps + 1 >= limit;
   const double beta = rho / rho_old;
   if (G.first && threadIdx.x == 0) {
     s[S_CG_STEPS] += 1;
-    s[S_STEPS] += 1;
+    s[S_STEPS] = steps + 1;
     s[S_R] = res;
     if (converged) {
       s[S_PCG_ACTIVE] = 0;
@@ -1612,6 +1662,7 @@ __global__ __launch_bounds__(ADV_THREADS) void k_cg_direction(Problem q) {
       s[S_PCG_FAILED] = 1;
     } else {
       s[S_RHO_NEXT] = rho;
+      if (spent) s[S_PCG_ACTIVE] = 0;
     }
   }
   if (converged || broke) return;
@@ -1827,6 +1878,7 @@ __global__ __launch_bounds__(CG_THREADS) void k_refine(Problem q) {
     s[S_RHO_NEXT] = rho;
     s[S_CONVERGED] = 0;
     s[S_REFINEMENTS] += 1;
+    s[S_STEP_LIMIT] = s[S_STEPS] + pcg_budget(q, g);  // each restart: the budget again
     if (rho > 0)
       s[S_PCG_ACTIVE] = 1;
     else
@@ -1872,7 +1924,7 @@ __global__ __launch_bounds__(256) void k_gauge(Problem q) {
   __syncthreads();
   if (tid == 0) s[S_TRIAL_VALID] = 0;
   if (active == 0 || failed != 0) return;
-  if (r0 > 0 && r > q.tol * q.tol * r0) {
+  if (r0 > 0 && r > q.gauge_tol * q.gauge_tol * r0) {
     if (tid == 0) s[S_PCG_FAILED] = 1;
     return;
   }

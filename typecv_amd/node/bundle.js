@@ -16,10 +16,13 @@ const BUNDLE_STATE = Object.freeze({
 });
 
 class HipDevice {
-  constructor(index = 0) {
+  /* `workingBytes`: ShadeDevice.open's option (device.ts:357-360), the device memory one bundle
+   * adjustment may hold; a larger problem streams its cross blocks or is refused. 0: no bound. */
+  constructor(index = 0, { workingBytes = 0 } = {}) {
     this.native = addon.open(index);
     this.tail = Promise.resolve();
     this.closed = false;
+    if (workingBytes) addon.setWorkingBytes(this.native, workingBytes);
   }
   /* Jobs run one after another on this device, in the order they were enqueued. */
   enqueue(job) {
@@ -41,15 +44,17 @@ class HipDevice {
 }
 
 /* tcv_ba_control as the addon's Float64Array(16). `schurCache` sets TCV_BA_FLAG_SCHUR_CACHE (4),
- * the direct route of the reference's option of that name. */
+ * the reference's option (honoured inside its guard: shared focal, at most 16 cameras,
+ * M (6C + 1) <= 262,144); `directSolve` sets TCV_BA_FLAG_DIRECT (8), the direct route for every
+ * bundle of at most 2,048 cameras. */
 function makeControl({ cameras, points, observations, directionalPoints = 0, anchor = 0,
   scaleView = 1, normalizeScale = true, sharedFocal = false, iterations = 20, maxCg = 24,
   precision = 'fp64', flags = 0, huber = 2, tolerance = 1e-6, stallTolerance = 1e-9,
-  refinementTolerance = 0, schurCache = false }) {
+  refinementTolerance = 0, schurCache = false, directSolve = false }) {
+  const bits = flags | (schurCache ? 4 : 0) | (directSolve ? 8 : 0);
   return Float64Array.from([cameras, points, directionalPoints, observations, anchor, scaleView,
     normalizeScale ? 1 : 0, sharedFocal ? 1 : 0, iterations, maxCg,
-    precision === 'mixed' ? 1 : 0, schurCache ? flags | 4 : flags, huber, tolerance,
-    stallTolerance, refinementTolerance]);
+    precision === 'mixed' ? 1 : 0, bits, huber, tolerance, stallTolerance, refinementTolerance]);
 }
 
 /* A packBundleResult array and its diagnostics, with checkBundleOutput's errors
@@ -99,6 +104,18 @@ function backwardPacked(device, control, scene, index, logFocal, cotangent,
     cotangent, maxCg, tolerance));
 }
 
+/* The Newton polish of a solve to a stationary point (tcv_ba_polish): the solve's output layout. */
+function polishPacked(device, control, scene, index, logFocal, lmIterations = 30, newtonSteps = 3,
+  maxCg = 4000, tolerance = 1e-12) {
+  return device.enqueue(() => addon.polish(device.native, control, scene, index, logFocal,
+    lmIterations, newtonSteps, maxCg, tolerance));
+}
+
+/* The Jacobian kernels' output (tcv_ba_jacobian): 23 doubles per observation. */
+function jacobianPacked(device, control, scene, index, logFocal = 0) {
+  return device.enqueue(() => addon.jacobian(device.native, control, scene, index, logFocal));
+}
+
 /* The fixed-scene observation evaluation (evaluateObservationsGpu's device part) of a request
  * packed as observationRequest packs it (src/geometry/gpu/observations.ts:132-197): the flags
  * (observations [+ tracks]) and the summary [error sum, inlier count]. */
@@ -112,6 +129,6 @@ offsets = null) {
 }
 
 module.exports = {
-  HipDevice, makeControl, solvePacked, solveBatchPacked, backwardPacked, evaluatePacked,
-  BUNDLE_STATE, addon,
+  HipDevice, makeControl, solvePacked, solveBatchPacked, backwardPacked, polishPacked,
+  jacobianPacked, evaluatePacked, BUNDLE_STATE, addon,
 };

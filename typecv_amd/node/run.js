@@ -10,10 +10,17 @@
  * evaluates a packed observation request (control: views, points, observations, tracks,
  * maxErrorPx, maxRelativeDepthStdDev, pixelNoiseStdDevPx) and writes <out prefix>.flags.u32 and
  * <out prefix>.summary.f64.
+ *   node run.js --concurrent <control.f64> <scene.f64> <index.u32> <out prefix>
+ * issues two solves on one context at once through the addon itself (no FIFO in between), then a
+ * third after both, and writes <out prefix>.0.f64, .1.f64 and .seq.f64;
+ *   node run.js --polish <control.f64> <scene.f64> <index.u32> <logFocal> <out.f64>
+ *   node run.js --jacobian <control.f64> <scene.f64> <index.u32> <logFocal> <out.f64>
+ * run tcv_ba_polish (30 damped steps, 3 Newton steps, PCG <= 4000 to 1e-12) or tcv_ba_jacobian.
  * `node run.js --exports` prints the addon's exported names (no device is touched).
  */
 const fs = require('fs');
-const { HipDevice, solvePacked, solveBatchPacked, evaluatePacked, addon } = require('./bundle.js');
+const { HipDevice, solvePacked, solveBatchPacked, polishPacked, jacobianPacked, evaluatePacked,
+  addon } = require('./bundle.js');
 
 function read(file, Type) {
   const bytes = fs.readFileSync(file);
@@ -40,6 +47,39 @@ async function main(argv) {
       fs.writeFileSync(`${prefix}.flags.u32`, Buffer.from(r.flags.buffer));
       fs.writeFileSync(`${prefix}.summary.f64`, Buffer.from(r.summary.buffer));
       console.log(JSON.stringify({ inliers: r.summary[1] }));
+    } finally {
+      device.dispose();
+    }
+    return 0;
+  }
+  if (argv[0] === '--concurrent') {
+    const [, controlFile, sceneFile, indexFile, prefix] = argv;
+    const control = read(controlFile, Float64Array);
+    const scene = read(sceneFile, Float64Array), index = read(indexFile, Uint32Array);
+    const device = new HipDevice(0);
+    try {
+      const both = await Promise.all([addon.solve(device.native, control, scene, index),
+        addon.solve(device.native, control, scene, index)]);
+      const after = await addon.solve(device.native, control, scene, index);
+      both.forEach((v, k) => fs.writeFileSync(`${prefix}.${k}.f64`, Buffer.from(v.buffer)));
+      fs.writeFileSync(`${prefix}.seq.f64`, Buffer.from(after.buffer));
+      console.log(JSON.stringify({ solves: 3 }));
+    } finally {
+      device.dispose();
+    }
+    return 0;
+  }
+  if (argv[0] === '--polish' || argv[0] === '--jacobian') {
+    const [mode, controlFile, sceneFile, indexFile, logFocal, outFile] = argv;
+    const control = read(controlFile, Float64Array);
+    const scene = read(sceneFile, Float64Array), index = read(indexFile, Uint32Array);
+    const device = new HipDevice(0);
+    try {
+      const values = mode === '--polish'
+        ? await polishPacked(device, control, scene, index, Number(logFocal), 30, 3, 4000, 1e-12)
+        : await jacobianPacked(device, control, scene, index, Number(logFocal));
+      fs.writeFileSync(outFile, Buffer.from(values.buffer));
+      console.log(JSON.stringify({ length: values.length }));
     } finally {
       device.dispose();
     }

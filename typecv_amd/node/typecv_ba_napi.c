@@ -12,8 +12,14 @@
  *   outputSize(control)                            -> number of doubles of solve's result
  *   deviceBytes(context), release(context)         -> tcv_ba_device_bytes, tcv_ba_release
  *   solve(context, control, scene, index)          -> Promise<Float64Array>
+ *   setWorkingBytes(context, bytes)                -> tcv_ba_set_working_bytes
  *   backward(context, control, scene, index, logFocal, cotangent, maxCg, tolerance)
  *                                                  -> Promise<Float64Array(2M)>
+ *   polish(context, control, scene, index, logFocal, lmIterations, newtonSteps, maxCg, tolerance)
+ *                                                  -> Promise<Float64Array> (tcv_ba_polish: the
+ *                                                     solve's output layout)
+ *   jacobian(context, control, scene, index, logFocal)
+ *                                                  -> Promise<Float64Array(23M)> (tcv_ba_jacobian)
  *   solveBatch(context, controls[], scenes[], indices[])
  *                                                  -> Promise<Float64Array[]> (tcv_ba_solve_batch)
  *   evaluate(context, control, cameras, points, pixels, indices, offsets|null)
@@ -138,6 +144,22 @@ static napi_value device_bytes(napi_env env, napi_callback_info info) {
   napi_value out;
   CHECK(env, napi_create_double(env, (double)tcv_ba_device_bytes(ctx), &out));
   return out;
+}
+
+static napi_value set_working_bytes(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  tcv_ba_context* ctx = argc > 1 ? context_of(env, argv[0]) : NULL;
+  if (!ctx) return NULL;
+  double bytes = 0;
+  CHECK(env, napi_get_value_double(env, argv[1], &bytes));
+  if (!(bytes >= 0 && bytes <= 9007199254740991.0) || bytes != (double)(uint64_t)bytes) {
+    napi_throw_range_error(env, NULL, "workingBytes must be a non-negative integer");
+    return NULL;
+  }
+  tcv_ba_set_working_bytes(ctx, (uint64_t)bytes);
+  return NULL;
 }
 
 static napi_value release(napi_env env, napi_callback_info info) {
@@ -291,7 +313,7 @@ static napi_value evaluate(napi_env env, napi_callback_info info) {
 
 /* ---------------------------------------------------------------- asynchronous jobs */
 
-enum { JOB_SOLVE = 0, JOB_BACKWARD = 1 };
+enum { JOB_SOLVE = 0, JOB_BACKWARD = 1, JOB_POLISH = 2, JOB_JACOBIAN = 3 };
 
 typedef struct {
   int kind;
@@ -305,7 +327,7 @@ typedef struct {
   const uint32_t* index;
   const double* cotangent;
   double log_focal, tolerance;
-  uint32_t max_cg;
+  uint32_t max_cg, lm_iterations, newton_steps;
   double* out;
   size_t out_len;
   tcv_ba_report report;
@@ -318,9 +340,16 @@ static void execute(napi_env env, void* data) {
   Job* j = data;
   if (j->kind == JOB_SOLVE)
     j->status = tcv_ba_solve(j->ctx, &j->control, j->scene, j->index, j->out, &j->report);
-  else
+  else if (j->kind == JOB_BACKWARD)
     j->status = tcv_ba_backward(j->ctx, &j->control, j->scene, j->index, j->log_focal,
                                 j->cotangent, j->max_cg, j->tolerance, j->out, NULL, &j->report);
+  else if (j->kind == JOB_POLISH)
+    j->status = tcv_ba_polish(j->ctx, &j->control, j->scene, j->index, j->log_focal,
+                              j->lm_iterations, j->newton_steps, j->max_cg, j->tolerance, j->out,
+                              &j->report);
+  else
+    j->status = tcv_ba_jacobian(j->ctx, &j->control, j->scene, j->index, j->log_focal, j->out,
+                                &j->report);
   if (j->status != TCV_BA_OK) snprintf(j->detail, sizeof j->detail, "%s", tcv_ba_last_error());
 }
 
@@ -437,6 +466,54 @@ static napi_value backward(napi_env env, napi_callback_info info) {
   j->out_len = 2 * (size_t)j->control.observations;
   j->out = malloc(j->out_len * sizeof(double));
   return queue(env, j, "typecv_ba.backward");
+}
+
+/* A JavaScript number that is an integer in [lo, hi], or 0 with a pending RangeError. */
+static int integer_of(napi_env env, napi_value v, double lo, double hi, uint32_t* out) {
+  double x = 0;
+  if (napi_get_value_double(env, v, &x) != napi_ok || !(x >= lo && x <= hi) ||
+      x != (double)(uint32_t)x) {
+    napi_throw_range_error(env, NULL, "integer argument out of range");
+    return 0;
+  }
+  *out = (uint32_t)x;
+  return 1;
+}
+
+static napi_value polish(napi_env env, napi_callback_info info) {
+  size_t argc = 9;
+  napi_value argv[9];
+  CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  Job* j = job_of(env, argv, argc, 9, JOB_POLISH);
+  if (!j) return NULL;
+  if (napi_get_value_double(env, argv[4], &j->log_focal) != napi_ok ||
+      !integer_of(env, argv[5], 0, 1000, &j->lm_iterations) ||
+      !integer_of(env, argv[6], 0, 100, &j->newton_steps) ||
+      !integer_of(env, argv[7], 1, 100000, &j->max_cg) ||
+      napi_get_value_double(env, argv[8], &j->tolerance) != napi_ok) {
+    for (int k = 0; k < j->nrefs; k++) napi_delete_reference(env, j->refs[k]);
+    free(j);
+    return NULL;
+  }
+  j->out_len = tcv_ba_output_size(&j->control);
+  j->out = malloc(j->out_len * sizeof(double));
+  return queue(env, j, "typecv_ba.polish");
+}
+
+static napi_value jacobian(napi_env env, napi_callback_info info) {
+  size_t argc = 5;
+  napi_value argv[5];
+  CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  Job* j = job_of(env, argv, argc, 5, JOB_JACOBIAN);
+  if (!j) return NULL;
+  if (napi_get_value_double(env, argv[4], &j->log_focal) != napi_ok) {
+    for (int k = 0; k < j->nrefs; k++) napi_delete_reference(env, j->refs[k]);
+    free(j);
+    return NULL;
+  }
+  j->out_len = 23 * (size_t)j->control.observations;
+  j->out = malloc((j->out_len ? j->out_len : 1) * sizeof(double));
+  return queue(env, j, "typecv_ba.jacobian");
 }
 
 /* ---------------------------------------------------------------- batches */
@@ -603,6 +680,9 @@ static napi_value init(napi_env env, napi_value exports) {
       {"outputSize", NULL, output_size, NULL, NULL, NULL, napi_enumerable, NULL},
       {"deviceBytes", NULL, device_bytes, NULL, NULL, NULL, napi_enumerable, NULL},
       {"release", NULL, release, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"setWorkingBytes", NULL, set_working_bytes, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"polish", NULL, polish, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"jacobian", NULL, jacobian, NULL, NULL, NULL, napi_enumerable, NULL},
       {"solve", NULL, solve, NULL, NULL, NULL, napi_enumerable, NULL},
       {"backward", NULL, backward, NULL, NULL, NULL, napi_enumerable, NULL},
       {"solveBatch", NULL, solve_batch, NULL, NULL, NULL, napi_enumerable, NULL},
