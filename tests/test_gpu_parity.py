@@ -42,7 +42,7 @@ def cases():
     yield "joint_focal", S.joint_focal_fixture()["input"], {
         "iterations": 40, "refineSharedFocal": True, "maxCgIterations": 256,
         "cgRelativeTolerance": 1e-10}
-    yield "ring_320", S.ring_bundle_fixture(views=64, points=960), {"iterations": 8}
+    yield "ring_64", S.ring_bundle_fixture(views=64, points=960), {"iterations": 8}
 
 
 CASES = list(cases())

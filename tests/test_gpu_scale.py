@@ -4,7 +4,8 @@
   matrix-free oracle at the parity bars: final cost within 1e-9 relative, every coordinate x within
   1e-7·max(1, |x|), the accept/reject/failure/input-error words equal.
 * The 500-camera trajectory scene (200,000 points, 1.89 M observations, shared focal, Huber 1 px,
-  PCG <= 256): its first 3 LM iterations against the oracle at the same bars.
+  PCG <= 256): its first 3 LM iterations against the oracle at the same bars with PCG solved to
+  1e-10, and at the reference's default 1e-6 with the coordinate bar widened to the PCG truncation.
 * The 2,000-camera scene (1 M points, 9.7 M observations, 30 LM iterations): the mixed form with
   FP64 refinement to 1e-10 against the FP64 form on the GPU: final cost within 1e-6 relative.
 * All 256 local bundles (cameras 5-40, points 500-20,000) in one mixed batch: a sample of bundles
@@ -24,8 +25,9 @@ pytestmark = pytest.mark.gpu
 SHARED_CG = lambda C: min(256, max(32, 12 * C + 1))  # noqa: E731
 
 
-def compare(got, expected, base, label):
-    """The parity bars; returns the largest absolute and relative coordinate differences."""
+def compare(got, expected, base, label, coordinate_bar=1e-7):
+    """The parity bars (coordinates relative to max(1, |x|)); returns the largest absolute and
+    relative coordinate differences."""
     cost = expected[base + 6]
     diff = np.abs(got[:base] - expected[:base])
     rel = diff / np.maximum(1.0, np.abs(expected[:base]))
@@ -34,7 +36,7 @@ def compare(got, expected, base, label):
           f"{diff.max():.2e}, max rel {rel.max():.2e}; words 10/11/17 "
           f"{got[base + 10]:.0f}/{got[base + 11]:.0f}/{got[base + 17]:.0f}", flush=True)
     assert abs(got[base + 6] - cost) <= 1e-9 * max(1.0, abs(cost))
-    assert rel.max() <= 1e-7
+    assert rel.max() <= coordinate_bar
     for word in (10, 11, 14, 17):
         assert got[base + word] == expected[base + word], word
     return diff.max(), rel.max()
@@ -49,19 +51,32 @@ def test_ring_10_cameras_matches_the_oracle(device):
     compare(got, expected, ring.C * 12 + ring.P * 3, f"C={ring.C} M={ring.M}")
 
 
-def test_trajectory_500_cameras_first_lm_iterations_match_the_oracle(device):
-    print("\n[scale] 500-camera trajectory, 3 LM iterations", flush=True)
+@pytest.mark.parametrize("tolerance", [1e-10, 1e-6])
+def test_trajectory_500_cameras_first_lm_iterations_match_the_oracle(device, tolerance):
+    """At cgRelativeTolerance 1e-10 every step is solved to rounding: the full bars, and every
+    coordinate within 1e-7 absolute as well. At the reference's default 1e-6 each PCG stops where
+    its recurrence residual first passes the tolerance, and the two summation orders reach that
+    point an iteration apart now and then (241 against 242 iterations over the 3 LM iterations):
+    the step then differs by the PCG truncation itself, about 1e-6 of it, and coordinates agree to
+    5e-5·max(1, |x|) (measured 4.9e-6) while the cost and the words keep their bars."""
+    print(f"\n[scale] 500-camera trajectory, 3 LM iterations, PCG tolerance {tolerance}",
+          flush=True)
     packed, _ = S.trajectory_scene(cameras=500, points=200_000, seed=2)
     control = B.make_control(packed.C, packed.P, packed.M, shared_focal=True, huber=1.0,
-                             iterations=3, max_cg=SHARED_CG(packed.C))
+                             iterations=3, max_cg=SHARED_CG(packed.C), tolerance=tolerance)
     got, report = B.solve_packed(device, control, packed)
     expected, stats = O.solve(control, packed)
     base = packed.C * 12 + packed.P * 3
     print(f"  PCG iterations: GPU {report['cg_iterations']}, oracle {stats['cg_iterations']:.0f}",
           flush=True)
     assert got[base + 10] + got[base + 11] == 3
-    compare(got, expected, base, f"C={packed.C} P={packed.P} M={packed.M}")
-    assert abs(got[base + 26] - expected[base + 26]) <= 1e-9
+    largest, _ = compare(got, expected, base, f"C={packed.C} P={packed.P} M={packed.M}",
+                         coordinate_bar=1e-7 if tolerance < 1e-9 else 5e-5)
+    if tolerance < 1e-9:
+        assert largest <= 1e-7
+    # the log focal scale: 1e-9 when solved to rounding; within the truncation otherwise
+    # (measured 4.0e-9 at 1e-6)
+    assert abs(got[base + 26] - expected[base + 26]) <= (1e-9 if tolerance < 1e-9 else 1e-7)
 
 
 def test_trajectory_2000_cameras_mixed_matches_fp64(device):

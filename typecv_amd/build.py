@@ -36,6 +36,10 @@ def compile_command(output: Path = LIBRARY, extra: list[str] | None = None) -> l
         "-fPIC",
         "-shared",
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+        # multiply-adds are fused within an expression only, never across statements: a value's
+        # bits then depend on its source expression alone, not on what else a kernel computes, so
+        # a cross block relinearised in one kernel equals the one another kernel stored
+        "-ffp-contract=on",
         "-Wall",
         "-Wno-unused-function",
         "-I", str(ROOT / "include"),
