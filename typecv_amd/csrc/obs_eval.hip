@@ -236,4 +236,5 @@ __global__ __launch_bounds__(64) void k_obs_summary(ObsProblem o) {
 
 }  // namespace tcvba
 
-#pragma clang fp contract(fast)
+// back to the engine's contraction (-ffp-contract=on: within an expression only)
+#pragma clang fp contract(on)
