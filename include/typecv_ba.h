@@ -268,6 +268,26 @@ int tcv_obs_evaluate(tcv_ba_context* context, const tcv_obs_control* control,
                      double* summary);
 
 /*
+ * The fixed-scene observation evaluation of tcv_obs_evaluate from DEVICE arrays of the current HIP
+ * device, allocating nothing: no hipMalloc, hipFree or allocating library call. No counterpart in
+ * the reference; for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). The arguments mean what they mean for
+ * tcv_obs_evaluate, but every array is a device array, `flags` and `summary` included, and work
+ * runs on `stream` (a hipStream_t, NULL for the default stream); the call returns when flags and
+ * summary are complete, with the same bits as tcv_obs_evaluate. The kernels' scratch (per-
+ * observation errors, per-64-observation partials, an index-check word) is one fixed device array
+ * of the library, 16.5 MiB on each GPU it runs on, held until the process ends: at most
+ * TCV_OBS_NOALLOC_MAX_OBSERVATIONS observations (more: TCV_BA_ERR_ARGUMENT). Calls that overlap on
+ * one GPU share that array and race: serialise them. Indices and track offsets are checked on the
+ * device before any kernel reads through them (TCV_BA_ERR_INDEX).
+ */
+#define TCV_OBS_NOALLOC_MAX_OBSERVATIONS 2097152u
+int tcv_obs_evaluate_noalloc(const tcv_obs_control* control, const double* cameras,
+                             const double* points, const double* pixels, const uint32_t* indices,
+                             const uint32_t* offsets, uint32_t* flags, double* summary,
+                             void* stream);
+
+/*
  * Differentiation of the solve. The reference operator has no derivative entries: these extend
  * `bundleAdjustGpu` (src/sfm/bundle/gpu/bundle.ts:803-839) with the pieces of a backward pass. All
  * take a host scene/index in the packBundleScene layouts (reference bundle-host.ts:121-218) holding

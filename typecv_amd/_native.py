@@ -47,6 +47,7 @@ EXPORTS = (
     "tcv_ba_polish",
     "tcv_ba_backward",
     "tcv_obs_evaluate",
+    "tcv_obs_evaluate_noalloc",
 )
 
 
@@ -184,6 +185,9 @@ def library() -> ctypes.CDLL:
     lib.tcv_ba_backward.restype = ctypes.c_int
     lib.tcv_obs_evaluate.argtypes = [vp, P(ObsControl), vp, vp, vp, vp, vp, vp, vp]
     lib.tcv_obs_evaluate.restype = ctypes.c_int
+    lib.tcv_obs_evaluate_noalloc.argtypes = [P(ObsControl), vp, vp, vp, vp, vp, vp, vp, vp]
+    lib.tcv_obs_evaluate_noalloc.restype = ctypes.c_int
+    lib.TCV_OBS_NOALLOC_MAX_OBSERVATIONS = 2097152
     _lib = lib
     return lib
 

@@ -2011,6 +2011,107 @@ static int body_obs_evaluate(tcv_ba_context* ctx, const tcv_obs_control* c, cons
   return TCV_BA_OK;
 }
 
+// The scratch of tcv_obs_evaluate_noalloc: the per-observation errors, the per-64-observation
+// partials and the index-check word of at most kNoallocObservations observations, in one device
+// array of the library (one copy per GPU, held until the process ends; 16.5 MiB). Host code uses
+// it only through the address hipGetSymbolAddress returns for the current device.
+constexpr uint32_t kNoallocObservations = TCV_OBS_NOALLOC_MAX_OBSERVATIONS;
+constexpr uint32_t kNoallocBlocks = kNoallocObservations / 64;
+constexpr size_t kNoallocWords = (size_t)kNoallocObservations + 2 * (size_t)kNoallocBlocks + 32;
+static __device__ __align__(256) double scratch[kNoallocWords];
+
+namespace {
+std::mutex g_scratch_lock;
+double* g_scratch_ptr[64] = {};  // the address of `scratch` on each device, once looked up
+
+// The address of `scratch` on the current device (looked up on the entry's first call there).
+int scratch_address(double** out) {
+  int device = 0;
+  HIP_TRY(hipGetDevice(&device));
+  if (device < 0 || device >= 64) {
+    set_error("device index outside the scratch table");
+    return TCV_BA_ERR_NO_DEVICE;
+  }
+  std::lock_guard<std::mutex> guard(g_scratch_lock);
+  if (!g_scratch_ptr[device]) {
+    void* ptr = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device] = (double*)ptr;
+  }
+  *out = g_scratch_ptr[device];
+  return TCV_BA_OK;
+}
+}  // namespace
+
+static int body_obs_evaluate_noalloc(const tcv_obs_control* c, const double* cameras,
+                                     const double* points, const double* pixels,
+                                     const uint32_t* indices, const uint32_t* offsets,
+                                     uint32_t* flags, double* summary, void* stream) {
+  if (!c || !flags || !summary) return TCV_BA_ERR_ARGUMENT;
+  const uint32_t M = c->observations, T = c->tracks;
+  const bool guarded = c->max_relative_depth_std_dev > 0;
+  if (!(c->max_error_px >= 1e-12 && c->max_error_px <= 1e6) ||
+      !(c->pixel_noise_std_dev_px > 0 && c->pixel_noise_std_dev_px < INFINITY) ||
+      !(c->max_relative_depth_std_dev >= 0 && c->max_relative_depth_std_dev < INFINITY)) {
+    set_error("Invalid GPU observation threshold");
+    return TCV_BA_ERR_ARGUMENT;
+  }
+  if ((M && (!cameras || !points || !pixels || !indices || !c->views || !c->points)) ||
+      (guarded && T && !offsets) || (!guarded && T) || (guarded && T != c->points)) {
+    set_error("observation arrays do not match the control");
+    return TCV_BA_ERR_ARGUMENT;
+  }
+  if (M > kNoallocObservations) {
+    set_error("tcv_obs_evaluate_noalloc takes at most " + std::to_string(kNoallocObservations) +
+              " observations (its fixed scratch); " + std::to_string(M) + " given");
+    return TCV_BA_ERR_ARGUMENT;
+  }
+  double* base = nullptr;
+  TRY(scratch_address(&base));
+  hipStream_t st = (hipStream_t)stream;
+  const uint32_t blocks = (M ? M : 1) / 64 + ((M ? M : 1) % 64 != 0);
+  ObsProblem o;
+  o.cameras = cameras;
+  o.points = points;
+  o.pixels = pixels;
+  o.indices = indices;
+  o.offsets = offsets;
+  o.flags = flags;
+  o.errors = base;
+  o.partials = base + kNoallocObservations;
+  o.summary = summary;
+  o.count = M;
+  o.tracks = guarded ? T : 0;
+  o.blocks = blocks;
+  o.max_error = c->max_error_px;
+  o.max_relative_depth_std = c->max_relative_depth_std_dev;
+  o.pixel_noise_std = c->pixel_noise_std_dev_px;
+  // the device arrays' indices are checked on the device before anything reads through them
+  uint32_t* bad = (uint32_t*)(base + kNoallocObservations + 2 * (size_t)kNoallocBlocks);
+  hipLaunchKernelGGL(k_obs_zero, dim3(1), dim3(64), 0, st, bad, 1u);
+  const uint32_t checked = M > o.tracks ? M : o.tracks;
+  if (checked)
+    hipLaunchKernelGGL(k_obs_check, dim3(blocks_for(checked, 256)), dim3(256), 0, st, o, c->views,
+                       c->points, bad);
+  HIP_TRY(hipGetLastError());
+  uint32_t word = 0;
+  HIP_TRY(hipMemcpyAsync(&word, bad, 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if (word) {
+    set_error(word & 1 ? "observation index outside the camera or point records"
+                       : "track offsets do not ascend from 0 to the observation count");
+    return TCV_BA_ERR_INDEX;
+  }
+  if (M) hipLaunchKernelGGL(k_obs_project, dim3(blocks_for(M, 256)), dim3(256), 0, st, o);
+  if (o.tracks) hipLaunchKernelGGL(k_obs_depth, dim3(blocks_for(o.tracks, 256)), dim3(256), 0, st, o);
+  if (M) hipLaunchKernelGGL(k_obs_partials, dim3(blocks_for(blocks, 256)), dim3(256), 0, st, o);
+  else hipLaunchKernelGGL(k_obs_zero_f64, dim3(1), dim3(64), 0, st, o.partials, 2u * blocks);
+  hipLaunchKernelGGL(k_obs_summary, dim3(1), dim3(64), 0, st, o);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(st));
+  return TCV_BA_OK;
+}
+
 static int body_ba_jacobian(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene,
                             const uint32_t* index, double log_focal, double* out,
                             tcv_ba_report* rep) {
@@ -2114,6 +2215,14 @@ int tcv_obs_evaluate(tcv_ba_context* ctx, const tcv_obs_control* c, const double
 int tcv_ba_jacobian(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene,
                     const uint32_t* index, double log_focal, double* out, tcv_ba_report* rep) {
   return locked(ctx, [&] { return body_ba_jacobian(ctx, c, scene, index, log_focal, out, rep); });
+}
+
+int tcv_obs_evaluate_noalloc(const tcv_obs_control* c, const double* cameras,
+                             const double* points, const double* pixels, const uint32_t* indices,
+                             const uint32_t* offsets, uint32_t* flags, double* summary,
+                             void* stream) {
+  return body_obs_evaluate_noalloc(c, cameras, points, pixels, indices, offsets, flags, summary,
+                                   stream);
 }
 
 int tcv_ba_set_working_bytes(tcv_ba_context* ctx, uint64_t bytes) {

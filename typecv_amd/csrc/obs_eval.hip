@@ -234,6 +234,31 @@ __global__ __launch_bounds__(64) void k_obs_summary(ObsProblem o) {
   }
 }
 
+// The index checks of the allocation-free entry (tcv_obs_evaluate_noalloc), whose arrays are
+// device arrays: every observation's view and landmark in range, and on guarded runs the track
+// offsets ascending from 0 to the observation count. Sets bit 1 (indices) or 2 (offsets) of *bad.
+__global__ __launch_bounds__(256) void k_obs_check(ObsProblem o, uint32_t views, uint32_t points,
+                                                   uint32_t* bad) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < o.count && (o.indices[2 * (size_t)i] >= views || o.indices[2 * (size_t)i + 1] >= points))
+    atomicOr(bad, 1u);
+  if (i < o.tracks) {
+    const uint32_t a = o.offsets[i], b = o.offsets[i + 1];
+    if (b < a || (i == 0 && a != 0) || (i + 1 == o.tracks && b != o.count)) atomicOr(bad, 2u);
+  }
+}
+
+// n words set to zero (the allocation-free entry clears its scratch words with a kernel, which a
+// captured graph replays correctly).
+__global__ void k_obs_zero(uint32_t* words, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) words[i] = 0;
+}
+__global__ void k_obs_zero_f64(double* words, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) words[i] = 0;
+}
+
 }  // namespace tcvba
 
 // back to the engine's contraction (-ffp-contract=on: within an expression only)
