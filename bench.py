@@ -154,7 +154,11 @@ def main() -> int:
                     help="skip the 500-camera and 256-bundle records")
     ap.add_argument("--cpu", action=argparse.BooleanOptionalAction, default=True,
                     help="time the CPU oracle baselines")
+    ap.add_argument("--forms", default="fp64,mixed,direct",
+                    help="the forms of the main workload to run (a profile takes one at a time); "
+                         "the value is the first")
     args = ap.parse_args()
+    forms = args.forms.split(",")
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(the matvec would shard by points with one all-reduce of 6C+1 doubles per PCG "
@@ -184,10 +188,12 @@ def main() -> int:
     base = C * 12 + P * 3
 
     # the value: the plain FP64 form, exactly `steps` timed solves
+    first = "mixed" if forms[0] == "mixed" else "fp64"
     control, reports, elapsed, f64 = solve_forms(args, B, _native, lib, device, packed, scene,
-                                                 index, out, "fp64", args.steps, args.warmup,
-                                                 label="fp64")
-    plain = form_record(packed, args.seed, "fp64", "stored", reports, elapsed, f64)
+                                                 index, out, first, args.steps, args.warmup,
+                                                 refine=args.refine if first == "mixed" else 0.0,
+                                                 label=first)
+    plain = form_record(packed, args.seed, first, "stored", reports, elapsed, f64)
     result = {
         "metric": METRIC,
         "value": plain["value"],
@@ -199,12 +205,12 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if first == "fp64" else "f32-products/f64-accumulation",
         "data": "synthetic (seeded trajectory scene, generated on the host, resident in HBM)",
         "config": {
             "workload": "shared-focal LM bundle adjustment, trajectory scene",
             "cameras": C, "points": P, "observations": M, "lm_iterations_max": args.iterations,
-            "max_cg": control.max_cg, "huber_px": 1.0, "precision": "fp64",
+            "max_cg": control.max_cg, "huber_px": 1.0, "precision": first,
             "outliers": info["outliers"], "focal_factor": info["focal_factor"],
             "operator": "stored", "linear_solver": "matrix-free PCG",
         },
@@ -216,6 +222,8 @@ def main() -> int:
 
     # the mixed form on the same scene, in the same process: FP32 cross blocks and Schur products,
     # FP64 accumulation, V^-1, preconditioner, dots and LM; refinement of the FP64 residual
+    if first != "fp64" or "mixed" not in forms:
+        return finish(args, result, device, packed, control, plain)
     _, reports, elapsed, mx = solve_forms(args, B, _native, lib, device, packed, scene, index,
                                           out, "mixed", sub_steps, 1, refine=args.refine,
                                           label="mixed")
@@ -234,6 +242,8 @@ def main() -> int:
         f"{mixed['final_cost_rel_err_vs_fp64']:.1e} from FP64")
 
     # the directSolve option on the same scene: a dense Cholesky solve per LM iteration
+    if "direct" not in forms:
+        return finish(args, result, device, packed, control, plain)
     _, reports, elapsed, dv = solve_forms(args, B, _native, lib, device, packed, scene, index,
                                           out, "fp64", sub_steps, 1, direct=True, label="direct")
     d_lm = sum(r["lm_iterations"] for r in reports)
@@ -246,14 +256,19 @@ def main() -> int:
         "linear_solver": "directSolve: dense Cholesky (64-column panels), certified by the "
                          "matrix-free residual",
     }
-    del scene, index, out
+    return finish(args, result, device, packed, control, plain)
+
+
+def finish(args, result, device, packed, control, plain) -> int:
+    """The sub-workload records and the CPU baselines, then the JSON line."""
+    import torch
+
     device.dispose()
     torch.cuda.empty_cache()
-
+    sub_steps = args.sub_steps or min(args.steps, 5)
     if not args.no_subworkloads:
         result["trajectory_500"] = trajectory_500(args, sub_steps)
         result["batched_256"] = batched_256(args, sub_steps)
-
     if args.cpu:
         result["cpu_baseline"] = cpu_baseline(packed, control, plain)
         result["cpu_dense_ring_10"] = cpu_dense_ring()

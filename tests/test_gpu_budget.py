@@ -27,10 +27,11 @@ pytestmark = pytest.mark.gpu
 
 def stored_bytes(control) -> int:
     """Bytes of the arrays only a resident solve holds: the point-major and camera-major cross
-    blocks (K words per observation, padded to 64 observations) and, with FP64 blocks, the fused
-    product's 64-byte rows."""
-    K = 21 if control.shared_focal else 18
-    ws = 4 if control.precision == _native.MIXED else 8
+    blocks (FP64: K = 18 or 21 words per observation; the mixed form's FP32 Jacobian factors: 9 or
+    11; padded to 64 observations) and, with FP64 blocks, the fused product's 64-byte rows."""
+    mixed = control.precision == _native.MIXED
+    K = (11 if control.shared_focal else 9) if mixed else (21 if control.shared_focal else 18)
+    ws = 4 if mixed else 8
     words = K * ((control.observations + 63) // 64) * 64
     return 2 * words * ws + (64 * control.observations if ws == 8 else 0)
 

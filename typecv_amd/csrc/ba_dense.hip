@@ -1094,6 +1094,7 @@ __global__ __launch_bounds__(CG_THREADS) void k_certify(Problem q, int final_pas
     s[S_RHO_NEXT] = rho;
     s[S_DIRECT] = 0;
     s[S_STEP_LIMIT] = s[S_STEPS] + pcg_budget(q, g);  // PCG from x with the bundle's budget
+    s[S_PASS_STOP] = q.tol * q.tol * s[S_R0];  // to the tolerance of the original system
     if (rho > 0)
       s[S_PCG_ACTIVE] = 1;
     else

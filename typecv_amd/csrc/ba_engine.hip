@@ -22,6 +22,7 @@
 
 #include "../../include/typecv_ba.h"
 #include "ba_kernels.hip"
+#include "ba_factored.hip"
 #include "ba_dense.hip"
 #include "obs_eval.hip"
 
@@ -367,7 +368,10 @@ struct Launch {
     hipLaunchKernelGGL((k_init_cg<SHARED>), dim3(q.B), dim3(CG_THREADS), 0, s, q);
   }
   static void point_sweep(const Problem& q, hipStream_t s) {
-    if (q.tprod)
+    if constexpr (sizeof(WT) == 4)
+      hipLaunchKernelGGL((k_point_sweep_factored<SHARED, SWEEP_SCHUR>), dim3(tile_blocks(q)),
+                         dim3(TILE * TILE_WAVES), 0, s, q);
+    else if (q.tprod)
       hipLaunchKernelGGL((k_point_sweep_fused<WT, SHARED>), dim3(tile_blocks(q)),
                          dim3(TILE * TILE_WAVES), 0, s, q);
     else
@@ -375,7 +379,10 @@ struct Launch {
                          dim3(TILE * TILE_WAVES), 0, s, q);
   }
   static void camera_sweep(const Problem& q, hipStream_t s) {
-    if (q.tprod)
+    if constexpr (sizeof(WT) == 4)
+      hipLaunchKernelGGL((k_camera_sweep_factored<SHARED>), dim3(q.n_tasks), dim3(TASK_THREADS),
+                         q.camera_lds, s, q);
+    else if (q.tprod)
       hipLaunchKernelGGL((k_camera_tsum<SHARED>), dim3(q.n_tasks), dim3(TASK_THREADS),
                          q.camera_lds, s, q);
     else
@@ -388,21 +395,37 @@ struct Launch {
     hipLaunchKernelGGL((k_cg_direction<SHARED>), dim3(q.n_groups), dim3(ADV_THREADS), 0, s, q);
   }
   static void update_points(const Problem& q, hipStream_t s) {
-    hipLaunchKernelGGL((k_point_sweep<WT, SHARED, SWEEP_UPDATE>), dim3(tile_blocks(q)),
-                       dim3(TILE * TILE_WAVES), 0, s, q);
+    if constexpr (sizeof(WT) == 4)
+      hipLaunchKernelGGL((k_point_sweep_factored<SHARED, SWEEP_UPDATE>), dim3(tile_blocks(q)),
+                         dim3(TILE * TILE_WAVES), 0, s, q);
+    else
+      hipLaunchKernelGGL((k_point_sweep<WT, SHARED, SWEEP_UPDATE>), dim3(tile_blocks(q)),
+                         dim3(TILE * TILE_WAVES), 0, s, q);
   }
   // The streamed route: the same sweeps with every cross block relinearised in registers.
   static void streamed_point_sweep(const Problem& q, hipStream_t s) {
-    hipLaunchKernelGGL((k_point_sweep<WT, SHARED, SWEEP_SCHUR, true>), dim3(tile_blocks(q)),
-                       dim3(TILE * TILE_WAVES), 0, s, q);
+    if constexpr (sizeof(WT) == 4)
+      hipLaunchKernelGGL((k_point_sweep_factored<SHARED, SWEEP_SCHUR, true>), dim3(tile_blocks(q)),
+                         dim3(TILE * TILE_WAVES), 0, s, q);
+    else
+      hipLaunchKernelGGL((k_point_sweep<WT, SHARED, SWEEP_SCHUR, true>), dim3(tile_blocks(q)),
+                         dim3(TILE * TILE_WAVES), 0, s, q);
   }
   static void streamed_camera_sweep(const Problem& q, hipStream_t s) {
-    hipLaunchKernelGGL((k_camera_sweep<WT, SHARED, false, true>), dim3(q.n_tasks),
-                       dim3(TASK_THREADS), 0, s, q);
+    if constexpr (sizeof(WT) == 4)
+      hipLaunchKernelGGL((k_camera_sweep_factored<SHARED, true>), dim3(q.n_tasks),
+                         dim3(TASK_THREADS), 0, s, q);
+    else
+      hipLaunchKernelGGL((k_camera_sweep<WT, SHARED, false, true>), dim3(q.n_tasks),
+                         dim3(TASK_THREADS), 0, s, q);
   }
   static void streamed_update(const Problem& q, hipStream_t s) {
-    hipLaunchKernelGGL((k_point_sweep<WT, SHARED, SWEEP_UPDATE, true>), dim3(tile_blocks(q)),
-                       dim3(TILE * TILE_WAVES), 0, s, q);
+    if constexpr (sizeof(WT) == 4)
+      hipLaunchKernelGGL((k_point_sweep_factored<SHARED, SWEEP_UPDATE, true>),
+                         dim3(tile_blocks(q)), dim3(TILE * TILE_WAVES), 0, s, q);
+    else
+      hipLaunchKernelGGL((k_point_sweep<WT, SHARED, SWEEP_UPDATE, true>), dim3(tile_blocks(q)),
+                         dim3(TILE * TILE_WAVES), 0, s, q);
   }
   static void probe(const Problem& q, const double* v, double* sv, double* zv, double* rhs,
                     hipStream_t s) {
@@ -648,7 +671,8 @@ std::vector<Need> solve_needs(tcv_ba_context* ctx, const tcv_ba_control* c, uint
   const uint32_t n_unstored = exact ? 0u : (uint32_t)plan.direct_bundles.size();
   const size_t n_tiles = plan.tiles.size() - 1, n_tasks = plan.tasks.size() / 4;
   const size_t n_cost = plan.cost_blocks.size() / 3, n_groups = plan.groups.size() / GROUP_WORDS;
-  const size_t w_words = (size_t)K * (((size_t)M + 63) / 64) * 64;
+  // the cross blocks (FP64, K words per observation) or the mixed form's FP32 factors (KF words)
+  const size_t w_words = (size_t)(ws == 4 ? kf_words(shared) : K) * (((size_t)M + 63) / 64) * 64;
   std::vector<Need> n = {
       {&ctx->scene, ((size_t)C * 18 + (size_t)P * 3 + (size_t)M * 2) * 8},
       {&ctx->trial, ((size_t)C * 12 + (size_t)P * 3) * 8},
@@ -804,7 +828,7 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
   const uint32_t n_tasks = (uint32_t)plan.tasks.size() / 4;
   const uint32_t n_cost = (uint32_t)plan.cost_blocks.size() / 3;
   const uint32_t n_groups = (uint32_t)plan.groups.size() / GROUP_WORDS;
-  const size_t w_words = (size_t)K * (((size_t)M + 63) / 64) * 64;
+  const size_t w_words = (size_t)(ws == 4 ? kf_words(shared) : K) * (((size_t)M + 63) / 64) * 64;
   const size_t vec_words = (size_t)m * 5 + (size_t)P * 3;
   bool streamed = residency == STREAMED;
   if (residency == DECIDE) TRY(admit(ctx, c, B, C, P, M, plan, exact, {}, streamed));

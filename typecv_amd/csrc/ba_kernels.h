@@ -54,6 +54,8 @@ enum : int {
   S_DUPLICATE = 40,      // set at set-up when a camera of the bundle observes one point twice
   S_DENSE_BAD = 41,      // a pivot of the multi-launch factorisation failed this LM iteration
   S_STEP_LIMIT = 42,     // word 23 at which the running PCG pass stops (its start + the budget)
+  S_PASS_STOP = 43,      // |r|^2 at which the running PCG pass has converged: tol^2 |r0|^2 for the
+                         // first pass; a refinement restart's own (k_refine)
   S_WORDS = 48
 };
 

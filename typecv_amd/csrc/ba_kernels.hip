@@ -330,6 +330,34 @@ __device__ __forceinline__ WT* wptr(void* base, uint32_t k, uint32_t i, uint32_t
 #endif
 }
 
+// The mixed form stores each observation's cross block as FP32 Jacobian factors (ba_factored.hip):
+constexpr int kf_words(bool shared) { return shared ? 11 : 9; }
+
+// The factor words of one observation: s du (3), s dv (3), q (3)[, s Jf (2)], zero where the point
+// does not project.
+__device__ __forceinline__ void cross_factors(const Lin& L, bool ok, bool shared, double* f) {
+  if (!ok) {
+    for (int k = 0; k < 11; k++) f[k] = 0;
+    return;
+  }
+  const double s = sqrt(L.w);
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    f[k] = s * L.du[k];
+    f[3 + k] = s * L.dv[k];
+    f[6 + k] = L.qc[k];
+  }
+  f[9] = shared ? s * L.Jf[0] : 0.0;
+  f[10] = shared ? s * L.Jf[1] : 0.0;
+}
+
+// Word k of observation (or slot) i's factors: blocked [M/64][KF][64] as the cross blocks are.
+template <bool SHARED>
+__device__ __forceinline__ float* fptr(void* base, uint32_t k, uint32_t i) {
+  constexpr uint32_t KF = kf_words(SHARED);
+  return static_cast<float*>(base) + ((size_t)(i >> 6) * KF + k) * 64 + (i & 63);
+}
+
 __device__ __forceinline__ double* vx(const Problem& q) { return q.vec; }
 __device__ __forceinline__ double* vr(const Problem& q) { return q.vec + q.m; }
 __device__ __forceinline__ double* vp(const Problem& q) { return q.vec + 2 * (size_t)q.m; }
@@ -576,8 +604,15 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_pass(Problem q) {
       } else {
         for (int k = 0; k < 21; k++) W[k] = 0;
       }
-      if (stores_w(g) && q.Wp)
-        for (uint32_t k = 0; k < K; k++) *wptr<WT, SHARED>(q.Wp, k, i, q.M) = (WT)W[k];
+      if (stores_w(g) && q.Wp) {
+        if constexpr (sizeof(WT) == 4) {
+          double fw[11];
+          cross_factors(L, ok, SHARED, fw);
+          for (int k = 0; k < kf_words(SHARED); k++) *fptr<SHARED>(q.Wp, k, i) = (float)fw[k];
+        } else {
+          for (uint32_t k = 0; k < K; k++) *wptr<WT, SHARED>(q.Wp, k, i, q.M) = (WT)W[k];
+        }
+      }
       double* o = st[lane];
       if (ok) {
         const double w = L.w;
@@ -737,8 +772,15 @@ __global__ __launch_bounds__(TASK_THREADS) void k_camera_pass(Problem q) {
     } else {
       for (int k = 0; k < 21; k++) W[k] = 0;
     }
-    if (stores_w(g) && q.Wc)
-      for (uint32_t k = 0; k < K; k++) *wptr<WT, SHARED>(q.Wc, k, n, q.M) = (WT)W[k];
+    if (stores_w(g) && q.Wc) {
+      if constexpr (sizeof(WT) == 4) {
+        double fw[11];
+        cross_factors(L, ok, SHARED, fw);
+        for (int k = 0; k < kf_words(SHARED); k++) *fptr<SHARED>(q.Wc, k, n) = (float)fw[k];
+      } else {
+        for (uint32_t k = 0; k < K; k++) *wptr<WT, SHARED>(q.Wc, k, n, q.M) = (WT)W[k];
+      }
+    }
     if (!ok) continue;
     const double w = L.w;
     // with a right-hand side given, the camera's own b_c enters once, in k_camera_finish
@@ -1010,6 +1052,7 @@ __global__ __launch_bounds__(CG_THREADS) void k_init_cg(Problem q) {
     s[S_RHO_NEXT] = rho;
     s[S_R0] = res;
     s[S_R] = res;
+    s[S_PASS_STOP] = q.tol * q.tol * res;
     if (res > 0 && rho > 0)
       s[S_PCG_ACTIVE] = 1;
     else if (res > 1e-24)
@@ -1625,7 +1668,7 @@ __global__ __launch_bounds__(ADV_THREADS) void k_cg_direction(Problem q) {
   const uint32_t f = 6 * q.C + G.bundle, row = G.has ? G.row : 6 * G.c0;
   double* p = vp(q);
   const double* z = vz(q);
-  double active = s[S_PCG_ACTIVE], rho_old = s[S_RHO], r0 = s[S_R0];
+  double active = s[S_PCG_ACTIVE], rho_old = s[S_RHO], stop = s[S_PASS_STOP];
   double steps = s[S_STEPS], limit = s[S_STEP_LIMIT];
   double res, rho;
   group_loads(q, G, 2, res, rho);
@@ -1633,7 +1676,7 @@ __global__ __launch_bounds__(ADV_THREADS) void k_cg_direction(Problem q) {
   double pf = SHARED ? p[f] : 0.0, zf = SHARED ? z[f] : 0.0;
   settle(active);
   settle(rho_old);
-  settle(r0);
+  settle(stop);
   settle(steps);
   settle(limit);
   settle(res);
@@ -1644,7 +1687,7 @@ __global__ __launch_bounds__(ADV_THREADS) void k_cg_direction(Problem q) {
   settle(zf);
   if (active == 0) return;
   block_sum2(res, rho, lds);
-  const bool converged = res <= q.tol * q.tol * r0;
+  const bool converged = res <= stop;
   const bool broke = !converged && !(rho > 0);
   // the pass's budget: the bundle stops unconverged, as the reference's host loop stops launching
   // at maxCgIterations (the gauge then rejects the step)
@@ -1879,6 +1922,10 @@ __global__ __launch_bounds__(CG_THREADS) void k_refine(Problem q) {
     s[S_CONVERGED] = 0;
     s[S_REFINEMENTS] += 1;
     s[S_STEP_LIMIT] = s[S_STEPS] + pcg_budget(q, g);  // each restart: the budget again
+    // the restart solves S d = r on the FP32 operator: to the PCG tolerance relative to this
+    // residual, and no tighter than a tenth of the refinement target (the next FP64 residual
+    // then lands below the target unless the operator's own error stops it)
+    s[S_PASS_STOP] = fmax(q.tol * q.tol * res, 0.01 * q.refine_tol * q.refine_tol * r0);
     if (rho > 0)
       s[S_PCG_ACTIVE] = 1;
     else
