@@ -137,6 +137,9 @@ typedef struct tcv_ba_report {
                                   picked by the same tuner; 0: the two plain sweeps */
   uint64_t streamed;           /* 1: the solve ran streamed under the context's working budget
                                   (no stored cross blocks: every sweep relinearises them) */
+  uint64_t held_points;        /* tcv_ba_polish, tcv_ba_backward: points held fixed by the last
+                                  exact-Hessian system, their blocks singular to rounding (tracks
+                                  run off towards infinity); zero adjoint rows */
 } tcv_ba_report;
 
 typedef struct tcv_ba_context tcv_ba_context;
@@ -325,22 +328,40 @@ int tcv_ba_polish(tcv_ba_context* context, const tcv_ba_control* control, const 
                   uint32_t newton_steps, uint32_t max_cg, double tolerance, double* out,
                   tcv_ba_report* report);
 
+/* Systems of the adjoint (tcv_ba_backward's `mode`). */
+enum tcv_ba_adjoint {
+  /* the exact Hessian of E (Gauss-Newton with the Huber curvature, plus the residual-weighted
+     second derivatives of the projection and the chart): the derivative at a stationary scene */
+  TCV_BA_ADJOINT_EXACT = 0,
+  /* the Gauss-Newton Hessian sum_i J_i^T D_i J_i (the Huber curvature kept, the second derivatives
+     dropped): positive semi-definite, and definite after the gauge regularisation, at any scene; the
+     usual linearised sensitivity of a least-squares solution, defined where the exact Hessian at the
+     forward's output is not positive definite. Its error against the exact derivative is of the
+     order of the residual-weighted curvature it drops */
+  TCV_BA_ADJOINT_GAUSS_NEWTON = 1
+};
+
 /*
- * The adjoint (backward) of the solve by implicit differentiation at a stationary scene:
- * for a cotangent v on the outputs (12 per camera: R row-major, t; 3 per point; then, on the shared
- * route, one for the log focal scale) returns pixel_cotangent[2M] = d<v, theta*>/d(observation
- * pixels). Solves H lambda = (dN)^T v~ with the exact Hessian H by the forward's Schur/PCG (to
- * `tolerance`, at most `max_cg` iterations), where v~ is v in the chart and N the gauge map
- * (directional renormalisation, the similarity of the one-fixed-camera gauge), then
- * pixel_cotangent_i = D_i J_i lambda with D_i = d(w r)/dr. Cross blocks are relinearised from the
- * scene, not kept from the forward. `adjoint` (optional, 6C + 3P + shared doubles: chart rows of each
- * camera, point rows, focal) receives lambda. TCV_BA_ERR_SOLVE when the system is not positive
- * definite or PCG does not converge.
+ * The adjoint (backward) of the solve by implicit differentiation: for a cotangent v on the outputs
+ * (12 per camera: R row-major, t; 3 per point; then, on the shared route, one for the log focal
+ * scale) returns pixel_cotangent[2M] = d<v, theta*>/d(observation pixels) and, when
+ * `focal_cotangent` is given, focal_cotangent[C] = d<v, theta*>/d(log of camera c's input focal,
+ * fx and fy scaled together). Solves H lambda = (dN)^T v~ with the `mode` system H (enum
+ * tcv_ba_adjoint) by the forward's Schur/PCG (to `tolerance`, at most `max_cg` iterations), where
+ * v~ is v in the chart and N the gauge map (directional renormalisation, the similarity of the
+ * one-fixed-camera gauge), then pixel_cotangent_i = D_i J_i lambda with D_i = d(w r)/dr and, since
+ * every column of J_i scales with the focal (d J_i/d log f = J_i),
+ *   focal_cotangent_c = -sum over camera c's observations of (D_i J_i lambda) . Jf_i
+ *                       + (J_i lambda) . (w_i r_i)   [the second term in the exact mode only]
+ * with Jf_i = d pixel_i/d log f. Cross blocks are relinearised from the scene, not kept from the
+ * forward. `adjoint` (optional, 6C + 3P + shared doubles: chart rows of each camera, point rows,
+ * focal) receives lambda. TCV_BA_ERR_SOLVE when the system is not positive definite or PCG does not
+ * converge (the exact mode needs a stationary scene, tcv_ba_polish).
  */
 int tcv_ba_backward(tcv_ba_context* context, const tcv_ba_control* control, const double* scene,
                     const uint32_t* index, double log_focal, const double* cotangent,
-                    uint32_t max_cg, double tolerance, double* pixel_cotangent, double* adjoint,
-                    tcv_ba_report* report);
+                    uint32_t max_cg, double tolerance, uint32_t mode, double* pixel_cotangent,
+                    double* focal_cotangent, double* adjoint, tcv_ba_report* report);
 
 #ifdef __cplusplus
 }

@@ -8,7 +8,10 @@ Only tests import this module. Everything here is built on the oracle's chart gr
   * `newton` runs dense Newton steps on it (a stationary point to rounding, for small problems
     with no gauge freedom);
   * `dense_vjp` is implicit differentiation with the dense Hessian: lambda = H^-1 v on the
-    variable rows, then D_i J_i lambda per observation, for a cotangent given in the chart.
+    variable rows, then D_i J_i lambda per observation, for a cotangent given in the chart;
+  * `dense_focal_vjp` is its input-focal part, -lambda . dg/d(eps_c) for eps_c the log of camera
+    c's input fx and fy, with dg/d(eps_c) by central differences of the same gradient (independent
+    of the engine's closed form in D_i J_i lambda, Jf_i and w_i r_i).
 """
 from __future__ import annotations
 
@@ -114,3 +117,22 @@ def dense_vjp(control, packed, scene, log_focal: float, chart_cotangent: np.ndar
             t = b[20] * (t - rh * (rh @ t))
         out[i] = t
     return out.reshape(-1), lam
+
+
+def dense_focal_vjp(control, scene, index, log_focal: float, lam: np.ndarray,
+                    h: float = 1e-6) -> np.ndarray:
+    """Input-focal cotangent (C) of implicit differentiation for the adjoint `lam` (chart layout,
+    lam = H^-1 v with H the central-difference Jacobian of the chart gradient g, as `dense_vjp`):
+    d<v, theta*>/d eps_c = -lam . dg/d eps_c, eps_c scaling camera c's input fx and fy by e^eps."""
+    C, P, sh, n = sizes(control)
+    rows = variable_rows(control, index)
+    base = 12 * C + 3 * P
+    out = np.zeros(C)
+    for c in range(C):
+        sp, sm = scene.copy(), scene.copy()
+        sp[base + 6 * c: base + 6 * c + 2] *= np.exp(h)
+        sm[base + 6 * c: base + 6 * c + 2] *= np.exp(-h)
+        gp, _, _ = O.gradient(control, sp, index, log_focal)
+        gm, _, _ = O.gradient(control, sm, index, log_focal)
+        out[c] = -float(lam[rows] @ ((gp - gm) / (2 * h))[rows])
+    return out

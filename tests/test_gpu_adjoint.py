@@ -7,7 +7,9 @@ residuals (h = 1e-6, the step of the reference's geometry tests) to 1e-5 relativ
 scene's oracle gradient at most 1e-8 of the forward output's, the anchor pose unchanged and the
 gauge baseline kept; the backward's pixel cotangent within 1e-6 (relative to its largest entry) of
 the dense implicit derivative, and within 1e-5 relative of central differences (h = 1e-4 px) of
-re-polished solutions, gauge maps and directional points included.
+re-polished solutions, gauge maps and directional points included; the input-focal cotangent within
+1e-6 of the dense -lambda . dg/d(log f_c) (differences of the oracle gradient) and 1e-5 of central
+differences (h = 1e-6 in log f_c) of re-polished solutions, on the shared and the fixed route.
 """
 import dataclasses
 
@@ -157,12 +159,17 @@ def test_backward_matches_dense_implicit_derivative(device):
     chart[6 * C:] = rng.normal(size=3 * P + 1)
     cot = np.zeros(12 * C + 3 * P + 1)
     cot[12 * C:] = chart[6 * C:]
-    pixels, adjoint, rep = B.backward(device, control, pol, lf, cot, tolerance=1e-12)
+    pixels, focal, adjoint, rep = B.backward(device, control, pol, lf, cot, tolerance=1e-12)
     ref, lam = A.dense_vjp(control, pol, pol.scene, lf, chart, control.huber)
     scale = np.max(np.abs(ref))
     assert np.max(np.abs(pixels.reshape(-1) - ref)) <= 1e-6 * scale
     rows = A.variable_rows(control, packed.index)
     assert np.max(np.abs(adjoint[rows] - lam[rows])) <= 1e-6 * np.max(np.abs(lam))
+    # the input-focal cotangent against -lambda . dg/d(eps_c) by differences of the oracle gradient
+    ref_focal = A.dense_focal_vjp(control, pol.scene, packed.index, lf, lam)
+    print(f"  focal: max |engine - dense| {np.max(np.abs(focal - ref_focal)):.2e} of "
+          f"{np.max(np.abs(ref_focal)):.2e}", flush=True)
+    assert np.max(np.abs(focal - ref_focal)) <= 1e-6 * np.max(np.abs(ref_focal))
 
 
 @pytest.mark.parametrize("case", ["joint_focal_one_fixed", "directional"])
@@ -179,7 +186,7 @@ def test_backward_matches_differences_of_polished_solutions(device, case):
     n_out = 12 * C + 3 * P + sh
     rng = np.random.default_rng(21)
     cot = rng.normal(size=n_out)
-    pixels, _, rep = B.backward(device, control, pol, lf, cot, tolerance=1e-12)
+    pixels, focal, _, rep = B.backward(device, control, pol, lf, cot, tolerance=1e-12)
     grad = pixels.reshape(-1)
     h = 1e-4
     pix = 18 * C + 3 * P
@@ -200,3 +207,24 @@ def test_backward_matches_differences_of_polished_solutions(device, case):
                              - B.output_log_focal(control, outs[1])) / (2 * h)
         an = grad @ du
         assert abs(fd - an) <= 1e-5 * max(abs(fd), 1e-2 * np.max(np.abs(grad))), (trial, fd, an)
+    # the input-focal cotangent: each camera's fx, fy scaled by e^(+-h)
+    h = 1e-6
+    base = 12 * C + 3 * P
+    fds = []
+    for c in range(C):
+        outs = []
+        for sign in (1, -1):
+            scene = pol.scene.copy()
+            scene[base + 6 * c: base + 6 * c + 2] *= np.exp(sign * h)
+            out, _ = B.polish(device, control, dataclasses.replace(pol, scene=scene), lf,
+                              iterations=4, tolerance=1e-13)
+            outs.append(out)
+        fd = cot[:n_out - sh] @ ((outs[0][:n_out - sh] - outs[1][:n_out - sh]) / (2 * h))
+        if sh:
+            fd += cot[-1] * (B.output_log_focal(control, outs[0])
+                             - B.output_log_focal(control, outs[1])) / (2 * h)
+        fds.append(fd)
+    fds = np.array(fds)
+    print(f"  focal: max |adjoint - fd| {np.max(np.abs(focal - fds)):.2e} of "
+          f"{np.max(np.abs(fds)):.2e}", flush=True)
+    assert np.max(np.abs(focal - fds)) <= 1e-5 * np.max(np.abs(fds))

@@ -187,8 +187,8 @@ def test_direct_polish_and_backward_match_pcg(device, case):
     cot = np.random.default_rng(5).normal(size=12 * packed.C + 3 * packed.P + sh)
     # 3e-12: at 1e-12 the certificate of one of these systems sits at its FP64 floor, misses after
     # both corrections and hands over to PCG (measured: the pixel cotangents then agree to 8e-13)
-    want, _, _ = B.backward(device, control, pol, lf, cot, tolerance=3e-12)
-    got, _, brep = B.backward(device, direct, pol, lf, cot, tolerance=3e-12)
+    want, _, _, _ = B.backward(device, control, pol, lf, cot, tolerance=3e-12)
+    got, _, _, brep = B.backward(device, direct, pol, lf, cot, tolerance=3e-12)
     rel = np.max(np.abs(got - want)) / np.max(np.abs(want))
     print(f"  backward: direct solves {brep['direct_solves']}, relative difference {rel:.2e}",
           flush=True)

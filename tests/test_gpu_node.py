@@ -169,3 +169,39 @@ def test_node_derivative_entries_match_python(device, tmp_path, mode):
     else:
         expected = B.jacobian(device, control, at, lf).reshape(-1)
     assert np.array_equal(got, expected)
+
+
+@pytest.mark.parametrize("mode", ["exact", "gaussNewton"])
+def test_node_backward_matches_python(device, tmp_path, mode):
+    """`backward` through the addon (tcv_ba_backward on a libuv worker) returns the Python binding's
+    pixel and input-focal cotangents bit for bit, in both adjoint modes."""
+    node = shutil.which("node")
+    if node is None or not (NODE_DIR / "typecv_ba.node").exists():
+        pytest.skip("no Node or no built addon")
+    print(f"\n[node] backward {mode}", flush=True)
+    packed, _ = S.trajectory_scene(cameras=10, points=600, seed=6)
+    control = B.make_control(packed.C, packed.P, packed.M, iterations=15, max_cg=121,
+                             shared_focal=True, huber=1.0)
+    out, _ = B.solve_packed(device, control, packed)
+    at = B.at_solution(packed, out)
+    lf = B.output_log_focal(control, out)
+    pol_out, _ = B.polish(device, control, at, lf, iterations=3, tolerance=1e-12,
+                          lm_iterations=30)
+    pol = B.at_solution(at, pol_out)
+    lf = B.output_log_focal(control, pol_out)
+    cot = np.random.default_rng(3).normal(size=12 * packed.C + 3 * packed.P + 1)
+    control_vector(control).tofile(tmp_path / "control.f64")
+    pol.scene.astype(np.float64).tofile(tmp_path / "scene.f64")
+    pol.index.astype(np.uint32).tofile(tmp_path / "index.u32")
+    cot.tofile(tmp_path / "cot.f64")
+    run = subprocess.run([node, str(NODE_DIR / "run.js"), "--backward",
+                          str(tmp_path / "control.f64"), str(tmp_path / "scene.f64"),
+                          str(tmp_path / "index.u32"), repr(lf), str(tmp_path / "cot.f64"), mode,
+                          str(tmp_path / "out")], capture_output=True, text=True, timeout=120)
+    assert run.returncode == 0, run.stderr
+    pixels, focal, _, _ = B.backward(device, control, pol, lf, cot, max_cg=4000,
+                                     tolerance=1e-10, mode=mode)
+    got_pixels = np.fromfile(tmp_path / "out.pixels.f64", dtype=np.float64)
+    got_focal = np.fromfile(tmp_path / "out.focal.f64", dtype=np.float64)
+    assert np.array_equal(got_pixels, pixels.reshape(-1))
+    assert np.array_equal(got_focal, focal)

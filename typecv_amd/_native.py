@@ -98,6 +98,7 @@ class Report(ctypes.Structure):
         ("camera_sweep_cap", ctypes.c_uint64),
         ("fused_product", ctypes.c_uint64),
         ("streamed", ctypes.c_uint64),
+        ("held_points", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -181,7 +182,7 @@ def library() -> ctypes.CDLL:
                                   ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, vp, P(Report)]
     lib.tcv_ba_polish.restype = ctypes.c_int
     lib.tcv_ba_backward.argtypes = [vp, c_control, vp, vp, ctypes.c_double, vp, ctypes.c_uint32,
-                                    ctypes.c_double, vp, vp, P(Report)]
+                                    ctypes.c_double, ctypes.c_uint32, vp, vp, vp, P(Report)]
     lib.tcv_ba_backward.restype = ctypes.c_int
     lib.tcv_obs_evaluate.argtypes = [vp, P(ObsControl), vp, vp, vp, vp, vp, vp, vp]
     lib.tcv_obs_evaluate.restype = ctypes.c_int

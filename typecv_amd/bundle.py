@@ -481,10 +481,18 @@ def polish(device: HipDevice, control: _native.Control, packed: PackedScene,
     return out, report.as_dict()
 
 
+ADJOINT_MODES = {"exact": 0, "gaussNewton": 1}
+
+
 def backward(device: HipDevice, control: _native.Control, packed: PackedScene, log_focal: float,
-             cotangent: np.ndarray, max_cg: int = 4000, tolerance: float = 1e-10):
+             cotangent: np.ndarray, max_cg: int = 4000, tolerance: float = 1e-10,
+             mode: str = "exact"):
     """`tcv_ba_backward`: the pixel cotangent (M, 2) of a cotangent on the outputs
-    (12C + 3P [+ 1]), the adjoint lambda (6C + 3P [+ 1], chart) and the report."""
+    (12C + 3P [+ 1]), the input-focal cotangent (C: d/d log of each camera's fx, fy), the adjoint
+    lambda (6C + 3P [+ 1], chart) and the report. `mode`: "exact" (the exact Hessian; needs a
+    stationary scene) or "gaussNewton" (the Gauss-Newton Hessian, defined at any scene)."""
+    if mode not in ADJOINT_MODES:
+        raise ValueError(f"adjoint mode must be one of {sorted(ADJOINT_MODES)}")
     lib = _native.library()
     n_out = control.cameras * 12 + control.points * 3 + (1 if control.shared_focal else 0)
     cot = np.ascontiguousarray(cotangent, dtype=np.float64)
@@ -493,18 +501,20 @@ def backward(device: HipDevice, control: _native.Control, packed: PackedScene, l
     scene = np.ascontiguousarray(packed.scene, dtype=np.float64)
     index = np.ascontiguousarray(packed.index, dtype=np.uint32)
     pixels = np.empty((packed.M, 2))
+    focal = np.empty(control.cameras)
     adjoint = np.empty(control.cameras * 6 + control.points * 3 + (1 if control.shared_focal else 0))
     report = _native.Report()
 
     def job():
         _native.check(lib.tcv_ba_backward(device.handle, ctypes.byref(control), scene.ctypes.data,
                                           index.ctypes.data, float(log_focal), cot.ctypes.data,
-                                          int(max_cg), float(tolerance), pixels.ctypes.data,
+                                          int(max_cg), float(tolerance), ADJOINT_MODES[mode],
+                                          pixels.ctypes.data, focal.ctypes.data,
                                           adjoint.ctypes.data, ctypes.byref(report)),
                       "tcv_ba_backward")
 
     device.enqueue(job)
-    return pixels, adjoint, report.as_dict()
+    return pixels, focal, adjoint, report.as_dict()
 
 
 # ------------------------------------------------------------------ the operator
@@ -639,16 +649,22 @@ bundleAdjustGpu = bundle_adjust_gpu
 
 def bundle_adjust_gpu_backward(device: HipDevice, scene: Mapping, options: Mapping | None,
                                result: Mapping, cotangent: Mapping, *, polish_steps: int = 3,
-                               tolerance: float = 1e-10, max_cg: int = 4000) -> dict:
+                               tolerance: float = 1e-10, max_cg: int = 4000,
+                               mode: str = "exact") -> dict:
     """The backward entry of `bundle_adjust_gpu`: for a cotangent on its outputs,
     {"views": [{"R": 3x3, "t": 3}] (result order), "landmarks": [{"xyz": 3}],
      "directionalConstraints": [{"direction": 3}] (when the result has them),
      "logFocal": d/d log f (shared-focal route)},
-    returns the cotangent of the observation pixels by implicit differentiation at the solution:
+    returns the cotangents of the observation pixels and of each view's input focal by implicit
+    differentiation at the solution:
     {"landmarks": [[2-vectors per observation]] (input landmark and observation order),
-     "directionalConstraints": [...] likewise, "pixels": (M, 2) in packed order, "report"}.
+     "directionalConstraints": [...] likewise, "pixels": (M, 2) in packed order,
+     "focal": [d/d log of view c's input fx and fy, scaled together] (result view order), "report"}.
     `polish_steps` Newton steps on the exact Hessian first make the returned solution stationary to
-    rounding (0 differentiates at the result as given). Entries the cotangent omits are zero."""
+    rounding (0 differentiates at the result as given). `mode` "gaussNewton" solves with the
+    Gauss-Newton Hessian instead of the exact one (defined at any scene; the usual linearised
+    sensitivity, off the exact derivative by the residual-weighted curvature it drops). Entries the
+    cotangent omits are zero."""
     options = dict(options or {})
     _check_option_enums(options)
     shared = bool(options.get("refineSharedFocal"))
@@ -697,13 +713,13 @@ def bundle_adjust_gpu_backward(device: HipDevice, scene: Mapping, options: Mappi
         cot[12 * C + 3 * i: 12 * C + 3 * i + 3] = np.asarray(d["direction"], dtype=np.float64)
     if shared:
         cot[-1] = float(cotangent.get("logFocal", 0.0))
-    pixels, adjoint, back = backward(device, control, at, log_focal, cot, max_cg=max_cg,
-                                     tolerance=tolerance)
+    pixels, focal, adjoint, back = backward(device, control, at, log_focal, cot, max_cg=max_cg,
+                                            tolerance=tolerance, mode=mode)
     lengths = [len(p["observations"]) for p in model["landmarks"]]
     starts = np.concatenate([[0], np.cumsum(lengths)])
     per_point = [[pixels[k].copy() for k in range(starts[i], starts[i + 1])] for i in range(P)]
     return {"landmarks": per_point[:finite], "directionalConstraints": per_point[finite:],
-            "pixels": pixels, "adjoint": adjoint, "logFocal": log_focal,
+            "pixels": pixels, "focal": focal, "adjoint": adjoint, "logFocal": log_focal,
             "report": {"polish": report, "backward": back}}
 
 

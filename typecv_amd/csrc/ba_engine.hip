@@ -357,9 +357,9 @@ struct Launch {
     hipLaunchKernelGGL((k_point_sweep<WT, SHARED, SWEEP_SOLVE>), dim3(tile_blocks(q)),
                        dim3(TILE * TILE_WAVES), 0, s, q);
   }
-  static void pixel_cotangent(const Problem& q, double* out, hipStream_t s) {
+  static void pixel_cotangent(const Problem& q, double* out, double* focal, hipStream_t s) {
     hipLaunchKernelGGL((k_pixel_cotangent<SHARED>), dim3(blocks_for(q.M, 256)), dim3(256), 0, s,
-                       q, out);
+                       q, out, focal);
   }
   static void jacobian(const Problem& q, double* out, hipStream_t s) {
     hipLaunchKernelGGL((k_jacobian<SHARED>), dim3(blocks_for(q.M, 256)), dim3(256), 0, s, q, out);
@@ -556,7 +556,7 @@ struct Ops {
   void (*refine)(const Problem&, hipStream_t);
   void (*exact_passes)(const Problem&, hipStream_t);
   void (*solve_points)(const Problem&, hipStream_t);
-  void (*pixel_cotangent)(const Problem&, double*, hipStream_t);
+  void (*pixel_cotangent)(const Problem&, double*, double*, hipStream_t);
   void (*jacobian)(const Problem&, double*, hipStream_t);
   void (*dense_solve)(const Problem&, hipStream_t);
   void (*certify)(const Problem&, int, double, hipStream_t);
@@ -1780,7 +1780,7 @@ static int body_ba_schur_probe(tcv_ba_context* ctx, const tcv_ba_control* c, con
   HIP_TRY(hipMemcpyAsync(ctx->h_state, q.state, S_WORDS * 8, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
   int32_t bad = 0;
-  for (uint32_t t = 0; t < q.n_tiles; t++) bad |= fail[t] != 0;
+  for (uint32_t t = 0; t < q.n_tiles; t++) bad |= (fail[t] & 1u) != 0;
   for (uint32_t k = 0; k < q.C; k++) bad |= !(inv[(size_t)k * 36] > 0);
   if (c->shared_focal && !(ctx->h_state[S_FOCAL_INV] > 0)) bad = 1;
   *status = bad;
@@ -1861,6 +1861,7 @@ static int body_ba_polish(tcv_ba_context* ctx, const tcv_ba_control* c, const do
   if (rep) {
     rep->refinement_steps = taken_newton;
     rep->direct_solves = (uint64_t)word(ctx, 0, S_DIRECT_SOLVES);
+    rep->held_points = (uint64_t)word(ctx, 0, S_HELD);
   }
   // the final cost; the initial words keep the polish input's
   hipLaunchKernelGGL(k_cost<false>, dim3(q.n_cost_blocks), dim3(COST_OBS), 0, st, q);
@@ -1882,11 +1883,16 @@ static int body_ba_polish(tcv_ba_context* ctx, const tcv_ba_control* c, const do
 
 static int body_ba_backward(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene,
                             const uint32_t* index, double log_focal, const double* cotangent,
-                            uint32_t max_cg, double tolerance, double* pixel_cotangent,
-                            double* adjoint, tcv_ba_report* rep) {
+                            uint32_t max_cg, double tolerance, uint32_t mode,
+                            double* pixel_cotangent, double* focal_cotangent, double* adjoint,
+                            tcv_ba_report* rep) {
   if (!ctx || !c || !scene || !index || !cotangent || !pixel_cotangent || max_cg < 1 ||
       max_cg > 100000)
     return TCV_BA_ERR_ARGUMENT;
+  if (mode != TCV_BA_ADJOINT_EXACT && mode != TCV_BA_ADJOINT_GAUSS_NEWTON) {
+    set_error("unknown adjoint mode");
+    return TCV_BA_ERR_ARGUMENT;
+  }
   if (rep) std::memset(rep, 0, sizeof *rep);
   const double t0 = now_ms();
   hipStream_t st = ctx->stream;
@@ -1896,6 +1902,7 @@ static int body_ba_backward(tcv_ba_context* ctx, const tcv_ba_control* c, const 
   TRY(prepare_exact(ctx, c, scene, index, log_focal, tolerance, st, fc, q, ops, rep));
   q.cg_budget = max_cg;
   q.cg_max = max_cg;
+  q.gauss_newton = mode == TCV_BA_ADJOINT_GAUSS_NEWTON;
   const size_t C = q.C, P = q.P, M = q.M, m = q.m;
   for (size_t k = 0; k < 12 * C + 3 * P + q.shared; k++)
     if (!std::isfinite(cotangent[k])) {
@@ -1904,7 +1911,8 @@ static int body_ba_backward(tcv_ba_context* ctx, const tcv_ba_control* c, const 
     }
   const std::vector<double> rhs = adjoint_rhs(c, scene, index, cotangent);
   TRY(ensure(ctx->rhs, rhs.size() * 8));
-  TRY(ensure(ctx->grad, 2 * M * 8));
+  // the pixel cotangent (2M), each observation's focal share (M), the per-camera sums (C)
+  TRY(ensure(ctx->grad, (3 * M + C) * 8));
   HIP_TRY(hipMemcpyAsync(ctx->rhs.ptr, rhs.data(), rhs.size() * 8, hipMemcpyHostToDevice, st));
   q.rhs_in = (const double*)ctx->rhs.ptr;
   const double ts = now_ms();
@@ -1925,9 +1933,15 @@ static int body_ba_backward(tcv_ba_context* ctx, const tcv_ba_control* c, const 
     return TCV_BA_ERR_SOLVE;
   }
   ops.solve_points(q, st);
-  ops.pixel_cotangent(q, (double*)ctx->grad.ptr, st);
+  double* grad = (double*)ctx->grad.ptr;
+  ops.pixel_cotangent(q, grad, focal_cotangent ? grad + 2 * M : nullptr, st);
+  if (focal_cotangent && C)
+    hipLaunchKernelGGL(k_focal_cotangent, dim3(C), dim3(256), 0, st, q, grad + 2 * M,
+                       grad + 3 * M);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipMemcpyAsync(pixel_cotangent, ctx->grad.ptr, 2 * M * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(pixel_cotangent, grad, 2 * M * 8, hipMemcpyDeviceToHost, st));
+  if (focal_cotangent)
+    HIP_TRY(hipMemcpyAsync(focal_cotangent, grad + 3 * M, C * 8, hipMemcpyDeviceToHost, st));
   if (adjoint) {
     HIP_TRY(hipMemcpyAsync(adjoint, q.vec, 6 * C * 8, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(adjoint + 6 * C, q.vec + 5 * m, 3 * P * 8, hipMemcpyDeviceToHost, st));
@@ -1940,6 +1954,7 @@ static int body_ba_backward(tcv_ba_context* ctx, const tcv_ba_control* c, const 
     rep->schur_ms = now_ms() - ts;
     rep->device_bytes = held_bytes(ctx);
     rep->direct_solves = (uint64_t)word(ctx, 0, S_DIRECT_SOLVES);
+    rep->held_points = (uint64_t)word(ctx, 0, S_HELD);
   }
   return TCV_BA_OK;
 }
@@ -2220,11 +2235,11 @@ int tcv_ba_polish(tcv_ba_context* ctx, const tcv_ba_control* c, const double* sc
 
 int tcv_ba_backward(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene,
                     const uint32_t* index, double log_focal, const double* cotangent,
-                    uint32_t max_cg, double tolerance, double* pixel_cotangent, double* adjoint,
-                    tcv_ba_report* rep) {
+                    uint32_t max_cg, double tolerance, uint32_t mode, double* pixel_cotangent,
+                    double* focal_cotangent, double* adjoint, tcv_ba_report* rep) {
   return locked(ctx, [&] {
-    return body_ba_backward(ctx, c, scene, index, log_focal, cotangent, max_cg, tolerance,
-                            pixel_cotangent, adjoint, rep);
+    return body_ba_backward(ctx, c, scene, index, log_focal, cotangent, max_cg, tolerance, mode,
+                            pixel_cotangent, focal_cotangent, adjoint, rep);
   });
 }
 

@@ -56,6 +56,7 @@ enum : int {
   S_STEP_LIMIT = 42,     // word 23 at which the running PCG pass stops (its start + the budget)
   S_PASS_STOP = 43,      // |r|^2 at which the running PCG pass has converged: tol^2 |r0|^2 for the
                          // first pass; a refinement restart's own (k_refine)
+  S_HELD = 44,           // exact-Hessian systems: points held fixed at the last PCG set-up
   S_WORDS = 48
 };
 
@@ -78,6 +79,10 @@ constexpr int GROUP_WORDS = 8;     // words of a PCG-step group record
 constexpr int CG_THREADS = 1024;   // threads of the per-bundle PCG kernels
 constexpr int COST_OBS = 256;      // observations per cost block
 constexpr int ASM_WORDS = 56;      // per-task camera assembly sums (see k_camera_pass)
+// exact-Hessian systems hold fixed a point whose block's smallest eigenvalue is below this fraction
+// of its largest (singular_to_rounding3); a well-triangulated point of a long trajectory sits
+// near 1e-5, one whose track has run off to 1e8 times the scene's depth near 1e-17
+constexpr double kHeldPoint = 1e-12;
 constexpr int DENSE_THREADS = 256; // threads of the direct route's row and factor workgroups
 constexpr int DENSE_TABLE = 8192;  // point ids of one pass of a row workgroup's lookup table
 constexpr int DENSE_NB = 32;       // block size of the dense Cholesky factorisation
@@ -155,6 +160,8 @@ struct Problem {
   const double* rhs_in;
   // exact-Hessian systems (polish, adjoint): regularise the one-fixed-camera similarity gauge
   uint32_t exact_gauge;
+  // the Gauss-Newton adjoint: the exact passes keep only the Huber curvature of each observation
+  uint32_t gauss_newton;
   // the stored cross blocks hold the exact Hessian's (the implicit operator relinearises the
   // Gauss-Newton blocks only, so it cannot certify such a system)
   uint32_t exact;

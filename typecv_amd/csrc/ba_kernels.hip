@@ -228,7 +228,9 @@ __host__ __device__ constexpr int up10(int j, int l) { return j * 10 - j * (j - 
 // weighted curvature of the projection; g . d2q/dtheta2 with g = rho_u du + rho_v dv, i.e.
 // 1/2 (g q^T + q g^T) - (g . q) I on omega-omega and (R_j x g) on omega-X_j; and rho . J on the
 // focal row and column (d pixel/d log f = pixel - c). Rows of a fixed camera stay zero.
-__host__ __device__ void exact_correction(const Lin& L, bool var, bool dir, double huber, double C[55]) {
+// `curvature` false (the Gauss-Newton adjoint): the Huber term -w a a^T only.
+__host__ __device__ void exact_correction(const Lin& L, bool var, bool dir, double huber, double C[55],
+                                          bool curvature = true) {
   for (int k = 0; k < 55; k++) C[k] = 0;
   double J[2][10];
   for (int k = 0; k < 6; k++) {
@@ -250,6 +252,7 @@ __host__ __device__ void exact_correction(const Lin& L, bool var, bool dir, doub
     for (int j = 0; j < 10; j++)
       for (int l = j; l < 10; l++) C[up10(j, l)] -= L.w * a[j] * a[l];
   }
+  if (!curvature) return;
   double G[9];
   projection_curvature(L.k, L.qc, rho0, rho1, G);
   // Q = dq/dtheta over omega, v, X (columns)
@@ -559,7 +562,9 @@ __global__ __launch_bounds__(COST_OBS) void k_finish_cost(Problem q) {
 // the reduced focal diagonal and RHS (bundleFocalPoints), summed per tile in point order.
 template <class WT, bool SHARED, bool EXACT>
 __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_pass(Problem q) {
-  constexpr int NS = SHARED ? 13 : 10;  // staged words per observation: V 6, b 3, [fc 3], bad
+  // staged words per observation: V 6, b 3, [fc 3], [exact systems: the Gauss-Newton V 6], bad
+  constexpr int GN = SHARED ? 12 : 9;
+  constexpr int NS = GN + (EXACT ? 6 : 0) + 1;
   __shared__ double st_all[TILE_WAVES][TILE][NS];
   __shared__ double fsum_all[TILE_WAVES][TILE][2];
   // the wave's index is uniform across its lanes: keep the tile's loads and branches scalar
@@ -581,6 +586,7 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_pass(Problem q) {
   const double lambda = sb[S_LAMBDA];
   const double fscale = SHARED ? focal_scale(sb[S_LOG_FOCAL]) : 1.0;
   double V[6] = {0, 0, 0, 0, 0, 0}, bp[3] = {0, 0, 0}, fc[3] = {0, 0, 0};
+  double Vg[EXACT ? 6 : 1] = {0};
   bool bad = false;
   const uint32_t K = SHARED ? 21 : 18;
   for (uint32_t base = s; base < e; base += TILE) {
@@ -595,7 +601,7 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_pass(Problem q) {
       if (ok) {
         cross_block(L, SHARED, W);
         if (EXACT) {
-          exact_correction(L, var, dir, q.huber, Cx);
+          exact_correction(L, var, dir, q.huber, Cx, !q.gauss_newton);
           for (int k = 0; k < 6; k++)
             for (int j = 0; j < 3; j++) W[k * 3 + j] += Cx[up10(k, 6 + j)];
           if (SHARED)
@@ -624,6 +630,8 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_pass(Problem q) {
         o[3] = a1 * L.Jp[1] + c1 * L.Jp[4];  // V11
         o[4] = a2 * L.Jp[1] + c2 * L.Jp[4];  // V21
         o[5] = a2 * L.Jp[2] + c2 * L.Jp[5];  // V22
+        if (EXACT)
+          for (int k = 0; k < 6; k++) o[GN + k] = o[k];
         o[6] = -(a0 * L.r[0]) - c0 * L.r[1];
         o[7] = -(a1 * L.r[0]) - c1 * L.r[1];
         o[8] = -(a2 * L.r[0]) - c2 * L.r[1];
@@ -656,12 +664,14 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_pass(Problem q) {
         for (int k = 0; k < 3; k++) bp[k] += o[6 + k];
         if (SHARED)
           for (int k = 0; k < 3; k++) fc[k] += o[9 + k];
+        if (EXACT)
+          for (int k = 0; k < 6; k++) Vg[k] += o[GN + k];
         bad = bad || o[NS - 1] != 0;
       }
     }
     wave_sync();
   }
-  bool failed = false;
+  bool failed = false, held = false;
   double dF = 0, dR = 0;
   if (has) {
     double inv[9];
@@ -673,6 +683,8 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_pass(Problem q) {
       A[0] += lambda * fmax(A[0], 1e-10);
       A[4] += lambda * fmax(A[4], 1e-10);
       A[8] += lambda * fmax(A[8], 1e-10);
+      double G[9] = {Vg[0], Vg[EXACT ? 1 : 0], Vg[EXACT ? 2 : 0], Vg[EXACT ? 1 : 0], Vg[EXACT ? 3 : 0],
+                     Vg[EXACT ? 4 : 0], Vg[EXACT ? 2 : 0], Vg[EXACT ? 4 : 0], Vg[EXACT ? 5 : 0]};
       if (q.exact_gauge && directional_point(g, my)) {
         // the radial direction of a directional point leaves every projection unchanged: a zero
         // row and column of the Hessian. Fix it with mu X X^T (X is unit after normalisation);
@@ -681,12 +693,23 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_pass(Problem q) {
         const double mu = fmax(fmax(A[0], A[4]), fmax(A[8], 1e-10));
         for (int j = 0; j < 3; j++)
           for (int k = 0; k < 3; k++) A[j * 3 + k] += mu * X[j] * X[k];
+        if (EXACT) {
+          const double mg = fmax(fmax(G[0], G[4]), fmax(G[8], 1e-10));
+          for (int j = 0; j < 3; j++)
+            for (int k = 0; k < 3; k++) G[j * 3 + k] += mg * X[j] * X[k];
+        }
       }
-      failed = !inverse3(A, inv);
+      // Undamped exact-Hessian systems (Newton polish, adjoint) hold fixed a point whose
+      // Gauss-Newton block sum w Jp^T Jp is singular to rounding: a track that has run off
+      // towards infinity, whose observations no longer depend on its position along the ray and
+      // which no finite step settles. Its zero block removes it from the reduced system; its
+      // observations still act on their cameras.
+      if (EXACT) held = q.exact_gauge && lambda == 0 && singular_to_rounding3(G, kHeldPoint);
+      if (!held) failed = !inverse3(A, inv);
     }
     double* out = q.pblk + (size_t)my * 12;
     double* pf = (SHARED && q.pfc) ? q.pfc + (size_t)my * 3 : nullptr;
-    if (failed) {
+    if (failed || held) {
       for (int k = 0; k < 12; k++) out[k] = 0;
       if (pf) pf[0] = pf[1] = pf[2] = 0;
     } else {
@@ -710,13 +733,15 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_pass(Problem q) {
     }
   }
   const unsigned long long any = __ballot(failed);
+  const uint32_t n_held = (uint32_t)__popcll(__ballot(held));
   if (SHARED) {
     fsum[lane][0] = dF;
     fsum[lane][1] = dR;
     wave_sync();
   }
   if (lane == 0) {
-    q.tile_fail[t] = any != 0ull;
+    // bit 0: a point block failed; above it, the points held fixed
+    q.tile_fail[t] = (any != 0ull ? 1u : 0u) | (n_held << 1);
     if (SHARED) {
       double f0 = 0, f1 = 0;
       for (uint32_t j = 0; j < ep - fp && j < (uint32_t)TILE; j++) {
@@ -763,7 +788,7 @@ __global__ __launch_bounds__(TASK_THREADS) void k_camera_pass(Problem q) {
     if (ok) {
       cross_block(L, SHARED, W);
       if (EXACT) {
-        exact_correction(L, var, dir, q.huber, Cx);
+        exact_correction(L, var, dir, q.huber, Cx, !q.gauss_newton);
         for (int k = 0; k < 6; k++)
           for (int j = 0; j < 3; j++) W[k * 3 + j] += Cx[up10(k, 6 + j)];
         if (SHARED)
@@ -971,17 +996,21 @@ __global__ __launch_bounds__(CG_THREADS) void k_init_cg(Problem q) {
     s[S_STEP_LIMIT] = pcg_budget(q, g);
   }
   if (active == 0) return;
-  double fail_tiles = 0, fail_cameras = 0;
-  for (uint32_t t = g.tile0 + tid; t < g.tile1; t += CG_THREADS)
-    fail_tiles += q.tile_fail[t] ? 1.0 : 0.0;
+  double fail_tiles = 0, fail_cameras = 0, held = 0;
+  for (uint32_t t = g.tile0 + tid; t < g.tile1; t += CG_THREADS) {
+    fail_tiles += (q.tile_fail[t] & 1u) ? 1.0 : 0.0;
+    held += (double)(q.tile_fail[t] >> 1);
+  }
   for (uint32_t c = g.cam0 + tid; c < g.cam1; c += CG_THREADS)
     if (!(q.camInv[(size_t)c * 36] > 0)) fail_cameras += 1;
   block_sum2(fail_tiles, fail_cameras, lds);
+  held = block_sum(held, lds);
   double fail = fail_tiles + fail_cameras;
   if (tid == 0) {
     s[S_FAIL_TILES] = fail_tiles;
     s[S_FAIL_CAMERAS] = fail_cameras;
     s[S_FAIL_FOCAL] = 0;
+    s[S_HELD] = held;
   }
   double hff = 0, bf = 0, tf = 0, tr = 0;
   if (SHARED) {
@@ -2095,9 +2124,12 @@ __global__ void k_newton_accept(Problem q) {
 // H lambda = v (cameras in x, the focal row in x, points in y),
 //   d<v, theta*>/d u_i = D_i J_i lambda,
 // D_i = d(w r)/dr at the observation (the implicit function theorem on sum_i J_i^T w_i r_i = 0 with
-// r_i = pixel_i(theta) - u_i). An observation that does not project gets 0.
+// r_i = pixel_i(theta) - u_i). An observation that does not project gets 0. `focal` (optional, M):
+// the observation's share of the input-focal cotangent, -(D_i J_i lambda) . Jf_i - (J_i lambda) .
+// (w_i r_i) (the second term in the exact mode; d J_i / d log f = J_i). Jf_i = pixel_i - (cx, cy)
+// is the reference's d(pixel)/d(log scale) (src/sfm/bundle/joint-focal.ts:150-154).
 template <bool SHARED>
-__global__ void k_pixel_cotangent(Problem q, double* out) {
+__global__ void k_pixel_cotangent(Problem q, double* out, double* focal) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= q.M) return;
   const uint32_t c = obs_camera(q)[i], p = obs_point(q)[i];
@@ -2106,7 +2138,7 @@ __global__ void k_pixel_cotangent(Problem q, double* out) {
   const Bundle& g = q.bundles[b];
   const double fscale = SHARED ? focal_scale(sb[S_LOG_FOCAL]) : 1.0;
   Lin L;
-  double t0 = 0, t1 = 0;
+  double t0 = 0, t1 = 0, share = 0;
   if (linearize(q, c, p, i, fscale, variable_camera(q, c), directional_point(g, p), L)) {
     const double* x = vx(q);
     const double* y = vy(q) + (size_t)p * 3;
@@ -2122,15 +2154,32 @@ __global__ void k_pixel_cotangent(Problem q, double* out) {
       t0 += L.Jf[0] * x[g.focal_row];
       t1 += L.Jf[1] * x[g.focal_row];
     }
+    const double j0 = t0, j1 = t1;  // J_i lambda
     const double e = sqrt(L.r[0] * L.r[0] + L.r[1] * L.r[1]);
     if (e > q.huber) {
       const double h0 = L.r[0] / e, h1 = L.r[1] / e, d = h0 * t0 + h1 * t1;
       t0 = L.w * (t0 - h0 * d);
       t1 = L.w * (t1 - h1 * d);
     }
+    share = -(t0 * L.Jf[0] + t1 * L.Jf[1]);
+    if (!q.gauss_newton) share -= j0 * (L.w * L.r[0]) + j1 * (L.w * L.r[1]);
   }
   out[2 * (size_t)i] = t0;
   out[2 * (size_t)i + 1] = t1;
+  if (focal) focal[i] = share;
+}
+
+// The input-focal cotangent of each camera: the shares of its observations summed in its list's
+// order (one workgroup per camera; strided partial sums, then the workgroup's fixed-order sum).
+__global__ __launch_bounds__(256) void k_focal_cotangent(Problem q, const double* share,
+                                                         double* out) {
+  __shared__ double lds[4];
+  const uint32_t c = blockIdx.x;
+  const uint32_t s = camera_start(q)[c], e = camera_start(q)[c + 1];
+  double sum = 0;
+  for (uint32_t n = s + threadIdx.x; n < e; n += blockDim.x) sum += share[camera_obs(q)[n]];
+  sum = block_sum(sum, lds);
+  if (threadIdx.x == 0) out[c] = sum;
 }
 
 // The per-observation linearisation (the Jacobian kernels' output): 23 words per observation,

@@ -137,10 +137,29 @@ __device__ __forceinline__ bool inverse3(const double V[9], double inv[9]) {
   return true;
 }
 
+// True when a symmetric positive semi-definite 3x3 block (row-major) is singular to rounding: its
+// smallest eigenvalue e0 below `tau` times its largest e2. With tr the trace, m2 the sum of the
+// principal 2x2 minors and e0 <= e1 <= e2: e1 e2 <= m2 <= 3 e1 e2 and e2 <= tr <= 3 e2, so
+// det / (m2 tr) lies in [e0 / (9 e2), e0 / e2] and m2 / tr^2 in [e1 / (9 e2), 3 e1 / e2]; the
+// second test catches two weak directions, where det is below its own rounding.
+__device__ __forceinline__ bool singular_to_rounding3(const double A[9], double tau) {
+  const double tr = A[0] + A[4] + A[8];
+  const double m00 = A[4] * A[8] - A[5] * A[7], m11 = A[0] * A[8] - A[2] * A[6],
+               m22 = A[0] * A[4] - A[1] * A[3];
+  const double m2 = m00 + m11 + m22;
+  const double det = A[0] * m00 - A[1] * (A[3] * A[8] - A[5] * A[6]) + A[2] * (A[3] * A[7] - A[4] * A[6]);
+  return tr > 0 && (fabs(m2) <= tau * tr * tr || fabs(det) <= tau * fabs(m2) * tr);
+}
+
 // Lower Cholesky factor of a symmetric positive definite 3x3 block (row-major V): V = L L^T with
 // L packed as l00, l10, l11, l20, l21, l22. The direct route splits V_p^-1 = L L^T so that each
 // observation's cross block becomes one factor G = W L of the product W V^-1 W^T = G G^T.
 __device__ __forceinline__ void lower_cholesky3(const double V[9], double L[6]) {
+  if (!(V[0] > 0)) {
+    // a held point's zero block (see singular_to_rounding3): a zero factor
+    for (int k = 0; k < 6; k++) L[k] = 0;
+    return;
+  }
   L[0] = sqrt(V[0]);
   L[1] = V[3] / L[0];
   L[2] = sqrt(V[4] - L[1] * L[1]);

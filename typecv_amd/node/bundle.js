@@ -97,11 +97,19 @@ function solveBatchPacked(device, controls, scenes, indices) {
   });
 }
 
-/* The backward entry: pixel cotangents (2M) of a cotangent on the solve's outputs. */
+/* The backward entry (tcv_ba_backward): for a cotangent on the solve's outputs, the pixel
+ * cotangents (2M) and the input-focal cotangent of each camera (C); mode 'exact' or
+ * 'gaussNewton'. */
 function backwardPacked(device, control, scene, index, logFocal, cotangent,
-  maxCg = 4000, tolerance = 1e-10) {
-  return device.enqueue(() => addon.backward(device.native, control, scene, index, logFocal,
-    cotangent, maxCg, tolerance));
+  maxCg = 4000, tolerance = 1e-10, mode = 'exact') {
+  const code = { exact: 0, gaussNewton: 1 }[mode];
+  if (code === undefined) throw new RangeError(`adjoint mode must be 'exact' or 'gaussNewton'`);
+  const M = control[3];
+  return device.enqueue(async () => {
+    const values = await addon.backward(device.native, control, scene, index, logFocal,
+      cotangent, maxCg, tolerance, code);
+    return { pixels: values.subarray(0, 2 * M), focal: values.subarray(2 * M) };
+  });
 }
 
 /* The Newton polish of a solve to a stationary point (tcv_ba_polish): the solve's output layout. */

@@ -15,12 +15,16 @@
  * third after both, and writes <out prefix>.0.f64, .1.f64 and .seq.f64;
  *   node run.js --polish <control.f64> <scene.f64> <index.u32> <logFocal> <out.f64>
  *   node run.js --jacobian <control.f64> <scene.f64> <index.u32> <logFocal> <out.f64>
- * run tcv_ba_polish (30 damped steps, 3 Newton steps, PCG <= 4000 to 1e-12) or tcv_ba_jacobian.
+ * run tcv_ba_polish (30 damped steps, 3 Newton steps, PCG <= 4000 to 1e-12) or tcv_ba_jacobian;
+ *   node run.js --backward <control.f64> <scene.f64> <index.u32> <logFocal> <cotangent.f64>
+ *       <exact|gaussNewton> <out prefix>
+ * runs tcv_ba_backward (PCG <= 4000 to 1e-10) and writes <out prefix>.pixels.f64 and
+ * <out prefix>.focal.f64.
  * `node run.js --exports` prints the addon's exported names (no device is touched).
  */
 const fs = require('fs');
-const { HipDevice, solvePacked, solveBatchPacked, polishPacked, jacobianPacked, evaluatePacked,
-  addon } = require('./bundle.js');
+const { HipDevice, solvePacked, solveBatchPacked, polishPacked, jacobianPacked, backwardPacked,
+  evaluatePacked, addon } = require('./bundle.js');
 
 function read(file, Type) {
   const bytes = fs.readFileSync(file);
@@ -80,6 +84,22 @@ async function main(argv) {
         : await jacobianPacked(device, control, scene, index, Number(logFocal));
       fs.writeFileSync(outFile, Buffer.from(values.buffer));
       console.log(JSON.stringify({ length: values.length }));
+    } finally {
+      device.dispose();
+    }
+    return 0;
+  }
+  if (argv[0] === '--backward') {
+    const [, controlFile, sceneFile, indexFile, logFocal, cotangentFile, mode, prefix] = argv;
+    const control = read(controlFile, Float64Array);
+    const scene = read(sceneFile, Float64Array), index = read(indexFile, Uint32Array);
+    const device = new HipDevice(0);
+    try {
+      const r = await backwardPacked(device, control, scene, index, Number(logFocal),
+        read(cotangentFile, Float64Array), 4000, 1e-10, mode);
+      fs.writeFileSync(`${prefix}.pixels.f64`, Buffer.from(r.pixels.slice().buffer));
+      fs.writeFileSync(`${prefix}.focal.f64`, Buffer.from(r.focal.slice().buffer));
+      console.log(JSON.stringify({ pixels: r.pixels.length, focal: r.focal.length }));
     } finally {
       device.dispose();
     }

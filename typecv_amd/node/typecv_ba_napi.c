@@ -13,8 +13,11 @@
  *   deviceBytes(context), release(context)         -> tcv_ba_device_bytes, tcv_ba_release
  *   solve(context, control, scene, index)          -> Promise<Float64Array>
  *   setWorkingBytes(context, bytes)                -> tcv_ba_set_working_bytes
- *   backward(context, control, scene, index, logFocal, cotangent, maxCg, tolerance)
- *                                                  -> Promise<Float64Array(2M)>
+ *   backward(context, control, scene, index, logFocal, cotangent, maxCg, tolerance[, mode])
+ *                                                  -> Promise<Float64Array(2M + C)> (tcv_ba_backward:
+ *                                                     the pixel cotangent, then the input-focal
+ *                                                     cotangent of each camera; mode 0 exact,
+ *                                                     1 Gauss-Newton)
  *   polish(context, control, scene, index, logFocal, lmIterations, newtonSteps, maxCg, tolerance)
  *                                                  -> Promise<Float64Array> (tcv_ba_polish: the
  *                                                     solve's output layout)
@@ -327,7 +330,7 @@ typedef struct {
   const uint32_t* index;
   const double* cotangent;
   double log_focal, tolerance;
-  uint32_t max_cg, lm_iterations, newton_steps;
+  uint32_t max_cg, lm_iterations, newton_steps, mode;
   double* out;
   size_t out_len;
   tcv_ba_report report;
@@ -342,7 +345,8 @@ static void execute(napi_env env, void* data) {
     j->status = tcv_ba_solve(j->ctx, &j->control, j->scene, j->index, j->out, &j->report);
   else if (j->kind == JOB_BACKWARD)
     j->status = tcv_ba_backward(j->ctx, &j->control, j->scene, j->index, j->log_focal,
-                                j->cotangent, j->max_cg, j->tolerance, j->out, NULL, &j->report);
+                                j->cotangent, j->max_cg, j->tolerance, j->mode, j->out,
+                                j->out + 2 * (size_t)j->control.observations, NULL, &j->report);
   else if (j->kind == JOB_POLISH)
     j->status = tcv_ba_polish(j->ctx, &j->control, j->scene, j->index, j->log_focal,
                               j->lm_iterations, j->newton_steps, j->max_cg, j->tolerance, j->out,
@@ -438,32 +442,34 @@ static napi_value solve(napi_env env, napi_callback_info info) {
 }
 
 static napi_value backward(napi_env env, napi_callback_info info) {
-  size_t argc = 8;
-  napi_value argv[8];
+  size_t argc = 9;
+  napi_value argv[9];
   CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
   Job* j = job_of(env, argv, argc, 8, JOB_BACKWARD);
   if (!j) return NULL;
   size_t nc = 0;
-  double max_cg = 0;
+  double max_cg = 0, mode = 0;
   if (napi_get_value_double(env, argv[4], &j->log_focal) != napi_ok ||
       !(j->cotangent = typed_data(env, argv[5], napi_float64_array, &nc)) ||
       napi_get_value_double(env, argv[6], &max_cg) != napi_ok ||
-      napi_get_value_double(env, argv[7], &j->tolerance) != napi_ok) {
+      napi_get_value_double(env, argv[7], &j->tolerance) != napi_ok ||
+      (argc > 8 && napi_get_value_double(env, argv[8], &mode) != napi_ok)) {
     for (int k = 0; k < j->nrefs; k++) napi_delete_reference(env, j->refs[k]);
     free(j);
     return NULL;
   }
   const size_t want = (size_t)j->control.cameras * 12 + (size_t)j->control.points * 3 +
                       (j->control.shared_focal ? 1 : 0);
-  if (nc != want || !(max_cg >= 1 && max_cg <= 100000)) {
+  if (nc != want || !(max_cg >= 1 && max_cg <= 100000) || !(mode == 0 || mode == 1)) {
     for (int k = 0; k < j->nrefs; k++) napi_delete_reference(env, j->refs[k]);
     free(j);
-    napi_throw_range_error(env, NULL, "cotangent length or maxCg out of range");
+    napi_throw_range_error(env, NULL, "cotangent length, maxCg or mode out of range");
     return NULL;
   }
   j->max_cg = (uint32_t)max_cg;
+  j->mode = (uint32_t)mode;
   napi_create_reference(env, argv[5], 1, &j->refs[j->nrefs++]);
-  j->out_len = 2 * (size_t)j->control.observations;
+  j->out_len = 2 * (size_t)j->control.observations + j->control.cameras;
   j->out = malloc(j->out_len * sizeof(double));
   return queue(env, j, "typecv_ba.backward");
 }
