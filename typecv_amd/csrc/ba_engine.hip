@@ -218,11 +218,8 @@ constexpr uint32_t kTaskObs = TASK_OBS;
 constexpr uint32_t kChunk = 8;  // PCG iterations between reads of the state words
 constexpr uint32_t kTimingStride = 8;  // TCV_BA_FLAG_TIMING times every 8th Schur product
 constexpr uint32_t kMaxRefinements = 4;  // mixed form: refinement restarts per LM iteration
-#ifndef TCV_DIRECT_PASSES
-#define TCV_DIRECT_PASSES 3
-#endif
-constexpr uint32_t kDirectPasses = TCV_DIRECT_PASSES;  // certificates of a direct solution per LM
-                                                        // iteration (corrections in between)
+constexpr uint32_t kDirectPasses = 3;  // certificates of a direct solution per LM iteration
+                                       // (corrections in between)
 
 // Bundle ranges, wave tiles of whole points of one bundle (at most 64 observations and 64 points;
 // a longer point alone), camera-list tasks of at most kTaskObs slots (each camera has at least

@@ -60,17 +60,8 @@ enum : int {
   S_WORDS = 48
 };
 
-constexpr int TILE = 64;  // observations per wave tile
-#ifndef TCV_TILE_WAVES
-#define TCV_TILE_WAVES 4
-#endif
-#ifndef TCV_W_BLOCKED
-#define TCV_W_BLOCKED 1
-#endif
-#ifndef TCV_W_NONTEMPORAL
-#define TCV_W_NONTEMPORAL 1
-#endif
-constexpr int TILE_WAVES = TCV_TILE_WAVES;  // wave tiles per workgroup of the point-tile kernels
+constexpr int TILE = 64;        // observations per wave tile
+constexpr int TILE_WAVES = 4;   // wave tiles per workgroup of the point-tile kernels
 constexpr int TASK_THREADS = 256;  // threads of a camera-task workgroup
 constexpr int TASK_OBS = 1024;     // most slots of a camera-list task (4 per thread)
 constexpr int ADV_THREADS = 256;   // threads of a PCG-step workgroup

@@ -91,28 +91,13 @@ __device__ __forceinline__ void block_sum2(double& a, double& b, double* lds) {
 // share that XCD's L2, where the vectors they gather (y by point in the camera sweep, a bundle's G
 // rows in the dense assembly) stay resident.
 __device__ __forceinline__ uint32_t xcd_contiguous(uint32_t i, uint32_t n) {
-#ifndef TCV_XCD_REMAP
-#define TCV_XCD_REMAP 1
-#endif
-#if TCV_XCD_REMAP
   const uint32_t q = n / 8, r = n % 8, x = i % 8, k = i / 8;
   return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
-#else
-  (void)n;
-  return i;
-#endif
 }
 
 // The wave tile of a wave of a point-tile kernel (TILE_WAVES tiles per workgroup).
 __device__ __forceinline__ uint32_t point_tile(uint32_t wave) {
-#ifndef TCV_POINT_XCD
-#define TCV_POINT_XCD 1
-#endif
-#if TCV_POINT_XCD
   return xcd_contiguous(blockIdx.x, gridDim.x) * TILE_WAVES + wave;
-#else
-  return blockIdx.x * TILE_WAVES + wave;
-#endif
 }
 
 // Marks a loaded value as used here: the load is issued (and waited for) before what follows, so a
@@ -136,11 +121,7 @@ __device__ __forceinline__ void wave_sync() {
 // to the gathered vectors.
 template <class WT>
 __device__ __forceinline__ double wload(const WT* p) {
-#if TCV_W_NONTEMPORAL
   return (double)__builtin_nontemporal_load(p);
-#else
-  return (double)*p;
-#endif
 }
 
 // Linearisation of one observation (linearizeBundle b:251-297 / j:287-336).
@@ -325,12 +306,8 @@ __device__ __forceinline__ void relinearised_w(const Problem& q, const Bundle& g
 template <class WT, bool SHARED>
 __device__ __forceinline__ WT* wptr(void* base, uint32_t k, uint32_t i, uint32_t M) {
   constexpr uint32_t K = SHARED ? 21 : 18;
-#if TCV_W_BLOCKED
   (void)M;
   return static_cast<WT*>(base) + ((size_t)(i >> 6) * K + k) * 64 + (i & 63);
-#else
-  return static_cast<WT*>(base) + (size_t)k * M + i;
-#endif
 }
 
 // The mixed form stores each observation's cross block as FP32 Jacobian factors (ba_factored.hip):
