@@ -143,3 +143,95 @@ def test_node_addon_loads_and_exports_the_operator():
     assert json.loads(out.stdout) == ["backward", "deviceBytes", "evaluate", "jacobian", "open",
                                        "outputSize", "polish", "release", "setWorkingBytes",
                                        "solve", "solveBatch"]
+
+
+# ------------------------------------------------------------------ AddressSanitizer (host code)
+
+def asan_runtime():
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        return None
+    path = subprocess.run(["gcc", "-print-file-name=libasan.so"], capture_output=True,
+                          text=True).stdout.strip()
+    return path if path and Path(path).is_file() else None
+
+
+ASAN_FLAGS = ["-fsanitize=address", "-fno-omit-frame-pointer", "-g", "-O1"]
+
+
+@pytest.mark.parametrize("shared", [False, True])
+def test_oracle_under_address_sanitizer(tmp_path, shared):
+    """The oracle's solve, chart gradient and dense solve built with -fsanitize=address (leak
+    checking on) run a fixture clean and return the regular build's arrays."""
+    import os
+    import subprocess
+    from oracle import oracle as O
+    from problems import packed_problem
+    if asan_runtime() is None:
+        pytest.skip("no AddressSanitizer runtime for gcc")
+    exe = tmp_path / "oracle_asan"
+    # the runtime linked statically, so that it comes first whatever the environment preloads
+    build = subprocess.run(["gcc", *ASAN_FLAGS, "-static-libasan", "-std=c11", "-fopenmp",
+                            "-ffp-contract=off", "-D_POSIX_C_SOURCE=199309L",
+                            "-I", str(ROOT / "include"),
+                            str(ROOT / "tests" / "asan" / "oracle_driver.c"),
+                            str(ROOT / "oracle" / "ba_oracle.c"), "-o", str(exe), "-lm"],
+                           capture_output=True, text=True)
+    assert build.returncode == 0, build.stderr
+    if shared:
+        model = S.joint_focal_fixture()["input"]
+        options = {"iterations": 12, "refineSharedFocal": True, "maxCgIterations": 64}
+    else:
+        model, options = S.bundle_fixture(outlier=True), {"iterations": 12, "huberPx": 1}
+    packed, control = packed_problem(model, options)
+    (tmp_path / "control.bin").write_bytes(bytes(control))
+    packed.scene.astype(np.float64).tofile(tmp_path / "scene.f64")
+    packed.index.astype(np.uint32).tofile(tmp_path / "index.u32")
+    run = subprocess.run([str(exe), str(tmp_path / "control.bin"), str(tmp_path / "scene.f64"),
+                          str(tmp_path / "index.u32"), str(tmp_path / "out")],
+                         capture_output=True, text=True, timeout=300,
+                         env=dict(os.environ, ASAN_OPTIONS="detect_leaks=1:halt_on_error=1",
+                                  OMP_NUM_THREADS="2"))
+    assert run.returncode == 0, run.stdout + run.stderr
+    assert "AddressSanitizer" not in run.stderr
+    out = np.fromfile(tmp_path / "out.solve.f64")
+    want, _ = O.solve(control, packed)
+    assert np.array_equal(out, want)
+    grad = np.fromfile(tmp_path / "out.gradient.f64")
+    assert np.array_equal(grad, O.gradient(control, packed.scene, packed.index)[0])
+    dense = np.fromfile(tmp_path / "out.dense.f64")
+    assert np.array_equal(dense, O.dense_solve(control, packed))
+
+
+def test_addon_host_code_under_address_sanitizer(tmp_path):
+    """The Node-API addon built with -fsanitize=address: every export's argument checks (valid and
+    invalid controls, wrong types and lengths, a non-context, opening a device) run clean in Node."""
+    import json
+    import os
+    import shutil
+    import subprocess
+    from typecv_amd.node import build as addon_build
+    runtime = asan_runtime()
+    include = addon_build.node_include()
+    if runtime is None or include is None or shutil.which("node") is None:
+        pytest.skip("no AddressSanitizer runtime, Node or Node-API headers")
+    addon = tmp_path / "typecv_ba.node"
+    package = ROOT / "typecv_amd"
+    build = subprocess.run(["gcc", *ASAN_FLAGS, "-std=c11", "-fPIC", "-shared",
+                            "-DNODE_GYP_MODULE_NAME=typecv_ba", "-I", str(include),
+                            "-I", str(ROOT / "include"), str(package / "node" / "typecv_ba_napi.c"),
+                            "-L", str(package), "-ltypecv_ba", f"-Wl,-rpath,{package}",
+                            "-o", str(addon)], capture_output=True, text=True)
+    assert build.returncode == 0, build.stderr
+    # the runtime goes first in the preload list (Node itself is not instrumented), ahead of
+    # whatever the environment already preloads; Node's own allocations are not leak-checked
+    preload = " ".join(x for x in (runtime, os.environ.get("LD_PRELOAD", "")) if x)
+    env = dict(os.environ, LD_PRELOAD=preload, ASAN_OPTIONS="detect_leaks=0:halt_on_error=1")
+    run = subprocess.run(["node", str(ROOT / "tests" / "asan" / "addon_checks.js"), str(addon)],
+                         capture_output=True, text=True, timeout=120, env=env)
+    assert run.returncode == 0, run.stdout + run.stderr
+    assert "AddressSanitizer" not in run.stderr
+    result = json.loads(run.stdout.strip().splitlines()[-1])
+    assert result["outputSize"] == 120 + 300 + 27
+    assert all(result["threw"].values()), result["threw"]
