@@ -36,12 +36,6 @@ HBM_PEAK_GBS = 8000.0
 # factors that make a 1 GiB device copy read and write 1 GiB. Keyed by
 # (cameras, points, seed, precision, operator); other workloads report null.
 MEASURED_TRAFFIC = {
-    # the fused product (k_point_sweep_fused + k_camera_tsum), the schedule tuner's pick here
-    (500, 200_000, 2, "fp64", "stored"): 2 * (180051.5 + 71652.3) * 1024 + (118175.7 + 142.2) * 1024,
-    # the plain sweeps (k_point_sweep + k_camera_sweep), the tuner's pick here
-    (2000, 1_000_000, 2, "fp64", "stored"): 2 * (910598.6 + 887339.6) * 1024 + (23442.2 + 625.7) * 1024,
-    # the factored sweeps of the mixed form
-    (2000, 1_000_000, 2, "mixed", "stored"): 2 * (309329.6 + 275035.2) * 1024 + (23437.7 + 625.3) * 1024,
 }
 
 
@@ -65,6 +59,17 @@ def matvec_bytes(C: int, P: int, M: int, shared: bool, factored: bool = False) -
     if shared:
         total += C * 6 * 8 * 2 + 24
     return total
+
+
+def onepass_bytes(C: int, P: int, M: int, shared: bool, factored: bool, rows: int) -> int:
+    """Algorithmic HBM bytes of one one-pass Schur product (k_schur_onepass + k_onepass_reduce):
+    every observation's stored words read once (FP64 cross blocks: 21 doubles shared, 18 fixed;
+    the mixed form's FP32 factors: 11 or 9 floats), its camera and point ids; V^-1 and the point
+    offsets; the camera vectors p read, the task partials written, and, for the factors, each
+    camera's rotation; the rows of camera partials written once and read once (8 doubles each)."""
+    per_obs = (11 if shared else 9) * 4 if factored else (21 if shared else 18) * 8
+    per_cam = 6 * 8 + 8 * 8 + (9 * 8 if factored else 0)
+    return M * (per_obs + 8) + P * (9 * 8 + 4) + C * per_cam + rows * 8 * 8 * 2 + (8 if shared else 0)
 
 
 def solve_forms(args, B, _native, lib, device, packed, scene, index, out, precision, steps,
@@ -108,7 +113,10 @@ def form_record(packed, seed, precision, operator, reports, elapsed, values):
     lm = sum(r["lm_iterations"] for r in reports)
     cg = sum(r["cg_iterations"] for r in reports)
     timed = sum(r["timed_matvecs"] for r in reports)  # the engine times every 8th product
-    bytes_per = matvec_bytes(C, P, M, True, factored=precision == "mixed")
+    rows = reports[-1].get("onepass_rows", 0)
+    onepass = rows > 0
+    bytes_per = (onepass_bytes(C, P, M, True, precision == "mixed", rows) if onepass else
+                 matvec_bytes(C, P, M, True, factored=precision == "mixed"))
     mv_ms = sum(r["schur_ms"] for r in reports) / max(1, timed)
     achieved = bytes_per / (mv_ms * 1e-3) / 1e9 if mv_ms > 0 else 0.0
     base = C * 12 + P * 3
@@ -127,7 +135,7 @@ def form_record(packed, seed, precision, operator, reports, elapsed, values):
         "schur_matvec": {
             "ms": mv_ms, "point_sweep_ms": sum(r["schur_point_ms"] for r in reports) / max(1, timed),
             "camera_sweep_ms": sum(r["schur_camera_ms"] for r in reports) / max(1, timed),
-            "bytes": bytes_per, "timed_products": timed,
+            "bytes": bytes_per, "timed_products": timed, "onepass_rows": rows,
             "share_of_solve": mv_ms * sum(r["matvecs"] for r in reports)
             / max(1e-9, sum(r["solve_ms"] for r in reports)),
             # camera-sweep workgroups per CU picked by the context's schedule tuner (0: no cap)
@@ -138,9 +146,11 @@ def form_record(packed, seed, precision, operator, reports, elapsed, values):
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": MEASURED_TRAFFIC.get((C, P, seed, precision, operator)),
+            "traffic": MEASURED_TRAFFIC.get((C, P, seed, precision,
+                                              "onepass" if onepass else operator)),
             "algorithmic_bytes": bytes_per,
-            "kernel": ("Schur matvec: k_point_sweep_factored + k_camera_sweep_factored"
+            "kernel": ("Schur product: k_schur_onepass + k_onepass_reduce" if onepass else
+                       "Schur matvec: k_point_sweep_factored + k_camera_sweep_factored"
                        if precision == "mixed" else
                        "Schur matvec: k_point_sweep_fused + k_camera_tsum" if fused else
                        "Schur matvec: k_point_sweep + k_camera_sweep"),

@@ -140,6 +140,8 @@ typedef struct tcv_ba_report {
   uint64_t held_points;        /* tcv_ba_polish, tcv_ba_backward: points held fixed by the last
                                   exact-Hessian system, their blocks singular to rounding (tracks
                                   run off towards infinity); zero adjoint rows */
+  uint64_t onepass_rows;       /* the one-pass Schur product's rows of camera partials (the sum of
+                                  its workgroups' camera windows); 0: the two sweeps ran */
 } tcv_ba_report;
 
 typedef struct tcv_ba_context tcv_ba_context;

@@ -99,6 +99,7 @@ class Report(ctypes.Structure):
         ("fused_product", ctypes.c_uint64),
         ("streamed", ctypes.c_uint64),
         ("held_points", ctypes.c_uint64),
+        ("onepass_rows", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:

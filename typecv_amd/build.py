@@ -15,7 +15,8 @@ ROOT = PACKAGE.parent
 CSRC = PACKAGE / "csrc"
 LIBRARY = PACKAGE / "libtypecv_ba.so"
 SOURCES = [CSRC / "ba_engine.hip"]
-DEPENDS = [CSRC / "ba_kernels.hip", CSRC / "ba_factored.hip", CSRC / "ba_dense.hip",
+DEPENDS = [CSRC / "ba_kernels.hip", CSRC / "ba_factored.hip", CSRC / "ba_onepass.hip",
+           CSRC / "ba_dense.hip",
            CSRC / "obs_eval.hip", CSRC / "ba_kernels.h", CSRC / "ba_math.h",
            ROOT / "include" / "typecv_ba.h"]
 

@@ -23,6 +23,7 @@
 #include "../../include/typecv_ba.h"
 #include "ba_kernels.hip"
 #include "ba_factored.hip"
+#include "ba_onepass.hip"
 #include "ba_dense.hip"
 #include "obs_eval.hip"
 
@@ -62,7 +63,7 @@ struct tcv_ba_context {
       rhs, grad, camS, camG, pfc, dense, direct_bundles, direct_cams, covis, small_bundles,
       tprod,
       obs_cameras, obs_points, obs_pixels, obs_indices, obs_offsets, obs_flags, obs_errors,
-      obs_partials, obs_summary;
+      obs_partials, obs_summary, op_tile, op_lo, op_off, op_cam, op_row, op_part;
   // the direct bundles factored by a sequence of launches, and their dense array sizes
   std::vector<uint32_t> large_bundles, large_n;
   std::vector<uint64_t> large_off;
@@ -86,7 +87,8 @@ struct tcv_ba_context {
            &groups,     &group_part,   &rhs,       &grad,      &camS,       &camG,   &pfc,
            &dense,      &direct_bundles, &direct_cams, &covis, &small_bundles, &tprod,
            &obs_cameras, &obs_points, &obs_pixels, &obs_indices, &obs_offsets, &obs_flags,
-           &obs_errors, &obs_partials, &obs_summary};
+           &obs_errors, &obs_partials, &obs_summary, &op_tile, &op_lo, &op_off, &op_cam, &op_row,
+           &op_part};
   }
 };
 
@@ -365,7 +367,10 @@ struct Launch {
     hipLaunchKernelGGL((k_init_cg<SHARED>), dim3(q.B), dim3(CG_THREADS), 0, s, q);
   }
   static void point_sweep(const Problem& q, hipStream_t s) {
-    if constexpr (sizeof(WT) == 4)
+    if (q.n_op)
+      hipLaunchKernelGGL((k_schur_onepass<WT, SHARED>), dim3(q.n_op), dim3(TILE * TILE_WAVES), 0,
+                         s, q);
+    else if constexpr (sizeof(WT) == 4)
       hipLaunchKernelGGL((k_point_sweep_factored<SHARED, SWEEP_SCHUR>), dim3(tile_blocks(q)),
                          dim3(TILE * TILE_WAVES), 0, s, q);
     else if (q.tprod)
@@ -376,7 +381,10 @@ struct Launch {
                          dim3(TILE * TILE_WAVES), 0, s, q);
   }
   static void camera_sweep(const Problem& q, hipStream_t s) {
-    if constexpr (sizeof(WT) == 4)
+    if (q.n_op)
+      hipLaunchKernelGGL((k_onepass_reduce<SHARED>), dim3(blocks_for((size_t)q.C * 64, 256)),
+                         dim3(256), 0, s, q);
+    else if constexpr (sizeof(WT) == 4)
       hipLaunchKernelGGL((k_camera_sweep_factored<SHARED>), dim3(q.n_tasks), dim3(TASK_THREADS),
                          q.camera_lds, s, q);
     else if (q.tprod)
@@ -401,7 +409,10 @@ struct Launch {
   }
   // The streamed route: the same sweeps with every cross block relinearised in registers.
   static void streamed_point_sweep(const Problem& q, hipStream_t s) {
-    if constexpr (sizeof(WT) == 4)
+    if (q.n_op)
+      hipLaunchKernelGGL((k_schur_onepass<WT, SHARED, true>), dim3(q.n_op),
+                         dim3(TILE * TILE_WAVES), 0, s, q);
+    else if constexpr (sizeof(WT) == 4)
       hipLaunchKernelGGL((k_point_sweep_factored<SHARED, SWEEP_SCHUR, true>), dim3(tile_blocks(q)),
                          dim3(TILE * TILE_WAVES), 0, s, q);
     else
@@ -409,7 +420,10 @@ struct Launch {
                          dim3(TILE * TILE_WAVES), 0, s, q);
   }
   static void streamed_camera_sweep(const Problem& q, hipStream_t s) {
-    if constexpr (sizeof(WT) == 4)
+    if (q.n_op)
+      hipLaunchKernelGGL((k_onepass_reduce<SHARED>), dim3(blocks_for((size_t)q.C * 64, 256)),
+                         dim3(256), 0, s, q);
+    else if constexpr (sizeof(WT) == 4)
       hipLaunchKernelGGL((k_camera_sweep_factored<SHARED, true>), dim3(q.n_tasks),
                          dim3(TASK_THREADS), 0, s, q);
     else
@@ -706,6 +720,17 @@ std::vector<Need> solve_needs(tcv_ba_context* ctx, const tcv_ba_control* c, uint
     // 40.0 us at 500 cameras), since the rows stay FP64 while the blocks it saves are halved
     if (ws == 8) n.push_back({&ctx->tprod, (size_t)M * 64});
   }
+  if (!exact) {
+    // the one-pass product's workgroups (at most OP_TILES tiles of one bundle each), windows, row
+    // tables and partial rows (at most OP_SPAN per workgroup)
+    const size_t n_op = (n_tiles + OP_TILES - 1) / OP_TILES + B;
+    n.push_back({&ctx->op_tile, (n_op + 1) * 4});
+    n.push_back({&ctx->op_lo, n_op * 4});
+    n.push_back({&ctx->op_off, (n_op + 1) * 4});
+    n.push_back({&ctx->op_cam, ((size_t)C + 1) * 4});
+    n.push_back({&ctx->op_row, n_op * OP_SPAN * 4});
+    n.push_back({&ctx->op_part, n_op * OP_SPAN * 8 * 8});
+  }
   if (!plan.direct_bundles.empty()) {
     n.push_back({&ctx->camS, (size_t)C * 36 * 8});
     n.push_back({&ctx->camG, (size_t)M * 18 * 8});
@@ -780,6 +805,9 @@ int host_plan(const tcv_ba_control* controls, uint32_t B, const uint32_t* index,
 // Residency of a setup call: decided by setup against the budget, or given by a caller that
 // admitted its staging with the plan already.
 enum Residency { DECIDE = -1, RESIDENT = 0, STREAMED = 1 };
+
+int plan_onepass(tcv_ba_context* ctx, Problem& q, const Plan& plan, hipStream_t st,
+                 tcv_ba_report* rep);
 
 // Allocate, validate the index and build the plan of a batch of B bundles whose packed arrays
 // `scene` / `index` (device pointers) hold the whole batch.
@@ -953,6 +981,7 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
       q.Wc = ctx->Wc.ptr;
     }
   }
+  if (!exact) TRY(plan_onepass(ctx, q, plan, st, rep));
   ctx->large_bundles = plan.large_bundles;
   ctx->large_n.clear();
   ctx->large_off.clear();
@@ -973,6 +1002,68 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
     rep->point_tiles = n_tiles;
     rep->camera_tasks = n_tasks;
   }
+  return TCV_BA_OK;
+}
+
+// The one-pass Schur product (ba_onepass.hip): workgroups of at most OP_TILES tiles
+// of one bundle, counted from the bundle's first tile (so a batch groups each bundle's tiles as its
+// solve alone does), the camera window of each, and, when every window spans at most OP_SPAN
+// cameras, each workgroup's first partial row and each camera's rows in workgroup order.
+// Otherwise q.n_op stays 0 and the two sweeps run.
+int plan_onepass(tcv_ba_context* ctx, Problem& q, const Plan& plan, hipStream_t st,
+                 tcv_ba_report* rep) {
+  q.n_op = 0;
+  std::vector<uint32_t> h_tile, h_bundle;
+  for (uint32_t b = 0; b < plan.bundles.size(); b++) {
+    const Bundle& g = plan.bundles[b];
+    for (uint32_t t = g.tile0; t < g.tile1; t += OP_TILES) {
+      h_tile.push_back(t);
+      h_bundle.push_back(b);
+    }
+  }
+  const uint32_t n_op = (uint32_t)h_tile.size();
+  if (n_op == 0) return TCV_BA_OK;
+  h_tile.push_back(q.n_tiles);
+  TRY(upload(ctx->op_tile, h_tile, st));
+  q.op_tile = (const uint32_t*)ctx->op_tile.ptr;
+  uint32_t* lo = (uint32_t*)ctx->op_lo.ptr;
+  uint32_t* off = (uint32_t*)ctx->op_off.ptr;
+  hipLaunchKernelGGL(k_onepass_windows, dim3(n_op), dim3(256), 0, st, q, lo, off);
+  HIP_TRY(hipGetLastError());
+  std::vector<uint32_t> h_lo(n_op), h_span(n_op);
+  HIP_TRY(hipMemcpyAsync(h_lo.data(), lo, n_op * 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(h_span.data(), off, n_op * 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  // the camera of row k of workgroup w's window (Window in ba_onepass.hip)
+  auto camera = [&](uint32_t w, uint32_t k) {
+    const Bundle& g = plan.bundles[h_bundle[w]];
+    const uint32_t v = (h_lo[w] & 0x7fffffffu) + k, n = g.cam1 - g.cam0;
+    return (h_lo[w] >> 31) && v >= g.cam0 + n ? v - n : v;
+  };
+  std::vector<uint32_t> h_off(n_op + 1, 0), h_cam(q.C + 1, 0);
+  for (uint32_t w = 0; w < n_op; w++) {
+    if (h_span[w] > (uint32_t)OP_SPAN) return TCV_BA_OK;  // a wide window: the two sweeps
+    h_off[w + 1] = h_off[w] + h_span[w];
+    for (uint32_t k = 0; k < h_span[w]; k++) h_cam[camera(w, k) + 1]++;
+  }
+  for (uint32_t c = 0; c < q.C; c++) h_cam[c + 1] += h_cam[c];
+  std::vector<uint32_t> h_row(h_cam[q.C]), fill(h_cam.begin(), h_cam.end() - 1);
+  for (uint32_t w = 0; w < n_op; w++)
+    for (uint32_t k = 0; k < h_span[w]; k++) h_row[fill[camera(w, k)]++] = h_off[w] + k;
+  TRY(upload(ctx->op_off, h_off, st));
+  TRY(upload(ctx->op_cam, h_cam, st));
+  TRY(upload(ctx->op_row, h_row, st));
+  q.n_op = n_op;
+  q.op_lo = lo;
+  q.op_off = (const uint32_t*)ctx->op_off.ptr;
+  q.op_cam = (const uint32_t*)ctx->op_cam.ptr;
+  q.op_row = (const uint32_t*)ctx->op_row.ptr;
+  q.op_part = (double*)ctx->op_part.ptr;
+  if (rep) rep->onepass_rows = h_off[n_op];
+  // nothing reads the camera-major blocks or the fused product's rows then
+  q.Wc = nullptr;
+  q.tprod = nullptr;
+  HIP_TRY(hipStreamSynchronize(st));  // the host tables are uploaded before they go out of scope
   return TCV_BA_OK;
 }
 
@@ -1057,8 +1148,8 @@ int run_pcg(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, cons
   // sweeps only; the tuning iterations are not multiples of kTimingStride); until then the plain
   // sweeps with no cap
   const bool tuned = ctx->tuned_shape == ctx->shape;
-  bool tune = !tuned && !q.streamed && ops.unstored_camera_sweep != nullptr && q.n_unstored < q.B &&
-              std::min(first, budget) >= kTuneFirst + kTuneRuns;
+  bool tune = !tuned && !q.streamed && q.n_op == 0 && ops.unstored_camera_sweep != nullptr &&
+              q.n_unstored < q.B && std::min(first, budget) >= kTuneFirst + kTuneRuns;
   static_assert(kTuneFirst > 0 && kTuneFirst + kTuneRuns <= kTimingStride, "tuning iterations");
   if (tune) TRY(ensure_tune_events(ctx));
   Problem qt = scheduled(q, tuned ? TuneCandidate{ctx->tuned_fused, ctx->camera_cap} : kTune[0]);

@@ -179,7 +179,24 @@ struct Problem {
   // launch schedule, read by the host only: dynamic LDS bytes reserved per camera-sweep workgroup,
   // which caps the workgroups a CU holds at once (0: no reservation)
   uint32_t camera_lds;
+  // the one-pass Schur product of the stored-block forms (ba_onepass.hip; n_op = 0: the two sweeps):
+  // n_op workgroups, workgroup w taking tiles op_tile[w] .. op_tile[w + 1] (at most OP_TILES, of
+  // one bundle); its camera window starts at op_lo[w] and its partial rows at op_off[w] of op_part
+  // (8 words per row, one row per camera of the window); camera c's rows are
+  // op_row[op_cam[c] .. op_cam[c + 1]), in workgroup order
+  uint32_t n_op;
+  const uint32_t* op_tile;
+  const uint32_t* op_lo;
+  const uint32_t* op_off;
+  const uint32_t* op_cam;
+  const uint32_t* op_row;
+  double* op_part;
 };
+
+// One-pass product: tiles per workgroup (OP_WAVE_TILES per wave) and the widest camera window.
+constexpr int OP_WAVE_TILES = 8;
+constexpr int OP_TILES = OP_WAVE_TILES * TILE_WAVES;
+constexpr int OP_SPAN = 64;
 
 // PCG iterations of one pass of bundle g.
 __host__ __device__ inline double pcg_budget(const Problem& q, const Bundle& g) {
