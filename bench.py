@@ -30,12 +30,16 @@ sys.path.insert(0, str(ROOT))
 METRIC = ("LM iters/sec + obs/sec, mixed vs FP64 at ≤1e-6 rel cost; Schur matvec HBM GB/s")
 HBM_PEAK_GBS = 8000.0
 
-# HBM bytes of one Schur product (point sweep + camera sweep) measured with
-# `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE` over `bench.py --mode probe`
-# (profiles/schur_trajectory500_fp64_traffic.md): FETCH_SIZE x 2.00 and WRITE_SIZE x 1.00, the
-# factors that make a 1 GiB device copy read and write 1 GiB. Keyed by
+# HBM bytes of one Schur product measured with `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE`
+# over a 2-LM-iteration solve (profiles/schur_trajectory2000_traffic.md): FETCH_SIZE x 2.00 and
+# WRITE_SIZE x 1.00, the factors that make a 1 GiB device copy read and write 1 GiB. Keyed by
 # (cameras, points, seed, precision, operator); other workloads report null.
 MEASURED_TRAFFIC = {
+    # the one-pass product (k_schur_onepass + k_onepass_reduce)
+    (2000, 1_000_000, 2, "fp64", "onepass"):
+        2 * (1091537.4 + 7673.8) * 1024 + (11394.9 + 625.0) * 1024,
+    (2000, 1_000_000, 2, "mixed", "onepass"):
+        2 * (410496.5 + 7673.8) * 1024 + (11390.9 + 625.0) * 1024,
 }
 
 

@@ -68,8 +68,9 @@ def test_trajectory_500_cameras_first_lm_iterations_match_the_oracle(device, tol
     got, report = B.solve_packed(device, control, packed)
     expected, stats = O.solve(control, packed)
     base = packed.C * 12 + packed.P * 3
-    print(f"  PCG iterations: GPU {report['cg_iterations']}, oracle {stats['cg_iterations']:.0f}",
-          flush=True)
+    print(f"  PCG iterations: GPU {report['cg_iterations']}, oracle {stats['cg_iterations']:.0f}; "
+          f"one-pass rows {report['onepass_rows']}", flush=True)
+    assert report["onepass_rows"] > 0  # the one-pass product ran
     assert got[base + 10] + got[base + 11] == 3
     largest, _ = compare(got, expected, base, f"C={packed.C} P={packed.P} M={packed.M}",
                          coordinate_bar=1e-7 if tolerance < 1e-9 else 5e-5)
