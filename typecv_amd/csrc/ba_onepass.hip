@@ -13,7 +13,8 @@
 // k_onepass_reduce sums each camera's rows in workgroup order. Every sum has a fixed order:
 //   * within a 64-observation pass, each camera row takes its lanes' terms one after another in
 //     lane order (rounds of at most one lane per row, the lane's rank among the lanes of its row
-//     found by ballots of the row's bits);
+//     found by ballots of the row's bits; the adds are LDS double additions, each row's in round
+//     order);
 //   * a wave adds its passes to its partials in tile order, the workgroup its four waves in wave
 //     order, the reduction the workgroups in index order.
 // So the product is deterministic (and the streamed route, which recomputes the factors, gives the
@@ -88,6 +89,20 @@ struct Window {
   __device__ uint32_t row(uint32_t c) const {
     return (wrapped && c - cam0 < half ? c + n : c) - lo;
   }
+  __device__ uint32_t camera(uint32_t r) const {
+    const uint32_t v = lo + r;
+    return wrapped && v >= cam0 + n ? v - n : v;
+  }
+};
+
+// The cameras of a workgroup's window, staged in LDS once per product: each camera's PCG direction
+// p_c, whether it is variable and, for the factors (NR = 9), its rotation. The tiles' gathers of
+// them become LDS reads.
+template <int NR>
+struct WindowCams {
+  double p[OP_SPAN][6];
+  double R[OP_SPAN][NR];
+  uint32_t var[OP_SPAN];
 };
 
 __device__ __forceinline__ Window window_of(const Problem& q, uint32_t w) {
@@ -116,21 +131,15 @@ struct ObsWords<float, SHARED> {
                                        const Bundle& g, double fscale) {
     load_factors32<SHARED, RECOMPUTE>(q, i, c, p, g, fscale, f);
   }
-  __device__ __forceinline__ void point_term(const Problem& q, uint32_t c, const double* vc,
-                                             double vf, bool var, bool dir, double t[3]) const {
-    const double* pose = q.scene + (size_t)c * 12;
-    double R[9], d[11];
-#pragma unroll
-    for (int k = 0; k < 9; k++) R[k] = pose[k];
+  __device__ __forceinline__ void point_term(const double* R, const double* vc, double vf,
+                                             bool var, bool dir, double t[3]) const {
+    double d[11];
     widen11(f, d);
     factored_point_term(d, R, vc, vf, var, dir, t);
   }
-  __device__ __forceinline__ void camera_term(const Problem& q, uint32_t c, const double* y,
-                                              bool var, bool dir, double* hv) const {
-    const double* pose = q.scene + (size_t)c * 12;
-    double R[9], d[11];
-#pragma unroll
-    for (int k = 0; k < 9; k++) R[k] = pose[k];
+  __device__ __forceinline__ void camera_term(const double* R, const double* y, bool var, bool dir,
+                                              double* hv) const {
+    double d[11];
     widen11(f, d);
     factored_camera_term<SHARED>(d, R, y, var, dir, hv);
   }
@@ -151,8 +160,8 @@ struct ObsWords<double, SHARED> {
     }
   }
   // k_point_sweep's expression: t_j = sum_k W[k][j] v_c[k] (+ W_f[j] v_f)
-  __device__ __forceinline__ void point_term(const Problem&, uint32_t, const double* vc, double vf,
-                                             bool, bool, double t[3]) const {
+  __device__ __forceinline__ void point_term(const double*, const double* vc, double vf, bool,
+                                             bool, double t[3]) const {
 #pragma unroll
     for (int j = 0; j < 3; j++) {
       double a = 0;
@@ -163,7 +172,7 @@ struct ObsWords<double, SHARED> {
     }
   }
   // W y: the camera rows and, on the shared route, the focal row
-  __device__ __forceinline__ void camera_term(const Problem&, uint32_t, const double* y, bool, bool,
+  __device__ __forceinline__ void camera_term(const double*, const double* y, bool, bool,
                                               double* hv) const {
 #pragma unroll
     for (int k = 0; k < 6; k++) hv[k] = w[k * 3] * y[0] + w[k * 3 + 1] * y[1] + w[k * 3 + 2] * y[2];
@@ -175,8 +184,9 @@ struct ObsWords<double, SHARED> {
 // added into `part` (one row per camera of the window).
 template <class WT, bool SHARED, bool RECOMPUTE>
 __device__ __forceinline__ void onepass_tile(const Problem& q, uint32_t t, uint32_t lane,
-                                             const Window& win, double (*st)[3],
-                                             double (*part)[SHARED ? 7 : 6]) {
+                                             const Window& win,
+                                             const WindowCams<sizeof(WT) == 4 ? 9 : 1>& wc,
+                                             double (*st)[3], double (*part)[SHARED ? 7 : 6]) {
   constexpr int NV = SHARED ? 7 : 6;
   const uint32_t fp = q.tiles[t], ep = q.tiles[t + 1], b = q.tile_bundle[t];
   const double* sb = bstate(q, b);
@@ -192,7 +202,7 @@ __device__ __forceinline__ void onepass_tile(const Problem& q, uint32_t t, uint3
   const double vf = SHARED ? v[6 * q.C + b] : 0.0;
   const double fscale = RECOMPUTE && SHARED ? focal_scale(sb[S_LOG_FOCAL]) : 1.0;
   const bool multi = e - s > (uint32_t)TILE;  // one point of more than 64 observations
-  uint32_t c = 0, pt = 0;
+  uint32_t c = 0, pt = 0, row = 0;
   bool var = false, dir = false;
   ObsWords<WT, SHARED> ow;
   double acc[3] = {0, 0, 0};
@@ -202,14 +212,12 @@ __device__ __forceinline__ void onepass_tile(const Problem& q, uint32_t t, uint3
     if (i < e) {
       c = obs_camera(q)[i];
       pt = obs_point(q)[i];
-      var = variable_camera(q, c);
+      row = win.row(c);
+      var = wc.var[row] != 0;
       dir = directional_point(g, pt);
       ow.template load<RECOMPUTE>(q, i, c, pt, g, fscale);
-      double vc[6];
-#pragma unroll
-      for (int k = 0; k < 6; k++) vc[k] = v[c * 6 + k];
       double tj[3];
-      ow.point_term(q, c, vc, vf, var, dir, tj);
+      ow.point_term(wc.R[row], wc.p[row], vf, var, dir, tj);
       st[lane][0] = tj[0];
       st[lane][1] = tj[1];
       st[lane][2] = tj[2];
@@ -235,16 +243,14 @@ __device__ __forceinline__ void onepass_tile(const Problem& q, uint32_t t, uint3
     if (multi && ok) {
       c = obs_camera(q)[i];
       pt = obs_point(q)[i];
-      var = variable_camera(q, c);
+      row = win.row(c);
+      var = wc.var[row] != 0;
       dir = directional_point(g, pt);
       ow.template load<RECOMPUTE>(q, i, c, pt, g, fscale);
     }
     double hv[NV];
-    uint32_t row = 0;
-    if (ok) {
-      ow.camera_term(q, c, st[pt - fp], var, dir, hv);
-      row = win.row(c);
-    }
+    if (ok) ow.camera_term(wc.R[row], st[pt - fp], var, dir, hv);
+    if (!ok) row = 0;
     // each lane's rank among the lanes of the same camera row: the rows' bits by ballot
     unsigned long long same = __ballot(ok);
 #pragma unroll
@@ -255,14 +261,18 @@ __device__ __forceinline__ void onepass_tile(const Problem& q, uint32_t t, uint3
     const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
         (uint32_t)(same >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)same, 0u));
     // rounds of one lane per row, in rank (lane) order: each row gets its lanes' terms one after
-    // another, without two lanes of one round writing the same row
+    // another. No two lanes of one round add to the same row, and the wave's LDS operations are
+    // applied in issue order, so each row's sum has a fixed order: the LDS adds (ds_add_f64, IEEE
+    // double additions done by the LDS) need no wait between rounds, only before the rows are read
     for (uint32_t r = 0; __ballot(ok && rank >= r) != 0ull; r++) {
       if (ok && rank == r) {
 #pragma unroll
-        for (int k = 0; k < NV; k++) part[row][k] += hv[k];
+        for (int k = 0; k < NV; k++)
+          __hip_atomic_fetch_add(&part[row][k], hv[k], __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_WAVEFRONT);
       }
-      wave_sync();
     }
+    wave_sync();
   }
 }
 
@@ -272,9 +282,28 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_schur_onepass(Problem q) {
   constexpr int NV = SHARED ? 7 : 6;
   __shared__ double st_all[TILE_WAVES][TILE][3];
   __shared__ double part_all[TILE_WAVES][OP_SPAN][NV];
+  __shared__ WindowCams<sizeof(WT) == 4 ? 9 : 1> wc;
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / TILE), lane = threadIdx.x % TILE;
   const uint32_t w = xcd_contiguous(blockIdx.x, gridDim.x);
   const Window win = window_of(q, w);
+  const uint32_t row0 = q.op_off[w], span = q.op_off[w + 1] - row0;
+  {
+    // the window's cameras: thread r stages row r (rows past the span stay unread)
+    const uint32_t r = threadIdx.x;
+    if (r < span) {
+      const uint32_t c = win.camera(r);
+      const double* v = vp(q) + (size_t)c * 6;
+#pragma unroll
+      for (int k = 0; k < 6; k++) wc.p[r][k] = v[k];
+      if constexpr (sizeof(WT) == 4) {
+        const double* pose = q.scene + (size_t)c * 12;
+#pragma unroll
+        for (int k = 0; k < 9; k++) wc.R[r][k] = pose[k];
+      }
+      wc.var[r] = variable_camera(q, c) ? 1u : 0u;
+    }
+  }
+  __syncthreads();
   static_assert(OP_SPAN == TILE, "one partial row per lane");
 #pragma unroll
   for (int k = 0; k < NV; k++) part_all[wave][lane][k] = 0;
@@ -282,9 +311,8 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_schur_onepass(Problem q) {
   const uint32_t t0 = q.op_tile[w] + wave * OP_WAVE_TILES;
   const uint32_t t1 = min(t0 + OP_WAVE_TILES, q.op_tile[w + 1]);
   for (uint32_t t = t0; t < t1; t++)
-    onepass_tile<WT, SHARED, RECOMPUTE>(q, t, lane, win, st_all[wave], part_all[wave]);
+    onepass_tile<WT, SHARED, RECOMPUTE>(q, t, lane, win, wc, st_all[wave], part_all[wave]);
   __syncthreads();
-  const uint32_t row0 = q.op_off[w], span = q.op_off[w + 1] - row0;
   double* out = q.op_part + (size_t)row0 * 8;
   for (uint32_t x = threadIdx.x; x < span * 8; x += blockDim.x) {
     const uint32_t k = x >> 3, j = x & 7;
