@@ -25,14 +25,28 @@ from typecv_amd import synthetic as S
 pytestmark = pytest.mark.gpu
 
 
-def stored_bytes(control) -> int:
+def tile_slots(control, packed) -> int:
+    """Slots of the point-major stored words: the engine's tiles of whole points (at most 64
+    observations and 64 points, a longer point alone), each from a 64-observation block of its
+    own."""
+    M, P = control.observations, control.points
+    ps = np.asarray(packed.index[2 * M: 2 * M + P + 1], dtype=np.int64)
+    slots, first = 0, 0
+    for p in range(1, P + 1):
+        if p == P or ps[p + 1] - ps[first] > 64 or p - first >= 64:
+            slots += (ps[p] - ps[first] + 63) // 64 * 64
+            first = p
+    return int(slots)
+
+
+def stored_bytes(control, packed) -> int:
     """Bytes of the arrays only a resident solve holds: the point-major and camera-major cross
-    blocks (FP64: K = 18 or 21 words per observation; the mixed form's FP32 Jacobian factors: 9 or
-    11; padded to 64 observations) and, with FP64 blocks, the fused product's 64-byte rows."""
+    blocks (FP64: K = 18 or 21 words per observation slot; the mixed form's FP32 Jacobian factors:
+    9 or 11) and, with FP64 blocks, the fused product's 64-byte rows."""
     mixed = control.precision == _native.MIXED
     K = (11 if control.shared_focal else 9) if mixed else (21 if control.shared_focal else 18)
     ws = 4 if mixed else 8
-    words = K * ((control.observations + 63) // 64) * 64
+    words = K * tile_slots(control, packed)
     return 2 * words * ws + (64 * control.observations if ws == 8 else 0)
 
 
@@ -55,7 +69,7 @@ def test_streamed_equals_resident(name, packed, kw, precision):
         resident, report = B.solve_packed(device, control, packed)
         assert report["streamed"] == 0
         held = device.statistics["liveBytes"]
-    floor = held - stored_bytes(control)
+    floor = held - stored_bytes(control, packed)
     for budget in (held - 1, floor):
         with B.HipDevice(0, working_bytes=budget) as device:
             streamed, report = B.solve_packed(device, control, packed)
@@ -74,7 +88,7 @@ def test_budget_refusal_before_allocation():
     control = B.make_control(packed.C, packed.P, packed.M, **kw)
     with B.HipDevice(0) as device:
         expected, _ = B.solve_packed(device, control, packed)
-        floor = device.statistics["liveBytes"] - stored_bytes(control)
+        floor = device.statistics["liveBytes"] - stored_bytes(control, packed)
     with B.HipDevice(0, working_bytes=floor - 1) as device:
         assert device.statistics["liveBytes"] == 0
         with pytest.raises(_native.EngineError, match="GPU BA needs") as err:

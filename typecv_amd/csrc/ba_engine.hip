@@ -59,6 +59,7 @@ struct tcv_ba_context {
   std::vector<DevBuf*> all;
   DevBuf scene, trial, index, out, bundles, tiles, tile_bundle, tasks, task_start, cost_blocks,
       camera_bundle, point_bundle, cam_point, Wp, Wc, pblk, tile_fail, tile_focal, task_part, camA,
+      tile_slot,
       camInv, camRhs, camHcf, camFocal, vec, state, cost_part, error, probe, groups, group_part,
       rhs, grad, camS, camG, pfc, dense, direct_bundles, direct_cams, covis, small_bundles,
       tprod,
@@ -87,7 +88,7 @@ struct tcv_ba_context {
            &groups,     &group_part,   &rhs,       &grad,      &camS,       &camG,   &pfc,
            &dense,      &direct_bundles, &direct_cams, &covis, &small_bundles, &tprod,
            &obs_cameras, &obs_points, &obs_pixels, &obs_indices, &obs_offsets, &obs_flags,
-           &obs_errors, &obs_partials, &obs_summary, &op_tile, &op_lo, &op_off, &op_cam, &op_row,
+           &obs_errors, &obs_partials, &obs_summary, &tile_slot, &op_tile, &op_lo, &op_off, &op_cam, &op_row,
            &op_part};
   }
 };
@@ -166,8 +167,9 @@ bool same_options(const tcv_ba_control& a, const tcv_ba_control& b) {
 
 struct Plan {
   std::vector<Bundle> bundles;
-  std::vector<uint32_t> tiles, tile_bundle, tasks, task_start, cost_blocks, camera_bundle,
-      point_bundle, groups, direct_bundles, direct_cams, small_bundles, large_bundles;
+  std::vector<uint32_t> tiles, tile_bundle, tile_slot, tasks, task_start, cost_blocks,
+      camera_bundle, point_bundle, groups, direct_bundles, direct_cams, small_bundles,
+      large_bundles;
   uint64_t dense_words = 0;
   uint32_t covis_words = 0;
 };
@@ -281,6 +283,12 @@ int build_plan(const std::vector<uint32_t>& ps, const std::vector<uint32_t>& cs,
     }
     plan.tiles.push_back(g.pt1);
     g.tile1 = (uint32_t)plan.tiles.size() - 1;
+    // each tile's stored words from a 64-observation block of their own
+    if (plan.tile_slot.empty()) plan.tile_slot.push_back(0);
+    for (uint32_t t = g.tile0; t < g.tile1; t++) {
+      const uint32_t n = ps[plan.tiles[t + 1]] - ps[plan.tiles[t]];
+      plan.tile_slot.push_back(plan.tile_slot.back() + (n + 63) / 64 * 64);
+    }
     while (plan.tile_bundle.size() < g.tile1) plan.tile_bundle.push_back(b);
     // camera tasks
     g.task0 = (uint32_t)(plan.tasks.size() / 4);
@@ -683,7 +691,7 @@ std::vector<Need> solve_needs(tcv_ba_context* ctx, const tcv_ba_control* c, uint
   const size_t n_tiles = plan.tiles.size() - 1, n_tasks = plan.tasks.size() / 4;
   const size_t n_cost = plan.cost_blocks.size() / 3, n_groups = plan.groups.size() / GROUP_WORDS;
   // the cross blocks (FP64, K words per observation) or the mixed form's FP32 factors (KF words)
-  const size_t w_words = (size_t)(ws == 4 ? kf_words(shared) : K) * (((size_t)M + 63) / 64) * 64;
+  const size_t w_words = (size_t)(ws == 4 ? kf_words(shared) : K) * plan.tile_slot.back();
   std::vector<Need> n = {
       {&ctx->scene, ((size_t)C * 18 + (size_t)P * 3 + (size_t)M * 2) * 8},
       {&ctx->trial, ((size_t)C * 12 + (size_t)P * 3) * 8},
@@ -705,6 +713,7 @@ std::vector<Need> solve_needs(tcv_ba_context* ctx, const tcv_ba_control* c, uint
       {&ctx->bundles, plan.bundles.size() * sizeof(Bundle)},
       {&ctx->tiles, plan.tiles.size() * 4},
       {&ctx->tile_bundle, plan.tile_bundle.size() * 4},
+      {&ctx->tile_slot, plan.tile_slot.size() * 4},
       {&ctx->tasks, plan.tasks.size() * 4},
       {&ctx->task_start, plan.task_start.size() * 4},
       {&ctx->cost_blocks, plan.cost_blocks.size() * 4},
@@ -853,7 +862,7 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
   const uint32_t n_tasks = (uint32_t)plan.tasks.size() / 4;
   const uint32_t n_cost = (uint32_t)plan.cost_blocks.size() / 3;
   const uint32_t n_groups = (uint32_t)plan.groups.size() / GROUP_WORDS;
-  const size_t w_words = (size_t)(ws == 4 ? kf_words(shared) : K) * (((size_t)M + 63) / 64) * 64;
+  const size_t w_words = (size_t)(ws == 4 ? kf_words(shared) : K) * plan.tile_slot.back();
   const size_t vec_words = (size_t)m * 5 + (size_t)P * 3;
   bool streamed = residency == STREAMED;
   if (residency == DECIDE) TRY(admit(ctx, c, B, C, P, M, plan, exact, {}, streamed));
@@ -869,6 +878,7 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
   TRY(upload(ctx->bundles, plan.bundles, st));
   TRY(upload(ctx->tiles, plan.tiles, st));
   TRY(upload(ctx->tile_bundle, plan.tile_bundle, st));
+  TRY(upload(ctx->tile_slot, plan.tile_slot, st));
   TRY(upload(ctx->tasks, plan.tasks, st));
   TRY(upload(ctx->task_start, plan.task_start, st));
   TRY(upload(ctx->cost_blocks, plan.cost_blocks, st));
@@ -906,6 +916,7 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
   q.bundles = (const Bundle*)ctx->bundles.ptr;
   q.tiles = (const uint32_t*)ctx->tiles.ptr;
   q.tile_bundle = (const uint32_t*)ctx->tile_bundle.ptr;
+  q.tile_slot = (const uint32_t*)ctx->tile_slot.ptr;
   q.tasks = (const uint32_t*)ctx->tasks.ptr;
   q.task_start = (const uint32_t*)ctx->task_start.ptr;
   q.cost_blocks = (const uint32_t*)ctx->cost_blocks.ptr;

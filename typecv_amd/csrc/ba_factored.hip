@@ -84,7 +84,7 @@ __device__ __forceinline__ void sweep_tile_factored(const Problem& q, uint32_t t
       const uint32_t c = obs_camera(q)[i], pt = obs_point(q)[i];
       const bool var = variable_camera(q, c), dir = directional_point(g, pt);
       double f[11], R[9], vc[6];
-      load_factors<SHARED, RECOMPUTE>(q, q.Wp, i, i, c, pt, g, fscale, f);
+      load_factors<SHARED, RECOMPUTE>(q, q.Wp, q.tile_slot[t] + (i - s), i, c, pt, g, fscale, f);
       const double* pose = q.scene + (size_t)c * 12;
 #pragma unroll
       for (int k = 0; k < 9; k++) R[k] = pose[k];

@@ -9,9 +9,10 @@
 // Data layout in HBM (C cameras, P points, M observations in all, K = 18 or 21 cross words):
 //   scene, trial, index    the reference's packed arrays (include/typecv_ba.h) of the whole batch
 //   Wp                     cross blocks W_i = w Jc^T Jp (+ the focal row w Jf^T Jp), observations in
-//                          point-major (input) order: the point sweep streams it
+//                          point-major (input) order, each tile's from a 64-observation block of
+//                          its own (tile_slot): the point sweep and the one-pass product stream it
 //   Wc                     the same blocks in camera-major order (each camera's list): the camera
-//                          sweep streams it. Both blocked [M/64][K][64] (see wptr)
+//                          sweep streams it. Both blocked [slots/64][K][64] (see wptr)
 //   cam_point[M]           the point of each camera-major slot
 //   pblk[P][12]            V^-1 (9, symmetrised) and the point RHS b_p (3)
 //   tiles[T+1]             first point of each wave tile: whole points of one bundle, at most 64
@@ -19,7 +20,8 @@
 //   tasks[4·N], task_start camera-list tasks (camera, first slot, end slot, bundle), at most 1024
 //                          slots, and the first task of each camera
 //   rows                   camera c owns unknowns 6c..6c+5; bundle b's focal unknown is 6C + b
-// Every reduction has a fixed order, so a run is bitwise reproducible; no float atomics.
+// Every reduction has a fixed order, so a run is bitwise reproducible: no global float atomics,
+// and the one-pass product's LDS adds reach each row one lane at a time in a fixed order.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -121,7 +123,10 @@ struct Problem {
   const uint32_t* index;
   const Bundle* bundles;
   const uint32_t* tiles;          // n_tiles + 1 first points
-  const uint32_t* tile_bundle;    // n_tiles
+  const uint32_t* tile_bundle;
+  // n_tiles + 1: the first slot of each tile in the point-major stored words (Wp), a multiple of
+  // 64, so that a tile's words start a 64-observation block and no two tiles share a block
+  const uint32_t* tile_slot;
   const uint32_t* tasks;          // 4 per task: camera, first slot, end slot, bundle
   const uint32_t* task_start;     // C + 1
   const uint32_t* cost_blocks;    // 3 per block: bundle, first and end observation

@@ -591,9 +591,11 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_pass(Problem q) {
         if constexpr (sizeof(WT) == 4) {
           double fw[11];
           cross_factors(L, ok, SHARED, fw);
-          for (int k = 0; k < kf_words(SHARED); k++) *fptr<SHARED>(q.Wp, k, i) = (float)fw[k];
+          const uint32_t slot = q.tile_slot[t] + (i - s);
+          for (int k = 0; k < kf_words(SHARED); k++) *fptr<SHARED>(q.Wp, k, slot) = (float)fw[k];
         } else {
-          for (uint32_t k = 0; k < K; k++) *wptr<WT, SHARED>(q.Wp, k, i, q.M) = (WT)W[k];
+          const uint32_t slot = q.tile_slot[t] + (i - s);
+          for (uint32_t k = 0; k < K; k++) *wptr<WT, SHARED>(q.Wp, k, slot, q.M) = (WT)W[k];
         }
       }
       double* o = st[lane];
@@ -1115,9 +1117,10 @@ __device__ __forceinline__ void sweep_tile(const Problem& q, uint32_t t, uint32_
   const uint32_t mys = has ? ps[my] : 0, mye = has ? ps[my + 1] : 0;
   const double vf = SHARED ? v[6 * q.C + b] : 0.0;  // the bundle's focal row
   const double fscale = RECOMPUTE && SHARED ? focal_scale(sb[S_LOG_FOCAL]) : 1.0;
+  const uint32_t ts = q.tile_slot[t];
   double acc[3] = {0, 0, 0};
   for (uint32_t base = s; base < e; base += TILE) {
-    const uint32_t i = base + lane;
+    const uint32_t i = base + lane, slot = ts + (i - s);
     if (i < e) {
       const uint32_t c = obs_camera(q)[i];
       double vc[6];
@@ -1130,11 +1133,11 @@ __device__ __forceinline__ void sweep_tile(const Problem& q, uint32_t t, uint32_
         double a = 0;
 #pragma unroll
         for (int k = 0; k < 6; k++)
-          a += (RECOMPUTE ? wr[k * 3 + j] : wload(wptr<WT, SHARED>(q.Wp, k * 3 + j, i, q.M))) *
+          a += (RECOMPUTE ? wr[k * 3 + j] : wload(wptr<WT, SHARED>(q.Wp, k * 3 + j, slot, q.M))) *
                vc[k];
         if (SHARED)
           a += (RECOMPUTE ? wr[(18 + j) % (SHARED ? 21 : 18)]
-                          : wload(wptr<WT, SHARED>(q.Wp, 18 + j, i, q.M))) * vf;
+                          : wload(wptr<WT, SHARED>(q.Wp, 18 + j, slot, q.M))) * vf;
         tj[j] = a;
       }
       st[lane][0] = tj[0];
@@ -1191,6 +1194,7 @@ __device__ __forceinline__ void sweep_tile_fused(const Problem& q, uint32_t t, u
   const bool has = my < ep;
   const uint32_t mys = has ? ps[my] : 0, mye = has ? ps[my + 1] : 0;
   const double vf = SHARED ? v[6 * q.C + b] : 0.0;
+  const uint32_t ts = q.tile_slot[t];
   double acc[3] = {0, 0, 0};
   if (e - s <= TILE) {
     const uint32_t i = s + lane;
@@ -1201,7 +1205,7 @@ __device__ __forceinline__ void sweep_tile_fused(const Problem& q, uint32_t t, u
       double vc[6];
       for (int k = 0; k < 6; k++) vc[k] = v[c * 6 + k];
 #pragma unroll
-      for (int k = 0; k < K; k++) w[k] = wload(wptr<WT, SHARED>(q.Wp, k, i, q.M));
+      for (int k = 0; k < K; k++) w[k] = wload(wptr<WT, SHARED>(q.Wp, k, ts + lane, q.M));
 #pragma unroll
       for (int j = 0; j < 3; j++) {
         double a = 0;
@@ -1242,8 +1246,9 @@ __device__ __forceinline__ void sweep_tile_fused(const Problem& q, uint32_t t, u
       for (int j = 0; j < 3; j++) {
         double a = 0;
 #pragma unroll
-        for (int k = 0; k < 6; k++) a += wload(wptr<WT, SHARED>(q.Wp, k * 3 + j, i, q.M)) * vc[k];
-        if (SHARED) a += wload(wptr<WT, SHARED>(q.Wp, 18 + j, i, q.M)) * vf;
+        for (int k = 0; k < 6; k++)
+          a += wload(wptr<WT, SHARED>(q.Wp, k * 3 + j, ts + (i - s), q.M)) * vc[k];
+        if (SHARED) a += wload(wptr<WT, SHARED>(q.Wp, 18 + j, ts + (i - s), q.M)) * vf;
         st[lane][j] = a;
       }
     }
@@ -1267,9 +1272,9 @@ __device__ __forceinline__ void sweep_tile_fused(const Problem& q, uint32_t t, u
     double* out = q.tprod + (size_t)i * 8;
 #pragma unroll
     for (int j = 0; j < NV; j++)
-      out[j] = wload(wptr<WT, SHARED>(q.Wp, j * 3, i, q.M)) * y0 +
-               wload(wptr<WT, SHARED>(q.Wp, j * 3 + 1, i, q.M)) * y1 +
-               wload(wptr<WT, SHARED>(q.Wp, j * 3 + 2, i, q.M)) * y2;
+      out[j] = wload(wptr<WT, SHARED>(q.Wp, j * 3, ts + (i - s), q.M)) * y0 +
+               wload(wptr<WT, SHARED>(q.Wp, j * 3 + 1, ts + (i - s), q.M)) * y1 +
+               wload(wptr<WT, SHARED>(q.Wp, j * 3 + 2, ts + (i - s), q.M)) * y2;
 #pragma unroll
     for (int j = NV; j < 8; j++) out[j] = 0.0;
   }

@@ -55,8 +55,9 @@ __device__ __forceinline__ void factored_camera_term(const double* f, const doub
 // The factors of observation i as stored (FP32; RECOMPUTE: relinearised and rounded as the passes
 // store them), kept in FP32 registers between the two halves.
 template <bool SHARED, bool RECOMPUTE>
-__device__ __forceinline__ void load_factors32(const Problem& q, uint32_t i, uint32_t c, uint32_t p,
-                                               const Bundle& g, double fscale, float* f) {
+__device__ __forceinline__ void load_factors32(const Problem& q, uint32_t slot, uint32_t i,
+                                               uint32_t c, uint32_t p, const Bundle& g,
+                                               double fscale, float* f) {
   constexpr int KF = kf_words(SHARED);
   if (RECOMPUTE) {
     Lin L;
@@ -68,7 +69,7 @@ __device__ __forceinline__ void load_factors32(const Problem& q, uint32_t i, uin
     for (int k = 0; k < KF; k++) f[k] = (float)raw[k];
   } else {
 #pragma unroll
-    for (int k = 0; k < KF; k++) f[k] = __builtin_nontemporal_load(fptr<SHARED>(q.Wp, k, i));
+    for (int k = 0; k < KF; k++) f[k] = __builtin_nontemporal_load(fptr<SHARED>(q.Wp, k, slot));
   }
 #pragma unroll
   for (int k = KF; k < 11; k++) f[k] = 0;
@@ -127,9 +128,9 @@ template <bool SHARED>
 struct ObsWords<float, SHARED> {
   float f[11];
   template <bool RECOMPUTE>
-  __device__ __forceinline__ void load(const Problem& q, uint32_t i, uint32_t c, uint32_t p,
-                                       const Bundle& g, double fscale) {
-    load_factors32<SHARED, RECOMPUTE>(q, i, c, p, g, fscale, f);
+  __device__ __forceinline__ void load(const Problem& q, uint32_t slot, uint32_t i, uint32_t c,
+                                       uint32_t p, const Bundle& g, double fscale) {
+    load_factors32<SHARED, RECOMPUTE>(q, slot, i, c, p, g, fscale, f);
   }
   __device__ __forceinline__ void point_term(const double* R, const double* vc, double vf,
                                              bool var, bool dir, double t[3]) const {
@@ -150,13 +151,13 @@ struct ObsWords<double, SHARED> {
   static constexpr int K = SHARED ? 21 : 18;
   double w[K];
   template <bool RECOMPUTE>
-  __device__ __forceinline__ void load(const Problem& q, uint32_t i, uint32_t c, uint32_t p,
-                                       const Bundle& g, double fscale) {
+  __device__ __forceinline__ void load(const Problem& q, uint32_t slot, uint32_t i, uint32_t c,
+                                       uint32_t p, const Bundle& g, double fscale) {
     if (RECOMPUTE) {
       relinearised_w<double, SHARED>(q, g, c, p, i, fscale, w);
     } else {
 #pragma unroll
-      for (int k = 0; k < K; k++) w[k] = wload(wptr<double, SHARED>(q.Wp, k, i, q.M));
+      for (int k = 0; k < K; k++) w[k] = wload(wptr<double, SHARED>(q.Wp, k, slot, q.M));
     }
   }
   // k_point_sweep's expression: t_j = sum_k W[k][j] v_c[k] (+ W_f[j] v_f)
@@ -215,7 +216,7 @@ __device__ __forceinline__ void onepass_tile(const Problem& q, uint32_t t, uint3
       row = win.row(c);
       var = wc.var[row] != 0;
       dir = directional_point(g, pt);
-      ow.template load<RECOMPUTE>(q, i, c, pt, g, fscale);
+      ow.template load<RECOMPUTE>(q, q.tile_slot[t] + (i - s), i, c, pt, g, fscale);
       double tj[3];
       ow.point_term(wc.R[row], wc.p[row], vf, var, dir, tj);
       st[lane][0] = tj[0];
@@ -246,7 +247,7 @@ __device__ __forceinline__ void onepass_tile(const Problem& q, uint32_t t, uint3
       row = win.row(c);
       var = wc.var[row] != 0;
       dir = directional_point(g, pt);
-      ow.template load<RECOMPUTE>(q, i, c, pt, g, fscale);
+      ow.template load<RECOMPUTE>(q, q.tile_slot[t] + (i - s), i, c, pt, g, fscale);
     }
     double hv[NV];
     if (ok) ow.camera_term(wc.R[row], st[pt - fp], var, dir, hv);
