@@ -64,7 +64,7 @@ struct tcv_ba_context {
       rhs, grad, camS, camG, pfc, dense, direct_bundles, direct_cams, covis, small_bundles,
       tprod,
       obs_cameras, obs_points, obs_pixels, obs_indices, obs_offsets, obs_flags, obs_errors,
-      obs_partials, obs_summary, op_tile, op_lo, op_off, op_cam, op_row, op_part;
+      obs_partials, obs_summary, op_tile, op_lo, op_off, op_cam, op_row, op_part, op_code, op_trec;
   // the direct bundles factored by a sequence of launches, and their dense array sizes
   std::vector<uint32_t> large_bundles, large_n;
   std::vector<uint64_t> large_off;
@@ -89,7 +89,7 @@ struct tcv_ba_context {
            &dense,      &direct_bundles, &direct_cams, &covis, &small_bundles, &tprod,
            &obs_cameras, &obs_points, &obs_pixels, &obs_indices, &obs_offsets, &obs_flags,
            &obs_errors, &obs_partials, &obs_summary, &tile_slot, &op_tile, &op_lo, &op_off, &op_cam, &op_row,
-           &op_part};
+           &op_part,    &op_code,      &op_trec};
   }
 };
 
@@ -739,6 +739,9 @@ std::vector<Need> solve_needs(tcv_ba_context* ctx, const tcv_ba_control* c, uint
     n.push_back({&ctx->op_cam, ((size_t)C + 1) * 4});
     n.push_back({&ctx->op_row, n_op * OP_SPAN * 4});
     n.push_back({&ctx->op_part, n_op * OP_SPAN * 8 * 8});
+    // each stored slot's code word and each tile's record
+    n.push_back({&ctx->op_code, (size_t)plan.tile_slot.back() * 4});
+    n.push_back({&ctx->op_trec, n_tiles * OP_RECORD * 4});
   }
   if (!plan.direct_bundles.empty()) {
     n.push_back({&ctx->camS, (size_t)C * 36 * 8});
@@ -1019,8 +1022,8 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
 // The one-pass Schur product (ba_onepass.hip): workgroups of at most OP_TILES tiles
 // of one bundle, counted from the bundle's first tile (so a batch groups each bundle's tiles as its
 // solve alone does), the camera window of each, and, when every window spans at most OP_SPAN
-// cameras, each workgroup's first partial row and each camera's rows in workgroup order.
-// Otherwise q.n_op stays 0 and the two sweeps run.
+// cameras, each workgroup's first partial row, each camera's rows in workgroup order, each stored
+// slot's code word and each tile's record. Otherwise q.n_op stays 0 and the two sweeps run.
 int plan_onepass(tcv_ba_context* ctx, Problem& q, const Plan& plan, hipStream_t st,
                  tcv_ba_report* rep) {
   q.n_op = 0;
@@ -1064,12 +1067,18 @@ int plan_onepass(tcv_ba_context* ctx, Problem& q, const Plan& plan, hipStream_t 
   TRY(upload(ctx->op_off, h_off, st));
   TRY(upload(ctx->op_cam, h_cam, st));
   TRY(upload(ctx->op_row, h_row, st));
-  q.n_op = n_op;
   q.op_lo = lo;
+  // every window spans at most OP_SPAN rows: each slot's code and each tile's record
+  hipLaunchKernelGGL(k_onepass_codes, dim3(n_op), dim3(TILE * TILE_WAVES), 0, st, q,
+                     (uint32_t*)ctx->op_code.ptr, (uint32_t*)ctx->op_trec.ptr);
+  HIP_TRY(hipGetLastError());
+  q.n_op = n_op;
   q.op_off = (const uint32_t*)ctx->op_off.ptr;
   q.op_cam = (const uint32_t*)ctx->op_cam.ptr;
   q.op_row = (const uint32_t*)ctx->op_row.ptr;
   q.op_part = (double*)ctx->op_part.ptr;
+  q.op_code = (const uint32_t*)ctx->op_code.ptr;
+  q.op_trec = (const uint32_t*)ctx->op_trec.ptr;
   if (rep) rep->onepass_rows = h_off[n_op];
   // nothing reads the camera-major blocks or the fused product's rows then
   q.Wc = nullptr;

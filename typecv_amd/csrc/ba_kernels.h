@@ -196,12 +196,18 @@ struct Problem {
   const uint32_t* op_cam;
   const uint32_t* op_row;
   double* op_part;
+  // per stored slot a code word (the row, rank and tile point of its observation, the segment of
+  // the tile's point numbered as its lane) and per tile OP_RECORD words (ba_onepass.hip), written
+  // at set-up
+  const uint32_t* op_code;
+  const uint32_t* op_trec;
 };
 
 // One-pass product: tiles per workgroup (OP_WAVE_TILES per wave) and the widest camera window.
 constexpr int OP_WAVE_TILES = 8;
 constexpr int OP_TILES = OP_WAVE_TILES * TILE_WAVES;
 constexpr int OP_SPAN = 64;
+constexpr int OP_RECORD = 8;  // words of a tile's record
 
 // PCG iterations of one pass of bundle g.
 __host__ __device__ inline double pcg_budget(const Problem& q, const Bundle& g) {

@@ -12,13 +12,20 @@
 // its solve alone does. The workgroup writes one row of partials per camera of its window;
 // k_onepass_reduce sums each camera's rows in workgroup order. Every sum has a fixed order:
 //   * within a 64-observation pass, each camera row takes its lanes' terms one after another in
-//     lane order (rounds of at most one lane per row, the lane's rank among the lanes of its row
-//     found by ballots of the row's bits; the adds are LDS double additions, each row's in round
-//     order);
+//     lane order (rounds of at most one lane per row, in the order of the lane's rank among the
+//     pass's lanes of its row; the adds are LDS double additions, each row's in round order);
 //   * a wave adds its passes to its partials in tile order, the workgroup its four waves in wave
 //     order, the reduction the workgroups in index order.
-// So the product is deterministic (and the streamed route, which recomputes the factors, gives the
+// So the product is deterministic (and the streamed route, which recomputes the words, gives the
 // same bits), though summed in another order than the two sweeps.
+//
+// What fixes those orders is fixed for a whole solve, so set-up (k_onepass_codes) writes it down
+// once: per stored slot a code word (the window row of its observation's camera, its rank, its
+// point within the tile, and the segment of the pass's observations of the tile's point numbered
+// as the slot's lane), and per tile a record of eight words (first slot, observations, first
+// point, points, first observation, rounds). A wave reads its eight tiles' records with one load
+// and, in the mixed form, loads the next tile's codes, factors and V^-1 before it works on the
+// current one, so the product waits on HBM about once per wave instead of three times per tile.
 #pragma once
 
 namespace tcvba {
@@ -50,29 +57,6 @@ __device__ __forceinline__ void factored_camera_term(const double* f, const doub
     }
   }
   if (SHARED) hv[NV - 1] = f[9] * h0 + f[10] * h1;
-}
-
-// The factors of observation i as stored (FP32; RECOMPUTE: relinearised and rounded as the passes
-// store them), kept in FP32 registers between the two halves.
-template <bool SHARED, bool RECOMPUTE>
-__device__ __forceinline__ void load_factors32(const Problem& q, uint32_t slot, uint32_t i,
-                                               uint32_t c, uint32_t p, const Bundle& g,
-                                               double fscale, float* f) {
-  constexpr int KF = kf_words(SHARED);
-  if (RECOMPUTE) {
-    Lin L;
-    const bool ok =
-        linearize(q, c, p, i, fscale, variable_camera(q, c), directional_point(g, p), L);
-    double raw[11];
-    cross_factors(L, ok, SHARED, raw);
-#pragma unroll
-    for (int k = 0; k < KF; k++) f[k] = (float)raw[k];
-  } else {
-#pragma unroll
-    for (int k = 0; k < KF; k++) f[k] = __builtin_nontemporal_load(fptr<SHARED>(q.Wp, k, slot));
-  }
-#pragma unroll
-  for (int k = KF; k < 11; k++) f[k] = 0;
 }
 
 __device__ __forceinline__ void widen11(const float* f, double* d) {
@@ -118,19 +102,61 @@ __device__ __forceinline__ Window window_of(const Problem& q, uint32_t w) {
   return x;
 }
 
+// The record words of a tile (op_trec, OP_RECORD words per tile).
+enum : uint32_t {
+  TR_SLOT = 0,    // first stored slot (a multiple of 64)
+  TR_OBS = 1,     // observations
+  TR_POINT = 2,   // first point
+  TR_POINTS = 3,  // points
+  TR_FIRST = 4,   // first observation (point-major)
+  TR_ROUNDS = 5,  // rounds of the camera half: the largest rank of any pass, plus one
+};
+static_assert(OP_RECORD * OP_WAVE_TILES == TILE, "a wave's records: one word per lane");
+
+// A slot's code word (op_code): bits 0-5 the window row of its observation's camera, 6-11 its rank
+// among the pass's observations of that row (lanes below it with the same row), 12-17 its point
+// counted from the tile's first point; bits 18-23 and 24-30 the segment [lo, hi) of the pass's
+// lanes that observe the tile's point numbered as the slot's lane (lo = hi = 0: none).
+struct SlotCode {
+  uint32_t row, rank, lp, lo, hi;
+  __device__ explicit SlotCode(uint32_t x)
+      : row(x & 63u), rank((x >> 6) & 63u), lp((x >> 12) & 63u), lo((x >> 18) & 63u),
+        hi((x >> 24) & 127u) {}
+};
+
+__device__ __forceinline__ uint32_t slot_code(uint32_t row, uint32_t rank, uint32_t lp,
+                                              uint32_t lo, uint32_t hi) {
+  return row | rank << 6 | lp << 12 | lo << 18 | hi << 24;
+}
+
 // The stored words of one observation that the two halves share. The mixed form keeps its FP32
-// factors (the camera's rotation is read again, from L2, for the camera half); the FP64 form its
-// cross block.
+// factors; the FP64 form its cross block. `load_stored` reads lane `lane` of one 64-slot block
+// (`blk`: the block's first word), `load_recomputed` relinearises them from the scene.
 template <class WT, bool SHARED>
 struct ObsWords;
 
 template <bool SHARED>
 struct ObsWords<float, SHARED> {
+  static constexpr int K = kf_words(SHARED);
   float f[11];
-  template <bool RECOMPUTE>
-  __device__ __forceinline__ void load(const Problem& q, uint32_t slot, uint32_t i, uint32_t c,
-                                       uint32_t p, const Bundle& g, double fscale) {
-    load_factors32<SHARED, RECOMPUTE>(q, slot, i, c, p, g, fscale, f);
+  __device__ __forceinline__ void load_stored(const float* blk, uint32_t lane) {
+#pragma unroll
+    for (int k = 0; k < K; k++) f[k] = __builtin_nontemporal_load(blk + k * TILE + lane);
+#pragma unroll
+    for (int k = K; k < 11; k++) f[k] = 0;
+  }
+  // relinearised and rounded as the passes store them
+  __device__ __forceinline__ void load_recomputed(const Problem& q, uint32_t i, uint32_t c,
+                                                  uint32_t p, const Bundle& g, double fscale) {
+    Lin L;
+    const bool ok =
+        linearize(q, c, p, i, fscale, variable_camera(q, c), directional_point(g, p), L);
+    double raw[11];
+    cross_factors(L, ok, SHARED, raw);
+#pragma unroll
+    for (int k = 0; k < K; k++) f[k] = (float)raw[k];
+#pragma unroll
+    for (int k = K; k < 11; k++) f[k] = 0;
   }
   __device__ __forceinline__ void point_term(const double* R, const double* vc, double vf,
                                              bool var, bool dir, double t[3]) const {
@@ -150,15 +176,13 @@ template <bool SHARED>
 struct ObsWords<double, SHARED> {
   static constexpr int K = SHARED ? 21 : 18;
   double w[K];
-  template <bool RECOMPUTE>
-  __device__ __forceinline__ void load(const Problem& q, uint32_t slot, uint32_t i, uint32_t c,
-                                       uint32_t p, const Bundle& g, double fscale) {
-    if (RECOMPUTE) {
-      relinearised_w<double, SHARED>(q, g, c, p, i, fscale, w);
-    } else {
+  __device__ __forceinline__ void load_stored(const double* blk, uint32_t lane) {
 #pragma unroll
-      for (int k = 0; k < K; k++) w[k] = wload(wptr<double, SHARED>(q.Wp, k, slot, q.M));
-    }
+    for (int k = 0; k < K; k++) w[k] = wload(blk + k * TILE + lane);
+  }
+  __device__ __forceinline__ void load_recomputed(const Problem& q, uint32_t i, uint32_t c,
+                                                  uint32_t p, const Bundle& g, double fscale) {
+    relinearised_w<double, SHARED>(q, g, c, p, i, fscale, w);
   }
   // k_point_sweep's expression: t_j = sum_k W[k][j] v_c[k] (+ W_f[j] v_f)
   __device__ __forceinline__ void point_term(const double*, const double* vc, double vf, bool,
@@ -181,98 +205,167 @@ struct ObsWords<double, SHARED> {
   }
 };
 
-// One tile of the wave: the point half (y_p of its points, kept in `st`), then the camera half
-// added into `part` (one row per camera of the window).
-template <class WT, bool SHARED, bool RECOMPUTE>
-__device__ __forceinline__ void onepass_tile(const Problem& q, uint32_t t, uint32_t lane,
-                                             const Window& win,
-                                             const WindowCams<sizeof(WT) == 4 ? 9 : 1>& wc,
+// The first word of 64-slot block `block` of the stored words.
+template <class WT, bool SHARED>
+__device__ __forceinline__ const WT* stored_block(const Problem& q, uint32_t block) {
+  return static_cast<const WT*>(q.Wp) + (size_t)block * ObsWords<WT, SHARED>::K * TILE;
+}
+
+// Word k (a wave-uniform index) of the wave's tile records.
+__device__ __forceinline__ uint32_t record_word(uint32_t rec, uint32_t k) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)rec, (int)k);
+}
+
+// What one tile needs before its LDS work: each lane's code, its stored words (the FP32 factors,
+// when AHEAD) and V^-1 of the tile's point numbered as the lane.
+template <class WT, bool SHARED>
+struct TileIn {
+  uint32_t code;
+  ObsWords<WT, SHARED> ow;
+  double vinv[9];
+};
+
+template <class WT, bool SHARED, bool AHEAD>
+__device__ __forceinline__ void load_tile(const Problem& q, uint32_t rec, uint32_t j,
+                                          uint32_t lane, TileIn<WT, SHARED>& in) {
+  const uint32_t slot = record_word(rec, OP_RECORD * j + TR_SLOT);
+  const uint32_t n = record_word(rec, OP_RECORD * j + TR_OBS);
+  const uint32_t fp = record_word(rec, OP_RECORD * j + TR_POINT);
+  const uint32_t np = record_word(rec, OP_RECORD * j + TR_POINTS);
+  in.code = n ? q.op_code[slot + lane] : 0u;
+  if (AHEAD && lane < n) in.ow.load_stored(stored_block<WT, SHARED>(q, slot >> 6), lane);
+  if (lane < np) {
+    const double* pb = q.pblk + (size_t)(fp + lane) * 12;
+#pragma unroll
+    for (int k = 0; k < 9; k++) in.vinv[k] = pb[k];
+  }
+}
+
+// The camera half's adds of one pass: round r adds the terms of the lanes of rank r, at most one
+// per row. The wave's LDS operations are applied in issue order, so each row's sum has a fixed
+// order; the LDS adds (ds_add_f64, IEEE double additions done by the LDS) need no wait between
+// rounds, only before the rows are read.
+template <int NV>
+__device__ __forceinline__ void add_rounds(double (*part)[NV], const SlotCode& k, bool ok,
+                                           uint32_t rounds, const double* hv) {
+  for (uint32_t r = 0; r < rounds; r++) {
+    if (ok && k.rank == r) {
+#pragma unroll
+      for (int x = 0; x < NV; x++)
+        __hip_atomic_fetch_add(&part[k.row][x], hv[x], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WAVEFRONT);
+    }
+  }
+}
+
+// The per-workgroup constants of a product's tiles.
+template <int NR>
+struct OnePassCtx {
+  const Problem& q;
+  const Bundle& g;
+  const Window& win;
+  const WindowCams<NR>& wc;
+  double vf, fscale;
+};
+
+// One tile of at most 64 observations (one pass): the point half (the point sweep's terms and
+// segment sums, y_p kept in `st`), then the camera half added into `part`.
+template <class WT, bool SHARED, bool RECOMPUTE, bool AHEAD, int NR>
+__device__ __forceinline__ void onepass_tile(const OnePassCtx<NR>& x, uint32_t rec, uint32_t j,
+                                             uint32_t lane, TileIn<WT, SHARED>& in,
                                              double (*st)[3], double (*part)[SHARED ? 7 : 6]) {
   constexpr int NV = SHARED ? 7 : 6;
-  const uint32_t fp = q.tiles[t], ep = q.tiles[t + 1], b = q.tile_bundle[t];
-  const double* sb = bstate(q, b);
-  if (sb[S_PCG_ACTIVE] == 0) return;
-  const Bundle& g = q.bundles[b];
-  if (q.n_unstored != 0 && !stores_w(g)) return;
-  const double* v = vp(q);
-  const uint32_t* ps = point_start(q);
-  const uint32_t s = ps[fp], e = ps[ep];
-  const uint32_t my = fp + lane;
-  const bool has = my < ep;
-  const uint32_t mys = has ? ps[my] : 0, mye = has ? ps[my + 1] : 0;
-  const double vf = SHARED ? v[6 * q.C + b] : 0.0;
-  const double fscale = RECOMPUTE && SHARED ? focal_scale(sb[S_LOG_FOCAL]) : 1.0;
-  const bool multi = e - s > (uint32_t)TILE;  // one point of more than 64 observations
-  uint32_t c = 0, pt = 0, row = 0;
-  bool var = false, dir = false;
-  ObsWords<WT, SHARED> ow;
+  const uint32_t slot = record_word(rec, OP_RECORD * j + TR_SLOT);
+  const uint32_t n = record_word(rec, OP_RECORD * j + TR_OBS);
+  const uint32_t fp = record_word(rec, OP_RECORD * j + TR_POINT);
+  const uint32_t np = record_word(rec, OP_RECORD * j + TR_POINTS);
+  const uint32_t first = record_word(rec, OP_RECORD * j + TR_FIRST);
+  const uint32_t rounds = record_word(rec, OP_RECORD * j + TR_ROUNDS);
+  const SlotCode k(in.code);
+  const bool ok = lane < n;
+  const uint32_t pt = fp + k.lp;
+  const bool var = x.wc.var[k.row] != 0, dir = directional_point(x.g, pt);
+  if (!AHEAD && ok) {
+    if (RECOMPUTE)
+      in.ow.load_recomputed(x.q, first + lane, x.win.camera(k.row), pt, x.g, x.fscale);
+    else
+      in.ow.load_stored(stored_block<WT, SHARED>(x.q, slot >> 6), lane);
+  }
+  if (ok) {
+    double tj[3];
+    in.ow.point_term(x.wc.R[k.row], x.wc.p[k.row], x.vf, var, dir, tj);
+    st[lane][0] = tj[0];
+    st[lane][1] = tj[1];
+    st[lane][2] = tj[2];
+  }
+  wave_sync();
   double acc[3] = {0, 0, 0};
-  // the point half (the point sweep's terms and sums)
-  for (uint32_t base = s; base < e; base += TILE) {
-    const uint32_t i = base + lane;
-    if (i < e) {
-      c = obs_camera(q)[i];
-      pt = obs_point(q)[i];
-      row = win.row(c);
-      var = wc.var[row] != 0;
-      dir = directional_point(g, pt);
-      ow.template load<RECOMPUTE>(q, q.tile_slot[t] + (i - s), i, c, pt, g, fscale);
+  if (lane < np) segment_sum(st, 0, k.lo, k.hi, acc);
+  wave_sync();
+  if (lane < np) {
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+      st[lane][c] = in.vinv[c * 3] * acc[0] + in.vinv[c * 3 + 1] * acc[1] + in.vinv[c * 3 + 2] * acc[2];
+  }
+  wave_sync();
+  double hv[NV];
+  if (ok) in.ow.camera_term(x.wc.R[k.row], st[k.lp], var, dir, hv);
+  add_rounds<NV>(part, k, ok, rounds, hv);
+  wave_sync();
+}
+
+// A tile of one point with more than 64 observations: both halves pass by pass, each pass's codes
+// and words loaded (or relinearised) where it is used.
+template <class WT, bool SHARED, bool RECOMPUTE, int NR>
+__device__ __forceinline__ void onepass_long_tile(const OnePassCtx<NR>& x, uint32_t rec, uint32_t j,
+                                               uint32_t lane, const double* vinv,
+                                               double (*st)[3], double (*part)[SHARED ? 7 : 6]) {
+  constexpr int NV = SHARED ? 7 : 6;
+  const uint32_t slot = record_word(rec, OP_RECORD * j + TR_SLOT);
+  const uint32_t n = record_word(rec, OP_RECORD * j + TR_OBS);
+  const uint32_t fp = record_word(rec, OP_RECORD * j + TR_POINT);
+  const uint32_t np = record_word(rec, OP_RECORD * j + TR_POINTS);
+  const uint32_t first = record_word(rec, OP_RECORD * j + TR_FIRST);
+  const uint32_t rounds = record_word(rec, OP_RECORD * j + TR_ROUNDS);
+  ObsWords<WT, SHARED> ow;
+  auto load = [&](uint32_t base, const SlotCode& k) {
+    if (RECOMPUTE)
+      ow.load_recomputed(x.q, first + base + lane, x.win.camera(k.row), fp + k.lp, x.g, x.fscale);
+    else
+      ow.load_stored(stored_block<WT, SHARED>(x.q, (slot + base) >> 6), lane);
+  };
+  double acc[3] = {0, 0, 0};
+  for (uint32_t base = 0; base < n; base += TILE) {
+    const SlotCode k(x.q.op_code[slot + base + lane]);
+    if (base + lane < n) {
+      load(base, k);
       double tj[3];
-      ow.point_term(wc.R[row], wc.p[row], vf, var, dir, tj);
+      ow.point_term(x.wc.R[k.row], x.wc.p[k.row], x.vf, x.wc.var[k.row] != 0,
+                    directional_point(x.g, fp + k.lp), tj);
       st[lane][0] = tj[0];
       st[lane][1] = tj[1];
       st[lane][2] = tj[2];
     }
     wave_sync();
-    if (has) {
-      const uint32_t l = mys > base ? mys : base, h = mye < base + TILE ? mye : base + TILE;
-      segment_sum(st, base, l, h, acc);
-    }
+    if (lane < np) segment_sum(st, 0, k.lo, k.hi, acc);
     wave_sync();
   }
-  if (has) {
-    const double* pb = q.pblk + (size_t)my * 12;
+  if (lane < np) {
 #pragma unroll
-    for (int j = 0; j < 3; j++)
-      st[lane][j] = pb[j * 3] * acc[0] + pb[j * 3 + 1] * acc[1] + pb[j * 3 + 2] * acc[2];
+    for (int c = 0; c < 3; c++)
+      st[lane][c] = vinv[c * 3] * acc[0] + vinv[c * 3 + 1] * acc[1] + vinv[c * 3 + 2] * acc[2];
   }
   wave_sync();
-  // the camera half, pass by pass (a point of more than 64 observations reloads its words)
-  for (uint32_t base = s; base < e; base += TILE) {
-    const uint32_t i = base + lane;
-    const bool ok = i < e;
-    if (multi && ok) {
-      c = obs_camera(q)[i];
-      pt = obs_point(q)[i];
-      row = win.row(c);
-      var = wc.var[row] != 0;
-      dir = directional_point(g, pt);
-      ow.template load<RECOMPUTE>(q, q.tile_slot[t] + (i - s), i, c, pt, g, fscale);
-    }
+  for (uint32_t base = 0; base < n; base += TILE) {
+    const SlotCode k(x.q.op_code[slot + base + lane]);
+    const bool ok = base + lane < n;
     double hv[NV];
-    if (ok) ow.camera_term(wc.R[row], st[pt - fp], var, dir, hv);
-    if (!ok) row = 0;
-    // each lane's rank among the lanes of the same camera row: the rows' bits by ballot
-    unsigned long long same = __ballot(ok);
-#pragma unroll
-    for (int bit = 0; bit < 6; bit++) {
-      const unsigned long long set = __ballot((row >> bit) & 1u);
-      same &= (row >> bit) & 1u ? set : ~set;
+    if (ok) {
+      load(base, k);
+      ow.camera_term(x.wc.R[k.row], st[k.lp], x.wc.var[k.row] != 0,
+                     directional_point(x.g, fp + k.lp), hv);
     }
-    const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
-        (uint32_t)(same >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)same, 0u));
-    // rounds of one lane per row, in rank (lane) order: each row gets its lanes' terms one after
-    // another. No two lanes of one round add to the same row, and the wave's LDS operations are
-    // applied in issue order, so each row's sum has a fixed order: the LDS adds (ds_add_f64, IEEE
-    // double additions done by the LDS) need no wait between rounds, only before the rows are read
-    for (uint32_t r = 0; __ballot(ok && rank >= r) != 0ull; r++) {
-      if (ok && rank == r) {
-#pragma unroll
-        for (int k = 0; k < NV; k++)
-          __hip_atomic_fetch_add(&part[row][k], hv[k], __ATOMIC_RELAXED,
-                                 __HIP_MEMORY_SCOPE_WAVEFRONT);
-      }
-    }
+    add_rounds<NV>(part, k, ok, rounds, hv);
     wave_sync();
   }
 }
@@ -281,11 +374,28 @@ __device__ __forceinline__ void onepass_tile(const Problem& q, uint32_t t, uint3
 template <class WT, bool SHARED, bool RECOMPUTE = false>
 __global__ __launch_bounds__(TILE* TILE_WAVES) void k_schur_onepass(Problem q) {
   constexpr int NV = SHARED ? 7 : 6;
+  constexpr int NR = sizeof(WT) == 4 ? 9 : 1;
+  // words loaded one tile ahead: the FP32 factors (not the FP64 blocks, whose registers would cut
+  // the waves a SIMD holds, nor relinearised words, which need the tile's codes first)
+  constexpr bool AHEAD = sizeof(WT) == 4 && !RECOMPUTE;
   __shared__ double st_all[TILE_WAVES][TILE][3];
   __shared__ double part_all[TILE_WAVES][OP_SPAN][NV];
-  __shared__ WindowCams<sizeof(WT) == 4 ? 9 : 1> wc;
+  __shared__ WindowCams<NR> wc;
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / TILE), lane = threadIdx.x % TILE;
   const uint32_t w = xcd_contiguous(blockIdx.x, gridDim.x);
+  const uint32_t tw0 = q.op_tile[w], tw1 = q.op_tile[w + 1];
+  const uint32_t b = q.tile_bundle[tw0];
+  const double* sb = bstate(q, b);
+  // the workgroup's tiles are of one bundle: its PCG state decides for all of them
+  if (sb[S_PCG_ACTIVE] == 0) return;
+  const Bundle& g = q.bundles[b];
+  if (q.n_unstored != 0 && !stores_w(g)) return;
+  // the wave's tiles and their records (lane OP_RECORD j + k: word k of the wave's tile j)
+  const uint32_t t0 = tw0 + wave * OP_WAVE_TILES;
+  const uint32_t nt = t0 < tw1 ? min((uint32_t)OP_WAVE_TILES, tw1 - t0) : 0u;
+  const uint32_t rec = lane < OP_RECORD * nt ? q.op_trec[(size_t)t0 * OP_RECORD + lane] : 0u;
+  TileIn<WT, SHARED> cur;
+  if (nt) load_tile<WT, SHARED, AHEAD>(q, rec, 0, lane, cur);
   const Window win = window_of(q, w);
   const uint32_t row0 = q.op_off[w], span = q.op_off[w + 1] - row0;
   {
@@ -296,7 +406,7 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_schur_onepass(Problem q) {
       const double* v = vp(q) + (size_t)c * 6;
 #pragma unroll
       for (int k = 0; k < 6; k++) wc.p[r][k] = v[k];
-      if constexpr (sizeof(WT) == 4) {
+      if constexpr (NR == 9) {
         const double* pose = q.scene + (size_t)c * 12;
 #pragma unroll
         for (int k = 0; k < 9; k++) wc.R[r][k] = pose[k];
@@ -309,21 +419,31 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_schur_onepass(Problem q) {
 #pragma unroll
   for (int k = 0; k < NV; k++) part_all[wave][lane][k] = 0;
   wave_sync();
-  const uint32_t t0 = q.op_tile[w] + wave * OP_WAVE_TILES;
-  const uint32_t t1 = min(t0 + OP_WAVE_TILES, q.op_tile[w + 1]);
-  for (uint32_t t = t0; t < t1; t++)
-    onepass_tile<WT, SHARED, RECOMPUTE>(q, t, lane, win, wc, st_all[wave], part_all[wave]);
+  const OnePassCtx<NR> x{q, g, win, wc, SHARED ? vp(q)[6 * q.C + b] : 0.0,
+                         RECOMPUTE && SHARED ? focal_scale(sb[S_LOG_FOCAL]) : 1.0};
+  for (uint32_t j = 0; j < nt; j++) {
+    TileIn<WT, SHARED> next;
+    if (j + 1 < nt) load_tile<WT, SHARED, AHEAD>(q, rec, j + 1, lane, next);
+    const uint32_t n = record_word(rec, OP_RECORD * j + TR_OBS);
+    if (n > (uint32_t)TILE)
+      onepass_long_tile<WT, SHARED, RECOMPUTE, NR>(x, rec, j, lane, cur.vinv, st_all[wave],
+                                                   part_all[wave]);
+    else if (n)
+      onepass_tile<WT, SHARED, RECOMPUTE, AHEAD, NR>(x, rec, j, lane, cur, st_all[wave],
+                                                     part_all[wave]);
+    cur = next;
+  }
   __syncthreads();
   double* out = q.op_part + (size_t)row0 * 8;
-  for (uint32_t x = threadIdx.x; x < span * 8; x += blockDim.x) {
-    const uint32_t k = x >> 3, j = x & 7;
+  for (uint32_t e = threadIdx.x; e < span * 8; e += blockDim.x) {
+    const uint32_t k = e >> 3, c = e & 7;
     double sum = 0;
-    if (j < (uint32_t)NV) {
-      sum = part_all[0][k][j];
+    if (c < (uint32_t)NV) {
+      sum = part_all[0][k][c];
 #pragma unroll
-      for (int u = 1; u < TILE_WAVES; u++) sum += part_all[u][k][j];
+      for (int u = 1; u < TILE_WAVES; u++) sum += part_all[u][k][c];
     }
-    out[x] = sum;
+    out[e] = sum;
   }
 }
 
@@ -399,6 +519,64 @@ __global__ __launch_bounds__(256) void k_onepass_windows(Problem q, uint32_t* lo
     } else {
       lo[w] = v[0];
       span[w] = plain;
+    }
+  }
+}
+
+// Set-up, after the windows: each slot's code word and each tile's record, in the product's
+// workgroup and wave layout (wave v of workgroup w takes tiles op_tile[w] + 8 v ...). A lane's rank
+// is the number of lanes below it, in its pass, whose observation has a camera of the same row
+// (the rows' bits compared by ballots).
+__global__ __launch_bounds__(TILE* TILE_WAVES) void k_onepass_codes(Problem q, uint32_t* code,
+                                                                     uint32_t* trec) {
+  const uint32_t wave = threadIdx.x / TILE, lane = threadIdx.x % TILE;
+  const uint32_t w = blockIdx.x;
+  const Window win = window_of(q, w);
+  const uint32_t t0 = q.op_tile[w] + wave * OP_WAVE_TILES;
+  const uint32_t t1 = min(t0 + OP_WAVE_TILES, q.op_tile[w + 1]);
+  const uint32_t* ps = point_start(q);
+  for (uint32_t t = t0; t < t1; t++) {
+    const uint32_t fp = q.tiles[t], ep = q.tiles[t + 1], slot = q.tile_slot[t];
+    const uint32_t s = ps[fp], e = ps[ep];
+    const uint32_t my = fp + lane;
+    const bool has = my < ep;
+    const uint32_t mys = has ? ps[my] : 0, mye = has ? ps[my + 1] : 0;
+    uint32_t rounds = 0;
+    for (uint32_t base = s; base < e; base += TILE) {
+      const uint32_t i = base + lane;
+      const bool ok = i < e;
+      const uint32_t row = ok ? win.row(obs_camera(q)[i]) : 0u;
+      const uint32_t lp = ok ? obs_point(q)[i] - fp : 0u;
+      unsigned long long same = __ballot(ok);
+#pragma unroll
+      for (int bit = 0; bit < 6; bit++) {
+        const unsigned long long set = __ballot((row >> bit) & 1u);
+        same &= (row >> bit) & 1u ? set : ~set;
+      }
+      const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+          (uint32_t)(same >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)same, 0u));
+      uint32_t lo = 0, hi = 0;
+      if (has) {
+        const uint32_t l = mys > base ? mys : base, h = mye < base + TILE ? mye : base + TILE;
+        if (l < h) {
+          lo = l - base;
+          hi = h - base;
+        }
+      }
+      code[slot + (base - s) + lane] = slot_code(ok ? row : 0u, ok ? rank : 0u, lp, lo, hi);
+      uint32_t r = ok ? rank + 1 : 0u;
+      for (int o = 32; o > 0; o >>= 1) r = max(r, (uint32_t)__shfl_xor(r, o, 64));
+      rounds = max(rounds, r);
+    }
+    if (lane < (uint32_t)OP_RECORD) {
+      const uint32_t word = lane == TR_SLOT     ? slot
+                            : lane == TR_OBS    ? e - s
+                            : lane == TR_POINT  ? fp
+                            : lane == TR_POINTS ? ep - fp
+                            : lane == TR_FIRST  ? s
+                            : lane == TR_ROUNDS ? rounds
+                                                : 0u;
+      trec[(size_t)t * OP_RECORD + lane] = word;
     }
   }
 }
