@@ -1314,10 +1314,11 @@ int run_lm(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, hipSt
         hipLaunchKernelGGL(k_stop_pcg, dim3(blocks_for(q.B, 64)), dim3(64), 0, st, q);
       for (uint32_t k = 0; k < kMaxRefinements; k++) {
         bool pending = false;
+        // every pass that converged or broke down (k_refine leaves the failures of the PCG
+        // set-up alone)
         for (uint32_t b = 0; b < q.B; b++)
-          pending |= word(ctx, b, S_CONVERGED) != 0 && word(ctx, b, S_PCG_FAILED) == 0 &&
-                     word(ctx, b, S_ACTIVE) != 0 && word(ctx, b, S_DIRECT) != 2 &&
-                     !unstored[b];
+          pending |= (word(ctx, b, S_CONVERGED) != 0 || word(ctx, b, S_PCG_FAILED) != 0) &&
+                     word(ctx, b, S_ACTIVE) != 0 && word(ctx, b, S_DIRECT) != 2 && !unstored[b];
         if (!pending) break;
         ops.refine(q, st);
         HIP_TRY(hipGetLastError());

@@ -1881,11 +1881,17 @@ __global__ __launch_bounds__(CG_THREADS) void k_refine(Problem q) {
   const uint32_t bi = blockIdx.x;
   const Bundle& g = q.bundles[bi];
   double* s = bstate(q, bi);
-  const double converged = s[S_CONVERGED], failed = s[S_PCG_FAILED], r0 = s[S_R0];
+  const double r0 = s[S_R0];
   const double active = s[S_ACTIVE];
-  // a certified direct solution, and a bundle whose PCG ran on the FP64 implicit operator (no stored
-  // cross blocks), have an FP64 residual already
-  if (active == 0 || converged == 0 || failed != 0 || s[S_DIRECT] == 2 || !stores_w(g)) return;
+  const bool setup_failed = s[S_FAIL_TILES] != 0 || s[S_FAIL_CAMERAS] != 0 || s[S_FAIL_FOCAL] != 0;
+  // a pass that converged on its own recurrence, or broke down (p.Sp or r.z no longer positive,
+  // which the FP32 operator's rounding can bring about near convergence), is judged by the FP64
+  // residual of its last iterate. A pass that ran out of budget is judged as the FP64 form judges
+  // it (the gauge rejects the step), and a failure at the PCG set-up (a block that is not positive
+  // definite) stays. A certified direct solution, and a bundle whose PCG ran on the FP64 implicit
+  // operator (no stored cross blocks), have an FP64 residual already.
+  const bool ended = s[S_CONVERGED] != 0 || (s[S_PCG_FAILED] != 0 && !setup_failed);
+  if (active == 0 || !ended || s[S_DIRECT] == 2 || !stores_w(g)) return;
   const uint32_t tid = threadIdx.x;
   double* x = vx(q);
   double* r = vr(q);
@@ -1908,7 +1914,10 @@ __global__ __launch_bounds__(CG_THREADS) void k_refine(Problem q) {
   }
   res = block_sum(res, lds);
   if (res <= q.refine_tol * q.refine_tol * r0) {
-    if (tid == 0) s[S_R] = res;
+    if (tid == 0) {
+      s[S_R] = res;
+      s[S_PCG_FAILED] = 0;
+    }
     return;
   }
   __syncthreads();  // every row of r before the block products
@@ -1937,10 +1946,12 @@ __global__ __launch_bounds__(CG_THREADS) void k_refine(Problem q) {
     // residual, and no tighter than a tenth of the refinement target (the next FP64 residual
     // then lands below the target unless the operator's own error stops it)
     s[S_PASS_STOP] = fmax(q.tol * q.tol * res, 0.01 * q.refine_tol * q.refine_tol * r0);
-    if (rho > 0)
+    if (rho > 0) {
       s[S_PCG_ACTIVE] = 1;
-    else
+      s[S_PCG_FAILED] = 0;
+    } else {
       s[S_PCG_FAILED] = 1;
+    }
   }
 }
 

@@ -11,6 +11,10 @@
 // belong to one bundle and are counted from the bundle's first tile, so a batch sums each bundle as
 // its solve alone does. The workgroup writes one row of partials per camera of its window;
 // k_onepass_reduce sums each camera's rows in workgroup order. Every sum has a fixed order:
+//   * a point's terms (its observations are consecutive lanes of a pass) by a segmented inclusive
+//     scan over DPP lane shifts (1, 2, 4 and 8 within each row of 16 lanes, then the last lanes of
+//     rows 0 and 2 into rows 1 and 3 and that of row 1 into rows 2 and 3): the segment's last lane
+//     holds its sum, with no LDS traffic;
 //   * within a 64-observation pass, each camera row takes its lanes' terms one after another in
 //     lane order (rounds of at most one lane per row, in the order of the lane's rank among the
 //     pass's lanes of its row; the adds are LDS double additions, each row's in round order);
@@ -21,11 +25,11 @@
 //
 // What fixes those orders is fixed for a whole solve, so set-up (k_onepass_codes) writes it down
 // once: per stored slot a code word (the window row of its observation's camera, its rank, its
-// point within the tile, and the segment of the pass's observations of the tile's point numbered
-// as the slot's lane), and per tile a record of eight words (first slot, observations, first
-// point, points, first observation, rounds). A wave reads its eight tiles' records with one load
-// and, in the mixed form, loads the next tile's codes, factors and V^-1 before it works on the
-// current one, so the product waits on HBM about once per wave instead of three times per tile.
+// point within the tile, its distance from the first lane of its point's segment in the pass and
+// whether it is the segment's last), and per tile a record of eight words (first slot,
+// observations, first point, points, first observation, rounds). A wave reads its eight tiles'
+// records with one load and, in the mixed form, loads the next tile's codes, factors and V^-1
+// before it works on the current one.
 #pragma once
 
 namespace tcvba {
@@ -115,18 +119,57 @@ static_assert(OP_RECORD * OP_WAVE_TILES == TILE, "a wave's records: one word per
 
 // A slot's code word (op_code): bits 0-5 the window row of its observation's camera, 6-11 its rank
 // among the pass's observations of that row (lanes below it with the same row), 12-17 its point
-// counted from the tile's first point; bits 18-23 and 24-30 the segment [lo, hi) of the pass's
-// lanes that observe the tile's point numbered as the slot's lane (lo = hi = 0: none).
+// counted from the tile's first point, 18-23 its distance from the first lane of its point's
+// observations in the pass, bit 24 set on the last of those lanes.
 struct SlotCode {
-  uint32_t row, rank, lp, lo, hi;
+  uint32_t row, rank, lp, dist;
+  bool last;
   __device__ explicit SlotCode(uint32_t x)
-      : row(x & 63u), rank((x >> 6) & 63u), lp((x >> 12) & 63u), lo((x >> 18) & 63u),
-        hi((x >> 24) & 127u) {}
+      : row(x & 63u), rank((x >> 6) & 63u), lp((x >> 12) & 63u), dist((x >> 18) & 63u),
+        last(((x >> 24) & 1u) != 0) {}
 };
 
 __device__ __forceinline__ uint32_t slot_code(uint32_t row, uint32_t rank, uint32_t lp,
-                                              uint32_t lo, uint32_t hi) {
-  return row | rank << 6 | lp << 12 | lo << 18 | hi << 24;
+                                              uint32_t dist, bool last) {
+  return row | rank << 6 | lp << 12 | dist << 18 | (last ? 1u : 0u) << 24;
+}
+
+// DPP lane moves of a double (two 32-bit moves): lane l takes the value of lane l - K of its row of
+// 16 lanes (0 where that lane is in another row), or, for the broadcasts, of the last lane of the
+// row before (rows 1 and 3 from lanes 15 and 47; rows 2 and 3 from lane 31; 0 in the other rows).
+enum : int { kDppRowShr = 0x110, kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143 };
+
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_move(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, ROW_MASK, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, ROW_MASK, 0xf, true);
+  return __longlong_as_double((long long)(((unsigned long long)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+
+// One step of the segmented scan: x += the moved x where `take`, x unchanged elsewhere.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ void scan_step(double x[3], bool take) {
+  double m[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) m[c] = dpp_move<CTRL, ROW_MASK>(x[c]);
+#pragma unroll
+  for (int c = 0; c < 3; c++) x[c] = take ? x[c] + m[c] : x[c];
+}
+
+// Segmented inclusive scan over the wave's 64 lanes: lane l, whose segment began `dist` lanes
+// before it, ends with the sum of x over its segment's lanes up to l (Hillis-Steele within each
+// row of 16 lanes, then the carries across rows). Every lane of the wave takes part.
+__device__ __forceinline__ void segmented_scan3(double x[3], uint32_t dist, uint32_t lane) {
+  const uint32_t in_row = lane & 15u, row = lane >> 4;
+  scan_step<kDppRowShr + 1, 0xf>(x, dist >= 1 && in_row >= 1);
+  scan_step<kDppRowShr + 2, 0xf>(x, dist >= 2 && in_row >= 2);
+  scan_step<kDppRowShr + 4, 0xf>(x, dist >= 4 && in_row >= 4);
+  scan_step<kDppRowShr + 8, 0xf>(x, dist >= 8 && in_row >= 8);
+  // lanes 15 and 47 into rows 1 and 3 where the segment began in the row before
+  scan_step<kDppRowBcast15, 0xa>(x, (row & 1u) != 0 && dist > in_row);
+  // lane 31 into rows 2 and 3 where the segment began at or before lane 31
+  scan_step<kDppRowBcast31, 0xc>(x, dist > (lane & 31u));
 }
 
 // The stored words of one observation that the two halves share. The mixed form keeps its FP32
@@ -291,18 +334,19 @@ __device__ __forceinline__ void onepass_tile(const OnePassCtx<NR>& x, uint32_t r
     else
       in.ow.load_stored(stored_block<WT, SHARED>(x.q, slot >> 6), lane);
   }
-  if (ok) {
-    double tj[3];
-    in.ow.point_term(x.wc.R[k.row], x.wc.p[k.row], x.vf, var, dir, tj);
-    st[lane][0] = tj[0];
-    st[lane][1] = tj[1];
-    st[lane][2] = tj[2];
+  // the point half: each point's terms summed by the scan; its last lane hands the sum to the
+  // point's lane (st[lp]), which forms y_p = V_p^-1 sum
+  double tj[3] = {0, 0, 0};
+  if (ok) in.ow.point_term(x.wc.R[k.row], x.wc.p[k.row], x.vf, var, dir, tj);
+  segmented_scan3(tj, k.dist, lane);
+  if (ok && k.last) {
+    st[k.lp][0] = tj[0];
+    st[k.lp][1] = tj[1];
+    st[k.lp][2] = tj[2];
   }
   wave_sync();
-  double acc[3] = {0, 0, 0};
-  if (lane < np) segment_sum(st, 0, k.lo, k.hi, acc);
-  wave_sync();
   if (lane < np) {
+    const double acc[3] = {st[lane][0], st[lane][1], st[lane][2]};
 #pragma unroll
     for (int c = 0; c < 3; c++)
       st[lane][c] = in.vinv[c * 3] * acc[0] + in.vinv[c * 3 + 1] * acc[1] + in.vinv[c * 3 + 2] * acc[2];
@@ -334,20 +378,29 @@ __device__ __forceinline__ void onepass_long_tile(const OnePassCtx<NR>& x, uint3
     else
       ow.load_stored(stored_block<WT, SHARED>(x.q, (slot + base) >> 6), lane);
   };
+  // the point's terms pass by pass: each pass's sum (the scan's last lane) added in pass order
   double acc[3] = {0, 0, 0};
   for (uint32_t base = 0; base < n; base += TILE) {
     const SlotCode k(x.q.op_code[slot + base + lane]);
-    if (base + lane < n) {
+    const bool ok = base + lane < n;
+    double tj[3] = {0, 0, 0};
+    if (ok) {
       load(base, k);
-      double tj[3];
       ow.point_term(x.wc.R[k.row], x.wc.p[k.row], x.vf, x.wc.var[k.row] != 0,
                     directional_point(x.g, fp + k.lp), tj);
-      st[lane][0] = tj[0];
-      st[lane][1] = tj[1];
-      st[lane][2] = tj[2];
+    }
+    segmented_scan3(tj, k.dist, lane);
+    if (ok && k.last) {
+      st[0][0] = tj[0];
+      st[0][1] = tj[1];
+      st[0][2] = tj[2];
     }
     wave_sync();
-    if (lane < np) segment_sum(st, 0, k.lo, k.hi, acc);
+    if (lane < np) {
+      acc[0] += st[0][0];
+      acc[1] += st[0][1];
+      acc[2] += st[0][2];
+    }
     wave_sync();
   }
   if (lane < np) {
@@ -538,15 +591,14 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_onepass_codes(Problem q, u
   for (uint32_t t = t0; t < t1; t++) {
     const uint32_t fp = q.tiles[t], ep = q.tiles[t + 1], slot = q.tile_slot[t];
     const uint32_t s = ps[fp], e = ps[ep];
-    const uint32_t my = fp + lane;
-    const bool has = my < ep;
-    const uint32_t mys = has ? ps[my] : 0, mye = has ? ps[my + 1] : 0;
     uint32_t rounds = 0;
     for (uint32_t base = s; base < e; base += TILE) {
       const uint32_t i = base + lane;
       const bool ok = i < e;
       const uint32_t row = ok ? win.row(obs_camera(q)[i]) : 0u;
-      const uint32_t lp = ok ? obs_point(q)[i] - fp : 0u;
+      const uint32_t pt = ok ? obs_point(q)[i] : fp;
+      // the lanes of the point's observations in this pass: [first, end)
+      const uint32_t first = ok ? max(ps[pt], base) : i, end = ok ? min(ps[pt + 1], base + TILE) : i + 1;
       unsigned long long same = __ballot(ok);
 #pragma unroll
       for (int bit = 0; bit < 6; bit++) {
@@ -555,15 +607,8 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_onepass_codes(Problem q, u
       }
       const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
           (uint32_t)(same >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)same, 0u));
-      uint32_t lo = 0, hi = 0;
-      if (has) {
-        const uint32_t l = mys > base ? mys : base, h = mye < base + TILE ? mye : base + TILE;
-        if (l < h) {
-          lo = l - base;
-          hi = h - base;
-        }
-      }
-      code[slot + (base - s) + lane] = slot_code(ok ? row : 0u, ok ? rank : 0u, lp, lo, hi);
+      code[slot + (base - s) + lane] =
+          slot_code(ok ? row : 0u, ok ? rank : 0u, pt - fp, i - first, i + 1 == end);
       uint32_t r = ok ? rank + 1 : 0u;
       for (int o = 32; o > 0; o >>= 1) r = max(r, (uint32_t)__shfl_xor(r, o, 64));
       rounds = max(rounds, r);
