@@ -15,9 +15,9 @@
 //     scan over DPP lane shifts (1, 2, 4 and 8 within each row of 16 lanes, then the last lanes of
 //     rows 0 and 2 into rows 1 and 3 and that of row 1 into rows 2 and 3): the segment's last lane
 //     holds its sum, with no LDS traffic;
-//   * within a 64-observation pass, each camera row takes its lanes' terms one after another in
-//     lane order (rounds of at most one lane per row, in the order of the lane's rank among the
-//     pass's lanes of its row; the adds are LDS double additions, each row's in round order);
+//   * within a 64-observation pass, each camera row component takes its lanes' terms in an order
+//     fixed by the lanes' ranks among the pass's lanes of that row (add_terms: in each LDS add of
+//     the pass the lanes of one row add distinct components, so no two lanes meet);
 //   * a wave adds its passes to its partials in tile order, the workgroup its four waves in wave
 //     order, the reduction the workgroups in index order.
 // So the product is deterministic (and the streamed route, which recomputes the words, gives the
@@ -113,7 +113,7 @@ enum : uint32_t {
   TR_POINT = 2,   // first point
   TR_POINTS = 3,  // points
   TR_FIRST = 4,   // first observation (point-major)
-  TR_ROUNDS = 5,  // rounds of the camera half: the largest rank of any pass, plus one
+  TR_ROUNDS = 5,  // the largest rank of any of the tile's passes, plus one
 };
 static_assert(OP_RECORD * OP_WAVE_TILES == TILE, "a wave's records: one word per lane");
 
@@ -284,19 +284,45 @@ __device__ __forceinline__ void load_tile(const Problem& q, uint32_t rec, uint32
   }
 }
 
-// The camera half's adds of one pass: round r adds the terms of the lanes of rank r, at most one
-// per row. The wave's LDS operations are applied in issue order, so each row's sum has a fixed
-// order; the LDS adds (ds_add_f64, IEEE double additions done by the LDS) need no wait between
-// rounds, only before the rows are read.
-template <int NV>
-__device__ __forceinline__ void add_rounds(double (*part)[NV], const SlotCode& k, bool ok,
-                                           uint32_t rounds, const double* hv) {
-  for (uint32_t r = 0; r < rounds; r++) {
-    if (ok && k.rank == r) {
+// h[j] = h[(j + S) % NV] where `on` (a register rotation by S, as selects).
+template <int NV, int S>
+__device__ __forceinline__ void rotate_if(double* h, bool on) {
+  double t[NV];
 #pragma unroll
-      for (int x = 0; x < NV; x++)
-        __hip_atomic_fetch_add(&part[k.row][x], hv[x], __ATOMIC_RELAXED,
+  for (int j = 0; j < NV; j++) t[j] = h[(j + S) % NV];
+#pragma unroll
+  for (int j = 0; j < NV; j++) h[j] = on ? t[j] : h[j];
+}
+
+// The camera half's adds of one pass, conflict-free and dense: the lanes are taken in groups of NV
+// ranks (rank / NV), and in step j of a group the lane of rank rho = rank % NV adds component
+// (rho + j) % NV of its term to its row. Lanes of one row have distinct ranks, so within a step
+// they add to distinct components, and NV steps (one LDS add each, every lane of the group active)
+// add every term. Each row component takes its lanes' terms in an order fixed by their ranks
+// (group by group; within a group rho = c, c - 1, ..., c + 1 mod NV for component c): the wave's
+// LDS operations are applied in issue order, and the LDS adds (ds_add_f64, IEEE double additions
+// done by the LDS) need no wait between steps, only before the rows are read.
+template <int NV>
+__device__ __forceinline__ void add_terms(double (*part)[NV], const SlotCode& k, bool ok,
+                                          uint32_t rounds, const double* hv) {
+  static_assert(NV <= 8, "the rotation covers ranks 0..7 of a group");
+  const uint32_t rho = k.rank % NV, grp = k.rank / NV;
+  double hr[NV];
+#pragma unroll
+  for (int j = 0; j < NV; j++) hr[j] = hv[j];
+  rotate_if<NV, 1>(hr, (rho & 1u) != 0);
+  rotate_if<NV, 2>(hr, (rho & 2u) != 0);
+  rotate_if<NV, 4>(hr, (rho & 4u) != 0);
+  const uint32_t groups = (rounds + NV - 1) / NV;
+  for (uint32_t gi = 0; gi < groups; gi++) {
+    const bool mine = ok && grp == gi;
+    uint32_t c = rho;
+#pragma unroll
+    for (int j = 0; j < NV; j++) {
+      if (mine)
+        __hip_atomic_fetch_add(&part[k.row][c], hr[j], __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_WAVEFRONT);
+      c = c + 1 == (uint32_t)NV ? 0u : c + 1;
     }
   }
 }
@@ -354,7 +380,7 @@ __device__ __forceinline__ void onepass_tile(const OnePassCtx<NR>& x, uint32_t r
   wave_sync();
   double hv[NV];
   if (ok) in.ow.camera_term(x.wc.R[k.row], st[k.lp], var, dir, hv);
-  add_rounds<NV>(part, k, ok, rounds, hv);
+  add_terms<NV>(part, k, ok, rounds, hv);
   wave_sync();
 }
 
@@ -418,7 +444,7 @@ __device__ __forceinline__ void onepass_long_tile(const OnePassCtx<NR>& x, uint3
       ow.camera_term(x.wc.R[k.row], st[k.lp], x.wc.var[k.row] != 0,
                      directional_point(x.g, fp + k.lp), hv);
     }
-    add_rounds<NV>(part, k, ok, rounds, hv);
+    add_terms<NV>(part, k, ok, rounds, hv);
     wave_sync();
   }
 }
