@@ -31,15 +31,15 @@ METRIC = ("LM iters/sec + obs/sec, mixed vs FP64 at ≤1e-6 rel cost; Schur matv
 HBM_PEAK_GBS = 8000.0
 
 # HBM bytes of one Schur product measured with `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE`
-# over a 2-LM-iteration solve (profiles/schur_trajectory2000_traffic.md): FETCH_SIZE x 2.00 and
+# over `bench.py --mode probe` (profiles/schur_trajectory2000_traffic.md): FETCH_SIZE x 2.00 and
 # WRITE_SIZE x 1.00, the factors that make a 1 GiB device copy read and write 1 GiB. Keyed by
 # (cameras, points, seed, precision, operator); other workloads report null.
 MEASURED_TRAFFIC = {
     # the one-pass product (k_schur_onepass + k_onepass_reduce)
     (2000, 1_000_000, 2, "fp64", "onepass"):
-        2 * (1091537.4 + 7673.8) * 1024 + (11394.9 + 625.0) * 1024,
+        2 * (936356.6 + 7674.3) * 1024 + (11392.0 + 625.0) * 1024,
     (2000, 1_000_000, 2, "mixed", "onepass"):
-        2 * (410496.5 + 7673.8) * 1024 + (11390.9 + 625.0) * 1024,
+        2 * (299037.2 + 7674.3) * 1024 + (11389.6 + 625.0) * 1024,
 }
 
 
@@ -65,15 +65,18 @@ def matvec_bytes(C: int, P: int, M: int, shared: bool, factored: bool = False) -
     return total
 
 
-def onepass_bytes(C: int, P: int, M: int, shared: bool, factored: bool, rows: int) -> int:
+def onepass_bytes(C: int, P: int, M: int, shared: bool, factored: bool, rows: int,
+                  tiles: int) -> int:
     """Algorithmic HBM bytes of one one-pass Schur product (k_schur_onepass + k_onepass_reduce):
     every observation's stored words read once (FP64 cross blocks: 21 doubles shared, 18 fixed;
-    the mixed form's FP32 factors: 11 or 9 floats), its camera and point ids; V^-1 and the point
-    offsets; the camera vectors p read, the task partials written, and, for the factors, each
-    camera's rotation; the rows of camera partials written once and read once (8 doubles each)."""
+    the mixed form's FP32 factors: 11 or 9 floats) and its code word (window row, rank, tile
+    point, place in its point's run); V^-1 of every point; each tile's 8-word record; the camera
+    vectors p read, the task partials written, and, for the factors, each camera's rotation; the
+    rows of camera partials written once and read once (8 doubles each)."""
     per_obs = (11 if shared else 9) * 4 if factored else (21 if shared else 18) * 8
     per_cam = 6 * 8 + 8 * 8 + (9 * 8 if factored else 0)
-    return M * (per_obs + 8) + P * (9 * 8 + 4) + C * per_cam + rows * 8 * 8 * 2 + (8 if shared else 0)
+    return (M * (per_obs + 4) + P * 9 * 8 + tiles * 8 * 4 + C * per_cam + rows * 8 * 8 * 2
+            + (8 if shared else 0))
 
 
 def solve_forms(args, B, _native, lib, device, packed, scene, index, out, precision, steps,
@@ -119,7 +122,8 @@ def form_record(packed, seed, precision, operator, reports, elapsed, values):
     timed = sum(r["timed_matvecs"] for r in reports)  # the engine times every 8th product
     rows = reports[-1].get("onepass_rows", 0)
     onepass = rows > 0
-    bytes_per = (onepass_bytes(C, P, M, True, precision == "mixed", rows) if onepass else
+    bytes_per = (onepass_bytes(C, P, M, True, precision == "mixed", rows,
+                               reports[-1]["point_tiles"]) if onepass else
                  matvec_bytes(C, P, M, True, factored=precision == "mixed"))
     mv_ms = sum(r["schur_ms"] for r in reports) / max(1, timed)
     achieved = bytes_per / (mv_ms * 1e-3) / 1e9 if mv_ms > 0 else 0.0
