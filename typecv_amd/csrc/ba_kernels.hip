@@ -1500,11 +1500,11 @@ struct Group {
   bool has, first;
 };
 
-// The workgroup's group: one record of GROUP_WORDS words (bundle, first and end camera, the
-// bundle's first and end group), loaded as one scalar block.
-__device__ __forceinline__ Group group_of(const Problem& q) {
+// Group gi: one record of GROUP_WORDS words (bundle, first and end camera, the bundle's first and
+// end group), loaded as one scalar block.
+__device__ __forceinline__ Group group_at(const Problem& q, uint32_t gi) {
   Group G;
-  const uint32_t* w = q.groups + (size_t)GROUP_WORDS * blockIdx.x;
+  const uint32_t* w = q.groups + (size_t)GROUP_WORDS * gi;
   G.bundle = w[0];
   G.c0 = w[1];
   G.c1 = w[2];
@@ -1513,7 +1513,7 @@ __device__ __forceinline__ Group group_of(const Problem& q) {
   G.rows = 6 * (G.c1 - G.c0);
   G.row = 6 * G.c0 + threadIdx.x;
   G.has = threadIdx.x < G.rows;
-  G.first = blockIdx.x == G.g0;
+  G.first = gi == G.g0;
   return G;
 }
 
@@ -1533,11 +1533,17 @@ __device__ __forceinline__ void group_loads(const Problem& q, const Group& G, in
 // bundle's state: the chain of dependent loads from the group record to the first store is then
 // two loads deep (three in k_cg_product, whose task partials follow the camera's task range)
 // instead of one per test. Threads past the group's rows load row 6 c0 and store nothing.
+// The LDS of the PCG-step bodies: a row exchange and the block sums' slots.
+struct CgLds {
+  double rr[ADV_THREADS];
+  double lds[2 * (ADV_THREADS / 64)];
+};
+
 template <bool SHARED>
-__global__ __launch_bounds__(ADV_THREADS) void k_cg_product(Problem q) {
+__device__ __forceinline__ void cg_product_group(const Problem& q, uint32_t gi, CgLds& L) {
   constexpr int NV = SHARED ? 7 : 6;
-  __shared__ double lds[2 * (ADV_THREADS / 64)];
-  const Group G = group_of(q);
+  double* lds = L.lds;
+  const Group G = group_at(q, gi);
   double* s = bstate(q, G.bundle);
   const double* p = vp(q);
   const uint32_t row = G.has ? G.row : 6 * G.c0, c = row / 6, j = row - 6 * c;
@@ -1577,17 +1583,23 @@ __global__ __launch_bounds__(ADV_THREADS) void k_cg_product(Problem q) {
   }
   block_sum2(d, fs, lds);
   if (threadIdx.x == 0) {
-    q.group_part[4 * (size_t)blockIdx.x] = d;
-    q.group_part[4 * (size_t)blockIdx.x + 1] = fs;
+    q.group_part[4 * (size_t)gi] = d;
+    q.group_part[4 * (size_t)gi + 1] = fs;
     if (G.first) s[S_RHO] = rho_next;
   }
 }
 
 template <bool SHARED>
-__global__ __launch_bounds__(ADV_THREADS) void k_cg_step(Problem q) {
-  __shared__ double rr[ADV_THREADS];
-  __shared__ double lds[2 * (ADV_THREADS / 64)];
-  const Group G = group_of(q);
+__global__ __launch_bounds__(ADV_THREADS) void k_cg_product(Problem q) {
+  __shared__ CgLds L;
+  cg_product_group<SHARED>(q, blockIdx.x, L);
+}
+
+template <bool SHARED>
+__device__ __forceinline__ void cg_step_group(const Problem& q, uint32_t gi, CgLds& L) {
+  double* rr = L.rr;
+  double* lds = L.lds;
+  const Group G = group_at(q, gi);
   double* s = bstate(q, G.bundle);
   const uint32_t tid = threadIdx.x, f = 6 * q.C + G.bundle;
   const uint32_t row = G.has ? G.row : 6 * G.c0, c = row / 6, j = row - 6 * c;
@@ -1666,15 +1678,21 @@ __global__ __launch_bounds__(ADV_THREADS) void k_cg_step(Problem q) {
   }
   block_sum2(res, rz, lds);
   if (tid == 0) {
-    q.group_part[4 * (size_t)blockIdx.x + 2] = res;
-    q.group_part[4 * (size_t)blockIdx.x + 3] = rz;
+    q.group_part[4 * (size_t)gi + 2] = res;
+    q.group_part[4 * (size_t)gi + 3] = rz;
   }
 }
 
 template <bool SHARED>
-__global__ __launch_bounds__(ADV_THREADS) void k_cg_direction(Problem q) {
-  __shared__ double lds[2 * (ADV_THREADS / 64)];
-  const Group G = group_of(q);
+__global__ __launch_bounds__(ADV_THREADS) void k_cg_step(Problem q) {
+  __shared__ CgLds L;
+  cg_step_group<SHARED>(q, blockIdx.x, L);
+}
+
+template <bool SHARED>
+__device__ __forceinline__ void cg_direction_group(const Problem& q, uint32_t gi, CgLds& L) {
+  double* lds = L.lds;
+  const Group G = group_at(q, gi);
   double* s = bstate(q, G.bundle);
   const uint32_t f = 6 * q.C + G.bundle, row = G.has ? G.row : 6 * G.c0;
   double* p = vp(q);
@@ -1722,6 +1740,12 @@ __global__ __launch_bounds__(ADV_THREADS) void k_cg_direction(Problem q) {
   if (converged || broke) return;
   if (G.has) p[G.row] = zr + beta * pr;
   if (SHARED && G.first && threadIdx.x == 0) p[f] = zf + beta * pf;
+}
+
+template <bool SHARED>
+__global__ __launch_bounds__(ADV_THREADS) void k_cg_direction(Problem q) {
+  __shared__ CgLds L;
+  cg_direction_group<SHARED>(q, blockIdx.x, L);
 }
 
 // The linear-system probe (bundle 0): S v, M^-1 v and the RHS of the assembled system.

@@ -449,19 +449,27 @@ __device__ __forceinline__ void onepass_long_tile(const OnePassCtx<NR>& x, uint3
   }
 }
 
-// One workgroup per op_tile range (OP_WAVE_TILES tiles per wave), XCD-contiguous order.
-template <class WT, bool SHARED, bool RECOMPUTE = false>
-__global__ __launch_bounds__(TILE* TILE_WAVES) void k_schur_onepass(Problem q) {
+// The LDS of one one-pass workgroup: each wave's point terms and y (st) and partial rows, and the
+// window's cameras.
+template <int NV, int NR>
+struct OnePassLds {
+  double st[TILE_WAVES][TILE][3];
+  double part[TILE_WAVES][OP_SPAN][NV];
+  WindowCams<NR> wc;
+};
+
+// One-pass workgroup w (its op_tile range, OP_WAVE_TILES tiles per wave). Called by every thread of
+// a workgroup of TILE * TILE_WAVES threads; returns with L free for reuse.
+template <class WT, bool SHARED, bool RECOMPUTE>
+__device__ __forceinline__ void onepass_workgroup(const Problem& q, uint32_t w,
+                                                  OnePassLds<SHARED ? 7 : 6,
+                                                             sizeof(WT) == 4 ? 9 : 1>& L) {
   constexpr int NV = SHARED ? 7 : 6;
   constexpr int NR = sizeof(WT) == 4 ? 9 : 1;
   // words loaded one tile ahead: the FP32 factors (not the FP64 blocks, whose registers would cut
   // the waves a SIMD holds, nor relinearised words, which need the tile's codes first)
   constexpr bool AHEAD = sizeof(WT) == 4 && !RECOMPUTE;
-  __shared__ double st_all[TILE_WAVES][TILE][3];
-  __shared__ double part_all[TILE_WAVES][OP_SPAN][NV];
-  __shared__ WindowCams<NR> wc;
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / TILE), lane = threadIdx.x % TILE;
-  const uint32_t w = xcd_contiguous(blockIdx.x, gridDim.x);
   const uint32_t tw0 = q.op_tile[w], tw1 = q.op_tile[w + 1];
   const uint32_t b = q.tile_bundle[tw0];
   const double* sb = bstate(q, b);
@@ -484,32 +492,32 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_schur_onepass(Problem q) {
       const uint32_t c = win.camera(r);
       const double* v = vp(q) + (size_t)c * 6;
 #pragma unroll
-      for (int k = 0; k < 6; k++) wc.p[r][k] = v[k];
+      for (int k = 0; k < 6; k++) L.wc.p[r][k] = v[k];
       if constexpr (NR == 9) {
         const double* pose = q.scene + (size_t)c * 12;
 #pragma unroll
-        for (int k = 0; k < 9; k++) wc.R[r][k] = pose[k];
+        for (int k = 0; k < 9; k++) L.wc.R[r][k] = pose[k];
       }
-      wc.var[r] = variable_camera(q, c) ? 1u : 0u;
+      L.wc.var[r] = variable_camera(q, c) ? 1u : 0u;
     }
   }
   __syncthreads();
   static_assert(OP_SPAN == TILE, "one partial row per lane");
 #pragma unroll
-  for (int k = 0; k < NV; k++) part_all[wave][lane][k] = 0;
+  for (int k = 0; k < NV; k++) L.part[wave][lane][k] = 0;
   wave_sync();
-  const OnePassCtx<NR> x{q, g, win, wc, SHARED ? vp(q)[6 * q.C + b] : 0.0,
+  const OnePassCtx<NR> x{q, g, win, L.wc, SHARED ? vp(q)[6 * q.C + b] : 0.0,
                          RECOMPUTE && SHARED ? focal_scale(sb[S_LOG_FOCAL]) : 1.0};
   for (uint32_t j = 0; j < nt; j++) {
     TileIn<WT, SHARED> next;
     if (j + 1 < nt) load_tile<WT, SHARED, AHEAD>(q, rec, j + 1, lane, next);
     const uint32_t n = record_word(rec, OP_RECORD * j + TR_OBS);
     if (n > (uint32_t)TILE)
-      onepass_long_tile<WT, SHARED, RECOMPUTE, NR>(x, rec, j, lane, cur.vinv, st_all[wave],
-                                                   part_all[wave]);
+      onepass_long_tile<WT, SHARED, RECOMPUTE, NR>(x, rec, j, lane, cur.vinv, L.st[wave],
+                                                   L.part[wave]);
     else if (n)
-      onepass_tile<WT, SHARED, RECOMPUTE, AHEAD, NR>(x, rec, j, lane, cur, st_all[wave],
-                                                     part_all[wave]);
+      onepass_tile<WT, SHARED, RECOMPUTE, AHEAD, NR>(x, rec, j, lane, cur, L.st[wave],
+                                                     L.part[wave]);
     cur = next;
   }
   __syncthreads();
@@ -518,23 +526,28 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_schur_onepass(Problem q) {
     const uint32_t k = e >> 3, c = e & 7;
     double sum = 0;
     if (c < (uint32_t)NV) {
-      sum = part_all[0][k][c];
+      sum = L.part[0][k][c];
 #pragma unroll
-      for (int u = 1; u < TILE_WAVES; u++) sum += part_all[u][k][c];
+      for (int u = 1; u < TILE_WAVES; u++) sum += L.part[u][k][c];
     }
     out[e] = sum;
   }
+  __syncthreads();
+}
+
+// One workgroup per op_tile range, XCD-contiguous order.
+template <class WT, bool SHARED, bool RECOMPUTE = false>
+__global__ __launch_bounds__(TILE* TILE_WAVES) void k_schur_onepass(Problem q) {
+  __shared__ OnePassLds<SHARED ? 7 : 6, sizeof(WT) == 4 ? 9 : 1> L;
+  onepass_workgroup<WT, SHARED, RECOMPUTE>(q, xcd_contiguous(blockIdx.x, gridDim.x), L);
 }
 
 // Each camera's rows of partials summed in workgroup order (a wave per camera: lanes take the rows
 // in turn, then a butterfly), written as the camera sweep's task partials: the whole sum in the
 // camera's first task, zero in the others.
 template <bool SHARED>
-__global__ __launch_bounds__(256) void k_onepass_reduce(Problem q) {
+__device__ __forceinline__ void onepass_reduce_camera(const Problem& q, uint32_t c, uint32_t lane) {
   constexpr int NV = SHARED ? 7 : 6;
-  const uint32_t c = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / 64);
-  const uint32_t lane = threadIdx.x & 63;
-  if (c >= q.C) return;
   const uint32_t b = q.camera_bundle[c];
   if (bstate(q, b)[S_PCG_ACTIVE] == 0) return;
   if (q.n_unstored != 0 && !stores_w(q.bundles[b])) return;  // the unstored sweeps write those
@@ -552,6 +565,12 @@ __global__ __launch_bounds__(256) void k_onepass_reduce(Problem q) {
   const uint32_t t0 = q.task_start[c], t1 = q.task_start[c + 1];
   if (lane < (uint32_t)NV)
     for (uint32_t t = t0; t < t1; t++) q.task_part[(size_t)t * 8 + lane] = t == t0 ? acc[lane] : 0.0;
+}
+
+template <bool SHARED>
+__global__ __launch_bounds__(256) void k_onepass_reduce(Problem q) {
+  const uint32_t c = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / 64);
+  if (c < q.C) onepass_reduce_camera<SHARED>(q, c, threadIdx.x & 63);
 }
 
 // Set-up: the camera window of each one-pass workgroup (Window): the narrower of the plain and the
