@@ -37,9 +37,9 @@ HBM_PEAK_GBS = 8000.0
 MEASURED_TRAFFIC = {
     # the one-pass product (k_schur_onepass + k_onepass_reduce)
     (2000, 1_000_000, 2, "fp64", "onepass"):
-        2 * (936356.6 + 7674.3) * 1024 + (11392.0 + 625.0) * 1024,
+        2 * (893176.4 + 7666.8) * 1024 + (11392.4 + 625.0) * 1024,
     (2000, 1_000_000, 2, "mixed", "onepass"):
-        2 * (299037.2 + 7674.3) * 1024 + (11389.6 + 625.0) * 1024,
+        2 * (288598.6 + 7666.8) * 1024 + (11389.7 + 625.0) * 1024,
 }
 
 

@@ -26,9 +26,9 @@ pytestmark = pytest.mark.gpu
 
 
 def tile_slots(control, packed) -> int:
-    """Slots of the point-major stored words: the engine's tiles of whole points (at most 64
-    observations and 64 points, a longer point alone), each from a 64-observation block of its
-    own."""
+    """Slots of the engine's tiles of whole points (at most 64 observations and 64 points, a longer
+    point alone), each tile a whole number of 64-slot passes: the count the engine sizes its
+    camera-major stored words by."""
     M, P = control.observations, control.points
     ps = np.asarray(packed.index[2 * M: 2 * M + P + 1], dtype=np.int64)
     slots, first = 0, 0
@@ -40,14 +40,15 @@ def tile_slots(control, packed) -> int:
 
 
 def stored_bytes(control, packed) -> int:
-    """Bytes of the arrays only a resident solve holds: the point-major and camera-major cross
-    blocks (FP64: K = 18 or 21 words per observation slot; the mixed form's FP32 Jacobian factors:
-    9 or 11) and, with FP64 blocks, the fused product's 64-byte rows."""
+    """Bytes of the arrays only a resident solve holds: the point-major cross blocks (FP64: K = 18
+    or 21 words per observation; the mixed form's FP32 Jacobian factors: 9 or 11), the camera-major
+    ones (K words per slot of 64-slot blocks) and, with FP64 blocks, the fused product's 64-byte
+    rows."""
     mixed = control.precision == _native.MIXED
     K = (11 if control.shared_focal else 9) if mixed else (21 if control.shared_focal else 18)
     ws = 4 if mixed else 8
-    words = K * tile_slots(control, packed)
-    return 2 * words * ws + (64 * control.observations if ws == 8 else 0)
+    words = K * control.observations + K * tile_slots(control, packed)
+    return words * ws + (64 * control.observations if ws == 8 else 0)
 
 
 def scenes():

@@ -692,6 +692,7 @@ std::vector<Need> solve_needs(tcv_ba_context* ctx, const tcv_ba_control* c, uint
   const size_t n_cost = plan.cost_blocks.size() / 3, n_groups = plan.groups.size() / GROUP_WORDS;
   // the cross blocks (FP64, K words per observation) or the mixed form's FP32 factors (KF words)
   const size_t w_words = (size_t)(ws == 4 ? kf_words(shared) : K) * plan.tile_slot.back();
+  const size_t wp_words = (size_t)(ws == 4 ? kf_words(shared) : K) * M;  // tile_word: no padding
   std::vector<Need> n = {
       {&ctx->scene, ((size_t)C * 18 + (size_t)P * 3 + (size_t)M * 2) * 8},
       {&ctx->trial, ((size_t)C * 12 + (size_t)P * 3) * 8},
@@ -722,7 +723,7 @@ std::vector<Need> solve_needs(tcv_ba_context* ctx, const tcv_ba_control* c, uint
       {&ctx->groups, plan.groups.size() * 4},
   };
   if (n_unstored < B && !streamed) {
-    n.push_back({&ctx->Wp, w_words * ws});
+    n.push_back({&ctx->Wp, wp_words * ws});
     n.push_back({&ctx->Wc, w_words * ws});
     // the fused FP64 product's rows (the tuner times it against the plain sweeps, which compute
     // the same bits); FP64 blocks only: over FP32 ones it is slower (70.8 + 27.4 against 42.6 +
@@ -989,7 +990,7 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
     // every sweep (the same bits)
     if (q.n_unstored < B && ctx->budget) q.streamed = 1;
     if (q.n_unstored < B && !q.streamed) {
-      TRY(ensure(ctx->Wp, w_words * ws));
+      TRY(ensure(ctx->Wp, (size_t)(ws == 4 ? kf_words(shared) : K) * M * ws));
       TRY(ensure(ctx->Wc, w_words * ws));
       q.Wp = ctx->Wp.ptr;
       q.Wc = ctx->Wc.ptr;

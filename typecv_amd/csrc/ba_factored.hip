@@ -18,11 +18,12 @@
 
 namespace tcvba {
 
-// The stored factors at position `slot` of `base` (observation i, camera c, point p), widened;
-// RECOMPUTE: relinearised from the scene and rounded as the passes store them (the streamed route,
-// the same bits).
+// The stored factors of observation i (camera c, point p), widened: word k at w0 + k * stride
+// (the point-major words: a tile's row length; the camera-major ones: 64); RECOMPUTE:
+// relinearised from the scene and rounded as the passes store them (the streamed route, the same
+// bits; w0 unused).
 template <bool SHARED, bool RECOMPUTE>
-__device__ __forceinline__ void load_factors(const Problem& q, const void* base, uint32_t slot,
+__device__ __forceinline__ void load_factors(const Problem& q, const float* w0, uint32_t stride,
                                              uint32_t i, uint32_t c, uint32_t p, const Bundle& g,
                                              double fscale, double* f) {
   constexpr int KF = kf_words(SHARED);
@@ -36,8 +37,7 @@ __device__ __forceinline__ void load_factors(const Problem& q, const void* base,
     for (int k = 0; k < KF; k++) f[k] = (double)(float)raw[k];
   } else {
 #pragma unroll
-    for (int k = 0; k < KF; k++)
-      f[k] = (double)__builtin_nontemporal_load(fptr<SHARED>(const_cast<void*>(base), k, slot));
+    for (int k = 0; k < KF; k++) f[k] = (double)__builtin_nontemporal_load(w0 + k * stride);
   }
   if (!SHARED) f[9] = f[10] = 0;
 }
@@ -84,7 +84,9 @@ __device__ __forceinline__ void sweep_tile_factored(const Problem& q, uint32_t t
       const uint32_t c = obs_camera(q)[i], pt = obs_point(q)[i];
       const bool var = variable_camera(q, c), dir = directional_point(g, pt);
       double f[11], R[9], vc[6];
-      load_factors<SHARED, RECOMPUTE>(q, q.Wp, q.tile_slot[t] + (i - s), i, c, pt, g, fscale, f);
+      const float* w0 =
+          RECOMPUTE ? nullptr : tile_word<float>(q.Wp, kf_words(SHARED), s, e - s, 0, i);
+      load_factors<SHARED, RECOMPUTE>(q, w0, e - s, i, c, pt, g, fscale, f);
       const double* pose = q.scene + (size_t)c * 12;
 #pragma unroll
       for (int k = 0; k < 9; k++) R[k] = pose[k];
@@ -171,8 +173,8 @@ __global__ __launch_bounds__(TASK_THREADS) void k_camera_sweep_factored(Problem 
     for (int k = 0; k < PER; k++) {
       const uint32_t n = slot[k];
       double fw[11];
-      load_factors<SHARED, RECOMPUTE>(q, q.Wc, n, RECOMPUTE ? camera_obs(q)[n] : n, c, pt[k], g,
-                                      fscale, fw);
+      load_factors<SHARED, RECOMPUTE>(q, RECOMPUTE ? nullptr : fptr<SHARED>(q.Wc, 0, n), TILE,
+                                      RECOMPUTE ? camera_obs(q)[n] : n, c, pt[k], g, fscale, fw);
       const bool dir = directional_point(g, pt[k]);
       const double b0 = R[0] * yv[k][0] + R[1] * yv[k][1] + R[2] * yv[k][2];
       const double b1 = R[3] * yv[k][0] + R[4] * yv[k][1] + R[5] * yv[k][2];

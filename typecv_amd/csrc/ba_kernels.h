@@ -9,8 +9,8 @@
 // Data layout in HBM (C cameras, P points, M observations in all, K = 18 or 21 cross words):
 //   scene, trial, index    the reference's packed arrays (include/typecv_ba.h) of the whole batch
 //   Wp                     cross blocks W_i = w Jc^T Jp (+ the focal row w Jf^T Jp), observations in
-//                          point-major (input) order, each tile's from a 64-observation block of
-//                          its own (tile_slot): the point sweep and the one-pass product stream it
+//                          point-major (input) order, each tile's as K rows of its observations'
+//                          words (tile_word): the point sweep and the one-pass product stream it
 //   Wc                     the same blocks in camera-major order (each camera's list): the camera
 //                          sweep streams it. Both blocked [slots/64][K][64] (see wptr)
 //   cam_point[M]           the point of each camera-major slot
@@ -124,8 +124,8 @@ struct Problem {
   const Bundle* bundles;
   const uint32_t* tiles;          // n_tiles + 1 first points
   const uint32_t* tile_bundle;
-  // n_tiles + 1: the first slot of each tile in the point-major stored words (Wp), a multiple of
-  // 64, so that a tile's words start a 64-observation block and no two tiles share a block
+  // n_tiles + 1: the first slot of each tile's code words (op_code, 64 slots per pass of the tile),
+  // a multiple of 64
   const uint32_t* tile_slot;
   const uint32_t* tasks;          // 4 per task: camera, first slot, end slot, bundle
   const uint32_t* task_start;     // C + 1

@@ -313,6 +313,17 @@ __device__ __forceinline__ WT* wptr(void* base, uint32_t k, uint32_t i, uint32_t
 // The mixed form stores each observation's cross block as FP32 Jacobian factors (ba_factored.hip):
 constexpr int kf_words(bool shared) { return shared ? 11 : 9; }
 
+// Word k of observation i in the point-major stored words (Wp, K words per observation: cross
+// blocks or factors), i in the tile whose observations are [s, s + n). Each tile's words are K rows
+// of n consecutive words, one row per word, tiles back to back in observation order (no padding):
+// a wave's load of one word reads one run of n words, and the tile's runs lie next to each other,
+// so its loads share their boundary cache lines.
+template <class T>
+__device__ __forceinline__ T* tile_word(void* base, uint32_t K, uint32_t s, uint32_t n, uint32_t k,
+                                        uint32_t i) {
+  return static_cast<T*>(base) + (size_t)K * s + (size_t)k * n + (i - s);
+}
+
 // The factor words of one observation: s du (3), s dv (3), q (3)[, s Jf (2)], zero where the point
 // does not project.
 __device__ __forceinline__ void cross_factors(const Lin& L, bool ok, bool shared, double* f) {
@@ -591,11 +602,10 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_pass(Problem q) {
         if constexpr (sizeof(WT) == 4) {
           double fw[11];
           cross_factors(L, ok, SHARED, fw);
-          const uint32_t slot = q.tile_slot[t] + (i - s);
-          for (int k = 0; k < kf_words(SHARED); k++) *fptr<SHARED>(q.Wp, k, slot) = (float)fw[k];
+          for (int k = 0; k < kf_words(SHARED); k++)
+            *tile_word<float>(q.Wp, kf_words(SHARED), s, e - s, k, i) = (float)fw[k];
         } else {
-          const uint32_t slot = q.tile_slot[t] + (i - s);
-          for (uint32_t k = 0; k < K; k++) *wptr<WT, SHARED>(q.Wp, k, slot, q.M) = (WT)W[k];
+          for (uint32_t k = 0; k < K; k++) *tile_word<WT>(q.Wp, K, s, e - s, k, i) = (WT)W[k];
         }
       }
       double* o = st[lane];
@@ -1117,10 +1127,10 @@ __device__ __forceinline__ void sweep_tile(const Problem& q, uint32_t t, uint32_
   const uint32_t mys = has ? ps[my] : 0, mye = has ? ps[my + 1] : 0;
   const double vf = SHARED ? v[6 * q.C + b] : 0.0;  // the bundle's focal row
   const double fscale = RECOMPUTE && SHARED ? focal_scale(sb[S_LOG_FOCAL]) : 1.0;
-  const uint32_t ts = q.tile_slot[t];
+  constexpr uint32_t KW = SHARED ? 21 : 18;
   double acc[3] = {0, 0, 0};
   for (uint32_t base = s; base < e; base += TILE) {
-    const uint32_t i = base + lane, slot = ts + (i - s);
+    const uint32_t i = base + lane;
     if (i < e) {
       const uint32_t c = obs_camera(q)[i];
       double vc[6];
@@ -1133,11 +1143,12 @@ __device__ __forceinline__ void sweep_tile(const Problem& q, uint32_t t, uint32_
         double a = 0;
 #pragma unroll
         for (int k = 0; k < 6; k++)
-          a += (RECOMPUTE ? wr[k * 3 + j] : wload(wptr<WT, SHARED>(q.Wp, k * 3 + j, slot, q.M))) *
+          a += (RECOMPUTE ? wr[k * 3 + j]
+                          : wload(tile_word<WT>(q.Wp, KW, s, e - s, k * 3 + j, i))) *
                vc[k];
         if (SHARED)
           a += (RECOMPUTE ? wr[(18 + j) % (SHARED ? 21 : 18)]
-                          : wload(wptr<WT, SHARED>(q.Wp, 18 + j, slot, q.M))) * vf;
+                          : wload(tile_word<WT>(q.Wp, KW, s, e - s, 18 + j, i))) * vf;
         tj[j] = a;
       }
       st[lane][0] = tj[0];
@@ -1194,7 +1205,7 @@ __device__ __forceinline__ void sweep_tile_fused(const Problem& q, uint32_t t, u
   const bool has = my < ep;
   const uint32_t mys = has ? ps[my] : 0, mye = has ? ps[my + 1] : 0;
   const double vf = SHARED ? v[6 * q.C + b] : 0.0;
-  const uint32_t ts = q.tile_slot[t];
+  const uint32_t n = e - s;
   double acc[3] = {0, 0, 0};
   if (e - s <= TILE) {
     const uint32_t i = s + lane;
@@ -1205,7 +1216,7 @@ __device__ __forceinline__ void sweep_tile_fused(const Problem& q, uint32_t t, u
       double vc[6];
       for (int k = 0; k < 6; k++) vc[k] = v[c * 6 + k];
 #pragma unroll
-      for (int k = 0; k < K; k++) w[k] = wload(wptr<WT, SHARED>(q.Wp, k, ts + lane, q.M));
+      for (int k = 0; k < K; k++) w[k] = wload(tile_word<WT>(q.Wp, K, s, n, k, i));
 #pragma unroll
       for (int j = 0; j < 3; j++) {
         double a = 0;
@@ -1247,8 +1258,8 @@ __device__ __forceinline__ void sweep_tile_fused(const Problem& q, uint32_t t, u
         double a = 0;
 #pragma unroll
         for (int k = 0; k < 6; k++)
-          a += wload(wptr<WT, SHARED>(q.Wp, k * 3 + j, ts + (i - s), q.M)) * vc[k];
-        if (SHARED) a += wload(wptr<WT, SHARED>(q.Wp, 18 + j, ts + (i - s), q.M)) * vf;
+          a += wload(tile_word<WT>(q.Wp, K, s, n, k * 3 + j, i)) * vc[k];
+        if (SHARED) a += wload(tile_word<WT>(q.Wp, K, s, n, 18 + j, i)) * vf;
         st[lane][j] = a;
       }
     }
@@ -1272,9 +1283,9 @@ __device__ __forceinline__ void sweep_tile_fused(const Problem& q, uint32_t t, u
     double* out = q.tprod + (size_t)i * 8;
 #pragma unroll
     for (int j = 0; j < NV; j++)
-      out[j] = wload(wptr<WT, SHARED>(q.Wp, j * 3, ts + (i - s), q.M)) * y0 +
-               wload(wptr<WT, SHARED>(q.Wp, j * 3 + 1, ts + (i - s), q.M)) * y1 +
-               wload(wptr<WT, SHARED>(q.Wp, j * 3 + 2, ts + (i - s), q.M)) * y2;
+      out[j] = wload(tile_word<WT>(q.Wp, K, s, n, j * 3, i)) * y0 +
+               wload(tile_word<WT>(q.Wp, K, s, n, j * 3 + 1, i)) * y1 +
+               wload(tile_word<WT>(q.Wp, K, s, n, j * 3 + 2, i)) * y2;
 #pragma unroll
     for (int j = NV; j < 8; j++) out[j] = 0.0;
   }

@@ -108,7 +108,7 @@ __device__ __forceinline__ Window window_of(const Problem& q, uint32_t w) {
 
 // The record words of a tile (op_trec, OP_RECORD words per tile).
 enum : uint32_t {
-  TR_SLOT = 0,    // first stored slot (a multiple of 64)
+  TR_SLOT = 0,    // first slot of its code words (a multiple of 64)
   TR_OBS = 1,     // observations
   TR_POINT = 2,   // first point
   TR_POINTS = 3,  // points
@@ -173,8 +173,9 @@ __device__ __forceinline__ void segmented_scan3(double x[3], uint32_t dist, uint
 }
 
 // The stored words of one observation that the two halves share. The mixed form keeps its FP32
-// factors; the FP64 form its cross block. `load_stored` reads lane `lane` of one 64-slot block
-// (`blk`: the block's first word), `load_recomputed` relinearises them from the scene.
+// factors; the FP64 form its cross block. `load_stored` reads lane `lane` of a tile's (or a pass
+// of a tile's) words (`w0`: their first word; word k at w0 + k * n, n the tile's observations),
+// `load_recomputed` relinearises them from the scene.
 template <class WT, bool SHARED>
 struct ObsWords;
 
@@ -182,9 +183,9 @@ template <bool SHARED>
 struct ObsWords<float, SHARED> {
   static constexpr int K = kf_words(SHARED);
   float f[11];
-  __device__ __forceinline__ void load_stored(const float* blk, uint32_t lane) {
+  __device__ __forceinline__ void load_stored(const float* w0, uint32_t n, uint32_t lane) {
 #pragma unroll
-    for (int k = 0; k < K; k++) f[k] = __builtin_nontemporal_load(blk + k * TILE + lane);
+    for (int k = 0; k < K; k++) f[k] = __builtin_nontemporal_load(w0 + k * n + lane);
 #pragma unroll
     for (int k = K; k < 11; k++) f[k] = 0;
   }
@@ -219,9 +220,9 @@ template <bool SHARED>
 struct ObsWords<double, SHARED> {
   static constexpr int K = SHARED ? 21 : 18;
   double w[K];
-  __device__ __forceinline__ void load_stored(const double* blk, uint32_t lane) {
+  __device__ __forceinline__ void load_stored(const double* w0, uint32_t n, uint32_t lane) {
 #pragma unroll
-    for (int k = 0; k < K; k++) w[k] = wload(blk + k * TILE + lane);
+    for (int k = 0; k < K; k++) w[k] = wload(w0 + k * n + lane);
   }
   __device__ __forceinline__ void load_recomputed(const Problem& q, uint32_t i, uint32_t c,
                                                   uint32_t p, const Bundle& g, double fscale) {
@@ -248,10 +249,10 @@ struct ObsWords<double, SHARED> {
   }
 };
 
-// The first word of 64-slot block `block` of the stored words.
+// The first stored word of the tile whose observations begin at `first` (tile_word's layout).
 template <class WT, bool SHARED>
-__device__ __forceinline__ const WT* stored_block(const Problem& q, uint32_t block) {
-  return static_cast<const WT*>(q.Wp) + (size_t)block * ObsWords<WT, SHARED>::K * TILE;
+__device__ __forceinline__ const WT* stored_tile(const Problem& q, uint32_t first) {
+  return static_cast<const WT*>(q.Wp) + (size_t)first * ObsWords<WT, SHARED>::K;
 }
 
 // Word k (a wave-uniform index) of the wave's tile records.
@@ -276,7 +277,9 @@ __device__ __forceinline__ void load_tile(const Problem& q, uint32_t rec, uint32
   const uint32_t fp = record_word(rec, OP_RECORD * j + TR_POINT);
   const uint32_t np = record_word(rec, OP_RECORD * j + TR_POINTS);
   in.code = n ? q.op_code[slot + lane] : 0u;
-  if (AHEAD && lane < n) in.ow.load_stored(stored_block<WT, SHARED>(q, slot >> 6), lane);
+  if (AHEAD && lane < n && n <= (uint32_t)TILE)
+    in.ow.load_stored(stored_tile<WT, SHARED>(q, record_word(rec, OP_RECORD * j + TR_FIRST)), n,
+                      lane);
   if (lane < np) {
     const double* pb = q.pblk + (size_t)(fp + lane) * 12;
 #pragma unroll
@@ -344,7 +347,6 @@ __device__ __forceinline__ void onepass_tile(const OnePassCtx<NR>& x, uint32_t r
                                              uint32_t lane, TileIn<WT, SHARED>& in,
                                              double (*st)[3], double (*part)[SHARED ? 7 : 6]) {
   constexpr int NV = SHARED ? 7 : 6;
-  const uint32_t slot = record_word(rec, OP_RECORD * j + TR_SLOT);
   const uint32_t n = record_word(rec, OP_RECORD * j + TR_OBS);
   const uint32_t fp = record_word(rec, OP_RECORD * j + TR_POINT);
   const uint32_t np = record_word(rec, OP_RECORD * j + TR_POINTS);
@@ -358,7 +360,7 @@ __device__ __forceinline__ void onepass_tile(const OnePassCtx<NR>& x, uint32_t r
     if (RECOMPUTE)
       in.ow.load_recomputed(x.q, first + lane, x.win.camera(k.row), pt, x.g, x.fscale);
     else
-      in.ow.load_stored(stored_block<WT, SHARED>(x.q, slot >> 6), lane);
+      in.ow.load_stored(stored_tile<WT, SHARED>(x.q, first), n, lane);
   }
   // the point half: each point's terms summed by the scan; its last lane hands the sum to the
   // point's lane (st[lp]), which forms y_p = V_p^-1 sum
@@ -402,7 +404,7 @@ __device__ __forceinline__ void onepass_long_tile(const OnePassCtx<NR>& x, uint3
     if (RECOMPUTE)
       ow.load_recomputed(x.q, first + base + lane, x.win.camera(k.row), fp + k.lp, x.g, x.fscale);
     else
-      ow.load_stored(stored_block<WT, SHARED>(x.q, (slot + base) >> 6), lane);
+      ow.load_stored(stored_tile<WT, SHARED>(x.q, first) + base, n, lane);
   };
   // the point's terms pass by pass: each pass's sum (the scan's last lane) added in pass order
   double acc[3] = {0, 0, 0};
