@@ -2151,7 +2151,7 @@ static int body_obs_evaluate(tcv_ba_context* ctx, const tcv_obs_control* c, cons
   if (o.tracks) hipLaunchKernelGGL(k_obs_depth, dim3(blocks_for(o.tracks, 256)), dim3(256), 0, st, o);
   if (M) hipLaunchKernelGGL(k_obs_partials, dim3(blocks_for(blocks, 256)), dim3(256), 0, st, o);
   else HIP_TRY(hipMemsetAsync(o.partials, 0, (size_t)blocks * 2 * 8, st));
-  hipLaunchKernelGGL(k_obs_summary, dim3(1), dim3(64), 0, st, o);
+  hipLaunchKernelGGL(k_obs_summary, dim3(1), dim3(SUMMARY_THREADS), 0, st, o);
   HIP_TRY(hipGetLastError());
   if (nflags)
     HIP_TRY(hipMemcpyAsync(flags, o.flags, nflags * 4, hipMemcpyDeviceToHost, st));
@@ -2255,7 +2255,7 @@ static int body_obs_evaluate_noalloc(const tcv_obs_control* c, const double* cam
   if (o.tracks) hipLaunchKernelGGL(k_obs_depth, dim3(blocks_for(o.tracks, 256)), dim3(256), 0, st, o);
   if (M) hipLaunchKernelGGL(k_obs_partials, dim3(blocks_for(blocks, 256)), dim3(256), 0, st, o);
   else hipLaunchKernelGGL(k_obs_zero_f64, dim3(1), dim3(64), 0, st, o.partials, 2u * blocks);
-  hipLaunchKernelGGL(k_obs_summary, dim3(1), dim3(64), 0, st, o);
+  hipLaunchKernelGGL(k_obs_summary, dim3(1), dim3(SUMMARY_THREADS), 0, st, o);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(st));
   return TCV_BA_OK;

@@ -11,7 +11,7 @@
 //                    per observation, the relative depth standard deviation along its optical axis
 //   k_obs_partials   one thread per 64 observations, errors and counts summed in lane order (the
 //                    CPU tier's indexedObservationPartialsOracle :128-137)
-//   k_obs_summary    one wave, the partials in block order (observationSummary,
+//   k_obs_summary    one workgroup, the error partials in block order (observationSummary,
 //                    triangulation.shade.ts:134-143)
 //
 // The arithmetic is the reference's expression by expression with no fused multiply-adds (the
@@ -203,34 +203,56 @@ __global__ __launch_bounds__(256) void k_obs_partials(ObsProblem o) {
   o.partials[2 * (size_t)b + 1] = n;
 }
 
-// One wave: each round loads 64 partials (one per lane, coalesced) and every lane adds them in
-// block order, reading lane k's values with v_readlane (a scalar broadcast, no LDS), so the error
-// sum is the sequential one. The counts are whole numbers (exact in any order) and are summed the
-// same way.
-__device__ __forceinline__ double obs_lane(double v, uint32_t k) {
-  const long long bits = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readlane((int)(bits & 0xffffffff), (int)k);
-  const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), (int)k);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
+// One workgroup. The error sum is the sequential one, block by block: wave 0 adds the blocks'
+// error partials in block order, one dependent addition after another, reading them from LDS 16 at
+// a time, while the workgroup stages the next chunk of SUMMARY_CHUNK partials into the other LDS
+// buffer. The counts are whole numbers, exact in any order: every thread adds those of the blocks
+// it stages, and the workgroup sums them at the end.
+constexpr uint32_t SUMMARY_THREADS = 1024;
+constexpr uint32_t SUMMARY_CHUNK = 2048;
 
-__global__ __launch_bounds__(64) void k_obs_summary(ObsProblem o) {
-  const uint32_t lane = threadIdx.x;
+__global__ __launch_bounds__(SUMMARY_THREADS) void k_obs_summary(ObsProblem o) {
+  __shared__ double buf[2][SUMMARY_CHUNK];
+  __shared__ double red[SUMMARY_THREADS / 64];
+  const uint32_t tid = threadIdx.x;
+  const double* p = o.partials;
   double e = 0, n = 0;
-  for (uint32_t b0 = 0; b0 < o.blocks; b0 += 64) {
-    const uint32_t b = b0 + lane;
-    const double pe = b < o.blocks ? o.partials[2 * (size_t)b] : 0.0;
-    const double pn = b < o.blocks ? o.partials[2 * (size_t)b + 1] : 0.0;
-    // lanes past the last block hold +0, which leaves the sums of non-negative terms unchanged
-#pragma unroll
-    for (uint32_t k = 0; k < 64; k++) {
-      e += obs_lane(pe, k);
-      n += obs_lane(pn, k);
+  auto stage = [&](uint32_t chunk) {
+    const uint32_t first = chunk * SUMMARY_CHUNK;
+    double* to = buf[chunk & 1];
+    for (uint32_t k = tid; k < SUMMARY_CHUNK && first + k < o.blocks; k += SUMMARY_THREADS) {
+      to[k] = p[2 * (size_t)(first + k)];
+      n += p[2 * (size_t)(first + k) + 1];
     }
+  };
+  const uint32_t chunks = (o.blocks + SUMMARY_CHUNK - 1) / SUMMARY_CHUNK;
+  if (chunks) stage(0);
+  __syncthreads();
+  for (uint32_t c = 0; c < chunks; c++) {
+    if (c + 1 < chunks) stage(c + 1);
+    if (tid < 64) {
+      const double* x = buf[c & 1];
+      const uint32_t len = min(SUMMARY_CHUNK, o.blocks - c * SUMMARY_CHUNK);
+      uint32_t k = 0;
+      for (; k + 16 <= len; k += 16) {
+        double v[16];
+#pragma unroll
+        for (uint32_t u = 0; u < 16; u++) v[u] = x[k + u];
+#pragma unroll
+        for (uint32_t u = 0; u < 16; u++) e += v[u];
+      }
+      for (; k < len; k++) e += x[k];
+    }
+    __syncthreads();
   }
-  if (lane == 0) {
+  for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off, 64);
+  if ((tid & 63) == 0) red[tid >> 6] = n;
+  __syncthreads();
+  if (tid == 0) {
+    double total = 0;
+    for (uint32_t w = 0; w < SUMMARY_THREADS / 64; w++) total += red[w];
     o.summary[0] = e;
-    o.summary[1] = n;
+    o.summary[1] = total;
   }
 }
 
