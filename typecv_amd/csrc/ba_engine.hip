@@ -64,7 +64,13 @@ struct tcv_ba_context {
       rhs, grad, camS, camG, pfc, dense, direct_bundles, direct_cams, covis, small_bundles,
       tprod,
       obs_cameras, obs_points, obs_pixels, obs_indices, obs_offsets, obs_flags, obs_errors,
-      obs_partials, obs_summary, op_tile, op_lo, op_off, op_cam, op_row, op_part, op_code, op_trec;
+      obs_partials, obs_summary, op_tile, op_lo, op_off, op_cam, op_row, op_part, op_code, op_trec,
+      pcg_map;
+  // per bundle (B + 1 prefix counts) its first one-pass workgroup, camera and PCG-step group, and
+  // the host side of pcg_map (run_pcg's compacted grids)
+  std::vector<uint32_t> bundle_ops, bundle_cams, bundle_groups, h_map;
+  // the bundles whose lists pcg_map holds (empty: none uploaded since the last set-up)
+  std::vector<char> map_on;
   // the direct bundles factored by a sequence of launches, and their dense array sizes
   std::vector<uint32_t> large_bundles, large_n;
   std::vector<uint64_t> large_off;
@@ -89,7 +95,7 @@ struct tcv_ba_context {
            &dense,      &direct_bundles, &direct_cams, &covis, &small_bundles, &tprod,
            &obs_cameras, &obs_points, &obs_pixels, &obs_indices, &obs_offsets, &obs_flags,
            &obs_errors, &obs_partials, &obs_summary, &tile_slot, &op_tile, &op_lo, &op_off, &op_cam, &op_row,
-           &op_part,    &op_code,      &op_trec};
+           &op_part,    &op_code,      &op_trec,   &pcg_map};
   }
 };
 
@@ -339,6 +345,11 @@ int build_plan(const std::vector<uint32_t>& ps, const std::vector<uint32_t>& cs,
 
 uint32_t blocks_for(size_t n, uint32_t threads) { return (uint32_t)((n + threads - 1) / threads); }
 uint32_t tile_blocks(const Problem& q) { return blocks_for(q.n_tiles, TILE_WAVES); }
+// The one-pass product's and its reduction's grids (the compacted ones when run_pcg set maps).
+uint32_t op_grid(const Problem& q) { return q.op_map ? q.n_op_map : q.n_op; }
+uint32_t reduce_grid(const Problem& q) {
+  return blocks_for((size_t)(q.cam_map ? q.n_cam_map : q.C) * 64, 256);
+}
 
 // The launches of one form (W storage type and route).
 template <class WT, bool SHARED>
@@ -375,9 +386,11 @@ struct Launch {
     hipLaunchKernelGGL((k_init_cg<SHARED>), dim3(q.B), dim3(CG_THREADS), 0, s, q);
   }
   static void point_sweep(const Problem& q, hipStream_t s) {
-    if (q.n_op)
-      hipLaunchKernelGGL((k_schur_onepass<WT, SHARED>), dim3(q.n_op), dim3(TILE * TILE_WAVES), 0,
-                         s, q);
+    if (q.n_op) {
+      if (op_grid(q))
+        hipLaunchKernelGGL((k_schur_onepass<WT, SHARED>), dim3(op_grid(q)),
+                           dim3(TILE * TILE_WAVES), 0, s, q);
+    }
     else if constexpr (sizeof(WT) == 4)
       hipLaunchKernelGGL((k_point_sweep_factored<SHARED, SWEEP_SCHUR>), dim3(tile_blocks(q)),
                          dim3(TILE * TILE_WAVES), 0, s, q);
@@ -389,9 +402,10 @@ struct Launch {
                          dim3(TILE * TILE_WAVES), 0, s, q);
   }
   static void camera_sweep(const Problem& q, hipStream_t s) {
-    if (q.n_op)
-      hipLaunchKernelGGL((k_onepass_reduce<SHARED>), dim3(blocks_for((size_t)q.C * 64, 256)),
-                         dim3(256), 0, s, q);
+    if (q.n_op) {
+      if (reduce_grid(q))
+        hipLaunchKernelGGL((k_onepass_reduce<SHARED>), dim3(reduce_grid(q)), dim3(256), 0, s, q);
+    }
     else if constexpr (sizeof(WT) == 4)
       hipLaunchKernelGGL((k_camera_sweep_factored<SHARED>), dim3(q.n_tasks), dim3(TASK_THREADS),
                          q.camera_lds, s, q);
@@ -403,9 +417,13 @@ struct Launch {
                          q.camera_lds, s, q);
   }
   static void advance(const Problem& q, hipStream_t s) {
-    hipLaunchKernelGGL((k_cg_product<SHARED>), dim3(q.n_groups), dim3(ADV_THREADS), 0, s, q);
-    hipLaunchKernelGGL((k_cg_step<SHARED>), dim3(q.n_groups), dim3(ADV_THREADS), 0, s, q);
-    hipLaunchKernelGGL((k_cg_direction<SHARED>), dim3(q.n_groups), dim3(ADV_THREADS), 0, s, q);
+    if (q.n_single_map)
+      hipLaunchKernelGGL((k_cg_single<SHARED>), dim3(q.n_single_map), dim3(ADV_THREADS), 0, s, q);
+    const uint32_t groups = q.group_map ? q.n_group_map : q.n_groups;
+    if (!groups) return;
+    hipLaunchKernelGGL((k_cg_product<SHARED>), dim3(groups), dim3(ADV_THREADS), 0, s, q);
+    hipLaunchKernelGGL((k_cg_step<SHARED>), dim3(groups), dim3(ADV_THREADS), 0, s, q);
+    hipLaunchKernelGGL((k_cg_direction<SHARED>), dim3(groups), dim3(ADV_THREADS), 0, s, q);
   }
   static void update_points(const Problem& q, hipStream_t s) {
     if constexpr (sizeof(WT) == 4)
@@ -417,9 +435,11 @@ struct Launch {
   }
   // The streamed route: the same sweeps with every cross block relinearised in registers.
   static void streamed_point_sweep(const Problem& q, hipStream_t s) {
-    if (q.n_op)
-      hipLaunchKernelGGL((k_schur_onepass<WT, SHARED, true>), dim3(q.n_op),
-                         dim3(TILE * TILE_WAVES), 0, s, q);
+    if (q.n_op) {
+      if (op_grid(q))
+        hipLaunchKernelGGL((k_schur_onepass<WT, SHARED, true>), dim3(op_grid(q)),
+                           dim3(TILE * TILE_WAVES), 0, s, q);
+    }
     else if constexpr (sizeof(WT) == 4)
       hipLaunchKernelGGL((k_point_sweep_factored<SHARED, SWEEP_SCHUR, true>), dim3(tile_blocks(q)),
                          dim3(TILE * TILE_WAVES), 0, s, q);
@@ -428,9 +448,10 @@ struct Launch {
                          dim3(TILE * TILE_WAVES), 0, s, q);
   }
   static void streamed_camera_sweep(const Problem& q, hipStream_t s) {
-    if (q.n_op)
-      hipLaunchKernelGGL((k_onepass_reduce<SHARED>), dim3(blocks_for((size_t)q.C * 64, 256)),
-                         dim3(256), 0, s, q);
+    if (q.n_op) {
+      if (reduce_grid(q))
+        hipLaunchKernelGGL((k_onepass_reduce<SHARED>), dim3(reduce_grid(q)), dim3(256), 0, s, q);
+    }
     else if constexpr (sizeof(WT) == 4)
       hipLaunchKernelGGL((k_camera_sweep_factored<SHARED, true>), dim3(q.n_tasks),
                          dim3(TASK_THREADS), 0, s, q);
@@ -743,6 +764,8 @@ std::vector<Need> solve_needs(tcv_ba_context* ctx, const tcv_ba_control* c, uint
     // each stored slot's code word and each tile's record
     n.push_back({&ctx->op_code, (size_t)plan.tile_slot.back() * 4});
     n.push_back({&ctx->op_trec, n_tiles * OP_RECORD * 4});
+    // the PCG loop's grids: one-pass workgroups, cameras and PCG-step groups
+    n.push_back({&ctx->pcg_map, (n_op + C + n_groups) * 4});
   }
   if (!plan.direct_bundles.empty()) {
     n.push_back({&ctx->camS, (size_t)C * 36 * 8});
@@ -996,6 +1019,14 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
       q.Wc = ctx->Wc.ptr;
     }
   }
+  ctx->bundle_cams.assign(B + 1, 0);
+  ctx->bundle_groups.assign(B + 1, 0);
+  for (uint32_t b = 0; b < B; b++) {
+    ctx->bundle_cams[b + 1] = plan.bundles[b].cam1;
+    ctx->bundle_groups[b + 1] = plan.bundles[b].group1;
+  }
+  ctx->bundle_ops.assign(B + 1, 0);
+  ctx->map_on.clear();
   if (!exact) TRY(plan_onepass(ctx, q, plan, st, rep));
   ctx->large_bundles = plan.large_bundles;
   ctx->large_n.clear();
@@ -1037,6 +1068,9 @@ int plan_onepass(tcv_ba_context* ctx, Problem& q, const Plan& plan, hipStream_t 
     }
   }
   const uint32_t n_op = (uint32_t)h_tile.size();
+  ctx->bundle_ops.assign(plan.bundles.size() + 1, 0);
+  for (uint32_t w = 0; w < n_op; w++) ctx->bundle_ops[h_bundle[w] + 1]++;
+  for (size_t b = 0; b < plan.bundles.size(); b++) ctx->bundle_ops[b + 1] += ctx->bundle_ops[b];
   if (n_op == 0) return TCV_BA_OK;
   h_tile.push_back(q.n_tiles);
   TRY(upload(ctx->op_tile, h_tile, st));
@@ -1154,6 +1188,56 @@ int tune_pick(tcv_ba_context* ctx, bool fused_available) {
   return TCV_BA_OK;
 }
 
+// The PCG loop's grids restricted to the bundles in `on`: their one-pass workgroups, the cameras
+// of the one-pass reduction and their PCG-step groups, in order, the groups of bundles with one
+// group apart (k_cg_single runs their three steps in one launch). The other bundles' workgroups
+// would return at once, yet each costs its dispatch: a product launch of the 256-bundle batch
+// while few of its bundles were still in PCG took 32 us at the median, 17.8 us at the 5th
+// percentile. out = q with the maps set (without maps, when pcg_map is not held: every bundle's
+// grid, three launches per step). Called only after a read of the state words (a stream
+// synchronisation), so the previous upload from h_map has completed.
+int compact_pcg(tcv_ba_context* ctx, const Problem& q, const std::vector<char>& on,
+                hipStream_t st, Problem& out) {
+  out = q;
+  out.op_map = out.cam_map = out.group_map = out.single_map = nullptr;
+  out.n_op_map = out.n_cam_map = out.n_group_map = out.n_single_map = 0;
+  const size_t words = (size_t)ctx->bundle_ops.back() + q.C + q.n_groups;
+  if (!ctx->pcg_map.ptr || ctx->pcg_map.bytes < words * 4 || ctx->bundle_ops.size() != q.B + 1 ||
+      ctx->bundle_cams.size() != q.B + 1 || ctx->bundle_groups.size() != q.B + 1)
+    return TCV_BA_OK;
+  std::vector<uint32_t>& h = ctx->h_map;
+  h.clear();
+  const std::vector<uint32_t>& groups = ctx->bundle_groups;
+  auto single = [&](uint32_t b) { return groups[b + 1] - groups[b] == 1; };
+  // the lists: one-pass workgroups, cameras, groups of bundles with several, groups of those with one
+  const std::vector<uint32_t>* ranges[4] = {&ctx->bundle_ops, &ctx->bundle_cams, &groups, &groups};
+  uint32_t n[4];
+  for (int k = 0; k < 4; k++) {
+    const size_t first = h.size();
+    for (uint32_t b = 0; b < q.B; b++)
+      if (on[b] && (k < 2 || single(b) == (k == 3)))
+        for (uint32_t i = (*ranges[k])[b]; i < (*ranges[k])[b + 1]; i++) h.push_back(i);
+    n[k] = (uint32_t)(h.size() - first);
+  }
+  // a list that covers the whole grid in order stays a null map: no lookup in those launches; the
+  // lists are uploaded when some map is not null and they differ from those pcg_map holds
+  const bool identity = n[0] == ctx->bundle_ops.back() && n[1] == q.C && n[2] == q.n_groups;
+  if (!identity && on != ctx->map_on) {
+    HIP_TRY(hipMemcpyAsync(ctx->pcg_map.ptr, h.data(), h.size() * 4, hipMemcpyHostToDevice, st));
+    ctx->map_on = on;
+  }
+  const uint32_t* base = (const uint32_t*)ctx->pcg_map.ptr;
+  if (n[0] != ctx->bundle_ops.back()) out.op_map = base;
+  if (n[1] != q.C) out.cam_map = base + n[0];
+  if (n[2] != q.n_groups) out.group_map = base + n[0] + n[1];
+  out.single_map = base + n[0] + n[1] + n[2];
+  out.n_op_map = n[0];
+  out.n_cam_map = n[1];
+  out.n_group_map = n[2];
+  out.n_single_map = n[3];
+  return TCV_BA_OK;
+}
+
 // PCG in chunks with one read of the state words after each: the first chunk `first` iterations,
 // later ones kChunk, at most `budget` in all. A chunk that outlives convergence costs only guarded
 // no-op launches (state word 19), as the reference's chunks do. With TCV_BA_FLAG_TIMING the two
@@ -1161,7 +1245,8 @@ int tune_pick(tcv_ba_context* ctx, bool fused_available) {
 // events (each event is a queue packet between kernels: timing every launch slows the solve by 7 %
 // at 500 cameras).
 int run_pcg(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, const Ops& ops,
-            hipStream_t st, tcv_ba_report* rep, uint32_t first, uint32_t budget) {
+            hipStream_t st, tcv_ba_report* rep, uint32_t first, uint32_t budget,
+            bool known = false) {
   const bool timing = (c->flags & TCV_BA_FLAG_TIMING) != 0 && rep;
   uint32_t done = 0, chunk = first;
   double matvecs_before = max_word(ctx, q, S_MATVECS);
@@ -1174,6 +1259,15 @@ int run_pcg(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, cons
   static_assert(kTuneFirst > 0 && kTuneFirst + kTuneRuns <= kTimingStride, "tuning iterations");
   if (tune) TRY(ensure_tune_events(ctx));
   Problem qt = scheduled(q, tuned ? TuneCandidate{ctx->tuned_fused, ctx->camera_cap} : kTune[0]);
+  // the launches cover the bundles whose PCG may run: those whose PCG the last read of the state
+  // words showed active (`known`: it followed this pass's set-up), else (a batch) those whose LM
+  // loop it showed active, or the one bundle; after each chunk those still in PCG
+  std::vector<char> on(q.B);
+  for (uint32_t b = 0; b < q.B; b++)
+    on[b] = known ? word(ctx, b, S_PCG_ACTIVE) != 0 : q.B == 1 || word(ctx, b, S_ACTIVE) != 0;
+  Problem qa;
+  TRY(compact_pcg(ctx, qt, on, st, qa));
+  bool tune_done = false;  // qt changed: qa follows it
   while (done < budget) {
     const uint32_t n = chunk < budget - done ? chunk : budget - done;
     for (uint32_t k = 0; k < n; k++) {
@@ -1188,12 +1282,12 @@ int run_pcg(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, cons
         HIP_TRY(hipEventRecord(ctx->tune_events[2 * r + 1], st));
       } else {
         if (timed) HIP_TRY(hipEventRecord(ctx->events[3 * k], st));
-        point_sweeps(qt, ops, st);
+        point_sweeps(qa, ops, st);
         if (timed) HIP_TRY(hipEventRecord(ctx->events[3 * k + 1], st));
-        camera_sweeps(qt, ops, st);
+        camera_sweeps(qa, ops, st);
         if (timed) HIP_TRY(hipEventRecord(ctx->events[3 * k + 2], st));
       }
-      ops.advance(qt, st);
+      ops.advance(qa, st);
     }
     HIP_TRY(hipGetLastError());
     if (rep) rep->matvec_launches += n;
@@ -1218,8 +1312,18 @@ int run_pcg(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, cons
       TRY(tune_pick(ctx, q.tprod != nullptr));
       qt = scheduled(q, TuneCandidate{ctx->tuned_fused, ctx->camera_cap});
       tune = false;
+      tune_done = true;
     }
     if (!any_word(ctx, q, S_PCG_ACTIVE)) break;
+    // (the lists are rebuilt, between the chunks on the host, only when they change)
+    bool changed = tune_done;
+    for (uint32_t b = 0; b < q.B; b++) {
+      const char v = word(ctx, b, S_PCG_ACTIVE) != 0;
+      changed |= v != on[b];
+      on[b] = v;
+    }
+    if (changed) TRY(compact_pcg(ctx, qt, on, st, qa));
+    tune_done = false;
     chunk = kChunk;
   }
   if (rep) {
@@ -1255,7 +1359,7 @@ int solve_linear(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q,
         ops.large_solve(q, ctx->large_bundles[j], ctx->large_n[j], ctx->large_off[j], 1, st);
   }
   if (!any_word(ctx, q, S_PCG_ACTIVE)) return TCV_BA_OK;
-  return run_pcg(ctx, c, q, ops, st, rep, first, budget);
+  return run_pcg(ctx, c, q, ops, st, rep, first, budget, true);
 }
 
 // Ends the PCG of every bundle whose budget ran out (the reference stops at maxCgIterations);
@@ -1325,7 +1429,7 @@ int run_lm(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, hipSt
         HIP_TRY(hipGetLastError());
         TRY(read_state(ctx, q, st, rep));
         if (!any_word(ctx, q, S_PCG_ACTIVE)) break;
-        TRY(run_pcg(ctx, c, q, ops, st, rep, kChunk, q.cg_max));
+        TRY(run_pcg(ctx, c, q, ops, st, rep, kChunk, q.cg_max, true));
         if (any_word(ctx, q, S_PCG_ACTIVE))
           hipLaunchKernelGGL(k_stop_pcg, dim3(blocks_for(q.B, 64)), dim3(64), 0, st, q);
       }

@@ -201,6 +201,16 @@ struct Problem {
   // at set-up
   const uint32_t* op_code;
   const uint32_t* op_trec;
+  // the PCG loop's grids restricted to the bundles whose PCG may be active (null: every bundle's):
+  // n_op_map one-pass workgroups, n_cam_map cameras of the one-pass reduction, n_group_map
+  // PCG-step groups of bundles with several groups and n_single_map groups of bundles with one
+  // (whose three PCG steps run as one launch, k_cg_single), each list in ascending order; the
+  // host builds them between PCG chunks from the state words it reads
+  const uint32_t* op_map;
+  const uint32_t* cam_map;
+  const uint32_t* group_map;
+  const uint32_t* single_map;
+  uint32_t n_op_map, n_cam_map, n_group_map, n_single_map;
 };
 
 // One-pass product: tiles per workgroup (OP_WAVE_TILES per wave) and the widest camera window.

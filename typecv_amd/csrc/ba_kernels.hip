@@ -1511,6 +1511,11 @@ struct Group {
   bool has, first;
 };
 
+// The PCG-step group of this workgroup: its index in the grid, or in the compacted grid's map.
+__device__ __forceinline__ uint32_t pcg_group(const Problem& q) {
+  return q.group_map ? q.group_map[blockIdx.x] : blockIdx.x;
+}
+
 // Group gi: one record of GROUP_WORDS words (bundle, first and end camera, the bundle's first and
 // end group), loaded as one scalar block.
 __device__ __forceinline__ Group group_at(const Problem& q, uint32_t gi) {
@@ -1603,7 +1608,7 @@ __device__ __forceinline__ void cg_product_group(const Problem& q, uint32_t gi, 
 template <bool SHARED>
 __global__ __launch_bounds__(ADV_THREADS) void k_cg_product(Problem q) {
   __shared__ CgLds L;
-  cg_product_group<SHARED>(q, blockIdx.x, L);
+  cg_product_group<SHARED>(q, pcg_group(q), L);
 }
 
 template <bool SHARED>
@@ -1697,7 +1702,7 @@ __device__ __forceinline__ void cg_step_group(const Problem& q, uint32_t gi, CgL
 template <bool SHARED>
 __global__ __launch_bounds__(ADV_THREADS) void k_cg_step(Problem q) {
   __shared__ CgLds L;
-  cg_step_group<SHARED>(q, blockIdx.x, L);
+  cg_step_group<SHARED>(q, pcg_group(q), L);
 }
 
 template <bool SHARED>
@@ -1756,7 +1761,23 @@ __device__ __forceinline__ void cg_direction_group(const Problem& q, uint32_t gi
 template <bool SHARED>
 __global__ __launch_bounds__(ADV_THREADS) void k_cg_direction(Problem q) {
   __shared__ CgLds L;
-  cg_direction_group<SHARED>(q, blockIdx.x, L);
+  cg_direction_group<SHARED>(q, pcg_group(q), L);
+}
+
+// The three steps for a bundle of one PCG-step group (every bundle of at most GROUP_CAMS cameras),
+// one workgroup per bundle: no other workgroup contributes to its sums, so the launches' boundaries
+// are the workgroup's barriers. The same bodies in the same order as the three launches, so the
+// same bits; the barriers also order each step's global writes (the state words, group_part, S p,
+// r, z) before the next step's reads, as a launch boundary does.
+template <bool SHARED>
+__global__ __launch_bounds__(ADV_THREADS) void k_cg_single(Problem q) {
+  __shared__ CgLds L;
+  const uint32_t gi = q.single_map[blockIdx.x];
+  cg_product_group<SHARED>(q, gi, L);
+  __syncthreads();
+  cg_step_group<SHARED>(q, gi, L);
+  __syncthreads();
+  cg_direction_group<SHARED>(q, gi, L);
 }
 
 // The linear-system probe (bundle 0): S v, M^-1 v and the RHS of the assembled system.

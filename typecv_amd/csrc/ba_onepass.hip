@@ -537,11 +537,12 @@ __device__ __forceinline__ void onepass_workgroup(const Problem& q, uint32_t w,
   __syncthreads();
 }
 
-// One workgroup per op_tile range, XCD-contiguous order.
+// One workgroup per op_tile range (or per entry of op_map), XCD-contiguous order.
 template <class WT, bool SHARED, bool RECOMPUTE = false>
 __global__ __launch_bounds__(TILE* TILE_WAVES) void k_schur_onepass(Problem q) {
   __shared__ OnePassLds<SHARED ? 7 : 6, sizeof(WT) == 4 ? 9 : 1> L;
-  onepass_workgroup<WT, SHARED, RECOMPUTE>(q, xcd_contiguous(blockIdx.x, gridDim.x), L);
+  const uint32_t i = xcd_contiguous(blockIdx.x, gridDim.x);
+  onepass_workgroup<WT, SHARED, RECOMPUTE>(q, q.op_map ? q.op_map[i] : i, L);
 }
 
 // Each camera's rows of partials summed in workgroup order (a wave per camera: lanes take the rows
@@ -571,8 +572,9 @@ __device__ __forceinline__ void onepass_reduce_camera(const Problem& q, uint32_t
 
 template <bool SHARED>
 __global__ __launch_bounds__(256) void k_onepass_reduce(Problem q) {
-  const uint32_t c = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / 64);
-  if (c < q.C) onepass_reduce_camera<SHARED>(q, c, threadIdx.x & 63);
+  const uint32_t k = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / 64);
+  if (k < (q.cam_map ? q.n_cam_map : q.C))
+    onepass_reduce_camera<SHARED>(q, q.cam_map ? q.cam_map[k] : k, threadIdx.x & 63);
 }
 
 // Set-up: the camera window of each one-pass workgroup (Window): the narrower of the plain and the
