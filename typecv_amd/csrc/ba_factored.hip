@@ -1,5 +1,6 @@
-// ba_factored.hip — the mixed form's Schur sweeps over FP32 Jacobian factors (included by
-// ba_engine.hip after ba_kernels.hip).
+// ba_factored.hip — the Schur sweeps over Jacobian factors: FP32 (the mixed form) or FP64 (the
+// FP64 form's point update when the one-pass product runs); included by ba_engine.hip after
+// ba_kernels.hip.
 //
 // An observation's cross block W = w Jc^T Jp (and the focal row w Jf^T Jp) is a product of its
 // Jacobian rows, and those are fixed by three small pieces: the projection derivative
@@ -14,16 +15,18 @@
 //   camera side W y            : h = D' (R y);  omega: q x (D'^T h),  v: D'^T h,  focal: Jf' . h
 // so a Schur product streams 2 (KF + 1) words per observation instead of 2 (K + 1). Sums keep the
 // stored sweeps' orders (segment sums per point, slot order and wave butterflies per camera task).
+// The FP64 form stores the same words in FP64 when the one-pass product runs (ba_onepass.hip): 88
+// (72) bytes per observation instead of the 168 (144) of its cross block.
 #pragma once
 
 namespace tcvba {
 
-// The stored factors of observation i (camera c, point p), widened: word k at w0 + k * stride
-// (the point-major words: a tile's row length; the camera-major ones: 64); RECOMPUTE:
-// relinearised from the scene and rounded as the passes store them (the streamed route, the same
-// bits; w0 unused).
-template <bool SHARED, bool RECOMPUTE>
-__device__ __forceinline__ void load_factors(const Problem& q, const float* w0, uint32_t stride,
+// The stored factors (FT: float or double) of observation i (camera c, point p), widened: word k
+// at w0 + k * stride (the point-major words: a tile's row length; the camera-major ones: 64);
+// RECOMPUTE: relinearised from the scene and rounded as the passes store them (the streamed route,
+// the same bits; w0 unused).
+template <class FT, bool SHARED, bool RECOMPUTE>
+__device__ __forceinline__ void load_factors(const Problem& q, const FT* w0, uint32_t stride,
                                              uint32_t i, uint32_t c, uint32_t p, const Bundle& g,
                                              double fscale, double* f) {
   constexpr int KF = kf_words(SHARED);
@@ -34,7 +37,7 @@ __device__ __forceinline__ void load_factors(const Problem& q, const float* w0, 
     double raw[11];
     cross_factors(L, ok, SHARED, raw);
 #pragma unroll
-    for (int k = 0; k < KF; k++) f[k] = (double)(float)raw[k];
+    for (int k = 0; k < KF; k++) f[k] = (double)(FT)raw[k];
   } else {
 #pragma unroll
     for (int k = 0; k < KF; k++) f[k] = (double)__builtin_nontemporal_load(w0 + k * stride);
@@ -59,8 +62,8 @@ __device__ __forceinline__ void factored_point_term(const double* f, const doubl
   for (int j = 0; j < 3; j++) t[j] = R[j] * a0 + R[3 + j] * a1 + R[6 + j] * a2;
 }
 
-// The point sweep over FP32 factors (the modes of sweep_tile: SCHUR on p, UPDATE on x).
-template <bool SHARED, int MODE, bool RECOMPUTE>
+// The point sweep over the factors (the modes of sweep_tile: SCHUR on p, UPDATE on x).
+template <class FT, bool SHARED, int MODE, bool RECOMPUTE>
 __device__ __forceinline__ void sweep_tile_factored(const Problem& q, uint32_t t, uint32_t lane,
                                                     double (*st)[3]) {
   const uint32_t fp = q.tiles[t], ep = q.tiles[t + 1], b = q.tile_bundle[t];
@@ -84,9 +87,8 @@ __device__ __forceinline__ void sweep_tile_factored(const Problem& q, uint32_t t
       const uint32_t c = obs_camera(q)[i], pt = obs_point(q)[i];
       const bool var = variable_camera(q, c), dir = directional_point(g, pt);
       double f[11], R[9], vc[6];
-      const float* w0 =
-          RECOMPUTE ? nullptr : tile_word<float>(q.Wp, kf_words(SHARED), s, e - s, 0, i);
-      load_factors<SHARED, RECOMPUTE>(q, w0, e - s, i, c, pt, g, fscale, f);
+      const FT* w0 = RECOMPUTE ? nullptr : tile_word<FT>(q.Wp, kf_words(SHARED), s, e - s, 0, i);
+      load_factors<FT, SHARED, RECOMPUTE>(q, w0, e - s, i, c, pt, g, fscale, f);
       const double* pose = q.scene + (size_t)c * 12;
 #pragma unroll
       for (int k = 0; k < 9; k++) R[k] = pose[k];
@@ -118,13 +120,13 @@ __device__ __forceinline__ void sweep_tile_factored(const Problem& q, uint32_t t
   }
 }
 
-template <bool SHARED, int MODE, bool RECOMPUTE = false>
+template <bool SHARED, int MODE, bool RECOMPUTE = false, class FT = float>
 __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_sweep_factored(Problem q) {
   __shared__ double st_all[TILE_WAVES][TILE][3];
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / TILE), lane = threadIdx.x % TILE;
   const uint32_t t = point_tile(wave);
   if (t >= q.n_tiles) return;
-  sweep_tile_factored<SHARED, MODE, RECOMPUTE>(q, t, lane, st_all[wave]);
+  sweep_tile_factored<FT, SHARED, MODE, RECOMPUTE>(q, t, lane, st_all[wave]);
 }
 
 // The camera sweep over FP32 factors: one workgroup per camera-list task, the camera's rotation
@@ -173,7 +175,7 @@ __global__ __launch_bounds__(TASK_THREADS) void k_camera_sweep_factored(Problem 
     for (int k = 0; k < PER; k++) {
       const uint32_t n = slot[k];
       double fw[11];
-      load_factors<SHARED, RECOMPUTE>(q, RECOMPUTE ? nullptr : fptr<SHARED>(q.Wc, 0, n), TILE,
+      load_factors<float, SHARED, RECOMPUTE>(q, RECOMPUTE ? nullptr : fptr<SHARED>(q.Wc, 0, n), TILE,
                                       RECOMPUTE ? camera_obs(q)[n] : n, c, pt[k], g, fscale, fw);
       const bool dir = directional_point(g, pt[k]);
       const double b0 = R[0] * yv[k][0] + R[1] * yv[k][1] + R[2] * yv[k][2];

@@ -599,11 +599,11 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_pass(Problem q) {
         for (int k = 0; k < 21; k++) W[k] = 0;
       }
       if (stores_w(g) && q.Wp) {
-        if constexpr (sizeof(WT) == 4) {
+        if (sizeof(WT) == 4 || (!EXACT && q.factored64)) {
           double fw[11];
           cross_factors(L, ok, SHARED, fw);
           for (int k = 0; k < kf_words(SHARED); k++)
-            *tile_word<float>(q.Wp, kf_words(SHARED), s, e - s, k, i) = (float)fw[k];
+            *tile_word<WT>(q.Wp, kf_words(SHARED), s, e - s, k, i) = (WT)fw[k];
         } else {
           for (uint32_t k = 0; k < K; k++) *tile_word<WT>(q.Wp, K, s, e - s, k, i) = (WT)W[k];
         }

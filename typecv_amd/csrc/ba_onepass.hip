@@ -1,5 +1,6 @@
-// ba_onepass.hip — the Schur product in one pass over the stored cross blocks (FP64) or FP32
-// Jacobian factors (mixed), included by ba_engine.hip after ba_factored.hip.
+// ba_onepass.hip — the Schur product in one pass over each observation's stored Jacobian factors
+// (ba_factored.hip; FP64 in the FP64 form, FP32 in the mixed), included by ba_engine.hip after
+// ba_factored.hip.
 //
 // The two sweeps read every observation's stored words twice: the point sweep for
 // y_p = V_p^-1 sum_i W_i^T p_c(i), the camera sweep for sum_i W_i y_p(i) in camera order. Here one
@@ -85,8 +86,8 @@ struct Window {
 };
 
 // The cameras of a workgroup's window, staged in LDS once per product: each camera's PCG direction
-// p_c, whether it is variable and, for the factors (NR = 9), its rotation. The tiles' gathers of
-// them become LDS reads.
+// p_c, whether it is variable and (NR = 9) its rotation, which the factors' products need. The
+// tiles' gathers of them become LDS reads.
 template <int NR>
 struct WindowCams {
   double p[OP_SPAN][6];
@@ -216,36 +217,35 @@ struct ObsWords<float, SHARED> {
   }
 };
 
+// The FP64 form's words in this product: the same factors in FP64 (the passes store them so when
+// the one-pass product runs), every product formed as for the FP32 ones.
 template <bool SHARED>
 struct ObsWords<double, SHARED> {
-  static constexpr int K = SHARED ? 21 : 18;
-  double w[K];
+  static constexpr int K = kf_words(SHARED);
+  double f[11];
   __device__ __forceinline__ void load_stored(const double* w0, uint32_t n, uint32_t lane) {
 #pragma unroll
-    for (int k = 0; k < K; k++) w[k] = wload(w0 + k * n + lane);
+    for (int k = 0; k < K; k++) f[k] = __builtin_nontemporal_load(w0 + k * n + lane);
+#pragma unroll
+    for (int k = K; k < 11; k++) f[k] = 0;
   }
+  // relinearised as the passes store them
   __device__ __forceinline__ void load_recomputed(const Problem& q, uint32_t i, uint32_t c,
                                                   uint32_t p, const Bundle& g, double fscale) {
-    relinearised_w<double, SHARED>(q, g, c, p, i, fscale, w);
-  }
-  // k_point_sweep's expression: t_j = sum_k W[k][j] v_c[k] (+ W_f[j] v_f)
-  __device__ __forceinline__ void point_term(const double*, const double* vc, double vf, bool,
-                                             bool, double t[3]) const {
+    Lin L;
+    const bool ok =
+        linearize(q, c, p, i, fscale, variable_camera(q, c), directional_point(g, p), L);
+    cross_factors(L, ok, SHARED, f);
 #pragma unroll
-    for (int j = 0; j < 3; j++) {
-      double a = 0;
-#pragma unroll
-      for (int k = 0; k < 6; k++) a += w[k * 3 + j] * vc[k];
-      if (SHARED) a += w[18 + j] * vf;
-      t[j] = a;
-    }
+    for (int k = K; k < 11; k++) f[k] = 0;
   }
-  // W y: the camera rows and, on the shared route, the focal row
-  __device__ __forceinline__ void camera_term(const double*, const double* y, bool, bool,
+  __device__ __forceinline__ void point_term(const double* R, const double* vc, double vf,
+                                             bool var, bool dir, double t[3]) const {
+    factored_point_term(f, R, vc, vf, var, dir, t);
+  }
+  __device__ __forceinline__ void camera_term(const double* R, const double* y, bool var, bool dir,
                                               double* hv) const {
-#pragma unroll
-    for (int k = 0; k < 6; k++) hv[k] = w[k * 3] * y[0] + w[k * 3 + 1] * y[1] + w[k * 3 + 2] * y[2];
-    if (SHARED) hv[6] = w[18] * y[0] + w[19] * y[1] + w[20] * y[2];
+    factored_camera_term<SHARED>(f, R, y, var, dir, hv);
   }
 };
 
@@ -465,9 +465,9 @@ struct OnePassLds {
 template <class WT, bool SHARED, bool RECOMPUTE>
 __device__ __forceinline__ void onepass_workgroup(const Problem& q, uint32_t w,
                                                   OnePassLds<SHARED ? 7 : 6,
-                                                             sizeof(WT) == 4 ? 9 : 1>& L) {
+                                                             9>& L) {
   constexpr int NV = SHARED ? 7 : 6;
-  constexpr int NR = sizeof(WT) == 4 ? 9 : 1;
+  constexpr int NR = 9;
   // words loaded one tile ahead: the FP32 factors (not the FP64 blocks, whose registers would cut
   // the waves a SIMD holds, nor relinearised words, which need the tile's codes first)
   constexpr bool AHEAD = sizeof(WT) == 4 && !RECOMPUTE;
@@ -540,7 +540,7 @@ __device__ __forceinline__ void onepass_workgroup(const Problem& q, uint32_t w,
 // One workgroup per op_tile range (or per entry of op_map), XCD-contiguous order.
 template <class WT, bool SHARED, bool RECOMPUTE = false>
 __global__ __launch_bounds__(TILE* TILE_WAVES) void k_schur_onepass(Problem q) {
-  __shared__ OnePassLds<SHARED ? 7 : 6, sizeof(WT) == 4 ? 9 : 1> L;
+  __shared__ OnePassLds<SHARED ? 7 : 6, 9> L;
   const uint32_t i = xcd_contiguous(blockIdx.x, gridDim.x);
   onepass_workgroup<WT, SHARED, RECOMPUTE>(q, q.op_map ? q.op_map[i] : i, L);
 }
