@@ -35,11 +35,11 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE x 1.00, the factors that make a 1 GiB device copy read and write 1 GiB. Keyed by
 # (cameras, points, seed, precision, operator); other workloads report null.
 MEASURED_TRAFFIC = {
-    # the one-pass product (k_schur_onepass + k_onepass_reduce)
+    # the one-pass product (k_schur_onepass + k_onepass_reduce) over FP64 or FP32 factors
     (2000, 1_000_000, 2, "fp64", "onepass"):
-        2 * (893176.4 + 7666.8) * 1024 + (11392.4 + 625.0) * 1024,
+        2 * (500232.6 + 7674.8) * 1024 + (11391.4 + 625.0) * 1024,
     (2000, 1_000_000, 2, "mixed", "onepass"):
-        2 * (288598.6 + 7666.8) * 1024 + (11389.7 + 625.0) * 1024,
+        2 * (288614.9 + 7674.8) * 1024 + (11389.8 + 625.0) * 1024,
 }
 
 
@@ -68,13 +68,13 @@ def matvec_bytes(C: int, P: int, M: int, shared: bool, factored: bool = False) -
 def onepass_bytes(C: int, P: int, M: int, shared: bool, factored: bool, rows: int,
                   tiles: int) -> int:
     """Algorithmic HBM bytes of one one-pass Schur product (k_schur_onepass + k_onepass_reduce):
-    every observation's stored words read once (FP64 cross blocks: 21 doubles shared, 18 fixed;
-    the mixed form's FP32 factors: 11 or 9 floats) and its code word (window row, rank, tile
-    point, place in its point's run); V^-1 of every point; each tile's 8-word record; the camera
-    vectors p read, the task partials written, and, for the factors, each camera's rotation; the
-    rows of camera partials written once and read once (8 doubles each)."""
-    per_obs = (11 if shared else 9) * 4 if factored else (21 if shared else 18) * 8
-    per_cam = 6 * 8 + 8 * 8 + (9 * 8 if factored else 0)
+    every observation's stored Jacobian factors read once (11 words shared, 9 fixed: FP32 in the
+    mixed form, `factored`, FP64 in the FP64 form) and its code word (window row, rank, tile point,
+    place in its point's run); V^-1 of every point; each tile's 8-word record; the camera vectors
+    p and rotations read, the task partials written; the rows of camera partials written once and
+    read once (8 doubles each)."""
+    per_obs = (11 if shared else 9) * (4 if factored else 8)
+    per_cam = 6 * 8 + 8 * 8 + 9 * 8
     return (M * (per_obs + 4) + P * 9 * 8 + tiles * 8 * 4 + C * per_cam + rows * 8 * 8 * 2
             + (8 if shared else 0))
 
