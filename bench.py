@@ -242,7 +242,11 @@ def main() -> int:
             "cameras": C, "points": P, "observations": M, "lm_iterations_max": args.iterations,
             "max_cg": control.max_cg, "huber_px": 1.0, "precision": first,
             "outliers": info["outliers"], "focal_factor": info["focal_factor"],
-            "operator": "stored", "linear_solver": "matrix-free PCG",
+            # the one-pass Schur product over each observation's stored Jacobian factors (FP64
+            # in the FP64 form, FP32 in the mixed one)
+            "operator": "stored Jacobian factors" if plain.get("schur_matvec", {}).get(
+                "onepass_rows") else "stored cross blocks",
+            "linear_solver": "matrix-free PCG",
         },
     }
     for key in ("observations_per_s", "pcg_iterations_per_s", "lm_iterations_per_step",
