@@ -40,15 +40,13 @@ def tile_slots(control, packed) -> int:
 
 
 def stored_bytes(control, packed) -> int:
-    """Bytes of the arrays only a resident solve holds: the point-major cross blocks (FP64: K = 18
-    or 21 words per observation; the mixed form's FP32 Jacobian factors: 9 or 11), the camera-major
-    ones (K words per slot of 64-slot blocks) and, with FP64 blocks, the fused product's 64-byte
-    rows."""
+    """Bytes of the arrays only a resident solve holds: each observation's Jacobian factors
+    (K = 11 words, 9 on the fixed route; FP64, or FP32 in the mixed form), point-major and
+    camera-major (K words per slot of 64-slot blocks)."""
     mixed = control.precision == _native.MIXED
-    K = (11 if control.shared_focal else 9) if mixed else (21 if control.shared_focal else 18)
+    K = 11 if control.shared_focal else 9
     ws = 4 if mixed else 8
-    words = K * control.observations + K * tile_slots(control, packed)
-    return words * ws + (64 * control.observations if ws == 8 else 0)
+    return (K * control.observations + K * tile_slots(control, packed)) * ws
 
 
 def scenes():

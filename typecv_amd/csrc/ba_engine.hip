@@ -394,6 +394,9 @@ struct Launch {
     else if constexpr (sizeof(WT) == 4)
       hipLaunchKernelGGL((k_point_sweep_factored<SHARED, SWEEP_SCHUR>), dim3(tile_blocks(q)),
                          dim3(TILE * TILE_WAVES), 0, s, q);
+    else if (q.factored64)
+      hipLaunchKernelGGL((k_point_sweep_factored<SHARED, SWEEP_SCHUR, false, double>),
+                         dim3(tile_blocks(q)), dim3(TILE * TILE_WAVES), 0, s, q);
     else if (q.tprod)
       hipLaunchKernelGGL((k_point_sweep_fused<WT, SHARED>), dim3(tile_blocks(q)),
                          dim3(TILE * TILE_WAVES), 0, s, q);
@@ -409,6 +412,9 @@ struct Launch {
     else if constexpr (sizeof(WT) == 4)
       hipLaunchKernelGGL((k_camera_sweep_factored<SHARED>), dim3(q.n_tasks), dim3(TASK_THREADS),
                          q.camera_lds, s, q);
+    else if (q.factored64)
+      hipLaunchKernelGGL((k_camera_sweep_factored<SHARED, false, double>), dim3(q.n_tasks),
+                         dim3(TASK_THREADS), q.camera_lds, s, q);
     else if (q.tprod)
       hipLaunchKernelGGL((k_camera_tsum<SHARED>), dim3(q.n_tasks), dim3(TASK_THREADS),
                          q.camera_lds, s, q);
@@ -446,6 +452,9 @@ struct Launch {
     else if constexpr (sizeof(WT) == 4)
       hipLaunchKernelGGL((k_point_sweep_factored<SHARED, SWEEP_SCHUR, true>), dim3(tile_blocks(q)),
                          dim3(TILE * TILE_WAVES), 0, s, q);
+    else if (q.factored64)
+      hipLaunchKernelGGL((k_point_sweep_factored<SHARED, SWEEP_SCHUR, true, double>),
+                         dim3(tile_blocks(q)), dim3(TILE * TILE_WAVES), 0, s, q);
     else
       hipLaunchKernelGGL((k_point_sweep<WT, SHARED, SWEEP_SCHUR, true>), dim3(tile_blocks(q)),
                          dim3(TILE * TILE_WAVES), 0, s, q);
@@ -457,6 +466,9 @@ struct Launch {
     }
     else if constexpr (sizeof(WT) == 4)
       hipLaunchKernelGGL((k_camera_sweep_factored<SHARED, true>), dim3(q.n_tasks),
+                         dim3(TASK_THREADS), 0, s, q);
+    else if (q.factored64)
+      hipLaunchKernelGGL((k_camera_sweep_factored<SHARED, true, double>), dim3(q.n_tasks),
                          dim3(TASK_THREADS), 0, s, q);
     else
       hipLaunchKernelGGL((k_camera_sweep<WT, SHARED, false, true>), dim3(q.n_tasks),
@@ -705,8 +717,9 @@ struct Need {
 };
 
 // Every device array a solve of `plan` allocates (besides the caller's staging): the resident
-// form stores the cross blocks (and, with FP64 blocks, the fused product's rows); the streamed
-// form relinearises them in every sweep and stores none.
+// form stores each observation's Jacobian factors (FP64 or, mixed, FP32), or for the exact systems
+// its FP64 cross block (and then the fused product's rows); the streamed form relinearises them in
+// every sweep and stores none.
 std::vector<Need> solve_needs(tcv_ba_context* ctx, const tcv_ba_control* c, uint32_t B,
                               uint32_t C, uint32_t P, uint32_t M, const Plan& plan, bool exact,
                               bool streamed) {
@@ -717,9 +730,10 @@ std::vector<Need> solve_needs(tcv_ba_context* ctx, const tcv_ba_control* c, uint
   const uint32_t n_unstored = exact ? 0u : (uint32_t)plan.direct_bundles.size();
   const size_t n_tiles = plan.tiles.size() - 1, n_tasks = plan.tasks.size() / 4;
   const size_t n_cost = plan.cost_blocks.size() / 3, n_groups = plan.groups.size() / GROUP_WORDS;
-  // the cross blocks (FP64, K words per observation) or the mixed form's FP32 factors (KF words)
-  const size_t w_words = (size_t)(ws == 4 ? kf_words(shared) : K) * plan.tile_slot.back();
-  const size_t wp_words = (size_t)(ws == 4 ? kf_words(shared) : K) * M;  // tile_word: no padding
+  // the factors (KF words per observation) or, for the exact systems, the cross blocks (K)
+  const uint32_t words = exact ? K : kf_words(shared);
+  const size_t w_words = (size_t)words * plan.tile_slot.back();
+  const size_t wp_words = (size_t)words * M;  // tile_word: no padding
   std::vector<Need> n = {
       {&ctx->scene, ((size_t)C * 18 + (size_t)P * 3 + (size_t)M * 2) * 8},
       {&ctx->trial, ((size_t)C * 12 + (size_t)P * 3) * 8},
@@ -752,10 +766,11 @@ std::vector<Need> solve_needs(tcv_ba_context* ctx, const tcv_ba_control* c, uint
   if (n_unstored < B && !streamed) {
     n.push_back({&ctx->Wp, wp_words * ws});
     n.push_back({&ctx->Wc, w_words * ws});
-    // the fused FP64 product's rows (the tuner times it against the plain sweeps, which compute
-    // the same bits); FP64 blocks only: over FP32 ones it is slower (70.8 + 27.4 against 42.6 +
-    // 40.0 us at 500 cameras), since the rows stay FP64 while the blocks it saves are halved
-    if (ws == 8) n.push_back({&ctx->tprod, (size_t)M * 64});
+    // the fused FP64 product's rows over the cross blocks of the exact systems (the tuner times it
+    // against the plain sweeps, which compute the same bits); over factors it is not offered (over
+    // FP32 cross blocks it was slower: 70.8 + 27.4 against 42.6 + 40.0 us at 500 cameras, since
+    // the rows stay FP64 while the blocks it saves are halved)
+    if (ws == 8 && exact) n.push_back({&ctx->tprod, (size_t)M * 64});
   }
   if (!exact) {
     // the one-pass product's workgroups (at most OP_TILES tiles of one bundle each), windows, row
@@ -895,13 +910,15 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
   const uint32_t n_tasks = (uint32_t)plan.tasks.size() / 4;
   const uint32_t n_cost = (uint32_t)plan.cost_blocks.size() / 3;
   const uint32_t n_groups = (uint32_t)plan.groups.size() / GROUP_WORDS;
-  const size_t w_words = (size_t)(ws == 4 ? kf_words(shared) : K) * plan.tile_slot.back();
+  // the stored words per observation: the factors, or the exact systems' cross blocks
+  const uint32_t stored_words = exact ? K : kf_words(shared);
+  const size_t w_words = (size_t)stored_words * plan.tile_slot.back();
   const size_t vec_words = (size_t)m * 5 + (size_t)P * 3;
   bool streamed = residency == STREAMED;
   if (residency == DECIDE) TRY(admit(ctx, c, B, C, P, M, plan, exact, {}, streamed));
   for (const Need& x : solve_needs(ctx, c, B, C, P, M, plan, exact, streamed))
     TRY(ensure(*x.buf, x.bytes));
-  const bool fused = !streamed && ws == 8 && n_unstored < B;
+  const bool fused = !streamed && ws == 8 && exact && n_unstored < B;
   TRY(ensure_host_state(ctx, (size_t)B * S_WORDS));
   if (direct) {
     TRY(upload(ctx->direct_bundles, plan.direct_bundles, st));
@@ -1019,7 +1036,7 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
     // every sweep (the same bits)
     if (q.n_unstored < B && ctx->budget) q.streamed = 1;
     if (q.n_unstored < B && !q.streamed) {
-      TRY(ensure(ctx->Wp, (size_t)(ws == 4 ? kf_words(shared) : K) * M * ws));
+      TRY(ensure(ctx->Wp, (size_t)stored_words * M * ws));
       TRY(ensure(ctx->Wc, w_words * ws));
       q.Wp = ctx->Wp.ptr;
       q.Wc = ctx->Wc.ptr;
@@ -1034,10 +1051,10 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
   ctx->bundle_ops.assign(B + 1, 0);
   ctx->map_on.clear();
   if (!exact) TRY(plan_onepass(ctx, q, plan, st, rep));
-  // with the one-pass product the FP64 form stores FP64 Jacobian factors (ba_factored.hip): 88
-  // bytes per observation instead of the cross block's 168, the product 187 against 291 us at
-  // 2,000 cameras
-  q.factored64 = q.n_op != 0 && c->precision == TCV_BA_FP64 ? 1u : 0u;
+  // the FP64 form stores FP64 Jacobian factors (ba_factored.hip): 88 bytes per observation
+  // instead of the cross block's 168 (the one-pass product 187 against 291 us at 2,000 cameras);
+  // the exact systems keep the cross blocks
+  q.factored64 = !exact && c->precision == TCV_BA_FP64 ? 1u : 0u;
   ctx->large_bundles = plan.large_bundles;
   ctx->large_n.clear();
   ctx->large_off.clear();

@@ -1,6 +1,5 @@
-// ba_factored.hip — the Schur sweeps over Jacobian factors: FP32 (the mixed form) or FP64 (the
-// FP64 form's point update when the one-pass product runs); included by ba_engine.hip after
-// ba_kernels.hip.
+// ba_factored.hip — the Schur sweeps over Jacobian factors, FP32 (the mixed form) or FP64 (the FP64
+// form); included by ba_engine.hip after ba_kernels.hip.
 //
 // An observation's cross block W = w Jc^T Jp (and the focal row w Jf^T Jp) is a product of its
 // Jacobian rows, and those are fixed by three small pieces: the projection derivative
@@ -15,8 +14,9 @@
 //   camera side W y            : h = D' (R y);  omega: q x (D'^T h),  v: D'^T h,  focal: Jf' . h
 // so a Schur product streams 2 (KF + 1) words per observation instead of 2 (K + 1). Sums keep the
 // stored sweeps' orders (segment sums per point, slot order and wave butterflies per camera task).
-// The FP64 form stores the same words in FP64 when the one-pass product runs (ba_onepass.hip): 88
-// (72) bytes per observation instead of the 168 (144) of its cross block.
+// The FP64 form stores the same words in FP64 (but for the exact systems of the polish and the
+// adjoint, whose exact-Hessian correction has no factored form): 88 (72) bytes per observation
+// instead of the 168 (144) of its cross block.
 #pragma once
 
 namespace tcvba {
@@ -129,9 +129,9 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_sweep_factored(Probl
   sweep_tile_factored<FT, SHARED, MODE, RECOMPUTE>(q, t, lane, st_all[wave]);
 }
 
-// The camera sweep over FP32 factors: one workgroup per camera-list task, the camera's rotation
-// and flags uniform, k_camera_sweep's slot order and reduction.
-template <bool SHARED, bool RECOMPUTE = false>
+// The camera sweep over the factors (FT: float or double): one workgroup per camera-list task, the
+// camera's rotation and flags uniform, k_camera_sweep's slot order and reduction.
+template <bool SHARED, bool RECOMPUTE = false, class FT = float>
 __global__ __launch_bounds__(TASK_THREADS) void k_camera_sweep_factored(Problem q) {
   constexpr int NV = SHARED ? 7 : 6;
   __shared__ double red[TASK_THREADS / 64][NV];
@@ -175,7 +175,7 @@ __global__ __launch_bounds__(TASK_THREADS) void k_camera_sweep_factored(Problem 
     for (int k = 0; k < PER; k++) {
       const uint32_t n = slot[k];
       double fw[11];
-      load_factors<float, SHARED, RECOMPUTE>(q, RECOMPUTE ? nullptr : fptr<SHARED>(q.Wc, 0, n), TILE,
+      load_factors<FT, SHARED, RECOMPUTE>(q, RECOMPUTE ? nullptr : fptr<SHARED, FT>(q.Wc, 0, n), TILE,
                                       RECOMPUTE ? camera_obs(q)[n] : n, c, pt[k], g, fscale, fw);
       const bool dir = directional_point(g, pt[k]);
       const double b0 = R[0] * yv[k][0] + R[1] * yv[k][1] + R[2] * yv[k][2];

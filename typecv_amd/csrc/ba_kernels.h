@@ -201,10 +201,9 @@ struct Problem {
   // at set-up
   const uint32_t* op_code;
   const uint32_t* op_trec;
-  // the FP64 form with the one-pass product: the passes store each observation's FP64 Jacobian
+  // the FP64 form (but for the exact systems): the passes store each observation's FP64 Jacobian
   // factors (kf_words doubles, the mixed form's words unrounded) in place of its cross block, and
-  // the one-pass product and the point update form every product from them (0: the cross blocks,
-  // for the two sweeps and the exact systems)
+  // the Schur products and the point update form every product from them (0: the cross blocks)
   uint32_t factored64;
   // the PCG loop's grids restricted to the bundles whose PCG may be active (null: every bundle's):
   // n_op_map one-pass workgroups, n_cam_map cameras of the one-pass reduction, n_group_map

@@ -342,11 +342,12 @@ __device__ __forceinline__ void cross_factors(const Lin& L, bool ok, bool shared
   f[10] = shared ? s * L.Jf[1] : 0.0;
 }
 
-// Word k of observation (or slot) i's factors: blocked [M/64][KF][64] as the cross blocks are.
-template <bool SHARED>
-__device__ __forceinline__ float* fptr(void* base, uint32_t k, uint32_t i) {
+// Word k of slot i's factors (FT: float or double): blocked [slots/64][KF][64] as the cross blocks
+// are.
+template <bool SHARED, class FT = float>
+__device__ __forceinline__ FT* fptr(void* base, uint32_t k, uint32_t i) {
   constexpr uint32_t KF = kf_words(SHARED);
-  return static_cast<float*>(base) + ((size_t)(i >> 6) * KF + k) * 64 + (i & 63);
+  return static_cast<FT*>(base) + ((size_t)(i >> 6) * KF + k) * 64 + (i & 63);
 }
 
 __device__ __forceinline__ double* vx(const Problem& q) { return q.vec; }
@@ -787,10 +788,10 @@ __global__ __launch_bounds__(TASK_THREADS) void k_camera_pass(Problem q) {
       for (int k = 0; k < 21; k++) W[k] = 0;
     }
     if (stores_w(g) && q.Wc) {
-      if constexpr (sizeof(WT) == 4) {
+      if (sizeof(WT) == 4 || (!EXACT && q.factored64)) {
         double fw[11];
         cross_factors(L, ok, SHARED, fw);
-        for (int k = 0; k < kf_words(SHARED); k++) *fptr<SHARED>(q.Wc, k, n) = (float)fw[k];
+        for (int k = 0; k < kf_words(SHARED); k++) *fptr<SHARED, WT>(q.Wc, k, n) = (WT)fw[k];
       } else {
         for (uint32_t k = 0; k < K; k++) *wptr<WT, SHARED>(q.Wc, k, n, q.M) = (WT)W[k];
       }
