@@ -56,3 +56,18 @@ def repeat_tracks(packed, lengths: dict, seed: int = 0, noise_px: float = 0.5):
     return pack_arrays(packed.poses(), packed.models(), packed.points(), np.array(out_len),
                        np.concatenate(out_cam), np.concatenate(out_px),
                        packed.section("cameraActive"))
+
+
+def shuffle_points(packed, seed: int = 0):
+    """The packed scene with its points (and their tracks) in a random order: consecutive points no
+    longer share cameras, so the one-pass product's camera windows pass 64 cameras and the two
+    sweeps run."""
+    from typecv_amd.packing import pack_arrays
+    import numpy as np
+
+    perm = np.random.default_rng(seed).permutation(packed.P)
+    ps = packed.section("pointStart").astype(np.int64)
+    order = np.concatenate([np.arange(ps[p], ps[p + 1]) for p in perm])
+    return pack_arrays(packed.poses(), packed.models(), packed.points()[perm],
+                       np.diff(ps)[perm], packed.section("observationCamera")[order],
+                       packed.pixels()[order], packed.section("cameraActive"))
