@@ -194,7 +194,12 @@ int tcv_ba_fail_reads(tcv_ba_context* context, uint32_t n);
 /*
  * Solve from host arrays: upload, run the LM loop, read back `out`. The call that the Node-API
  * addon runs as async work for `bundleAdjustGpu` (reference src/sfm/bundle/gpu/bundle.ts:803-839:
- * everything inside `device.enqueue`, bundle.ts:178-799 and joint-focal.ts equivalents).
+ * everything inside `device.enqueue`, bundle.ts:178-799 and joint-focal.ts equivalents). When the
+ * given point order keeps some 32 consecutive tiles of points from fitting a window of 64 cameras
+ * (the one-pass Schur product's condition), the points are solved in the order of the lowest
+ * camera of their tracks and `out` holds them back in the given order; the results differ from
+ * the given order's only by the order of sums over points. tcv_ba_solve_batch does the same per
+ * problem; tcv_ba_solve_device takes the points in the given order.
  */
 int tcv_ba_solve(tcv_ba_context* context, const tcv_ba_control* control, const double* scene,
                  const uint32_t* index, double* out, tcv_ba_report* report);
