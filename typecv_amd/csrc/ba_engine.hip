@@ -66,7 +66,7 @@ struct tcv_ba_context {
       tprod,
       obs_cameras, obs_points, obs_pixels, obs_indices, obs_offsets, obs_flags, obs_errors,
       obs_partials, obs_summary, op_tile, op_lo, op_off, op_cam, op_row, op_part, op_code, op_trec,
-      pcg_map;
+      pcg_map, cam_dot;
   // per bundle (B + 1 prefix counts) its first one-pass workgroup, camera and PCG-step group, and
   // the host side of pcg_map (run_pcg's compacted grids)
   std::vector<uint32_t> bundle_ops, bundle_cams, bundle_groups, h_map;
@@ -96,7 +96,7 @@ struct tcv_ba_context {
            &dense,      &direct_bundles, &direct_cams, &covis, &small_bundles, &tprod,
            &obs_cameras, &obs_points, &obs_pixels, &obs_indices, &obs_offsets, &obs_flags,
            &obs_errors, &obs_partials, &obs_summary, &tile_slot, &op_tile, &op_lo, &op_off, &op_cam, &op_row,
-           &op_part,    &op_code,      &op_trec,   &pcg_map};
+           &op_part,    &op_code,      &op_trec,   &pcg_map,   &cam_dot};
   }
 };
 
@@ -423,6 +423,21 @@ struct Launch {
       hipLaunchKernelGGL((k_camera_sweep<WT, SHARED>), dim3(q.n_tasks), dim3(TASK_THREADS),
                          q.camera_lds, s, q);
   }
+  // After the one-pass product: each camera's rows summed and S p finished (k_onepass_product),
+  // then the PCG step without k_cg_product.
+  static void onepass_product(const Problem& q, hipStream_t s) {
+    if (reduce_grid(q))
+      hipLaunchKernelGGL((k_onepass_product<SHARED>), dim3(reduce_grid(q)), dim3(256), 0, s, q);
+  }
+  static void advance_rows(const Problem& q, hipStream_t s) {
+    if (q.n_single_map)
+      hipLaunchKernelGGL((k_cg_single<SHARED, false>), dim3(q.n_single_map), dim3(ADV_THREADS), 0,
+                         s, q);
+    const uint32_t groups = q.group_map ? q.n_group_map : q.n_groups;
+    if (!groups) return;
+    hipLaunchKernelGGL((k_cg_step<SHARED>), dim3(groups), dim3(ADV_THREADS), 0, s, q);
+    hipLaunchKernelGGL((k_cg_direction<SHARED>), dim3(groups), dim3(ADV_THREADS), 0, s, q);
+  }
   static void advance(const Problem& q, hipStream_t s) {
     if (q.n_single_map)
       hipLaunchKernelGGL((k_cg_single<SHARED>), dim3(q.n_single_map), dim3(ADV_THREADS), 0, s, q);
@@ -627,6 +642,10 @@ struct Ops {
   void (*unstored_update)(const Problem&, hipStream_t);
   void (*large_factor)(const Problem&, uint32_t, uint32_t, uint64_t, hipStream_t);
   void (*large_solve)(const Problem&, uint32_t, uint32_t, uint64_t, int, hipStream_t);
+  // the PCG loop after the one-pass product (null for the implicit operator, whose sweeps never
+  // run it): S p finished per camera with the rows' reduction, then the step and direction
+  void (*onepass_product)(const Problem&, hipStream_t);
+  void (*advance_rows)(const Problem&, hipStream_t);
 };
 
 // The PCG sweeps of one iteration over the stored and the unstored bundles.
@@ -651,12 +670,14 @@ Ops ops_for(bool implicit, bool streamed) {
         L::refine,       L::exact_passes,  L::solve_points,    L::pixel_cotangent,
         L::jacobian,     L::dense_solve,   L::certify,         L::dense_correct,
         L::unstored_p_sweep, L::unstored_camera_sweep, L::implicit_update,
-        L::large_factor, L::large_solve};
+        L::large_factor, L::large_solve, L::onepass_product, L::advance_rows};
   if (implicit) {
     o.point_sweep = L::implicit_p_sweep;
     o.camera_sweep = L::implicit_camera_sweep;
     o.unstored_point_sweep = nullptr;
     o.unstored_camera_sweep = nullptr;
+    o.onepass_product = nullptr;
+    o.advance_rows = nullptr;
   } else if (streamed) {
     o.point_sweep = L::streamed_point_sweep;
     o.camera_sweep = L::streamed_camera_sweep;
@@ -788,6 +809,8 @@ std::vector<Need> solve_needs(tcv_ba_context* ctx, const tcv_ba_control* c, uint
     n.push_back({&ctx->op_trec, n_tiles * OP_RECORD * 4});
     // the PCG loop's grids: one-pass workgroups, cameras and PCG-step groups
     n.push_back({&ctx->pcg_map, (n_op + C + n_groups) * 4});
+    // each camera's p . S p and focal share after the one-pass product
+    n.push_back({&ctx->cam_dot, (size_t)C * 2 * 8});
   }
   if (!plan.direct_bundles.empty()) {
     n.push_back({&ctx->camS, (size_t)C * 36 * 8});
@@ -1286,7 +1309,11 @@ int run_pcg(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, cons
               q.n_unstored < q.B && std::min(first, budget) >= kTuneFirst + kTuneRuns;
   static_assert(kTuneFirst > 0 && kTuneFirst + kTuneRuns <= kTimingStride, "tuning iterations");
   if (tune) TRY(ensure_tune_events(ctx));
-  Problem qt = scheduled(q, tuned ? TuneCandidate{ctx->tuned_fused, ctx->camera_cap} : kTune[0]);
+  // after the one-pass product S p is finished per camera (k_onepass_product): one launch less
+  const bool rows = q.n_op != 0 && ops.onepass_product != nullptr;
+  Problem qr = q;
+  qr.cam_dot = rows ? (double*)ctx->cam_dot.ptr : nullptr;
+  Problem qt = scheduled(qr, tuned ? TuneCandidate{ctx->tuned_fused, ctx->camera_cap} : kTune[0]);
   // the launches cover the bundles whose PCG may run: those whose PCG the last read of the state
   // words showed active (`known`: it followed this pass's set-up), else (a batch) those whose LM
   // loop it showed active, or the one bundle; after each chunk those still in PCG
@@ -1312,10 +1339,15 @@ int run_pcg(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, cons
         if (timed) HIP_TRY(hipEventRecord(ctx->events[3 * k], st));
         point_sweeps(qa, ops, st);
         if (timed) HIP_TRY(hipEventRecord(ctx->events[3 * k + 1], st));
-        camera_sweeps(qa, ops, st);
+        if (!rows) {
+          camera_sweeps(qa, ops, st);
+        } else {
+          if (ops.unstored_camera_sweep && q.n_unstored) ops.unstored_camera_sweep(qa, st);
+          ops.onepass_product(qa, st);
+        }
         if (timed) HIP_TRY(hipEventRecord(ctx->events[3 * k + 2], st));
       }
-      ops.advance(qa, st);
+      (rows ? ops.advance_rows : ops.advance)(qa, st);
     }
     HIP_TRY(hipGetLastError());
     if (rep) rep->matvec_launches += n;
@@ -1338,7 +1370,7 @@ int run_pcg(tcv_ba_context* ctx, const tcv_ba_control* c, const Problem& q, cons
     }
     if (tune) {
       TRY(tune_pick(ctx, q.tprod != nullptr));
-      qt = scheduled(q, TuneCandidate{ctx->tuned_fused, ctx->camera_cap});
+      qt = scheduled(qr, TuneCandidate{ctx->tuned_fused, ctx->camera_cap});
       tune = false;
       tune_done = true;
     }

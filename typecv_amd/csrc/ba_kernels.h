@@ -205,6 +205,10 @@ struct Problem {
   // factors (kf_words doubles, the mixed form's words unrounded) in place of its cross block, and
   // the Schur products and the point update form every product from them (0: the cross blocks)
   uint32_t factored64;
+  // after the one-pass product, p . S p and the focal share of each camera (2 per camera), written
+  // by k_onepass_product with the camera's rows of S p and summed by k_cg_step over the bundle's
+  // cameras (null: the group partials of k_cg_product)
+  double* cam_dot;
   // the PCG loop's grids restricted to the bundles whose PCG may be active (null: every bundle's):
   // n_op_map one-pass workgroups, n_cam_map cameras of the one-pass reduction, n_group_map
   // PCG-step groups of bundles with several groups and n_single_map groups of bundles with one

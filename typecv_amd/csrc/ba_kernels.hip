@@ -1505,8 +1505,10 @@ __device__ double schur_finish(const Problem& q, const Bundle& g, const double* 
 //   k_cg_step       focal row, p . S p, alpha and its tests; x += alpha p, r -= alpha S p,
 //                   z = M^-1 r (the camera's 6x6 block); per group |r|^2 and r . z
 //   k_cg_direction  |r|^2 and r . z, the convergence and breakdown tests, beta, p = z + beta p
-// rho of the next iteration goes to word S_RHO_NEXT (the launch still reads word 18) and becomes
-// word 18 at the next k_cg_product.
+// rho of the next iteration goes to word S_RHO_NEXT (the launch still reads word 18); k_cg_step
+// reads it there and its first group copies it to word 18 for the next k_cg_direction. After the
+// one-pass product, k_onepass_product finishes S p per camera and k_cg_step sums the cameras'
+// p . S p (cam_dot) instead of k_cg_product's group partials.
 struct Group {
   uint32_t bundle, c0, c1, g0, g1, row, rows;
   bool has, first;
@@ -1546,6 +1548,18 @@ __device__ __forceinline__ void group_loads(const Problem& q, const Group& G, in
   }
 }
 
+// This thread's share of the sums over bundle g's cameras of their p . S p and focal shares
+// (cam_dot); block_sum2 completes them, in camera order.
+__device__ __forceinline__ void camera_loads(const Problem& q, const Bundle& g, double& a,
+                                             double& b) {
+  a = 0;
+  b = 0;
+  for (uint32_t c = g.cam0 + threadIdx.x; c < g.cam1; c += blockDim.x) {
+    a += q.cam_dot[2 * (size_t)c];
+    b += q.cam_dot[2 * (size_t)c + 1];
+  }
+}
+
 // Each launch below issues all of its loads (state words, partials, its rows) before it tests the
 // bundle's state: the chain of dependent loads from the group record to the first store is then
 // two loads deep (three in k_cg_product, whose task partials follow the camera's task range)
@@ -1564,7 +1578,7 @@ __device__ __forceinline__ void cg_product_group(const Problem& q, uint32_t gi, 
   double* s = bstate(q, G.bundle);
   const double* p = vp(q);
   const uint32_t row = G.has ? G.row : 6 * G.c0, c = row / 6, j = row - 6 * c;
-  double active = s[S_PCG_ACTIVE], rho_next = s[S_RHO_NEXT];
+  double active = s[S_PCG_ACTIVE];
   double pf = SHARED ? p[6 * q.C + G.bundle] : 0.0, hcf = SHARED ? q.camHcf[row] : 0.0;
   double pr = p[row];
   double A[6], pc[6], part[NV];
@@ -1575,7 +1589,6 @@ __device__ __forceinline__ void cg_product_group(const Problem& q, uint32_t gi, 
   }
   camera_sums<NV>(q, c, part);
   settle(active);
-  settle(rho_next);
   settle(pf);
   settle(hcf);
   settle(pr);
@@ -1602,7 +1615,6 @@ __device__ __forceinline__ void cg_product_group(const Problem& q, uint32_t gi, 
   if (threadIdx.x == 0) {
     q.group_part[4 * (size_t)gi] = d;
     q.group_part[4 * (size_t)gi + 1] = fs;
-    if (G.first) s[S_RHO] = rho_next;
   }
 }
 
@@ -1625,10 +1637,13 @@ __device__ __forceinline__ void cg_step_group(const Problem& q, uint32_t gi, CgL
   double* z = vz(q);
   double* sv = vs(q);
   const double* p = vp(q);
-  double active = s[S_PCG_ACTIVE], rho = s[S_RHO];
+  double active = s[S_PCG_ACTIVE], rho = s[S_RHO_NEXT];
   double fdiag = SHARED ? s[S_FOCAL_DIAG] : 0.0, finv = SHARED ? s[S_FOCAL_INV] : 0.0;
   double dot, fsum;
-  group_loads(q, G, 0, dot, fsum);
+  if (q.cam_dot)
+    camera_loads(q, q.bundles[G.bundle], dot, fsum);
+  else
+    group_loads(q, G, 0, dot, fsum);
   double xr = x[row], pr = p[row], rv = r[row], svr = sv[row];
   double pf = SHARED ? p[f] : 0.0, xf = SHARED ? x[f] : 0.0, rf = SHARED ? r[f] : 0.0;
   double inv[6];
@@ -1659,6 +1674,7 @@ __device__ __forceinline__ void cg_step_group(const Problem& q, uint32_t gi, CgL
   const double alpha = rho / den;
   const bool broke = !(den > 0) || !(fabs(alpha) < 1e20);
   if (G.first && tid == 0) {
+    s[S_RHO] = rho;
     s[S_MATVECS] += 1;
     if (SHARED) sv[f] = sf;
     if (broke) {
@@ -1770,12 +1786,15 @@ __global__ __launch_bounds__(ADV_THREADS) void k_cg_direction(Problem q) {
 // are the workgroup's barriers. The same bodies in the same order as the three launches, so the
 // same bits; the barriers also order each step's global writes (the state words, group_part, S p,
 // r, z) before the next step's reads, as a launch boundary does.
-template <bool SHARED>
+// PRODUCT false: after k_onepass_product, which has finished S p, the step and the direction only.
+template <bool SHARED, bool PRODUCT = true>
 __global__ __launch_bounds__(ADV_THREADS) void k_cg_single(Problem q) {
   __shared__ CgLds L;
   const uint32_t gi = q.single_map[blockIdx.x];
-  cg_product_group<SHARED>(q, gi, L);
-  __syncthreads();
+  if (PRODUCT) {
+    cg_product_group<SHARED>(q, gi, L);
+    __syncthreads();
+  }
   cg_step_group<SHARED>(q, gi, L);
   __syncthreads();
   cg_direction_group<SHARED>(q, gi, L);

@@ -577,6 +577,76 @@ __global__ __launch_bounds__(256) void k_onepass_reduce(Problem q) {
     onepass_reduce_camera<SHARED>(q, q.cam_map ? q.cam_map[k] : k, threadIdx.x & 63);
 }
 
+// In the PCG loop after the one-pass product: camera c's rows of partials summed as
+// onepass_reduce_camera sums them (or, for a bundle whose cross blocks are not stored, the task
+// partials of its own camera sweep), then its rows of S p, (S p)_c = A_c p_c [+ H_cf p_f] - the
+// partials, by lanes 0..5 in k_cg_product's expressions, and the camera's p . S p and focal share
+// H_cf . p_c - partial[6], summed over the six rows in row order, for k_cg_step (cam_dot). One
+// launch with the reduction's parallelism, a wave per camera, in place of the reduction and
+// k_cg_product.
+template <bool SHARED>
+__device__ __forceinline__ void onepass_product_camera(const Problem& q, uint32_t c,
+                                                       uint32_t lane) {
+  constexpr int NV = SHARED ? 7 : 6;
+  const uint32_t b = q.camera_bundle[c];
+  if (bstate(q, b)[S_PCG_ACTIVE] == 0) return;
+  const double* p = vp(q);
+  const uint32_t j = lane < 6 ? lane : 0, row = 6 * c + j;
+  double A[6], pc[6];
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    A[k] = q.camA[(size_t)c * 36 + j * 6 + k];
+    pc[k] = p[6 * (size_t)c + k];
+  }
+  const double pr = p[row], pf = SHARED ? p[6 * q.C + b] : 0.0;
+  const double hcf = SHARED ? q.camHcf[row] : 0.0;
+  double part[NV];
+  if (q.n_unstored != 0 && !stores_w(q.bundles[b])) {
+    camera_sums<NV>(q, c, part);
+  } else {
+    double acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) acc[k] = 0;
+    for (uint32_t r = q.op_cam[c] + lane; r < q.op_cam[c + 1]; r += 64) {
+      const double* rw = q.op_part + (size_t)q.op_row[r] * 8;
+#pragma unroll
+      for (int k = 0; k < NV; k++) acc[k] += rw[k];
+    }
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+      for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_xor(acc[k], o, 64);
+      part[k] = 0.0 + acc[k];  // what camera_sums forms from the one task partial
+    }
+  }
+  double a = SHARED ? hcf * pf : 0.0, av = 0;
+#pragma unroll
+  for (int k = 0; k < 6; k++) av += A[k] * pc[k];
+  a += av;
+  double pj = part[0];
+#pragma unroll
+  for (int k = 1; k < 6; k++) pj = j == (uint32_t)k ? part[k] : pj;
+  a -= pj;
+  const double d = pr * a, fs = SHARED ? hcf * pr - (j == 0 ? part[NV - 1] : 0.0) : 0.0;
+  if (lane < 6) vs(q)[row] = a;
+  double dc = 0, fc = 0;
+#pragma unroll
+  for (int r = 0; r < 6; r++) {
+    dc += __shfl(d, r, 64);
+    fc += __shfl(fs, r, 64);
+  }
+  if (lane == 0) {
+    q.cam_dot[2 * (size_t)c] = dc;
+    q.cam_dot[2 * (size_t)c + 1] = fc;
+  }
+}
+
+template <bool SHARED>
+__global__ __launch_bounds__(256) void k_onepass_product(Problem q) {
+  const uint32_t k = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / 64);
+  if (k < (q.cam_map ? q.n_cam_map : q.C))
+    onepass_product_camera<SHARED>(q, q.cam_map ? q.cam_map[k] : k, threadIdx.x & 63);
+}
+
 // Set-up: the camera window of each one-pass workgroup (Window): the narrower of the plain and the
 // wrapped range of its observations' cameras (span 0 for a workgroup without observations).
 __global__ __launch_bounds__(256) void k_onepass_windows(Problem q, uint32_t* lo, uint32_t* span) {
