@@ -35,7 +35,8 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE x 1.00, the factors that make a 1 GiB device copy read and write 1 GiB. Keyed by
 # (cameras, points, seed, precision, operator); other workloads report null.
 MEASURED_TRAFFIC = {
-    # the one-pass product (k_schur_onepass + k_onepass_reduce) over FP64 or FP32 factors
+    # the one-pass product over FP64 or FP32 factors (k_schur_onepass + k_onepass_reduce, the
+    # probe's reduction, which moves about 0.7 MB less per product than k_onepass_product)
     (2000, 1_000_000, 2, "fp64", "onepass"):
         2 * (500232.6 + 7674.8) * 1024 + (11391.4 + 625.0) * 1024,
     (2000, 1_000_000, 2, "mixed", "onepass"):
@@ -67,14 +68,15 @@ def matvec_bytes(C: int, P: int, M: int, shared: bool, factored: bool = False) -
 
 def onepass_bytes(C: int, P: int, M: int, shared: bool, factored: bool, rows: int,
                   tiles: int) -> int:
-    """Algorithmic HBM bytes of one one-pass Schur product (k_schur_onepass + k_onepass_reduce):
-    every observation's stored Jacobian factors read once (11 words shared, 9 fixed: FP32 in the
-    mixed form, `factored`, FP64 in the FP64 form) and its code word (window row, rank, tile point,
-    place in its point's run); V^-1 of every point; each tile's 8-word record; the camera vectors
-    p and rotations read, the task partials written; the rows of camera partials written once and
-    read once (8 doubles each)."""
+    """Algorithmic HBM bytes of one one-pass Schur product as the PCG loop runs it (k_schur_onepass +
+    k_onepass_product): every observation's stored Jacobian factors read once (11 words shared, 9
+    fixed: FP32 in the mixed form, `factored`, FP64 in the FP64 form) and its code word (window
+    row, rank, tile point, place in its point's run); V^-1 of every point; each tile's 8-word
+    record; per camera p, its rotation and damped block A read, its rows of S p and its p.Sp and
+    focal share written; the rows of camera partials written once and read once (8 doubles
+    each)."""
     per_obs = (11 if shared else 9) * (4 if factored else 8)
-    per_cam = 6 * 8 + 8 * 8 + 9 * 8
+    per_cam = 6 * 8 + 9 * 8 + 36 * 8 + 8 * 8
     return (M * (per_obs + 4) + P * 9 * 8 + tiles * 8 * 4 + C * per_cam + rows * 8 * 8 * 2
             + (8 if shared else 0))
 
@@ -157,7 +159,7 @@ def form_record(packed, seed, precision, operator, reports, elapsed, values):
             "traffic": MEASURED_TRAFFIC.get((C, P, seed, precision,
                                               "onepass" if onepass else operator)),
             "algorithmic_bytes": bytes_per,
-            "kernel": ("Schur product: k_schur_onepass + k_onepass_reduce" if onepass else
+            "kernel": ("Schur product: k_schur_onepass + k_onepass_product" if onepass else
                        "Schur matvec: k_point_sweep_factored + k_camera_sweep_factored"
                        if precision == "mixed" else
                        "Schur matvec: k_point_sweep_fused + k_camera_tsum" if fused else
