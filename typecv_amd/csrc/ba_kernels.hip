@@ -1554,9 +1554,23 @@ __device__ __forceinline__ void camera_loads(const Problem& q, const Bundle& g, 
                                              double& b) {
   a = 0;
   b = 0;
-  for (uint32_t c = g.cam0 + threadIdx.x; c < g.cam1; c += blockDim.x) {
-    a += q.cam_dot[2 * (size_t)c];
-    b += q.cam_dot[2 * (size_t)c + 1];
+  // eight cameras' loads issued together, then added in camera order
+  constexpr int U = 8;
+  for (uint32_t c0 = g.cam0 + threadIdx.x; c0 < g.cam1; c0 += U * blockDim.x) {
+    double2 v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const uint32_t c = c0 + u * blockDim.x;
+      v[u] = c < g.cam1 ? *reinterpret_cast<const double2*>(q.cam_dot + 2 * (size_t)c)
+                        : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (c0 + u * blockDim.x < g.cam1) {
+        a += v[u].x;
+        b += v[u].y;
+      }
+    }
   }
 }
 
