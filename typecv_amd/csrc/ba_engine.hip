@@ -356,8 +356,12 @@ uint32_t reduce_grid(const Problem& q) {
 template <class WT, bool SHARED>
 struct Launch {
   static void point_pass(const Problem& q, hipStream_t s) {
-    hipLaunchKernelGGL((k_point_pass<WT, SHARED, false>), dim3(tile_blocks(q)),
-                       dim3(TILE * TILE_WAVES), 0, s, q);
+    if (sizeof(WT) == 8 && q.factored64)
+      hipLaunchKernelGGL((k_point_pass<WT, SHARED, false, true>), dim3(tile_blocks(q)),
+                         dim3(TILE * TILE_WAVES), 0, s, q);
+    else
+      hipLaunchKernelGGL((k_point_pass<WT, SHARED, false>), dim3(tile_blocks(q)),
+                         dim3(TILE * TILE_WAVES), 0, s, q);
   }
   static void camera_pass(const Problem& q, hipStream_t s) {
     hipLaunchKernelGGL((k_camera_pass<WT, SHARED, false>), dim3(q.n_tasks), dim3(TASK_THREADS), 0,

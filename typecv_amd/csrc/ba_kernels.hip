@@ -546,10 +546,11 @@ __global__ __launch_bounds__(COST_OBS) void k_finish_cost(Problem q) {
 // ------------------------------------------------------------------ assembly
 
 // One wave per tile of whole points, one lane per observation: linearise, write the cross blocks
-// (point-major), stage the point terms in LDS; then one lane per point sums its observations in
-// order, damps V, inverts it and writes the point block. The shared route adds the point's share of
-// the reduced focal diagonal and RHS (bundleFocalPoints), summed per tile in point order.
-template <class WT, bool SHARED, bool EXACT>
+// or (FACTORS) the Jacobian factors (point-major), stage the point terms in LDS; then one lane per
+// point sums its observations in order, damps V, inverts it and writes the point block. The shared
+// route adds the point's share of the reduced focal diagonal and RHS (bundleFocalPoints), summed
+// per tile in point order.
+template <class WT, bool SHARED, bool EXACT, bool FACTORS = (sizeof(WT) == 4)>
 __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_pass(Problem q) {
   // staged words per observation: V 6, b 3, [fc 3], [exact systems: the Gauss-Newton V 6], bad
   constexpr int GN = SHARED ? 12 : 9;
@@ -600,7 +601,7 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_pass(Problem q) {
         for (int k = 0; k < 21; k++) W[k] = 0;
       }
       if (stores_w(g) && q.Wp) {
-        if (sizeof(WT) == 4 || (!EXACT && q.factored64)) {
+        if constexpr (FACTORS) {
           double fw[11];
           cross_factors(L, ok, SHARED, fw);
           for (int k = 0; k < kf_words(SHARED); k++)
@@ -748,14 +749,35 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_pass(Problem q) {
 __host__ __device__ constexpr int up6(int j, int l) { return j * 6 - j * (j - 1) / 2 + (l - j); }
 
 // One workgroup per camera-list task: relinearise each observation of the task in camera-major
-// order, write its cross block to the camera-major copy, and sum
+// order, write its cross block (or factors) to the camera-major copy, and sum
 //   U = sum w Jc^T Jc (21), Q = sum W V^-1 W^T (21), rhs = -sum w Jc^T r - W V^-1 b_p (6)
 //   [shared] H_cf = sum w Jc^T Jf (6), H_ff = sum w Jf^T Jf, b_f = -sum w Jf^T r
-// into ASM_WORDS per task, in a fixed order.
+// into ASM_WORDS per task, in a fixed order. Two passes over the task's observations, each
+// relinearising them: the first sums U and the focal words, the second Q and the RHS and writes
+// the stored words, so that a thread holds half of the 56 sums at once (twice the waves a SIMD
+// holds); every word takes the same terms in the same order as one pass would give it.
+template <int NW>
+__device__ __forceinline__ void task_sums(double* acc, double (*red)[29], double* out,
+                                          const int* word) {
+#pragma unroll
+  for (int k = 0; k < NW; k++)
+    for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_xor(acc[k], o, 64);
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+    for (int k = 0; k < NW; k++) red[wave][k] = acc[k];
+  __syncthreads();
+  if (threadIdx.x < (uint32_t)NW) {
+    double v = 0;
+    for (int w = 0; w < TASK_THREADS / 64; w++) v += red[w][threadIdx.x];
+    out[word[threadIdx.x]] = v;
+  }
+  __syncthreads();
+}
+
 template <class WT, bool SHARED, bool EXACT>
 __global__ __launch_bounds__(TASK_THREADS) void k_camera_pass(Problem q) {
-  constexpr int NV = SHARED ? 56 : 48;
-  __shared__ double red[TASK_THREADS / 64][NV];
+  constexpr int N1 = SHARED ? 29 : 21, N2 = 27;
+  __shared__ double red[TASK_THREADS / 64][29];
   const uint32_t t = blockIdx.x;
   const uint32_t c = q.tasks[4 * t], s = q.tasks[4 * t + 1], e = q.tasks[4 * t + 2];
   const uint32_t b = q.camera_bundle[c];
@@ -765,97 +787,119 @@ __global__ __launch_bounds__(TASK_THREADS) void k_camera_pass(Problem q) {
   const bool var = variable_camera(q, c);
   const double fscale = SHARED ? focal_scale(sb[S_LOG_FOCAL]) : 1.0;
   const uint32_t K = SHARED ? 21 : 18;
-  double acc[NV];
+  // with a right-hand side given, the camera's own b_c enters once, in k_camera_finish
+  const double rw = (EXACT && q.rhs_in) ? 0.0 : 1.0;
+  double* out = q.task_part + (size_t)t * ASM_WORDS;
+  // pass 1: U (words 0-20), H_cf (48-53), H_ff (54), b_f (55)
+  {
+    double acc[N1];
 #pragma unroll
-  for (int k = 0; k < NV; k++) acc[k] = 0;
-  for (uint32_t n = s + threadIdx.x; n < e; n += TASK_THREADS) {
-    const uint32_t i = camera_obs(q)[n], p = q.cam_point[n];
-    const bool dir = directional_point(g, p);
-    Lin L;
-    const bool ok = linearize(q, c, p, i, fscale, var, dir, L);
-    double W[21];
-    double Cx[EXACT ? 55 : 1];
-    if (ok) {
-      cross_block(L, SHARED, W);
-      if (EXACT) {
-        exact_correction(L, var, dir, q.huber, Cx, !q.gauss_newton);
-        for (int k = 0; k < 6; k++)
-          for (int j = 0; j < 3; j++) W[k * 3 + j] += Cx[up10(k, 6 + j)];
-        if (SHARED)
-          for (int j = 0; j < 3; j++) W[18 + j] += Cx[up10(6 + j, 9)];
-      }
-    } else {
-      for (int k = 0; k < 21; k++) W[k] = 0;
-    }
-    if (stores_w(g) && q.Wc) {
-      if (sizeof(WT) == 4 || (!EXACT && q.factored64)) {
-        double fw[11];
-        cross_factors(L, ok, SHARED, fw);
-        for (int k = 0; k < kf_words(SHARED); k++) *fptr<SHARED, WT>(q.Wc, k, n) = (WT)fw[k];
-      } else {
-        for (uint32_t k = 0; k < K; k++) *wptr<WT, SHARED>(q.Wc, k, n, q.M) = (WT)W[k];
-      }
-    }
-    if (!ok) continue;
-    const double w = L.w;
-    // with a right-hand side given, the camera's own b_c enters once, in k_camera_finish
-    const double rw = (EXACT && q.rhs_in) ? 0.0 : 1.0;
-    if (var) {
-      const double* pb = q.pblk + (size_t)p * 12;
-      double Vi[9], bpv[3];
-      for (int k = 0; k < 9; k++) Vi[k] = pb[k];
-      for (int k = 0; k < 3; k++) bpv[k] = pb[9 + k];
-      double T[18];
-#pragma unroll
-      for (int j = 0; j < 6; j++)
-#pragma unroll
-        for (int l = 0; l < 3; l++)
-          T[j * 3 + l] = W[j * 3] * Vi[l] + W[j * 3 + 1] * Vi[3 + l] + W[j * 3 + 2] * Vi[6 + l];
-      if (q.camG) {
-        // the direct route's factor of this slot's share of W V^-1 W^T
-        double Lf[6];
-        lower_cholesky3(Vi, Lf);
-        double* G = q.camG + (size_t)n * 18;
+    for (int k = 0; k < N1; k++) acc[k] = 0;
+    for (uint32_t n = s + threadIdx.x; n < e; n += TASK_THREADS) {
+      const uint32_t i = camera_obs(q)[n], p = q.cam_point[n];
+      const bool dir = directional_point(g, p);
+      Lin L;
+      const bool ok = linearize(q, c, p, i, fscale, var, dir, L);
+      if (!ok) continue;
+      double Cx[EXACT ? 55 : 1];
+      if (EXACT) exact_correction(L, var, dir, q.huber, Cx, !q.gauss_newton);
+      const double w = L.w;
+      if (var) {
 #pragma unroll
         for (int j = 0; j < 6; j++) {
-          G[j * 3] = W[j * 3] * Lf[0] + W[j * 3 + 1] * Lf[1] + W[j * 3 + 2] * Lf[3];
-          G[j * 3 + 1] = W[j * 3 + 1] * Lf[2] + W[j * 3 + 2] * Lf[4];
-          G[j * 3 + 2] = W[j * 3 + 2] * Lf[5];
+          const double wj0 = w * L.Jc[j], wj1 = w * L.Jc[6 + j];
+#pragma unroll
+          for (int l = j; l < 6; l++) {
+            acc[up6(j, l)] += wj0 * L.Jc[l] + wj1 * L.Jc[6 + l];
+            if (EXACT) acc[up6(j, l)] += Cx[up10(j, l)];
+          }
+          if (SHARED) acc[21 + j] += wj0 * L.Jf[0] + wj1 * L.Jf[1];
+          if (SHARED && EXACT) acc[21 + j] += Cx[up10(j, 9)];
         }
       }
-#pragma unroll
-      for (int j = 0; j < 6; j++) {
-        const double wj0 = w * L.Jc[j], wj1 = w * L.Jc[6 + j];
-#pragma unroll
-        for (int l = j; l < 6; l++) {
-          acc[up6(j, l)] += wj0 * L.Jc[l] + wj1 * L.Jc[6 + l];
-          if (EXACT) acc[up6(j, l)] += Cx[up10(j, l)];
-          acc[21 + up6(j, l)] +=
-              T[j * 3] * W[l * 3] + T[j * 3 + 1] * W[l * 3 + 1] + T[j * 3 + 2] * W[l * 3 + 2];
-        }
-        acc[42 + j] += rw * (-(wj0 * L.r[0]) - wj1 * L.r[1]) -
-                       (T[j * 3] * bpv[0] + T[j * 3 + 1] * bpv[1] + T[j * 3 + 2] * bpv[2]);
-        if (SHARED) acc[48 + j] += wj0 * L.Jf[0] + wj1 * L.Jf[1];
-        if (SHARED && EXACT) acc[48 + j] += Cx[up10(j, 9)];
+      if (SHARED) {
+        acc[27] += w * (L.Jf[0] * L.Jf[0] + L.Jf[1] * L.Jf[1]);
+        if (EXACT) acc[27] += Cx[up10(9, 9)];
+        acc[28] += rw * -(w * (L.Jf[0] * L.r[0] + L.Jf[1] * L.r[1]));
       }
     }
-    if (SHARED) {
-      acc[54] += w * (L.Jf[0] * L.Jf[0] + L.Jf[1] * L.Jf[1]);
-      if (EXACT) acc[54] += Cx[up10(9, 9)];
-      acc[55] += rw * -(w * (L.Jf[0] * L.r[0] + L.Jf[1] * L.r[1]));
-    }
+    static constexpr int words[29] = {0,  1,  2,  3,  4,  5,  6,  7,  8,  9,  10, 11, 12, 13, 14,
+                                      15, 16, 17, 18, 19, 20, 48, 49, 50, 51, 52, 53, 54, 55};
+    task_sums<N1>(acc, red, out, words);
   }
+  // pass 2: Q (words 21-41) and the RHS (42-47); the stored words
+  {
+    double acc[N2];
 #pragma unroll
-  for (int k = 0; k < NV; k++)
-    for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_xor(acc[k], o, 64);
-  const int wave = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0)
-    for (int k = 0; k < NV; k++) red[wave][k] = acc[k];
-  __syncthreads();
-  if (threadIdx.x < NV) {
-    double v = 0;
-    for (int w = 0; w < TASK_THREADS / 64; w++) v += red[w][threadIdx.x];
-    q.task_part[(size_t)t * ASM_WORDS + threadIdx.x] = v;
+    for (int k = 0; k < N2; k++) acc[k] = 0;
+    for (uint32_t n = s + threadIdx.x; n < e; n += TASK_THREADS) {
+      const uint32_t i = camera_obs(q)[n], p = q.cam_point[n];
+      const bool dir = directional_point(g, p);
+      Lin L;
+      const bool ok = linearize(q, c, p, i, fscale, var, dir, L);
+      double W[21];
+      if (ok) {
+        cross_block(L, SHARED, W);
+        if (EXACT) {
+          double Cx[55];
+          exact_correction(L, var, dir, q.huber, Cx, !q.gauss_newton);
+          for (int k = 0; k < 6; k++)
+            for (int j = 0; j < 3; j++) W[k * 3 + j] += Cx[up10(k, 6 + j)];
+          if (SHARED)
+            for (int j = 0; j < 3; j++) W[18 + j] += Cx[up10(6 + j, 9)];
+        }
+      } else {
+        for (int k = 0; k < 21; k++) W[k] = 0;
+      }
+      if (stores_w(g) && q.Wc) {
+        if (sizeof(WT) == 4 || (!EXACT && q.factored64)) {
+          double fw[11];
+          cross_factors(L, ok, SHARED, fw);
+          for (int k = 0; k < kf_words(SHARED); k++) *fptr<SHARED, WT>(q.Wc, k, n) = (WT)fw[k];
+        } else {
+          for (uint32_t k = 0; k < K; k++) *wptr<WT, SHARED>(q.Wc, k, n, q.M) = (WT)W[k];
+        }
+      }
+      if (!ok) continue;
+      const double w = L.w;
+      if (var) {
+        const double* pb = q.pblk + (size_t)p * 12;
+        double Vi[9], bpv[3];
+        for (int k = 0; k < 9; k++) Vi[k] = pb[k];
+        for (int k = 0; k < 3; k++) bpv[k] = pb[9 + k];
+        double T[18];
+#pragma unroll
+        for (int j = 0; j < 6; j++)
+#pragma unroll
+          for (int l = 0; l < 3; l++)
+            T[j * 3 + l] = W[j * 3] * Vi[l] + W[j * 3 + 1] * Vi[3 + l] + W[j * 3 + 2] * Vi[6 + l];
+        if (q.camG) {
+          // the direct route's factor of this slot's share of W V^-1 W^T
+          double Lf[6];
+          lower_cholesky3(Vi, Lf);
+          double* G = q.camG + (size_t)n * 18;
+#pragma unroll
+          for (int j = 0; j < 6; j++) {
+            G[j * 3] = W[j * 3] * Lf[0] + W[j * 3 + 1] * Lf[1] + W[j * 3 + 2] * Lf[3];
+            G[j * 3 + 1] = W[j * 3 + 1] * Lf[2] + W[j * 3 + 2] * Lf[4];
+            G[j * 3 + 2] = W[j * 3 + 2] * Lf[5];
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < 6; j++) {
+          const double wj0 = w * L.Jc[j], wj1 = w * L.Jc[6 + j];
+#pragma unroll
+          for (int l = j; l < 6; l++)
+            acc[up6(j, l)] +=
+                T[j * 3] * W[l * 3] + T[j * 3 + 1] * W[l * 3 + 1] + T[j * 3 + 2] * W[l * 3 + 2];
+          acc[21 + j] += rw * (-(wj0 * L.r[0]) - wj1 * L.r[1]) -
+                         (T[j * 3] * bpv[0] + T[j * 3 + 1] * bpv[1] + T[j * 3 + 2] * bpv[2]);
+        }
+      }
+    }
+    static constexpr int words[27] = {21, 22, 23, 24, 25, 26, 27, 28, 29, 30, 31, 32, 33, 34,
+                                      35, 36, 37, 38, 39, 40, 41, 42, 43, 44, 45, 46, 47};
+    task_sums<N2>(acc, red, out, words);
   }
 }
 
