@@ -27,6 +27,7 @@
 #include "ba_onepass.hip"
 #include "ba_dense.hip"
 #include "obs_eval.hip"
+#include "ba_order.hip"
 
 using namespace tcvba;
 
@@ -66,7 +67,7 @@ struct tcv_ba_context {
       tprod,
       obs_cameras, obs_points, obs_pixels, obs_indices, obs_offsets, obs_flags, obs_errors,
       obs_partials, obs_summary, op_tile, op_lo, op_off, op_cam, op_row, op_part, op_code, op_trec,
-      pcg_map, cam_dot;
+      pcg_map, cam_dot, ord_scene, ord_index, ord_key, ord_perm, ord_sort;
   // per bundle (B + 1 prefix counts) its first one-pass workgroup, camera and PCG-step group, and
   // the host side of pcg_map (run_pcg's compacted grids)
   std::vector<uint32_t> bundle_ops, bundle_cams, bundle_groups, h_map;
@@ -96,7 +97,8 @@ struct tcv_ba_context {
            &dense,      &direct_bundles, &direct_cams, &covis, &small_bundles, &tprod,
            &obs_cameras, &obs_points, &obs_pixels, &obs_indices, &obs_offsets, &obs_flags,
            &obs_errors, &obs_partials, &obs_summary, &tile_slot, &op_tile, &op_lo, &op_off, &op_cam, &op_row,
-           &op_part,    &op_code,      &op_trec,   &pcg_map,   &cam_dot};
+           &op_part,    &op_code,      &op_trec,   &pcg_map,   &cam_dot,   &ord_scene,
+           &ord_index,  &ord_key,      &ord_perm,  &ord_sort};
   }
 };
 
@@ -1780,49 +1782,106 @@ std::vector<uint32_t> point_tiles(const uint32_t* ps, uint32_t P) {
   return tiles;
 }
 
-// Whether every one-pass workgroup of a one-bundle problem (OP_TILES consecutive tiles) observes
-// at most OP_SPAN cameras, plain or wrapped round the camera loop, as k_onepass_windows measures:
-// the condition for the one-pass product.
-bool onepass_fits(const uint32_t* obs_camera, const uint32_t* ps, uint32_t P, uint32_t C) {
+// Per point the extremes of its track's cameras, plain and wrapped round the camera loop (a camera
+// k < C/2 counted as k + C), as k_onepass_windows measures a window; lo > hi for a point without
+// observations.
+struct TrackSpan {
+  uint32_t lo = UINT32_MAX, hi = 0, wlo = UINT32_MAX, whi = 0;
+};
+
+// The one-pass rows of the points taken in `order` (given point of each), whose track offsets in
+// that order are `ps`: the sum of the window spans of the one-pass workgroups (OP_TILES consecutive
+// tiles each), as set-up will count them; `fits` says whether every window spans at most OP_SPAN
+// cameras, the condition for the one-pass product.
+uint64_t window_rows(const std::vector<TrackSpan>& ts, const uint32_t* ps,
+                     const std::vector<uint32_t>& order, bool& fits) {
+  const uint32_t P = (uint32_t)ts.size();
   const std::vector<uint32_t> tiles = point_tiles(ps, P);
-  const uint32_t half = C / 2, n_tiles = (uint32_t)tiles.size() - 1;
-  const uint32_t n_op = (n_tiles + OP_TILES - 1) / OP_TILES;
-  std::vector<char> wide(n_op, 0);
-  parallel_for(n_op, [&](size_t w0, size_t w1) {
-    for (size_t w = w0; w < w1; w++) {
-      const uint32_t t0 = (uint32_t)w * OP_TILES, t1 = std::min<uint32_t>(t0 + OP_TILES, n_tiles);
-      uint32_t lo = UINT32_MAX, hi = 0, wlo = UINT32_MAX, whi = 0;
-      for (uint32_t i = ps[tiles[t0]]; i < ps[tiles[t1]]; i++) {
-        const uint32_t cam = obs_camera[i], v = cam < half ? cam + C : cam;
-        lo = std::min(lo, cam);
-        hi = std::max(hi, cam);
-        wlo = std::min(wlo, v);
-        whi = std::max(whi, v);
-      }
-      wide[w] = lo <= hi && std::min(hi - lo, whi - wlo) + 1 > (uint32_t)OP_SPAN;
+  const uint32_t n_tiles = (uint32_t)tiles.size() - 1;
+  uint64_t rows = 0;
+  fits = true;
+  for (uint32_t t0 = 0; t0 < n_tiles; t0 += OP_TILES) {
+    const uint32_t t1 = std::min<uint32_t>(t0 + OP_TILES, n_tiles);
+    TrackSpan w;
+    for (uint32_t j = tiles[t0]; j < tiles[t1]; j++) {
+      const TrackSpan& x = ts[order[j]];
+      w.lo = std::min(w.lo, x.lo);
+      w.hi = std::max(w.hi, x.hi);
+      w.wlo = std::min(w.wlo, x.wlo);
+      w.whi = std::max(w.whi, x.whi);
     }
-  });
-  return std::find(wide.begin(), wide.end(), 1) == wide.end();
+    if (w.lo > w.hi) continue;
+    const uint32_t span = std::min(w.hi - w.lo, w.whi - w.wlo) + 1;
+    rows += span;
+    fits = fits && span <= (uint32_t)OP_SPAN;
+  }
+  return rows;
+}
+
+// The given point of each new one: ascending key (the lowest camera of the track, C for none), a
+// stable counting sort within each group, the directional points (from first_dir on) after the
+// others.
+std::vector<uint32_t> sort_points(const std::vector<uint32_t>& key, uint32_t C,
+                                  uint32_t first_dir) {
+  const uint32_t P = (uint32_t)key.size();
+  std::vector<uint32_t> order(P), count(C + 2);
+  for (const auto& [g0, g1] : {std::pair<uint32_t, uint32_t>{0, first_dir}, {first_dir, P}}) {
+    std::fill(count.begin(), count.end(), 0u);
+    for (uint32_t p = g0; p < g1; p++) count[key[p] + 1]++;
+    for (uint32_t k = 0; k <= C; k++) count[k + 1] += count[k];
+    for (uint32_t p = g0; p < g1; p++) order[g0 + count[key[p]]++] = p;
+  }
+  return order;
+}
+
+// Whether the solve entries consider another order for the points of a one-bundle problem: more
+// cameras than one window holds, and a solve that runs the one-pass product (a bundle the direct
+// route takes runs it only for PCG hand-overs, so its points stay as given).
+bool may_order(const tcv_ba_control* c) {
+  if (c->cameras <= (uint32_t)OP_SPAN || c->points == 0 || c->observations == 0) return false;
+  return (c->flags & TCV_BA_FLAG_DIRECT) == 0 || c->cameras > DENSE_LARGE_CAMERAS;
+}
+
+// The order the solve entries give the points of one problem (tracks `ts`, given offsets `ps`):
+// by the lowest camera of their tracks (sort_points), taken when its one-pass windows fit and the
+// given order's do not, or take at least an eighth more rows (partial rows written per product
+// and summed per camera). True with `order` (the given point of each new one) and `nps` (the new
+// track offsets) filled; false to keep the given order.
+bool choose_order(const std::vector<TrackSpan>& ts, const uint32_t* ps, uint32_t C,
+                  uint32_t first_dir, std::vector<uint32_t>& order, std::vector<uint32_t>& nps) {
+  const uint32_t P = (uint32_t)ts.size();
+  std::vector<uint32_t> key(P);
+  for (uint32_t p = 0; p < P; p++) key[p] = std::min(ts[p].lo, C);
+  order = sort_points(key, C, first_dir);
+  nps.assign(P + 1, 0);
+  for (uint32_t j = 0; j < P; j++) nps[j + 1] = nps[j] + (ps[order[j] + 1] - ps[order[j]]);
+  std::vector<uint32_t> given(P);
+  for (uint32_t p = 0; p < P; p++) given[p] = p;
+  bool fits = false, nfits = false;
+  const uint64_t rows = window_rows(ts, ps, given, fits);
+  const uint64_t nrows = window_rows(ts, nps.data(), order, nfits);
+  return nfits && (!fits || 8 * nrows <= 7 * rows);
 }
 
 // Host arrays of one problem whose points are not in an order the one-pass product can take
 // (points listed as their tracks were created, say, rather than by the cameras that see them) would
 // run the two sweeps: on the 2,000-camera trajectory scene with its points shuffled, 495 us per
-// FP64 product against 189 us. When the given order does not fit (onepass_fits), the points are
-// ordered by the lowest camera of their tracks (a stable counting sort; the directional points
-// stay after the others, and a point without observations goes last in its group), their
-// observations follow them in their given order and each camera's list is rebuilt ascending; if
-// that order fits, `oscene` / `oindex` receive the reordered arrays and `perm` the given point of
-// each new one, and the result is true. Nothing changes (false, `perm` empty) when the given order
-// fits, when the index is not one the engine accepts (its own checks then report it as before),
-// or when the new order would not fit either. The reordered problem is the same problem: only sums
-// over points change their order.
+// FP64 product against 189 us; points in an order that fits but spreads each workgroup's tracks
+// over more cameras write more partial rows. So the points are ordered by the lowest camera of
+// their tracks (a stable counting sort; the directional points stay after the others, and a point
+// without observations goes last in its group) when choose_order takes that order, their
+// observations follow them in their given order and each camera's list is rebuilt ascending:
+// `oscene` / `oindex` receive the reordered arrays and `perm` the given point of each new one, and
+// the result is true. Nothing changes (false, `perm` empty) when may_order or choose_order keeps
+// the given order or when the index is not one the engine accepts (its own checks then report it
+// as before). The
+// reordered problem is the same problem: only sums over points change their order.
 bool order_points(const tcv_ba_control* c, const double* scene, const uint32_t* index,
                   std::vector<double>& oscene, std::vector<uint32_t>& oindex,
                   std::vector<uint32_t>& perm) {
   perm.clear();
   const uint32_t C = c->cameras, P = c->points, M = c->observations;
-  if (C <= (uint32_t)OP_SPAN || P == 0) return false;  // every window fits
+  if (!may_order(c)) return false;
   const uint32_t *ocam = index, *opt = index + M, *ps = index + 2 * (size_t)M,
                  *cs = ps + P + 1, *co = cs + C + 1, *act = co + M;
   // the offsets' checks of build_plan and those of k_check_points / k_check_tasks: observations
@@ -1843,28 +1902,27 @@ bool order_points(const tcv_ba_control* c, const double* scene, const uint32_t* 
       for (uint32_t n = cs[k]; n < cs[k + 1]; n++)
         if (co[n] >= M || ocam[co[n]] != k || (n > cs[k] && co[n - 1] >= co[n])) bad[1] = 1;
   });
-  if (bad[0] || bad[1] || onepass_fits(ocam, ps, P, C)) return false;
-  // keys: the lowest camera of each track (C for an empty one); a counting sort per group
-  std::vector<uint32_t> key(P, C);
+  if (bad[0] || bad[1]) return false;
+  std::vector<TrackSpan> ts(P);
+  const uint32_t half = C / 2;
   parallel_for(P, [&](size_t p0, size_t p1) {
     for (size_t p = p0; p < p1; p++)
-      for (uint32_t i = ps[p]; i < ps[p + 1]; i++) key[p] = std::min(key[p], ocam[i]);
+      for (uint32_t i = ps[p]; i < ps[p + 1]; i++) {
+        const uint32_t cam = ocam[i], v = cam < half ? cam + C : cam;
+        ts[p].lo = std::min(ts[p].lo, cam);
+        ts[p].hi = std::max(ts[p].hi, cam);
+        ts[p].wlo = std::min(ts[p].wlo, v);
+        ts[p].whi = std::max(ts[p].whi, v);
+      }
   });
-  const uint32_t first_dir = P - c->directional_points;
-  std::vector<uint32_t> order(P), count(C + 2);
-  for (const auto& [g0, g1] : {std::pair<uint32_t, uint32_t>{0, first_dir}, {first_dir, P}}) {
-    std::fill(count.begin(), count.end(), 0u);
-    for (uint32_t p = g0; p < g1; p++) count[key[p] + 1]++;
-    for (uint32_t k = 0; k <= C; k++) count[k + 1] += count[k];
-    for (uint32_t p = g0; p < g1; p++) order[g0 + count[key[p]]++] = p;
-  }
+  std::vector<uint32_t> order, hnps;
+  if (!choose_order(ts, ps, C, P - c->directional_points, order, hnps)) return false;
   // the reordered index: observations in the new point order, then each camera's list, its new
   // observations ascending
   oindex.assign(tcv_ba_index_size(c), 0);
   uint32_t *ncam = oindex.data(), *npt = ncam + M, *nps = npt + M, *ncs = nps + P + 1,
            *nco = ncs + C + 1, *nact = nco + M;
-  nps[0] = 0;
-  for (uint32_t j = 0; j < P; j++) nps[j + 1] = nps[j] + (ps[order[j] + 1] - ps[order[j]]);
+  std::memcpy(nps, hnps.data(), ((size_t)P + 1) * 4);
   parallel_for(P, [&](size_t j0, size_t j1) {
     for (size_t j = j0; j < j1; j++)
       for (uint32_t i = ps[order[j]], n = nps[j]; i < ps[order[j] + 1]; i++, n++) {
@@ -1872,10 +1930,6 @@ bool order_points(const tcv_ba_control* c, const double* scene, const uint32_t* 
         npt[n] = (uint32_t)j;
       }
   });
-  if (!onepass_fits(ncam, nps, P, C)) {
-    oindex.clear();
-    return false;
-  }
   std::memcpy(ncs, cs, ((size_t)C + 1) * 4);
   std::vector<uint32_t> fill(cs, cs + C);
   for (uint32_t n = 0; n < M; n++) nco[fill[ncam[n]]++] = n;
@@ -1897,6 +1951,130 @@ bool order_points(const tcv_ba_control* c, const double* scene, const uint32_t* 
   });
   perm = std::move(order);
   return true;
+}
+
+// The bits of a camera index below C (the radix sort's key width).
+int camera_bits(uint32_t C) {
+  int bits = 1;
+  while (bits < 32 && (C >> bits) != 0) bits++;
+  return bits;
+}
+
+// order_points for device arrays: the same choice of order (choose_order), from the tracks' camera
+// extremes measured on the device; when it takes the new order, the reordered scene goes into
+// `nscene` and the index into `nindex`, ctx->ord_perm keeps the given point of each new one, and
+// `moved` is set (unpermute_points puts the solved points back). Nothing changes when the given
+// order is kept (may_order, choose_order) or when the index is not one the engine accepts (its
+// checks then report it as before).
+int order_device(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene,
+                 const uint32_t* index, hipStream_t st, DevBuf& nscene_buf, DevBuf& nindex_buf,
+                 bool& moved) {
+  moved = false;
+  const uint32_t C = c->cameras, P = c->points, M = c->observations;
+  if (!may_order(c)) return TCV_BA_OK;
+  const uint32_t *ocam = index, *opt = index + M, *ps = index + 2 * (size_t)M,
+                 *cs = ps + P + 1, *co = cs + C + 1, *act = co + M;
+  std::vector<uint32_t> hps(P + 1), hcs(C + 1);
+  HIP_TRY(hipMemcpyAsync(hps.data(), ps, (P + 1) * 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(hcs.data(), cs, (C + 1) * 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if (hps[0] != 0 || hps[P] != M || hcs[0] != 0 || hcs[C] != M) return TCV_BA_OK;
+  for (uint32_t p = 0; p < P; p++)
+    if (hps[p + 1] < hps[p]) return TCV_BA_OK;
+  for (uint32_t k = 0; k < C; k++)
+    if (hcs[k + 1] < hcs[k]) return TCV_BA_OK;
+  // the tracks' camera extremes and the checks of the words the reordering reads
+  static_assert(sizeof(TrackSpan) == 16, "four words per point");
+  TRY(ensure(ctx->ord_key, (size_t)P * 16 + 16));
+  uint32_t* span = (uint32_t*)ctx->ord_key.ptr;
+  uint32_t* flag = span + 4 * (size_t)P;
+  HIP_TRY(hipMemsetAsync(flag, 0, 4, st));
+  hipLaunchKernelGGL(k_order_spans, dim3(blocks_for(P, ORDER_THREADS)), dim3(ORDER_THREADS), 0, st,
+                     ocam, opt, ps, P, C, span, flag);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_order_lists, dim3(C), dim3(ORDER_THREADS), 0, st, ocam, cs, co, M, flag);
+  HIP_TRY(hipGetLastError());
+  std::vector<TrackSpan> ts(P);
+  uint32_t bad = 0;
+  HIP_TRY(hipMemcpyAsync(ts.data(), span, (size_t)P * 16, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(&bad, flag, 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if (bad) return TCV_BA_OK;
+  std::vector<uint32_t> order, nps;
+  if (!choose_order(ts, hps.data(), C, P - c->directional_points, order, nps)) return TCV_BA_OK;
+  // the reordered index and scene
+  const size_t ss = tcv_ba_scene_size(c), is = tcv_ba_index_size(c);
+  TRY(upload(ctx->ord_perm, order, st));
+  TRY(ensure(nscene_buf, ss * 8));
+  TRY(ensure(nindex_buf, is * 4));
+  double* nscene = (double*)nscene_buf.ptr;
+  uint32_t* nidx = (uint32_t*)nindex_buf.ptr;
+  uint32_t *ncam = nidx, *npt = ncam + M, *nps_d = npt + M, *ncs = nps_d + P + 1,
+           *nco = ncs + C + 1, *nact = nco + M;
+  HIP_TRY(hipMemcpyAsync(nps_d, nps.data(), ((size_t)P + 1) * 4, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(ncs, cs, ((size_t)C + 1) * 4, hipMemcpyDeviceToDevice, st));
+  HIP_TRY(hipMemcpyAsync(nact, act, (size_t)C * 4, hipMemcpyDeviceToDevice, st));
+  const size_t o_pt = (size_t)12 * C, o_model = o_pt + 3 * (size_t)P;
+  const size_t o_pix = o_model + 6 * (size_t)C;
+  HIP_TRY(hipMemcpyAsync(nscene, scene, o_pt * 8, hipMemcpyDeviceToDevice, st));
+  HIP_TRY(hipMemcpyAsync(nscene + o_model, scene + o_model, (size_t)C * 6 * 8,
+                         hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(k_order_gather, dim3(blocks_for(P, ORDER_THREADS)), dim3(ORDER_THREADS), 0,
+                     st, (const uint32_t*)ctx->ord_perm.ptr, ps, nps_d, ocam, scene + o_pt,
+                     scene + o_pix, P, ncam, npt, nscene + o_pt, nscene + o_pix);
+  HIP_TRY(hipGetLastError());
+  // each camera's list: its new observations ascending, a stable sort of the observations by
+  // camera
+  const int bits = camera_bits(C);
+  size_t temp = 0;
+  HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (const uint32_t*)nullptr,
+                                             (uint32_t*)nullptr, (const uint32_t*)nullptr,
+                                             (uint32_t*)nullptr, (int)M, 0, bits, st));
+  // the sort's temporary storage 256-byte aligned after the ids and sorted cameras
+  const size_t words = ((2 * (size_t)M + 63) / 64) * 64;
+  TRY(ensure(ctx->ord_sort, words * 4 + temp));
+  uint32_t* ids = (uint32_t*)ctx->ord_sort.ptr;
+  uint32_t* sorted_cam = ids + M;
+  hipLaunchKernelGGL(k_order_iota, dim3(blocks_for(M, ORDER_THREADS)), dim3(ORDER_THREADS), 0, st,
+                     ids, M);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipcub::DeviceRadixSort::SortPairs((void*)(ids + words), temp, (const uint32_t*)ncam,
+                                             sorted_cam, (const uint32_t*)ids, nco, (int)M, 0,
+                                             bits, st));
+  moved = true;
+  return TCV_BA_OK;
+}
+
+// The points of a solve of reordered arrays back in the given order: `out` (device) holds the
+// solve's output, `staging` (device, 3P doubles) takes its points while k_order_scatter writes
+// each back to its given index (ctx->ord_perm).
+int unpermute_points(tcv_ba_context* ctx, const tcv_ba_control* c, double* out, double* staging,
+                     hipStream_t st) {
+  const uint32_t P = c->points;
+  double* pts = out + (size_t)12 * c->cameras;
+  HIP_TRY(hipMemcpyAsync(staging, pts, (size_t)P * 24, hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(k_order_scatter, dim3(blocks_for(P, ORDER_THREADS)), dim3(ORDER_THREADS), 0,
+                     st, (const uint32_t*)ctx->ord_perm.ptr, (const double*)staging, P, pts);
+  HIP_TRY(hipGetLastError());
+  return TCV_BA_OK;
+}
+
+// The device bytes order_device may hold beyond the solve's for one problem: the reordered scene
+// and index, the tracks' extremes, the permutation and the radix sort's arrays.
+int order_bytes(const tcv_ba_control* c, size_t& scene, size_t& index, size_t& key, size_t& perm,
+                size_t& sort) {
+  const uint32_t M = c->observations;
+  size_t temp = 0;
+  HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (const uint32_t*)nullptr,
+                                             (uint32_t*)nullptr, (const uint32_t*)nullptr,
+                                             (uint32_t*)nullptr, (int)M, 0, camera_bits(c->cameras),
+                                             (hipStream_t)0));
+  scene = tcv_ba_scene_size(c) * 8;
+  index = tcv_ba_index_size(c) * 4;
+  key = (size_t)c->points * 16 + 16;
+  perm = (size_t)c->points * 4;
+  sort = ((2 * (size_t)M + 63) / 64) * 64 * 4 + temp;
+  return TCV_BA_OK;
 }
 
 }  // namespace
@@ -1998,7 +2176,20 @@ static int body_ba_solve_device(tcv_ba_context* ctx, const tcv_ba_control* c, co
   if (!ctx || !c || !scene || !index || !out) return TCV_BA_ERR_ARGUMENT;
   if (rep) std::memset(rep, 0, sizeof *rep);
   hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
-  return solve_device(ctx, c, 1, scene, index, out, rep, nullptr, st);
+  TRY(check_control(c));
+  HIP_TRY(hipSetDevice(ctx->device));
+  // points in the order the one-pass product takes best (order_device); the output's points go back
+  // to the given order. Under a working budget the given order is kept: set-up admits the solve's
+  // own arrays only.
+  bool moved = false;
+  if (!ctx->budget) TRY(order_device(ctx, c, scene, index, st, ctx->probe, ctx->index, moved));
+  if (!moved) return solve_device(ctx, c, 1, scene, index, out, rep, nullptr, st);
+  TRY(solve_device(ctx, c, 1, (const double*)ctx->probe.ptr, (const uint32_t*)ctx->index.ptr, out,
+                   rep, nullptr, st));
+  // the solved points staged in the reordered scene's point words
+  TRY(unpermute_points(ctx, c, out, (double*)ctx->probe.ptr + (size_t)12 * c->cameras, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  return TCV_BA_OK;
 }
 
 static int body_ba_solve(tcv_ba_context* ctx, const tcv_ba_control* c, const double* scene,
@@ -2008,25 +2199,22 @@ static int body_ba_solve(tcv_ba_context* ctx, const tcv_ba_control* c, const dou
   TRY(check_control(c));
   HIP_TRY(hipSetDevice(ctx->device));
   const size_t ss = tcv_ba_scene_size(c), is = tcv_ba_index_size(c), os = tcv_ba_output_size(c);
-  // points in an order the one-pass product can take (order_points); the output's points go back
-  // to the given order
-  std::vector<double> oscene;
-  std::vector<uint32_t> oindex, perm;
-  if (order_points(c, scene, index, oscene, oindex, perm)) {
-    scene = oscene.data();
-    index = oindex.data();
-  } else {
-    perm.clear();
-  }
-  // the residency plan under the working budget, with the staging of the host arrays, before
-  // anything is allocated
+  // the residency plan under the working budget, with the staging of the host arrays (and, when
+  // the points may be reordered, of the reordered arrays), before anything is allocated
   int residency = RESIDENT;
   if (ctx->budget) {
     Plan plan;
     bool streamed = false;
+    std::vector<Need> staging = {{&ctx->probe, ss * 8}, {&ctx->index, is * 4}, {&ctx->out, os * 8}};
+    if (may_order(c)) {
+      size_t b_scene, b_index, b_key, b_perm, b_sort;
+      TRY(order_bytes(c, b_scene, b_index, b_key, b_perm, b_sort));
+      staging.insert(staging.end(), {{&ctx->ord_scene, b_scene}, {&ctx->ord_index, b_index},
+                                     {&ctx->ord_key, b_key}, {&ctx->ord_perm, b_perm},
+                                     {&ctx->ord_sort, b_sort}});
+    }
     TRY(host_plan(c, 1, index, c->cameras, c->points, c->observations, false, plan));
-    TRY(admit(ctx, c, 1, c->cameras, c->points, c->observations, plan, false,
-              {{&ctx->probe, ss * 8}, {&ctx->index, is * 4}, {&ctx->out, os * 8}}, streamed));
+    TRY(admit(ctx, c, 1, c->cameras, c->points, c->observations, plan, false, staging, streamed));
     residency = streamed ? STREAMED : RESIDENT;
   }
   TRY(ensure(ctx->probe, ss * 8));  // staging of the host scene
@@ -2035,16 +2223,20 @@ static int body_ba_solve(tcv_ba_context* ctx, const tcv_ba_control* c, const dou
   hipStream_t st = ctx->stream;
   HIP_TRY(hipMemcpyAsync(ctx->probe.ptr, scene, ss * 8, hipMemcpyHostToDevice, st));
   HIP_TRY(hipMemcpyAsync(ctx->index.ptr, index, is * 4, hipMemcpyHostToDevice, st));
-  TRY(solve_device(ctx, c, 1, (const double*)ctx->probe.ptr, (const uint32_t*)ctx->index.ptr,
+  // points in the order the one-pass product takes best (order_device, into the context's
+  // reordered arrays); the output's points go back to the given order
+  bool moved = false;
+  TRY(order_device(ctx, c, (const double*)ctx->probe.ptr, (const uint32_t*)ctx->index.ptr, st,
+                   ctx->ord_scene, ctx->ord_index, moved));
+  const DevBuf& s_scene = moved ? ctx->ord_scene : ctx->probe;
+  const DevBuf& s_index = moved ? ctx->ord_index : ctx->index;
+  TRY(solve_device(ctx, c, 1, (const double*)s_scene.ptr, (const uint32_t*)s_index.ptr,
                    (double*)ctx->out.ptr, rep, nullptr, st, residency));
+  if (moved)
+    TRY(unpermute_points(ctx, c, (double*)ctx->out.ptr,
+                         (double*)ctx->ord_scene.ptr + (size_t)12 * c->cameras, st));
   HIP_TRY(hipMemcpyAsync(out, ctx->out.ptr, os * 8, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
-  if (!perm.empty()) {
-    double* pts = out + (size_t)12 * c->cameras;
-    const std::vector<double> solved(pts, pts + 3 * (size_t)c->points);
-    for (uint32_t j = 0; j < c->points; j++)
-      for (int k = 0; k < 3; k++) pts[3 * (size_t)perm[j] + k] = solved[3 * (size_t)j + k];
-  }
   if (rep) rep->device_bytes = held_bytes(ctx);
   return TCV_BA_OK;
 }
