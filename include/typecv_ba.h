@@ -194,20 +194,23 @@ int tcv_ba_fail_reads(tcv_ba_context* context, uint32_t n);
 /*
  * Solve from host arrays: upload, run the LM loop, read back `out`. The call that the Node-API
  * addon runs as async work for `bundleAdjustGpu` (reference src/sfm/bundle/gpu/bundle.ts:803-839:
- * everything inside `device.enqueue`, bundle.ts:178-799 and joint-focal.ts equivalents). When the
- * given point order keeps some 32 consecutive tiles of points from fitting a window of 64 cameras
- * (the one-pass Schur product's condition), the points are solved in the order of the lowest
- * camera of their tracks and `out` holds them back in the given order; the results differ from
- * the given order's only by the order of sums over points. tcv_ba_solve_batch does the same per
- * problem; tcv_ba_solve_device takes the points in the given order.
+ * everything inside `device.enqueue`, bundle.ts:178-799 and joint-focal.ts equivalents). Points
+ * may be solved in another order than the given one: the order of the lowest camera of their
+ * tracks, taken when every 32 consecutive tiles of points then fit a window of 64 cameras (the
+ * one-pass Schur product's condition) and the given order's do not, or write at least an eighth
+ * more partial rows; not for a problem the direct route takes (TCV_BA_FLAG_DIRECT, at most 2,048
+ * cameras). `out` holds the points back in the given order; the results differ from the given
+ * order's only by the order of sums over points. tcv_ba_solve_batch and tcv_ba_solve_device do
+ * the same per problem (the device entry on the device, not under a working budget).
  */
 int tcv_ba_solve(tcv_ba_context* context, const tcv_ba_control* control, const double* scene,
                  const uint32_t* index, double* out, tcv_ba_report* report);
 
 /*
  * Solve from device arrays already resident in HBM (`scene`, `index`, `out` are device pointers of
- * the context's device). `scene` is read, never written. Work runs on `stream` (a hipStream_t, or
- * NULL for the context's own stream) and the call returns when `out` is complete.
+ * the context's device). `scene` and `index` are read, never written; a reordering of the points
+ * (tcv_ba_solve) works on copies in the context's buffers. Work runs on `stream` (a hipStream_t,
+ * or NULL for the context's own stream) and the call returns when `out` is complete.
  */
 int tcv_ba_solve_device(tcv_ba_context* context, const tcv_ba_control* control,
                         const double* scene, const uint32_t* index, double* out,
