@@ -196,7 +196,7 @@ int tcv_ba_fail_reads(tcv_ba_context* context, uint32_t n);
  * addon runs as async work for `bundleAdjustGpu` (reference src/sfm/bundle/gpu/bundle.ts:803-839:
  * everything inside `device.enqueue`, bundle.ts:178-799 and joint-focal.ts equivalents). Points
  * may be solved in another order than the given one: the order of the lowest camera of their
- * tracks, taken when every 32 consecutive tiles of points then fit a window of 64 cameras (the
+ * tracks, then their highest, taken when every 32 consecutive tiles of points then fit a window of 64 cameras (the
  * one-pass Schur product's condition) and the given order's do not, or write at least an eighth
  * more partial rows; not for a problem the direct route takes (TCV_BA_FLAG_DIRECT, at most 2,048
  * cameras). `out` holds the points back in the given order; the results differ from the given

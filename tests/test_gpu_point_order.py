@@ -1,16 +1,17 @@
 """Points given in an order the one-pass Schur product cannot take, on the GPU.
 
-tcv_ba_solve orders the points of host arrays by the lowest camera of their tracks when their given
-order would leave some one-pass workgroup's camera window wider than 64 cameras, solves, and puts
-the output points back in the given order. The scene here is a 150-camera trajectory with its
-points shuffled:
+tcv_ba_solve orders the points of host arrays by the lowest camera of their tracks, then by their
+highest, when their given order would leave some one-pass workgroup's camera window wider than 64
+cameras, solves, and puts the output points back in the given order. The scene here is a
+150-camera trajectory with its points shuffled:
 
 * the solve runs the one-pass product (report.onepass_rows > 0) and matches the oracle on the same
   shuffled input at tests/test_gpu_parity.py's bars (cost 1e-9 relative, coordinates
   1e-7·max(1, |x|), accept/reject/failure/input-error words equal);
 * it equals, bit for bit, the solve of the same points given in the engine's own order (a stable
-  sort by lowest camera, directional points after the others) with the output points permuted back:
-  the reordering is the whole difference, and putting the points back loses nothing.
+  sort by lowest camera, then highest camera, directional points after the others) with the output
+  points permuted back: the reordering is the whole difference, and putting the points back loses
+  nothing.
 
 Both entries also take that order when the given one fits but writes at least an eighth more
 one-pass partial rows (the trajectory generator's order, by the camera that created each point):
@@ -49,15 +50,16 @@ def scene():
 
 
 def engine_order(packed, directional=0):
-    """The order tcv_ba_solve gives the points: stable by the lowest camera of each track, the
-    directional points (the last `directional`) after the others."""
+    """The order tcv_ba_solve gives the points: stable by the lowest camera of each track, then by
+    its highest camera, the directional points (the last `directional`) after the others."""
     ps = packed.section("pointStart").astype(np.int64)
     cams = packed.section("observationCamera").astype(np.int64)
-    key = np.array([cams[ps[p]:ps[p + 1]].min() if ps[p + 1] > ps[p] else packed.C
-                    for p in range(packed.P)])
-    first_dir = packed.P - directional
-    return np.concatenate([np.argsort(key[:first_dir], kind="stable"),
-                           first_dir + np.argsort(key[first_dir:], kind="stable")])
+    lo = np.array([cams[ps[p]:ps[p + 1]].min() if ps[p + 1] > ps[p] else packed.C
+                   for p in range(packed.P)])
+    hi = np.array([cams[ps[p]:ps[p + 1]].max() if ps[p + 1] > ps[p] else packed.C
+                   for p in range(packed.P)])
+    d = packed.P - directional
+    return np.concatenate([np.lexsort((hi[:d], lo[:d])), d + np.lexsort((hi[d:], lo[d:]))])
 
 
 def reordered(packed, perm):

@@ -1798,38 +1798,49 @@ uint64_t window_rows(const std::vector<TrackSpan>& ts, const uint32_t* ps,
   const uint32_t P = (uint32_t)ts.size();
   const std::vector<uint32_t> tiles = point_tiles(ps, P);
   const uint32_t n_tiles = (uint32_t)tiles.size() - 1;
+  const uint32_t n_op = (n_tiles + OP_TILES - 1) / OP_TILES;
+  std::vector<uint32_t> span(n_op, 0);
+  parallel_for(n_op, [&](size_t w0, size_t w1) {
+    for (size_t w = w0; w < w1; w++) {
+      const uint32_t t0 = (uint32_t)w * OP_TILES, t1 = std::min<uint32_t>(t0 + OP_TILES, n_tiles);
+      TrackSpan x;
+      for (uint32_t j = tiles[t0]; j < tiles[t1]; j++) {
+        const TrackSpan& y = ts[order[j]];
+        x.lo = std::min(x.lo, y.lo);
+        x.hi = std::max(x.hi, y.hi);
+        x.wlo = std::min(x.wlo, y.wlo);
+        x.whi = std::max(x.whi, y.whi);
+      }
+      if (x.lo <= x.hi) span[w] = std::min(x.hi - x.lo, x.whi - x.wlo) + 1;
+    }
+  });
   uint64_t rows = 0;
   fits = true;
-  for (uint32_t t0 = 0; t0 < n_tiles; t0 += OP_TILES) {
-    const uint32_t t1 = std::min<uint32_t>(t0 + OP_TILES, n_tiles);
-    TrackSpan w;
-    for (uint32_t j = tiles[t0]; j < tiles[t1]; j++) {
-      const TrackSpan& x = ts[order[j]];
-      w.lo = std::min(w.lo, x.lo);
-      w.hi = std::max(w.hi, x.hi);
-      w.wlo = std::min(w.wlo, x.wlo);
-      w.whi = std::max(w.whi, x.whi);
-    }
-    if (w.lo > w.hi) continue;
-    const uint32_t span = std::min(w.hi - w.lo, w.whi - w.wlo) + 1;
-    rows += span;
-    fits = fits && span <= (uint32_t)OP_SPAN;
+  for (uint32_t v : span) {
+    rows += v;
+    fits = fits && v <= (uint32_t)OP_SPAN;
   }
   return rows;
 }
 
-// The given point of each new one: ascending key (the lowest camera of the track, C for none), a
-// stable counting sort within each group, the directional points (from first_dir on) after the
-// others.
-std::vector<uint32_t> sort_points(const std::vector<uint32_t>& key, uint32_t C,
+// The given point of each new one: ascending lowest camera of the track, then ascending highest
+// camera (C and C for a track without observations), by two stable counting sorts within each
+// group, the directional points (from first_dir on) after the others.
+std::vector<uint32_t> sort_points(const std::vector<TrackSpan>& ts, uint32_t C,
                                   uint32_t first_dir) {
-  const uint32_t P = (uint32_t)key.size();
-  std::vector<uint32_t> order(P), count(C + 2);
+  const uint32_t P = (uint32_t)ts.size();
+  auto lo = [&](uint32_t p) { return std::min(ts[p].lo, C); };
+  auto hi = [&](uint32_t p) { return ts[p].lo > ts[p].hi ? C : ts[p].hi; };
+  std::vector<uint32_t> order(P), by_hi(P), count(C + 2);
   for (const auto& [g0, g1] : {std::pair<uint32_t, uint32_t>{0, first_dir}, {first_dir, P}}) {
     std::fill(count.begin(), count.end(), 0u);
-    for (uint32_t p = g0; p < g1; p++) count[key[p] + 1]++;
+    for (uint32_t p = g0; p < g1; p++) count[hi(p) + 1]++;
     for (uint32_t k = 0; k <= C; k++) count[k + 1] += count[k];
-    for (uint32_t p = g0; p < g1; p++) order[g0 + count[key[p]]++] = p;
+    for (uint32_t p = g0; p < g1; p++) by_hi[g0 + count[hi(p)]++] = p;
+    std::fill(count.begin(), count.end(), 0u);
+    for (uint32_t p = g0; p < g1; p++) count[lo(p) + 1]++;
+    for (uint32_t k = 0; k <= C; k++) count[k + 1] += count[k];
+    for (uint32_t j = g0; j < g1; j++) order[g0 + count[lo(by_hi[j])]++] = by_hi[j];
   }
   return order;
 }
@@ -1843,16 +1854,14 @@ bool may_order(const tcv_ba_control* c) {
 }
 
 // The order the solve entries give the points of one problem (tracks `ts`, given offsets `ps`):
-// by the lowest camera of their tracks (sort_points), taken when its one-pass windows fit and the
-// given order's do not, or take at least an eighth more rows (partial rows written per product
-// and summed per camera). True with `order` (the given point of each new one) and `nps` (the new
+// by the lowest, then the highest camera of their tracks (sort_points), taken when its one-pass
+// windows fit and the given order's do not, or take at least an eighth more rows (partial rows
+// written per product and summed per camera). True with `order` (the given point of each new one) and `nps` (the new
 // track offsets) filled; false to keep the given order.
 bool choose_order(const std::vector<TrackSpan>& ts, const uint32_t* ps, uint32_t C,
                   uint32_t first_dir, std::vector<uint32_t>& order, std::vector<uint32_t>& nps) {
   const uint32_t P = (uint32_t)ts.size();
-  std::vector<uint32_t> key(P);
-  for (uint32_t p = 0; p < P; p++) key[p] = std::min(ts[p].lo, C);
-  order = sort_points(key, C, first_dir);
+  order = sort_points(ts, C, first_dir);
   nps.assign(P + 1, 0);
   for (uint32_t j = 0; j < P; j++) nps[j + 1] = nps[j] + (ps[order[j] + 1] - ps[order[j]]);
   std::vector<uint32_t> given(P);
@@ -1867,9 +1876,9 @@ bool choose_order(const std::vector<TrackSpan>& ts, const uint32_t* ps, uint32_t
 // (points listed as their tracks were created, say, rather than by the cameras that see them) would
 // run the two sweeps: on the 2,000-camera trajectory scene with its points shuffled, 495 us per
 // FP64 product against 189 us; points in an order that fits but spreads each workgroup's tracks
-// over more cameras write more partial rows. So the points are ordered by the lowest camera of
-// their tracks (a stable counting sort; the directional points stay after the others, and a point
-// without observations goes last in its group) when choose_order takes that order, their
+// over more cameras write more partial rows. So the points are ordered by the lowest, then the
+// highest camera of their tracks (sort_points; the directional points stay after the others, and
+// a point without observations goes last in its group) when choose_order takes that order, their
 // observations follow them in their given order and each camera's list is rebuilt ascending:
 // `oscene` / `oindex` receive the reordered arrays and `perm` the given point of each new one, and
 // the result is true. Nothing changes (false, `perm` empty) when may_order or choose_order keeps

@@ -1,14 +1,13 @@
-// ba_order.hip — the device entry's point ordering, included by ba_engine.hip.
+// ba_order.hip — the solve entries' point ordering on the device, included by ba_engine.hip.
 //
 // The one-pass Schur product needs each workgroup's OP_TILES consecutive point tiles to observe at
-// most OP_SPAN cameras, which holds when points come ordered by the cameras that see them. Device
-// arrays are put in the order the host-array entries give theirs (the lowest camera of each track,
-// stable, directional points after the others) when that order fits where the given one does not
-// or writes fewer partial rows: these kernels measure each track's cameras, check the index words
-// the reordering relies on, gather the reordered scene and index, and put the solved points back
-// in the given order. The choice and the sort of the keys run on the host (a counting sort, as for
-// host arrays); each camera's list of the reordered observations is a stable radix sort of their
-// cameras on the device.
+// most OP_SPAN cameras, which holds when points come ordered by the cameras that see them. The
+// points are put in the lowest, then the highest camera of each track (stable, directional points
+// after the others) when that order fits where the given one does not or writes fewer partial
+// rows: these kernels measure each track's cameras, check the index words the reordering relies
+// on, gather the reordered scene and index, and put the solved points back in the given order.
+// The choice and the sorts of the keys run on the host (counting sorts); each camera's list of the
+// reordered observations is a stable radix sort of their cameras on the device.
 #pragma once
 
 #include <hipcub/hipcub.hpp>
