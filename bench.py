@@ -69,15 +69,19 @@ def matvec_bytes(C: int, P: int, M: int, shared: bool, factored: bool = False) -
 def onepass_bytes(C: int, P: int, M: int, shared: bool, factored: bool, rows: int,
                   tiles: int) -> int:
     """Algorithmic HBM bytes of one one-pass Schur product as the PCG loop runs it (k_schur_onepass +
-    k_onepass_product): every observation's stored Jacobian factors read once (11 words shared, 9
-    fixed: FP32 in the mixed form, `factored`, FP64 in the FP64 form) and its code word (window
-    row, rank, tile point, place in its point's run); V^-1 of every point; each tile's 8-word
-    record; per camera p, its rotation and damped block A read, its rows of S p and its p.Sp and
-    focal share written; the rows of camera partials written once and read once (8 doubles
-    each)."""
-    per_obs = (11 if shared else 9) * (4 if factored else 8)
-    per_cam = 6 * 8 + 9 * 8 + 36 * 8 + 8 * 8
-    return (M * (per_obs + 4) + P * 9 * 8 + tiles * 8 * 4 + C * per_cam + rows * 8 * 8 * 2
+    k_onepass_product): every observation's stored Jacobian factors read once (FP32 in the mixed
+    form, `factored`: 11 words shared, 9 fixed; FP64 in the FP64 form: 8 or 6, the camera-frame
+    point recomputed from each point's coordinates, read once, and the window's poses) and its code
+    word (window row, rank, tile point, place in its point's run); V^-1 of every point; each
+    tile's 8-word record; per camera p, its rotation (FP64 form: its pose) and damped block A
+    read, its rows of S p and its p.Sp and focal share written; the rows of camera partials
+    written once and read once (8 doubles each)."""
+    if factored:
+        per_obs, per_point, pose = (11 if shared else 9) * 4, 9 * 8, 9 * 8
+    else:
+        per_obs, per_point, pose = (8 if shared else 6) * 8, 12 * 8, 12 * 8
+    per_cam = 6 * 8 + pose + 36 * 8 + 8 * 8
+    return (M * (per_obs + 4) + P * per_point + tiles * 8 * 4 + C * per_cam + rows * 8 * 8 * 2
             + (8 if shared else 0))
 
 

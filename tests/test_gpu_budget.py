@@ -41,10 +41,10 @@ def tile_slots(control, packed) -> int:
 
 def stored_bytes(control, packed) -> int:
     """Bytes of the arrays only a resident solve holds: each observation's Jacobian factors
-    (K = 11 words, 9 on the fixed route; FP64, or FP32 in the mixed form), point-major and
+    (FP64: K = 8 words, 6 on the fixed route; FP32 in the mixed form: 11 or 9), point-major and
     camera-major (K words per slot of 64-slot blocks)."""
     mixed = control.precision == _native.MIXED
-    K = 11 if control.shared_focal else 9
+    K = (11 if control.shared_focal else 9) if mixed else (8 if control.shared_focal else 6)
     ws = 4 if mixed else 8
     return (K * control.observations + K * tile_slots(control, packed)) * ws
 

@@ -759,7 +759,7 @@ std::vector<Need> solve_needs(tcv_ba_context* ctx, const tcv_ba_control* c, uint
   const size_t n_tiles = plan.tiles.size() - 1, n_tasks = plan.tasks.size() / 4;
   const size_t n_cost = plan.cost_blocks.size() / 3, n_groups = plan.groups.size() / GROUP_WORDS;
   // the factors (KF words per observation) or, for the exact systems, the cross blocks (K)
-  const uint32_t words = exact ? K : kf_words(shared);
+  const uint32_t words = exact ? K : kf_words_of(ws, shared);
   const size_t w_words = (size_t)words * plan.tile_slot.back();
   const size_t wp_words = (size_t)words * M;  // tile_word: no padding
   std::vector<Need> n = {
@@ -941,7 +941,7 @@ int setup(tcv_ba_context* ctx, const tcv_ba_control* controls, uint32_t B, const
   const uint32_t n_cost = (uint32_t)plan.cost_blocks.size() / 3;
   const uint32_t n_groups = (uint32_t)plan.groups.size() / GROUP_WORDS;
   // the stored words per observation: the factors, or the exact systems' cross blocks
-  const uint32_t stored_words = exact ? K : kf_words(shared);
+  const uint32_t stored_words = exact ? K : kf_words_of(ws, shared);
   const size_t w_words = (size_t)stored_words * plan.tile_slot.back();
   const size_t vec_words = (size_t)m * 5 + (size_t)P * 3;
   bool streamed = residency == STREAMED;

@@ -14,22 +14,30 @@
 //   camera side W y            : h = D' (R y);  omega: q x (D'^T h),  v: D'^T h,  focal: Jf' . h
 // so a Schur product streams 2 (KF + 1) words per observation instead of 2 (K + 1). Sums keep the
 // stored sweeps' orders (segment sums per point, slot order and wave butterflies per camera task).
-// The FP64 form stores the same words in FP64 (but for the exact systems of the polish and the
-// adjoint, whose exact-Hessian correction has no factored form): 88 (72) bytes per observation
-// instead of the 168 (144) of its cross block.
+// The FP64 form (but for the exact systems of the polish and the adjoint, whose exact-Hessian
+// correction has no factored form) stores D' and Jf' in FP64, 64 (48) bytes per observation
+// instead of the 168 (144) of its cross block, and recomputes q = R X + t from the pose and the
+// point where the products use it (camera_point, the bits the passes compute): the products'
+// gathers of the pose and the point come from cache, where stored q would be 24 more bytes of
+// stream per observation.
 #pragma once
 
 namespace tcvba {
 
-// The stored factors (FT: float or double) of observation i (camera c, point p), widened: word k
-// at w0 + k * stride (the point-major words: a tile's row length; the camera-major ones: 64);
-// RECOMPUTE: relinearised from the scene and rounded as the passes store them (the streamed route,
-// the same bits; w0 unused).
+// The factors' camera-frame point q of a pose and a point (f[6..8]; the FP64 form's).
+__device__ __forceinline__ void point_factor(const double* pose, const double* X, bool dir,
+                                             double* f) {
+  camera_point(pose, X[0], X[1], X[2], dir, f + 6);
+}
+
+// The factors of observation i (camera c, point p), widened: the stored words (FT: float or double;
+// stored word k at w0 + k * stride, the point-major words: a tile's row length, the camera-major
+// ones: 64), with q from the pose and the point in the FP64 form; RECOMPUTE: relinearised from the
+// scene and rounded as the passes store them (the streamed route, the same bits; w0 unused).
 template <class FT, bool SHARED, bool RECOMPUTE>
 __device__ __forceinline__ void load_factors(const Problem& q, const FT* w0, uint32_t stride,
                                              uint32_t i, uint32_t c, uint32_t p, const Bundle& g,
                                              double fscale, double* f) {
-  constexpr int KF = kf_words(SHARED);
   if (RECOMPUTE) {
     Lin L;
     const bool ok =
@@ -37,10 +45,14 @@ __device__ __forceinline__ void load_factors(const Problem& q, const FT* w0, uin
     double raw[11];
     cross_factors(L, ok, SHARED, raw);
 #pragma unroll
-    for (int k = 0; k < KF; k++) f[k] = (double)(FT)raw[k];
+    for (int k = 0; k < 11; k++) f[k] = (double)(FT)raw[k];
   } else {
 #pragma unroll
-    for (int k = 0; k < KF; k++) f[k] = (double)__builtin_nontemporal_load(w0 + k * stride);
+    for (int k = 0; k < kf_words<FT>(SHARED); k++)
+      f[kf_raw<FT>(k)] = (double)__builtin_nontemporal_load(w0 + k * stride);
+    if (sizeof(FT) == 8)
+      point_factor(q.scene + (size_t)c * 12, q.scene + (size_t)q.C * 12 + (size_t)p * 3,
+                   directional_point(g, p), f);
   }
   if (!SHARED) f[9] = f[10] = 0;
 }
@@ -87,7 +99,7 @@ __device__ __forceinline__ void sweep_tile_factored(const Problem& q, uint32_t t
       const uint32_t c = obs_camera(q)[i], pt = obs_point(q)[i];
       const bool var = variable_camera(q, c), dir = directional_point(g, pt);
       double f[11], R[9], vc[6];
-      const FT* w0 = RECOMPUTE ? nullptr : tile_word<FT>(q.Wp, kf_words(SHARED), s, e - s, 0, i);
+      const FT* w0 = RECOMPUTE ? nullptr : tile_word<FT>(q.Wp, kf_words<FT>(SHARED), s, e - s, 0, i);
       load_factors<FT, SHARED, RECOMPUTE>(q, w0, e - s, i, c, pt, g, fscale, f);
       const double* pose = q.scene + (size_t)c * 12;
 #pragma unroll

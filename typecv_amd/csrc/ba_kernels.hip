@@ -138,6 +138,19 @@ struct Lin {
   Model k;        // camera model at the focal scale
 };
 
+// The camera-frame point q = R X + t of a pose (R row-major, then t) and a point X; R X for a
+// directional point. The one expression of q: the FP64 form's passes store factors without it and
+// its products recompute it, the same bits (the tests of streamed against resident solves hold
+// the two to bit equality).
+__device__ __forceinline__ void camera_point(const double* pose, double x0, double x1, double x2,
+                                             bool dir, double* qc) {
+#pragma unroll
+  for (int r = 0; r < 3; r++) {
+    qc[r] = pose[r * 3] * x0 + pose[r * 3 + 1] * x1 + pose[r * 3 + 2] * x2;
+    if (!dir) qc[r] += pose[9 + r];
+  }
+}
+
 __device__ __forceinline__ bool linearize(const Problem& q, uint32_t c, uint32_t p, uint32_t i,
                                           double fscale, bool var, bool dir, Lin& L) {
   const double* pose = q.scene + (size_t)c * 12;
@@ -145,13 +158,8 @@ __device__ __forceinline__ bool linearize(const Problem& q, uint32_t c, uint32_t
   double R[12];
 #pragma unroll
   for (int j = 0; j < 12; j++) R[j] = pose[j];
-  const double x0 = X[0], x1 = X[1], x2 = X[2];
   double qc[3];
-#pragma unroll
-  for (int r = 0; r < 3; r++) {
-    qc[r] = R[r * 3] * x0 + R[r * 3 + 1] * x1 + R[r * 3 + 2] * x2;
-    if (!dir) qc[r] += R[9 + r];
-  }
+  camera_point(R, X[0], X[1], X[2], dir, qc);
   const Model k = camera_model(q, c, fscale);
   double px, py, du[3], dv[3];
   if (!project(k, qc, px, py, du, dv)) return false;
@@ -310,8 +318,23 @@ __device__ __forceinline__ WT* wptr(void* base, uint32_t k, uint32_t i, uint32_t
   return static_cast<WT*>(base) + ((size_t)(i >> 6) * K + k) * 64 + (i & 63);
 }
 
-// The mixed form stores each observation's cross block as FP32 Jacobian factors (ba_factored.hip):
-constexpr int kf_words(bool shared) { return shared ? 11 : 9; }
+// The factored forms store each observation's cross block as its Jacobian factors
+// (ba_factored.hip; FT: float in the mixed form, double in the FP64 form): s du (3), s dv (3),
+// [FP32: the camera-frame point q (3),] [s Jf (2)]. The FP64 form recomputes q from the pose and
+// the point where the factors are used (camera_point) instead of storing it. Stored word k is
+// word kf_raw(k) of cross_factors' eleven.
+template <class FT>
+__host__ __device__ constexpr int kf_words(bool shared) {
+  return sizeof(FT) == 8 ? (shared ? 8 : 6) : (shared ? 11 : 9);
+}
+template <class FT>
+__host__ __device__ constexpr int kf_raw(int k) {
+  return sizeof(FT) == 8 && k >= 6 ? k + 3 : k;
+}
+// kf_words of a form by its stored word size (8: FP64, 4: FP32).
+constexpr int kf_words_of(size_t ws, bool shared) {
+  return ws == 8 ? kf_words<double>(shared) : kf_words<float>(shared);
+}
 
 // Word k of observation i in the point-major stored words (Wp, K words per observation: cross
 // blocks or factors), i in the tile whose observations are [s, s + n). Each tile's words are K rows
@@ -346,7 +369,7 @@ __device__ __forceinline__ void cross_factors(const Lin& L, bool ok, bool shared
 // are.
 template <bool SHARED, class FT = float>
 __device__ __forceinline__ FT* fptr(void* base, uint32_t k, uint32_t i) {
-  constexpr uint32_t KF = kf_words(SHARED);
+  constexpr uint32_t KF = kf_words<FT>(SHARED);
   return static_cast<FT*>(base) + ((size_t)(i >> 6) * KF + k) * 64 + (i & 63);
 }
 
@@ -604,8 +627,8 @@ __global__ __launch_bounds__(TILE* TILE_WAVES) void k_point_pass(Problem q) {
         if constexpr (FACTORS) {
           double fw[11];
           cross_factors(L, ok, SHARED, fw);
-          for (int k = 0; k < kf_words(SHARED); k++)
-            *tile_word<WT>(q.Wp, kf_words(SHARED), s, e - s, k, i) = (WT)fw[k];
+          for (int k = 0; k < kf_words<WT>(SHARED); k++)
+            *tile_word<WT>(q.Wp, kf_words<WT>(SHARED), s, e - s, k, i) = (WT)fw[kf_raw<WT>(k)];
         } else {
           for (uint32_t k = 0; k < K; k++) *tile_word<WT>(q.Wp, K, s, e - s, k, i) = (WT)W[k];
         }
@@ -855,7 +878,8 @@ __global__ __launch_bounds__(TASK_THREADS) void k_camera_pass(Problem q) {
         if (sizeof(WT) == 4 || (!EXACT && q.factored64)) {
           double fw[11];
           cross_factors(L, ok, SHARED, fw);
-          for (int k = 0; k < kf_words(SHARED); k++) *fptr<SHARED, WT>(q.Wc, k, n) = (WT)fw[k];
+          for (int k = 0; k < kf_words<WT>(SHARED); k++)
+            *fptr<SHARED, WT>(q.Wc, k, n) = (WT)fw[kf_raw<WT>(k)];
         } else {
           for (uint32_t k = 0; k < K; k++) *wptr<WT, SHARED>(q.Wc, k, n, q.M) = (WT)W[k];
         }

@@ -86,14 +86,22 @@ struct Window {
 };
 
 // The cameras of a workgroup's window, staged in LDS once per product: each camera's PCG direction
-// p_c, whether it is variable and (NR = 9) its rotation, which the factors' products need. The
-// tiles' gathers of them become LDS reads.
+// p_c, whether it is variable and its rotation (NR = 9), which the factors' products need, or its
+// whole pose, rotation then translation (NR = 12: the FP64 form, whose products recompute the
+// camera-frame points). The tiles' gathers of them become LDS reads.
 template <int NR>
 struct WindowCams {
+  static_assert(NR == 9 || NR == 12, "a rotation, or a whole pose");
   double p[OP_SPAN][6];
   double R[OP_SPAN][NR];
   uint32_t var[OP_SPAN];
 };
+
+// The pose words a window stages per camera for the products over WT factors.
+template <class WT>
+constexpr int window_words() {
+  return sizeof(WT) == 8 ? 12 : 9;
+}
 
 __device__ __forceinline__ Window window_of(const Problem& q, uint32_t w) {
   const uint32_t raw = q.op_lo[w];
@@ -182,7 +190,7 @@ struct ObsWords;
 
 template <bool SHARED>
 struct ObsWords<float, SHARED> {
-  static constexpr int K = kf_words(SHARED);
+  static constexpr int K = kf_words<float>(SHARED);
   float f[11];
   __device__ __forceinline__ void load_stored(const float* w0, uint32_t n, uint32_t lane) {
 #pragma unroll
@@ -190,6 +198,8 @@ struct ObsWords<float, SHARED> {
 #pragma unroll
     for (int k = K; k < 11; k++) f[k] = 0;
   }
+  // q is among the stored words
+  __device__ __forceinline__ void set_point(const double*, const double*, bool) {}
   // relinearised and rounded as the passes store them
   __device__ __forceinline__ void load_recomputed(const Problem& q, uint32_t i, uint32_t c,
                                                   uint32_t p, const Bundle& g, double fscale) {
@@ -199,9 +209,7 @@ struct ObsWords<float, SHARED> {
     double raw[11];
     cross_factors(L, ok, SHARED, raw);
 #pragma unroll
-    for (int k = 0; k < K; k++) f[k] = (float)raw[k];
-#pragma unroll
-    for (int k = K; k < 11; k++) f[k] = 0;
+    for (int k = 0; k < 11; k++) f[k] = (float)raw[k];
   }
   __device__ __forceinline__ void point_term(const double* R, const double* vc, double vf,
                                              bool var, bool dir, double t[3]) const {
@@ -221,13 +229,17 @@ struct ObsWords<float, SHARED> {
 // the one-pass product runs), every product formed as for the FP32 ones.
 template <bool SHARED>
 struct ObsWords<double, SHARED> {
-  static constexpr int K = kf_words(SHARED);
+  static constexpr int K = kf_words<double>(SHARED);
   double f[11];
   __device__ __forceinline__ void load_stored(const double* w0, uint32_t n, uint32_t lane) {
 #pragma unroll
-    for (int k = 0; k < K; k++) f[k] = __builtin_nontemporal_load(w0 + k * n + lane);
-#pragma unroll
-    for (int k = K; k < 11; k++) f[k] = 0;
+    for (int k = 0; k < K; k++)
+      f[kf_raw<double>(k)] = __builtin_nontemporal_load(w0 + k * n + lane);
+    if (!SHARED) f[9] = f[10] = 0;
+  }
+  // q from the window camera's pose and the point, as the passes compute it
+  __device__ __forceinline__ void set_point(const double* pose, const double* X, bool dir) {
+    camera_point(pose, X[0], X[1], X[2], dir, f + 6);
   }
   // relinearised as the passes store them
   __device__ __forceinline__ void load_recomputed(const Problem& q, uint32_t i, uint32_t c,
@@ -236,8 +248,6 @@ struct ObsWords<double, SHARED> {
     const bool ok =
         linearize(q, c, p, i, fscale, variable_camera(q, c), directional_point(g, p), L);
     cross_factors(L, ok, SHARED, f);
-#pragma unroll
-    for (int k = K; k < 11; k++) f[k] = 0;
   }
   __device__ __forceinline__ void point_term(const double* R, const double* vc, double vf,
                                              bool var, bool dir, double t[3]) const {
@@ -250,6 +260,11 @@ struct ObsWords<double, SHARED> {
 };
 
 // The first stored word of the tile whose observations begin at `first` (tile_word's layout).
+// The scene coordinates of point p.
+__device__ __forceinline__ const double* point_of(const Problem& q, uint32_t p) {
+  return q.scene + (size_t)q.C * 12 + (size_t)p * 3;
+}
+
 template <class WT, bool SHARED>
 __device__ __forceinline__ const WT* stored_tile(const Problem& q, uint32_t first) {
   return static_cast<const WT*>(q.Wp) + (size_t)first * ObsWords<WT, SHARED>::K;
@@ -362,6 +377,7 @@ __device__ __forceinline__ void onepass_tile(const OnePassCtx<NR>& x, uint32_t r
     else
       in.ow.load_stored(stored_tile<WT, SHARED>(x.q, first), n, lane);
   }
+  if (!RECOMPUTE && ok) in.ow.set_point(x.wc.R[k.row], point_of(x.q, pt), dir);
   // the point half: each point's terms summed by the scan; its last lane hands the sum to the
   // point's lane (st[lp]), which forms y_p = V_p^-1 sum
   double tj[3] = {0, 0, 0};
@@ -401,10 +417,12 @@ __device__ __forceinline__ void onepass_long_tile(const OnePassCtx<NR>& x, uint3
   const uint32_t rounds = record_word(rec, OP_RECORD * j + TR_ROUNDS);
   ObsWords<WT, SHARED> ow;
   auto load = [&](uint32_t base, const SlotCode& k) {
-    if (RECOMPUTE)
+    if (RECOMPUTE) {
       ow.load_recomputed(x.q, first + base + lane, x.win.camera(k.row), fp + k.lp, x.g, x.fscale);
-    else
+    } else {
       ow.load_stored(stored_tile<WT, SHARED>(x.q, first) + base, n, lane);
+      ow.set_point(x.wc.R[k.row], point_of(x.q, fp + k.lp), directional_point(x.g, fp + k.lp));
+    }
   };
   // the point's terms pass by pass: each pass's sum (the scan's last lane) added in pass order
   double acc[3] = {0, 0, 0};
@@ -465,9 +483,9 @@ struct OnePassLds {
 template <class WT, bool SHARED, bool RECOMPUTE>
 __device__ __forceinline__ void onepass_workgroup(const Problem& q, uint32_t w,
                                                   OnePassLds<SHARED ? 7 : 6,
-                                                             9>& L) {
+                                                             window_words<WT>()>& L) {
   constexpr int NV = SHARED ? 7 : 6;
-  constexpr int NR = 9;
+  constexpr int NR = window_words<WT>();
   // words loaded one tile ahead: the FP32 factors (not the FP64 blocks, whose registers would cut
   // the waves a SIMD holds, nor relinearised words, which need the tile's codes first)
   constexpr bool AHEAD = sizeof(WT) == 4 && !RECOMPUTE;
@@ -495,11 +513,9 @@ __device__ __forceinline__ void onepass_workgroup(const Problem& q, uint32_t w,
       const double* v = vp(q) + (size_t)c * 6;
 #pragma unroll
       for (int k = 0; k < 6; k++) L.wc.p[r][k] = v[k];
-      if constexpr (NR == 9) {
-        const double* pose = q.scene + (size_t)c * 12;
+      const double* pose = q.scene + (size_t)c * 12;
 #pragma unroll
-        for (int k = 0; k < 9; k++) L.wc.R[r][k] = pose[k];
-      }
+      for (int k = 0; k < NR; k++) L.wc.R[r][k] = pose[k];
       L.wc.var[r] = variable_camera(q, c) ? 1u : 0u;
     }
   }
@@ -540,7 +556,7 @@ __device__ __forceinline__ void onepass_workgroup(const Problem& q, uint32_t w,
 // One workgroup per op_tile range (or per entry of op_map), XCD-contiguous order.
 template <class WT, bool SHARED, bool RECOMPUTE = false>
 __global__ __launch_bounds__(TILE* TILE_WAVES) void k_schur_onepass(Problem q) {
-  __shared__ OnePassLds<SHARED ? 7 : 6, 9> L;
+  __shared__ OnePassLds<SHARED ? 7 : 6, window_words<WT>()> L;
   const uint32_t i = xcd_contiguous(blockIdx.x, gridDim.x);
   onepass_workgroup<WT, SHARED, RECOMPUTE>(q, q.op_map ? q.op_map[i] : i, L);
 }
