@@ -35,12 +35,12 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE x 1.00, the factors that make a 1 GiB device copy read and write 1 GiB. Keyed by
 # (cameras, points, seed, precision, operator); other workloads report null.
 MEASURED_TRAFFIC = {
-    # the one-pass product over FP64 or FP32 factors (k_schur_onepass + k_onepass_reduce, the
-    # probe's reduction, which moves about 0.7 MB less per product than k_onepass_product)
+    # the one-pass product over FP64 or FP32 factors as the PCG loop runs it (k_schur_onepass +
+    # k_onepass_product): HBM bytes per product, 2 x FETCH_SIZE + WRITE_SIZE (KiB per dispatch)
     (2000, 1_000_000, 2, "fp64", "onepass"):
-        2 * (500232.6 + 7674.8) * 1024 + (11391.4 + 625.0) * 1024,
+        2 * (391699.6 + 4384.9) * 1024 + (5536.4 + 140.6) * 1024,
     (2000, 1_000_000, 2, "mixed", "onepass"):
-        2 * (288614.9 + 7674.8) * 1024 + (11389.8 + 625.0) * 1024,
+        2 * (288646.1 + 4384.9) * 1024 + (5536.1 + 140.6) * 1024,
 }
 
 
